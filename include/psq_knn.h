@@ -1,0 +1,134 @@
+/* psq_knn.h — C-ABI of the MI355X neighbour-graph library (libpsq_knn.so).
+ *
+ * Drop-in boundary for Polariseq's neighbour-graph step. Each entry point names
+ * the reference interface it replaces (paths relative to the reference root).
+ *
+ * Rules, for every function:
+ *  - Return 0 (PSQ_OK) on success, a negative PSQ_E* code on failure. Nothing
+ *    aborts or throws across this boundary. psq_last_error() gives the message
+ *    of the last failure on the calling thread.
+ *  - Host-pointer entry points (no "_device" suffix) take caller-owned host
+ *    buffers and copy to and from the GPU. Entries past nnz are not touched.
+ *  - "_device" entry points take device pointers on the current HIP device and
+ *    run on `stream` (NULL = the default stream). They return after the
+ *    results are complete in device memory.
+ *  - The library owns its device working set. It grows as needed, is kept
+ *    between calls, and is released by psq_release_workspace(). One call at a
+ *    time per device: calls that overlap on one device must be serialised by
+ *    the caller.
+ *  - Graph layout (crates/polariseq-core/src/backend.rs:37-46): CSR with indptr
+ *    u32[n_obs+1], indices u32[nnz], dists f32[nnz]. Rows are ascending by
+ *    distance, ties by index. Distances are Euclidean (sqrt of the squared
+ *    distance). No self-loops, no duplicates, at most k per row.
+ */
+#ifndef PSQ_KNN_H
+#define PSQ_KNN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSQ_OK 0
+#define PSQ_E_INVALID (-1)     /* bad argument: empty data, k == 0, short outputs, ... */
+#define PSQ_E_UNSUPPORTED (-2) /* valid, but outside what this build supports */
+#define PSQ_E_DEVICE (-3)      /* a HIP call failed (message names it) */
+#define PSQ_E_NOMEM (-4)       /* device memory (or the debug budget) exhausted */
+
+#define PSQ_PRECISION_FP32 0        /* plain form: exact f32 sq_dist of the reference */
+#define PSQ_PRECISION_BF16_RERANK 1 /* BF16 MFMA join tiles, FP32 re-rank of each row's top 2k */
+
+#define PSQ_INIT_RP_FOREST 0 /* random-projection forest (the reference's start) */
+#define PSQ_INIT_RANDOM 1    /* random fill only */
+#define PSQ_INIT_LEARNED 2   /* candidates from a learned encoder's codes (model != NULL) */
+
+/* Mirrors NndescentOptions, crates/polariseq-core/src/nndescent.rs:41-76.
+ * 0 means "auto" for the sized fields, resolved as in nndescent.rs:722-752. */
+typedef struct psq_nndescent_opts {
+  uint64_t k;              /* neighbours per node (>= 1) */
+  uint64_t max_candidates; /* 0 -> min(60, 2k) */
+  uint64_t max_iterations; /* 0 -> max(5, round(log2 n)) */
+  double delta;            /* stop when changed <= floor(delta * n * k); default 0.001 */
+  uint64_t n_trees;        /* 0 -> min(48, 8 + round(sqrt(n)/10)) */
+  uint64_t leaf_size;      /* 0 -> max(10, k) */
+  uint64_t gather_max;     /* accepted for compatibility; a memory choice that never changes the graph */
+  uint64_t seed;
+  int32_t precision; /* PSQ_PRECISION_* */
+  int32_t init;      /* PSQ_INIT_* */
+  const void* model; /* learned encoder (PSQ_INIT_LEARNED), see psq_encoder_* */
+  int32_t device;    /* HIP device ordinal for host-pointer entries */
+  uint32_t slots;    /* proposal slots per row per iteration (0 -> auto, power of two >= 64) */
+} psq_nndescent_opts;
+
+/* Defaults of NndescentOptions::default (nndescent.rs:63-76) with k = 15. */
+void psq_nndescent_opts_default(psq_nndescent_opts* o);
+
+/* crates/polariseq-core/src/nndescent.rs:648-650: max(1, min(k, n_obs - 1)). */
+uint64_t psq_effective_k(uint64_t n_obs, uint64_t k);
+
+/* Replaces nndescent_into (crates/polariseq-core/src/nndescent.rs:699-707) and,
+ * through the Python binding, nndescent_dense (crates/polariseq-python/src/
+ * lib.rs:507-566). data is f32[n_obs][n_dim] on the host. indptr_len must be
+ * n_obs + 1; indices and dists must hold cap >= n_obs * psq_effective_k(n_obs, k)
+ * entries. Writes *nnz_out. Errors instead of the reference's panics
+ * (nndescent.rs:708-719). */
+int psq_nndescent_into(const float* data, uint64_t n_obs, uint64_t n_dim,
+                       const psq_nndescent_opts* opts, uint32_t* indptr, uint64_t indptr_len,
+                       uint32_t* indices, float* dists, uint64_t cap, uint64_t* nnz_out);
+
+/* The same on device pointers (embedding resident in HBM, graph left in HBM). */
+int psq_nndescent_device(const float* d_data, uint64_t n_obs, uint64_t n_dim,
+                         const psq_nndescent_opts* opts, uint32_t* d_indptr, uint64_t indptr_len,
+                         uint32_t* d_indices, float* d_dists, uint64_t cap, uint64_t* nnz_out,
+                         void* stream);
+
+/* Exact kNN, self excluded, ordered by (FP64 distance, index): the order of
+ * by_distance / knn_exact (crates/polariseq-core/src/doublets.rs:544-651).
+ * queries: u32 indices into data, or NULL for all n points (n_q must then be n).
+ * Outputs idx_out u32[n_q][k] and dist_out f32[n_q][k] (Euclidean, from the FP64
+ * squared distance). Rows with fewer than k other points are padded with index
+ * 0xFFFFFFFF and distance +inf. */
+int psq_knn_exact(const float* data, uint64_t n, uint64_t d, const uint32_t* queries, uint64_t n_q,
+                  uint64_t k, uint32_t* idx_out, float* dist_out, int32_t device);
+int psq_knn_exact_device(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d_queries,
+                         uint64_t n_q, uint64_t k, uint32_t* d_idx_out, float* d_dist_out,
+                         void* stream);
+
+/* Replaces neighbors() / neighbors_dense (crates/polariseq-core/src/neighbors.rs:101-147,
+ * crates/polariseq-python/src/lib.rs:474-501): k_with_self = min(n_neighbors, n_obs);
+ * each row holds k_with_self - 1 neighbours, self excluded. metric 0 = euclidean
+ * (sqrt distance), 1 = cosine (1 - cos). Output capacity n_obs * (k_with_self - 1). */
+int psq_neighbors_exact(const float* embedding, uint64_t n_obs, uint64_t n_dim,
+                        uint64_t n_neighbors, int32_t metric, uint32_t* indptr, uint32_t* indices,
+                        float* dists, uint64_t cap, uint64_t* nnz_out, int32_t device);
+
+/* Per-run statistics of the last NN-Descent call on this thread (for reports). */
+typedef struct psq_nndescent_stats {
+  uint64_t n_obs, n_dim, k, max_candidates, max_iterations, n_trees, leaf_size, slots, stop_at;
+  uint64_t iterations;
+  uint64_t changed[64];   /* per iteration */
+  uint64_t pairs[64];     /* distance pairs evaluated in the join, per iteration */
+  uint64_t cand_rows[64]; /* Σ_i (|new_i| + |old_i|), the join's row gathers, per iteration */
+  double ms_init, ms_iterations, ms_extract, ms_total;
+  double ms_join[64]; /* join kernel time per iteration (HIP events) */
+} psq_nndescent_stats;
+int psq_last_stats(psq_nndescent_stats* out);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* psq_last_error(void);
+
+/* Frees the cached device working set of the current device. */
+int psq_release_workspace(void);
+
+/* Debug cap on the library's device allocations, in bytes (0 = none): an
+ * allocation past it fails with PSQ_E_NOMEM, so out-of-memory handling can be
+ * tested from the host. */
+void psq_set_device_budget(uint64_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSQ_KNN_H */

@@ -1,0 +1,20 @@
+"""polariseq_amd: MI355X-native neighbour-graph step for Polariseq.
+
+NN-Descent kNN on a PCA embedding, and an exact tiled kNN, as hand-written HIP
+kernels for gfx950 behind a C-ABI (include/psq_knn.h, lib/libpsq_knn.so).
+The Python functions mirror the reference binding's ``nndescent_dense`` and
+``neighbors_dense`` (crates/polariseq-python/src/lib.rs:471-566).
+"""
+
+from .neighbors import (  # noqa: F401
+    effective_k,
+    knn_exact,
+    knn_exact_device,
+    last_stats,
+    nndescent_dense,
+    nndescent_device,
+    neighbors_dense,
+    release_workspace,
+)
+
+__version__ = "0.1.0"

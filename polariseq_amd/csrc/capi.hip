@@ -1,0 +1,671 @@
+// C-ABI of libpsq_knn.so: argument checks, device working set, the NN-Descent
+// iteration loop and the exact kNN driver. Declarations and the reference
+// interfaces each entry replaces are in include/psq_knn.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/psq_knn.h"
+#include "psq_kernels.hpp"
+
+using namespace psq;
+
+namespace {
+
+thread_local std::string g_err;
+thread_local psq_nndescent_stats g_stats;
+uint64_t g_budget = 0;  // 0 = no cap
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPC(call)                                                                         \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess) {                                                                \
+      return fail(e_ == hipErrorOutOfMemory ? PSQ_E_NOMEM : PSQ_E_DEVICE, "%s failed: %s", \
+                  #call, hipGetErrorString(e_));                                           \
+    }                                                                                      \
+  } while (0)
+
+#define TRY(expr)               \
+  do {                          \
+    int rc_ = (expr);           \
+    if (rc_ != PSQ_OK) return rc_; \
+  } while (0)
+
+// Grow-only named device buffers, one set per device.
+struct Workspace {
+  struct Buf {
+    void* p = nullptr;
+    size_t bytes = 0;
+  };
+  std::map<std::string, Buf> bufs;
+  uint64_t total = 0;
+  void* pinned = nullptr;  // small pinned host area for counters
+
+  int get(const char* name, size_t bytes, void** out) {
+    Buf& b = bufs[name];
+    if (b.bytes < bytes) {
+      if (b.p) {
+        hipError_t e = hipFree(b.p);
+        if (e != hipSuccess) return fail(PSQ_E_DEVICE, "hipFree failed: %s", hipGetErrorString(e));
+        total -= b.bytes;
+        b.p = nullptr;
+        b.bytes = 0;
+      }
+      size_t want = std::max<size_t>(bytes, 256);
+      if (g_budget && total + want > g_budget)
+        return fail(PSQ_E_NOMEM, "device budget exceeded: %s needs %zu bytes (%llu in use, cap %llu)",
+                    name, want, (unsigned long long)total, (unsigned long long)g_budget);
+      hipError_t e = hipMalloc(&b.p, want);
+      if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(e == hipErrorOutOfMemory ? PSQ_E_NOMEM : PSQ_E_DEVICE,
+                    "hipMalloc(%zu) for %s failed: %s", want, name, hipGetErrorString(e));
+      }
+      b.bytes = want;
+      total += want;
+    }
+    *out = b.p;
+    return PSQ_OK;
+  }
+  int host_counters(void** out) {
+    if (!pinned) {
+      hipError_t e = hipHostMalloc(&pinned, 4096, hipHostMallocDefault);
+      if (e != hipSuccess) return fail(PSQ_E_DEVICE, "hipHostMalloc failed: %s", hipGetErrorString(e));
+    }
+    *out = pinned;
+    return PSQ_OK;
+  }
+  void release() {
+    for (auto& kv : bufs)
+      if (kv.second.p) (void)hipFree(kv.second.p);
+    bufs.clear();
+    total = 0;
+    if (pinned) (void)hipHostFree(pinned);
+    pinned = nullptr;
+  }
+};
+
+std::mutex g_ws_mu;
+std::map<int, Workspace> g_ws;
+
+int workspace(Workspace** out) {
+  int dev = 0;
+  HIPC(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> g(g_ws_mu);
+  *out = &g_ws[dev];
+  return PSQ_OK;
+}
+
+template <typename T>
+int wbuf(Workspace* ws, const char* name, size_t count, T** out) {
+  void* p = nullptr;
+  TRY(ws->get(name, count * sizeof(T), &p));
+  *out = static_cast<T*>(p);
+  return PSQ_OK;
+}
+
+inline unsigned grid_for(size_t count, unsigned block) {
+  size_t g = (count + block - 1) / block;
+  return (unsigned)std::max<size_t>(1, std::min<size_t>(g, 1u << 30));
+}
+
+int fill_u64(uint64_t* p, size_t count, uint64_t v, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_u64, dim3(std::min<unsigned>(grid_for(count, 256), 65536)), dim3(256),
+                     0, s, p, count, v);
+  HIPC(hipGetLastError());
+  return PSQ_OK;
+}
+int fill_u32(uint32_t* p, size_t count, uint32_t v, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_u32, dim3(std::min<unsigned>(grid_for(count, 256), 65536)), dim3(256),
+                     0, s, p, count, v);
+  HIPC(hipGetLastError());
+  return PSQ_OK;
+}
+int fill_u8(uint8_t* p, size_t count, uint8_t v, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_u8, dim3(std::min<unsigned>(grid_for(count, 256), 65536)), dim3(256),
+                     0, s, p, count, v);
+  HIPC(hipGetLastError());
+  return PSQ_OK;
+}
+
+// Exclusive scan of n u32 values into out[0..n], out[n] = total.
+int scan_u32(Workspace* ws, const uint32_t* in, uint32_t* out, uint32_t n, hipStream_t s) {
+  const uint32_t nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
+  uint32_t* sums = nullptr;
+  TRY(wbuf(ws, "scan_sums", std::max<uint32_t>(nb, 1), &sums));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(SCAN_BLOCK), 0, s, in, out, n, sums);
+    HIPC(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, s, sums, nb, out + n);
+  HIPC(hipGetLastError());
+  if (n > 0) {
+    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(SCAN_BLOCK), 0, s, out, n, sums);
+    HIPC(hipGetLastError());
+  }
+  return PSQ_OK;
+}
+
+uint32_t round_up(uint32_t v, uint32_t m) { return (v + m - 1) / m * m; }
+
+struct Resolved {
+  uint64_t n, d, k, m, n_iters, n_trees, leaf_size, S, stop_at, seed;
+  uint32_t dp;
+};
+
+// nndescent.rs:722-752.
+int resolve(uint64_t n, uint64_t d, const psq_nndescent_opts* o, Resolved* r) {
+  if (!o) return fail(PSQ_E_INVALID, "opts is NULL");
+  if (n == 0 || d == 0) return fail(PSQ_E_INVALID, "empty data: n_obs=%llu n_dim=%llu",
+                                    (unsigned long long)n, (unsigned long long)d);
+  if (o->k == 0) return fail(PSQ_E_INVALID, "k must be >= 1");
+  if (n >= (1ull << 31)) return fail(PSQ_E_UNSUPPORTED, "n_obs must be < 2^31");
+  if (d > 256) return fail(PSQ_E_UNSUPPORTED, "n_dim > 256 is not supported on the GPU path");
+  r->n = n;
+  r->d = d;
+  r->k = std::max<uint64_t>(1, std::min<uint64_t>(o->k, n - 1));
+  r->m = std::max<uint64_t>(1, o->max_candidates > 0 ? o->max_candidates
+                                                      : std::min<uint64_t>(2 * r->k, 60));
+  r->n_iters = o->max_iterations > 0
+                   ? o->max_iterations
+                   : std::max<uint64_t>(5, (uint64_t)std::llround(std::log2((double)n)));
+  r->n_trees = o->n_trees > 0
+                   ? o->n_trees
+                   : std::min<uint64_t>(48, 8 + (uint64_t)std::llround(std::sqrt((double)n) / 10.0));
+  r->leaf_size = o->leaf_size > 0 ? o->leaf_size : std::max<uint64_t>(r->k, 10);
+  uint64_t S = 64;
+  while (S < 4 * r->k) S *= 2;
+  if (o->slots) {
+    S = o->slots;
+    if (S < 64 || (S & (S - 1)) != 0) return fail(PSQ_E_INVALID, "slots must be a power of two >= 64");
+  }
+  r->S = S;
+  r->stop_at = (uint64_t)(o->delta * (double)n * (double)r->k);
+  r->seed = o->seed;
+  r->dp = round_up((uint32_t)d, 32);
+  if (r->k > 64) return fail(PSQ_E_UNSUPPORTED, "k > 64 is not supported on the GPU path");
+  if (r->m > 64) return fail(PSQ_E_UNSUPPORTED, "max_candidates > 64 is not supported on the GPU path");
+  if (r->S + r->k > 512) return fail(PSQ_E_UNSUPPORTED, "slots + k > 512 is not supported");
+  if (o->precision != PSQ_PRECISION_FP32)
+    return fail(PSQ_E_UNSUPPORTED, "precision %d is not available in this build", o->precision);
+  if (o->init != PSQ_INIT_RANDOM && o->init != PSQ_INIT_RP_FOREST)
+    return fail(PSQ_E_UNSUPPORTED, "init %d is not available in this build", o->init);
+  return PSQ_OK;
+}
+
+int launch_merge(const Resolved& R, uint64_t* G, uint64_t* slots, uint8_t* touched, float* thr,
+                 int force, unsigned long long* changed, hipStream_t s) {
+  const unsigned grid = grid_for(R.n, 4);
+  const uint64_t need = R.S + R.k;
+  if (need <= 128)
+    hipLaunchKernelGGL(k_merge<2>, dim3(grid), dim3(256), 0, s, G, (uint32_t)R.n, (uint32_t)R.k,
+                       slots, (uint32_t)R.S, touched, thr, force, changed);
+  else if (need <= 256)
+    hipLaunchKernelGGL(k_merge<4>, dim3(grid), dim3(256), 0, s, G, (uint32_t)R.n, (uint32_t)R.k,
+                       slots, (uint32_t)R.S, touched, thr, force, changed);
+  else
+    hipLaunchKernelGGL(k_merge<8>, dim3(grid), dim3(256), 0, s, G, (uint32_t)R.n, (uint32_t)R.k,
+                       slots, (uint32_t)R.S, touched, thr, force, changed);
+  HIPC(hipGetLastError());
+  return PSQ_OK;
+}
+
+int launch_join(const Resolved& R, const float* Xp, const uint32_t* newl, const uint32_t* oldl,
+                const uint32_t* ncnt, const float* thr, uint64_t* slots, uint8_t* touched,
+                uint64_t iter, unsigned long long* counters, hipStream_t s) {
+  const unsigned grid = grid_for(R.n, 4);
+  const size_t smem = (size_t)4 * R.m * R.dp * sizeof(float);
+  const int cpl = (2 * R.m > 64) ? 2 : 1;
+  const bool d50 = (R.d == 50);
+  void (*kern)(const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, const uint32_t*,
+               const uint32_t*, uint32_t, const float*, uint64_t*, uint8_t*, uint32_t, uint64_t,
+               uint64_t, unsigned long long*);
+  if (d50)
+    kern = cpl == 1 ? k_join<50, 1> : k_join<50, 2>;
+  else
+    kern = cpl == 1 ? k_join<0, 1> : k_join<0, 2>;
+  if (smem > 65536) HIPC(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(256), smem, s, Xp, (uint32_t)R.n, (uint32_t)R.d, R.dp,
+                     newl, oldl, ncnt, (uint32_t)R.m, thr, slots, touched, (uint32_t)R.S, R.seed,
+                     iter, counters);
+  HIPC(hipGetLastError());
+  return PSQ_OK;
+}
+
+double ms_between(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return -1.0;
+  return ms;
+}
+
+struct EventPool {
+  std::vector<hipEvent_t> ev;
+  ~EventPool() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+  }
+  int make(hipEvent_t* out) {
+    hipEvent_t e;
+    hipError_t rc = hipEventCreate(&e);
+    if (rc != hipSuccess) return fail(PSQ_E_DEVICE, "hipEventCreate failed: %s", hipGetErrorString(rc));
+    ev.push_back(e);
+    *out = e;
+    return PSQ_OK;
+  }
+};
+
+int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
+                          const psq_nndescent_opts* opts, uint32_t* d_indptr, uint64_t indptr_len,
+                          uint32_t* d_indices, float* d_dists, uint64_t cap, uint64_t* nnz_out,
+                          hipStream_t s) {
+  auto t0 = std::chrono::steady_clock::now();
+  Resolved R;
+  TRY(resolve(n_obs, n_dim, opts, &R));
+  if (indptr_len != n_obs + 1)
+    return fail(PSQ_E_INVALID, "indptr must have n_obs + 1 = %llu entries (got %llu)",
+                (unsigned long long)(n_obs + 1), (unsigned long long)indptr_len);
+  const uint64_t need = n_obs * R.k;
+  if (cap < need)
+    return fail(PSQ_E_INVALID, "outputs must hold %llu row pointers and %llu neighbours",
+                (unsigned long long)(n_obs + 1), (unsigned long long)need);
+  if (!d_data || !d_indptr || !d_indices || !d_dists || !nnz_out)
+    return fail(PSQ_E_INVALID, "NULL pointer argument");
+  if (opts->init == PSQ_INIT_RP_FOREST && R.n_trees > 0 && n_obs > R.leaf_size)
+    return fail(PSQ_E_UNSUPPORTED, "the RP-forest start is not built yet; use init=PSQ_INIT_RANDOM");
+
+  Workspace* ws;
+  TRY(workspace(&ws));
+  const uint32_t n = (uint32_t)R.n, k = (uint32_t)R.k, m = (uint32_t)R.m, S = (uint32_t)R.S;
+  float* Xp;
+  uint64_t *G, *slots, *rev;
+  float* thr;
+  uint8_t* touched;
+  uint32_t *rcount, *roff, *newl, *oldl, *ncnt, *rowcnt;
+  unsigned long long* ctr;
+  TRY(wbuf(ws, "Xp", (size_t)n * R.dp, &Xp));
+  TRY(wbuf(ws, "G", (size_t)n * k, &G));
+  TRY(wbuf(ws, "slots", (size_t)n * S, &slots));
+  TRY(wbuf(ws, "thr", n, &thr));
+  TRY(wbuf(ws, "touched", n, &touched));
+  TRY(wbuf(ws, "rcount", n, &rcount));
+  TRY(wbuf(ws, "roff", (size_t)n + 1, &roff));
+  TRY(wbuf(ws, "rev", (size_t)n * k, &rev));
+  TRY(wbuf(ws, "newl", (size_t)n * m, &newl));
+  TRY(wbuf(ws, "oldl", (size_t)n * m, &oldl));
+  TRY(wbuf(ws, "ncnt", n, &ncnt));
+  TRY(wbuf(ws, "rowcnt", n, &rowcnt));
+  TRY(wbuf(ws, "counters", 8, &ctr));
+  void* hp;
+  TRY(ws->host_counters(&hp));
+  unsigned long long* hctr = static_cast<unsigned long long*>(hp);
+
+  EventPool evp;
+  hipEvent_t e_start, e_init, e_iter_end, e_end;
+  TRY(evp.make(&e_start));
+  TRY(evp.make(&e_init));
+  TRY(evp.make(&e_iter_end));
+  TRY(evp.make(&e_end));
+  HIPC(hipEventRecord(e_start, s));
+
+  hipLaunchKernelGGL(k_pad_rows, dim3(std::min<unsigned>(grid_for((size_t)n * R.dp, 256), 65536)),
+                     dim3(256), 0, s, d_data, n, (uint32_t)R.d, R.dp, Xp);
+  HIPC(hipGetLastError());
+  TRY(fill_u64(G, (size_t)n * k, EMPTY_KEY, s));
+  TRY(fill_u64(slots, (size_t)n * S, EMPTY_KEY, s));
+  TRY(fill_u8(touched, n, 0, s));
+  TRY(fill_u32(rcount, n, 0, s));
+  TRY(fill_u64(reinterpret_cast<uint64_t*>(ctr), 8, 0, s));
+  {
+    float inf = __builtin_inff();
+    uint32_t bits;
+    std::memcpy(&bits, &inf, 4);
+    TRY(fill_u32(reinterpret_cast<uint32_t*>(thr), n, bits, s));
+  }
+
+  // ── start: random fill, then one forced merge sorts every row ──
+  hipLaunchKernelGGL(k_random_fill, dim3(grid_for(n, 256)), dim3(256), 0, s, Xp, n,
+                     (uint32_t)R.d, R.dp, G, k, R.seed, touched);
+  HIPC(hipGetLastError());
+  TRY(launch_merge(R, G, slots, touched, thr, 1, ctr + 2, s));
+  HIPC(hipEventRecord(e_init, s));
+
+  // ── iterations ──
+  psq_nndescent_stats st;
+  std::memset(&st, 0, sizeof(st));
+  std::vector<hipEvent_t> jb, je;
+  uint64_t iters = 0;
+  for (uint64_t it = 0; it < R.n_iters; ++it) {
+    TRY(fill_u64(reinterpret_cast<uint64_t*>(ctr), 4, 0, s));
+    hipLaunchKernelGGL(k_rev_count, dim3(grid_for((size_t)n * k, 256)), dim3(256), 0, s, G,
+                       (size_t)n * k, rcount);
+    HIPC(hipGetLastError());
+    TRY(scan_u32(ws, rcount, roff, n, s));
+    hipLaunchKernelGGL(k_rev_scatter, dim3(grid_for((size_t)n * k, 256)), dim3(256), 0, s, G, n, k,
+                       R.seed, it, roff, rcount, rev);
+    HIPC(hipGetLastError());
+    hipLaunchKernelGGL(k_sample_select, dim3(grid_for(n, 4)), dim3(256), 0, s, G, n, k, m, R.seed,
+                       it, roff, rev, newl, oldl, ncnt);
+    HIPC(hipGetLastError());
+    hipEvent_t a, b;
+    TRY(evp.make(&a));
+    TRY(evp.make(&b));
+    HIPC(hipEventRecord(a, s));
+    TRY(launch_join(R, Xp, newl, oldl, ncnt, thr, slots, touched, it, ctr, s));
+    HIPC(hipEventRecord(b, s));
+    jb.push_back(a);
+    je.push_back(b);
+    TRY(launch_merge(R, G, slots, touched, thr, 0, ctr + 2, s));
+    HIPC(hipMemcpyAsync(hctr, ctr, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    const uint64_t changed = hctr[2];
+    if (it < 64) {
+      st.changed[it] = changed;
+      st.pairs[it] = hctr[0];
+      st.cand_rows[it] = hctr[1];
+    }
+    ++iters;
+    if (changed <= R.stop_at) break;
+  }
+  HIPC(hipEventRecord(e_iter_end, s));
+
+  // ── extraction ──
+  hipLaunchKernelGGL(k_row_counts, dim3(grid_for(n, 256)), dim3(256), 0, s, G, n, k, rowcnt);
+  HIPC(hipGetLastError());
+  TRY(scan_u32(ws, rowcnt, d_indptr, n, s));
+  hipLaunchKernelGGL(k_extract, dim3(grid_for((size_t)n * k, 256)), dim3(256), 0, s, G, n, k,
+                     d_indptr, d_indices, d_dists);
+  HIPC(hipGetLastError());
+  HIPC(hipEventRecord(e_end, s));
+  uint32_t nnz32 = 0;
+  HIPC(hipMemcpyAsync(&nnz32, d_indptr + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIPC(hipStreamSynchronize(s));
+  *nnz_out = nnz32;
+
+  st.n_obs = R.n;
+  st.n_dim = R.d;
+  st.k = R.k;
+  st.max_candidates = R.m;
+  st.max_iterations = R.n_iters;
+  st.n_trees = opts->init == PSQ_INIT_RP_FOREST ? R.n_trees : 0;
+  st.leaf_size = R.leaf_size;
+  st.slots = R.S;
+  st.stop_at = R.stop_at;
+  st.iterations = iters;
+  st.ms_init = ms_between(e_start, e_init);
+  st.ms_iterations = ms_between(e_init, e_iter_end);
+  st.ms_extract = ms_between(e_iter_end, e_end);
+  for (size_t i = 0; i < jb.size() && i < 64; ++i) st.ms_join[i] = ms_between(jb[i], je[i]);
+  st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  g_stats = st;
+  return PSQ_OK;
+}
+
+int set_device(int32_t device) {
+  int count = 0;
+  HIPC(hipGetDeviceCount(&count));
+  if (count <= 0) return fail(PSQ_E_DEVICE, "no HIP device is visible");
+  if (device < 0 || device >= count)
+    return fail(PSQ_E_INVALID, "device %d out of range (%d visible)", device, count);
+  HIPC(hipSetDevice(device));
+  return PSQ_OK;
+}
+
+int knn_exact_device_impl(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d_queries,
+                          uint64_t nq, uint64_t k, uint32_t* d_idx, float* d_dist, hipStream_t s) {
+  if (n == 0 || d == 0) return fail(PSQ_E_INVALID, "empty data: n=%llu d=%llu",
+                                    (unsigned long long)n, (unsigned long long)d);
+  if (k == 0) return fail(PSQ_E_INVALID, "k must be >= 1");
+  if (!d_queries && nq != n) return fail(PSQ_E_INVALID, "queries is NULL but n_q != n");
+  if (k > 64) return fail(PSQ_E_UNSUPPORTED, "k > 64 is not supported by the exact GPU path");
+  if (d > 256) return fail(PSQ_E_UNSUPPORTED, "n_dim > 256 is not supported on the GPU path");
+  if (n >= (1ull << 31) || nq >= (1ull << 31)) return fail(PSQ_E_UNSUPPORTED, "n must be < 2^31");
+  if (!d_data || !d_idx || !d_dist) return fail(PSQ_E_INVALID, "NULL pointer argument");
+  if (nq == 0) return PSQ_OK;
+  Workspace* ws;
+  TRY(workspace(&ws));
+  const uint32_t dp = round_up((uint32_t)d, 32);
+  const uint32_t K = k <= 16 ? 16 : (k <= 32 ? 32 : 64);
+  const uint32_t cap = 128;
+  const uint32_t qblocks = (uint32_t)((nq + 255) / 256);
+  uint32_t nseg = std::max<uint32_t>(1, (uint32_t)std::min<uint64_t>((n + 63) / 64, (2048 + qblocks - 1) / qblocks));
+  uint32_t seg_len = round_up((uint32_t)((n + nseg - 1) / nseg), 64);
+  nseg = (uint32_t)((n + seg_len - 1) / seg_len);
+  float *Xp, *partial, *T;
+  uint32_t *cnt, *cand, *ovf, *novf;
+  TRY(wbuf(ws, "Xp", (size_t)n * dp, &Xp));
+  TRY(wbuf(ws, "ex_partial", (size_t)nq * nseg * K, &partial));
+  TRY(wbuf(ws, "ex_T", nq, &T));
+  TRY(wbuf(ws, "ex_cnt", nq, &cnt));
+  TRY(wbuf(ws, "ex_cand", (size_t)nq * cap, &cand));
+  TRY(wbuf(ws, "ex_ovf", nq, &ovf));
+  TRY(wbuf(ws, "ex_novf", 1, &novf));
+  hipLaunchKernelGGL(k_pad_rows, dim3(std::min<unsigned>(grid_for((size_t)n * dp, 256), 65536)),
+                     dim3(256), 0, s, d_data, (uint32_t)n, (uint32_t)d, dp, Xp);
+  HIPC(hipGetLastError());
+  TRY(fill_u32(cnt, nq, 0, s));
+  TRY(fill_u32(novf, 1, 0, s));
+  const size_t smem = (size_t)64 * dp * sizeof(float);
+  const bool d50 = d == 50;
+  dim3 g1(qblocks, nseg);
+  void (*p1)(const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t, uint32_t,
+             uint32_t, float*);
+  if (K == 16) p1 = d50 ? k_exact_pass1<50, 16> : k_exact_pass1<0, 16>;
+  else if (K == 32) p1 = d50 ? k_exact_pass1<50, 32> : k_exact_pass1<0, 32>;
+  else p1 = d50 ? k_exact_pass1<50, 64> : k_exact_pass1<0, 64>;
+  if (smem > 65536) HIPC(hipFuncSetAttribute((const void*)p1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+  hipLaunchKernelGGL(p1, g1, dim3(256), smem, s, Xp, (uint32_t)n, (uint32_t)d, dp, d_queries,
+                     (uint32_t)nq, seg_len, nseg, partial);
+  HIPC(hipGetLastError());
+  hipLaunchKernelGGL(k_exact_tau, dim3(grid_for(nq, 128)), dim3(128), 0, s, partial, (uint32_t)nq,
+                     nseg, K, (uint32_t)k, (uint32_t)d, T);
+  HIPC(hipGetLastError());
+  void (*p2)(const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t, uint32_t,
+             const float*, uint32_t*, uint32_t*, uint32_t) = d50 ? k_exact_pass2<50> : k_exact_pass2<0>;
+  if (smem > 65536) HIPC(hipFuncSetAttribute((const void*)p2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+  hipLaunchKernelGGL(p2, g1, dim3(256), smem, s, Xp, (uint32_t)n, (uint32_t)d, dp, d_queries,
+                     (uint32_t)nq, seg_len, T, cnt, cand, cap);
+  HIPC(hipGetLastError());
+  hipLaunchKernelGGL(k_exact_rerank, dim3(grid_for(nq, 4)), dim3(256), 0, s, Xp, (uint32_t)n,
+                     (uint32_t)d, dp, d_queries, (uint32_t)nq, (uint32_t)k, cnt, cand, cap, d_idx,
+                     d_dist, ovf, novf);
+  HIPC(hipGetLastError());
+  uint32_t h_novf = 0;
+  HIPC(hipMemcpyAsync(&h_novf, novf, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIPC(hipStreamSynchronize(s));
+  if (h_novf > 0) {
+    double* sd;
+    uint32_t* si;
+    TRY(wbuf(ws, "ex_fb_d", (size_t)h_novf * k, &sd));
+    TRY(wbuf(ws, "ex_fb_i", (size_t)h_novf * k, &si));
+    hipLaunchKernelGGL(k_exact_fallback, dim3(grid_for(h_novf, 64)), dim3(64), 0, s, Xp,
+                       (uint32_t)n, (uint32_t)d, dp, d_queries, (uint32_t)k, T, ovf, h_novf, sd, si,
+                       d_idx, d_dist);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(s));
+  }
+  return PSQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void psq_nndescent_opts_default(psq_nndescent_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->k = 15;
+  o->delta = 0.001;
+  o->precision = PSQ_PRECISION_FP32;
+  o->init = PSQ_INIT_RP_FOREST;
+}
+
+uint64_t psq_effective_k(uint64_t n_obs, uint64_t k) {
+  uint64_t nm1 = n_obs > 0 ? n_obs - 1 : 0;
+  uint64_t e = k < nm1 ? k : nm1;
+  return e > 0 ? e : 1;
+}
+
+const char* psq_last_error(void) { return g_err.c_str(); }
+
+int psq_last_stats(psq_nndescent_stats* out) {
+  if (!out) return fail(PSQ_E_INVALID, "out is NULL");
+  *out = g_stats;
+  return PSQ_OK;
+}
+
+void psq_set_device_budget(uint64_t bytes) { g_budget = bytes; }
+
+int psq_release_workspace(void) {
+  Workspace* ws;
+  TRY(workspace(&ws));
+  std::lock_guard<std::mutex> g(g_ws_mu);
+  ws->release();
+  return PSQ_OK;
+}
+
+int psq_nndescent_device(const float* d_data, uint64_t n_obs, uint64_t n_dim,
+                         const psq_nndescent_opts* opts, uint32_t* d_indptr, uint64_t indptr_len,
+                         uint32_t* d_indices, float* d_dists, uint64_t cap, uint64_t* nnz_out,
+                         void* stream) {
+  g_err.clear();
+  return nndescent_device_impl(d_data, n_obs, n_dim, opts, d_indptr, indptr_len, d_indices, d_dists,
+                               cap, nnz_out, static_cast<hipStream_t>(stream));
+}
+
+int psq_nndescent_into(const float* data, uint64_t n_obs, uint64_t n_dim,
+                       const psq_nndescent_opts* opts, uint32_t* indptr, uint64_t indptr_len,
+                       uint32_t* indices, float* dists, uint64_t cap, uint64_t* nnz_out) {
+  g_err.clear();
+  Resolved R;
+  TRY(resolve(n_obs, n_dim, opts, &R));
+  if (!data || !indptr || !indices || !dists || !nnz_out)
+    return fail(PSQ_E_INVALID, "NULL pointer argument");
+  if (indptr_len != n_obs + 1)
+    return fail(PSQ_E_INVALID, "indptr must have n_obs + 1 = %llu entries (got %llu)",
+                (unsigned long long)(n_obs + 1), (unsigned long long)indptr_len);
+  if (cap < n_obs * R.k)
+    return fail(PSQ_E_INVALID, "outputs must hold %llu row pointers and %llu neighbours",
+                (unsigned long long)(n_obs + 1), (unsigned long long)(n_obs * R.k));
+  TRY(set_device(opts->device));
+  Workspace* ws;
+  TRY(workspace(&ws));
+  float* d_data;
+  uint32_t *d_ip, *d_ix;
+  float* d_dx;
+  TRY(wbuf(ws, "io_data", (size_t)n_obs * n_dim, &d_data));
+  TRY(wbuf(ws, "io_indptr", (size_t)n_obs + 1, &d_ip));
+  TRY(wbuf(ws, "io_indices", (size_t)n_obs * R.k, &d_ix));
+  TRY(wbuf(ws, "io_dists", (size_t)n_obs * R.k, &d_dx));
+  HIPC(hipMemcpy(d_data, data, (size_t)n_obs * n_dim * sizeof(float), hipMemcpyHostToDevice));
+  uint64_t nnz = 0;
+  TRY(nndescent_device_impl(d_data, n_obs, n_dim, opts, d_ip, n_obs + 1, d_ix, d_dx,
+                            n_obs * R.k, &nnz, nullptr));
+  HIPC(hipMemcpy(indptr, d_ip, (size_t)(n_obs + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  HIPC(hipMemcpy(indices, d_ix, (size_t)nnz * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  HIPC(hipMemcpy(dists, d_dx, (size_t)nnz * sizeof(float), hipMemcpyDeviceToHost));
+  *nnz_out = nnz;
+  return PSQ_OK;
+}
+
+int psq_knn_exact_device(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d_queries,
+                         uint64_t n_q, uint64_t k, uint32_t* d_idx_out, float* d_dist_out,
+                         void* stream) {
+  g_err.clear();
+  return knn_exact_device_impl(d_data, n, d, d_queries, n_q, k, d_idx_out, d_dist_out,
+                               static_cast<hipStream_t>(stream));
+}
+
+int psq_knn_exact(const float* data, uint64_t n, uint64_t d, const uint32_t* queries, uint64_t n_q,
+                  uint64_t k, uint32_t* idx_out, float* dist_out, int32_t device) {
+  g_err.clear();
+  if (n == 0 || d == 0) return fail(PSQ_E_INVALID, "empty data: n=%llu d=%llu",
+                                    (unsigned long long)n, (unsigned long long)d);
+  if (k == 0) return fail(PSQ_E_INVALID, "k must be >= 1");
+  if (!queries && n_q != n) return fail(PSQ_E_INVALID, "queries is NULL but n_q != n");
+  if (!data || !idx_out || !dist_out) return fail(PSQ_E_INVALID, "NULL pointer argument");
+  if (queries)
+    for (uint64_t i = 0; i < n_q; ++i)
+      if (queries[i] >= n) return fail(PSQ_E_INVALID, "query %llu is out of range", (unsigned long long)i);
+  TRY(set_device(device));
+  Workspace* ws;
+  TRY(workspace(&ws));
+  float* d_data;
+  uint32_t *d_q = nullptr, *d_idx;
+  float* d_dist;
+  TRY(wbuf(ws, "io_data", (size_t)n * d, &d_data));
+  if (queries) TRY(wbuf(ws, "io_queries", (size_t)n_q, &d_q));
+  TRY(wbuf(ws, "io_indices", (size_t)std::max<uint64_t>(n_q * k, 1), &d_idx));
+  TRY(wbuf(ws, "io_dists", (size_t)std::max<uint64_t>(n_q * k, 1), &d_dist));
+  HIPC(hipMemcpy(d_data, data, (size_t)n * d * sizeof(float), hipMemcpyHostToDevice));
+  if (queries) HIPC(hipMemcpy(d_q, queries, (size_t)n_q * sizeof(uint32_t), hipMemcpyHostToDevice));
+  TRY(knn_exact_device_impl(d_data, n, d, d_q, n_q, k, d_idx, d_dist, nullptr));
+  HIPC(hipMemcpy(idx_out, d_idx, (size_t)n_q * k * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  HIPC(hipMemcpy(dist_out, d_dist, (size_t)n_q * k * sizeof(float), hipMemcpyDeviceToHost));
+  return PSQ_OK;
+}
+
+int psq_neighbors_exact(const float* embedding, uint64_t n_obs, uint64_t n_dim,
+                        uint64_t n_neighbors, int32_t metric, uint32_t* indptr, uint32_t* indices,
+                        float* dists, uint64_t cap, uint64_t* nnz_out, int32_t device) {
+  g_err.clear();
+  // neighbors.rs:107-115
+  if (n_obs == 0 || n_dim == 0)
+    return fail(PSQ_E_INVALID, "embedding has zero rows (%llu) or dims (%llu)",
+                (unsigned long long)n_obs, (unsigned long long)n_dim);
+  if (n_neighbors == 0) return fail(PSQ_E_INVALID, "n_neighbors must be >= 1, got 0");
+  if (metric != 0 && metric != 1) return fail(PSQ_E_INVALID, "unknown metric %d", metric);
+  if (!embedding || !indptr || !nnz_out) return fail(PSQ_E_INVALID, "NULL pointer argument");
+  const uint64_t kws = std::min<uint64_t>(n_neighbors, n_obs);
+  const uint64_t kk = kws - 1;
+  if (cap < n_obs * kk)
+    return fail(PSQ_E_INVALID, "outputs must hold %llu neighbours", (unsigned long long)(n_obs * kk));
+  indptr[0] = 0;
+  if (kk == 0) {
+    for (uint64_t i = 0; i <= n_obs; ++i) indptr[i] = 0;
+    *nnz_out = 0;
+    return PSQ_OK;
+  }
+  std::vector<float> unit;
+  const float* src = embedding;
+  if (metric == 1) {  // cosine: unit rows; 1 - cos = |a - b|^2 / 2 on unit vectors
+    unit.resize(n_obs * n_dim);
+    for (uint64_t i = 0; i < n_obs; ++i) {
+      double s = 0;
+      for (uint64_t t = 0; t < n_dim; ++t) s += (double)embedding[i * n_dim + t] * embedding[i * n_dim + t];
+      double nr = std::max(std::sqrt(s), 1e-12);
+      for (uint64_t t = 0; t < n_dim; ++t) unit[i * n_dim + t] = (float)(embedding[i * n_dim + t] / nr);
+    }
+    src = unit.data();
+  }
+  std::vector<float> dd(n_obs * kk);
+  TRY(psq_knn_exact(src, n_obs, n_dim, nullptr, n_obs, kk, indices, dd.data(), device));
+  for (uint64_t i = 0; i < n_obs; ++i) {
+    for (uint64_t s = 0; s < kk; ++s) {
+      float v = dd[i * kk + s];
+      dists[i * kk + s] = metric == 1 ? (float)((double)v * (double)v / 2.0) : v;
+    }
+    indptr[i + 1] = (uint32_t)((i + 1) * kk);
+  }
+  *nnz_out = n_obs * kk;
+  return PSQ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,343 @@
+// Exact kNN for gfx950: FP32 filter with a proven error margin, FP64 re-rank.
+//
+// Order and tie rule follow by_distance / knn_exact of the reference
+// (crates/polariseq-core/src/doublets.rs:544-651): distance, then index, the
+// query itself excluded. The ranking distance is FP64, Σ (double(a)−double(b))²
+// summed in dimension order with no fused multiply-add, so the result is
+// bit-exact against the CPU oracle that computes the same sequence.
+//
+// pass 1  one lane per query, points streamed through LDS and read by
+//         broadcast; each lane keeps its K smallest FP32 distances per point
+//         segment (registers, sorted insertion).
+// tau     the k-th smallest over all segments -> threshold T = tau * F, where
+//         F = 1 + 4 (d + 8) 2^-24 bounds the FP32 error of both ranks (each
+//         FP32 distance is within (d + 4) u of the exact value, relative,
+//         because every term is non-negative).
+// pass 2  the same scan; every point with d32 <= T becomes a candidate.
+//         The true top-k is inside: for a true neighbour p,
+//         d32(p) <= (1 + e) d(p) <= (1 + e) tau* <= tau (1 + e) / (1 - e) <= T.
+// rerank  one wave per query ranks its candidates by (d64, index).
+// fallback  a query with more candidates than the buffer (many exact ties)
+//         is finished by one thread scanning every point.
+#include "psq_kernels.hpp"
+
+namespace psq {
+
+constexpr int EX_BLOCK = 256;  // queries per block (one per lane)
+constexpr int EX_TILE = 64;    // points per LDS tile
+
+// FP32 distance of the filter: four FMA chains, a fixed order, so pass 1 and
+// pass 2 produce the same bits.
+template <int D>
+__device__ inline float filt_dist_fixed(const float* __restrict__ p, const float (&q)[D]) {
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+#pragma unroll
+  for (int t = 0; t < D; t += 4) {
+    float d0 = p[t] - q[t];
+    a0 = fmaf(d0, d0, a0);
+    if (t + 1 < D) {
+      float d1 = p[t + 1] - q[t + 1];
+      a1 = fmaf(d1, d1, a1);
+    }
+    if (t + 2 < D) {
+      float d2 = p[t + 2] - q[t + 2];
+      a2 = fmaf(d2, d2, a2);
+    }
+    if (t + 3 < D) {
+      float d3 = p[t + 3] - q[t + 3];
+      a3 = fmaf(d3, d3, a3);
+    }
+  }
+  return (a0 + a1) + (a2 + a3);
+}
+__device__ inline float filt_dist_rt(const float* __restrict__ p, const float* __restrict__ q, int d) {
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+  int t = 0;
+  for (; t + 4 <= d; t += 4) {
+    float d0 = p[t] - q[t], d1 = p[t + 1] - q[t + 1], d2 = p[t + 2] - q[t + 2],
+          d3 = p[t + 3] - q[t + 3];
+    a0 = fmaf(d0, d0, a0);
+    a1 = fmaf(d1, d1, a1);
+    a2 = fmaf(d2, d2, a2);
+    a3 = fmaf(d3, d3, a3);
+  }
+  if (t < d) {
+    float d0 = p[t] - q[t];
+    a0 = fmaf(d0, d0, a0);
+  }
+  if (t + 1 < d) {
+    float d1 = p[t + 1] - q[t + 1];
+    a1 = fmaf(d1, d1, a1);
+  }
+  if (t + 2 < d) {
+    float d2 = p[t + 2] - q[t + 2];
+    a2 = fmaf(d2, d2, a2);
+  }
+  return (a0 + a1) + (a2 + a3);
+}
+
+// The FP64 ranking distance, identical operation sequence to the oracle.
+__device__ inline double dist64(const float* __restrict__ a, const float* __restrict__ b, int d) {
+#pragma clang fp contract(off)
+  double s = 0.0;
+  for (int t = 0; t < d; ++t) {
+    double x = (double)a[t] - (double)b[t];
+    s = s + x * x;
+  }
+  return s;
+}
+
+__device__ inline void load_tile(const float* __restrict__ Xp, uint32_t dp, uint32_t p0,
+                                 uint32_t np, float* tile) {
+  // np points of dp floats, float4 at a time
+  const uint32_t n4 = np * dp / 4;
+  const float4* src = reinterpret_cast<const float4*>(Xp + (size_t)p0 * dp);
+  float4* dst = reinterpret_cast<float4*>(tile);
+  for (uint32_t e = threadIdx.x; e < n4; e += blockDim.x) dst[e] = src[e];
+}
+
+template <int D, int K>
+__global__ __launch_bounds__(EX_BLOCK) void k_exact_pass1(const float* __restrict__ Xp, uint32_t n,
+                                                          uint32_t d, uint32_t dp,
+                                                          const uint32_t* __restrict__ queries,
+                                                          uint32_t nq, uint32_t seg_len,
+                                                          uint32_t nseg, float* __restrict__ partial) {
+  extern __shared__ float tile[];  // EX_TILE * dp
+  const uint32_t qslot = blockIdx.x * EX_BLOCK + threadIdx.x;
+  const bool qvalid = qslot < nq;
+  const uint32_t qi = qvalid ? (queries ? queries[qslot] : qslot) : 0u;
+  constexpr int DR = D > 0 ? D : 1;
+  float qv[DR];
+  if (D > 0) {
+#pragma unroll
+    for (int t = 0; t < DR; ++t) qv[t] = Xp[(size_t)qi * dp + t];
+  }
+  const float* qrow = Xp + (size_t)qi * dp;
+  float top[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) top[j] = __builtin_inff();
+  const uint32_t s0 = blockIdx.y * seg_len;
+  const uint32_t s1 = min(n, s0 + seg_len);
+  for (uint32_t base = s0; base < s1; base += EX_TILE) {
+    const uint32_t np = min((uint32_t)EX_TILE, s1 - base);
+    __syncthreads();
+    load_tile(Xp, dp, base, np, tile);
+    __syncthreads();
+    for (uint32_t j = 0; j < np; ++j) {
+      const float* pr = tile + (size_t)j * dp;
+      float dist = D > 0 ? filt_dist_fixed<DR>(pr, qv) : filt_dist_rt(pr, qrow, (int)d);
+      if (base + j == qi) dist = __builtin_inff();
+      if (dist < top[K - 1]) {
+#pragma unroll
+        for (int t = K - 1; t > 0; --t) {
+          float a = top[t - 1];
+          top[t] = dist < a ? a : (dist < top[t] ? dist : top[t]);
+        }
+        top[0] = dist < top[0] ? dist : top[0];
+      }
+    }
+  }
+  if (qvalid) {
+    float* out = partial + ((size_t)qslot * nseg + blockIdx.y) * K;
+#pragma unroll
+    for (int j = 0; j < K; ++j) out[j] = top[j];
+  }
+}
+
+__global__ void k_exact_tau(const float* __restrict__ partial, uint32_t nq, uint32_t nseg,
+                            uint32_t K, uint32_t k, uint32_t d, float* __restrict__ T) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  float top[64];
+#pragma unroll
+  for (int j = 0; j < 64; ++j) top[j] = __builtin_inff();
+  const float* src = partial + (size_t)q * nseg * K;
+  for (uint32_t e = 0; e < nseg * K; ++e) {
+    float v = src[e];
+    if (v < top[63]) {
+#pragma unroll
+      for (int t = 63; t > 0; --t) {
+        float a = top[t - 1];
+        top[t] = v < a ? a : (v < top[t] ? v : top[t]);
+      }
+      top[0] = v < top[0] ? v : top[0];
+    }
+  }
+  float tau = __builtin_inff();
+#pragma unroll
+  for (int j = 0; j < 64; ++j)
+    if ((uint32_t)j == k - 1) tau = top[j];
+  if (tau >= __builtin_inff()) {
+    T[q] = tau;
+    return;
+  }
+  // F = 1 + 4 (d + 8) 2^-24, applied in FP64 and rounded up to FP32; the small
+  // absolute term covers underflow of squared differences.
+  double F = 1.0 + 4.0 * ((double)d + 8.0) * 5.9604644775390625e-08;
+  double Td = (double)tau * F + 1e-30;
+  float Tf = (float)Td;
+  if ((double)Tf < Td) Tf = nextafterf(Tf, __builtin_inff());
+  T[q] = Tf;
+}
+
+template <int D>
+__global__ __launch_bounds__(EX_BLOCK) void k_exact_pass2(const float* __restrict__ Xp, uint32_t n,
+                                                          uint32_t d, uint32_t dp,
+                                                          const uint32_t* __restrict__ queries,
+                                                          uint32_t nq, uint32_t seg_len,
+                                                          const float* __restrict__ T,
+                                                          uint32_t* __restrict__ cnt,
+                                                          uint32_t* __restrict__ cand, uint32_t cap) {
+  extern __shared__ float tile[];
+  const uint32_t qslot = blockIdx.x * EX_BLOCK + threadIdx.x;
+  const bool qvalid = qslot < nq;
+  const uint32_t qi = qvalid ? (queries ? queries[qslot] : qslot) : 0u;
+  const float thr = qvalid ? T[qslot] : -1.0f;
+  constexpr int DR = D > 0 ? D : 1;
+  float qv[DR];
+  if (D > 0) {
+#pragma unroll
+    for (int t = 0; t < DR; ++t) qv[t] = Xp[(size_t)qi * dp + t];
+  }
+  const float* qrow = Xp + (size_t)qi * dp;
+  const uint32_t s0 = blockIdx.y * seg_len;
+  const uint32_t s1 = min(n, s0 + seg_len);
+  for (uint32_t base = s0; base < s1; base += EX_TILE) {
+    const uint32_t np = min((uint32_t)EX_TILE, s1 - base);
+    __syncthreads();
+    load_tile(Xp, dp, base, np, tile);
+    __syncthreads();
+    for (uint32_t j = 0; j < np; ++j) {
+      const float* pr = tile + (size_t)j * dp;
+      float dist = D > 0 ? filt_dist_fixed<DR>(pr, qv) : filt_dist_rt(pr, qrow, (int)d);
+      if (qvalid && dist <= thr && base + j != qi) {
+        uint32_t pos = atomicAdd(&cnt[qslot], 1u);
+        if (pos < cap) cand[(size_t)qslot * cap + pos] = base + j;
+      }
+    }
+  }
+}
+
+// One wave per query: rank each candidate by (d64, index) among all of them.
+__global__ __launch_bounds__(256) void k_exact_rerank(
+    const float* __restrict__ Xp, uint32_t n, uint32_t d, uint32_t dp,
+    const uint32_t* __restrict__ queries, uint32_t nq, uint32_t k, const uint32_t* __restrict__ cnt,
+    const uint32_t* __restrict__ cand, uint32_t cap, uint32_t* __restrict__ idx_out,
+    float* __restrict__ dist_out, uint32_t* __restrict__ overflow, uint32_t* __restrict__ n_overflow) {
+  constexpr int CPL = 2;  // cap <= 128
+  const int w = threadIdx.x / WAVE;
+  const int l = lane_id();
+  const uint32_t qs = blockIdx.x * 4 + w;
+  if (qs >= nq) return;  // wave-uniform
+  const uint32_t c = cnt[qs];
+  if (c > cap) {
+    if (l == 0) overflow[atomicAdd(n_overflow, 1u)] = qs;
+    return;
+  }
+  const uint32_t qi = queries ? queries[qs] : qs;
+  const float* qrow = Xp + (size_t)qi * dp;
+  double dv[CPL];
+  uint32_t iv[CPL];
+#pragma unroll
+  for (int r = 0; r < CPL; ++r) {
+    const uint32_t e = r * WAVE + l;
+    if (e < c) {
+      iv[r] = cand[(size_t)qs * cap + e];
+      dv[r] = dist64(qrow, Xp + (size_t)iv[r] * dp, (int)d);
+    } else {
+      iv[r] = EMPTY;
+      dv[r] = __builtin_inf();
+    }
+  }
+  uint32_t rank[CPL] = {0, 0};
+  for (uint32_t j = 0; j < c; ++j) {
+    const int src = (int)(j & (WAVE - 1));
+    const int rj = (int)(j / WAVE);
+    double dj = 0.0;
+    uint32_t ij = 0;
+#pragma unroll
+    for (int r = 0; r < CPL; ++r) {
+      double t = __shfl(dv[r], src, WAVE);
+      uint32_t u = (uint32_t)__shfl((int)iv[r], src, WAVE);
+      if (r == rj) {
+        dj = t;
+        ij = u;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < CPL; ++r) rank[r] += (dj < dv[r] || (dj == dv[r] && ij < iv[r])) ? 1u : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < CPL; ++r) {
+    const uint32_t e = r * WAVE + l;
+    if (e < c && rank[r] < k) {
+      idx_out[(size_t)qs * k + rank[r]] = iv[r];
+      dist_out[(size_t)qs * k + rank[r]] = (float)__dsqrt_rn(dv[r]);
+    }
+  }
+  for (uint32_t pos = c + l; pos < k; pos += WAVE) {
+    idx_out[(size_t)qs * k + pos] = EMPTY;
+    dist_out[(size_t)qs * k + pos] = __builtin_inff();
+  }
+}
+
+// One thread per overflowed query: scan every point, keep the k smallest
+// (d64, index) among those passing the FP32 threshold (sorted insertion in a
+// global scratch list).
+__global__ void k_exact_fallback(const float* __restrict__ Xp, uint32_t n, uint32_t d, uint32_t dp,
+                                 const uint32_t* __restrict__ queries, uint32_t k,
+                                 const float* __restrict__ T, const uint32_t* __restrict__ overflow,
+                                 uint32_t n_overflow, double* __restrict__ sd,
+                                 uint32_t* __restrict__ si, uint32_t* __restrict__ idx_out,
+                                 float* __restrict__ dist_out) {
+  const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= n_overflow) return;
+  const uint32_t qs = overflow[o];
+  const uint32_t qi = queries ? queries[qs] : qs;
+  const float* qrow = Xp + (size_t)qi * dp;
+  const float thr = T[qs];
+  double* ld = sd + (size_t)o * k;
+  uint32_t* li = si + (size_t)o * k;
+  uint32_t cntk = 0;
+  for (uint32_t p = 0; p < n; ++p) {
+    if (p == qi) continue;
+    const float* pr = Xp + (size_t)p * dp;
+    if (!(filt_dist_rt(pr, qrow, (int)d) <= thr)) continue;
+    double dd = dist64(qrow, pr, (int)d);
+    if (cntk == k && !(dd < ld[k - 1] || (dd == ld[k - 1] && p < li[k - 1]))) continue;
+    uint32_t pos = cntk < k ? cntk : k - 1;
+    while (pos > 0 && (dd < ld[pos - 1] || (dd == ld[pos - 1] && p < li[pos - 1]))) {
+      ld[pos] = ld[pos - 1];
+      li[pos] = li[pos - 1];
+      --pos;
+    }
+    ld[pos] = dd;
+    li[pos] = p;
+    if (cntk < k) ++cntk;
+  }
+  for (uint32_t s = 0; s < k; ++s) {
+    idx_out[(size_t)qs * k + s] = s < cntk ? li[s] : EMPTY;
+    dist_out[(size_t)qs * k + s] = s < cntk ? (float)__dsqrt_rn(ld[s]) : __builtin_inff();
+  }
+}
+
+template __global__ void k_exact_pass1<0, 16>(const float*, uint32_t, uint32_t, uint32_t,
+                                              const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+template __global__ void k_exact_pass1<0, 32>(const float*, uint32_t, uint32_t, uint32_t,
+                                              const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+template __global__ void k_exact_pass1<0, 64>(const float*, uint32_t, uint32_t, uint32_t,
+                                              const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+template __global__ void k_exact_pass1<50, 16>(const float*, uint32_t, uint32_t, uint32_t,
+                                               const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+template __global__ void k_exact_pass1<50, 32>(const float*, uint32_t, uint32_t, uint32_t,
+                                               const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+template __global__ void k_exact_pass1<50, 64>(const float*, uint32_t, uint32_t, uint32_t,
+                                               const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+template __global__ void k_exact_pass2<0>(const float*, uint32_t, uint32_t, uint32_t,
+                                          const uint32_t*, uint32_t, uint32_t, const float*,
+                                          uint32_t*, uint32_t*, uint32_t);
+template __global__ void k_exact_pass2<50>(const float*, uint32_t, uint32_t, uint32_t,
+                                           const uint32_t*, uint32_t, uint32_t, const float*,
+                                           uint32_t*, uint32_t*, uint32_t);
+
+}  // namespace psq

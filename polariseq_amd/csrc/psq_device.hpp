@@ -1,0 +1,221 @@
+// Device-side building blocks shared by the kNN kernels (gfx950, wave64).
+//
+// Hashes restate crates/polariseq-core/src/nndescent.rs:80-100; the 8-lane
+// squared distance restates crates/polariseq-core/src/simd.rs:45-81 operation for
+// operation (this file is compiled with -ffp-contract=off, as Rust never fuses).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace psq {
+
+constexpr uint32_t EMPTY = 0xFFFFFFFFu;
+constexpr uint64_t EMPTY_KEY = ~0ull;
+constexpr uint64_t GOLD = 0x9E3779B97F4A7C15ull;
+constexpr int WAVE = 64;
+constexpr int BUCKET = 8;  // slots per proposal bucket
+
+__host__ __device__ inline uint64_t splitmix(uint64_t x) {
+  x += GOLD;
+  uint64_t z = x;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+// nndescent.rs:91-95, the 24-bit numerator of hash_unit.
+__host__ __device__ inline uint32_t hash_prio24(uint64_t seed, uint64_t a, uint64_t b) {
+  return (uint32_t)(splitmix(seed ^ splitmix(a) ^ splitmix(b * GOLD)) >> 40);
+}
+// nndescent.rs:97-100.
+__host__ __device__ inline uint64_t hash_index(uint64_t seed, uint64_t a, uint64_t b, uint64_t n) {
+  return splitmix(seed ^ splitmix(a) ^ splitmix(b + 0x12345678ull)) % n;
+}
+// Forest and bucket hashes (splitmix chains of the same finaliser).
+__host__ __device__ inline uint64_t tree_hash(uint64_t tseed, uint64_t level, uint64_t a,
+                                              uint64_t salt) {
+  return splitmix(tseed ^ splitmix(level * GOLD + salt) ^ splitmix(a * GOLD + 0x51ED27ull));
+}
+__host__ __device__ inline uint32_t slot_hash(uint64_t seed, uint64_t iter, uint32_t q) {
+  return (uint32_t)(splitmix((seed ^ 0xA5A5A5A5DEADBEEFull) + (iter << 32) + q) >> 32);
+}
+
+// Row key: (dist_bits << 32) | (idx << 1) | is_new. dist >= 0, so the unsigned
+// order of keys is (dist, idx, flag); EMPTY_KEY sorts last.
+__device__ inline uint64_t make_key(float d, uint32_t idx, uint32_t is_new) {
+  return ((uint64_t)__float_as_uint(d) << 32) | ((uint64_t)idx << 1) | is_new;
+}
+__device__ inline uint32_t key_idx(uint64_t key) { return ((uint32_t)key) >> 1; }
+__device__ inline float key_dist(uint64_t key) { return __uint_as_float((uint32_t)(key >> 32)); }
+
+// ── wave primitives (64 lanes) ──
+__device__ inline uint64_t shfl_xor64(uint64_t v, int m) {
+  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+  lo = __shfl_xor(lo, m, WAVE);
+  hi = __shfl_xor(hi, m, WAVE);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ inline uint64_t shfl64(uint64_t v, int src) {
+  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+  lo = __shfl(lo, src, WAVE);
+  hi = __shfl(hi, src, WAVE);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ inline int lane_id() { return (int)(threadIdx.x & (WAVE - 1)); }
+
+// Bitonic sort of one u64 per lane, ascending by lane.
+__device__ inline uint64_t wave_sort64(uint64_t v) {
+  const int l = lane_id();
+#pragma unroll
+  for (int size = 2; size <= WAVE; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      uint64_t o = shfl_xor64(v, stride);
+      bool up = ((l & size) == 0);
+      bool lower = ((l & stride) == 0);
+      uint64_t mn = v < o ? v : o, mx = v < o ? o : v;
+      v = (up == lower) ? mn : mx;
+    }
+  }
+  return v;
+}
+// v holds an ascending-by-lane bitonic sequence's first half property: sort a
+// bitonic sequence of 64 ascending.
+__device__ inline uint64_t wave_bitonic_merge64(uint64_t v) {
+  const int l = lane_id();
+#pragma unroll
+  for (int stride = WAVE >> 1; stride > 0; stride >>= 1) {
+    uint64_t o = shfl_xor64(v, stride);
+    bool lower = ((l & stride) == 0);
+    uint64_t mn = v < o ? v : o, mx = v < o ? o : v;
+    v = lower ? mn : mx;
+  }
+  return v;
+}
+// best, chunk: ascending by lane. Returns the 64 smallest of their union, ascending.
+__device__ inline uint64_t wave_merge_keep64(uint64_t best, uint64_t chunk_sorted) {
+  uint64_t rev = shfl64(chunk_sorted, WAVE - 1 - lane_id());
+  uint64_t m = best < rev ? best : rev;
+  return wave_bitonic_merge64(m);
+}
+
+// Two u64 per lane: element e = r * 64 + lane. Ascending bitonic sort of 128.
+__device__ inline void wave_sort128(uint64_t& a, uint64_t& b) {
+  const int l = lane_id();
+#pragma unroll
+  for (int size = 2; size <= 2 * WAVE; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      if (stride == WAVE) {
+        // pairs (a at e=l, b at e=64+l); direction from bit `size` of e (size >= 128 -> up)
+        bool up = true;
+        uint64_t mn = a < b ? a : b, mx = a < b ? b : a;
+        a = up ? mn : mx;
+        b = up ? mx : mn;
+      } else {
+        uint64_t oa = shfl_xor64(a, stride), ob = shfl_xor64(b, stride);
+        bool lower = ((l & stride) == 0);
+        bool upa = ((l & size) == 0);
+        bool upb = (((WAVE + l) & size) == 0);
+        uint64_t mna = a < oa ? a : oa, mxa = a < oa ? oa : a;
+        uint64_t mnb = b < ob ? b : ob, mxb = b < ob ? ob : b;
+        a = (upa == lower) ? mna : mxa;
+        b = (upb == lower) ? mnb : mxb;
+      }
+    }
+  }
+}
+
+// ── distances ──
+// crates/polariseq-core/src/simd.rs:45-62 for a row `p` (any address space)
+// against row `q`, both of length d: 8 lane sums over chunks of 8, the fixed
+// reduce ((a0+a4)+(a1+a5))+((a2+a6)+(a3+a7)), then the sequential tail.
+template <typename PA, typename PB>
+__device__ inline float sq_dist_ref(PA p, PB q, int d) {
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  const int c = d >> 3;
+  for (int ch = 0; ch < c; ++ch) {
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+      float t = p[ch * 8 + l] - q[ch * 8 + l];
+      acc[l] = acc[l] + t * t;
+    }
+  }
+  float tail = 0.f;
+  for (int i = c * 8; i < d; ++i) {
+    float t = p[i] - q[i];
+    tail = tail + t * t;
+  }
+  return (((acc[0] + acc[4]) + (acc[1] + acc[5])) + ((acc[2] + acc[6]) + (acc[3] + acc[7]))) + tail;
+}
+// crates/polariseq-core/src/simd.rs:22-37.
+template <typename PA, typename PB>
+__device__ inline float dot_ref(PA p, PB q, int d) {
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  const int c = d >> 3;
+  for (int ch = 0; ch < c; ++ch) {
+#pragma unroll
+    for (int l = 0; l < 8; ++l) acc[l] = acc[l] + p[ch * 8 + l] * q[ch * 8 + l];
+  }
+  float tail = 0.f;
+  for (int i = c * 8; i < d; ++i) tail = tail + p[i] * q[i];
+  return (((acc[0] + acc[4]) + (acc[1] + acc[5])) + ((acc[2] + acc[6]) + (acc[3] + acc[7]))) + tail;
+}
+
+// NR u64 per lane, element e = r * 64 + lane. Ascending bitonic sort of 64*NR keys.
+template <int NR>
+__device__ inline void wave_sort_n(uint64_t (&v)[NR]) {
+  const int l = lane_id();
+#pragma unroll
+  for (int size = 2; size <= WAVE * NR; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      if (stride >= WAVE) {
+        const int rs = stride / WAVE;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          const int rp = r ^ rs;
+          if (r < rp) {
+            const bool up = (((r * WAVE + l) & size) == 0);
+            uint64_t x = v[r], y = v[rp];
+            uint64_t mn = x < y ? x : y, mx = x < y ? y : x;
+            v[r] = up ? mn : mx;
+            v[rp] = up ? mx : mn;
+          }
+        }
+      } else {
+        const bool lower = ((l & stride) == 0);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          uint64_t o = shfl_xor64(v[r], stride);
+          const bool up = (((r * WAVE + l) & size) == 0);
+          uint64_t mn = v[r] < o ? v[r] : o, mx = v[r] < o ? o : v[r];
+          v[r] = (up == lower) ? mn : mx;
+        }
+      }
+    }
+  }
+}
+
+// simd.rs:45-62 with the dimension fixed at compile time, so a register array
+// can hold one operand. Same operation order as sq_dist_ref.
+template <int D, typename PA, typename PB>
+__device__ inline float sq_dist_fixed(PA p, PB q) {
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int ch = 0; ch < D / 8; ++ch) {
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+      float t = p[ch * 8 + l] - q[ch * 8 + l];
+      acc[l] = acc[l] + t * t;
+    }
+  }
+  float tail = 0.f;
+#pragma unroll
+  for (int i = (D / 8) * 8; i < D; ++i) {
+    float t = p[i] - q[i];
+    tail = tail + t * t;
+  }
+  return (((acc[0] + acc[4]) + (acc[1] + acc[5])) + ((acc[2] + acc[6]) + (acc[3] + acc[7]))) + tail;
+}
+
+}  // namespace psq

@@ -1,0 +1,126 @@
+"""GPU parity of NN-Descent.
+
+* Bit-exact against the oracle's statement of the GPU algorithm
+  (oracle/gpu_model.cpp): same indptr, indices and distance bits.
+* The reference's own invariants (crates/polariseq-core/src/nndescent.rs:1059-1199):
+  k per row, no self, no duplicates, sorted rows, determinism, the in-place form
+  equal to the allocating form, tiny inputs.
+* Recall at least the reference restatement's (oracle/nndescent_port.cpp) on the
+  same input, against exact kNN.
+* Distances are the reference's f32 sq_dist, square-rooted, bit for bit.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import polariseq_amd as P
+from polariseq_amd import datasets as ds
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(g, m):
+    assert np.array_equal(g["indptr"], m["indptr"]), "indptr differs"
+    assert np.array_equal(g["indices"], m["indices"]), "indices differ"
+    assert np.array_equal(g["data"].view(np.uint32), m["data"].view(np.uint32)), "distances differ"
+
+
+@pytest.mark.parametrize("n,d,k,seed", [(5, 3, 10, 0), (100, 8, 10, 1), (3000, 50, 15, 2),
+                                        (20000, 50, 15, 0), (5000, 20, 30, 3), (2000, 50, 1, 4),
+                                        (4000, 50, 30, 5), (1000, 64, 7, 6)])
+def test_bit_exact_against_the_oracle_random_start(orc, n, d, k, seed):
+    x = ds.mixture(n, d, 8, seed=seed) if n > 20 else ds.clustered(n, d, 2, seed=0)
+    g = P.nndescent_dense(x, k, seed=seed, init="random")
+    m, _ = orc.gpu_model_nndescent(x, k, seed=seed, init=1)
+    _same(g, m)
+
+
+def test_options_reach_the_kernels(orc):
+    x = ds.mixture(6000, 50, 12, seed=9)
+    kw = dict(max_iterations=3, max_candidates=20, delta=0.0)
+    g = P.nndescent_dense(x, 12, seed=11, init="random", **kw)
+    m, _ = orc.gpu_model_nndescent(x, 12, seed=11, init=1, **kw)
+    _same(g, m)
+    assert P.last_stats()["iterations"] == 3
+
+
+def test_structure_determinism_and_in_place_form():
+    x = ds.clustered(2000, 8, 5, seed=1)
+    a = P.nndescent_dense(x, 10, seed=1, init="random")
+    b = P.nndescent_dense(x, 10, seed=1, init="random")
+    assert np.array_equal(a["indices"], b["indices"])
+    assert np.array_equal(a["data"].view(np.uint32), b["data"].view(np.uint32))
+    n, k = 2000, 10
+    ip, ix, dx = a["indptr"], a["indices"], a["data"]
+    for i in range(n):
+        row = ix[ip[i]:ip[i + 1]]
+        assert len(row) == k and i not in row and len(set(row.tolist())) == k
+        assert (np.diff(dx[ip[i]:ip[i + 1]]) >= 0).all()
+    # nndescent.rs:1059-1079: dirty caller buffers give the same graph
+    oip = np.full(n + 1, 7, np.uint32)
+    oix = np.full(n * k, 9, np.uint32)
+    odx = np.full(n * k, 1.0, np.float32)
+    r = P.nndescent_dense(x, k, 0, 1, out=(oip, oix, odx), init="random")
+    assert r["nnz"] == len(ix) and r["n_obs"] == n
+    assert np.array_equal(oip, ip) and np.array_equal(oix, ix)
+    assert np.array_equal(odx.view(np.uint32), dx.view(np.uint32))
+
+
+def test_tiny_inputs_work():
+    x = ds.clustered(5, 3, 2, seed=0)
+    g = P.nndescent_dense(x, 10, init="random")
+    assert (np.diff(g["indptr"]) == 4).all()
+
+
+def test_distances_are_the_reference_sq_dist(orc):
+    x = ds.mixture(3000, 50, 6, seed=3)
+    g = P.nndescent_dense(x, 15, init="random")
+    ip, ix, dx = g["indptr"], g["indices"], g["data"]
+    for i in range(0, 3000, 53):
+        for s in range(ip[i], ip[i + 1]):
+            want = np.float32(np.sqrt(orc.sq_dist(x[i], x[ix[s]])))
+            assert dx[s].view(np.uint32) == want.view(np.uint32)
+
+
+def test_recall_at_least_the_reference_restatement(orc):
+    # 20k x 50 cells, 20-component mixture, k = 15, seed 0.
+    x = ds.mixture(20000, 50, 20, seed=0)
+    q = np.random.default_rng(1).choice(20000, 4000, replace=False).astype(np.uint32)
+    ei, _ = orc.knn_exact64(x, 15, q)
+    gp, _ = orc.nndescent_port(x, 15, seed=0)
+    g = P.nndescent_dense(x, 15, seed=0, init="random")
+    rp = ds.recall(gp["indices"].reshape(-1, 15)[q], ei).mean()
+    rg = ds.recall(g["indices"].reshape(-1, 15)[q], ei).mean()
+    assert rg >= 0.95
+    assert rg >= rp - 0.002, (rg, rp)
+
+
+def test_device_entry_and_workspace_reuse(orc):
+    import torch
+    x = ds.mixture(5000, 50, 9, seed=8)
+    t = torch.from_numpy(x).cuda()
+    host = P.nndescent_dense(x, 15, seed=2, init="random")
+    for _ in range(2):  # the cached working set is reused, results unchanged
+        ip, ix, dx, nnz = P.nndescent_device(t, 15, seed=2, init="random")
+        assert nnz == len(host["indices"])
+        assert np.array_equal(ix[:nnz].cpu().numpy().view(np.uint32), host["indices"])
+        assert np.array_equal(dx[:nnz].cpu().numpy().view(np.uint32), host["data"].view(np.uint32))
+    # a smaller problem after a larger one reuses the larger buffers
+    small = P.nndescent_dense(x[:700], 10, seed=2, init="random")
+    m, _ = orc.gpu_model_nndescent(x[:700], 10, seed=2, init=1)
+    _same(small, m)
+
+
+def test_device_budget_surfaces_as_an_error():
+    from polariseq_amd import _native as N
+    P.release_workspace()
+    N.lib().psq_set_device_budget(1 << 20)
+    try:
+        with pytest.raises(N.PsqError, match="budget"):
+            P.nndescent_dense(ds.mixture(20000, 50, 5, seed=0), 15, init="random")
+    finally:
+        N.lib().psq_set_device_budget(0)
+    g = P.nndescent_dense(ds.mixture(500, 50, 5, seed=0), 15, init="random")
+    assert len(g["indices"]) == 500 * 15
