@@ -67,6 +67,12 @@ static inline uint32_t slot_hash(uint64_t seed, uint64_t iter, uint32_t q) {
   return (uint32_t)(h >> 32);
 }
 
+static inline uint32_t leaf_group(size_t leaf_size) {
+  uint32_t g = 16;
+  while (g < leaf_size && g < 64) g *= 2;
+  return g;
+}
+
 struct Params {
   size_t n, d, k, m, n_iters, n_trees, leaf_size, slots, stop_at;
   uint64_t seed;
@@ -226,17 +232,24 @@ int64_t psq_o_gpu_model_nndescent(const float* data, uint64_t n_obs, uint64_t n_
     pool.parallel_for(P.n_trees, 1, [&](size_t t) {
       build_tree(data, P, splitmix(seed ^ (uint64_t)(t + 1)), ids[t], leaves[t]);
     });
+    // Leaf pairs, tree after tree. A leaf larger than the group size LG
+    // (smallest power of two >= leaf_size, within [16, 64]) contributes pairs
+    // inside consecutive chunks of LG members only.
+    const uint32_t LG = leaf_group(P.leaf_size);
     for (size_t t = 0; t < P.n_trees; ++t) {
       pool.parallel_for(leaves[t].size(), 64, [&](size_t li) {
-        auto [start, c] = leaves[t][li];
-        const uint32_t* mem = ids[t].data() + start;
+        auto [start, cnt] = leaves[t][li];
         std::vector<uint64_t> props;
-        for (uint32_t a = 0; a < c; ++a) {
-          props.clear();
-          uint32_t p = mem[a];
-          for (uint32_t b = 0; b < c; ++b)
-            if (b != a) props.push_back(make_key(sq_dist(row(p), row(mem[b]), d), mem[b], 1));
-          merge_row(&G[(size_t)p * k], k, props);
+        for (uint32_t c0 = 0; c0 < cnt; c0 += LG) {
+          const uint32_t c = std::min(LG, cnt - c0);
+          const uint32_t* mem = ids[t].data() + start + c0;
+          for (uint32_t a = 0; a < c; ++a) {
+            props.clear();
+            uint32_t p = mem[a];
+            for (uint32_t b = 0; b < c; ++b)
+              if (b != a) props.push_back(make_key(sq_dist(row(p), row(mem[b]), d), mem[b], 1));
+            merge_row(&G[(size_t)p * k], k, props);
+          }
         }
       });
     }

@@ -272,6 +272,169 @@ struct EventPool {
   }
 };
 
+uint32_t leaf_group(uint64_t leaf_size) {
+  uint32_t g = 16;
+  while (g < leaf_size && g < 64) g *= 2;
+  return g;
+}
+
+template <int D>
+int launch_leaf_merge(uint32_t KM, uint32_t LG, const float* Xp, uint32_t d, uint32_t dp, uint32_t n,
+                      const uint32_t* ids_t, const uint32_t* gs, const uint32_t* gc, uint32_t nl,
+                      uint64_t* G, uint32_t k, hipStream_t s) {
+  const unsigned bt = dp <= 128 ? 256 : 64;
+  if (bt < LG) return fail(PSQ_E_UNSUPPORTED, "leaf group larger than the block");
+  const size_t smem = (size_t)bt * dp * sizeof(float);
+  const unsigned grid = grid_for(nl, bt / LG);
+  void (*kern)(const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, const uint32_t*,
+               const uint32_t*, uint32_t, uint64_t*, uint32_t) = nullptr;
+  if (KM == 16 && LG == 16) kern = k_leaf_merge<D, 16, 16>;
+  else if (KM == 32 && LG == 16) kern = k_leaf_merge<D, 32, 16>;
+  else if (KM == 32 && LG == 32) kern = k_leaf_merge<D, 32, 32>;
+  else if (KM == 64 && LG == 16) kern = k_leaf_merge<D, 64, 16>;
+  else if (KM == 64 && LG == 32) kern = k_leaf_merge<D, 64, 32>;
+  else kern = k_leaf_merge<D, 64, 64>;
+  if (smem > 65536) HIPC(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(bt), smem, s, Xp, d, dp, n, ids_t, gs, gc, nl, G, k);
+  HIPC(hipGetLastError());
+  return PSQ_OK;
+}
+
+// RP forest start: trees built level by level, a batch of trees at a time (so
+// every position index fits in 32 bits), then the leaf pairs of each tree are
+// merged into the rows, tree after tree (nndescent.rs:767-817).
+int build_forest(Workspace* ws, const Resolved& R, const float* Xp, uint64_t* G, hipStream_t s,
+                 uint32_t* hbuf) {
+  const uint32_t n = (uint32_t)R.n, d = (uint32_t)R.d, dp = R.dp, k = (uint32_t)R.k;
+  const uint32_t ls = (uint32_t)R.leaf_size;
+  const uint64_t T = R.n_trees;
+  const uint64_t TB = std::max<uint64_t>(1, std::min<uint64_t>(T, ((1ull << 31) - 2) / n));
+  const uint32_t LG = leaf_group(ls);
+  uint32_t KM = k <= 16 ? 16 : (k <= 32 ? 32 : 64);
+  if (LG > KM) KM = LG;
+  std::vector<uint64_t> hseeds(T);
+  for (uint64_t t = 0; t < T; ++t) hseeds[t] = splitmix(R.seed ^ (t + 1));
+  uint64_t* tseeds;
+  TRY(wbuf(ws, "f_tseeds", T, &tseeds));
+  HIPC(hipMemcpyAsync(tseeds, hseeds.data(), T * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+  const size_t total_cap = (size_t)TB * n;
+  const size_t node_cap = (size_t)TB * (n / (ls + 1) + 1);
+  uint32_t *ids[2], *segv[2], *fl, *fr, *sl, *sr;
+  TRY(wbuf(ws, "f_ids0", total_cap, &ids[0]));
+  TRY(wbuf(ws, "f_ids1", total_cap, &ids[1]));
+  TRY(wbuf(ws, "f_seg0", total_cap, &segv[0]));
+  TRY(wbuf(ws, "f_seg1", total_cap, &segv[1]));
+  TRY(wbuf(ws, "f_fl", total_cap, &fl));
+  TRY(wbuf(ws, "f_fr", total_cap, &fr));
+  TRY(wbuf(ws, "f_sl", total_cap + 1, &sl));
+  TRY(wbuf(ws, "f_sr", total_cap + 1, &sr));
+  uint32_t *ns[2], *nc[2], *nt[2], *nlv, *mode, *nact, *nleaf, *aoff, *loff, *child;
+  float *normals, *offsets;
+  TRY(wbuf(ws, "f_ns0", node_cap, &ns[0]));
+  TRY(wbuf(ws, "f_ns1", node_cap, &ns[1]));
+  TRY(wbuf(ws, "f_nc0", node_cap, &nc[0]));
+  TRY(wbuf(ws, "f_nc1", node_cap, &nc[1]));
+  TRY(wbuf(ws, "f_nt0", node_cap, &nt[0]));
+  TRY(wbuf(ws, "f_nt1", node_cap, &nt[1]));
+  TRY(wbuf(ws, "f_nl", node_cap, &nlv));
+  TRY(wbuf(ws, "f_mode", node_cap, &mode));
+  TRY(wbuf(ws, "f_nact", node_cap, &nact));
+  TRY(wbuf(ws, "f_nleaf", node_cap, &nleaf));
+  TRY(wbuf(ws, "f_aoff", node_cap + 1, &aoff));
+  TRY(wbuf(ws, "f_loff", node_cap + 1, &loff));
+  TRY(wbuf(ws, "f_child", 2 * node_cap, &child));
+  TRY(wbuf(ws, "f_normals", node_cap * dp, &normals));
+  TRY(wbuf(ws, "f_offsets", node_cap, &offsets));
+  // leaves: each leaf holds at least one point, so a batch has at most TB * n
+  uint32_t *lt, *lsrt, *lcnt, *gs, *gc, *tcount;
+  TRY(wbuf(ws, "f_lt", total_cap, &lt));
+  TRY(wbuf(ws, "f_ls", total_cap, &lsrt));
+  TRY(wbuf(ws, "f_lc", total_cap, &lcnt));
+  TRY(wbuf(ws, "f_gs", total_cap, &gs));
+  TRY(wbuf(ws, "f_gc", total_cap, &gc));
+  TRY(wbuf(ws, "f_tcount", TB, &tcount));
+
+  for (uint64_t t0 = 0; t0 < T; t0 += TB) {
+    const uint32_t tb = (uint32_t)std::min<uint64_t>(TB, T - t0);
+    const size_t total = (size_t)tb * n;
+    hipLaunchKernelGGL(k_forest_init, dim3(grid_for(total, 256)), dim3(256), 0, s, ids[0], segv[0],
+                       n, tb, ls);
+    HIPC(hipGetLastError());
+    std::vector<uint32_t> h0s(tb, 0), h0c(tb, n), h0t(tb);
+    for (uint32_t t = 0; t < tb; ++t) h0t[t] = t;
+    uint32_t A = 0, leaf_base = 0;
+    if (n > ls) {
+      A = tb;
+      HIPC(hipMemcpyAsync(ns[0], h0s.data(), tb * 4, hipMemcpyHostToDevice, s));
+      HIPC(hipMemcpyAsync(nc[0], h0c.data(), tb * 4, hipMemcpyHostToDevice, s));
+      HIPC(hipMemcpyAsync(nt[0], h0t.data(), tb * 4, hipMemcpyHostToDevice, s));
+    } else {
+      HIPC(hipMemcpyAsync(lt, h0t.data(), tb * 4, hipMemcpyHostToDevice, s));
+      HIPC(hipMemcpyAsync(lsrt, h0s.data(), tb * 4, hipMemcpyHostToDevice, s));
+      HIPC(hipMemcpyAsync(lcnt, h0c.data(), tb * 4, hipMemcpyHostToDevice, s));
+      leaf_base = tb;
+    }
+    int cur = 0;
+    for (uint32_t L = 0; A > 0; ++L) {
+      const int nx = cur ^ 1;
+      hipLaunchKernelGGL(k_forest_planes, dim3(grid_for(A, 4)), dim3(256), 0, s, Xp, d, dp, n,
+                         ids[cur], ns[cur], nc[cur], nt[cur], A, tseeds + t0, L, normals, offsets);
+      HIPC(hipGetLastError());
+      hipLaunchKernelGGL(k_forest_sides, dim3(grid_for(total, 256)), dim3(256), 0, s, Xp, d, dp, n,
+                         total, ids[cur], segv[cur], nt[cur], tseeds + t0, L, normals, offsets, fl, fr);
+      HIPC(hipGetLastError());
+      TRY(scan_u32(ws, fl, sl, (uint32_t)total, s));
+      TRY(scan_u32(ws, fr, sr, (uint32_t)total, s));
+      hipLaunchKernelGGL(k_forest_nodes, dim3(grid_for(A, 256)), dim3(256), 0, s, ns[cur], nc[cur],
+                         nt[cur], A, n, sl, sr, ls, L, nlv, mode, nact, nleaf);
+      HIPC(hipGetLastError());
+      TRY(scan_u32(ws, nact, aoff, A, s));
+      TRY(scan_u32(ws, nleaf, loff, A, s));
+      HIPC(hipMemcpyAsync(hbuf, aoff + A, 4, hipMemcpyDeviceToHost, s));
+      HIPC(hipMemcpyAsync(hbuf + 1, loff + A, 4, hipMemcpyDeviceToHost, s));
+      HIPC(hipStreamSynchronize(s));
+      const uint32_t nextA = hbuf[0], newleaves = hbuf[1];
+      if ((size_t)nextA > node_cap || (size_t)leaf_base + newleaves > total_cap)
+        return fail(PSQ_E_DEVICE, "forest bookkeeping overflow (nodes %u, leaves %u)", nextA,
+                    leaf_base + newleaves);
+      hipLaunchKernelGGL(k_forest_emit, dim3(grid_for(A, 256)), dim3(256), 0, s, ns[cur], nc[cur],
+                         nt[cur], A, nlv, mode, aoff, loff, ls, L, ns[nx], nc[nx], nt[nx], child, lt,
+                         lsrt, lcnt, leaf_base);
+      HIPC(hipGetLastError());
+      hipLaunchKernelGGL(k_forest_scatter, dim3(grid_for(total, 256)), dim3(256), 0, s, total, n,
+                         ids[cur], segv[cur], ns[cur], nt[cur], nlv, mode, child, sl, sr, ids[nx],
+                         segv[nx]);
+      HIPC(hipGetLastError());
+      cur = nx;
+      A = nextA;
+      leaf_base += newleaves;
+    }
+    // group leaves by tree
+    TRY(fill_u32(tcount, tb, 0, s));
+    hipLaunchKernelGGL(k_leaf_hist, dim3(grid_for(leaf_base, 256)), dim3(256), 0, s, lt, leaf_base,
+                       tcount);
+    HIPC(hipGetLastError());
+    std::vector<uint32_t> hc(tb), hoff(tb + 1, 0);
+    HIPC(hipMemcpyAsync(hc.data(), tcount, tb * 4, hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    for (uint32_t t = 0; t < tb; ++t) hoff[t + 1] = hoff[t] + hc[t];
+    HIPC(hipMemcpyAsync(tcount, hoff.data(), tb * 4, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_leaf_group, dim3(grid_for(leaf_base, 256)), dim3(256), 0, s, lt, lsrt,
+                       lcnt, leaf_base, tcount, gs, gc);
+    HIPC(hipGetLastError());
+    // leaf pairs, tree after tree
+    for (uint32_t t = 0; t < tb; ++t) {
+      if (hc[t] == 0) continue;
+      const uint32_t* ids_t = ids[cur] + (size_t)t * n;
+      if (d == 50)
+        TRY(launch_leaf_merge<50>(KM, LG, Xp, d, dp, n, ids_t, gs + hoff[t], gc + hoff[t], hc[t], G, k, s));
+      else
+        TRY(launch_leaf_merge<0>(KM, LG, Xp, d, dp, n, ids_t, gs + hoff[t], gc + hoff[t], hc[t], G, k, s));
+    }
+  }
+  return PSQ_OK;
+}
+
 int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
                           const psq_nndescent_opts* opts, uint32_t* d_indptr, uint64_t indptr_len,
                           uint32_t* d_indices, float* d_dists, uint64_t cap, uint64_t* nnz_out,
@@ -288,8 +451,6 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
                 (unsigned long long)(n_obs + 1), (unsigned long long)need);
   if (!d_data || !d_indptr || !d_indices || !d_dists || !nnz_out)
     return fail(PSQ_E_INVALID, "NULL pointer argument");
-  if (opts->init == PSQ_INIT_RP_FOREST && R.n_trees > 0 && n_obs > R.leaf_size)
-    return fail(PSQ_E_UNSUPPORTED, "the RP-forest start is not built yet; use init=PSQ_INIT_RANDOM");
 
   Workspace* ws;
   TRY(workspace(&ws));
@@ -340,7 +501,9 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
     TRY(fill_u32(reinterpret_cast<uint32_t*>(thr), n, bits, s));
   }
 
-  // ── start: random fill, then one forced merge sorts every row ──
+  // ── start: RP forest (or nothing), random fill, then one forced merge sorts every row ──
+  if (opts->init == PSQ_INIT_RP_FOREST && R.n_trees > 0)
+    TRY(build_forest(ws, R, Xp, G, s, static_cast<uint32_t*>(hp) + 64));
   hipLaunchKernelGGL(k_random_fill, dim3(grid_for(n, 256)), dim3(256), 0, s, Xp, n,
                      (uint32_t)R.d, R.dp, G, k, R.seed, touched);
   HIPC(hipGetLastError());

@@ -1,0 +1,324 @@
+// Random-projection forest start for NN-Descent (gfx950).
+//
+// Reference: crates/polariseq-core/src/nndescent.rs:356-641 (trees) and 767-817
+// (all pairs inside each leaf, applied tree after tree). The GPU builds every
+// tree at once, level by level. Per tree, `ids` is a permutation of the points
+// and every node is a contiguous segment of it. At level L, a node (start, c)
+// with c > leaf_size:
+//   * takes the hyperplane through members a = seg[h(L, start, 0) % c] and
+//     b = seg[h(L, start, r) % c] (r = 1.., up to 8 redraws while b == a):
+//     normal = x_a - x_b, offset = -Σ normal_t * 0.5 * (x_a,t + x_b,t), a
+//     sequential f32 fold (nndescent.rs:367-385);
+//   * sends a point left when dot(normal, x) + offset > 0 (the 8-lane dot of
+//     simd.rs:22-37); |margin| < 1e-8 draws a hash bit (nndescent.rs:439-444);
+//   * if every point lands on one side, hash bits split it instead, and if
+//     that fails too the node becomes a leaf;
+//   * is partitioned stably: left members first, in their order.
+// Children with at most leaf_size points (or past depth 100) become leaves.
+// Then, tree by tree, each point merges the distances to its leaf-mates into
+// its row (top-k, deduplicated). oracle/gpu_model.cpp states the same steps.
+#include "psq_kernels.hpp"
+
+namespace psq {
+
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+
+__global__ void k_forest_init(uint32_t* __restrict__ ids, uint32_t* __restrict__ segv, uint32_t n,
+                              uint32_t T, uint32_t leaf_size) {
+  size_t pos = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos >= (size_t)n * T) return;
+  uint32_t t = (uint32_t)(pos / n), p = (uint32_t)(pos - (size_t)t * n);
+  ids[pos] = p;
+  segv[pos] = n > leaf_size ? t : NONE;
+}
+
+// One wave per active node: pivots, normal and offset.
+__global__ __launch_bounds__(256) void k_forest_planes(
+    const float* __restrict__ Xp, uint32_t d, uint32_t dp, uint32_t n,
+    const uint32_t* __restrict__ ids, const uint32_t* __restrict__ nstart,
+    const uint32_t* __restrict__ ncount, const uint32_t* __restrict__ ntree, uint32_t A,
+    const uint64_t* __restrict__ tseeds, uint32_t level, float* __restrict__ normals,
+    float* __restrict__ offsets) {
+  const int w = threadIdx.x / WAVE, l = lane_id();
+  const uint32_t v = blockIdx.x * 4 + w;
+  if (v >= A) return;
+  const uint32_t start = nstart[v], c = ncount[v], t = ntree[v];
+  const uint64_t ts = tseeds[t];
+  const uint32_t* seg = ids + (size_t)t * n + start;
+  const uint32_t a = seg[tree_hash(ts, level, start, 0) % c];
+  uint32_t b = seg[tree_hash(ts, level, start, 1) % c];
+  for (uint64_t r = 2; b == a && r < 10; ++r) b = seg[tree_hash(ts, level, start, r) % c];
+  const float* xa = Xp + (size_t)a * dp;
+  const float* xb = Xp + (size_t)b * dp;
+  float* nv = normals + (size_t)v * dp;
+  for (uint32_t j = l; j < dp; j += WAVE) nv[j] = j < d ? xa[j] - xb[j] : 0.0f;
+  if (l == 0) {
+    float s = -0.0f;
+    for (uint32_t j = 0; j < d; ++j) {
+      float nj = xa[j] - xb[j];
+      s = s + nj * 0.5f * (xa[j] + xb[j]);
+    }
+    offsets[v] = -s;
+  }
+}
+
+// One thread per position of every tree: side bits for the hyperplane and for
+// the hash fallback.
+__global__ void k_forest_sides(const float* __restrict__ Xp, uint32_t d, uint32_t dp, uint32_t n,
+                               size_t total, const uint32_t* __restrict__ ids,
+                               const uint32_t* __restrict__ segv, const uint32_t* __restrict__ ntree,
+                               const uint64_t* __restrict__ tseeds, uint32_t level,
+                               const float* __restrict__ normals, const float* __restrict__ offsets,
+                               uint32_t* __restrict__ fl, uint32_t* __restrict__ fr) {
+  size_t pos = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos >= total) return;
+  const uint32_t v = segv[pos];
+  if (v == NONE) {
+    fl[pos] = 0;
+    fr[pos] = 0;
+    return;
+  }
+  const uint32_t i = ids[pos];
+  const uint64_t ts = tseeds[ntree[v]];
+  const float margin = dot_ref(normals + (size_t)v * dp, Xp + (size_t)i * dp, (int)d) + offsets[v];
+  bool left;
+  if (fabsf(margin) < 1e-8f)
+    left = (tree_hash(ts, level, i, 2) & 1) == 0;
+  else
+    left = margin > 0.0f;
+  fl[pos] = left ? 1u : 0u;
+  fr[pos] = (tree_hash(ts, level, i, 3) & 1) == 0 ? 1u : 0u;
+}
+
+// One thread per active node: split mode and how many children it emits.
+// mode 0 = hyperplane, 1 = hash bits, 2 = no split (the node is a leaf).
+__global__ void k_forest_nodes(const uint32_t* __restrict__ nstart, const uint32_t* __restrict__ ncount,
+                               const uint32_t* __restrict__ ntree, uint32_t A, uint32_t n,
+                               const uint32_t* __restrict__ sl, const uint32_t* __restrict__ sr,
+                               uint32_t leaf_size, uint32_t level, uint32_t* __restrict__ nl_out,
+                               uint32_t* __restrict__ mode_out, uint32_t* __restrict__ nact,
+                               uint32_t* __restrict__ nleaf) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= A) return;
+  const size_t g0 = (size_t)ntree[v] * n + nstart[v];
+  const uint32_t c = ncount[v];
+  const uint32_t cl = sl[g0 + c] - sl[g0];
+  const uint32_t rl = sr[g0 + c] - sr[g0];
+  uint32_t mode, nl;
+  if (cl != 0 && cl != c) {
+    mode = 0;
+    nl = cl;
+  } else if (rl != 0 && rl != c) {
+    mode = 1;
+    nl = rl;
+  } else {
+    mode = 2;
+    nl = c;
+  }
+  mode_out[v] = mode;
+  nl_out[v] = nl;
+  if (mode == 2) {
+    nact[v] = 0;
+    nleaf[v] = 1;
+    return;
+  }
+  const bool child_deep = level + 1 > 100;
+  const bool lleaf = child_deep || nl <= leaf_size;
+  const bool rleaf = child_deep || (c - nl) <= leaf_size;
+  nact[v] = (lleaf ? 0u : 1u) + (rleaf ? 0u : 1u);
+  nleaf[v] = (lleaf ? 1u : 0u) + (rleaf ? 1u : 0u);
+}
+
+// One thread per active node: next-level nodes, leaves and the child map.
+__global__ void k_forest_emit(const uint32_t* __restrict__ nstart, const uint32_t* __restrict__ ncount,
+                              const uint32_t* __restrict__ ntree, uint32_t A,
+                              const uint32_t* __restrict__ nl_in, const uint32_t* __restrict__ mode_in,
+                              const uint32_t* __restrict__ act_off, const uint32_t* __restrict__ leaf_off,
+                              uint32_t leaf_size, uint32_t level, uint32_t* __restrict__ x_start,
+                              uint32_t* __restrict__ x_count, uint32_t* __restrict__ x_tree,
+                              uint32_t* __restrict__ child, uint32_t* __restrict__ l_tree,
+                              uint32_t* __restrict__ l_start, uint32_t* __restrict__ l_count,
+                              uint32_t leaf_base) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= A) return;
+  const uint32_t start = nstart[v], c = ncount[v], t = ntree[v];
+  const uint32_t mode = mode_in[v], nl = nl_in[v];
+  uint32_t ao = act_off[v], lo = leaf_base + leaf_off[v];
+  if (mode == 2) {
+    l_tree[lo] = t;
+    l_start[lo] = start;
+    l_count[lo] = c;
+    child[2 * v] = NONE;
+    child[2 * v + 1] = NONE;
+    return;
+  }
+  const bool child_deep = level + 1 > 100;
+  const uint32_t cs[2] = {start, start + nl};
+  const uint32_t cc[2] = {nl, c - nl};
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    if (child_deep || cc[s] <= leaf_size) {
+      l_tree[lo] = t;
+      l_start[lo] = cs[s];
+      l_count[lo] = cc[s];
+      ++lo;
+      child[2 * v + s] = NONE;
+    } else {
+      x_start[ao] = cs[s];
+      x_count[ao] = cc[s];
+      x_tree[ao] = t;
+      child[2 * v + s] = ao;
+      ++ao;
+    }
+  }
+}
+
+// One thread per position: stable partition into the ping-pong buffers.
+__global__ void k_forest_scatter(size_t total, uint32_t n, const uint32_t* __restrict__ ids_in,
+                                 const uint32_t* __restrict__ segv_in,
+                                 const uint32_t* __restrict__ nstart, const uint32_t* __restrict__ ntree,
+                                 const uint32_t* __restrict__ nl_in, const uint32_t* __restrict__ mode_in,
+                                 const uint32_t* __restrict__ child, const uint32_t* __restrict__ sl,
+                                 const uint32_t* __restrict__ sr, uint32_t* __restrict__ ids_out,
+                                 uint32_t* __restrict__ segv_out) {
+  size_t pos = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos >= total) return;
+  const uint32_t v = segv_in[pos];
+  const uint32_t id = ids_in[pos];
+  if (v == NONE) {
+    ids_out[pos] = id;
+    segv_out[pos] = NONE;
+    return;
+  }
+  const uint32_t mode = mode_in[v];
+  if (mode == 2) {
+    ids_out[pos] = id;
+    segv_out[pos] = NONE;
+    return;
+  }
+  const uint32_t* X = mode == 0 ? sl : sr;
+  const size_t g0 = (size_t)ntree[v] * n + nstart[v];
+  const uint32_t rank_left = X[pos] - X[g0];
+  const bool left = (X[pos + 1] - X[pos]) != 0;
+  size_t np;
+  uint32_t ch;
+  if (left) {
+    np = g0 + rank_left;
+    ch = child[2 * v];
+  } else {
+    np = g0 + nl_in[v] + ((uint32_t)(pos - g0) - rank_left);
+    ch = child[2 * v + 1];
+  }
+  ids_out[np] = id;
+  segv_out[np] = ch;
+}
+
+// Leaves grouped by tree (order inside a tree is irrelevant: its leaves are
+// disjoint and each merge is a set function).
+__global__ void k_leaf_hist(const uint32_t* __restrict__ l_tree, uint32_t nl,
+                            uint32_t* __restrict__ tcount) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nl) atomicAdd(&tcount[l_tree[i]], 1u);
+}
+__global__ void k_leaf_group(const uint32_t* __restrict__ l_tree, const uint32_t* __restrict__ l_start,
+                             const uint32_t* __restrict__ l_count, uint32_t nl,
+                             uint32_t* __restrict__ tcursor, uint32_t* __restrict__ g_start,
+                             uint32_t* __restrict__ g_count) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nl) return;
+  uint32_t pos = atomicAdd(&tcursor[l_tree[i]], 1u);
+  g_start[pos] = l_start[i];
+  g_count[pos] = l_count[i];
+}
+
+// Leaf pairs of one tree. A group of LG lanes takes one leaf; a leaf of more
+// than LG members is taken in consecutive chunks of LG (pairs within a chunk).
+// Each lane holds its point's row (KMAX keys, sorted) in registers and inserts
+// the keys of its chunk-mates: a key enters when it beats the row's worst and
+// its (dist, idx) is not already present.
+template <int D, int KMAX, int LG>
+__global__ __launch_bounds__(256) void k_leaf_merge(
+    const float* __restrict__ Xp, uint32_t d, uint32_t dp, uint32_t n,
+    const uint32_t* __restrict__ ids_t, const uint32_t* __restrict__ g_start,
+    const uint32_t* __restrict__ g_count, uint32_t nleaves, uint64_t* __restrict__ G, uint32_t k) {
+  extern __shared__ float s_rows[];  // [blockDim.x / LG groups][LG][dp]
+  __shared__ uint32_t s_rounds;
+  const uint32_t GPB = blockDim.x / LG;
+  const int g = threadIdx.x / LG, j = threadIdx.x % LG;
+  const uint32_t li = blockIdx.x * GPB + g;
+  const bool lvalid = li < nleaves;
+  const uint32_t start = lvalid ? g_start[li] : 0u, cnt = lvalid ? g_count[li] : 0u;
+  const uint32_t nch = (cnt + LG - 1) / LG;
+  if (threadIdx.x == 0) s_rounds = 0;
+  __syncthreads();
+  if (j == 0 && lvalid) atomicMax(&s_rounds, nch);
+  __syncthreads();
+  const uint32_t rounds = s_rounds;
+  float* grows = s_rows + (size_t)g * LG * dp;
+  for (uint32_t r = 0; r < rounds; ++r) {
+    const uint32_t c0 = r * LG;
+    const uint32_t cc = (r < nch) ? min((uint32_t)LG, cnt - c0) : 0u;
+    const bool valid = (uint32_t)j < cc;
+    const uint32_t p = valid ? ids_t[start + c0 + j] : 0u;
+    if (valid) {
+      const float4* src = reinterpret_cast<const float4*>(Xp + (size_t)p * dp);
+      float4* dst = reinterpret_cast<float4*>(grows + (size_t)j * dp);
+      for (uint32_t e = 0; e < dp / 4; ++e) dst[e] = src[e];
+    }
+    __syncthreads();
+    if (valid) {
+      uint64_t row[KMAX];
+      uint64_t* gr = G + (size_t)p * k;
+#pragma unroll
+      for (int s = 0; s < KMAX; ++s) row[s] = (uint32_t)s < k ? gr[s] : EMPTY_KEY;
+      const float* me = grows + (size_t)j * dp;
+      for (uint32_t q = 0; q < cc; ++q) {
+        if (q == (uint32_t)j) continue;
+        const float* other = grows + (size_t)q * dp;
+        const float dist = D > 0 ? sq_dist_fixed<(D > 0 ? D : 1)>(me, other) : sq_dist_ref(me, other, (int)d);
+        const uint32_t qi = ids_t[start + c0 + q];
+        const uint64_t key = make_key(dist, qi, 1);
+        uint64_t worst = EMPTY_KEY;
+#pragma unroll
+        for (int s = 0; s < KMAX; ++s)
+          if ((uint32_t)s == k - 1) worst = row[s];
+        if (key >= worst) continue;
+        bool dup = false;
+#pragma unroll
+        for (int s = 0; s < KMAX; ++s) dup |= ((uint32_t)s < k) && ((row[s] >> 1) == (key >> 1));
+        if (dup) continue;
+#pragma unroll
+        for (int s = KMAX - 1; s > 0; --s) {
+          if ((uint32_t)s < k) {
+            const uint64_t a = row[s - 1];
+            row[s] = key < a ? a : (key < row[s] ? key : row[s]);
+          }
+        }
+        row[0] = key < row[0] ? key : row[0];
+      }
+#pragma unroll
+      for (int s = 0; s < KMAX; ++s)
+        if ((uint32_t)s < k) gr[s] = row[s];
+    }
+    __syncthreads();
+  }
+}
+
+#define PSQ_LEAF_INST(D, KM, LG)                                                              \
+  template __global__ void k_leaf_merge<D, KM, LG>(const float*, uint32_t, uint32_t, uint32_t, \
+                                                   const uint32_t*, const uint32_t*,          \
+                                                   const uint32_t*, uint32_t, uint64_t*, uint32_t);
+PSQ_LEAF_INST(0, 16, 16)
+PSQ_LEAF_INST(0, 32, 16)
+PSQ_LEAF_INST(0, 32, 32)
+PSQ_LEAF_INST(0, 64, 16)
+PSQ_LEAF_INST(0, 64, 32)
+PSQ_LEAF_INST(0, 64, 64)
+PSQ_LEAF_INST(50, 16, 16)
+PSQ_LEAF_INST(50, 32, 16)
+PSQ_LEAF_INST(50, 32, 32)
+PSQ_LEAF_INST(50, 64, 16)
+PSQ_LEAF_INST(50, 64, 32)
+PSQ_LEAF_INST(50, 64, 64)
+
+}  // namespace psq

@@ -34,6 +34,38 @@ __global__ void k_row_counts(const uint64_t* G, uint32_t n, uint32_t k, uint32_t
 __global__ void k_extract(const uint64_t* G, uint32_t n, uint32_t k, const uint32_t* off,
                           uint32_t* indices, float* dists);
 
+// RP forest (forest_kernels.hip)
+__global__ void k_forest_init(uint32_t* ids, uint32_t* segv, uint32_t n, uint32_t T, uint32_t leaf_size);
+__global__ void k_forest_planes(const float* Xp, uint32_t d, uint32_t dp, uint32_t n, const uint32_t* ids,
+                                const uint32_t* nstart, const uint32_t* ncount, const uint32_t* ntree,
+                                uint32_t A, const uint64_t* tseeds, uint32_t level, float* normals,
+                                float* offsets);
+__global__ void k_forest_sides(const float* Xp, uint32_t d, uint32_t dp, uint32_t n, size_t total,
+                               const uint32_t* ids, const uint32_t* segv, const uint32_t* ntree,
+                               const uint64_t* tseeds, uint32_t level, const float* normals,
+                               const float* offsets, uint32_t* fl, uint32_t* fr);
+__global__ void k_forest_nodes(const uint32_t* nstart, const uint32_t* ncount, const uint32_t* ntree,
+                               uint32_t A, uint32_t n, const uint32_t* sl, const uint32_t* sr,
+                               uint32_t leaf_size, uint32_t level, uint32_t* nl_out, uint32_t* mode_out,
+                               uint32_t* nact, uint32_t* nleaf);
+__global__ void k_forest_emit(const uint32_t* nstart, const uint32_t* ncount, const uint32_t* ntree,
+                              uint32_t A, const uint32_t* nl_in, const uint32_t* mode_in,
+                              const uint32_t* act_off, const uint32_t* leaf_off, uint32_t leaf_size,
+                              uint32_t level, uint32_t* x_start, uint32_t* x_count, uint32_t* x_tree,
+                              uint32_t* child, uint32_t* l_tree, uint32_t* l_start, uint32_t* l_count,
+                              uint32_t leaf_base);
+__global__ void k_forest_scatter(size_t total, uint32_t n, const uint32_t* ids_in, const uint32_t* segv_in,
+                                 const uint32_t* nstart, const uint32_t* ntree, const uint32_t* nl_in,
+                                 const uint32_t* mode_in, const uint32_t* child, const uint32_t* sl,
+                                 const uint32_t* sr, uint32_t* ids_out, uint32_t* segv_out);
+__global__ void k_leaf_hist(const uint32_t* l_tree, uint32_t nl, uint32_t* tcount);
+__global__ void k_leaf_group(const uint32_t* l_tree, const uint32_t* l_start, const uint32_t* l_count,
+                             uint32_t nl, uint32_t* tcursor, uint32_t* g_start, uint32_t* g_count);
+template <int D, int KMAX, int LG>
+__global__ void k_leaf_merge(const float* Xp, uint32_t d, uint32_t dp, uint32_t n, const uint32_t* ids_t,
+                             const uint32_t* g_start, const uint32_t* g_count, uint32_t nleaves,
+                             uint64_t* G, uint32_t k);
+
 // exact kNN (exact_kernels.hip)
 template <int D, int K>
 __global__ void k_exact_pass1(const float* Xp, uint32_t n, uint32_t d, uint32_t dp,
