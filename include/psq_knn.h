@@ -104,6 +104,33 @@ int psq_neighbors_exact(const float* embedding, uint64_t n_obs, uint64_t n_dim,
                         uint64_t n_neighbors, int32_t metric, uint32_t* indptr, uint32_t* indices,
                         float* dists, uint64_t cap, uint64_t* nnz_out, int32_t device);
 
+/* ── learned encoder (no reference counterpart; used by PSQ_INIT_LEARNED) ──
+ * A 50 -> 128 -> 32 MLP (GELU, BF16-rounded weights, FP32 accumulation),
+ * trained with a triplet margin loss on exact-kNN triplets of a subset. */
+typedef struct psq_encoder psq_encoder;
+typedef struct psq_encoder_train_opts {
+  uint64_t subset; /* cells in the training subset (0 -> 200000), evenly spaced */
+  uint64_t k_pos;  /* positives drawn from the exact top-k_pos (0 -> 15) */
+  uint64_t batch;  /* triplets per step (0 -> 2048) */
+  uint64_t steps;  /* Adam steps (0 -> 1000) */
+  float lr;        /* 0 -> 1e-3 */
+  float margin;    /* triplet margin on squared code distances (0 -> 1.0) */
+  uint64_t seed;
+} psq_encoder_train_opts;
+int psq_encoder_create(psq_encoder** out, uint64_t seed, int32_t device);
+void psq_encoder_destroy(psq_encoder* e);
+int psq_encoder_train(psq_encoder* e, const float* d_data, uint64_t n, uint64_t d,
+                      const psq_encoder_train_opts* o, void* stream);
+int psq_encoder_encode(const psq_encoder* e, const float* d_data, uint64_t n, uint64_t d,
+                       float* d_codes, void* stream);
+int psq_encoder_params(const psq_encoder* e, float* host_params, uint64_t count, float* mu_scale);
+int psq_encoder_set_params(psq_encoder* e, const float* host_params, uint64_t count, float mu,
+                           float inv_scale);
+int psq_encoder_grad(const psq_encoder* e, const float* d_data, uint64_t n, uint64_t d,
+                     const uint32_t* rows, uint64_t B, float margin, float* host_grad,
+                     double* loss_out);
+int psq_encoder_info(const psq_encoder* e, double* last_loss, double* train_ms, uint64_t* steps);
+
 /* Per-run statistics of the last NN-Descent call on this thread (for reports). */
 typedef struct psq_nndescent_stats {
   uint64_t n_obs, n_dim, k, max_candidates, max_iterations, n_trees, leaf_size, slots, stop_at;

@@ -223,11 +223,14 @@ def nndescent_port(data: np.ndarray, k: int, max_iterations: int = 0, seed: int 
 def gpu_model_nndescent(data: np.ndarray, k: int, max_iterations: int = 0, seed: int = 0,
                         max_candidates: int = 0, n_trees: int = 0, delta: float = 0.001,
                         leaf_size: int = 0, slots: int = 0, init: int = 0,
-                        init_graph: np.ndarray | None = None, threads: int | None = None):
+                        init_graph: np.ndarray | None = None, threads: int | None = None,
+                        split_data: np.ndarray | None = None):
     """The GPU algorithm's semantics on the CPU; returns (graph dict, ModelStats).
 
     ``init``: 0 = RP forest, 1 = random fill only, 2 = seed rows from ``init_graph``
-    (u32[n, c] candidate indices, 0xFFFFFFFF = none).
+    (u32[n, c] candidate indices, 0xFFFFFFFF = none), 3 = RP forest splitting the
+    rows of ``split_data`` (f32[n, ds], e.g. learned codes), leaf pairs measured on
+    ``data``.
     """
     data = _f32(data)
     n, d = data.shape
@@ -243,6 +246,10 @@ def gpu_model_nndescent(data: np.ndarray, k: int, max_iterations: int = 0, seed:
     if init_graph is not None:
         ig = np.ascontiguousarray(init_graph, dtype=np.uint32)
         ig_cols = ig.shape[1]
+    if split_data is not None:
+        ig = np.ascontiguousarray(split_data, dtype=np.float32)
+        ig_cols = ig.shape[1]
+        init = 3
     nnz = lib().psq_o_gpu_model_nndescent(
         _ptr(data), n, d, k, max_candidates, max_iterations, delta, n_trees, leaf_size, slots,
         seed, (init & 0xFF) | (ig_cols << 8), None if ig is None else _ptr(ig),

@@ -109,9 +109,9 @@ static size_t merge_row(uint64_t* row, size_t k, std::vector<uint64_t>& props) {
 }
 
 // One RP tree, level by level. Appends leaves as (start, count) into `ids`.
-static void build_tree(const float* data, const Params& P, uint64_t tseed, std::vector<uint32_t>& ids,
-                       std::vector<std::pair<uint32_t, uint32_t>>& leaves) {
-  const size_t n = P.n, d = P.d;
+static void build_tree(const float* data, size_t d, const Params& P, uint64_t tseed,
+                       std::vector<uint32_t>& ids, std::vector<std::pair<uint32_t, uint32_t>>& leaves) {
+  const size_t n = P.n;
   ids.resize(n);
   for (size_t i = 0; i < n; ++i) ids[i] = (uint32_t)i;
   std::vector<std::pair<uint32_t, uint32_t>> level_nodes{{0u, (uint32_t)n}}, next;
@@ -192,7 +192,9 @@ struct psq_o_model_stats {
 };
 
 // init: low byte 0 = RP forest, 1 = random fill only, 2 = seed rows from
-// init_graph (u32[n][cols], cols in bits 8..31 of init).
+// init_graph (u32[n][cols], cols in bits 8..31 of init), 3 = RP forest whose
+// hyperplanes split the f32 rows passed in init_graph (n x ds, ds in bits
+// 8..31), e.g. learned codes, with leaf pairs measured on `data`.
 int64_t psq_o_gpu_model_nndescent(const float* data, uint64_t n_obs, uint64_t n_dim, uint64_t k_req,
                                   uint64_t max_candidates, uint64_t max_iterations, double delta,
                                   uint64_t n_trees_req, uint64_t leaf_size_req, uint64_t slots_req,
@@ -226,11 +228,14 @@ int64_t psq_o_gpu_model_nndescent(const float* data, uint64_t n_obs, uint64_t n_
   std::vector<uint64_t> G(n * k, EMPTY_KEY);
 
   // ── 1. initial rows ──
-  if (init_kind == 0) {
+  if (init_kind == 0 || init_kind == 3) {
+    if (init_kind == 3 && (!init_graph || init_cols == 0)) return -1;
+    const float* sdata = init_kind == 3 ? reinterpret_cast<const float*>(init_graph) : data;
+    const size_t sd = init_kind == 3 ? init_cols : d;
     std::vector<std::vector<uint32_t>> ids(P.n_trees);
     std::vector<std::vector<std::pair<uint32_t, uint32_t>>> leaves(P.n_trees);
     pool.parallel_for(P.n_trees, 1, [&](size_t t) {
-      build_tree(data, P, splitmix(seed ^ (uint64_t)(t + 1)), ids[t], leaves[t]);
+      build_tree(sdata, sd, P, splitmix(seed ^ (uint64_t)(t + 1)), ids[t], leaves[t]);
     });
     // Leaf pairs, tree after tree. A leaf larger than the group size LG
     // (smallest power of two >= leaf_size, within [16, 64]) contributes pairs
@@ -458,7 +463,7 @@ int64_t psq_o_gpu_model_nndescent(const float* data, uint64_t n_obs, uint64_t n_
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
     stats->iterations = iters_done;
-    stats->n_trees = init_kind == 0 ? P.n_trees : 0;
+    stats->n_trees = (init_kind == 0 || init_kind == 3) ? P.n_trees : 0;
     stats->max_candidates = m;
     stats->leaf_size = P.leaf_size;
     stats->max_iterations = P.n_iters;

@@ -17,4 +17,6 @@ from .neighbors import (  # noqa: F401
     release_workspace,
 )
 
+from .encoder import Encoder  # noqa: F401,E402
+
 __version__ = "0.1.0"

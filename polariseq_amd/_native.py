@@ -34,7 +34,27 @@ EXPORTS = (
     "psq_last_error",
     "psq_release_workspace",
     "psq_set_device_budget",
+    "psq_encoder_create",
+    "psq_encoder_destroy",
+    "psq_encoder_train",
+    "psq_encoder_encode",
+    "psq_encoder_params",
+    "psq_encoder_set_params",
+    "psq_encoder_grad",
+    "psq_encoder_info",
 )
+
+
+class EncoderTrainOpts(ctypes.Structure):
+    _fields_ = [
+        ("subset", ctypes.c_uint64),
+        ("k_pos", ctypes.c_uint64),
+        ("batch", ctypes.c_uint64),
+        ("steps", ctypes.c_uint64),
+        ("lr", ctypes.c_float),
+        ("margin", ctypes.c_float),
+        ("seed", ctypes.c_uint64),
+    ]
 
 
 class NndescentOpts(ctypes.Structure):
@@ -138,6 +158,24 @@ def _declare(L: ctypes.CDLL) -> None:
     L.psq_release_workspace.argtypes = []
     L.psq_set_device_budget.restype = None
     L.psq_set_device_budget.argtypes = [U64]
+    F32, PP = ctypes.c_float, ctypes.POINTER(ctypes.c_void_p)
+    L.psq_encoder_create.restype = ctypes.c_int
+    L.psq_encoder_create.argtypes = [PP, U64, I32]
+    L.psq_encoder_destroy.restype = None
+    L.psq_encoder_destroy.argtypes = [P]
+    L.psq_encoder_train.restype = ctypes.c_int
+    L.psq_encoder_train.argtypes = [P, P, U64, U64, ctypes.POINTER(EncoderTrainOpts), P]
+    L.psq_encoder_encode.restype = ctypes.c_int
+    L.psq_encoder_encode.argtypes = [P, P, U64, U64, P, P]
+    L.psq_encoder_params.restype = ctypes.c_int
+    L.psq_encoder_params.argtypes = [P, P, U64, P]
+    L.psq_encoder_set_params.restype = ctypes.c_int
+    L.psq_encoder_set_params.argtypes = [P, P, U64, F32, F32]
+    L.psq_encoder_grad.restype = ctypes.c_int
+    L.psq_encoder_grad.argtypes = [P, P, U64, U64, P, U64, F32, P, ctypes.POINTER(ctypes.c_double)]
+    L.psq_encoder_info.restype = ctypes.c_int
+    L.psq_encoder_info.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                   ctypes.POINTER(U64)]
 
 
 class PsqError(RuntimeError):

@@ -19,12 +19,8 @@
 
 using namespace psq;
 
-namespace {
-
+namespace psq_host {
 thread_local std::string g_err;
-thread_local psq_nndescent_stats g_stats;
-uint64_t g_budget = 0;  // 0 = no cap
-
 int fail(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
@@ -34,6 +30,17 @@ int fail(int code, const char* fmt, ...) {
   g_err = buf;
   return code;
 }
+int knn_exact_device_impl(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d_queries,
+                          uint64_t nq, uint64_t k, uint32_t* d_idx, float* d_dist, hipStream_t s);
+}  // namespace psq_host
+
+using psq_host::fail;
+using psq_host::g_err;
+
+namespace {
+
+thread_local psq_nndescent_stats g_stats;
+uint64_t g_budget = 0;  // 0 = no cap
 
 #define HIPC(call)                                                                         \
   do {                                                                                     \
@@ -167,6 +174,11 @@ int scan_u32(Workspace* ws, const uint32_t* in, uint32_t* out, uint32_t n, hipSt
 
 uint32_t round_up(uint32_t v, uint32_t m) { return (v + m - 1) / m * m; }
 
+// BF16 join: a proposal is kept when its approximate distance is below the
+// destination's threshold times this factor (the approximation error is a few
+// parts per thousand on centred vectors; the merge re-ranks exactly).
+constexpr float BF16_SLACK = 1.03f;
+
 struct Resolved {
   uint64_t n, d, k, m, n_iters, n_trees, leaf_size, S, stop_at, seed;
   uint32_t dp;
@@ -205,10 +217,18 @@ int resolve(uint64_t n, uint64_t d, const psq_nndescent_opts* o, Resolved* r) {
   if (r->k > 64) return fail(PSQ_E_UNSUPPORTED, "k > 64 is not supported on the GPU path");
   if (r->m > 64) return fail(PSQ_E_UNSUPPORTED, "max_candidates > 64 is not supported on the GPU path");
   if (r->S + r->k > 512) return fail(PSQ_E_UNSUPPORTED, "slots + k > 512 is not supported");
-  if (o->precision != PSQ_PRECISION_FP32)
-    return fail(PSQ_E_UNSUPPORTED, "precision %d is not available in this build", o->precision);
-  if (o->init != PSQ_INIT_RANDOM && o->init != PSQ_INIT_RP_FOREST)
-    return fail(PSQ_E_UNSUPPORTED, "init %d is not available in this build", o->init);
+  if (o->precision != PSQ_PRECISION_FP32 && o->precision != PSQ_PRECISION_BF16_RERANK)
+    return fail(PSQ_E_INVALID, "unknown precision %d", o->precision);
+  if (o->precision == PSQ_PRECISION_BF16_RERANK && d > 64)
+    return fail(PSQ_E_UNSUPPORTED, "the BF16 form supports n_dim <= 64");
+  if (o->precision == PSQ_PRECISION_BF16_RERANK && r->S > 256)
+    return fail(PSQ_E_UNSUPPORTED, "the BF16 form supports slots <= 256");
+  if (o->init != PSQ_INIT_RANDOM && o->init != PSQ_INIT_RP_FOREST && o->init != PSQ_INIT_LEARNED)
+    return fail(PSQ_E_INVALID, "unknown init %d", o->init);
+  if (o->init == PSQ_INIT_LEARNED && !o->model)
+    return fail(PSQ_E_INVALID, "init=learned needs a trained encoder (model is NULL)");
+  if (o->init == PSQ_INIT_LEARNED && d != 50)
+    return fail(PSQ_E_UNSUPPORTED, "the learned encoder takes 50 dimensions");
   return PSQ_OK;
 }
 
@@ -303,8 +323,10 @@ int launch_leaf_merge(uint32_t KM, uint32_t LG, const float* Xp, uint32_t d, uin
 // RP forest start: trees built level by level, a batch of trees at a time (so
 // every position index fits in 32 bits), then the leaf pairs of each tree are
 // merged into the rows, tree after tree (nndescent.rs:767-817).
-int build_forest(Workspace* ws, const Resolved& R, const float* Xp, uint64_t* G, hipStream_t s,
-                 uint32_t* hbuf) {
+// Xs (ds dims, row stride dps) is what the hyperplanes split: the embedding
+// itself, or the learned codes. Leaf pairs are measured on Xp (the embedding).
+int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds, uint32_t dps,
+                 const float* Xp, uint64_t* G, hipStream_t s, uint32_t* hbuf) {
   const uint32_t n = (uint32_t)R.n, d = (uint32_t)R.d, dp = R.dp, k = (uint32_t)R.k;
   const uint32_t ls = (uint32_t)R.leaf_size;
   const uint64_t T = R.n_trees;
@@ -343,7 +365,7 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xp, uint64_t* G,
   TRY(wbuf(ws, "f_aoff", node_cap + 1, &aoff));
   TRY(wbuf(ws, "f_loff", node_cap + 1, &loff));
   TRY(wbuf(ws, "f_child", 2 * node_cap, &child));
-  TRY(wbuf(ws, "f_normals", node_cap * dp, &normals));
+  TRY(wbuf(ws, "f_normals", node_cap * dps, &normals));
   TRY(wbuf(ws, "f_offsets", node_cap, &offsets));
   // leaves: each leaf holds at least one point, so a batch has at most TB * n
   uint32_t *lt, *lsrt, *lcnt, *gs, *gc, *tcount;
@@ -377,10 +399,10 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xp, uint64_t* G,
     int cur = 0;
     for (uint32_t L = 0; A > 0; ++L) {
       const int nx = cur ^ 1;
-      hipLaunchKernelGGL(k_forest_planes, dim3(grid_for(A, 4)), dim3(256), 0, s, Xp, d, dp, n,
+      hipLaunchKernelGGL(k_forest_planes, dim3(grid_for(A, 4)), dim3(256), 0, s, Xs, ds, dps, n,
                          ids[cur], ns[cur], nc[cur], nt[cur], A, tseeds + t0, L, normals, offsets);
       HIPC(hipGetLastError());
-      hipLaunchKernelGGL(k_forest_sides, dim3(grid_for(total, 256)), dim3(256), 0, s, Xp, d, dp, n,
+      hipLaunchKernelGGL(k_forest_sides, dim3(grid_for(total, 256)), dim3(256), 0, s, Xs, ds, dps, n,
                          total, ids[cur], segv[cur], nt[cur], tseeds + t0, L, normals, offsets, fl, fr);
       HIPC(hipGetLastError());
       TRY(scan_u32(ws, fl, sl, (uint32_t)total, s));
@@ -503,7 +525,13 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
 
   // ── start: RP forest (or nothing), random fill, then one forced merge sorts every row ──
   if (opts->init == PSQ_INIT_RP_FOREST && R.n_trees > 0)
-    TRY(build_forest(ws, R, Xp, G, s, static_cast<uint32_t*>(hp) + 64));
+    TRY(build_forest(ws, R, Xp, (uint32_t)R.d, R.dp, Xp, G, s, static_cast<uint32_t*>(hp) + 64));
+  if (opts->init == PSQ_INIT_LEARNED && R.n_trees > 0) {
+    float* codes;
+    TRY(wbuf(ws, "codes", (size_t)n * 32, &codes));
+    TRY(psq_encoder_encode(static_cast<const psq_encoder*>(opts->model), d_data, n, R.d, codes, s));
+    TRY(build_forest(ws, R, codes, 32u, 32u, Xp, G, s, static_cast<uint32_t*>(hp) + 64));
+  }
   hipLaunchKernelGGL(k_random_fill, dim3(grid_for(n, 256)), dim3(256), 0, s, Xp, n,
                      (uint32_t)R.d, R.dp, G, k, R.seed, touched);
   HIPC(hipGetLastError());
@@ -513,6 +541,7 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   // ── iterations ──
   psq_nndescent_stats st;
   std::memset(&st, 0, sizeof(st));
+  const bool bf16 = opts->precision == PSQ_PRECISION_BF16_RERANK;
   std::vector<hipEvent_t> jb, je;
   uint64_t iters = 0;
   for (uint64_t it = 0; it < R.n_iters; ++it) {
@@ -531,11 +560,32 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
     TRY(evp.make(&a));
     TRY(evp.make(&b));
     HIPC(hipEventRecord(a, s));
-    TRY(launch_join(R, Xp, newl, oldl, ncnt, thr, slots, touched, it, ctr, s));
+    if (bf16) {
+      hipLaunchKernelGGL(k_join_bf16, dim3(grid_for(n, 4)), dim3(256), 0, s, Xp, n, (uint32_t)R.d,
+                         R.dp, newl, oldl, ncnt, m, thr, BF16_SLACK, slots, touched, S, R.seed, it,
+                         ctr);
+      HIPC(hipGetLastError());
+    } else {
+      TRY(launch_join(R, Xp, newl, oldl, ncnt, thr, slots, touched, it, ctr, s));
+    }
     HIPC(hipEventRecord(b, s));
     jb.push_back(a);
     je.push_back(b);
-    TRY(launch_merge(R, G, slots, touched, thr, 0, ctr + 2, s));
+    if (bf16) {
+      const unsigned g4 = grid_for(n, 4);
+      if (S <= 64)
+        hipLaunchKernelGGL(k_merge_rerank<1>, dim3(g4), dim3(256), 0, s, Xp, (uint32_t)R.d, R.dp, G,
+                           n, k, slots, S, touched, thr, ctr + 2);
+      else if (S <= 128)
+        hipLaunchKernelGGL(k_merge_rerank<2>, dim3(g4), dim3(256), 0, s, Xp, (uint32_t)R.d, R.dp, G,
+                           n, k, slots, S, touched, thr, ctr + 2);
+      else
+        hipLaunchKernelGGL(k_merge_rerank<4>, dim3(g4), dim3(256), 0, s, Xp, (uint32_t)R.d, R.dp, G,
+                           n, k, slots, S, touched, thr, ctr + 2);
+      HIPC(hipGetLastError());
+    } else {
+      TRY(launch_merge(R, G, slots, touched, thr, 0, ctr + 2, s));
+    }
     HIPC(hipMemcpyAsync(hctr, ctr, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
     const uint64_t changed = hctr[2];
@@ -591,8 +641,11 @@ int set_device(int32_t device) {
   return PSQ_OK;
 }
 
-int knn_exact_device_impl(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d_queries,
-                          uint64_t nq, uint64_t k, uint32_t* d_idx, float* d_dist, hipStream_t s) {
+}  // namespace
+
+int psq_host::knn_exact_device_impl(const float* d_data, uint64_t n, uint64_t d,
+                                    const uint32_t* d_queries, uint64_t nq, uint64_t k,
+                                    uint32_t* d_idx, float* d_dist, hipStream_t s) {
   if (n == 0 || d == 0) return fail(PSQ_E_INVALID, "empty data: n=%llu d=%llu",
                                     (unsigned long long)n, (unsigned long long)d);
   if (k == 0) return fail(PSQ_E_INVALID, "k must be >= 1");
@@ -667,7 +720,7 @@ int knn_exact_device_impl(const float* d_data, uint64_t n, uint64_t d, const uin
   return PSQ_OK;
 }
 
-}  // namespace
+using psq_host::knn_exact_device_impl;
 
 extern "C" {
 

@@ -267,37 +267,6 @@ __global__ __launch_bounds__(256) void k_sample_select(
   }
 }
 
-// ── proposals: bucket cascade ──
-// Insert `key` into the 8-slot bucket of row `row` chosen by the neighbour's
-// hash. Each bucket holds its 8 smallest distinct keys in ascending order: a
-// key starts at its rank among the slots read (slots only decrease, so the
-// smaller ones stay smaller), then an atomic min either places it and carries
-// the displaced key down, or carries it on. Equal keys stop the cascade.
-__device__ inline void propose(uint64_t* __restrict__ slots, uint8_t* __restrict__ touched,
-                               uint32_t row, uint64_t key, uint32_t S, uint32_t bmask,
-                               uint64_t seed, uint64_t iter) {
-  uint32_t b = slot_hash(seed, iter, key_idx(key)) & bmask;
-  uint64_t* bk = slots + (size_t)row * S + (size_t)b * BUCKET;
-  const ulonglong2* b2 = reinterpret_cast<const ulonglong2*>(bk);
-  ulonglong2 v01 = b2[3];
-  if (key >= v01.y) return;  // 8 smaller keys already present (or this one)
-  ulonglong2 a = b2[0], bb = b2[1], c = b2[2];
-  uint64_t v[8] = {a.x, a.y, bb.x, bb.y, c.x, c.y, v01.x, v01.y};
-  int r = 0;
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    if (v[j] == key) return;
-    r += (v[j] < key);
-  }
-  uint64_t cur = key;
-  for (int j = r; j < BUCKET; ++j) {
-    uint64_t old = atomicMin((unsigned long long*)&bk[j], (unsigned long long)cur);
-    if (old == cur || old == EMPTY_KEY) break;
-    cur = old > cur ? old : cur;
-  }
-  touched[row] = 1;
-}
-
 // ── local join (nndescent.rs:912-969) ──
 // One wave per cell i. Lane t holds candidate slot t (and t + 64 when CPL = 2):
 // slots [0, nn) are the new list, [nn, nn + no) the old list. The new rows are

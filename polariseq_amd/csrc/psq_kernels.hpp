@@ -66,6 +66,33 @@ __global__ void k_leaf_merge(const float* Xp, uint32_t d, uint32_t dp, uint32_t 
                              const uint32_t* g_start, const uint32_t* g_count, uint32_t nleaves,
                              uint64_t* G, uint32_t k);
 
+// precision form (bf16_kernels.hip)
+__global__ void k_join_bf16(const float* Xp, uint32_t n, uint32_t d, uint32_t dp, const uint32_t* newl,
+                            const uint32_t* oldl, const uint32_t* ncnt, uint32_t m, const float* thr,
+                            float slack, uint64_t* slots, uint8_t* touched, uint32_t S, uint64_t seed,
+                            uint64_t iter, unsigned long long* counters);
+template <int NR>
+__global__ void k_merge_rerank(const float* Xp, uint32_t d, uint32_t dp, uint64_t* G, uint32_t n,
+                               uint32_t k, uint64_t* slots, uint32_t S, uint8_t* touched, float* thr,
+                               unsigned long long* changed);
+
+// learned encoder (mlp_kernels.hip)
+constexpr int MLP_N_PARAMS = 128 * 50 + 128 + 32 * 128 + 32;
+__global__ void k_mlp_round(const float* w, float* wr, uint32_t count);
+__global__ void k_mlp_forward(const float* X, uint32_t ldx, const uint32_t* idx, uint32_t rows,
+                              const float* pr, float mu, float inv_scale, float* hpre_out, float* z_out);
+__global__ void k_triplet_loss(const float* z, uint32_t B, float margin, float* dz, float* loss_terms);
+__global__ void k_mlp_backward(const float* X, uint32_t ldx, const uint32_t* idx, uint32_t rows,
+                               const float* pr, float mu, float inv_scale, const float* hpre,
+                               const float* dz, float* partial);
+__global__ void k_mlp_reduce(const float* partial, uint32_t nblocks, float* grad);
+__global__ void k_adam(float* w, float* wr, const float* g, float* m1, float* m2, float lr, float b1,
+                       float b2, float eps, float c1, float c2);
+__global__ void k_triplets(const uint32_t* knn, uint32_t kk, uint32_t ns, uint32_t B, uint64_t seed,
+                           uint64_t step, uint32_t* rows);
+__global__ void k_gather_rows(const float* src, uint32_t d, const uint32_t* idx, uint32_t rows, float* dst);
+__global__ void k_moments(const float* X, uint32_t n, uint32_t d, uint32_t chunk, double* partial);
+
 // exact kNN (exact_kernels.hip)
 template <int D, int K>
 __global__ void k_exact_pass1(const float* Xp, uint32_t n, uint32_t d, uint32_t dp,

@@ -1,0 +1,49 @@
+"""GPU checks of the precision form (BF16 MFMA join tiles + FP32 re-rank).
+
+The candidate ranking uses BF16 MFMA dot products whose internal accumulation
+order is the hardware's, so this form is not compared bit for bit with a CPU
+model. What is checked: every output distance is the reference's exact f32
+sq_dist (simd.rs:45-62) square-rooted, bit for bit; the graph invariants of
+nndescent.rs:1081-1107; determinism for a seed; recall close to the plain form's.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import polariseq_amd as P
+from polariseq_amd import datasets as ds
+
+pytestmark = pytest.mark.gpu
+
+
+def _valid(g, n, k):
+    ip, ix, dx = g["indptr"], g["indices"], g["data"]
+    assert (np.diff(ip) == k).all()
+    rows = ix.reshape(n, k)
+    assert not (rows == np.arange(n)[:, None]).any()
+    s = np.sort(rows, axis=1)
+    assert not (s[:, 1:] == s[:, :-1]).any()
+    assert (np.diff(dx.reshape(n, k), axis=1) >= 0).all()
+
+
+@pytest.mark.parametrize("k", [15, 30])
+def test_bf16_form_exact_distances_invariants_and_recall(orc, k):
+    n = 20000
+    x = ds.mixture(n, 50, 20, seed=0)
+    g = P.nndescent_dense(x, k, seed=0, precision="bf16")
+    _valid(g, n, k)
+    ix, dx = g["indices"].reshape(n, k), g["data"].reshape(n, k)
+    for i in range(0, n, 211):
+        for s in range(k):
+            want = np.float32(np.sqrt(orc.sq_dist(x[i], x[ix[i, s]])))
+            assert dx[i, s].view(np.uint32) == want.view(np.uint32)
+    again = P.nndescent_dense(x, k, seed=0, precision="bf16")
+    assert np.array_equal(again["indices"], g["indices"])
+    q = np.random.default_rng(1).choice(n, 3000, replace=False).astype(np.uint32)
+    ei, _ = orc.knn_exact64(x, k, q)
+    plain = P.nndescent_dense(x, k, seed=0)
+    rb = ds.recall(ix[q], ei).mean()
+    rp = ds.recall(plain["indices"].reshape(n, k)[q], ei).mean()
+    assert rb >= 0.95 and rb >= rp - 0.01, (rb, rp)
