@@ -52,7 +52,14 @@ def main() -> int:
     ap.add_argument("--components", type=int, default=150)
     ap.add_argument("--k", type=int, default=15)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--init", default="random", choices=["random", "rp_forest"])
+    ap.add_argument("--form", default="all", choices=["plain", "bf16", "learned_bf16", "all"],
+                    help="plain = FP32 + RP forest (the drop-in); bf16 = BF16 join + re-rank; "
+                         "learned_bf16 = learned seeding + BF16; all = plain and learned_bf16")
+    ap.add_argument("--init", default="rp_forest", choices=["random", "rp_forest"],
+                    help="start of the plain and bf16 forms")
+    ap.add_argument("--learned-trees", type=int, default=8)
+    ap.add_argument("--train-steps", type=int, default=600)
+    ap.add_argument("--train-subset", type=int, default=200_000)
     ap.add_argument("--recall-queries", type=int, default=100_000)
     ap.add_argument("--cpu-cells", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
@@ -76,33 +83,78 @@ def main() -> int:
     xt = torch.from_numpy(x).cuda()
     torch.cuda.synchronize()
 
-    def step():
-        return P.nndescent_device(xt, k, seed=args.seed, init=args.init)
-
-    for w in range(args.warmup):
+    forms = ["plain", "learned_bf16"] if args.form == "all" else [args.form]
+    enc = None
+    train_info = None
+    if "learned_bf16" in forms:
         t = time.perf_counter()
-        step()
+        enc = P.Encoder(seed=args.seed)
+        train_info = enc.train(xt, subset=args.train_subset, steps=args.train_steps, seed=args.seed)
         torch.cuda.synchronize()
-        log(f"warmup {w + 1}/{args.warmup}: {time.perf_counter() - t:.3f} s")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    out = None
-    stats = []
-    for s in range(args.steps):
-        out = step()
-        stats.append(P.last_stats())
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+        log(f"encoder trained: {train_info} ({time.perf_counter() - t:.2f} s wall)")
+
+    def make_step(form):
+        if form == "plain":
+            return lambda: P.nndescent_device(xt, k, seed=args.seed, init=args.init)
+        if form == "bf16":
+            return lambda: P.nndescent_device(xt, k, seed=args.seed, init=args.init,
+                                              precision="bf16")
+        return lambda: P.nndescent_device(xt, k, seed=args.seed, init="learned", model=enc,
+                                          precision="bf16", n_trees=args.learned_trees)
+
+    results = {}
+    for form in forms:
+        step = make_step(form)
+        for w in range(args.warmup):
+            t = time.perf_counter()
+            step()
+            torch.cuda.synchronize()
+            log(f"[{form}] warmup {w + 1}/{args.warmup}: {time.perf_counter() - t:.3f} s")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = None
+        stats = []
+        for _ in range(args.steps):
+            out = step()
+            stats.append(P.last_stats())
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        st = stats[-1]
+        log(f"[{form}] {1000 * elapsed / args.steps:.1f} ms/step, {n * args.steps / elapsed:,.0f} "
+            f"cells/s; iterations {st['iterations']}, ms init {st['ms_init']:.1f} iter "
+            f"{st['ms_iterations']:.1f} extract {st['ms_extract']:.1f}, changed {st['changed']}, "
+            f"join ms {[round(v, 2) for v in st['ms_join']]}")
+        results[form] = {"elapsed": elapsed, "stats": stats, "out": out}
+
+    head = forms[-1]
+    elapsed = results[head]["elapsed"]
+    stats = results[head]["stats"]
+    out = results[head]["out"]
     ms_per_step = 1000.0 * elapsed / args.steps
     value = n * args.steps / elapsed
-    log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step, {value:,.0f} cells/s")
     st = stats[-1]
-    log(f"last step: iterations {st['iterations']}, ms init {st['ms_init']:.1f} "
-        f"iter {st['ms_iterations']:.1f} extract {st['ms_extract']:.1f}, changed {st['changed']}")
 
-    # roofline of the dominant kernel (the local join), over every timed launch:
-    # algorithmic bytes = candidate rows gathered x d x 4 B + the two candidate lists
-    # (2 m x 4 B per cell) + one threshold per candidate (4 B).
+    # recall@15 of each form against exact kNN over all cells (same queries)
+    nq = min(args.recall_queries, n)
+    q = np.sort(np.random.default_rng(12345).choice(n, nq, replace=False)).astype(np.int32)
+    qt = torch.from_numpy(q).cuda()
+    t = time.perf_counter()
+    ei, _ = P.knn_exact_device(xt, k, qt)
+    torch.cuda.synchronize()
+    log(f"exact kNN for {nq} queries: {time.perf_counter() - t:.2f} s")
+    ei = ei.cpu().numpy()
+    recalls = {}
+    for form, r in results.items():
+        ix = r["out"][1]
+        gq = ix[: n * k].view(n, k)[qt.long()].cpu().numpy()
+        rec = ds.recall(gq, ei)
+        recalls[form] = (float(rec.mean()), float(np.percentile(rec, 5)))
+        log(f"[{form}] recall@{k} = {recalls[form][0]:.4f} (p05 {recalls[form][1]:.3f})")
+    recall = recalls[head][0]
+
+    # roofline of the dominant kernel of the head form (its local join), over every
+    # timed launch: algorithmic bytes = candidate rows gathered x (d x 4 B + 4 B
+    # threshold) + the two candidate lists (2m x 4 B per cell) + the count word.
     m = st["max_candidates"]
     join_bytes = 0.0
     join_ms = 0.0
@@ -115,22 +167,9 @@ def main() -> int:
     achieved = join_bytes / (join_ms / 1000.0) / 1e9 if join_ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_join<50,1>", "launches": launches,
-                "avg_launch_ms": round(join_ms / max(launches, 1), 3),
+                "kernel": "k_join_bf16" if "bf16" in head else "k_join<50,1>",
+                "launches": launches, "avg_launch_ms": round(join_ms / max(launches, 1), 3),
                 "algorithmic_bytes_per_launch": round(join_bytes / max(launches, 1))}
-
-    # recall@15 against exact kNN over all cells
-    ip, ix, dx, nnz = out
-    nq = min(args.recall_queries, n)
-    q = np.sort(np.random.default_rng(12345).choice(n, nq, replace=False)).astype(np.int32)
-    t = time.perf_counter()
-    ei, _ = P.knn_exact_device(xt, k, torch.from_numpy(q).cuda())
-    torch.cuda.synchronize()
-    log(f"exact kNN for {nq} queries: {time.perf_counter() - t:.2f} s")
-    g = ix[: n * k].view(n, k)[torch.from_numpy(q).cuda().long()].cpu().numpy()
-    rec = ds.recall(g, ei.cpu().numpy())
-    recall = float(rec.mean())
-    log(f"recall@{k} = {recall:.4f} (p05 {np.percentile(rec, 5):.3f})")
 
     cpu = None
     if not args.no_cpu:
@@ -148,22 +187,42 @@ def main() -> int:
                          f" {ct:.2f} s"}
         log(f"CPU baseline: {nc / ct:,.0f} cells/s ({ct:.2f} s)")
 
+    forms_out = {}
+    for form, r in results.items():
+        s_ = r["stats"][-1]
+        forms_out[form] = {
+            "cells_per_s": round(n * args.steps / r["elapsed"], 1),
+            "ms_per_step": round(1000.0 * r["elapsed"] / args.steps, 3),
+            "recall_at_15": round(recalls[form][0], 5), "recall_p05": round(recalls[form][1], 4),
+            "iterations": s_["iterations"], "ms_init": round(s_["ms_init"], 2),
+            "ms_iterations": round(s_["ms_iterations"], 2), "ms_extract": round(s_["ms_extract"], 2),
+            "n_trees": s_["n_trees"]}
+    desc = {"plain": "plain GPU NN-Descent (FP32 join, RP forest start)",
+            "bf16": "BF16 MFMA join + FP32 re-rank, RP forest start",
+            "learned_bf16": "learned seeding + BF16 MFMA join with FP32 re-rank"}[head]
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "cells/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16" if "bf16" in head else "f32",
         "data": "synthetic Gaussian-mixture embedding (numpy, seeded); no dataset",
-        "config": {"workload": f"plain GPU NN-Descent, {n} x 50 FP32 cells, "
-                               f"{args.components}-component mixture, k={k}, init={args.init}",
-                   "n_cells": n, "dims": 50, "k": k, "form": "plain", "init": args.init,
-                   "seed": args.seed, "components": args.components,
-                   "iterations": st["iterations"], "ms_init": round(st["ms_init"], 2),
-                   "ms_iterations": round(st["ms_iterations"], 2),
-                   "ms_extract": round(st["ms_extract"], 2)},
+        "config": {"workload": f"{desc}, {n} x 50 FP32 cells, {args.components}-component mixture, "
+                               f"k={k}, seed {args.seed}",
+                   "n_cells": n, "dims": 50, "k": k, "form": head, "seed": args.seed,
+                   "components": args.components},
         "recall_at_15": round(recall, 5),
+        "forms": forms_out,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if "plain" in results and head != "plain":
+        line["speedup_vs_plain"] = round(forms_out[head]["cells_per_s"] /
+                                         forms_out["plain"]["cells_per_s"], 3)
+    if train_info is not None:
+        line["encoder"] = {"train_ms": round(train_info["train_ms"], 1),
+                           "steps": train_info["steps"], "subset": args.train_subset,
+                           "last_loss": round(train_info["last_loss"], 5),
+                           "note": "trained once before the timed steps and reused by every step"}
     print(json.dumps(line), flush=True)
     return 0
 

@@ -96,6 +96,30 @@ int psq_knn_exact_device(const float* d_data, uint64_t n, uint64_t d, const uint
                          uint64_t n_q, uint64_t k, uint32_t* d_idx_out, float* d_dist_out,
                          void* stream);
 
+/* Batched exact kNN: n_samples independent samples stored as contiguous row
+ * ranges [offsets[s], offsets[s+1]) of data (offsets is a HOST array of
+ * n_samples + 1 entries, offsets[0] = 0, offsets[n_samples] = n_total). Every
+ * point is a query against the points of its own sample, self excluded, in the
+ * order of psq_knn_exact; indices returned are global row indices. All samples
+ * run in one persistent launch per pass; tile shapes come from an autotuner
+ * that times candidate shapes once per shape class and caches the winner.
+ * Reference analogue: Scrublet's per-sample exact kNN
+ * (crates/polariseq-core/src/doublets.rs:557-651, 1085-1113, 1366-1374). */
+int psq_knn_exact_batched(const float* data, uint64_t n_total, uint64_t d, const uint64_t* offsets,
+                          uint64_t n_samples, uint64_t k, uint32_t* idx_out, float* dist_out,
+                          int32_t device);
+int psq_knn_exact_batched_device(const float* d_data, uint64_t n_total, uint64_t d,
+                                 const uint64_t* offsets, uint64_t n_samples, uint64_t k,
+                                 uint32_t* d_idx_out, float* d_dist_out, void* stream);
+typedef struct psq_batched_report {
+  uint32_t block, tile, seg_len; /* the tile shape used */
+  uint32_t tuned_now;            /* 1 when this call ran the autotuner */
+  uint32_t configs_tried, units, overflow_queries, pad;
+  double tune_ms, ms_pass1, ms_pass2, ms_rerank, ms_total;
+  double pairs; /* Σ n_s^2 distance pairs of the samples */
+} psq_batched_report;
+int psq_batched_last_report(psq_batched_report* out);
+
 /* Replaces neighbors() / neighbors_dense (crates/polariseq-core/src/neighbors.rs:101-147,
  * crates/polariseq-python/src/lib.rs:474-501): k_with_self = min(n_neighbors, n_obs);
  * each row holds k_with_self - 1 neighbours, self excluded. metric 0 = euclidean

@@ -7,7 +7,10 @@ The Python functions mirror the reference binding's ``nndescent_dense`` and
 """
 
 from .neighbors import (  # noqa: F401
+    batched_report,
     effective_k,
+    knn_exact_batched,
+    knn_exact_batched_device,
     knn_exact,
     knn_exact_device,
     last_stats,

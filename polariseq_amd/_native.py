@@ -42,7 +42,23 @@ EXPORTS = (
     "psq_encoder_set_params",
     "psq_encoder_grad",
     "psq_encoder_info",
+    "psq_knn_exact_batched",
+    "psq_knn_exact_batched_device",
+    "psq_batched_last_report",
 )
+
+
+class BatchedReport(ctypes.Structure):
+    _fields_ = [
+        ("block", ctypes.c_uint32), ("tile", ctypes.c_uint32), ("seg_len", ctypes.c_uint32),
+        ("tuned_now", ctypes.c_uint32), ("configs_tried", ctypes.c_uint32),
+        ("units", ctypes.c_uint32), ("overflow_queries", ctypes.c_uint32), ("pad", ctypes.c_uint32),
+        ("tune_ms", ctypes.c_double), ("ms_pass1", ctypes.c_double), ("ms_pass2", ctypes.c_double),
+        ("ms_rerank", ctypes.c_double), ("ms_total", ctypes.c_double), ("pairs", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
 
 
 class EncoderTrainOpts(ctypes.Structure):
@@ -173,6 +189,12 @@ def _declare(L: ctypes.CDLL) -> None:
     L.psq_encoder_set_params.argtypes = [P, P, U64, F32, F32]
     L.psq_encoder_grad.restype = ctypes.c_int
     L.psq_encoder_grad.argtypes = [P, P, U64, U64, P, U64, F32, P, ctypes.POINTER(ctypes.c_double)]
+    L.psq_knn_exact_batched.restype = ctypes.c_int
+    L.psq_knn_exact_batched.argtypes = [P, U64, U64, P, U64, U64, P, P, I32]
+    L.psq_knn_exact_batched_device.restype = ctypes.c_int
+    L.psq_knn_exact_batched_device.argtypes = [P, U64, U64, P, U64, U64, P, P, P]
+    L.psq_batched_last_report.restype = ctypes.c_int
+    L.psq_batched_last_report.argtypes = [ctypes.POINTER(BatchedReport)]
     L.psq_encoder_info.restype = ctypes.c_int
     L.psq_encoder_info.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                    ctypes.POINTER(U64)]

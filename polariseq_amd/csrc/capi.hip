@@ -713,7 +713,7 @@ int psq_host::knn_exact_device_impl(const float* d_data, uint64_t n, uint64_t d,
     TRY(wbuf(ws, "ex_fb_i", (size_t)h_novf * k, &si));
     hipLaunchKernelGGL(k_exact_fallback, dim3(grid_for(h_novf, 64)), dim3(64), 0, s, Xp,
                        (uint32_t)n, (uint32_t)d, dp, d_queries, (uint32_t)k, T, ovf, h_novf, sd, si,
-                       d_idx, d_dist);
+                       d_idx, d_dist, nullptr, 0u);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(s));
   }

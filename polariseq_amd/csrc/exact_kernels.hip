@@ -289,7 +289,8 @@ __global__ void k_exact_fallback(const float* __restrict__ Xp, uint32_t n, uint3
                                  const float* __restrict__ T, const uint32_t* __restrict__ overflow,
                                  uint32_t n_overflow, double* __restrict__ sd,
                                  uint32_t* __restrict__ si, uint32_t* __restrict__ idx_out,
-                                 float* __restrict__ dist_out) {
+                                 float* __restrict__ dist_out, const uint64_t* __restrict__ off,
+                                 uint32_t S) {
   const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
   if (o >= n_overflow) return;
   const uint32_t qs = overflow[o];
@@ -299,7 +300,17 @@ __global__ void k_exact_fallback(const float* __restrict__ Xp, uint32_t n, uint3
   double* ld = sd + (size_t)o * k;
   uint32_t* li = si + (size_t)o * k;
   uint32_t cntk = 0;
-  for (uint32_t p = 0; p < n; ++p) {
+  uint32_t p0 = 0, p1 = n;
+  if (off) {  // batched: scan the query's own sample only
+    uint32_t lo = 0, hi = S;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (off[mid] <= qi) lo = mid; else hi = mid;
+    }
+    p0 = (uint32_t)off[lo];
+    p1 = (uint32_t)off[lo + 1];
+  }
+  for (uint32_t p = p0; p < p1; ++p) {
     if (p == qi) continue;
     const float* pr = Xp + (size_t)p * dp;
     if (!(filt_dist_rt(pr, qrow, (int)d) <= thr)) continue;
@@ -339,5 +350,203 @@ template __global__ void k_exact_pass2<0>(const float*, uint32_t, uint32_t, uint
 template __global__ void k_exact_pass2<50>(const float*, uint32_t, uint32_t, uint32_t,
                                            const uint32_t*, uint32_t, uint32_t, const float*,
                                            uint32_t*, uint32_t*, uint32_t);
+
+}  // namespace psq
+
+// ── batched form: many independent samples in one persistent launch ──
+//
+// Samples are contiguous row ranges [off[s], off[s+1]) of one array. Every
+// point of a sample is a query against the points of its own sample (self
+// excluded), as Scrublet's per-sample kNN does (crates/polariseq-core/src/
+// doublets.rs:1085-1113, 1366-1374). Work units (sample, query block, point
+// segment) are pulled from an atomic counter by a grid sized to the device;
+// every block leaves the loop once the counter passes the unit count. The
+// block size (queries per block), the LDS tile and the segment length come
+// from the autotuner (capi.hip).
+namespace psq {
+
+template <int D, int K>
+__global__ __launch_bounds__(256) void k_bexact_pass1(
+    const float* __restrict__ Xp, uint32_t d, uint32_t dp, const uint64_t* __restrict__ off,
+    const uint32_t* __restrict__ units, uint32_t U, uint32_t seg_len, uint32_t tile,
+    const uint32_t* __restrict__ nseg_of, const uint64_t* __restrict__ part_off,
+    uint32_t* __restrict__ counter, float* __restrict__ partial) {
+  extern __shared__ float btile[];
+  __shared__ uint32_t s_u;
+  constexpr int DR = D > 0 ? D : 1;
+  for (;;) {
+    if (threadIdx.x == 0) s_u = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t u = s_u;
+    __syncthreads();
+    if (u >= U) break;
+    const uint32_t s = units[3 * u], qb = units[3 * u + 1], seg = units[3 * u + 2];
+    const uint64_t o = off[s];
+    const uint32_t ns = (uint32_t)(off[s + 1] - o);
+    const uint32_t ql = qb * blockDim.x + threadIdx.x;
+    const bool qvalid = ql < ns;
+    const uint64_t qi = o + (qvalid ? ql : 0u);
+    float qv[DR];
+    if (D > 0) {
+#pragma unroll
+      for (int t = 0; t < DR; ++t) qv[t] = Xp[qi * dp + t];
+    }
+    const float* qrow = Xp + qi * dp;
+    float top[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) top[j] = __builtin_inff();
+    const uint32_t s0 = seg * seg_len, s1 = min(ns, s0 + seg_len);
+    for (uint32_t base = s0; base < s1; base += tile) {
+      const uint32_t np = min(tile, s1 - base);
+      __syncthreads();
+      {
+        const uint32_t n4 = np * dp / 4;
+        const float4* src = reinterpret_cast<const float4*>(Xp + (o + base) * dp);
+        float4* dst = reinterpret_cast<float4*>(btile);
+        for (uint32_t e = threadIdx.x; e < n4; e += blockDim.x) dst[e] = src[e];
+      }
+      __syncthreads();
+      for (uint32_t j = 0; j < np; ++j) {
+        const float* pr = btile + (size_t)j * dp;
+        float dist = D > 0 ? filt_dist_fixed<DR>(pr, qv) : filt_dist_rt(pr, qrow, (int)d);
+        if (base + j == ql) dist = __builtin_inff();
+        if (dist < top[K - 1]) {
+#pragma unroll
+          for (int t = K - 1; t > 0; --t) {
+            float a = top[t - 1];
+            top[t] = dist < a ? a : (dist < top[t] ? dist : top[t]);
+          }
+          top[0] = dist < top[0] ? dist : top[0];
+        }
+      }
+    }
+    if (qvalid) {
+      float* out = partial + part_off[s] + ((size_t)ql * nseg_of[s] + seg) * K;
+#pragma unroll
+      for (int j = 0; j < K; ++j) out[j] = top[j];
+    }
+  }
+}
+
+// tau per global query: the k-th smallest over its sample's segments.
+__global__ void k_bexact_tau(const float* __restrict__ partial, const uint64_t* __restrict__ off,
+                             uint32_t S, const uint32_t* __restrict__ nseg_of,
+                             const uint64_t* __restrict__ part_off, uint64_t nq, uint32_t K,
+                             uint32_t k, uint32_t d, float* __restrict__ T) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nq) return;
+  uint32_t lo = 0, hi = S;  // sample of g: off[lo] <= g < off[lo + 1]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (off[mid] <= g) lo = mid; else hi = mid;
+  }
+  const uint32_t s = lo;
+  const uint64_t ql = g - off[s];
+  const uint32_t ns = nseg_of[s];
+  float top[64];
+#pragma unroll
+  for (int j = 0; j < 64; ++j) top[j] = __builtin_inff();
+  const float* src = partial + part_off[s] + ql * ns * K;
+  for (uint32_t e = 0; e < ns * K; ++e) {
+    const float v = src[e];
+    if (v < top[63]) {
+#pragma unroll
+      for (int t = 63; t > 0; --t) {
+        const float a = top[t - 1];
+        top[t] = v < a ? a : (v < top[t] ? v : top[t]);
+      }
+      top[0] = v < top[0] ? v : top[0];
+    }
+  }
+  float tau = __builtin_inff();
+#pragma unroll
+  for (int j = 0; j < 64; ++j)
+    if ((uint32_t)j == k - 1) tau = top[j];
+  if (tau >= __builtin_inff()) {
+    T[g] = tau;
+    return;
+  }
+  const double F = 1.0 + 4.0 * ((double)d + 8.0) * 5.9604644775390625e-08;
+  const double Td = (double)tau * F + 1e-30;
+  float Tf = (float)Td;
+  if ((double)Tf < Td) Tf = nextafterf(Tf, __builtin_inff());
+  T[g] = Tf;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_bexact_pass2(
+    const float* __restrict__ Xp, uint32_t d, uint32_t dp, const uint64_t* __restrict__ off,
+    const uint32_t* __restrict__ units, uint32_t U, uint32_t seg_len, uint32_t tile,
+    const float* __restrict__ T, uint32_t* __restrict__ counter, uint32_t* __restrict__ cnt,
+    uint32_t* __restrict__ cand, uint32_t cap) {
+  extern __shared__ float btile[];
+  __shared__ uint32_t s_u;
+  constexpr int DR = D > 0 ? D : 1;
+  for (;;) {
+    if (threadIdx.x == 0) s_u = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t u = s_u;
+    __syncthreads();
+    if (u >= U) break;
+    const uint32_t s = units[3 * u], qb = units[3 * u + 1], seg = units[3 * u + 2];
+    const uint64_t o = off[s];
+    const uint32_t ns = (uint32_t)(off[s + 1] - o);
+    const uint32_t ql = qb * blockDim.x + threadIdx.x;
+    const bool qvalid = ql < ns;
+    const uint64_t qi = o + (qvalid ? ql : 0u);
+    const float thr = qvalid ? T[qi] : -1.0f;
+    float qv[DR];
+    if (D > 0) {
+#pragma unroll
+      for (int t = 0; t < DR; ++t) qv[t] = Xp[qi * dp + t];
+    }
+    const float* qrow = Xp + qi * dp;
+    const uint32_t s0 = seg * seg_len, s1 = min(ns, s0 + seg_len);
+    for (uint32_t base = s0; base < s1; base += tile) {
+      const uint32_t np = min(tile, s1 - base);
+      __syncthreads();
+      {
+        const uint32_t n4 = np * dp / 4;
+        const float4* src = reinterpret_cast<const float4*>(Xp + (o + base) * dp);
+        float4* dst = reinterpret_cast<float4*>(btile);
+        for (uint32_t e = threadIdx.x; e < n4; e += blockDim.x) dst[e] = src[e];
+      }
+      __syncthreads();
+      for (uint32_t j = 0; j < np; ++j) {
+        const float* pr = btile + (size_t)j * dp;
+        const float dist = D > 0 ? filt_dist_fixed<DR>(pr, qv) : filt_dist_rt(pr, qrow, (int)d);
+        if (qvalid && dist <= thr && base + j != ql) {
+          const uint32_t pos = atomicAdd(&cnt[qi], 1u);
+          if (pos < cap) cand[qi * cap + pos] = (uint32_t)(o + base + j);
+        }
+      }
+    }
+  }
+}
+
+template __global__ void k_bexact_pass1<0, 16>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
+                                               uint32_t, uint32_t, uint32_t, const uint32_t*, const uint64_t*,
+                                               uint32_t*, float*);
+template __global__ void k_bexact_pass1<0, 32>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
+                                               uint32_t, uint32_t, uint32_t, const uint32_t*, const uint64_t*,
+                                               uint32_t*, float*);
+template __global__ void k_bexact_pass1<0, 64>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
+                                               uint32_t, uint32_t, uint32_t, const uint32_t*, const uint64_t*,
+                                               uint32_t*, float*);
+template __global__ void k_bexact_pass1<50, 16>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
+                                                uint32_t, uint32_t, uint32_t, const uint32_t*, const uint64_t*,
+                                                uint32_t*, float*);
+template __global__ void k_bexact_pass1<50, 32>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
+                                                uint32_t, uint32_t, uint32_t, const uint32_t*, const uint64_t*,
+                                                uint32_t*, float*);
+template __global__ void k_bexact_pass1<50, 64>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
+                                                uint32_t, uint32_t, uint32_t, const uint32_t*, const uint64_t*,
+                                                uint32_t*, float*);
+template __global__ void k_bexact_pass2<0>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
+                                           uint32_t, uint32_t, uint32_t, const float*, uint32_t*, uint32_t*,
+                                           uint32_t*, uint32_t);
+template __global__ void k_bexact_pass2<50>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
+                                            uint32_t, uint32_t, uint32_t, const float*, uint32_t*, uint32_t*,
+                                            uint32_t*, uint32_t);
 
 }  // namespace psq

@@ -455,7 +455,9 @@ __global__ void k_extract(const uint64_t* __restrict__ G, uint32_t n, uint32_t k
   if (key == EMPTY_KEY) return;
   uint32_t o = off[i] + s;
   indices[o] = key_idx(key);
-  dists[o] = __fsqrt_rn(key_dist(key));
+  // sqrt in FP64 then rounded to FP32: with 53 >= 2*24 + 2 bits the result is
+  // the correctly rounded f32 sqrt (gfx950's f32 sqrt is not), as Rust's f32::sqrt.
+  dists[o] = (float)__dsqrt_rn((double)key_dist(key));
 }
 
 // explicit instantiations used by the host code

@@ -112,6 +112,20 @@ __global__ void k_exact_rerank(const float* Xp, uint32_t n, uint32_t d, uint32_t
 __global__ void k_exact_fallback(const float* Xp, uint32_t n, uint32_t d, uint32_t dp,
                                  const uint32_t* queries, uint32_t k, const float* T,
                                  const uint32_t* overflow, uint32_t n_overflow, double* sd,
-                                 uint32_t* si, uint32_t* idx_out, float* dist_out);
+                                 uint32_t* si, uint32_t* idx_out, float* dist_out, const uint64_t* off,
+                                 uint32_t S);
+template <int D, int K>
+__global__ void k_bexact_pass1(const float* Xp, uint32_t d, uint32_t dp, const uint64_t* off,
+                               const uint32_t* units, uint32_t U, uint32_t seg_len, uint32_t tile,
+                               const uint32_t* nseg_of, const uint64_t* part_off, uint32_t* counter,
+                               float* partial);
+__global__ void k_bexact_tau(const float* partial, const uint64_t* off, uint32_t S,
+                             const uint32_t* nseg_of, const uint64_t* part_off, uint64_t nq, uint32_t K,
+                             uint32_t k, uint32_t d, float* T);
+template <int D>
+__global__ void k_bexact_pass2(const float* Xp, uint32_t d, uint32_t dp, const uint64_t* off,
+                               const uint32_t* units, uint32_t U, uint32_t seg_len, uint32_t tile,
+                               const float* T, uint32_t* counter, uint32_t* cnt, uint32_t* cand,
+                               uint32_t cap);
 
 }  // namespace psq

@@ -125,6 +125,42 @@ def knn_exact(data, k: int, queries=None, *, device: int = 0):
     return idx, dist
 
 
+def knn_exact_batched(data, offsets, k: int, *, device: int = 0):
+    """Exact kNN of every point within its own sample: samples are the row ranges
+    [offsets[s], offsets[s+1]). Returns (idx u32[n, k] global row indices,
+    dist f32[n, k]); one persistent launch per pass over all samples, tile shapes
+    from the autotuner."""
+    x = _as_f32_2d(data)
+    n, d = x.shape
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    idx = np.empty((n, k), np.uint32)
+    dist = np.empty((n, k), np.float32)
+    N.check(N.lib().psq_knn_exact_batched(x.ctypes.data, n, d, off.ctypes.data, off.size - 1,
+                                          int(k), idx.ctypes.data, dist.ctypes.data, int(device)))
+    return idx, dist
+
+
+def knn_exact_batched_device(x, offsets, k: int):
+    """Device form of knn_exact_batched (x: CUDA float32 (n, d); offsets: host array)."""
+    import torch
+    if not (x.is_cuda and x.dtype == torch.float32 and x.dim() == 2 and x.is_contiguous()):
+        raise ValueError("x must be a contiguous float32 CUDA tensor of shape (n, d)")
+    n, d = x.shape
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    idx = torch.empty((n, k), dtype=torch.int32, device=x.device)
+    dist = torch.empty((n, k), dtype=torch.float32, device=x.device)
+    N.check(N.lib().psq_knn_exact_batched_device(x.data_ptr(), n, d, off.ctypes.data, off.size - 1,
+                                                 int(k), idx.data_ptr(), dist.data_ptr(),
+                                                 _stream_ptr()))
+    return idx, dist
+
+
+def batched_report() -> dict:
+    r = N.BatchedReport()
+    N.check(N.lib().psq_batched_last_report(ctypes.byref(r)))
+    return r.as_dict()
+
+
 # ── device-resident forms (torch tensors on the GPU) ──
 
 def _stream_ptr():
