@@ -63,6 +63,9 @@ def main() -> int:
     ap.add_argument("--recall-queries", type=int, default=100_000)
     ap.add_argument("--cpu-cells", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 5 device-to-device copies of 1 GiB (the known-size kernel the "
+                         "HBM counters of a profile are scaled by)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -77,6 +80,14 @@ def main() -> int:
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_dispatch": a.numel() * 4, "dispatches": 5}), flush=True)
+        return 0
     n, k = args.n, args.k
     log(f"generating {n} x 50 mixture ({args.components} components, seed {args.seed})")
     x = ds.mixture(n, 50, args.components, seed=args.seed)
