@@ -18,10 +18,14 @@
 //    (nndescent.rs:244-249). Priorities are the same hash_unit numerators
 //    (nndescent.rs:872), keyed by the row's sorted slot.
 //  * Join proposals for a row are routed by a hash of the neighbour to one of
-//    S/8 buckets of 8 slots. A bucket keeps the 8 smallest distinct keys routed
-//    to it in one iteration, which is order-independent and so deterministic.
-//    A key of the final top-k is lost only when 8 smaller keys share its bucket.
-//    The reference buffers every proposal (nndescent.rs:925-968).
+//    S/8 buckets of 8 slots. The row's current neighbours occupy ("pin") the
+//    slots of their buckets (at most 8 per bucket), and a proposal of a
+//    neighbour the row already holds is dropped. A bucket keeps the smallest
+//    distinct proposal keys routed to it in one iteration, up to its free
+//    slots, which is order-independent and so deterministic. A key of the
+//    final top-k can be lost only when 8 smaller keys or neighbours share its
+//    bucket. The reference buffers every proposal (nndescent.rs:925-968) and
+//    drops duplicates when it pushes them (nndescent.rs:176-181).
 //  * RP trees are built level by level. A node's hyperplane passes through two
 //    members picked by hash from its segment; margins use the reference's
 //    8-lane dot and the same offset fold (nndescent.rs:367-385, 439). A
@@ -426,15 +430,27 @@ int64_t psq_o_gpu_model_nndescent(const float* data, uint64_t n_obs, uint64_t n_
     pool.parallel_for(n, 256, [&](size_t p) {
       if (poff[p] == poff[p + 1]) return;
       std::vector<std::vector<uint64_t>> bk(n_buckets);
+      std::vector<size_t> pinned(n_buckets, 0);
+      const uint64_t* rowp = &G[p * k];
+      auto in_row = [&](uint32_t q) {
+        for (size_t s = 0; s < k; ++s)
+          if (rowp[s] != EMPTY_KEY && key_idx(rowp[s]) == q) return true;
+        return false;
+      };
+      for (size_t s = 0; s < k; ++s)
+        if (rowp[s] != EMPTY_KEY) pinned[slot_hash(seed, iter, key_idx(rowp[s])) & (n_buckets - 1)]++;
       for (uint32_t r = poff[p]; r < poff[p + 1]; ++r) {
         uint64_t key = pk[r];
+        if (in_row(key_idx(key))) continue;
         bk[slot_hash(seed, iter, key_idx(key)) & (n_buckets - 1)].push_back(key);
       }
       std::vector<uint64_t> pr;
-      for (auto& b : bk) {
+      for (size_t bi = 0; bi < n_buckets; ++bi) {
+        auto& b = bk[bi];
+        const size_t cap = BUCKET - std::min(BUCKET, pinned[bi]);
         std::sort(b.begin(), b.end());
         b.erase(std::unique(b.begin(), b.end()), b.end());
-        for (size_t t = 0; t < b.size() && t < BUCKET; ++t) pr.push_back(b[t]);
+        for (size_t t = 0; t < b.size() && t < cap; ++t) pr.push_back(b[t]);
       }
       ch[p] = merge_row(&G[p * k], k, pr);
     });

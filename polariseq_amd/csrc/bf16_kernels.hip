@@ -5,7 +5,8 @@
 // nndescent.rs:912-969), with two changes:
 //  * join: per cell i the candidate rows are centred on x_i in FP32 (all of
 //    them are near x_i, so the centred vectors are short), rounded to BF16 and
-//    staged in LDS. One v_mfma_f32_16x16x32_bf16 pass gives the Gram tile of
+//    staged in LDS with coalesced loads. Proposals are batched through the
+//    wave's LDS list (see pb_append / pb_flush). One v_mfma_f32_16x16x32_bf16 pass gives the Gram tile of
 //    new rows x all candidates; d~(p,q) = |p~|^2 + |q~|^2 - 2 p~.q~, with the
 //    norms summed in FP32 from the same BF16 values. Proposals carry d~.
 //  * merge: d~ differs between cells, so proposals are deduplicated on the
@@ -20,67 +21,69 @@ namespace psq {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int BF_MAXC = 128;  // candidate slots per cell (2m, m <= 64)
-constexpr int BF_KD = 64;     // padded dimensions (d <= 64)
+constexpr int BF_KD = 64;  // padded dimensions (d <= 64)
+constexpr int BF_RS = 72;  // LDS row stride in bf16: 144 bytes, so 16 rows hit distinct banks
 
-// One wave per cell; 4 waves per block. LDS per wave: BF_MAXC x 64 bf16 rows
-// (16 KB) + per-slot candidate id, threshold and norm.
-__global__ __launch_bounds__(256) void k_join_bf16(
+// One wave (one block) per cell. LDS: pb_key[PB_CAP] u64 | rows[R][BF_RS] bf16
+// | ids[R] | thr[R] | nrm[R] | pb_row[PB_CAP], R = 2m rounded up to 16.
+__global__ __launch_bounds__(64) void k_join_bf16(
     const float* __restrict__ Xp, uint32_t n, uint32_t d, uint32_t dp,
     const uint32_t* __restrict__ newl, const uint32_t* __restrict__ oldl,
     const uint32_t* __restrict__ ncnt, uint32_t m, const float* __restrict__ thr, float slack,
     uint64_t* __restrict__ slots, uint8_t* __restrict__ touched, uint32_t S, uint64_t seed,
     uint64_t iter, unsigned long long* __restrict__ counters) {
-  __shared__ __attribute__((aligned(16))) __bf16 s_v[4][BF_MAXC][BF_KD];
-  __shared__ uint32_t s_id[4][BF_MAXC];
-  __shared__ float s_thr[4][BF_MAXC];
-  __shared__ float s_nrm[4][BF_MAXC];
-  const int w = threadIdx.x / WAVE, l = lane_id();
-  const uint32_t i = blockIdx.x * 4 + w;
-  const uint32_t cnt = i < n ? ncnt[i] : 0u;
+  extern __shared__ __attribute__((aligned(16))) uint64_t lds64[];
+  const uint32_t R = (2 * m + 15) / 16 * 16;
+  uint64_t* pb_key = lds64;
+  __bf16* s_v = reinterpret_cast<__bf16*>(lds64 + PB_CAP);
+  uint32_t* s_id = reinterpret_cast<uint32_t*>(s_v + (size_t)R * BF_RS);
+  float* s_thr = reinterpret_cast<float*>(s_id + R);
+  float* s_nrm = s_thr + R;
+  uint32_t* pb_row = reinterpret_cast<uint32_t*>(s_nrm + R);
+  const uint32_t l = (uint32_t)lane_id();
+  const uint32_t i = blockIdx.x;
+  if (i >= n) return;  // block-uniform
+  const uint32_t cnt = ncnt[i];
   const uint32_t nn = cnt & 0xFFFF, no = cnt >> 16, nc = nn + no;
+  if (nn == 0) return;
   const uint32_t bmask = S / BUCKET - 1;
-  // stage centred BF16 rows (lane per slot)
-  const float* xi = Xp + (size_t)(i < n ? i : 0) * dp;
-  for (uint32_t s = l; s < BF_MAXC; s += WAVE) {
-    uint32_t v = EMPTY;
-    if (s < nn)
-      v = newl[(size_t)i * m + s];
-    else if (s < nc)
-      v = oldl[(size_t)i * m + (s - nn)];
-    s_id[w][s] = v;
-    float nrm = 0.0f;
-    if (v != EMPTY) {
-      const float* xv = Xp + (size_t)v * dp;
-      for (uint32_t t = 0; t < BF_KD; t += 8) {
-        bf16x8 pk;
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const uint32_t tt = t + u;
-          const float c = tt < d ? xv[tt] - xi[tt] : 0.0f;
-          const __bf16 b = (__bf16)c;
-          pk[u] = b;
-          const float bf = (float)b;
-          nrm = fmaf(bf, bf, nrm);
-        }
-        *reinterpret_cast<bf16x8*>(&s_v[w][s][t]) = pk;
-      }
-      s_thr[w][s] = thr[v] * slack;
-    } else {
-      for (uint32_t t = 0; t < BF_KD; t += 8) {
-        bf16x8 z;
-#pragma unroll
-        for (int u = 0; u < 8; ++u) z[u] = (__bf16)0.0f;
-        *reinterpret_cast<bf16x8*>(&s_v[w][s][t]) = z;
-      }
-      s_thr[w][s] = 0.0f;
-    }
-    s_nrm[w][s] = nrm;
+  for (uint32_t s = l; s < nc; s += WAVE) {
+    const uint32_t v = s < nn ? newl[(size_t)i * m + s] : oldl[(size_t)i * m + (s - nn)];
+    s_id[s] = v;
+    s_thr[s] = thr[v] * slack;
   }
   __syncthreads();
+  // centred rows, rounded to BF16: 16 chunks of 4 dimensions per row
+  const float* xi = Xp + (size_t)i * dp;
+  for (uint32_t e = l; e < nc * (BF_KD / 4); e += WAVE) {
+    const uint32_t s = e / (BF_KD / 4), c = e % (BF_KD / 4);
+    float4 dv = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (4 * c < dp) {
+      const float4 xv = *reinterpret_cast<const float4*>(Xp + (size_t)s_id[s] * dp + 4 * c);
+      const float4 x0 = *reinterpret_cast<const float4*>(xi + 4 * c);
+      dv = make_float4(xv.x - x0.x, xv.y - x0.y, xv.z - x0.z, xv.w - x0.w);
+    }
+    __bf16* dst = s_v + (size_t)s * BF_RS + 4 * c;
+    dst[0] = (__bf16)dv.x;
+    dst[1] = (__bf16)dv.y;
+    dst[2] = (__bf16)dv.z;
+    dst[3] = (__bf16)dv.w;
+  }
+  __syncthreads();
+  for (uint32_t s = l; s < nc; s += WAVE) {
+    float nrm = 0.0f;
+    const __bf16* row = s_v + (size_t)s * BF_RS;
+    for (int t = 0; t < BF_KD; ++t) {
+      const float bf = (float)row[t];
+      nrm = fmaf(bf, bf, nrm);
+    }
+    s_nrm[s] = nrm;
+  }
+  __syncthreads();
+  uint32_t pcnt = 0;
   uint64_t pairs = 0;
   const uint32_t rtiles = (nn + 15) / 16, ctiles = (nc + 15) / 16;
-  const int lr = l % 16, lk = (l / 16) * 8;
+  const uint32_t lr = l % 16, lk = (l / 16) * 8;
   for (uint32_t rt = 0; rt < rtiles; ++rt) {
     for (uint32_t ct = 0; ct < ctiles; ++ct) {
       // skip tiles entirely below the new x new diagonal
@@ -88,32 +91,41 @@ __global__ __launch_bounds__(256) void k_join_bf16(
       f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
       for (int kk = 0; kk < BF_KD; kk += 32) {
-        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&s_v[w][rt * 16 + lr][kk + lk]);
-        const bf16x8 b = *reinterpret_cast<const bf16x8*>(&s_v[w][ct * 16 + lr][kk + lk]);
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(s_v + (size_t)(rt * 16 + lr) * BF_RS + kk + lk);
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(s_v + (size_t)(ct * 16 + lr) * BF_RS + kk + lk);
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
       }
       const uint32_t c = ct * 16 + lr;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const uint32_t r = rt * 16 + (l / 16) * 4 + v;
-        if (r >= nn || c >= nc) continue;
-        const uint32_t p = s_id[w][r], q = s_id[w][c];
-        const bool valid = (c < nn) ? (c > r) : (q != p);
-        if (!valid) continue;
-        ++pairs;
-        float dist = s_nrm[w][r] + s_nrm[w][c] - 2.0f * acc[v];
-        dist = dist > 0.0f ? dist : 0.0f;
-        if (dist < s_thr[w][r]) propose(slots, touched, p, make_key(dist, q, 1), S, bmask, seed, iter);
-        if (dist < s_thr[w][c]) propose(slots, touched, q, make_key(dist, p, 1), S, bmask, seed, iter);
+        bool pa = false, pb = false;
+        uint32_t p = 0, q = 0;
+        float dist = 0.0f;
+        if (r < nn && c < nc) {
+          p = s_id[r];
+          q = s_id[c];
+          if ((c < nn) ? (c > r) : (q != p)) {
+            ++pairs;
+            dist = s_nrm[r] + s_nrm[c] - 2.0f * acc[v];
+            dist = dist > 0.0f ? dist : 0.0f;
+            pa = dist < s_thr[r];
+            pb = dist < s_thr[c];
+          }
+        }
+        pb_append(pb_row, pb_key, pcnt, pa, p, make_key(dist, q, 1));
+        pb_append(pb_row, pb_key, pcnt, pb, q, make_key(dist, p, 1));
+        pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, bmask, seed, iter);
       }
     }
   }
+  pb_flush(pb_row, pb_key, pcnt, true, slots, touched, S, bmask, seed, iter);
   uint64_t tot = pairs;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) tot += shfl_xor64(tot, o);
-  if (l == 0 && nn > 0) {
-    atomicAdd(&counters[0], (unsigned long long)tot);
-    atomicAdd(&counters[1], (unsigned long long)nc);
+  if (l == 0) {
+    ctr_add(counters, 0, (unsigned long long)tot);
+    ctr_add(counters, 1, (unsigned long long)nc);
   }
 }
 
@@ -138,8 +150,7 @@ __global__ __launch_bounds__(256) void k_merge_rerank(
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
     const uint32_t e = r * WAVE + l;
-    v[r] = e < S ? sl[e] : EMPTY_KEY;
-    if (e < S) sl[e] = EMPTY_KEY;
+    v[r] = e < S ? bucket_to_row(sl[e]) : EMPTY_KEY;
   }
   if (l == 0) touched[p] = 0;
   // 1. neighbours already in the row
@@ -210,7 +221,7 @@ __global__ __launch_bounds__(256) void k_merge_rerank(
   uint32_t tot = nchanged;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, WAVE);
-  if (l == 0 && tot) atomicAdd(&changed[0], (unsigned long long)tot);
+  if (l == 0 && tot) ctr_add(changed, 2, (unsigned long long)tot);
 }
 
 template __global__ void k_merge_rerank<1>(const float*, uint32_t, uint32_t, uint64_t*, uint32_t,

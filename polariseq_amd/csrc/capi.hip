@@ -249,24 +249,24 @@ int launch_merge(const Resolved& R, uint64_t* G, uint64_t* slots, uint8_t* touch
   return PSQ_OK;
 }
 
+// LDS row stride of the join kernels: d rounded up to 4 floats, with an odd
+// number of 16-byte units so that consecutive rows start on different banks.
+uint32_t join_row_stride(uint64_t d) { return (uint32_t)(round_up((uint32_t)d, 4) | 4u); }
+
 int launch_join(const Resolved& R, const float* Xp, const uint32_t* newl, const uint32_t* oldl,
                 const uint32_t* ncnt, const float* thr, uint64_t* slots, uint8_t* touched,
                 uint64_t iter, unsigned long long* counters, hipStream_t s) {
-  const unsigned grid = grid_for(R.n, 4);
-  const size_t smem = (size_t)4 * R.m * R.dp * sizeof(float);
-  const int cpl = (2 * R.m > 64) ? 2 : 1;
-  const bool d50 = (R.d == 50);
-  void (*kern)(const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, const uint32_t*,
-               const uint32_t*, uint32_t, const float*, uint64_t*, uint8_t*, uint32_t, uint64_t,
-               uint64_t, unsigned long long*);
-  if (d50)
-    kern = cpl == 1 ? k_join<50, 1> : k_join<50, 2>;
-  else
-    kern = cpl == 1 ? k_join<0, 1> : k_join<0, 2>;
-  if (smem > 65536) HIPC(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(256), smem, s, Xp, (uint32_t)R.n, (uint32_t)R.d, R.dp,
-                     newl, oldl, ncnt, (uint32_t)R.m, thr, slots, touched, (uint32_t)R.S, R.seed,
-                     iter, counters);
+  const uint32_t RS = join_row_stride(R.d);
+  const size_t smem = (size_t)PB_CAP * 12 + (size_t)2 * R.m * (RS * 4 + 8);
+  void (*kern)(const float*, uint32_t, uint32_t, uint32_t, uint32_t, const uint32_t*,
+               const uint32_t*, const uint32_t*, uint32_t, const float*, uint64_t*, uint8_t*,
+               uint32_t, uint64_t, uint64_t, unsigned long long*) =
+      R.d == 50 ? k_join<50> : k_join<0>;
+  if (smem > 65536)
+    HIPC(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+  hipLaunchKernelGGL(kern, dim3((unsigned)R.n), dim3(64), smem, s, Xp, (uint32_t)R.n, (uint32_t)R.d,
+                     R.dp, RS, newl, oldl, ncnt, (uint32_t)R.m, thr, slots, touched, (uint32_t)R.S,
+                     R.seed, iter, counters);
   HIPC(hipGetLastError());
   return PSQ_OK;
 }
@@ -495,7 +495,7 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   TRY(wbuf(ws, "oldl", (size_t)n * m, &oldl));
   TRY(wbuf(ws, "ncnt", n, &ncnt));
   TRY(wbuf(ws, "rowcnt", n, &rowcnt));
-  TRY(wbuf(ws, "counters", 8, &ctr));
+  TRY(wbuf(ws, "counters", (size_t)(CTR_STRIPES + 1) * CTR_WORDS, &ctr));
   void* hp;
   TRY(ws->host_counters(&hp));
   unsigned long long* hctr = static_cast<unsigned long long*>(hp);
@@ -515,7 +515,7 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   TRY(fill_u64(slots, (size_t)n * S, EMPTY_KEY, s));
   TRY(fill_u8(touched, n, 0, s));
   TRY(fill_u32(rcount, n, 0, s));
-  TRY(fill_u64(reinterpret_cast<uint64_t*>(ctr), 8, 0, s));
+  TRY(fill_u64(reinterpret_cast<uint64_t*>(ctr), (size_t)(CTR_STRIPES + 1) * CTR_WORDS, 0, s));
   {
     float inf = __builtin_inff();
     uint32_t bits;
@@ -535,7 +535,7 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   hipLaunchKernelGGL(k_random_fill, dim3(grid_for(n, 256)), dim3(256), 0, s, Xp, n,
                      (uint32_t)R.d, R.dp, G, k, R.seed, touched);
   HIPC(hipGetLastError());
-  TRY(launch_merge(R, G, slots, touched, thr, 1, ctr + 2, s));
+  TRY(launch_merge(R, G, slots, touched, thr, 1, ctr, s));
   HIPC(hipEventRecord(e_init, s));
 
   // ── iterations ──
@@ -545,7 +545,7 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   std::vector<hipEvent_t> jb, je;
   uint64_t iters = 0;
   for (uint64_t it = 0; it < R.n_iters; ++it) {
-    TRY(fill_u64(reinterpret_cast<uint64_t*>(ctr), 4, 0, s));
+    TRY(fill_u64(reinterpret_cast<uint64_t*>(ctr), (size_t)CTR_STRIPES * CTR_WORDS, 0, s));
     hipLaunchKernelGGL(k_rev_count, dim3(grid_for((size_t)n * k, 256)), dim3(256), 0, s, G,
                        (size_t)n * k, rcount);
     HIPC(hipGetLastError());
@@ -554,14 +554,16 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
                        R.seed, it, roff, rcount, rev);
     HIPC(hipGetLastError());
     hipLaunchKernelGGL(k_sample_select, dim3(grid_for(n, 4)), dim3(256), 0, s, G, n, k, m, R.seed,
-                       it, roff, rev, newl, oldl, ncnt);
+                       it, roff, rev, newl, oldl, ncnt, slots, S);
     HIPC(hipGetLastError());
     hipEvent_t a, b;
     TRY(evp.make(&a));
     TRY(evp.make(&b));
     HIPC(hipEventRecord(a, s));
     if (bf16) {
-      hipLaunchKernelGGL(k_join_bf16, dim3(grid_for(n, 4)), dim3(256), 0, s, Xp, n, (uint32_t)R.d,
+      const uint32_t Rr = (2 * m + 15) / 16 * 16;
+      const size_t smem = (size_t)PB_CAP * 12 + (size_t)Rr * (72 * 2 + 12);
+      hipLaunchKernelGGL(k_join_bf16, dim3(n), dim3(64), smem, s, Xp, n, (uint32_t)R.d,
                          R.dp, newl, oldl, ncnt, m, thr, BF16_SLACK, slots, touched, S, R.seed, it,
                          ctr);
       HIPC(hipGetLastError());
@@ -575,18 +577,21 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
       const unsigned g4 = grid_for(n, 4);
       if (S <= 64)
         hipLaunchKernelGGL(k_merge_rerank<1>, dim3(g4), dim3(256), 0, s, Xp, (uint32_t)R.d, R.dp, G,
-                           n, k, slots, S, touched, thr, ctr + 2);
+                           n, k, slots, S, touched, thr, ctr);
       else if (S <= 128)
         hipLaunchKernelGGL(k_merge_rerank<2>, dim3(g4), dim3(256), 0, s, Xp, (uint32_t)R.d, R.dp, G,
-                           n, k, slots, S, touched, thr, ctr + 2);
+                           n, k, slots, S, touched, thr, ctr);
       else
         hipLaunchKernelGGL(k_merge_rerank<4>, dim3(g4), dim3(256), 0, s, Xp, (uint32_t)R.d, R.dp, G,
-                           n, k, slots, S, touched, thr, ctr + 2);
+                           n, k, slots, S, touched, thr, ctr);
       HIPC(hipGetLastError());
     } else {
-      TRY(launch_merge(R, G, slots, touched, thr, 0, ctr + 2, s));
+      TRY(launch_merge(R, G, slots, touched, thr, 0, ctr, s));
     }
-    HIPC(hipMemcpyAsync(hctr, ctr, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    unsigned long long* csum = ctr + (size_t)CTR_STRIPES * CTR_WORDS;
+    hipLaunchKernelGGL(k_ctr_reduce, dim3(1), dim3(64), 0, s, ctr, csum);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(hctr, csum, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
     const uint64_t changed = hctr[2];
     if (it < 64) {
@@ -617,7 +622,7 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   st.k = R.k;
   st.max_candidates = R.m;
   st.max_iterations = R.n_iters;
-  st.n_trees = opts->init == PSQ_INIT_RP_FOREST ? R.n_trees : 0;
+  st.n_trees = opts->init == PSQ_INIT_RANDOM ? 0 : R.n_trees;
   st.leaf_size = R.leaf_size;
   st.slots = R.S;
   st.stop_at = R.stop_at;

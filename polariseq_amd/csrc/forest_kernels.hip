@@ -214,19 +214,47 @@ __global__ void k_forest_scatter(size_t total, uint32_t n, const uint32_t* __res
 }
 
 // Leaves grouped by tree (order inside a tree is irrelevant: its leaves are
-// disjoint and each merge is a set function).
+// disjoint and each merge is a set function). Leaves are emitted level by level
+// in node order, and nodes stay ordered by tree, so a wave's leaves form a few
+// runs of one tree: one atomic per run, not per leaf.
+__device__ inline void leaf_runs(const uint32_t* __restrict__ l_tree, uint32_t nl, uint32_t i,
+                                 uint32_t& t, bool& head, uint32_t& run_len, int& head_lane) {
+  const int l = lane_id();
+  const bool valid = i < nl;
+  t = valid ? l_tree[i] : NONE;
+  const uint32_t prev = (uint32_t)__shfl((int)t, (l + WAVE - 1) & (WAVE - 1), WAVE);
+  head = valid && (l == 0 || prev != t);
+  const uint64_t hm = __ballot(head);
+  const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));  // valid lanes are a prefix
+  const uint64_t above = hm & ~((2ull << l) - 1ull);            // heads after lane l
+  const uint32_t end = above ? (uint32_t)__builtin_ctzll(above) : nvalid;
+  run_len = head ? end - (uint32_t)l : 0u;
+  const uint64_t upto = hm & ((2ull << l) - 1ull);               // heads at or before lane l
+  head_lane = upto ? 63 - __builtin_clzll(upto) : 0;
+}
 __global__ void k_leaf_hist(const uint32_t* __restrict__ l_tree, uint32_t nl,
                             uint32_t* __restrict__ tcount) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nl) atomicAdd(&tcount[l_tree[i]], 1u);
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t t, run_len;
+  bool head;
+  int hl;
+  leaf_runs(l_tree, nl, i, t, head, run_len, hl);
+  if (head) atomicAdd(&tcount[t], run_len);
 }
 __global__ void k_leaf_group(const uint32_t* __restrict__ l_tree, const uint32_t* __restrict__ l_start,
                              const uint32_t* __restrict__ l_count, uint32_t nl,
                              uint32_t* __restrict__ tcursor, uint32_t* __restrict__ g_start,
                              uint32_t* __restrict__ g_count) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t t, run_len;
+  bool head;
+  int hl;
+  leaf_runs(l_tree, nl, i, t, head, run_len, hl);
+  uint32_t base = 0;
+  if (head) base = atomicAdd(&tcursor[t], run_len);
+  base = (uint32_t)__shfl((int)base, hl, WAVE);
   if (i >= nl) return;
-  uint32_t pos = atomicAdd(&tcursor[l_tree[i]], 1u);
+  const uint32_t pos = base + ((uint32_t)lane_id() - (uint32_t)hl);
   g_start[pos] = l_start[i];
   g_count[pos] = l_count[i];
 }

@@ -28,6 +28,15 @@ __global__ void k_fill_u32(uint32_t* p, size_t count, uint32_t v) {
   size_t stride = (size_t)gridDim.x * blockDim.x;
   for (; i < count; i += stride) p[i] = v;
 }
+// Sums the counter stripes: out[w] = sum over stripes of ctr[stripe][w].
+__global__ void k_ctr_reduce(const unsigned long long* __restrict__ ctr,
+                             unsigned long long* __restrict__ out) {
+  const uint32_t w = threadIdx.x;
+  if (w >= CTR_WORDS) return;
+  unsigned long long t = 0;
+  for (uint32_t st = 0; st < CTR_STRIPES; ++st) t += ctr[(size_t)st * CTR_WORDS + w];
+  out[w] = t;
+}
 __global__ void k_fill_u8(uint8_t* p, size_t count, uint8_t v) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -180,7 +189,8 @@ __global__ void k_rev_scatter(const uint64_t* __restrict__ G, uint32_t n, uint32
 __global__ __launch_bounds__(256) void k_sample_select(
     uint64_t* __restrict__ G, uint32_t n, uint32_t k, uint32_t m, uint64_t seed, uint64_t iter,
     const uint32_t* __restrict__ roff, const uint64_t* __restrict__ rev,
-    uint32_t* __restrict__ newl, uint32_t* __restrict__ oldl, uint32_t* __restrict__ ncnt) {
+    uint32_t* __restrict__ newl, uint32_t* __restrict__ oldl, uint32_t* __restrict__ ncnt,
+    uint64_t* __restrict__ slots, uint32_t S) {
   __shared__ uint64_t s_fk[4][WAVE];
   __shared__ uint32_t s_kill[4][WAVE];
   const int w = threadIdx.x / WAVE;
@@ -264,114 +274,146 @@ __global__ __launch_bounds__(256) void k_sample_select(
     }
     if (l == 0) ncnt[i] = nn | (no << 16);
     if (fvalid && (fkey & 1) && found) G[(size_t)i * k + l] = fkey & ~1ull;
+    // Bucket layout for this iteration's join: the row's neighbours pinned,
+    // ascending by index inside each bucket, at most BUCKET per bucket; the
+    // other slots empty. Every row is rewritten, so no bucket state carries over.
+    const uint32_t bmask = S / BUCKET - 1;
+    const uint32_t mb = fvalid ? (slot_hash(seed, iter, me) & bmask) : EMPTY;
+    uint32_t rank = 0;
+    for (uint32_t s = 0; s < k; ++s) {
+      const uint32_t bs = __shfl(mb, (int)s, WAVE), is = __shfl(me, (int)s, WAVE);
+      rank += (fvalid && bs == mb && is < me) ? 1u : 0u;
+    }
+    for (uint32_t e0 = 0; e0 < S; e0 += WAVE) {
+      const uint32_t e = e0 + l;
+      const uint32_t eb = e / BUCKET, ej = e % BUCKET;
+      uint64_t val = EMPTY_KEY;
+      for (uint32_t s = 0; s < k; ++s) {
+        const uint32_t bs = __shfl(mb, (int)s, WAVE), rs = __shfl(rank, (int)s, WAVE);
+        const uint32_t is = __shfl(me, (int)s, WAVE);
+        if (bs == eb && rs == ej) val = pin_key(is);
+      }
+      if (e < S) slots[(size_t)i * S + e] = val;
+    }
   }
 }
 
 // ── local join (nndescent.rs:912-969) ──
-// One wave per cell i. Lane t holds candidate slot t (and t + 64 when CPL = 2):
-// slots [0, nn) are the new list, [nn, nn + no) the old list. The new rows are
-// staged in LDS; for each new entry a (a wave-uniform loop), every lane computes
-// sq_dist(new[a], its candidate) with the new row read by LDS broadcast and its
-// own row in registers. Pairs: new×new with slot > a, and new×old with the
-// candidate != new[a]. A pair proposes each endpoint to the other when its
-// distance beats that endpoint's threshold (snapshot of the row's worst).
+// One wave (one block) per cell i. The candidate rows (new list, then old
+// list) are staged in LDS with coalesced 16-byte loads, at a row stride RS
+// (floats, RS / 4 odd) that spreads rows over the LDS banks. The cell's pairs,
+// new x new (a < b) then new x old with different points, are numbered and
+// dealt to the 64 lanes, so every lane computes a distance per round. A pair
+// whose distance beats an endpoint's threshold (snapshot of that row's worst)
+// proposes the other endpoint to it; proposals are batched through the wave's
+// LDS list. D = 50 fixes the dimension at compile time; D = 0 reads it at run
+// time. Both use the reference's operation order (simd.rs:45-62).
 template <int D>
-__device__ inline void load_row(const float* __restrict__ src, float (&q)[D]) {
-  // rows are 128-byte aligned (padded stride)
-  const float4* s4 = reinterpret_cast<const float4*>(src);
+__device__ inline float lds_sq_dist(const float* pa, const float* pb, uint32_t d) {
+  if (D > 0) {
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int t = 0; t < D / 4; ++t) {
-    float4 v = s4[t];
-    q[4 * t] = v.x;
-    q[4 * t + 1] = v.y;
-    q[4 * t + 2] = v.z;
-    q[4 * t + 3] = v.w;
+    for (int ch = 0; ch < D / 8; ++ch) {
+      const float4 a0 = *reinterpret_cast<const float4*>(pa + ch * 8);
+      const float4 a1 = *reinterpret_cast<const float4*>(pa + ch * 8 + 4);
+      const float4 b0 = *reinterpret_cast<const float4*>(pb + ch * 8);
+      const float4 b1 = *reinterpret_cast<const float4*>(pb + ch * 8 + 4);
+      float t;
+      t = a0.x - b0.x; acc[0] = acc[0] + t * t;
+      t = a0.y - b0.y; acc[1] = acc[1] + t * t;
+      t = a0.z - b0.z; acc[2] = acc[2] + t * t;
+      t = a0.w - b0.w; acc[3] = acc[3] + t * t;
+      t = a1.x - b1.x; acc[4] = acc[4] + t * t;
+      t = a1.y - b1.y; acc[5] = acc[5] + t * t;
+      t = a1.z - b1.z; acc[6] = acc[6] + t * t;
+      t = a1.w - b1.w; acc[7] = acc[7] + t * t;
+    }
+    float tail = 0.f;
+#pragma unroll
+    for (int j = (D / 8) * 8; j < D; ++j) {
+      const float t = pa[j] - pb[j];
+      tail = tail + t * t;
+    }
+    return (((acc[0] + acc[4]) + (acc[1] + acc[5])) + ((acc[2] + acc[6]) + (acc[3] + acc[7]))) +
+           tail;
   }
-#pragma unroll
-  for (int t = (D / 4) * 4; t < D; ++t) q[t] = src[t];
+  return sq_dist_ref(pa, pb, (int)d);
 }
 
-template <int D, int CPL>
-__global__ __launch_bounds__(256) void k_join(
-    const float* __restrict__ Xp, uint32_t n, uint32_t d, uint32_t dp,
+template <int D>
+__global__ __launch_bounds__(64) void k_join(
+    const float* __restrict__ Xp, uint32_t n, uint32_t d, uint32_t dp, uint32_t RS,
     const uint32_t* __restrict__ newl, const uint32_t* __restrict__ oldl,
     const uint32_t* __restrict__ ncnt, uint32_t m, const float* __restrict__ thr,
     uint64_t* __restrict__ slots, uint8_t* __restrict__ touched, uint32_t S, uint64_t seed,
     uint64_t iter, unsigned long long* __restrict__ counters) {
-  extern __shared__ float s_rows[];  // [4 waves][m][dp]
-  const int w = threadIdx.x / WAVE;
-  const int l = lane_id();
-  const uint32_t i = blockIdx.x * 4 + w;
-  const uint32_t cnt = i < n ? ncnt[i] : 0u;  // wave-uniform
-  const uint32_t nn = cnt & 0xFFFF, no = cnt >> 16;
-  float* prow = s_rows + (size_t)w * m * dp;
+  // LDS: pb_key[PB_CAP] u64 | rows[2m][RS] f32 | ids[2m] | thr[2m] | pb_row[PB_CAP]
+  extern __shared__ __attribute__((aligned(16))) float lds[];
+  uint64_t* pb_key = reinterpret_cast<uint64_t*>(lds);
+  float* rows = lds + 2 * PB_CAP;
+  uint32_t* s_id = reinterpret_cast<uint32_t*>(rows + (size_t)2 * m * RS);
+  float* s_thr = reinterpret_cast<float*>(s_id + 2 * m);
+  uint32_t* pb_row = reinterpret_cast<uint32_t*>(s_thr + 2 * m);
+  const uint32_t l = (uint32_t)lane_id();
+  const uint32_t i = blockIdx.x;
+  if (i >= n) return;  // block-uniform
+  const uint32_t cnt = ncnt[i];
+  const uint32_t nn = cnt & 0xFFFF, no = cnt >> 16, nc = nn + no;
+  if (nn == 0) return;
   const uint32_t bmask = S / BUCKET - 1;
-
-  uint32_t cand[CPL];
-  float cthr[CPL];
-  constexpr int DR = D > 0 ? D : 1;
-  float q[CPL][DR];
-#pragma unroll
-  for (int c = 0; c < CPL; ++c) {
-    const uint32_t s = l + c * WAVE;
-    uint32_t v = EMPTY;
-    if (s < nn)
-      v = newl[(size_t)i * m + s];
-    else if (s < nn + no)
-      v = oldl[(size_t)i * m + (s - nn)];
-    cand[c] = v;
-    cthr[c] = v != EMPTY ? thr[v] : 0.0f;
-    if (D > 0) {
-      if (v != EMPTY) {
-        load_row<DR>(Xp + (size_t)v * dp, q[c]);
-      } else {
-#pragma unroll
-        for (int t = 0; t < DR; ++t) q[c][t] = 0.0f;
-      }
-    }
-  }
-  // stage the new rows (slots < nn live in lanes < nn of group 0, nn <= m <= 64)
-  for (uint32_t e = l; e < nn * dp; e += WAVE) {
-    uint32_t a = e / dp, t = e - a * dp;
-    uint32_t v = newl[(size_t)i * m + a];
-    prow[e] = Xp[(size_t)v * dp + t];
+  for (uint32_t s = l; s < nc; s += WAVE) {
+    const uint32_t v = s < nn ? newl[(size_t)i * m + s] : oldl[(size_t)i * m + (s - nn)];
+    s_id[s] = v;
+    s_thr[s] = thr[v];
   }
   __syncthreads();
-  uint64_t pairs = 0;
-  for (uint32_t a = 0; a < nn; ++a) {
-    const uint32_t p = (uint32_t)__shfl((int)cand[0], (int)a, WAVE);
-    const float thr_p = __shfl(cthr[0], (int)a, WAVE);
-    const float* pr = prow + (size_t)a * dp;
-#pragma unroll
-    for (int c = 0; c < CPL; ++c) {
-      const uint32_t s = l + c * WAVE;
-      const bool valid = (s < nn) ? (s > a) : (s < nn + no && cand[c] != p);
-      if (__ballot(valid) == 0) continue;
-      if (!valid) continue;
-      ++pairs;
-      float dist;
-      if (D > 0)
-        dist = sq_dist_fixed<DR>(pr, q[c]);
-      else
-        dist = sq_dist_ref(pr, Xp + (size_t)cand[c] * dp, (int)d);
-      if (dist < thr_p) propose(slots, touched, p, make_key(dist, cand[c], 1), S, bmask, seed, iter);
-      if (dist < cthr[c]) propose(slots, touched, cand[c], make_key(dist, p, 1), S, bmask, seed, iter);
-    }
+  const uint32_t c4 = D > 0 ? (uint32_t)((D + 3) / 4) : (d + 3) / 4;
+  for (uint32_t e = l; e < nc * c4; e += WAVE) {
+    const uint32_t s = e / c4, c = e - s * c4;
+    const float4 v = *reinterpret_cast<const float4*>(Xp + (size_t)s_id[s] * dp + 4 * c);
+    *reinterpret_cast<float4*>(rows + (size_t)s * RS + 4 * c) = v;
   }
+  __syncthreads();
+  const uint32_t P1 = nn * (nn - 1) / 2, P = P1 + nn * no;
+  uint32_t pcnt = 0;
+  uint64_t pairs = 0;
+  for (uint32_t base = 0; base < P; base += WAVE) {
+    const uint32_t t = base + l;
+    bool pa = false, pb = false;
+    uint32_t ia = 0, ib = 0;
+    float dist = 0.0f;
+    if (t < P) {
+      uint32_t a, b;
+      pair_decode(t, nn, no, P1, a, b);
+      ia = s_id[a];
+      ib = s_id[b];
+      if (ia != ib) {
+        ++pairs;
+        dist = lds_sq_dist<D>(rows + (size_t)a * RS, rows + (size_t)b * RS, d);
+        pa = dist < s_thr[a];
+        pb = dist < s_thr[b];
+      }
+    }
+    pb_append(pb_row, pb_key, pcnt, pa, ia, make_key(dist, ib, 1));
+    pb_append(pb_row, pb_key, pcnt, pb, ib, make_key(dist, ia, 1));
+    pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, bmask, seed, iter);
+  }
+  pb_flush(pb_row, pb_key, pcnt, true, slots, touched, S, bmask, seed, iter);
   // counters[0] += pairs, counters[1] += nn + no (candidate rows gathered)
   uint64_t tot = pairs;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) tot += shfl_xor64(tot, o);
-  if (l == 0 && nn > 0) {
-    atomicAdd(&counters[0], (unsigned long long)tot);
-    atomicAdd(&counters[1], (unsigned long long)(nn + no));
+  if (l == 0) {
+    ctr_add(counters, 0, (unsigned long long)tot);
+    ctr_add(counters, 1, (unsigned long long)nc);
   }
 }
 
 // ── merge (the apply of nndescent.rs:307-324 as a set function) ──
-// One wave per touched row: top-k of (row ∪ slots), deduplicated on (dist, idx)
-// keeping the smaller flag (an entry already in the row keeps its flag). The
-// slots are reset. `changed` counts entries of the new row absent from the old.
+// One wave per touched row: top-k of (row ∪ proposals), deduplicated on (dist,
+// idx) keeping the smaller flag (an entry already in the row keeps its flag).
+// Pinned slots are skipped. Word 2 of the striped counters `changed` counts
+// entries of the new row absent from the old.
 template <int NR>
 __global__ __launch_bounds__(256) void k_merge(uint64_t* __restrict__ G, uint32_t n, uint32_t k,
                                                uint64_t* __restrict__ slots, uint32_t S,
@@ -390,8 +432,7 @@ __global__ __launch_bounds__(256) void k_merge(uint64_t* __restrict__ G, uint32_
   for (int r = 0; r < NR; ++r) {
     const uint32_t e = r * WAVE + l;
     if (e < S) {
-      v[r] = sl[e];
-      sl[e] = EMPTY_KEY;
+      v[r] = bucket_to_row(sl[e]);
     } else if (e < S + k) {
       v[r] = row[e - S];
     } else {
@@ -432,7 +473,7 @@ __global__ __launch_bounds__(256) void k_merge(uint64_t* __restrict__ G, uint32_
   uint32_t tot = nchanged;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, WAVE);
-  if (l == 0 && tot) atomicAdd(&changed[0], (unsigned long long)tot);
+  if (l == 0 && tot) ctr_add(changed, 2, (unsigned long long)tot);
 }
 
 // ── extraction (nndescent.rs:977-1002) ──
@@ -461,22 +502,14 @@ __global__ void k_extract(const uint64_t* __restrict__ G, uint32_t n, uint32_t k
 }
 
 // explicit instantiations used by the host code
-template __global__ void k_join<0, 1>(const float*, uint32_t, uint32_t, uint32_t, const uint32_t*,
-                                      const uint32_t*, const uint32_t*, uint32_t, const float*,
-                                      uint64_t*, uint8_t*, uint32_t, uint64_t, uint64_t,
-                                      unsigned long long*);
-template __global__ void k_join<0, 2>(const float*, uint32_t, uint32_t, uint32_t, const uint32_t*,
-                                      const uint32_t*, const uint32_t*, uint32_t, const float*,
-                                      uint64_t*, uint8_t*, uint32_t, uint64_t, uint64_t,
-                                      unsigned long long*);
-template __global__ void k_join<50, 1>(const float*, uint32_t, uint32_t, uint32_t, const uint32_t*,
-                                       const uint32_t*, const uint32_t*, uint32_t, const float*,
-                                       uint64_t*, uint8_t*, uint32_t, uint64_t, uint64_t,
-                                       unsigned long long*);
-template __global__ void k_join<50, 2>(const float*, uint32_t, uint32_t, uint32_t, const uint32_t*,
-                                       const uint32_t*, const uint32_t*, uint32_t, const float*,
-                                       uint64_t*, uint8_t*, uint32_t, uint64_t, uint64_t,
-                                       unsigned long long*);
+template __global__ void k_join<0>(const float*, uint32_t, uint32_t, uint32_t, uint32_t,
+                                   const uint32_t*, const uint32_t*, const uint32_t*, uint32_t,
+                                   const float*, uint64_t*, uint8_t*, uint32_t, uint64_t, uint64_t,
+                                   unsigned long long*);
+template __global__ void k_join<50>(const float*, uint32_t, uint32_t, uint32_t, uint32_t,
+                                    const uint32_t*, const uint32_t*, const uint32_t*, uint32_t,
+                                    const float*, uint64_t*, uint8_t*, uint32_t, uint64_t, uint64_t,
+                                    unsigned long long*);
 template __global__ void k_merge<2>(uint64_t*, uint32_t, uint32_t, uint64_t*, uint32_t, uint8_t*,
                                     float*, int, unsigned long long*);
 template __global__ void k_merge<4>(uint64_t*, uint32_t, uint32_t, uint64_t*, uint32_t, uint8_t*,

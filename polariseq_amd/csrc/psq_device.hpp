@@ -218,35 +218,132 @@ __device__ inline float sq_dist_fixed(PA p, PB q) {
   return (((acc[0] + acc[4]) + (acc[1] + acc[5])) + ((acc[2] + acc[6]) + (acc[3] + acc[7]))) + tail;
 }
 
+// ── statistics counters ──
+// Counters are striped over CTR_STRIPES lines of CTR_WORDS u64: a block adds to
+// its own stripe and k_ctr_reduce sums the stripes, so millions of per-wave
+// additions do not serialise on one address.
+constexpr uint32_t CTR_STRIPES = 256, CTR_WORDS = 8;
+__device__ inline void ctr_add(unsigned long long* ctr, uint32_t word, unsigned long long v) {
+  atomicAdd(&ctr[(size_t)(blockIdx.x % CTR_STRIPES) * CTR_WORDS + word], v);
+}
+
 // ── proposals: bucket cascade ──
-// Insert `key` into the 8-slot bucket of row `row` chosen by the neighbour's
-// hash. Each bucket holds its 8 smallest distinct keys in ascending order: a
-// key starts at its rank among the slots read (slots only decrease, so the
-// smaller ones stay smaller), then an atomic min either places it and carries
-// the displaced key down, or carries it on. Equal keys stop the cascade.
+// Each row owns S/8 buckets of 8 slots; a neighbour q goes to bucket
+// slot_hash(seed, iter, q) & (S/8 - 1). Before the join, a row's buckets are
+// laid out with its current neighbours "pinned": pinned key = idx << 1 (dist
+// field 0), ascending, at most 8 per bucket. A proposal is stored as its row
+// key plus 1 << 32 (dist field + 1), so it sorts after every pinned key and
+// pinned slots never move. A proposal of a neighbour the row already holds is
+// rejected by reading the bucket, with no atomic.
+__device__ inline uint64_t pin_key(uint32_t idx) { return (uint64_t)idx << 1; }
+__device__ inline uint64_t bucket_key(uint64_t row_key) { return row_key + (1ull << 32); }
+// Bucket slot -> row key; EMPTY_KEY for empty and pinned slots.
+__device__ inline uint64_t bucket_to_row(uint64_t b) {
+  return (b == EMPTY_KEY || (b >> 32) == 0) ? EMPTY_KEY : b - (1ull << 32);
+}
+
+// Insert a proposal (row key `key`, distance finite) into its bucket. The
+// unpinned part of a bucket holds the smallest distinct proposal keys in
+// ascending order: a key starts at its rank among the slots read (slots only
+// decrease, so the smaller ones stay smaller), then an atomic min either places
+// it and carries the displaced key down, or carries it on. Equal keys stop the
+// cascade, so the content is a set function of the proposals.
 __device__ inline void propose(uint64_t* __restrict__ slots, uint8_t* __restrict__ touched,
                                uint32_t row, uint64_t key, uint32_t S, uint32_t bmask,
                                uint64_t seed, uint64_t iter) {
-  uint32_t b = slot_hash(seed, iter, key_idx(key)) & bmask;
+  const uint32_t q = key_idx(key);
+  uint32_t b = slot_hash(seed, iter, q) & bmask;
   uint64_t* bk = slots + (size_t)row * S + (size_t)b * BUCKET;
   const ulonglong2* b2 = reinterpret_cast<const ulonglong2*>(bk);
+  const uint64_t bkey = bucket_key(key), pin = pin_key(q);
   ulonglong2 v01 = b2[3];
-  if (key >= v01.y) return;  // 8 smaller keys already present (or this one)
+  if (bkey >= v01.y) return;  // 8 smaller keys already present (or this one)
   ulonglong2 a = b2[0], bb = b2[1], c = b2[2];
   uint64_t v[8] = {a.x, a.y, bb.x, bb.y, c.x, c.y, v01.x, v01.y};
   int r = 0;
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
-    if (v[j] == key) return;
-    r += (v[j] < key);
+    if (v[j] == bkey || v[j] == pin) return;
+    r += (v[j] < bkey);
   }
-  uint64_t cur = key;
+  uint64_t cur = bkey;
   for (int j = r; j < BUCKET; ++j) {
     uint64_t old = atomicMin((unsigned long long*)&bk[j], (unsigned long long)cur);
     if (old == cur || old == EMPTY_KEY) break;
     cur = old > cur ? old : cur;
   }
   touched[row] = 1;
+}
+
+// ── join helpers: pair enumeration and the per-wave proposal list ──
+// Pairs of one cell, numbered t = 0 .. P1 + nn*no - 1: first the new x new
+// upper triangle (a < b < nn) in row order, then new x old (a < nn, b = nn + j).
+// off(a) = a * (2nn - a - 1) / 2 pairs precede row a of the triangle.
+__host__ __device__ inline void pair_decode(uint32_t t, uint32_t nn, uint32_t no, uint32_t P1,
+                                            uint32_t& a, uint32_t& b) {
+  if (t < P1) {
+    const float fn = 2.0f * (float)nn - 1.0f;
+    const float disc = fn * fn - 8.0f * (float)t;
+    int32_t aa = (int32_t)((fn - sqrtf(disc > 0.0f ? disc : 0.0f)) * 0.5f);
+    const int32_t amax = (int32_t)nn - 2;
+    aa = aa < 0 ? 0 : (aa > amax ? amax : aa);
+    const uint32_t n2 = 2 * nn - 1;
+    while (aa > 0 && (uint32_t)aa * (n2 - (uint32_t)aa) / 2 > t) --aa;
+    while (aa < amax && (uint32_t)(aa + 1) * (n2 - (uint32_t)(aa + 1)) / 2 <= t) ++aa;
+    a = (uint32_t)aa;
+    b = a + 1 + (t - a * (n2 - a) / 2);
+  } else {
+    const uint32_t u = t - P1;
+    a = u / no;
+    b = nn + (u - a * no);
+  }
+}
+
+// Proposals a wave finds are appended to an LDS list and sent 64 at a time,
+// one per lane, so a batch's bucket reads and atomics are in flight together.
+// The join kernels run one wave per block, so __syncthreads() is a wave barrier
+// and every call below is made by the whole wave. Capacity: < 64 entries are
+// held between calls and one append adds at most 128.
+constexpr uint32_t PB_CAP = 192;
+__device__ inline void pb_append(uint32_t* pb_row, uint64_t* pb_key, uint32_t& cnt, bool has,
+                                 uint32_t row, uint64_t key) {
+  const uint64_t mask = __ballot(has);
+  if (has) {
+    const uint32_t pos = cnt + (uint32_t)__popcll(mask & ((1ull << lane_id()) - 1ull));
+    pb_row[pos] = row;
+    pb_key[pos] = key;
+  }
+  cnt += (uint32_t)__popcll(mask);
+}
+// Sends every full batch of 64 (all entries when `all`), keeping the rest.
+__device__ inline void pb_flush(uint32_t* pb_row, uint64_t* pb_key, uint32_t& cnt, bool all,
+                                uint64_t* __restrict__ slots, uint8_t* __restrict__ touched,
+                                uint32_t S, uint32_t bmask, uint64_t seed, uint64_t iter) {
+  if (cnt < (all ? 1u : (uint32_t)WAVE)) return;
+  const uint32_t l = (uint32_t)lane_id();
+  __syncthreads();
+  uint32_t done = 0;
+  for (; done + WAVE <= cnt; done += WAVE)
+    propose(slots, touched, pb_row[done + l], pb_key[done + l], S, bmask, seed, iter);
+  const uint32_t rest = cnt - done;
+  if (all) {
+    if (l < rest) propose(slots, touched, pb_row[done + l], pb_key[done + l], S, bmask, seed, iter);
+    cnt = 0;
+    return;
+  }
+  uint32_t rr = 0;
+  uint64_t rk = 0;
+  if (l < rest) {
+    rr = pb_row[done + l];
+    rk = pb_key[done + l];
+  }
+  __syncthreads();
+  if (l < rest) {
+    pb_row[l] = rr;
+    pb_key[l] = rk;
+  }
+  __syncthreads();
+  cnt = rest;
 }
 
 }  // namespace psq

@@ -16,14 +16,17 @@ __global__ void k_scan_add(uint32_t* out, uint32_t n, const uint32_t* sums);
 
 __global__ void k_random_fill(const float* Xp, uint32_t n, uint32_t d, uint32_t dp, uint64_t* G,
                               uint32_t k, uint64_t seed, uint8_t* touched);
+__global__ void k_ctr_reduce(const unsigned long long* ctr, unsigned long long* out);
 __global__ void k_rev_count(const uint64_t* G, size_t nk, uint32_t* rcount);
 __global__ void k_rev_scatter(const uint64_t* G, uint32_t n, uint32_t k, uint64_t seed,
                               uint64_t iter, const uint32_t* roff, uint32_t* rcount, uint64_t* rev);
 __global__ void k_sample_select(uint64_t* G, uint32_t n, uint32_t k, uint32_t m, uint64_t seed,
                                 uint64_t iter, const uint32_t* roff, const uint64_t* rev,
-                                uint32_t* newl, uint32_t* oldl, uint32_t* ncnt);
-template <int D, int CPL>
-__global__ void k_join(const float* Xp, uint32_t n, uint32_t d, uint32_t dp, const uint32_t* newl,
+                                uint32_t* newl, uint32_t* oldl, uint32_t* ncnt, uint64_t* slots,
+                                uint32_t S);
+template <int D>
+__global__ void k_join(const float* Xp, uint32_t n, uint32_t d, uint32_t dp, uint32_t RS,
+                       const uint32_t* newl,
                        const uint32_t* oldl, const uint32_t* ncnt, uint32_t m, const float* thr,
                        uint64_t* slots, uint8_t* touched, uint32_t S, uint64_t seed, uint64_t iter,
                        unsigned long long* counters);

@@ -29,7 +29,8 @@ def _same(g, m):
 
 @pytest.mark.parametrize("n,d,k,seed", [(5, 3, 10, 0), (100, 8, 10, 1), (3000, 50, 15, 2),
                                         (20000, 50, 15, 0), (5000, 20, 30, 3), (2000, 50, 1, 4),
-                                        (4000, 50, 30, 5), (1000, 64, 7, 6)])
+                                        (4000, 50, 30, 5), (1000, 64, 7, 6),
+                                        (3000, 128, 15, 8), (1500, 256, 20, 9)])
 def test_bit_exact_against_the_oracle_random_start(orc, n, d, k, seed):
     x = ds.mixture(n, d, 8, seed=seed) if n > 20 else ds.clustered(n, d, 2, seed=0)
     g = P.nndescent_dense(x, k, seed=seed, init="random")
