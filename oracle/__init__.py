@@ -112,7 +112,7 @@ def _declare(L: ctypes.CDLL) -> None:
                                        ctypes.POINTER(PortStats)]
     L.psq_o_gpu_model_nndescent.restype = ctypes.c_int64
     L.psq_o_gpu_model_nndescent.argtypes = [_P, _U64, _U64, _U64, _U64, _U64, ctypes.c_double,
-                                            _U64, _U64, _U64, _U64, _U64, _P, ctypes.c_int, _P,
+                                            _U64, _U64, _U64, _U64, _U64, _U64, _P, ctypes.c_int, _P,
                                             _P, _P, ctypes.POINTER(ModelStats)]
 
 
@@ -224,13 +224,14 @@ def gpu_model_nndescent(data: np.ndarray, k: int, max_iterations: int = 0, seed:
                         max_candidates: int = 0, n_trees: int = 0, delta: float = 0.001,
                         leaf_size: int = 0, slots: int = 0, init: int = 0,
                         init_graph: np.ndarray | None = None, threads: int | None = None,
-                        split_data: np.ndarray | None = None):
+                        split_data: np.ndarray | None = None, heap_k: int = 0):
     """The GPU algorithm's semantics on the CPU; returns (graph dict, ModelStats).
 
     ``init``: 0 = RP forest, 1 = random fill only, 2 = seed rows from ``init_graph``
     (u32[n, c] candidate indices, 0xFFFFFFFF = none), 3 = RP forest splitting the
     rows of ``split_data`` (f32[n, ds], e.g. learned codes), leaf pairs measured on
-    ``data``.
+    ``data``. ``heap_k``: entries each row keeps while iterating (0 = k); the
+    output holds each row's first k.
     """
     data = _f32(data)
     n, d = data.shape
@@ -252,7 +253,7 @@ def gpu_model_nndescent(data: np.ndarray, k: int, max_iterations: int = 0, seed:
         init = 3
     nnz = lib().psq_o_gpu_model_nndescent(
         _ptr(data), n, d, k, max_candidates, max_iterations, delta, n_trees, leaf_size, slots,
-        seed, (init & 0xFF) | (ig_cols << 8), None if ig is None else _ptr(ig),
+        heap_k, seed, (init & 0xFF) | (ig_cols << 8), None if ig is None else _ptr(ig),
         threads or default_threads(), _ptr(ip), _ptr(ix), _ptr(dx), ctypes.byref(st))
     if nnz < 0:
         raise ValueError("invalid input")

@@ -202,21 +202,26 @@ struct psq_o_model_stats {
 int64_t psq_o_gpu_model_nndescent(const float* data, uint64_t n_obs, uint64_t n_dim, uint64_t k_req,
                                   uint64_t max_candidates, uint64_t max_iterations, double delta,
                                   uint64_t n_trees_req, uint64_t leaf_size_req, uint64_t slots_req,
-                                  uint64_t seed, uint64_t init, const uint32_t* init_graph,
+                                  uint64_t heap_req, uint64_t seed, uint64_t init,
+                                  const uint32_t* init_graph,
                                   int threads, uint32_t* indptr_out, uint32_t* indices_out,
                                   float* dists_out, psq_o_model_stats* stats) {
   if (n_obs == 0 || n_dim == 0 || k_req == 0) return -1;
   Params P;
   P.n = n_obs;
   P.d = n_dim;
-  P.k = std::max<size_t>(1, std::min<size_t>(k_req, P.n - 1));
-  P.m = std::max<size_t>(1, max_candidates > 0 ? max_candidates : std::min<size_t>(2 * P.k, 60));
+  // ko = neighbours output per row; k = entries each row keeps while iterating
+  // (heap_req, at least ko; 0 -> ko). Options derived from k in the reference
+  // (candidates, leaf size) follow ko; bucket slots and the stop rule follow k.
+  const size_t ko = std::max<size_t>(1, std::min<size_t>(k_req, P.n - 1));
+  P.k = std::min<size_t>(std::max<size_t>(ko, heap_req), P.n - 1);
+  P.m = std::max<size_t>(1, max_candidates > 0 ? max_candidates : std::min<size_t>(2 * ko, 60));
   P.n_iters = max_iterations > 0 ? max_iterations
                                  : std::max<size_t>(5, (size_t)std::llround(std::log2((double)P.n)));
   P.n_trees = n_trees_req > 0
                   ? n_trees_req
                   : std::min<size_t>(48, 8 + (size_t)std::llround(std::sqrt((double)P.n) / 10.0));
-  P.leaf_size = leaf_size_req > 0 ? leaf_size_req : std::max<size_t>(P.k, 10);
+  P.leaf_size = leaf_size_req > 0 ? leaf_size_req : std::max<size_t>(ko, 10);
   size_t S = 64;
   while (S < 4 * P.k) S *= 2;
   P.slots = slots_req > 0 ? slots_req : S;
@@ -467,7 +472,7 @@ int64_t psq_o_gpu_model_nndescent(const float* data, uint64_t n_obs, uint64_t n_
   size_t pos = 0;
   indptr_out[0] = 0;
   for (size_t i = 0; i < n; ++i) {
-    for (size_t s = 0; s < k; ++s) {
+    for (size_t s = 0; s < ko; ++s) {
       uint64_t key = G[i * k + s];
       if (key == EMPTY_KEY) break;
       indices_out[pos] = key_idx(key);

@@ -90,6 +90,7 @@ class NndescentOpts(ctypes.Structure):
         ("model", ctypes.c_void_p),
         ("device", ctypes.c_int32),
         ("slots", ctypes.c_uint32),
+        ("heap_k", ctypes.c_uint32),
     ]
 
 
@@ -105,6 +106,7 @@ class NndescentStats(ctypes.Structure):
         ("slots", ctypes.c_uint64),
         ("stop_at", ctypes.c_uint64),
         ("iterations", ctypes.c_uint64),
+        ("heap_k", ctypes.c_uint64),
         ("changed", ctypes.c_uint64 * 64),
         ("pairs", ctypes.c_uint64 * 64),
         ("cand_rows", ctypes.c_uint64 * 64),
@@ -122,6 +124,7 @@ class NndescentStats(ctypes.Structure):
             "max_candidates": int(self.max_candidates), "max_iterations": int(self.max_iterations),
             "n_trees": int(self.n_trees), "leaf_size": int(self.leaf_size),
             "slots": int(self.slots), "stop_at": int(self.stop_at), "iterations": it,
+            "heap_k": int(self.heap_k),
             "changed": [int(v) for v in self.changed[:it]],
             "pairs": [int(v) for v in self.pairs[:it]],
             "cand_rows": [int(v) for v in self.cand_rows[:it]],
@@ -223,7 +226,7 @@ def check(rc: int) -> None:
 def make_opts(k: int, max_iterations: int = 0, seed: int = 0, max_candidates: int = 0,
               n_trees: int = 0, delta: float = 0.001, leaf_size: int = 0,
               precision: str = "fp32", init: str = "rp_forest", device: int = 0,
-              slots: int = 0, model=None) -> NndescentOpts:
+              slots: int = 0, model=None, heap_k: int = 0) -> NndescentOpts:
     o = NndescentOpts()
     lib().psq_nndescent_opts_default(ctypes.byref(o))
     o.k = int(k)
@@ -237,6 +240,7 @@ def make_opts(k: int, max_iterations: int = 0, seed: int = 0, max_candidates: in
     o.init = INIT[init]
     o.device = int(device)
     o.slots = int(slots)
+    o.heap_k = int(heap_k)
     o.model = model
     return o
 

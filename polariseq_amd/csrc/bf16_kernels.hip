@@ -10,10 +10,11 @@
 //    new rows x all candidates; d~(p,q) = |p~|^2 + |q~|^2 - 2 p~.q~, with the
 //    norms summed in FP32 from the same BF16 values. Proposals carry d~.
 //  * merge: d~ differs between cells, so proposals are deduplicated on the
-//    neighbour index; the 2k smallest new proposals are re-ranked with the
-//    exact f32 sq_dist (simd.rs:45-62 order) and the row becomes the top k of
-//    (row ∪ re-ranked), all exact. Output distances are therefore the plain
-//    form's, bit for bit, for every edge.
+//    neighbour index; the 2k smallest new proposals (k = output neighbours)
+//    are re-ranked with the exact f32 sq_dist (simd.rs:45-62 order) and the
+//    row becomes the top of (row ∪ re-ranked), all exact. Rows keep 2k
+//    entries by default and the graph takes the first k. Output distances are
+//    therefore the plain form's, bit for bit, for every edge.
 #include "psq_kernels.hpp"
 
 namespace psq {
@@ -131,13 +132,14 @@ __global__ __launch_bounds__(64) void k_join_bf16(
 
 // One wave per touched row. Slots hold approximate keys. Steps: drop proposals
 // whose neighbour is already in the row; inside each bucket (a neighbour always
-// hashes to the same bucket) keep a neighbour's smallest key; take the 2k
-// smallest survivors, recompute their exact f32 distance, and keep the top k of
-// (row ∪ re-ranked). NR: S / 64 registers per lane for the slots.
+// hashes to the same bucket) keep a neighbour's smallest key; take the rr
+// smallest survivors (rr = twice the output k), recompute their exact f32
+// distance, and keep the top k of (row ∪ re-ranked), k being the row size.
+// NR: S / 64 registers per lane for the slots.
 template <int NR>
 __global__ __launch_bounds__(256) void k_merge_rerank(
     const float* __restrict__ Xp, uint32_t d, uint32_t dp, uint64_t* __restrict__ G, uint32_t n,
-    uint32_t k, uint64_t* __restrict__ slots, uint32_t S, uint8_t* __restrict__ touched,
+    uint32_t k, uint32_t rr, uint64_t* __restrict__ slots, uint32_t S, uint8_t* __restrict__ touched,
     float* __restrict__ thr, unsigned long long* __restrict__ changed) {
   const int w = threadIdx.x / WAVE, l = lane_id();
   const uint32_t p = blockIdx.x * 4 + w;
@@ -174,24 +176,24 @@ __global__ __launch_bounds__(256) void k_merge_rerank(
     }
     if (dup) v[r] = EMPTY_KEY;
   }
-  // 3. the 2k smallest approximate keys
+  // 3. the rr smallest approximate keys
   wave_sort_n<NR>(v);
   // ranks are positions in the sorted order: element e = r * 64 + lane
-  // 4. exact distances for ranks < 2k (at most 64 lanes' worth since 2k <= 128)
+  // 4. exact distances for ranks < rr (rr <= 128)
   const float* xp = Xp + (size_t)p * dp;
   uint64_t ex[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
     const uint32_t e = r * WAVE + l;
     ex[r] = EMPTY_KEY;
-    if (e < 2 * k && v[r] != EMPTY_KEY) {
+    if (e < rr && v[r] != EMPTY_KEY) {
       const uint32_t q = key_idx(v[r]);
       const float dd = sq_dist_ref(xp, Xp + (size_t)q * dp, (int)d);
       ex[r] = make_key(dd, q, 1);
     }
   }
   // 5. top k of (row ∪ re-ranked): row entries and the re-ranked keys
-  //    (2k + k <= 3 * 64 elements) sorted together.
+  //    (k + rr <= 3 * 64 elements) sorted together.
   uint64_t u[4];
   u[0] = orow;
 #pragma unroll
@@ -225,13 +227,13 @@ __global__ __launch_bounds__(256) void k_merge_rerank(
 }
 
 template __global__ void k_merge_rerank<1>(const float*, uint32_t, uint32_t, uint64_t*, uint32_t,
-                                           uint32_t, uint64_t*, uint32_t, uint8_t*, float*,
+                                           uint32_t, uint32_t, uint64_t*, uint32_t, uint8_t*, float*,
                                            unsigned long long*);
 template __global__ void k_merge_rerank<4>(const float*, uint32_t, uint32_t, uint64_t*, uint32_t,
-                                           uint32_t, uint64_t*, uint32_t, uint8_t*, float*,
+                                           uint32_t, uint32_t, uint64_t*, uint32_t, uint8_t*, float*,
                                            unsigned long long*);
 template __global__ void k_merge_rerank<2>(const float*, uint32_t, uint32_t, uint64_t*, uint32_t,
-                                           uint32_t, uint64_t*, uint32_t, uint8_t*, float*,
+                                           uint32_t, uint32_t, uint64_t*, uint32_t, uint8_t*, float*,
                                            unsigned long long*);
 
 }  // namespace psq

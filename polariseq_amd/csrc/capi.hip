@@ -180,7 +180,8 @@ uint32_t round_up(uint32_t v, uint32_t m) { return (v + m - 1) / m * m; }
 constexpr float BF16_SLACK = 1.03f;
 
 struct Resolved {
-  uint64_t n, d, k, m, n_iters, n_trees, leaf_size, S, stop_at, seed;
+  // ko: neighbours output per row; k: entries each row keeps while iterating.
+  uint64_t n, d, k, ko, m, n_iters, n_trees, leaf_size, S, stop_at, seed;
   uint32_t dp;
 };
 
@@ -194,16 +195,21 @@ int resolve(uint64_t n, uint64_t d, const psq_nndescent_opts* o, Resolved* r) {
   if (d > 256) return fail(PSQ_E_UNSUPPORTED, "n_dim > 256 is not supported on the GPU path");
   r->n = n;
   r->d = d;
-  r->k = std::max<uint64_t>(1, std::min<uint64_t>(o->k, n - 1));
+  r->ko = std::max<uint64_t>(1, std::min<uint64_t>(o->k, n - 1));
+  const bool bf16 = o->precision == PSQ_PRECISION_BF16_RERANK;
+  uint64_t heap = o->heap_k > 0 ? o->heap_k : (bf16 ? std::min<uint64_t>(2 * r->ko, 64) : r->ko);
+  r->k = std::min<uint64_t>(std::max<uint64_t>(heap, r->ko), n - 1);
+  // options the reference derives from k (candidates, leaf size) follow the
+  // output k; bucket slots and the stop rule follow the row size
   r->m = std::max<uint64_t>(1, o->max_candidates > 0 ? o->max_candidates
-                                                      : std::min<uint64_t>(2 * r->k, 60));
+                                                      : std::min<uint64_t>(2 * r->ko, 60));
   r->n_iters = o->max_iterations > 0
                    ? o->max_iterations
                    : std::max<uint64_t>(5, (uint64_t)std::llround(std::log2((double)n)));
   r->n_trees = o->n_trees > 0
                    ? o->n_trees
                    : std::min<uint64_t>(48, 8 + (uint64_t)std::llround(std::sqrt((double)n) / 10.0));
-  r->leaf_size = o->leaf_size > 0 ? o->leaf_size : std::max<uint64_t>(r->k, 10);
+  r->leaf_size = o->leaf_size > 0 ? o->leaf_size : std::max<uint64_t>(r->ko, 10);
   uint64_t S = 64;
   while (S < 4 * r->k) S *= 2;
   if (o->slots) {
@@ -214,7 +220,7 @@ int resolve(uint64_t n, uint64_t d, const psq_nndescent_opts* o, Resolved* r) {
   r->stop_at = (uint64_t)(o->delta * (double)n * (double)r->k);
   r->seed = o->seed;
   r->dp = round_up((uint32_t)d, 32);
-  if (r->k > 64) return fail(PSQ_E_UNSUPPORTED, "k > 64 is not supported on the GPU path");
+  if (r->k > 64) return fail(PSQ_E_UNSUPPORTED, "k and heap_k > 64 are not supported on the GPU path");
   if (r->m > 64) return fail(PSQ_E_UNSUPPORTED, "max_candidates > 64 is not supported on the GPU path");
   if (r->S + r->k > 512) return fail(PSQ_E_UNSUPPORTED, "slots + k > 512 is not supported");
   if (o->precision != PSQ_PRECISION_FP32 && o->precision != PSQ_PRECISION_BF16_RERANK)
@@ -467,7 +473,7 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   if (indptr_len != n_obs + 1)
     return fail(PSQ_E_INVALID, "indptr must have n_obs + 1 = %llu entries (got %llu)",
                 (unsigned long long)(n_obs + 1), (unsigned long long)indptr_len);
-  const uint64_t need = n_obs * R.k;
+  const uint64_t need = n_obs * R.ko;
   if (cap < need)
     return fail(PSQ_E_INVALID, "outputs must hold %llu row pointers and %llu neighbours",
                 (unsigned long long)(n_obs + 1), (unsigned long long)need);
@@ -575,15 +581,16 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
     je.push_back(b);
     if (bf16) {
       const unsigned g4 = grid_for(n, 4);
+      const uint32_t rr = (uint32_t)std::min<uint64_t>(2 * R.ko, 2 * R.k);
       if (S <= 64)
         hipLaunchKernelGGL(k_merge_rerank<1>, dim3(g4), dim3(256), 0, s, Xp, (uint32_t)R.d, R.dp, G,
-                           n, k, slots, S, touched, thr, ctr);
+                           n, k, rr, slots, S, touched, thr, ctr);
       else if (S <= 128)
         hipLaunchKernelGGL(k_merge_rerank<2>, dim3(g4), dim3(256), 0, s, Xp, (uint32_t)R.d, R.dp, G,
-                           n, k, slots, S, touched, thr, ctr);
+                           n, k, rr, slots, S, touched, thr, ctr);
       else
         hipLaunchKernelGGL(k_merge_rerank<4>, dim3(g4), dim3(256), 0, s, Xp, (uint32_t)R.d, R.dp, G,
-                           n, k, slots, S, touched, thr, ctr);
+                           n, k, rr, slots, S, touched, thr, ctr);
       HIPC(hipGetLastError());
     } else {
       TRY(launch_merge(R, G, slots, touched, thr, 0, ctr, s));
@@ -605,11 +612,12 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   HIPC(hipEventRecord(e_iter_end, s));
 
   // ── extraction ──
-  hipLaunchKernelGGL(k_row_counts, dim3(grid_for(n, 256)), dim3(256), 0, s, G, n, k, rowcnt);
+  const uint32_t ko = (uint32_t)R.ko;
+  hipLaunchKernelGGL(k_row_counts, dim3(grid_for(n, 256)), dim3(256), 0, s, G, n, k, ko, rowcnt);
   HIPC(hipGetLastError());
   TRY(scan_u32(ws, rowcnt, d_indptr, n, s));
-  hipLaunchKernelGGL(k_extract, dim3(grid_for((size_t)n * k, 256)), dim3(256), 0, s, G, n, k,
-                     d_indptr, d_indices, d_dists);
+  hipLaunchKernelGGL(k_extract, dim3(grid_for((size_t)n * ko, 256)), dim3(256), 0, s, G, n, k,
+                     ko, d_indptr, d_indices, d_dists);
   HIPC(hipGetLastError());
   HIPC(hipEventRecord(e_end, s));
   uint32_t nnz32 = 0;
@@ -619,7 +627,8 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
 
   st.n_obs = R.n;
   st.n_dim = R.d;
-  st.k = R.k;
+  st.k = R.ko;
+  st.heap_k = R.k;
   st.max_candidates = R.m;
   st.max_iterations = R.n_iters;
   st.n_trees = opts->init == PSQ_INIT_RANDOM ? 0 : R.n_trees;
@@ -782,9 +791,9 @@ int psq_nndescent_into(const float* data, uint64_t n_obs, uint64_t n_dim,
   if (indptr_len != n_obs + 1)
     return fail(PSQ_E_INVALID, "indptr must have n_obs + 1 = %llu entries (got %llu)",
                 (unsigned long long)(n_obs + 1), (unsigned long long)indptr_len);
-  if (cap < n_obs * R.k)
+  if (cap < n_obs * R.ko)
     return fail(PSQ_E_INVALID, "outputs must hold %llu row pointers and %llu neighbours",
-                (unsigned long long)(n_obs + 1), (unsigned long long)(n_obs * R.k));
+                (unsigned long long)(n_obs + 1), (unsigned long long)(n_obs * R.ko));
   TRY(set_device(opts->device));
   Workspace* ws;
   TRY(workspace(&ws));
@@ -793,12 +802,12 @@ int psq_nndescent_into(const float* data, uint64_t n_obs, uint64_t n_dim,
   float* d_dx;
   TRY(wbuf(ws, "io_data", (size_t)n_obs * n_dim, &d_data));
   TRY(wbuf(ws, "io_indptr", (size_t)n_obs + 1, &d_ip));
-  TRY(wbuf(ws, "io_indices", (size_t)n_obs * R.k, &d_ix));
-  TRY(wbuf(ws, "io_dists", (size_t)n_obs * R.k, &d_dx));
+  TRY(wbuf(ws, "io_indices", (size_t)n_obs * R.ko, &d_ix));
+  TRY(wbuf(ws, "io_dists", (size_t)n_obs * R.ko, &d_dx));
   HIPC(hipMemcpy(d_data, data, (size_t)n_obs * n_dim * sizeof(float), hipMemcpyHostToDevice));
   uint64_t nnz = 0;
   TRY(nndescent_device_impl(d_data, n_obs, n_dim, opts, d_ip, n_obs + 1, d_ix, d_dx,
-                            n_obs * R.k, &nnz, nullptr));
+                            n_obs * R.ko, &nnz, nullptr));
   HIPC(hipMemcpy(indptr, d_ip, (size_t)(n_obs + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
   HIPC(hipMemcpy(indices, d_ix, (size_t)nnz * sizeof(uint32_t), hipMemcpyDeviceToHost));
   HIPC(hipMemcpy(dists, d_dx, (size_t)nnz * sizeof(float), hipMemcpyDeviceToHost));

@@ -477,22 +477,23 @@ __global__ __launch_bounds__(256) void k_merge(uint64_t* __restrict__ G, uint32_
 }
 
 // ── extraction (nndescent.rs:977-1002) ──
-__global__ void k_row_counts(const uint64_t* __restrict__ G, uint32_t n, uint32_t k,
+// Rows hold k entries; the graph takes the first ko (ko <= k) of each.
+__global__ void k_row_counts(const uint64_t* __restrict__ G, uint32_t n, uint32_t k, uint32_t ko,
                              uint32_t* __restrict__ cnt) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint64_t* r = G + (size_t)i * k;
   uint32_t c = 0;
-  while (c < k && r[c] != EMPTY_KEY) ++c;
+  while (c < ko && r[c] != EMPTY_KEY) ++c;
   cnt[i] = c;
 }
-__global__ void k_extract(const uint64_t* __restrict__ G, uint32_t n, uint32_t k,
+__global__ void k_extract(const uint64_t* __restrict__ G, uint32_t n, uint32_t k, uint32_t ko,
                           const uint32_t* __restrict__ off, uint32_t* __restrict__ indices,
                           float* __restrict__ dists) {
   size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)n * k) return;
-  uint32_t i = (uint32_t)(e / k), s = (uint32_t)(e - (size_t)i * k);
-  uint64_t key = G[e];
+  if (e >= (size_t)n * ko) return;
+  uint32_t i = (uint32_t)(e / ko), s = (uint32_t)(e - (size_t)i * ko);
+  uint64_t key = G[(size_t)i * k + s];
   if (key == EMPTY_KEY) return;
   uint32_t o = off[i] + s;
   indices[o] = key_idx(key);

@@ -33,8 +33,8 @@ __global__ void k_join(const float* Xp, uint32_t n, uint32_t d, uint32_t dp, uin
 template <int NR>
 __global__ void k_merge(uint64_t* G, uint32_t n, uint32_t k, uint64_t* slots, uint32_t S,
                         uint8_t* touched, float* thr, int force, unsigned long long* changed);
-__global__ void k_row_counts(const uint64_t* G, uint32_t n, uint32_t k, uint32_t* cnt);
-__global__ void k_extract(const uint64_t* G, uint32_t n, uint32_t k, const uint32_t* off,
+__global__ void k_row_counts(const uint64_t* G, uint32_t n, uint32_t k, uint32_t ko, uint32_t* cnt);
+__global__ void k_extract(const uint64_t* G, uint32_t n, uint32_t k, uint32_t ko, const uint32_t* off,
                           uint32_t* indices, float* dists);
 
 // RP forest (forest_kernels.hip)
@@ -76,8 +76,8 @@ __global__ void k_join_bf16(const float* Xp, uint32_t n, uint32_t d, uint32_t dp
                             uint64_t iter, unsigned long long* counters);
 template <int NR>
 __global__ void k_merge_rerank(const float* Xp, uint32_t d, uint32_t dp, uint64_t* G, uint32_t n,
-                               uint32_t k, uint64_t* slots, uint32_t S, uint8_t* touched, float* thr,
-                               unsigned long long* changed);
+                               uint32_t k, uint32_t rr, uint64_t* slots, uint32_t S, uint8_t* touched,
+                               float* thr, unsigned long long* changed);
 
 // learned encoder (mlp_kernels.hip)
 constexpr int MLP_N_PARAMS = 128 * 50 + 128 + 32 * 128 + 32;

@@ -28,18 +28,19 @@ def _valid(g, n, k):
     assert (np.diff(dx.reshape(n, k), axis=1) >= 0).all()
 
 
-@pytest.mark.parametrize("k", [15, 30])
-def test_bf16_form_exact_distances_invariants_and_recall(orc, k):
+@pytest.mark.parametrize("k,heap_k", [(15, 0), (30, 0), (15, 15)])
+def test_bf16_form_exact_distances_invariants_and_recall(orc, k, heap_k):
     n = 20000
     x = ds.mixture(n, 50, 20, seed=0)
-    g = P.nndescent_dense(x, k, seed=0, precision="bf16")
+    g = P.nndescent_dense(x, k, seed=0, precision="bf16", heap_k=heap_k)
+    assert P.last_stats()["heap_k"] == (heap_k or min(2 * k, 64))
     _valid(g, n, k)
     ix, dx = g["indices"].reshape(n, k), g["data"].reshape(n, k)
     for i in range(0, n, 211):
         for s in range(k):
             want = np.float32(np.sqrt(orc.sq_dist(x[i], x[ix[i, s]])))
             assert dx[i, s].view(np.uint32) == want.view(np.uint32)
-    again = P.nndescent_dense(x, k, seed=0, precision="bf16")
+    again = P.nndescent_dense(x, k, seed=0, precision="bf16", heap_k=heap_k)
     assert np.array_equal(again["indices"], g["indices"])
     q = np.random.default_rng(1).choice(n, 3000, replace=False).astype(np.uint32)
     ei, _ = orc.knn_exact64(x, k, q)

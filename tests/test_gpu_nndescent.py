@@ -68,6 +68,16 @@ def test_options_reach_the_kernels(orc):
     assert P.last_stats()["iterations"] == 3
 
 
+@pytest.mark.parametrize("init,model_init", [("random", 1), ("rp_forest", 0)])
+def test_wider_rows_bit_exact_against_the_oracle(orc, init, model_init):
+    x = ds.mixture(6000, 50, 10, seed=4)
+    g = P.nndescent_dense(x, 12, seed=3, init=init, n_trees=5, heap_k=24)
+    st = P.last_stats()
+    assert st["k"] == 12 and st["heap_k"] == 24
+    m, _ = orc.gpu_model_nndescent(x, 12, seed=3, init=model_init, n_trees=5, heap_k=24)
+    _same(g, m)
+
+
 def test_structure_determinism_and_in_place_form():
     x = ds.clustered(2000, 8, 5, seed=1)
     a = P.nndescent_dense(x, 10, seed=1, init="random")

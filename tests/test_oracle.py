@@ -276,3 +276,21 @@ def test_model_recall_matches_the_port_on_the_cpu_workload(orc):
     rm = ds.recall(gm["indices"].reshape(-1, 15)[q], ei).mean()
     assert rm >= rp - 0.002, (rm, rp)
     assert rm >= 0.95
+
+
+def test_model_wider_rows_output_the_first_k_and_raise_recall(orc):
+    # Rows of heap_k > k entries while iterating; the graph is each row's first k.
+    x = ds.mixture(8000, 50, 12, seed=3)
+    q = np.arange(0, 8000, 4, dtype=np.uint32)
+    ei, _ = orc.knn_exact64(x, 10, q)
+    g10, st10 = orc.gpu_model_nndescent(x, 10, seed=2, n_trees=4)
+    g20, st20 = orc.gpu_model_nndescent(x, 10, seed=2, n_trees=4, heap_k=20)
+    _valid(g20, 8000, 10)
+    r10 = ds.recall(g10["indices"].reshape(-1, 10)[q], ei).mean()
+    r20 = ds.recall(g20["indices"].reshape(-1, 10)[q], ei).mean()
+    assert r20 > r10, (r20, r10)
+    # candidates and leaf size follow the output k, slots and the stop rule the row size
+    assert st20.max_candidates == st10.max_candidates == 20
+    assert st20.leaf_size == st10.leaf_size == 10
+    assert st20.slots == 128 and st10.slots == 64
+    assert st20.stop_at == int(0.001 * 8000 * 20)
