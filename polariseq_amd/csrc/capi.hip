@@ -178,6 +178,7 @@ uint32_t round_up(uint32_t v, uint32_t m) { return (v + m - 1) / m * m; }
 // destination's threshold times this factor (the approximation error is a few
 // parts per thousand on centred vectors; the merge re-ranks exactly).
 constexpr float BF16_SLACK = 1.03f;
+constexpr uint32_t BF16_SV_CAP = 320;  // screened-pair list of k_join_bf16
 
 struct Resolved {
   // ko: neighbours output per row; k: entries each row keeps while iterating.
@@ -222,13 +223,11 @@ int resolve(uint64_t n, uint64_t d, const psq_nndescent_opts* o, Resolved* r) {
   r->dp = round_up((uint32_t)d, 32);
   if (r->k > 64) return fail(PSQ_E_UNSUPPORTED, "k and heap_k > 64 are not supported on the GPU path");
   if (r->m > 64) return fail(PSQ_E_UNSUPPORTED, "max_candidates > 64 is not supported on the GPU path");
-  if (r->S + r->k > 512) return fail(PSQ_E_UNSUPPORTED, "slots + k > 512 is not supported");
+  if (r->S > 256) return fail(PSQ_E_UNSUPPORTED, "slots > 256 is not supported");
   if (o->precision != PSQ_PRECISION_FP32 && o->precision != PSQ_PRECISION_BF16_RERANK)
     return fail(PSQ_E_INVALID, "unknown precision %d", o->precision);
   if (o->precision == PSQ_PRECISION_BF16_RERANK && d > 64)
     return fail(PSQ_E_UNSUPPORTED, "the BF16 form supports n_dim <= 64");
-  if (o->precision == PSQ_PRECISION_BF16_RERANK && r->S > 256)
-    return fail(PSQ_E_UNSUPPORTED, "the BF16 form supports slots <= 256");
   if (o->init != PSQ_INIT_RANDOM && o->init != PSQ_INIT_RP_FOREST && o->init != PSQ_INIT_LEARNED)
     return fail(PSQ_E_INVALID, "unknown init %d", o->init);
   if (o->init == PSQ_INIT_LEARNED && !o->model)
@@ -240,17 +239,19 @@ int resolve(uint64_t n, uint64_t d, const psq_nndescent_opts* o, Resolved* r) {
 
 int launch_merge(const Resolved& R, uint64_t* G, uint64_t* slots, uint8_t* touched, float* thr,
                  int force, unsigned long long* changed, hipStream_t s) {
-  const unsigned grid = grid_for(R.n, 4);
-  const uint64_t need = R.S + R.k;
-  if (need <= 128)
-    hipLaunchKernelGGL(k_merge<2>, dim3(grid), dim3(256), 0, s, G, (uint32_t)R.n, (uint32_t)R.k,
-                       slots, (uint32_t)R.S, touched, thr, force, changed);
-  else if (need <= 256)
-    hipLaunchKernelGGL(k_merge<4>, dim3(grid), dim3(256), 0, s, G, (uint32_t)R.n, (uint32_t)R.k,
-                       slots, (uint32_t)R.S, touched, thr, force, changed);
+  const dim3 grid((unsigned)R.n), block(64);
+  const uint32_t n = (uint32_t)R.n, k = (uint32_t)R.k, S = (uint32_t)R.S;
+  if (S == 64)
+    hipLaunchKernelGGL((k_merge<1, 2>), grid, block, 0, s, G, n, k, slots, S, touched, thr, force,
+                       changed);
+  else if (S == 128)
+    hipLaunchKernelGGL((k_merge<2, 4>), grid, block, 0, s, G, n, k, slots, S, touched, thr, force,
+                       changed);
+  else if (S == 256)
+    hipLaunchKernelGGL((k_merge<4, 8>), grid, block, 0, s, G, n, k, slots, S, touched, thr, force,
+                       changed);
   else
-    hipLaunchKernelGGL(k_merge<8>, dim3(grid), dim3(256), 0, s, G, (uint32_t)R.n, (uint32_t)R.k,
-                       slots, (uint32_t)R.S, touched, thr, force, changed);
+    return fail(PSQ_E_UNSUPPORTED, "slots must be 64, 128 or 256");
   HIPC(hipGetLastError());
   return PSQ_OK;
 }
@@ -559,7 +560,7 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
     hipLaunchKernelGGL(k_rev_scatter, dim3(grid_for((size_t)n * k, 256)), dim3(256), 0, s, G, n, k,
                        R.seed, it, roff, rcount, rev);
     HIPC(hipGetLastError());
-    hipLaunchKernelGGL(k_sample_select, dim3(grid_for(n, 4)), dim3(256), 0, s, G, n, k, m, R.seed,
+    hipLaunchKernelGGL(k_sample_select, dim3(n), dim3(64), 0, s, G, n, k, m, R.seed,
                        it, roff, rev, newl, oldl, ncnt, slots, S);
     HIPC(hipGetLastError());
     hipEvent_t a, b;
@@ -567,11 +568,16 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
     TRY(evp.make(&b));
     HIPC(hipEventRecord(a, s));
     if (bf16) {
+      const uint32_t RS = join_row_stride(R.d);
       const uint32_t Rr = (2 * m + 15) / 16 * 16;
-      const size_t smem = (size_t)PB_CAP * 12 + (size_t)Rr * (72 * 2 + 12);
-      hipLaunchKernelGGL(k_join_bf16, dim3(n), dim3(64), smem, s, Xp, n, (uint32_t)R.d,
-                         R.dp, newl, oldl, ncnt, m, thr, BF16_SLACK, slots, touched, S, R.seed, it,
-                         ctr);
+      const size_t smem = (size_t)PB_CAP * 12 + (size_t)(Rr + 1) * RS * 4 + (size_t)Rr * 12 +
+                          (size_t)BF16_SV_CAP * 4;
+      auto kern = R.d == 50 ? k_join_bf16<50> : k_join_bf16<0>;
+      if (smem > 65536)
+        HIPC(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)smem));
+      hipLaunchKernelGGL(kern, dim3(n), dim3(64), smem, s, Xp, n, (uint32_t)R.d, R.dp, RS, newl,
+                         oldl, ncnt, m, thr, BF16_SLACK, slots, touched, S, R.seed, it, ctr);
       HIPC(hipGetLastError());
     } else {
       TRY(launch_join(R, Xp, newl, oldl, ncnt, thr, slots, touched, it, ctr, s));
@@ -579,22 +585,7 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
     HIPC(hipEventRecord(b, s));
     jb.push_back(a);
     je.push_back(b);
-    if (bf16) {
-      const unsigned g4 = grid_for(n, 4);
-      const uint32_t rr = (uint32_t)std::min<uint64_t>(2 * R.ko, 2 * R.k);
-      if (S <= 64)
-        hipLaunchKernelGGL(k_merge_rerank<1>, dim3(g4), dim3(256), 0, s, Xp, (uint32_t)R.d, R.dp, G,
-                           n, k, rr, slots, S, touched, thr, ctr);
-      else if (S <= 128)
-        hipLaunchKernelGGL(k_merge_rerank<2>, dim3(g4), dim3(256), 0, s, Xp, (uint32_t)R.d, R.dp, G,
-                           n, k, rr, slots, S, touched, thr, ctr);
-      else
-        hipLaunchKernelGGL(k_merge_rerank<4>, dim3(g4), dim3(256), 0, s, Xp, (uint32_t)R.d, R.dp, G,
-                           n, k, rr, slots, S, touched, thr, ctr);
-      HIPC(hipGetLastError());
-    } else {
-      TRY(launch_merge(R, G, slots, touched, thr, 0, ctr, s));
-    }
+    TRY(launch_merge(R, G, slots, touched, thr, 0, ctr, s));
     unsigned long long* csum = ctr + (size_t)CTR_STRIPES * CTR_WORDS;
     hipLaunchKernelGGL(k_ctr_reduce, dim3(1), dim3(64), 0, s, ctr, csum);
     HIPC(hipGetLastError());
@@ -605,6 +596,7 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
       st.changed[it] = changed;
       st.pairs[it] = hctr[0];
       st.cand_rows[it] = hctr[1];
+      st.exact_pairs[it] = bf16 ? hctr[3] : hctr[0];
     }
     ++iters;
     if (changed <= R.stop_at) break;

@@ -181,121 +181,114 @@ __global__ void k_rev_scatter(const uint64_t* __restrict__ G, uint32_t n, uint32
   rev[pos] = (cls << 63) | (pr << 32) | (uint64_t)i;
 }
 
-// One wave per row i. Candidates: the row's own entries (forward, priority of
-// slot i*k+s) and the reverse entries from rev[roff[i]..roff[i+1]). Per flag
-// class, a neighbour seen twice keeps its smaller priority; then the m smallest
-// (priority, neighbour) keys form the new and old lists. Entries of the row
-// that made the new list lose their new flag (nndescent.rs:894-909).
-__global__ __launch_bounds__(256) void k_sample_select(
+// One wave (one block) per row i. Candidates: the row's own entries (forward,
+// priority of slot i*k+s) and the reverse entries rev[roff[i]..roff[i+1]).
+// Per flag class, a neighbour seen twice keeps its smaller priority (a forward
+// entry wins ties); then the m smallest (priority, neighbour) keys form the new
+// and old lists. Entries of the row that made the new list lose their new flag
+// (nndescent.rs:894-909). Candidate t < k is forward entry t, t >= k reverse
+// entry t - k; they are taken 64 at a time. Key: (class << 63) | (prio24 << 32)
+// | neighbour, class 0 = new, so one sort orders new before old.
+// The row's neighbours are also pinned into its buckets for this iteration's
+// join (at most BUCKET per bucket, in any order; the other slots empty).
+__global__ __launch_bounds__(64) void k_sample_select(
     uint64_t* __restrict__ G, uint32_t n, uint32_t k, uint32_t m, uint64_t seed, uint64_t iter,
     const uint32_t* __restrict__ roff, const uint64_t* __restrict__ rev,
     uint32_t* __restrict__ newl, uint32_t* __restrict__ oldl, uint32_t* __restrict__ ncnt,
     uint64_t* __restrict__ slots, uint32_t S) {
-  __shared__ uint64_t s_fk[4][WAVE];
-  __shared__ uint32_t s_kill[4][WAVE];
-  const int w = threadIdx.x / WAVE;
-  const int l = lane_id();
-  const uint32_t i = blockIdx.x * 4 + w;
-  const bool active = i < n;  // wave-uniform
-  const uint64_t fkey = (active && l < (int)k) ? G[(size_t)i * k + l] : EMPTY_KEY;
+  __shared__ uint64_t s_fk[WAVE];     // forward keys
+  __shared__ uint64_t s_kkey[WAVE];   // key of the reverse entry that replaced a forward one
+  __shared__ uint64_t s_pin[256];     // bucket layout, S <= 256
+  __shared__ uint32_t s_bcnt[32];     // pinned per bucket
+  const uint32_t l = (uint32_t)lane_id();
+  const uint32_t i = blockIdx.x;
+  if (i >= n) return;  // block-uniform
+  const uint64_t fkey = l < k ? G[(size_t)i * k + l] : EMPTY_KEY;
   const bool fvalid = fkey != EMPTY_KEY;
+  const uint32_t me = key_idx(fkey);
   uint64_t fk = EMPTY_KEY;
   if (fvalid) {
-    uint64_t cls = (fkey & 1) ? 0ull : 1ull;
-    uint64_t pr = hash_prio24(seed, iter, (uint64_t)i * k + l);
-    fk = (cls << 63) | (pr << 32) | key_idx(fkey);
+    const uint64_t cls = (fkey & 1) ? 0ull : 1ull;
+    const uint64_t pr = hash_prio24(seed, iter, (uint64_t)i * k + l);
+    fk = (cls << 63) | (pr << 32) | me;
   }
-  s_fk[w][l] = fk;
-  s_kill[w][l] = 0;
+  s_fk[l] = fk;
+  s_kkey[l] = EMPTY_KEY;
+  if (l < 32) s_bcnt[l] = 0;
   __syncthreads();
-  const uint32_t r0 = active ? roff[i] : 0u, r1 = active ? roff[i + 1] : 0u;
-  // pass 1: a reverse entry with a smaller priority than the matching forward
-  // entry of the same class removes the forward one.
-  for (uint32_t base = r0; base < r1; base += WAVE) {
-    uint32_t e = base + l;
-    if (e < r1) {
-      uint64_t rv = rev[e];
-      uint32_t src = (uint32_t)rv;
-      uint64_t cp = rv >> 32;  // class and priority
-      for (uint32_t f = 0; f < k; ++f) {
-        uint64_t fv = s_fk[w][f];
-        if (fv != EMPTY_KEY && (uint32_t)fv == src && (fv >> 63) == (cp >> 31)) {
-          if ((cp & 0x7FFFFFFFull) < ((fv >> 32) & 0x7FFFFFFFull)) s_kill[w][f] = 1;
+  const uint32_t r0 = roff[i], nrev = roff[i + 1] - r0, T = k + nrev;
+  const uint32_t nvec = (T + WAVE - 1) / WAVE;
+  // Reverse candidate t of vector j (t >= k): its key, or EMPTY when a forward
+  // entry of the same class and neighbour has a priority <= its own. When its
+  // priority is smaller, it replaces that forward entry (record = true).
+  auto rev_key = [&](uint32_t t, bool record) -> uint64_t {
+    if (t < k || t >= T) return EMPTY_KEY;
+    const uint64_t rv = rev[r0 + (t - k)];
+    const uint32_t src = (uint32_t)rv;
+    const uint64_t cp = rv >> 32;  // class and priority
+    for (uint32_t f = 0; f < k; ++f) {
+      const uint64_t fv = s_fk[f];
+      if (fv != EMPTY_KEY && (uint32_t)fv == src && (fv >> 63) == (cp >> 31)) {
+        if ((cp & 0x7FFFFFFFull) < ((fv >> 32) & 0x7FFFFFFFull)) {
+          if (record) s_kkey[f] = rv;
+        } else {
+          return EMPTY_KEY;
         }
       }
     }
+    return rv;
+  };
+  // pass 1: every replacement is known before a forward key is used
+  uint64_t key0 = rev_key(l, true);
+  for (uint32_t j = 1; j < nvec; ++j) (void)rev_key(j * WAVE + l, true);
+  __syncthreads();
+  if (l < k) key0 = (fvalid && s_kkey[l] == EMPTY_KEY) ? fk : EMPTY_KEY;
+  // pass 2: selection
+  uint64_t bestN = EMPTY_KEY, bestO = EMPTY_KEY;
+  uint32_t nn = 0, no = 0;
+  for (uint32_t j = 0; j < nvec; ++j) {
+    const uint64_t key = j == 0 ? key0 : rev_key(j * WAVE + l, false);
+    const uint64_t sk = wave_sort64(key);
+    const uint32_t cN = (uint32_t)__popcll(__ballot(sk != EMPTY_KEY && (sk >> 63) == 0));
+    const uint32_t cO = (uint32_t)__popcll(__ballot(sk != EMPTY_KEY && (sk >> 63) == 1));
+    const uint64_t vN = l < cN ? sk : EMPTY_KEY;
+    const uint64_t sh = shfl64(sk, (int)((cN + l) & (WAVE - 1)));
+    const uint64_t vO = l < cO ? sh : EMPTY_KEY;
+    if (j == 0) {
+      bestN = vN;
+      bestO = vO;
+    } else {
+      bestN = wave_merge_keep64(bestN, vN);
+      bestO = wave_merge_keep64(bestO, vO);
+    }
+    nn += cN;
+    no += cO;
+  }
+  nn = nn < m ? nn : m;
+  no = no < m ? no : m;
+  if (l < m) {
+    newl[(size_t)i * m + l] = l < nn ? (uint32_t)bestN : EMPTY;
+    oldl[(size_t)i * m + l] = l < no ? (uint32_t)bestO : EMPTY;
+  }
+  if (l == 0) ncnt[i] = nn | (no << 16);
+  // flag update: the surviving new-class key of this neighbour (the entry's own
+  // or the reverse entry that replaced it) is among the nn selected
+  const uint64_t lastN = shfl64(bestN, (int)(nn > 0 ? nn - 1 : 0));
+  if (fvalid && (fkey & 1) && nn > 0) {
+    const uint64_t kk = s_kkey[l] != EMPTY_KEY ? s_kkey[l] : fk;
+    if (kk <= lastN) G[(size_t)i * k + l] = fkey & ~1ull;
+  }
+  // bucket layout
+  const uint32_t bmask = S / BUCKET - 1;
+  for (uint32_t e = l; e < S; e += WAVE) s_pin[e] = EMPTY_KEY;
+  __syncthreads();
+  if (fvalid) {
+    const uint32_t b = slot_hash(seed, iter, me) & bmask;
+    const uint32_t r = atomicAdd(&s_bcnt[b], 1u);
+    if (r < BUCKET) s_pin[b * BUCKET + r] = pin_key(me);
   }
   __syncthreads();
-  // pass 2: selection, keeping the 64 smallest keys per class.
-  uint64_t bestN, bestO;
-  {
-    uint64_t key = (fvalid && !s_kill[w][l]) ? fk : EMPTY_KEY;
-    uint64_t kn = (key != EMPTY_KEY && (key >> 63) == 0) ? key : EMPTY_KEY;
-    uint64_t ko = (key != EMPTY_KEY && (key >> 63) == 1) ? key : EMPTY_KEY;
-    bestN = wave_sort64(kn);
-    bestO = wave_sort64(ko);
-  }
-  for (uint32_t base = r0; base < r1; base += WAVE) {
-    uint32_t e = base + l;
-    uint64_t key = EMPTY_KEY;
-    if (e < r1) {
-      uint64_t rv = rev[e];
-      uint32_t src = (uint32_t)rv;
-      uint64_t cp = rv >> 32;
-      bool dead = false;
-      for (uint32_t f = 0; f < k; ++f) {
-        uint64_t fv = s_fk[w][f];
-        if (fv != EMPTY_KEY && (uint32_t)fv == src && (fv >> 63) == (cp >> 31)) {
-          if (((fv >> 32) & 0x7FFFFFFFull) <= (cp & 0x7FFFFFFFull)) dead = true;
-        }
-      }
-      if (!dead) key = rv;
-    }
-    uint64_t kn = (key != EMPTY_KEY && (key >> 63) == 0) ? key : EMPTY_KEY;
-    uint64_t ko = (key != EMPTY_KEY && (key >> 63) == 1) ? key : EMPTY_KEY;
-    bestN = wave_merge_keep64(bestN, wave_sort64(kn));
-    bestO = wave_merge_keep64(bestO, wave_sort64(ko));
-  }
-  const bool inN = (l < (int)m) && bestN != EMPTY_KEY;
-  const bool inO = (l < (int)m) && bestO != EMPTY_KEY;
-  const uint32_t nn = __popcll(__ballot(inN));
-  const uint32_t no = __popcll(__ballot(inO));
-  // flag update: a wave-uniform loop over the new list (all lanes shuffle)
-  const uint32_t me = key_idx(fkey);
-  bool found = false;
-  for (uint32_t r = 0; r < nn; ++r) {
-    uint32_t v = (uint32_t)shfl64(bestN, (int)r);
-    found |= (v == me);
-  }
-  if (active) {
-    if (l < (int)m) {
-      newl[(size_t)i * m + l] = inN ? (uint32_t)bestN : EMPTY;
-      oldl[(size_t)i * m + l] = inO ? (uint32_t)bestO : EMPTY;
-    }
-    if (l == 0) ncnt[i] = nn | (no << 16);
-    if (fvalid && (fkey & 1) && found) G[(size_t)i * k + l] = fkey & ~1ull;
-    // Bucket layout for this iteration's join: the row's neighbours pinned,
-    // ascending by index inside each bucket, at most BUCKET per bucket; the
-    // other slots empty. Every row is rewritten, so no bucket state carries over.
-    const uint32_t bmask = S / BUCKET - 1;
-    const uint32_t mb = fvalid ? (slot_hash(seed, iter, me) & bmask) : EMPTY;
-    uint32_t rank = 0;
-    for (uint32_t s = 0; s < k; ++s) {
-      const uint32_t bs = __shfl(mb, (int)s, WAVE), is = __shfl(me, (int)s, WAVE);
-      rank += (fvalid && bs == mb && is < me) ? 1u : 0u;
-    }
-    for (uint32_t e0 = 0; e0 < S; e0 += WAVE) {
-      const uint32_t e = e0 + l;
-      const uint32_t eb = e / BUCKET, ej = e % BUCKET;
-      uint64_t val = EMPTY_KEY;
-      for (uint32_t s = 0; s < k; ++s) {
-        const uint32_t bs = __shfl(mb, (int)s, WAVE), rs = __shfl(rank, (int)s, WAVE);
-        const uint32_t is = __shfl(me, (int)s, WAVE);
-        if (bs == eb && rs == ej) val = pin_key(is);
-      }
-      if (e < S) slots[(size_t)i * S + e] = val;
-    }
-  }
+  for (uint32_t e = l; e < S; e += WAVE) slots[(size_t)i * S + e] = s_pin[e];
 }
 
 // ── local join (nndescent.rs:912-969) ──
@@ -308,38 +301,6 @@ __global__ __launch_bounds__(256) void k_sample_select(
 // proposes the other endpoint to it; proposals are batched through the wave's
 // LDS list. D = 50 fixes the dimension at compile time; D = 0 reads it at run
 // time. Both use the reference's operation order (simd.rs:45-62).
-template <int D>
-__device__ inline float lds_sq_dist(const float* pa, const float* pb, uint32_t d) {
-  if (D > 0) {
-    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int ch = 0; ch < D / 8; ++ch) {
-      const float4 a0 = *reinterpret_cast<const float4*>(pa + ch * 8);
-      const float4 a1 = *reinterpret_cast<const float4*>(pa + ch * 8 + 4);
-      const float4 b0 = *reinterpret_cast<const float4*>(pb + ch * 8);
-      const float4 b1 = *reinterpret_cast<const float4*>(pb + ch * 8 + 4);
-      float t;
-      t = a0.x - b0.x; acc[0] = acc[0] + t * t;
-      t = a0.y - b0.y; acc[1] = acc[1] + t * t;
-      t = a0.z - b0.z; acc[2] = acc[2] + t * t;
-      t = a0.w - b0.w; acc[3] = acc[3] + t * t;
-      t = a1.x - b1.x; acc[4] = acc[4] + t * t;
-      t = a1.y - b1.y; acc[5] = acc[5] + t * t;
-      t = a1.z - b1.z; acc[6] = acc[6] + t * t;
-      t = a1.w - b1.w; acc[7] = acc[7] + t * t;
-    }
-    float tail = 0.f;
-#pragma unroll
-    for (int j = (D / 8) * 8; j < D; ++j) {
-      const float t = pa[j] - pb[j];
-      tail = tail + t * t;
-    }
-    return (((acc[0] + acc[4]) + (acc[1] + acc[5])) + ((acc[2] + acc[6]) + (acc[3] + acc[7]))) +
-           tail;
-  }
-  return sq_dist_ref(pa, pb, (int)d);
-}
-
 template <int D>
 __global__ __launch_bounds__(64) void k_join(
     const float* __restrict__ Xp, uint32_t n, uint32_t d, uint32_t dp, uint32_t RS,
@@ -410,70 +371,93 @@ __global__ __launch_bounds__(64) void k_join(
 }
 
 // ── merge (the apply of nndescent.rs:307-324 as a set function) ──
-// One wave per touched row: top-k of (row ∪ proposals), deduplicated on (dist,
-// idx) keeping the smaller flag (an entry already in the row keeps its flag).
-// Pinned slots are skipped. Word 2 of the striped counters `changed` counts
-// entries of the new row absent from the old.
-template <int NR>
-__global__ __launch_bounds__(256) void k_merge(uint64_t* __restrict__ G, uint32_t n, uint32_t k,
-                                               uint64_t* __restrict__ slots, uint32_t S,
-                                               uint8_t* __restrict__ touched, float* __restrict__ thr,
-                                               int force, unsigned long long* __restrict__ changed) {
-  const int w = threadIdx.x / WAVE;
-  const int l = lane_id();
-  const uint32_t p = blockIdx.x * 4 + w;
+// One wave (one block) per touched row (every row when `force`): the row
+// becomes the top k of (row ∪ proposals). Word 2 of the striped counters
+// `changed` counts entries of the new row absent from the old.
+// Proposals never repeat a neighbour of the row (pinned slots reject those)
+// and are distinct (a neighbour has one bucket, whose cascade stops on equal
+// keys), so when they fit one 64-lane vector the merge is: compact them, sort,
+// and keep the 64 smallest of (row, proposals) with one bitonic merge (the row
+// is sorted, except before the forced first merge, which sorts it). Otherwise
+// every slot and the row are sorted together and deduplicated on (dist, idx),
+// an entry already in the row keeping its flag.
+template <int NR, int NT>
+__global__ __launch_bounds__(64) void k_merge(uint64_t* __restrict__ G, uint32_t n, uint32_t k,
+                                              uint64_t* __restrict__ slots, uint32_t S,
+                                              uint8_t* __restrict__ touched, float* __restrict__ thr,
+                                              int force, unsigned long long* __restrict__ changed) {
+  __shared__ uint64_t s_prop[WAVE];
+  const uint32_t l = (uint32_t)lane_id();
+  const uint32_t p = blockIdx.x;
   if (p >= n) return;
   if (!force && !touched[p]) return;
   uint64_t* row = G + (size_t)p * k;
-  uint64_t* sl = slots + (size_t)p * S;
-  const uint64_t orow = l < (int)k ? row[l] : EMPTY_KEY;
-  uint64_t v[NR];
-#pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    const uint32_t e = r * WAVE + l;
-    if (e < S) {
-      v[r] = bucket_to_row(sl[e]);
-    } else if (e < S + k) {
-      v[r] = row[e - S];
-    } else {
-      v[r] = EMPTY_KEY;
-    }
-  }
+  const uint64_t* sl = slots + (size_t)p * S;
+  const uint64_t orow = l < k ? row[l] : EMPTY_KEY;
   if (l == 0) touched[p] = 0;
-  wave_sort_n<NR>(v);
-  // dedup on (dist, idx): drop an element equal to its predecessor's
-  uint32_t before = 0;
-  uint32_t nkept = 0;
-  uint32_t nchanged = 0;
-  uint64_t lastkey = EMPTY_KEY;
+  uint64_t v[NR];
+  uint32_t total = 0;
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
-    uint64_t prev = shfl64(v[r], (l + WAVE - 1) & (WAVE - 1));
-    if (l == 0) prev = (r > 0) ? lastkey : EMPTY_KEY;
-    const bool has_prev = (r > 0) || (l > 0);
-    const bool keep = v[r] != EMPTY_KEY && !(has_prev && (v[r] >> 1) == (prev >> 1));
-    const uint64_t mask = __ballot(keep);
-    const uint32_t rank = before + __popcll(mask & ((1ull << l) - 1ull));
-    bool found = false;
-    for (uint32_t s2 = 0; s2 < k; ++s2) {
-      uint64_t o = shfl64(orow, (int)s2);
-      found |= (o != EMPTY_KEY && (o >> 1) == (v[r] >> 1));
-    }
-    if (keep && rank < k) {
-      row[rank] = v[r];
-      nchanged += !found;
-      if (rank == k - 1) thr[p] = key_dist(v[r]);
-    }
-    before += __popcll(mask);
-    lastkey = shfl64(v[r], WAVE - 1);
+    v[r] = bucket_to_row(sl[r * WAVE + l]);
+    const bool has = v[r] != EMPTY_KEY;
+    const uint64_t mask = __ballot(has);
+    const uint32_t pos = total + (uint32_t)__popcll(mask & ((1ull << l) - 1ull));
+    if (has && pos < (uint32_t)WAVE) s_prop[pos] = v[r];
+    total += (uint32_t)__popcll(mask);
   }
-  nkept = before < k ? before : k;
-  for (uint32_t pos = nkept + l; pos < k; pos += WAVE) row[pos] = EMPTY_KEY;
-  if (nkept < k && l == 0) thr[p] = __builtin_inff();
-  uint32_t tot = nchanged;
+  __syncthreads();
+  uint32_t nchanged = 0;
+  if (total <= (uint32_t)WAVE) {
+    const uint64_t pv = l < total ? s_prop[l] : EMPTY_KEY;
+    const uint64_t sr = force ? wave_sort64(orow) : orow;
+    const uint64_t mg = wave_merge_keep64(sr, wave_sort64(pv));
+    const uint32_t nall = (uint32_t)__popcll(__ballot(mg != EMPTY_KEY));
+    const uint32_t nk = nall < k ? nall : k;
+    const uint64_t kth = shfl64(mg, (int)(nk > 0 ? nk - 1 : 0));
+    const uint32_t nold =
+        nk > 0 ? (uint32_t)__popcll(__ballot(orow != EMPTY_KEY && orow <= kth)) : 0u;
+    if (l < k) row[l] = l < nk ? mg : EMPTY_KEY;
+    if (l == 0) {
+      thr[p] = nk == k ? key_dist(kth) : __builtin_inff();
+      nchanged = nk - nold;
+    }
+  } else {
+    uint64_t u[NT];
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, WAVE);
-  if (l == 0 && tot) ctr_add(changed, 2, (unsigned long long)tot);
+    for (int r = 0; r < NT; ++r) u[r] = r < NR ? v[r] : (r == NR ? orow : EMPTY_KEY);
+    wave_sort_n<NT>(u);
+    // dedup on (dist, idx): drop an element equal to its predecessor's
+    uint32_t before = 0;
+    uint64_t lastkey = EMPTY_KEY;
+#pragma unroll
+    for (int r = 0; r < NT; ++r) {
+      uint64_t prev = shfl64(u[r], (int)((l + WAVE - 1) & (WAVE - 1)));
+      if (l == 0) prev = (r > 0) ? lastkey : EMPTY_KEY;
+      const bool has_prev = (r > 0) || (l > 0);
+      const bool keep = u[r] != EMPTY_KEY && !(has_prev && (u[r] >> 1) == (prev >> 1));
+      const uint64_t mask = __ballot(keep);
+      const uint32_t rank = before + (uint32_t)__popcll(mask & ((1ull << l) - 1ull));
+      bool found = false;
+      for (uint32_t s2 = 0; s2 < k; ++s2) {
+        const uint64_t o = shfl64(orow, (int)s2);
+        found |= (o != EMPTY_KEY && (o >> 1) == (u[r] >> 1));
+      }
+      if (keep && rank < k) {
+        row[rank] = u[r];
+        nchanged += !found;
+        if (rank == k - 1) thr[p] = key_dist(u[r]);
+      }
+      before += (uint32_t)__popcll(mask);
+      lastkey = shfl64(u[r], WAVE - 1);
+    }
+    const uint32_t nkept = before < k ? before : k;
+    for (uint32_t pos = nkept + l; pos < k; pos += WAVE) row[pos] = EMPTY_KEY;
+    if (nkept < k && l == 0) thr[p] = __builtin_inff();
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) nchanged += __shfl_xor(nchanged, o, WAVE);
+  }
+  if (l == 0 && nchanged) ctr_add(changed, 2, (unsigned long long)nchanged);
 }
 
 // ── extraction (nndescent.rs:977-1002) ──
@@ -511,11 +495,12 @@ template __global__ void k_join<50>(const float*, uint32_t, uint32_t, uint32_t, 
                                     const uint32_t*, const uint32_t*, const uint32_t*, uint32_t,
                                     const float*, uint64_t*, uint8_t*, uint32_t, uint64_t, uint64_t,
                                     unsigned long long*);
-template __global__ void k_merge<2>(uint64_t*, uint32_t, uint32_t, uint64_t*, uint32_t, uint8_t*,
-                                    float*, int, unsigned long long*);
-template __global__ void k_merge<4>(uint64_t*, uint32_t, uint32_t, uint64_t*, uint32_t, uint8_t*,
-                                    float*, int, unsigned long long*);
-template __global__ void k_merge<8>(uint64_t*, uint32_t, uint32_t, uint64_t*, uint32_t, uint8_t*,
-                                    float*, int, unsigned long long*);
+
+#define PSQ_MERGE_INST(NR, NT)                                                                 \
+  template __global__ void k_merge<NR, NT>(uint64_t*, uint32_t, uint32_t, uint64_t*, uint32_t,   \
+                                           uint8_t*, float*, int, unsigned long long*);
+PSQ_MERGE_INST(1, 2)
+PSQ_MERGE_INST(2, 4)
+PSQ_MERGE_INST(4, 8)
 
 }  // namespace psq

@@ -275,6 +275,40 @@ __device__ inline void propose(uint64_t* __restrict__ slots, uint8_t* __restrict
   touched[row] = 1;
 }
 
+// sq_dist of two rows in LDS (16-byte aligned): D = 50 fixed at compile time with
+// vector reads, D = 0 any d. Operation order of simd.rs:45-62.
+template <int D>
+__device__ inline float lds_sq_dist(const float* pa, const float* pb, uint32_t d) {
+  if (D > 0) {
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ch = 0; ch < D / 8; ++ch) {
+      const float4 a0 = *reinterpret_cast<const float4*>(pa + ch * 8);
+      const float4 a1 = *reinterpret_cast<const float4*>(pa + ch * 8 + 4);
+      const float4 b0 = *reinterpret_cast<const float4*>(pb + ch * 8);
+      const float4 b1 = *reinterpret_cast<const float4*>(pb + ch * 8 + 4);
+      float t;
+      t = a0.x - b0.x; acc[0] = acc[0] + t * t;
+      t = a0.y - b0.y; acc[1] = acc[1] + t * t;
+      t = a0.z - b0.z; acc[2] = acc[2] + t * t;
+      t = a0.w - b0.w; acc[3] = acc[3] + t * t;
+      t = a1.x - b1.x; acc[4] = acc[4] + t * t;
+      t = a1.y - b1.y; acc[5] = acc[5] + t * t;
+      t = a1.z - b1.z; acc[6] = acc[6] + t * t;
+      t = a1.w - b1.w; acc[7] = acc[7] + t * t;
+    }
+    float tail = 0.f;
+#pragma unroll
+    for (int j = (D / 8) * 8; j < D; ++j) {
+      const float t = pa[j] - pb[j];
+      tail = tail + t * t;
+    }
+    return (((acc[0] + acc[4]) + (acc[1] + acc[5])) + ((acc[2] + acc[6]) + (acc[3] + acc[7]))) +
+           tail;
+  }
+  return sq_dist_ref(pa, pb, (int)d);
+}
+
 // ── join helpers: pair enumeration and the per-wave proposal list ──
 // Pairs of one cell, numbered t = 0 .. P1 + nn*no - 1: first the new x new
 // upper triangle (a < b < nn) in row order, then new x old (a < nn, b = nn + j).

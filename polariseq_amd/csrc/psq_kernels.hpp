@@ -30,7 +30,7 @@ __global__ void k_join(const float* Xp, uint32_t n, uint32_t d, uint32_t dp, uin
                        const uint32_t* oldl, const uint32_t* ncnt, uint32_t m, const float* thr,
                        uint64_t* slots, uint8_t* touched, uint32_t S, uint64_t seed, uint64_t iter,
                        unsigned long long* counters);
-template <int NR>
+template <int NR, int NT>
 __global__ void k_merge(uint64_t* G, uint32_t n, uint32_t k, uint64_t* slots, uint32_t S,
                         uint8_t* touched, float* thr, int force, unsigned long long* changed);
 __global__ void k_row_counts(const uint64_t* G, uint32_t n, uint32_t k, uint32_t ko, uint32_t* cnt);
@@ -70,14 +70,12 @@ __global__ void k_leaf_merge(const float* Xp, uint32_t d, uint32_t dp, uint32_t 
                              uint64_t* G, uint32_t k);
 
 // precision form (bf16_kernels.hip)
-__global__ void k_join_bf16(const float* Xp, uint32_t n, uint32_t d, uint32_t dp, const uint32_t* newl,
-                            const uint32_t* oldl, const uint32_t* ncnt, uint32_t m, const float* thr,
-                            float slack, uint64_t* slots, uint8_t* touched, uint32_t S, uint64_t seed,
-                            uint64_t iter, unsigned long long* counters);
-template <int NR>
-__global__ void k_merge_rerank(const float* Xp, uint32_t d, uint32_t dp, uint64_t* G, uint32_t n,
-                               uint32_t k, uint32_t rr, uint64_t* slots, uint32_t S, uint8_t* touched,
-                               float* thr, unsigned long long* changed);
+template <int D>
+__global__ void k_join_bf16(const float* Xp, uint32_t n, uint32_t d, uint32_t dp, uint32_t RS,
+                            const uint32_t* newl, const uint32_t* oldl, const uint32_t* ncnt,
+                            uint32_t m, const float* thr, float slack, uint64_t* slots,
+                            uint8_t* touched, uint32_t S, uint64_t seed, uint64_t iter,
+                            unsigned long long* counters);
 
 // learned encoder (mlp_kernels.hip)
 constexpr int MLP_N_PARAMS = 128 * 50 + 128 + 32 * 128 + 32;
