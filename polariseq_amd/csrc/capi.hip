@@ -311,10 +311,11 @@ int launch_leaf_merge(uint32_t KM, uint32_t LG, const float* Xp, uint32_t d, uin
                       uint64_t* G, uint32_t k, hipStream_t s) {
   const unsigned bt = dp <= 128 ? 256 : 64;
   if (bt < LG) return fail(PSQ_E_UNSUPPORTED, "leaf group larger than the block");
-  const size_t smem = (size_t)bt * dp * sizeof(float);
+  const uint32_t RS = join_row_stride(d);
+  const size_t smem = (size_t)bt * (RS + 1) * sizeof(float);
   const unsigned grid = grid_for(nl, bt / LG);
-  void (*kern)(const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, const uint32_t*,
-               const uint32_t*, uint32_t, uint64_t*, uint32_t) = nullptr;
+  void (*kern)(const float*, uint32_t, uint32_t, uint32_t, uint32_t, const uint32_t*,
+               const uint32_t*, const uint32_t*, uint32_t, uint64_t*, uint32_t) = nullptr;
   if (KM == 16 && LG == 16) kern = k_leaf_merge<D, 16, 16>;
   else if (KM == 32 && LG == 16) kern = k_leaf_merge<D, 32, 16>;
   else if (KM == 32 && LG == 32) kern = k_leaf_merge<D, 32, 32>;
@@ -322,7 +323,7 @@ int launch_leaf_merge(uint32_t KM, uint32_t LG, const float* Xp, uint32_t d, uin
   else if (KM == 64 && LG == 32) kern = k_leaf_merge<D, 64, 32>;
   else kern = k_leaf_merge<D, 64, 64>;
   if (smem > 65536) HIPC(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(bt), smem, s, Xp, d, dp, n, ids_t, gs, gc, nl, G, k);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(bt), smem, s, Xp, d, dp, RS, n, ids_t, gs, gc, nl, G, k);
   HIPC(hipGetLastError());
   return PSQ_OK;
 }

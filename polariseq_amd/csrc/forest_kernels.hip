@@ -261,15 +261,44 @@ __global__ void k_leaf_group(const uint32_t* __restrict__ l_tree, const uint32_t
 
 // Leaf pairs of one tree. A group of LG lanes takes one leaf; a leaf of more
 // than LG members is taken in consecutive chunks of LG (pairs within a chunk).
-// Each lane holds its point's row (KMAX keys, sorted) in registers and inserts
-// the keys of its chunk-mates: a key enters when it beats the row's worst and
-// its (dist, idx) is not already present.
+// The group stages its chunk's rows in LDS with coalesced 16-byte loads, at a
+// row stride RS (floats, RS / 4 odd) that puts the group's rows on distinct
+// banks. Each lane keeps its point's row (D = 50: in registers) and its graph
+// row (KMAX keys, sorted) in registers, and inserts the keys of its
+// chunk-mates: a key enters when it beats the row's worst and its (dist, idx)
+// is not already present.
+template <int D>
+__device__ inline float reg_lds_sq_dist(const float (&me)[D > 0 ? D : 1], const float* other) {
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int ch = 0; ch < D / 8; ++ch) {
+    const float4 o0 = *reinterpret_cast<const float4*>(other + ch * 8);
+    const float4 o1 = *reinterpret_cast<const float4*>(other + ch * 8 + 4);
+    float t;
+    t = me[ch * 8 + 0] - o0.x; acc[0] = acc[0] + t * t;
+    t = me[ch * 8 + 1] - o0.y; acc[1] = acc[1] + t * t;
+    t = me[ch * 8 + 2] - o0.z; acc[2] = acc[2] + t * t;
+    t = me[ch * 8 + 3] - o0.w; acc[3] = acc[3] + t * t;
+    t = me[ch * 8 + 4] - o1.x; acc[4] = acc[4] + t * t;
+    t = me[ch * 8 + 5] - o1.y; acc[5] = acc[5] + t * t;
+    t = me[ch * 8 + 6] - o1.z; acc[6] = acc[6] + t * t;
+    t = me[ch * 8 + 7] - o1.w; acc[7] = acc[7] + t * t;
+  }
+  float tail = 0.f;
+#pragma unroll
+  for (int j = (D / 8) * 8; j < D; ++j) {
+    const float t = me[j] - other[j];
+    tail = tail + t * t;
+  }
+  return (((acc[0] + acc[4]) + (acc[1] + acc[5])) + ((acc[2] + acc[6]) + (acc[3] + acc[7]))) + tail;
+}
+
 template <int D, int KMAX, int LG>
 __global__ __launch_bounds__(256) void k_leaf_merge(
-    const float* __restrict__ Xp, uint32_t d, uint32_t dp, uint32_t n,
+    const float* __restrict__ Xp, uint32_t d, uint32_t dp, uint32_t RS, uint32_t n,
     const uint32_t* __restrict__ ids_t, const uint32_t* __restrict__ g_start,
     const uint32_t* __restrict__ g_count, uint32_t nleaves, uint64_t* __restrict__ G, uint32_t k) {
-  extern __shared__ float s_rows[];  // [blockDim.x / LG groups][LG][dp]
+  extern __shared__ __attribute__((aligned(16))) float s_rows[];  // [groups][LG][RS] | ids [groups][LG]
   __shared__ uint32_t s_rounds;
   const uint32_t GPB = blockDim.x / LG;
   const int g = threadIdx.x / LG, j = threadIdx.x % LG;
@@ -282,16 +311,21 @@ __global__ __launch_bounds__(256) void k_leaf_merge(
   if (j == 0 && lvalid) atomicMax(&s_rounds, nch);
   __syncthreads();
   const uint32_t rounds = s_rounds;
-  float* grows = s_rows + (size_t)g * LG * dp;
+  float* grows = s_rows + (size_t)g * LG * RS;
+  uint32_t* gids = reinterpret_cast<uint32_t*>(s_rows + (size_t)GPB * LG * RS) + g * LG;
+  const uint32_t c4 = (d + 3) / 4;
+  constexpr int DR = D > 0 ? D : 1;
   for (uint32_t r = 0; r < rounds; ++r) {
     const uint32_t c0 = r * LG;
     const uint32_t cc = (r < nch) ? min((uint32_t)LG, cnt - c0) : 0u;
     const bool valid = (uint32_t)j < cc;
     const uint32_t p = valid ? ids_t[start + c0 + j] : 0u;
-    if (valid) {
-      const float4* src = reinterpret_cast<const float4*>(Xp + (size_t)p * dp);
-      float4* dst = reinterpret_cast<float4*>(grows + (size_t)j * dp);
-      for (uint32_t e = 0; e < dp / 4; ++e) dst[e] = src[e];
+    gids[j] = p;
+    __syncthreads();
+    for (uint32_t e = j; e < cc * c4; e += LG) {
+      const uint32_t rr = e / c4, c = e - rr * c4;
+      const float4 v = *reinterpret_cast<const float4*>(Xp + (size_t)gids[rr] * dp + 4 * c);
+      *reinterpret_cast<float4*>(grows + (size_t)rr * RS + 4 * c) = v;
     }
     __syncthreads();
     if (valid) {
@@ -299,12 +333,25 @@ __global__ __launch_bounds__(256) void k_leaf_merge(
       uint64_t* gr = G + (size_t)p * k;
 #pragma unroll
       for (int s = 0; s < KMAX; ++s) row[s] = (uint32_t)s < k ? gr[s] : EMPTY_KEY;
-      const float* me = grows + (size_t)j * dp;
+      const float* mine = grows + (size_t)j * RS;
+      float me[DR];
+      if (D > 0) {
+#pragma unroll
+        for (int t = 0; t < DR / 4; ++t) {
+          const float4 v = *reinterpret_cast<const float4*>(mine + 4 * t);
+          me[4 * t] = v.x;
+          me[4 * t + 1] = v.y;
+          me[4 * t + 2] = v.z;
+          me[4 * t + 3] = v.w;
+        }
+#pragma unroll
+        for (int t = (DR / 4) * 4; t < DR; ++t) me[t] = mine[t];
+      }
       for (uint32_t q = 0; q < cc; ++q) {
         if (q == (uint32_t)j) continue;
-        const float* other = grows + (size_t)q * dp;
-        const float dist = D > 0 ? sq_dist_fixed<(D > 0 ? D : 1)>(me, other) : sq_dist_ref(me, other, (int)d);
-        const uint32_t qi = ids_t[start + c0 + q];
+        const float* other = grows + (size_t)q * RS;
+        const float dist = D > 0 ? reg_lds_sq_dist<D>(me, other) : sq_dist_ref(mine, other, (int)d);
+        const uint32_t qi = gids[q];
         const uint64_t key = make_key(dist, qi, 1);
         uint64_t worst = EMPTY_KEY;
 #pragma unroll
@@ -334,7 +381,7 @@ __global__ __launch_bounds__(256) void k_leaf_merge(
 
 #define PSQ_LEAF_INST(D, KM, LG)                                                              \
   template __global__ void k_leaf_merge<D, KM, LG>(const float*, uint32_t, uint32_t, uint32_t, \
-                                                   const uint32_t*, const uint32_t*,          \
+                                                   uint32_t, const uint32_t*, const uint32_t*, \
                                                    const uint32_t*, uint32_t, uint64_t*, uint32_t);
 PSQ_LEAF_INST(0, 16, 16)
 PSQ_LEAF_INST(0, 32, 16)

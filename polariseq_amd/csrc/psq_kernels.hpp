@@ -65,9 +65,9 @@ __global__ void k_leaf_hist(const uint32_t* l_tree, uint32_t nl, uint32_t* tcoun
 __global__ void k_leaf_group(const uint32_t* l_tree, const uint32_t* l_start, const uint32_t* l_count,
                              uint32_t nl, uint32_t* tcursor, uint32_t* g_start, uint32_t* g_count);
 template <int D, int KMAX, int LG>
-__global__ void k_leaf_merge(const float* Xp, uint32_t d, uint32_t dp, uint32_t n, const uint32_t* ids_t,
-                             const uint32_t* g_start, const uint32_t* g_count, uint32_t nleaves,
-                             uint64_t* G, uint32_t k);
+__global__ void k_leaf_merge(const float* Xp, uint32_t d, uint32_t dp, uint32_t RS, uint32_t n,
+                             const uint32_t* ids_t, const uint32_t* g_start, const uint32_t* g_count,
+                             uint32_t nleaves, uint64_t* G, uint32_t k);
 
 // precision form (bf16_kernels.hip)
 template <int D>
