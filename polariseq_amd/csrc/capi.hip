@@ -174,9 +174,9 @@ int scan_u32(Workspace* ws, const uint32_t* in, uint32_t* out, uint32_t n, hipSt
 
 uint32_t round_up(uint32_t v, uint32_t m) { return (v + m - 1) / m * m; }
 
-// BF16 join: a proposal is kept when its approximate distance is below the
-// destination's threshold times this factor (the approximation error is a few
-// parts per thousand on centred vectors; the merge re-ranks exactly).
+// BF16 join: a pair passes the screen when its approximate distance is below an
+// endpoint's threshold times this factor (the approximation error is a few parts
+// per thousand on centred vectors); passing pairs are re-ranked exactly.
 constexpr float BF16_SLACK = 1.03f;
 constexpr uint32_t BF16_SV_CAP = 320;  // screened-pair list of k_join_bf16
 
@@ -329,8 +329,10 @@ int launch_leaf_merge(uint32_t KM, uint32_t LG, const float* Xp, uint32_t d, uin
 }
 
 // RP forest start: trees built level by level, a batch of trees at a time (so
-// every position index fits in 32 bits), then the leaf pairs of each tree are
-// merged into the rows, tree after tree (nndescent.rs:767-817).
+// every position index fits in 32 bits). A subtree of at most `cap` points is
+// finished in LDS by one block (k_forest_finish). Leaves are recorded as counts
+// at their start positions, compacted per tree, and the leaf pairs of each tree
+// are merged into the rows, tree after tree (nndescent.rs:767-817).
 // Xs (ds dims, row stride dps) is what the hyperplanes split: the embedding
 // itself, or the learned codes. Leaf pairs are measured on Xp (the embedding).
 int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds, uint32_t dps,
@@ -342,6 +344,17 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
   const uint32_t LG = leaf_group(ls);
   uint32_t KM = k <= 16 ? 16 : (k <= 32 ? 32 : 64);
   if (LG > KM) KM = LG;
+  // LDS subtree size: rows at stride RSs within 52 KB (two blocks fit a CU's
+  // LDS), at most 512 points
+  const uint32_t RSs = join_row_stride(ds);
+  uint32_t cap = std::min<uint32_t>(512, (52u * 1024u) / (RSs * 4u)) / 32u * 32u;
+  if (cap <= 2 * ls) cap = 0;  // too small to pay off: every level runs globally
+  const uint32_t NC = cap / (ls + 1) + 2;
+  const size_t fin_smem = ((size_t)cap * RSs + (size_t)NC * RSs + NC) * 4 +
+                          ((size_t)7 * cap + 2 + (size_t)8 * NC) * 4;
+  if (cap > 0 && fin_smem > 65536)
+    HIPC(hipFuncSetAttribute((const void*)k_forest_finish,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)fin_smem));
   std::vector<uint64_t> hseeds(T);
   for (uint64_t t = 0; t < T; ++t) hseeds[t] = splitmix(R.seed ^ (t + 1));
   uint64_t* tseeds;
@@ -349,7 +362,7 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
   HIPC(hipMemcpyAsync(tseeds, hseeds.data(), T * sizeof(uint64_t), hipMemcpyHostToDevice, s));
   const size_t total_cap = (size_t)TB * n;
   const size_t node_cap = (size_t)TB * (n / (ls + 1) + 1);
-  uint32_t *ids[2], *segv[2], *fl, *fr, *sl, *sr;
+  uint32_t *ids[2], *segv[2], *fl, *fr, *sl, *sr, *lcnt_pos;
   TRY(wbuf(ws, "f_ids0", total_cap, &ids[0]));
   TRY(wbuf(ws, "f_ids1", total_cap, &ids[1]));
   TRY(wbuf(ws, "f_seg0", total_cap, &segv[0]));
@@ -358,7 +371,9 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
   TRY(wbuf(ws, "f_fr", total_cap, &fr));
   TRY(wbuf(ws, "f_sl", total_cap + 1, &sl));
   TRY(wbuf(ws, "f_sr", total_cap + 1, &sr));
-  uint32_t *ns[2], *nc[2], *nt[2], *nlv, *mode, *nact, *nleaf, *aoff, *loff, *child;
+  TRY(wbuf(ws, "f_lcnt", total_cap, &lcnt_pos));
+  uint32_t *ns[2], *nc[2], *nt[2], *nlv, *mode, *nact, *nsm, *aoff, *soff, *child;
+  uint32_t *sm_start, *sm_count, *sm_tree, *sm_level, *toff;
   float *normals, *offsets;
   TRY(wbuf(ws, "f_ns0", node_cap, &ns[0]));
   TRY(wbuf(ws, "f_ns1", node_cap, &ns[1]));
@@ -369,20 +384,21 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
   TRY(wbuf(ws, "f_nl", node_cap, &nlv));
   TRY(wbuf(ws, "f_mode", node_cap, &mode));
   TRY(wbuf(ws, "f_nact", node_cap, &nact));
-  TRY(wbuf(ws, "f_nleaf", node_cap, &nleaf));
+  TRY(wbuf(ws, "f_nsm", node_cap, &nsm));
   TRY(wbuf(ws, "f_aoff", node_cap + 1, &aoff));
-  TRY(wbuf(ws, "f_loff", node_cap + 1, &loff));
+  TRY(wbuf(ws, "f_soff", node_cap + 1, &soff));
   TRY(wbuf(ws, "f_child", 2 * node_cap, &child));
   TRY(wbuf(ws, "f_normals", node_cap * dps, &normals));
   TRY(wbuf(ws, "f_offsets", node_cap, &offsets));
+  TRY(wbuf(ws, "f_sm_start", node_cap, &sm_start));
+  TRY(wbuf(ws, "f_sm_count", node_cap, &sm_count));
+  TRY(wbuf(ws, "f_sm_tree", node_cap, &sm_tree));
+  TRY(wbuf(ws, "f_sm_level", node_cap, &sm_level));
+  TRY(wbuf(ws, "f_toff", TB + 1, &toff));
   // leaves: each leaf holds at least one point, so a batch has at most TB * n
-  uint32_t *lt, *lsrt, *lcnt, *gs, *gc, *tcount;
-  TRY(wbuf(ws, "f_lt", total_cap, &lt));
-  TRY(wbuf(ws, "f_ls", total_cap, &lsrt));
-  TRY(wbuf(ws, "f_lc", total_cap, &lcnt));
+  uint32_t *gs, *gc;
   TRY(wbuf(ws, "f_gs", total_cap, &gs));
   TRY(wbuf(ws, "f_gc", total_cap, &gc));
-  TRY(wbuf(ws, "f_tcount", TB, &tcount));
 
   for (uint64_t t0 = 0; t0 < T; t0 += TB) {
     const uint32_t tb = (uint32_t)std::min<uint64_t>(TB, T - t0);
@@ -390,19 +406,25 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
     hipLaunchKernelGGL(k_forest_init, dim3(grid_for(total, 256)), dim3(256), 0, s, ids[0], segv[0],
                        n, tb, ls);
     HIPC(hipGetLastError());
-    std::vector<uint32_t> h0s(tb, 0), h0c(tb, n), h0t(tb);
+    TRY(fill_u32(lcnt_pos, total, 0, s));
+    std::vector<uint32_t> h0s(tb, 0), h0c(tb, n), h0t(tb), h0l(tb, 0);
     for (uint32_t t = 0; t < tb; ++t) h0t[t] = t;
-    uint32_t A = 0, leaf_base = 0;
-    if (n > ls) {
+    uint32_t A = 0, small_base = 0;
+    if (n <= ls) {
+      // every tree is one leaf
+      for (uint32_t t = 0; t < tb; ++t)
+        HIPC(hipMemcpyAsync(lcnt_pos + (size_t)t * n, &h0c[t], 4, hipMemcpyHostToDevice, s));
+    } else if (n <= cap) {
+      HIPC(hipMemcpyAsync(sm_start, h0s.data(), tb * 4, hipMemcpyHostToDevice, s));
+      HIPC(hipMemcpyAsync(sm_count, h0c.data(), tb * 4, hipMemcpyHostToDevice, s));
+      HIPC(hipMemcpyAsync(sm_tree, h0t.data(), tb * 4, hipMemcpyHostToDevice, s));
+      HIPC(hipMemcpyAsync(sm_level, h0l.data(), tb * 4, hipMemcpyHostToDevice, s));
+      small_base = tb;
+    } else {
       A = tb;
       HIPC(hipMemcpyAsync(ns[0], h0s.data(), tb * 4, hipMemcpyHostToDevice, s));
       HIPC(hipMemcpyAsync(nc[0], h0c.data(), tb * 4, hipMemcpyHostToDevice, s));
       HIPC(hipMemcpyAsync(nt[0], h0t.data(), tb * 4, hipMemcpyHostToDevice, s));
-    } else {
-      HIPC(hipMemcpyAsync(lt, h0t.data(), tb * 4, hipMemcpyHostToDevice, s));
-      HIPC(hipMemcpyAsync(lsrt, h0s.data(), tb * 4, hipMemcpyHostToDevice, s));
-      HIPC(hipMemcpyAsync(lcnt, h0c.data(), tb * 4, hipMemcpyHostToDevice, s));
-      leaf_base = tb;
     }
     int cur = 0;
     for (uint32_t L = 0; A > 0; ++L) {
@@ -416,20 +438,20 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
       TRY(scan_u32(ws, fl, sl, (uint32_t)total, s));
       TRY(scan_u32(ws, fr, sr, (uint32_t)total, s));
       hipLaunchKernelGGL(k_forest_nodes, dim3(grid_for(A, 256)), dim3(256), 0, s, ns[cur], nc[cur],
-                         nt[cur], A, n, sl, sr, ls, L, nlv, mode, nact, nleaf);
+                         nt[cur], A, n, sl, sr, ls, cap, L, nlv, mode, nact, nsm);
       HIPC(hipGetLastError());
       TRY(scan_u32(ws, nact, aoff, A, s));
-      TRY(scan_u32(ws, nleaf, loff, A, s));
+      TRY(scan_u32(ws, nsm, soff, A, s));
       HIPC(hipMemcpyAsync(hbuf, aoff + A, 4, hipMemcpyDeviceToHost, s));
-      HIPC(hipMemcpyAsync(hbuf + 1, loff + A, 4, hipMemcpyDeviceToHost, s));
+      HIPC(hipMemcpyAsync(hbuf + 1, soff + A, 4, hipMemcpyDeviceToHost, s));
       HIPC(hipStreamSynchronize(s));
-      const uint32_t nextA = hbuf[0], newleaves = hbuf[1];
-      if ((size_t)nextA > node_cap || (size_t)leaf_base + newleaves > total_cap)
-        return fail(PSQ_E_DEVICE, "forest bookkeeping overflow (nodes %u, leaves %u)", nextA,
-                    leaf_base + newleaves);
+      const uint32_t nextA = hbuf[0], newsmall = hbuf[1];
+      if ((size_t)nextA > node_cap || (size_t)small_base + newsmall > node_cap)
+        return fail(PSQ_E_DEVICE, "forest bookkeeping overflow (nodes %u, subtrees %u)", nextA,
+                    small_base + newsmall);
       hipLaunchKernelGGL(k_forest_emit, dim3(grid_for(A, 256)), dim3(256), 0, s, ns[cur], nc[cur],
-                         nt[cur], A, nlv, mode, aoff, loff, ls, L, ns[nx], nc[nx], nt[nx], child, lt,
-                         lsrt, lcnt, leaf_base);
+                         nt[cur], A, n, nlv, mode, aoff, soff, ls, cap, L, ns[nx], nc[nx], nt[nx],
+                         child, lcnt_pos, small_base, sm_start, sm_count, sm_tree, sm_level);
       HIPC(hipGetLastError());
       hipLaunchKernelGGL(k_forest_scatter, dim3(grid_for(total, 256)), dim3(256), 0, s, total, n,
                          ids[cur], segv[cur], ns[cur], nt[cur], nlv, mode, child, sl, sr, ids[nx],
@@ -437,29 +459,35 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
       HIPC(hipGetLastError());
       cur = nx;
       A = nextA;
-      leaf_base += newleaves;
+      small_base += newsmall;
     }
-    // group leaves by tree
-    TRY(fill_u32(tcount, tb, 0, s));
-    hipLaunchKernelGGL(k_leaf_hist, dim3(grid_for(leaf_base, 256)), dim3(256), 0, s, lt, leaf_base,
-                       tcount);
+    if (small_base > 0) {
+      hipLaunchKernelGGL(k_forest_finish, dim3(small_base), dim3(256), fin_smem, s, Xs, ds, dps,
+                         RSs, n, ids[cur], sm_start, sm_count, sm_tree, sm_level, tseeds + t0, ls,
+                         cap, NC, lcnt_pos);
+      HIPC(hipGetLastError());
+    }
+    // leaf list, grouped by tree
+    hipLaunchKernelGGL(k_leaf_flags, dim3(grid_for(total, 256)), dim3(256), 0, s, lcnt_pos, total, fl);
     HIPC(hipGetLastError());
-    std::vector<uint32_t> hc(tb), hoff(tb + 1, 0);
-    HIPC(hipMemcpyAsync(hc.data(), tcount, tb * 4, hipMemcpyDeviceToHost, s));
+    TRY(scan_u32(ws, fl, sl, (uint32_t)total, s));
+    hipLaunchKernelGGL(k_leaf_compact, dim3(grid_for(total, 256)), dim3(256), 0, s, lcnt_pos, total,
+                       n, sl, gs, gc);
+    HIPC(hipGetLastError());
+    hipLaunchKernelGGL(k_tree_offsets, dim3(grid_for(tb + 1, 64)), dim3(64), 0, s, sl, n, tb, toff);
+    HIPC(hipGetLastError());
+    std::vector<uint32_t> hoff(tb + 1, 0);
+    HIPC(hipMemcpyAsync(hoff.data(), toff, (tb + 1) * 4, hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
-    for (uint32_t t = 0; t < tb; ++t) hoff[t + 1] = hoff[t] + hc[t];
-    HIPC(hipMemcpyAsync(tcount, hoff.data(), tb * 4, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_leaf_group, dim3(grid_for(leaf_base, 256)), dim3(256), 0, s, lt, lsrt,
-                       lcnt, leaf_base, tcount, gs, gc);
-    HIPC(hipGetLastError());
     // leaf pairs, tree after tree
     for (uint32_t t = 0; t < tb; ++t) {
-      if (hc[t] == 0) continue;
+      const uint32_t nl = hoff[t + 1] - hoff[t];
+      if (nl == 0) continue;
       const uint32_t* ids_t = ids[cur] + (size_t)t * n;
       if (d == 50)
-        TRY(launch_leaf_merge<50>(KM, LG, Xp, d, dp, n, ids_t, gs + hoff[t], gc + hoff[t], hc[t], G, k, s));
+        TRY(launch_leaf_merge<50>(KM, LG, Xp, d, dp, n, ids_t, gs + hoff[t], gc + hoff[t], nl, G, k, s));
       else
-        TRY(launch_leaf_merge<0>(KM, LG, Xp, d, dp, n, ids_t, gs + hoff[t], gc + hoff[t], hc[t], G, k, s));
+        TRY(launch_leaf_merge<0>(KM, LG, Xp, d, dp, n, ids_t, gs + hoff[t], gc + hoff[t], nl, G, k, s));
     }
   }
   return PSQ_OK;

@@ -92,12 +92,14 @@ __global__ void k_forest_sides(const float* __restrict__ Xp, uint32_t d, uint32_
 
 // One thread per active node: split mode and how many children it emits.
 // mode 0 = hyperplane, 1 = hash bits, 2 = no split (the node is a leaf).
+// A child is a leaf (at most leaf_size points, or past depth 100), small (at
+// most cap points: finished in LDS by k_forest_finish) or active.
 __global__ void k_forest_nodes(const uint32_t* __restrict__ nstart, const uint32_t* __restrict__ ncount,
                                const uint32_t* __restrict__ ntree, uint32_t A, uint32_t n,
                                const uint32_t* __restrict__ sl, const uint32_t* __restrict__ sr,
-                               uint32_t leaf_size, uint32_t level, uint32_t* __restrict__ nl_out,
-                               uint32_t* __restrict__ mode_out, uint32_t* __restrict__ nact,
-                               uint32_t* __restrict__ nleaf) {
+                               uint32_t leaf_size, uint32_t cap, uint32_t level,
+                               uint32_t* __restrict__ nl_out, uint32_t* __restrict__ mode_out,
+                               uint32_t* __restrict__ nact, uint32_t* __restrict__ nsmall) {
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= A) return;
   const size_t g0 = (size_t)ntree[v] * n + nstart[v];
@@ -119,35 +121,41 @@ __global__ void k_forest_nodes(const uint32_t* __restrict__ nstart, const uint32
   nl_out[v] = nl;
   if (mode == 2) {
     nact[v] = 0;
-    nleaf[v] = 1;
+    nsmall[v] = 0;
     return;
   }
   const bool child_deep = level + 1 > 100;
-  const bool lleaf = child_deep || nl <= leaf_size;
-  const bool rleaf = child_deep || (c - nl) <= leaf_size;
-  nact[v] = (lleaf ? 0u : 1u) + (rleaf ? 0u : 1u);
-  nleaf[v] = (lleaf ? 1u : 0u) + (rleaf ? 1u : 0u);
+  uint32_t a = 0, sm = 0;
+  const uint32_t cc[2] = {nl, c - nl};
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    if (child_deep || cc[s] <= leaf_size) continue;
+    if (cc[s] <= cap) ++sm;
+    else ++a;
+  }
+  nact[v] = a;
+  nsmall[v] = sm;
 }
 
-// One thread per active node: next-level nodes, leaves and the child map.
+// One thread per active node: next-level nodes, small subtrees, leaves (the
+// count at the leaf's start position) and the child map.
 __global__ void k_forest_emit(const uint32_t* __restrict__ nstart, const uint32_t* __restrict__ ncount,
-                              const uint32_t* __restrict__ ntree, uint32_t A,
+                              const uint32_t* __restrict__ ntree, uint32_t A, uint32_t n,
                               const uint32_t* __restrict__ nl_in, const uint32_t* __restrict__ mode_in,
-                              const uint32_t* __restrict__ act_off, const uint32_t* __restrict__ leaf_off,
-                              uint32_t leaf_size, uint32_t level, uint32_t* __restrict__ x_start,
-                              uint32_t* __restrict__ x_count, uint32_t* __restrict__ x_tree,
-                              uint32_t* __restrict__ child, uint32_t* __restrict__ l_tree,
-                              uint32_t* __restrict__ l_start, uint32_t* __restrict__ l_count,
-                              uint32_t leaf_base) {
+                              const uint32_t* __restrict__ act_off, const uint32_t* __restrict__ small_off,
+                              uint32_t leaf_size, uint32_t cap, uint32_t level,
+                              uint32_t* __restrict__ x_start, uint32_t* __restrict__ x_count,
+                              uint32_t* __restrict__ x_tree, uint32_t* __restrict__ child,
+                              uint32_t* __restrict__ lcnt_pos, uint32_t small_base,
+                              uint32_t* __restrict__ sm_start, uint32_t* __restrict__ sm_count,
+                              uint32_t* __restrict__ sm_tree, uint32_t* __restrict__ sm_level) {
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= A) return;
   const uint32_t start = nstart[v], c = ncount[v], t = ntree[v];
   const uint32_t mode = mode_in[v], nl = nl_in[v];
-  uint32_t ao = act_off[v], lo = leaf_base + leaf_off[v];
+  uint32_t ao = act_off[v], so = small_base + small_off[v];
   if (mode == 2) {
-    l_tree[lo] = t;
-    l_start[lo] = start;
-    l_count[lo] = c;
+    lcnt_pos[(size_t)t * n + start] = c;
     child[2 * v] = NONE;
     child[2 * v + 1] = NONE;
     return;
@@ -158,10 +166,14 @@ __global__ void k_forest_emit(const uint32_t* __restrict__ nstart, const uint32_
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
     if (child_deep || cc[s] <= leaf_size) {
-      l_tree[lo] = t;
-      l_start[lo] = cs[s];
-      l_count[lo] = cc[s];
-      ++lo;
+      lcnt_pos[(size_t)t * n + cs[s]] = cc[s];
+      child[2 * v + s] = NONE;
+    } else if (cc[s] <= cap) {
+      sm_start[so] = cs[s];
+      sm_count[so] = cc[s];
+      sm_tree[so] = t;
+      sm_level[so] = level + 1;
+      ++so;
       child[2 * v + s] = NONE;
     } else {
       x_start[ao] = cs[s];
@@ -213,50 +225,219 @@ __global__ void k_forest_scatter(size_t total, uint32_t n, const uint32_t* __res
   segv_out[np] = ch;
 }
 
-// Leaves grouped by tree (order inside a tree is irrelevant: its leaves are
-// disjoint and each merge is a set function). Leaves are emitted level by level
-// in node order, and nodes stay ordered by tree, so a wave's leaves form a few
-// runs of one tree: one atomic per run, not per leaf.
-__device__ inline void leaf_runs(const uint32_t* __restrict__ l_tree, uint32_t nl, uint32_t i,
-                                 uint32_t& t, bool& head, uint32_t& run_len, int& head_lane) {
-  const int l = lane_id();
-  const bool valid = i < nl;
-  t = valid ? l_tree[i] : NONE;
-  const uint32_t prev = (uint32_t)__shfl((int)t, (l + WAVE - 1) & (WAVE - 1), WAVE);
-  head = valid && (l == 0 || prev != t);
-  const uint64_t hm = __ballot(head);
-  const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));  // valid lanes are a prefix
-  const uint64_t above = hm & ~((2ull << l) - 1ull);            // heads after lane l
-  const uint32_t end = above ? (uint32_t)__builtin_ctzll(above) : nvalid;
-  run_len = head ? end - (uint32_t)l : 0u;
-  const uint64_t upto = hm & ((2ull << l) - 1ull);               // heads at or before lane l
-  head_lane = upto ? 63 - __builtin_clzll(upto) : 0;
+// In-block exclusive scan of a[0..c) (c <= 2 * blockDim.x = 512), total in a[c].
+// blockDim.x == 256; wsum: 4 words of LDS.
+__device__ inline void block_scan_512(uint32_t* a, uint32_t c, uint32_t* wsum) {
+  const uint32_t t = threadIdx.x, l = (uint32_t)lane_id(), w = t / WAVE;
+  const uint32_t v0 = 2 * t < c ? a[2 * t] : 0u, v1 = 2 * t + 1 < c ? a[2 * t + 1] : 0u;
+  const uint32_t s = v0 + v1;
+  uint32_t x = s;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, WAVE);
+    if ((int)l >= o) x += y;
+  }
+  if (l == WAVE - 1) wsum[w] = x;
+  __syncthreads();
+  uint32_t wpre = 0;
+  for (uint32_t j = 0; j < w; ++j) wpre += wsum[j];
+  const uint32_t excl = wpre + x - s;
+  if (2 * t < c) a[2 * t] = excl;
+  if (2 * t + 1 < c) a[2 * t + 1] = excl + v0;
+  if (t == blockDim.x - 1) a[c] = wpre + x;
+  __syncthreads();
 }
-__global__ void k_leaf_hist(const uint32_t* __restrict__ l_tree, uint32_t nl,
-                            uint32_t* __restrict__ tcount) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t t, run_len;
-  bool head;
-  int hl;
-  leaf_runs(l_tree, nl, i, t, head, run_len, hl);
-  if (head) atomicAdd(&tcount[t], run_len);
+
+// Subtrees of at most C points are finished by one block each (256 threads):
+// their points' split rows are loaded into LDS once, then every remaining level
+// of the subtree runs there with the steps of the level kernels above (same
+// hashes on the global segment start, hyperplanes, margins, tie and fallback
+// rules, stable partition), so the tree equals the one the level kernels would
+// build. Leaves are written as counts at their start positions; the final
+// order of the subtree's positions is written back to ids.
+// LDS: rows[C][RSs] | normals[NC][RSs] | offsets[NC] | gid[C] | lid[2][C] |
+// pn[2][C] | scan_l[C+1] | scan_r[C+1] | nst[2][NC] | ncn[2][NC] | mode[NC] |
+// nl[NC] | child[2 NC]. NC bounds the active nodes of one level (C / (leaf_size
+// + 1) + 1).
+__global__ __launch_bounds__(256) void k_forest_finish(
+    const float* __restrict__ Xs, uint32_t ds, uint32_t dps, uint32_t RSs, uint32_t n,
+    uint32_t* __restrict__ ids, const uint32_t* __restrict__ sm_start,
+    const uint32_t* __restrict__ sm_count, const uint32_t* __restrict__ sm_tree,
+    const uint32_t* __restrict__ sm_level, const uint64_t* __restrict__ tseeds, uint32_t leaf_size,
+    uint32_t C, uint32_t NC, uint32_t* __restrict__ lcnt_pos) {
+  extern __shared__ __attribute__((aligned(16))) float lds[];
+  __shared__ uint32_t s_nA, s_wsum[4];
+  float* rows = lds;
+  float* normals = rows + (size_t)C * RSs;
+  float* offs = normals + (size_t)NC * RSs;
+  uint32_t* gid = reinterpret_cast<uint32_t*>(offs + NC);
+  uint32_t* lid[2] = {gid + C, gid + 2 * C};
+  uint32_t* pn[2] = {gid + 3 * C, gid + 4 * C};
+  uint32_t* scl = gid + 5 * C;
+  uint32_t* scr = scl + C + 1;
+  uint32_t* nst[2] = {scr + C + 1, scr + C + 1 + NC};
+  uint32_t* ncn[2] = {nst[1] + NC, nst[1] + 2 * NC};
+  uint32_t* nmode = ncn[1] + NC;
+  uint32_t* nnl = nmode + NC;
+  uint32_t* child = nnl + NC;
+  const uint32_t tid = threadIdx.x, w = tid / WAVE, l = (uint32_t)lane_id();
+  const uint32_t b = blockIdx.x;
+  const uint32_t start = sm_start[b], c = sm_count[b], t = sm_tree[b], L0 = sm_level[b];
+  const uint64_t ts = tseeds[t];
+  uint32_t* tids = ids + (size_t)t * n + start;
+  uint32_t* tl = lcnt_pos + (size_t)t * n + start;
+  for (uint32_t e = tid; e < c; e += blockDim.x) {
+    gid[e] = tids[e];
+    lid[0][e] = e;
+    pn[0][e] = 0;
+  }
+  if (tid == 0) {
+    nst[0][0] = 0;
+    ncn[0][0] = c;
+  }
+  __syncthreads();
+  const uint32_t c4 = (ds + 3) / 4;
+  for (uint32_t e = tid; e < c * c4; e += blockDim.x) {
+    const uint32_t r = e / c4, ch = e - r * c4;
+    const float4 v = *reinterpret_cast<const float4*>(Xs + (size_t)gid[r] * dps + 4 * ch);
+    *reinterpret_cast<float4*>(rows + (size_t)r * RSs + 4 * ch) = v;
+  }
+  __syncthreads();
+  uint32_t A = 1;
+  int cur = 0;
+  for (uint32_t L = L0; A > 0; ++L) {
+    const int nx = cur ^ 1;
+    // hyperplanes, one wave per node
+    for (uint32_t v = w; v < A; v += blockDim.x / WAVE) {
+      const uint32_t s0 = nst[cur][v], cv = ncn[cur][v], gs0 = start + s0;
+      const uint32_t* ln = lid[cur] + s0;
+      const uint32_t ea = ln[tree_hash(ts, L, gs0, 0) % cv];
+      uint32_t eb = ln[tree_hash(ts, L, gs0, 1) % cv];
+      for (uint64_t r = 2; eb == ea && r < 10; ++r) eb = ln[tree_hash(ts, L, gs0, r) % cv];
+      const float* xa = rows + (size_t)ea * RSs;
+      const float* xb = rows + (size_t)eb * RSs;
+      float* nv = normals + (size_t)v * RSs;
+      for (uint32_t j = l; j < RSs; j += WAVE) nv[j] = j < ds ? xa[j] - xb[j] : 0.0f;
+      if (l == 0) {
+        float sacc = -0.0f;
+        for (uint32_t j = 0; j < ds; ++j) {
+          const float nj = xa[j] - xb[j];
+          sacc = sacc + nj * 0.5f * (xa[j] + xb[j]);
+        }
+        offs[v] = -sacc;
+      }
+    }
+    __syncthreads();
+    // sides
+    for (uint32_t pos = tid; pos < c; pos += blockDim.x) {
+      const uint32_t v = pn[cur][pos];
+      uint32_t fl = 0, fr = 0;
+      if (v != NONE) {
+        const uint32_t e = lid[cur][pos], g = gid[e];
+        const float margin = dot_ref(normals + (size_t)v * RSs, rows + (size_t)e * RSs, (int)ds) + offs[v];
+        bool left;
+        if (fabsf(margin) < 1e-8f)
+          left = (tree_hash(ts, L, g, 2) & 1) == 0;
+        else
+          left = margin > 0.0f;
+        fl = left ? 1u : 0u;
+        fr = (tree_hash(ts, L, g, 3) & 1) == 0 ? 1u : 0u;
+      }
+      scl[pos] = fl;
+      scr[pos] = fr;
+    }
+    __syncthreads();
+    block_scan_512(scl, c, s_wsum);
+    block_scan_512(scr, c, s_wsum);
+    if (tid == 0) s_nA = 0;
+    __syncthreads();
+    // split decisions, leaves and next-level nodes
+    for (uint32_t v = tid; v < A; v += blockDim.x) {
+      const uint32_t s0 = nst[cur][v], cv = ncn[cur][v];
+      const uint32_t cl = scl[s0 + cv] - scl[s0], rl = scr[s0 + cv] - scr[s0];
+      uint32_t mode, nl;
+      if (cl != 0 && cl != cv) {
+        mode = 0;
+        nl = cl;
+      } else if (rl != 0 && rl != cv) {
+        mode = 1;
+        nl = rl;
+      } else {
+        mode = 2;
+        nl = cv;
+      }
+      nmode[v] = mode;
+      nnl[v] = nl;
+      if (mode == 2) {
+        tl[s0] = cv;
+        child[2 * v] = NONE;
+        child[2 * v + 1] = NONE;
+        continue;
+      }
+      const bool deep = L + 1 > 100;
+      const uint32_t cs[2] = {s0, s0 + nl};
+      const uint32_t cc[2] = {nl, cv - nl};
+      for (int s = 0; s < 2; ++s) {
+        if (deep || cc[s] <= leaf_size) {
+          tl[cs[s]] = cc[s];
+          child[2 * v + s] = NONE;
+        } else {
+          const uint32_t x = atomicAdd(&s_nA, 1u);
+          nst[nx][x] = cs[s];
+          ncn[nx][x] = cc[s];
+          child[2 * v + s] = x;
+        }
+      }
+    }
+    __syncthreads();
+    // stable partition
+    for (uint32_t pos = tid; pos < c; pos += blockDim.x) {
+      const uint32_t v = pn[cur][pos], e = lid[cur][pos];
+      if (v == NONE || nmode[v] == 2) {
+        lid[nx][pos] = e;
+        pn[nx][pos] = NONE;
+        continue;
+      }
+      const uint32_t* X = nmode[v] == 0 ? scl : scr;
+      const uint32_t s0 = nst[cur][v];
+      const uint32_t rank = X[pos] - X[s0];
+      const bool left = (X[pos + 1] - X[pos]) != 0;
+      const uint32_t np = left ? s0 + rank : s0 + nnl[v] + ((pos - s0) - rank);
+      lid[nx][np] = e;
+      pn[nx][np] = child[2 * v + (left ? 0 : 1)];
+    }
+    __syncthreads();
+    A = s_nA;
+    cur = nx;
+    __syncthreads();
+  }
+  for (uint32_t pos = tid; pos < c; pos += blockDim.x) tids[pos] = gid[lid[cur][pos]];
 }
-__global__ void k_leaf_group(const uint32_t* __restrict__ l_tree, const uint32_t* __restrict__ l_start,
-                             const uint32_t* __restrict__ l_count, uint32_t nl,
-                             uint32_t* __restrict__ tcursor, uint32_t* __restrict__ g_start,
-                             uint32_t* __restrict__ g_count) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t t, run_len;
-  bool head;
-  int hl;
-  leaf_runs(l_tree, nl, i, t, head, run_len, hl);
-  uint32_t base = 0;
-  if (head) base = atomicAdd(&tcursor[t], run_len);
-  base = (uint32_t)__shfl((int)base, hl, WAVE);
-  if (i >= nl) return;
-  const uint32_t pos = base + ((uint32_t)lane_id() - (uint32_t)hl);
-  g_start[pos] = l_start[i];
-  g_count[pos] = l_count[i];
+
+// Leaf list from the per-position counts: flags, then (after a scan of the
+// flags into off) each leaf's start within its tree and its count. Leaves come
+// out grouped by tree, in position order.
+__global__ void k_leaf_flags(const uint32_t* __restrict__ lcnt_pos, size_t total,
+                             uint32_t* __restrict__ flag) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < total) flag[e] = lcnt_pos[e] != 0 ? 1u : 0u;
+}
+__global__ void k_leaf_compact(const uint32_t* __restrict__ lcnt_pos, size_t total, uint32_t n,
+                               const uint32_t* __restrict__ off, uint32_t* __restrict__ g_start,
+                               uint32_t* __restrict__ g_count) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  const uint32_t c = lcnt_pos[e];
+  if (c == 0) return;
+  const uint32_t o = off[e];
+  g_start[o] = (uint32_t)(e % n);
+  g_count[o] = c;
+}
+// Tree t's leaves start at off[t * n]; tb + 1 values gathered for the host.
+__global__ void k_tree_offsets(const uint32_t* __restrict__ off, uint32_t n, uint32_t tb,
+                               uint32_t* __restrict__ out) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t <= tb) out[t] = off[(size_t)t * n];
 }
 
 // Leaf pairs of one tree. A group of LG lanes takes one leaf; a leaf of more

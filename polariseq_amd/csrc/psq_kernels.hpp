@@ -49,21 +49,28 @@ __global__ void k_forest_sides(const float* Xp, uint32_t d, uint32_t dp, uint32_
                                const float* offsets, uint32_t* fl, uint32_t* fr);
 __global__ void k_forest_nodes(const uint32_t* nstart, const uint32_t* ncount, const uint32_t* ntree,
                                uint32_t A, uint32_t n, const uint32_t* sl, const uint32_t* sr,
-                               uint32_t leaf_size, uint32_t level, uint32_t* nl_out, uint32_t* mode_out,
-                               uint32_t* nact, uint32_t* nleaf);
+                               uint32_t leaf_size, uint32_t cap, uint32_t level, uint32_t* nl_out,
+                               uint32_t* mode_out, uint32_t* nact, uint32_t* nsmall);
 __global__ void k_forest_emit(const uint32_t* nstart, const uint32_t* ncount, const uint32_t* ntree,
-                              uint32_t A, const uint32_t* nl_in, const uint32_t* mode_in,
-                              const uint32_t* act_off, const uint32_t* leaf_off, uint32_t leaf_size,
-                              uint32_t level, uint32_t* x_start, uint32_t* x_count, uint32_t* x_tree,
-                              uint32_t* child, uint32_t* l_tree, uint32_t* l_start, uint32_t* l_count,
-                              uint32_t leaf_base);
+                              uint32_t A, uint32_t n, const uint32_t* nl_in, const uint32_t* mode_in,
+                              const uint32_t* act_off, const uint32_t* small_off, uint32_t leaf_size,
+                              uint32_t cap, uint32_t level, uint32_t* x_start, uint32_t* x_count,
+                              uint32_t* x_tree, uint32_t* child, uint32_t* lcnt_pos,
+                              uint32_t small_base, uint32_t* sm_start, uint32_t* sm_count,
+                              uint32_t* sm_tree, uint32_t* sm_level);
 __global__ void k_forest_scatter(size_t total, uint32_t n, const uint32_t* ids_in, const uint32_t* segv_in,
                                  const uint32_t* nstart, const uint32_t* ntree, const uint32_t* nl_in,
                                  const uint32_t* mode_in, const uint32_t* child, const uint32_t* sl,
                                  const uint32_t* sr, uint32_t* ids_out, uint32_t* segv_out);
-__global__ void k_leaf_hist(const uint32_t* l_tree, uint32_t nl, uint32_t* tcount);
-__global__ void k_leaf_group(const uint32_t* l_tree, const uint32_t* l_start, const uint32_t* l_count,
-                             uint32_t nl, uint32_t* tcursor, uint32_t* g_start, uint32_t* g_count);
+__global__ void k_forest_finish(const float* Xs, uint32_t ds, uint32_t dps, uint32_t RSs, uint32_t n,
+                                uint32_t* ids, const uint32_t* sm_start, const uint32_t* sm_count,
+                                const uint32_t* sm_tree, const uint32_t* sm_level,
+                                const uint64_t* tseeds, uint32_t leaf_size, uint32_t C, uint32_t NC,
+                                uint32_t* lcnt_pos);
+__global__ void k_leaf_flags(const uint32_t* lcnt_pos, size_t total, uint32_t* flag);
+__global__ void k_leaf_compact(const uint32_t* lcnt_pos, size_t total, uint32_t n, const uint32_t* off,
+                               uint32_t* g_start, uint32_t* g_count);
+__global__ void k_tree_offsets(const uint32_t* off, uint32_t n, uint32_t tb, uint32_t* out);
 template <int D, int KMAX, int LG>
 __global__ void k_leaf_merge(const float* Xp, uint32_t d, uint32_t dp, uint32_t RS, uint32_t n,
                              const uint32_t* ids_t, const uint32_t* g_start, const uint32_t* g_count,
