@@ -344,14 +344,11 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
   const uint32_t LG = leaf_group(ls);
   uint32_t KM = k <= 16 ? 16 : (k <= 32 ? 32 : 64);
   if (LG > KM) KM = LG;
-  // LDS subtree size: rows at stride RSs within 52 KB (two blocks fit a CU's
-  // LDS), at most 512 points
+  // subtrees of at most `cap` points are finished by one wave each
   const uint32_t RSs = join_row_stride(ds);
-  uint32_t cap = std::min<uint32_t>(512, (52u * 1024u) / (RSs * 4u)) / 32u * 32u;
-  if (cap <= 2 * ls) cap = 0;  // too small to pay off: every level runs globally
+  const uint32_t cap = ls < 128 ? 256u : 0u;
   const uint32_t NC = cap / (ls + 1) + 2;
-  const size_t fin_smem = ((size_t)cap * RSs + (size_t)NC * RSs + NC) * 4 +
-                          ((size_t)7 * cap + 2 + (size_t)8 * NC) * 4;
+  const size_t fin_smem = ((size_t)7 * cap + 2 + (size_t)13 * NC + (size_t)2 * NC * RSs) * 4;
   if (cap > 0 && fin_smem > 65536)
     HIPC(hipFuncSetAttribute((const void*)k_forest_finish,
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)fin_smem));
@@ -462,7 +459,7 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
       small_base += newsmall;
     }
     if (small_base > 0) {
-      hipLaunchKernelGGL(k_forest_finish, dim3(small_base), dim3(256), fin_smem, s, Xs, ds, dps,
+      hipLaunchKernelGGL(k_forest_finish, dim3(small_base), dim3(64), fin_smem, s, Xs, ds, dps,
                          RSs, n, ids[cur], sm_start, sm_count, sm_tree, sm_level, tseeds + t0, ls,
                          cap, NC, lcnt_pos);
       HIPC(hipGetLastError());

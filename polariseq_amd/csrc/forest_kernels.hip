@@ -225,52 +225,50 @@ __global__ void k_forest_scatter(size_t total, uint32_t n, const uint32_t* __res
   segv_out[np] = ch;
 }
 
-// In-block exclusive scan of a[0..c) (c <= 2 * blockDim.x = 512), total in a[c].
-// blockDim.x == 256; wsum: 4 words of LDS.
-__device__ inline void block_scan_512(uint32_t* a, uint32_t c, uint32_t* wsum) {
-  const uint32_t t = threadIdx.x, l = (uint32_t)lane_id(), w = t / WAVE;
-  const uint32_t v0 = 2 * t < c ? a[2 * t] : 0u, v1 = 2 * t + 1 < c ? a[2 * t + 1] : 0u;
-  const uint32_t s = v0 + v1;
-  uint32_t x = s;
-#pragma unroll
-  for (int o = 1; o < WAVE; o <<= 1) {
-    const uint32_t y = __shfl_up(x, o, WAVE);
-    if ((int)l >= o) x += y;
+// dot_ref (simd.rs:22-37 order) of an LDS normal with a global row whose
+// chunks of 8 are read as two 16-byte loads (rows are 16-byte aligned).
+__device__ inline float dot_ref_row4(const float* nv, const float* __restrict__ row, uint32_t d) {
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  const uint32_t c = d >> 3;
+  for (uint32_t ch = 0; ch < c; ++ch) {
+    const float4 a = *reinterpret_cast<const float4*>(row + ch * 8);
+    const float4 b = *reinterpret_cast<const float4*>(row + ch * 8 + 4);
+    acc[0] = acc[0] + nv[ch * 8 + 0] * a.x;
+    acc[1] = acc[1] + nv[ch * 8 + 1] * a.y;
+    acc[2] = acc[2] + nv[ch * 8 + 2] * a.z;
+    acc[3] = acc[3] + nv[ch * 8 + 3] * a.w;
+    acc[4] = acc[4] + nv[ch * 8 + 4] * b.x;
+    acc[5] = acc[5] + nv[ch * 8 + 5] * b.y;
+    acc[6] = acc[6] + nv[ch * 8 + 6] * b.z;
+    acc[7] = acc[7] + nv[ch * 8 + 7] * b.w;
   }
-  if (l == WAVE - 1) wsum[w] = x;
-  __syncthreads();
-  uint32_t wpre = 0;
-  for (uint32_t j = 0; j < w; ++j) wpre += wsum[j];
-  const uint32_t excl = wpre + x - s;
-  if (2 * t < c) a[2 * t] = excl;
-  if (2 * t + 1 < c) a[2 * t + 1] = excl + v0;
-  if (t == blockDim.x - 1) a[c] = wpre + x;
-  __syncthreads();
+  float tail = 0.f;
+  for (uint32_t i = c * 8; i < d; ++i) tail = tail + nv[i] * row[i];
+  return (((acc[0] + acc[4]) + (acc[1] + acc[5])) + ((acc[2] + acc[6]) + (acc[3] + acc[7]))) + tail;
 }
 
-// Subtrees of at most C points are finished by one block each (256 threads):
-// their points' split rows are loaded into LDS once, then every remaining level
-// of the subtree runs there with the steps of the level kernels above (same
-// hashes on the global segment start, hyperplanes, margins, tie and fallback
-// rules, stable partition), so the tree equals the one the level kernels would
-// build. Leaves are written as counts at their start positions; the final
-// order of the subtree's positions is written back to ids.
-// LDS: rows[C][RSs] | normals[NC][RSs] | offsets[NC] | gid[C] | lid[2][C] |
-// pn[2][C] | scan_l[C+1] | scan_r[C+1] | nst[2][NC] | ncn[2][NC] | mode[NC] |
-// nl[NC] | child[2 NC]. NC bounds the active nodes of one level (C / (leaf_size
-// + 1) + 1).
-__global__ __launch_bounds__(256) void k_forest_finish(
+// Subtrees of at most C (<= 256) points are finished by one wave each (a block
+// of 64 threads, so __syncthreads() is a wave barrier): every remaining level
+// of the subtree runs with the steps of the level kernels above (same hashes
+// on the global segment start, hyperplanes, offset fold, margins, tie and
+// fallback rules, stable partition), so the tree equals the one the level
+// kernels would build. The subtree's rows, read once per level, stay in the
+// cache; all bookkeeping is in LDS. Leaves are written as counts at their
+// start positions; the final order of the subtree's positions goes back to ids.
+// LDS (u32 unless noted): gid[C] | lid[2][C] | pn[2][C] | scan_l[C+1] |
+// scan_r[C+1] | nst[2][NC] | ncn[2][NC] | mode[NC] | nl[NC] | child[2 NC] |
+// piv_a[NC] | piv_b[NC] | offsets[NC] f32 | normals[NC][RSs] f32 |
+// terms[NC][RSs] f32. NC bounds the active nodes of one level
+// (C / (leaf_size + 1) + 1).
+__global__ __launch_bounds__(64) void k_forest_finish(
     const float* __restrict__ Xs, uint32_t ds, uint32_t dps, uint32_t RSs, uint32_t n,
     uint32_t* __restrict__ ids, const uint32_t* __restrict__ sm_start,
     const uint32_t* __restrict__ sm_count, const uint32_t* __restrict__ sm_tree,
     const uint32_t* __restrict__ sm_level, const uint64_t* __restrict__ tseeds, uint32_t leaf_size,
     uint32_t C, uint32_t NC, uint32_t* __restrict__ lcnt_pos) {
-  extern __shared__ __attribute__((aligned(16))) float lds[];
-  __shared__ uint32_t s_nA, s_wsum[4];
-  float* rows = lds;
-  float* normals = rows + (size_t)C * RSs;
-  float* offs = normals + (size_t)NC * RSs;
-  uint32_t* gid = reinterpret_cast<uint32_t*>(offs + NC);
+  extern __shared__ __attribute__((aligned(16))) uint32_t ldsu[];
+  __shared__ uint32_t s_nA;
+  uint32_t* gid = ldsu;
   uint32_t* lid[2] = {gid + C, gid + 2 * C};
   uint32_t* pn[2] = {gid + 3 * C, gid + 4 * C};
   uint32_t* scl = gid + 5 * C;
@@ -280,61 +278,66 @@ __global__ __launch_bounds__(256) void k_forest_finish(
   uint32_t* nmode = ncn[1] + NC;
   uint32_t* nnl = nmode + NC;
   uint32_t* child = nnl + NC;
-  const uint32_t tid = threadIdx.x, w = tid / WAVE, l = (uint32_t)lane_id();
+  uint32_t* piva = child + 2 * NC;
+  uint32_t* pivb = piva + NC;
+  float* offs = reinterpret_cast<float*>(pivb + NC);
+  float* normals = offs + NC;
+  float* terms = normals + (size_t)NC * RSs;
+  const uint32_t l = (uint32_t)lane_id();
   const uint32_t b = blockIdx.x;
   const uint32_t start = sm_start[b], c = sm_count[b], t = sm_tree[b], L0 = sm_level[b];
   const uint64_t ts = tseeds[t];
   uint32_t* tids = ids + (size_t)t * n + start;
   uint32_t* tl = lcnt_pos + (size_t)t * n + start;
-  for (uint32_t e = tid; e < c; e += blockDim.x) {
+  for (uint32_t e = l; e < c; e += WAVE) {
     gid[e] = tids[e];
     lid[0][e] = e;
     pn[0][e] = 0;
   }
-  if (tid == 0) {
+  if (l == 0) {
     nst[0][0] = 0;
     ncn[0][0] = c;
-  }
-  __syncthreads();
-  const uint32_t c4 = (ds + 3) / 4;
-  for (uint32_t e = tid; e < c * c4; e += blockDim.x) {
-    const uint32_t r = e / c4, ch = e - r * c4;
-    const float4 v = *reinterpret_cast<const float4*>(Xs + (size_t)gid[r] * dps + 4 * ch);
-    *reinterpret_cast<float4*>(rows + (size_t)r * RSs + 4 * ch) = v;
   }
   __syncthreads();
   uint32_t A = 1;
   int cur = 0;
   for (uint32_t L = L0; A > 0; ++L) {
     const int nx = cur ^ 1;
-    // hyperplanes, one wave per node
-    for (uint32_t v = w; v < A; v += blockDim.x / WAVE) {
+    // pivots, one lane per node
+    for (uint32_t v = l; v < A; v += WAVE) {
       const uint32_t s0 = nst[cur][v], cv = ncn[cur][v], gs0 = start + s0;
       const uint32_t* ln = lid[cur] + s0;
       const uint32_t ea = ln[tree_hash(ts, L, gs0, 0) % cv];
       uint32_t eb = ln[tree_hash(ts, L, gs0, 1) % cv];
       for (uint64_t r = 2; eb == ea && r < 10; ++r) eb = ln[tree_hash(ts, L, gs0, r) % cv];
-      const float* xa = rows + (size_t)ea * RSs;
-      const float* xb = rows + (size_t)eb * RSs;
-      float* nv = normals + (size_t)v * RSs;
-      for (uint32_t j = l; j < RSs; j += WAVE) nv[j] = j < ds ? xa[j] - xb[j] : 0.0f;
-      if (l == 0) {
-        float sacc = -0.0f;
-        for (uint32_t j = 0; j < ds; ++j) {
-          const float nj = xa[j] - xb[j];
-          sacc = sacc + nj * 0.5f * (xa[j] + xb[j]);
-        }
-        offs[v] = -sacc;
-      }
+      piva[v] = gid[ea];
+      pivb[v] = gid[eb];
+    }
+    __syncthreads();
+    // normals and the terms of the offset fold, one lane per (node, dimension)
+    for (uint32_t x = l; x < A * ds; x += WAVE) {
+      const uint32_t v = x / ds, j = x - v * ds;
+      const float xa = Xs[(size_t)piva[v] * dps + j], xb = Xs[(size_t)pivb[v] * dps + j];
+      const float nj = xa - xb;
+      normals[(size_t)v * RSs + j] = nj;
+      terms[(size_t)v * RSs + j] = nj * 0.5f * (xa + xb);
+    }
+    __syncthreads();
+    // offsets: the sequential fold, one lane per node
+    for (uint32_t v = l; v < A; v += WAVE) {
+      float sacc = -0.0f;
+      const float* tv = terms + (size_t)v * RSs;
+      for (uint32_t j = 0; j < ds; ++j) sacc = sacc + tv[j];
+      offs[v] = -sacc;
     }
     __syncthreads();
     // sides
-    for (uint32_t pos = tid; pos < c; pos += blockDim.x) {
+    for (uint32_t pos = l; pos < c; pos += WAVE) {
       const uint32_t v = pn[cur][pos];
       uint32_t fl = 0, fr = 0;
       if (v != NONE) {
-        const uint32_t e = lid[cur][pos], g = gid[e];
-        const float margin = dot_ref(normals + (size_t)v * RSs, rows + (size_t)e * RSs, (int)ds) + offs[v];
+        const uint32_t g = gid[lid[cur][pos]];
+        const float margin = dot_ref_row4(normals + (size_t)v * RSs, Xs + (size_t)g * dps, ds) + offs[v];
         bool left;
         if (fabsf(margin) < 1e-8f)
           left = (tree_hash(ts, L, g, 2) & 1) == 0;
@@ -347,12 +350,47 @@ __global__ __launch_bounds__(256) void k_forest_finish(
       scr[pos] = fr;
     }
     __syncthreads();
-    block_scan_512(scl, c, s_wsum);
-    block_scan_512(scr, c, s_wsum);
-    if (tid == 0) s_nA = 0;
+    // exclusive scans over positions: lane l takes positions [4l, 4l + 4)
+    {
+      uint32_t vl[4], vr[4], sl_ = 0, sr_ = 0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t pos = 4 * l + u;
+        vl[u] = pos < c ? scl[pos] : 0u;
+        vr[u] = pos < c ? scr[pos] : 0u;
+        sl_ += vl[u];
+        sr_ += vr[u];
+      }
+      uint32_t xl = sl_, xr = sr_;
+#pragma unroll
+      for (int o = 1; o < WAVE; o <<= 1) {
+        const uint32_t yl = __shfl_up(xl, o, WAVE), yr = __shfl_up(xr, o, WAVE);
+        if ((int)l >= o) {
+          xl += yl;
+          xr += yr;
+        }
+      }
+      uint32_t el = xl - sl_, er = xr - sr_;
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t pos = 4 * l + u;
+        if (pos < c) {
+          scl[pos] = el;
+          scr[pos] = er;
+        }
+        el += vl[u];
+        er += vr[u];
+      }
+      if (l == WAVE - 1) {
+        scl[c] = xl;
+        scr[c] = xr;
+      }
+      if (l == 0) s_nA = 0;
+    }
     __syncthreads();
     // split decisions, leaves and next-level nodes
-    for (uint32_t v = tid; v < A; v += blockDim.x) {
+    for (uint32_t v = l; v < A; v += WAVE) {
       const uint32_t s0 = nst[cur][v], cv = ncn[cur][v];
       const uint32_t cl = scl[s0 + cv] - scl[s0], rl = scr[s0 + cv] - scr[s0];
       uint32_t mode, nl;
@@ -391,7 +429,7 @@ __global__ __launch_bounds__(256) void k_forest_finish(
     }
     __syncthreads();
     // stable partition
-    for (uint32_t pos = tid; pos < c; pos += blockDim.x) {
+    for (uint32_t pos = l; pos < c; pos += WAVE) {
       const uint32_t v = pn[cur][pos], e = lid[cur][pos];
       if (v == NONE || nmode[v] == 2) {
         lid[nx][pos] = e;
@@ -411,7 +449,7 @@ __global__ __launch_bounds__(256) void k_forest_finish(
     cur = nx;
     __syncthreads();
   }
-  for (uint32_t pos = tid; pos < c; pos += blockDim.x) tids[pos] = gid[lid[cur][pos]];
+  for (uint32_t pos = l; pos < c; pos += WAVE) tids[pos] = gid[lid[cur][pos]];
 }
 
 // Leaf list from the per-position counts: flags, then (after a scan of the
