@@ -62,6 +62,9 @@ typedef struct psq_nndescent_opts {
   uint32_t slots;    /* proposal slots per row per iteration (0 -> auto, power of two >= 4 heap_k) */
   uint32_t heap_k;   /* entries each row keeps while iterating, >= k (0 -> k for FP32,
                         min(2k, 64) for BF16_RERANK); the output holds each row's first k */
+  uint32_t reorder;  /* renumber the points in the first tree's leaf order while iterating
+                        (0 -> on for BF16_RERANK with a forest start, 1 on, 2 off); the graph
+                        is returned in the caller's numbering either way */
 } psq_nndescent_opts;
 
 /* Defaults of NndescentOptions::default (nndescent.rs:63-76) with k = 15. */
@@ -162,6 +165,7 @@ typedef struct psq_nndescent_stats {
   uint64_t n_obs, n_dim, k, max_candidates, max_iterations, n_trees, leaf_size, slots, stop_at;
   uint64_t iterations;
   uint64_t heap_k;        /* entries kept per row while iterating */
+  uint64_t reordered;     /* 1 when the points were renumbered while iterating */
   uint64_t changed[64];   /* per iteration */
   uint64_t pairs[64];     /* distance pairs evaluated in the join, per iteration */
   uint64_t cand_rows[64]; /* Σ_i (|new_i| + |old_i|), the join's row gathers, per iteration */

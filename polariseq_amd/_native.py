@@ -91,6 +91,7 @@ class NndescentOpts(ctypes.Structure):
         ("device", ctypes.c_int32),
         ("slots", ctypes.c_uint32),
         ("heap_k", ctypes.c_uint32),
+        ("reorder", ctypes.c_uint32),
     ]
 
 
@@ -107,6 +108,7 @@ class NndescentStats(ctypes.Structure):
         ("stop_at", ctypes.c_uint64),
         ("iterations", ctypes.c_uint64),
         ("heap_k", ctypes.c_uint64),
+        ("reordered", ctypes.c_uint64),
         ("changed", ctypes.c_uint64 * 64),
         ("pairs", ctypes.c_uint64 * 64),
         ("cand_rows", ctypes.c_uint64 * 64),
@@ -125,7 +127,7 @@ class NndescentStats(ctypes.Structure):
             "max_candidates": int(self.max_candidates), "max_iterations": int(self.max_iterations),
             "n_trees": int(self.n_trees), "leaf_size": int(self.leaf_size),
             "slots": int(self.slots), "stop_at": int(self.stop_at), "iterations": it,
-            "heap_k": int(self.heap_k),
+            "heap_k": int(self.heap_k), "reordered": int(self.reordered),
             "changed": [int(v) for v in self.changed[:it]],
             "pairs": [int(v) for v in self.pairs[:it]],
             "cand_rows": [int(v) for v in self.cand_rows[:it]],
@@ -228,7 +230,7 @@ def check(rc: int) -> None:
 def make_opts(k: int, max_iterations: int = 0, seed: int = 0, max_candidates: int = 0,
               n_trees: int = 0, delta: float = 0.001, leaf_size: int = 0,
               precision: str = "fp32", init: str = "rp_forest", device: int = 0,
-              slots: int = 0, model=None, heap_k: int = 0) -> NndescentOpts:
+              slots: int = 0, model=None, heap_k: int = 0, reorder: str = "auto") -> NndescentOpts:
     o = NndescentOpts()
     lib().psq_nndescent_opts_default(ctypes.byref(o))
     o.k = int(k)
@@ -243,6 +245,7 @@ def make_opts(k: int, max_iterations: int = 0, seed: int = 0, max_candidates: in
     o.device = int(device)
     o.slots = int(slots)
     o.heap_k = int(heap_k)
+    o.reorder = {"auto": 0, "on": 1, "off": 2}[reorder]
     o.model = model
     return o
 

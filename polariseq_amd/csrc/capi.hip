@@ -336,7 +336,8 @@ int launch_leaf_merge(uint32_t KM, uint32_t LG, const float* Xp, uint32_t d, uin
 // Xs (ds dims, row stride dps) is what the hyperplanes split: the embedding
 // itself, or the learned codes. Leaf pairs are measured on Xp (the embedding).
 int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds, uint32_t dps,
-                 const float* Xp, uint64_t* G, hipStream_t s, uint32_t* hbuf) {
+                 const float* Xp, uint64_t* G, hipStream_t s, uint32_t* hbuf,
+                 const uint32_t** order0 = nullptr) {
   const uint32_t n = (uint32_t)R.n, d = (uint32_t)R.d, dp = R.dp, k = (uint32_t)R.k;
   const uint32_t ls = (uint32_t)R.leaf_size;
   const uint64_t T = R.n_trees;
@@ -476,6 +477,12 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
     std::vector<uint32_t> hoff(tb + 1, 0);
     HIPC(hipMemcpyAsync(hoff.data(), toff, (tb + 1) * 4, hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
+    if (t0 == 0 && order0) {  // tree 0: the points in leaf order
+      uint32_t* o;
+      TRY(wbuf(ws, "order0", n, &o));
+      HIPC(hipMemcpyAsync(o, ids[cur], (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+      *order0 = o;
+    }
     // leaf pairs, tree after tree
     for (uint32_t t = 0; t < tb; ++t) {
       const uint32_t nl = hoff[t + 1] - hoff[t];
@@ -557,13 +564,39 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   }
 
   // ── start: RP forest (or nothing), random fill, then one forced merge sorts every row ──
+  const bool bf16 = opts->precision == PSQ_PRECISION_BF16_RERANK;
+  const bool forest = opts->init != PSQ_INIT_RANDOM && R.n_trees > 0;
+  const bool reorder = forest && (opts->reorder == 1 || (opts->reorder == 0 && bf16));
+  const uint32_t* perm = nullptr;
+  uint32_t* inv = nullptr;
   if (opts->init == PSQ_INIT_RP_FOREST && R.n_trees > 0)
-    TRY(build_forest(ws, R, Xp, (uint32_t)R.d, R.dp, Xp, G, s, static_cast<uint32_t*>(hp) + 64));
+    TRY(build_forest(ws, R, Xp, (uint32_t)R.d, R.dp, Xp, G, s, static_cast<uint32_t*>(hp) + 64,
+                     reorder ? &perm : nullptr));
   if (opts->init == PSQ_INIT_LEARNED && R.n_trees > 0) {
     float* codes;
     TRY(wbuf(ws, "codes", (size_t)n * 32, &codes));
     TRY(psq_encoder_encode(static_cast<const psq_encoder*>(opts->model), d_data, n, R.d, codes, s));
-    TRY(build_forest(ws, R, codes, 32u, 32u, Xp, G, s, static_cast<uint32_t*>(hp) + 64));
+    TRY(build_forest(ws, R, codes, 32u, 32u, Xp, G, s, static_cast<uint32_t*>(hp) + 64,
+                     reorder ? &perm : nullptr));
+  }
+  if (perm) {
+    // renumber the points in the first tree's leaf order: neighbours get nearby
+    // indices, so rows, buckets and candidate rows touched together share cache
+    float* Xq;
+    uint64_t* Gq;
+    TRY(wbuf(ws, "inv", n, &inv));
+    TRY(wbuf(ws, "Xq", (size_t)n * R.dp, &Xq));
+    TRY(wbuf(ws, "Gq", (size_t)n * k, &Gq));
+    hipLaunchKernelGGL(k_perm_inverse, dim3(grid_for(n, 256)), dim3(256), 0, s, perm, n, inv);
+    HIPC(hipGetLastError());
+    hipLaunchKernelGGL(k_perm_rows, dim3(grid_for((size_t)n * (R.dp / 4), 256)), dim3(256), 0, s, Xp,
+                       R.dp, perm, n, Xq);
+    HIPC(hipGetLastError());
+    hipLaunchKernelGGL(k_perm_graph, dim3(grid_for((size_t)n * k, 256)), dim3(256), 0, s, G, k, perm,
+                       inv, n, Gq);
+    HIPC(hipGetLastError());
+    Xp = Xq;
+    G = Gq;
   }
   hipLaunchKernelGGL(k_random_fill, dim3(grid_for(n, 256)), dim3(256), 0, s, Xp, n,
                      (uint32_t)R.d, R.dp, G, k, R.seed, touched);
@@ -574,7 +607,6 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   // ── iterations ──
   psq_nndescent_stats st;
   std::memset(&st, 0, sizeof(st));
-  const bool bf16 = opts->precision == PSQ_PRECISION_BF16_RERANK;
   std::vector<hipEvent_t> jb, je;
   uint64_t iters = 0;
   for (uint64_t it = 0; it < R.n_iters; ++it) {
@@ -631,11 +663,11 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
 
   // ── extraction ──
   const uint32_t ko = (uint32_t)R.ko;
-  hipLaunchKernelGGL(k_row_counts, dim3(grid_for(n, 256)), dim3(256), 0, s, G, n, k, ko, rowcnt);
+  hipLaunchKernelGGL(k_row_counts, dim3(grid_for(n, 256)), dim3(256), 0, s, G, n, k, ko, inv, rowcnt);
   HIPC(hipGetLastError());
   TRY(scan_u32(ws, rowcnt, d_indptr, n, s));
   hipLaunchKernelGGL(k_extract, dim3(grid_for((size_t)n * ko, 256)), dim3(256), 0, s, G, n, k,
-                     ko, d_indptr, d_indices, d_dists);
+                     ko, perm, inv, d_indptr, d_indices, d_dists);
   HIPC(hipGetLastError());
   HIPC(hipEventRecord(e_end, s));
   uint32_t nnz32 = 0;
@@ -647,6 +679,7 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   st.n_dim = R.d;
   st.k = R.ko;
   st.heap_k = R.k;
+  st.reordered = perm ? 1 : 0;
   st.max_candidates = R.m;
   st.max_iterations = R.n_iters;
   st.n_trees = opts->init == PSQ_INIT_RANDOM ? 0 : R.n_trees;

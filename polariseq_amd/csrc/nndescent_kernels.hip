@@ -461,29 +461,61 @@ __global__ __launch_bounds__(64) void k_merge(uint64_t* __restrict__ G, uint32_t
 }
 
 // ── extraction (nndescent.rs:977-1002) ──
-// Rows hold k entries; the graph takes the first ko (ko <= k) of each.
+// Rows hold k entries; the graph takes the first ko (ko <= k) of each. With a
+// renumbering (perm[p] = original index of internal point p, inv its inverse),
+// output row o is internal row inv[o] and neighbour indices map back through
+// perm; otherwise perm and inv are NULL.
 __global__ void k_row_counts(const uint64_t* __restrict__ G, uint32_t n, uint32_t k, uint32_t ko,
-                             uint32_t* __restrict__ cnt) {
+                             const uint32_t* __restrict__ inv, uint32_t* __restrict__ cnt) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint64_t* r = G + (size_t)i * k;
+  const uint64_t* r = G + (size_t)(inv ? inv[i] : i) * k;
   uint32_t c = 0;
   while (c < ko && r[c] != EMPTY_KEY) ++c;
   cnt[i] = c;
 }
 __global__ void k_extract(const uint64_t* __restrict__ G, uint32_t n, uint32_t k, uint32_t ko,
+                          const uint32_t* __restrict__ perm, const uint32_t* __restrict__ inv,
                           const uint32_t* __restrict__ off, uint32_t* __restrict__ indices,
                           float* __restrict__ dists) {
   size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (size_t)n * ko) return;
   uint32_t i = (uint32_t)(e / ko), s = (uint32_t)(e - (size_t)i * ko);
-  uint64_t key = G[(size_t)i * k + s];
+  uint64_t key = G[(size_t)(inv ? inv[i] : i) * k + s];
   if (key == EMPTY_KEY) return;
   uint32_t o = off[i] + s;
-  indices[o] = key_idx(key);
+  indices[o] = perm ? perm[key_idx(key)] : key_idx(key);
   // sqrt in FP64 then rounded to FP32: with 53 >= 2*24 + 2 bits the result is
   // the correctly rounded f32 sqrt (gfx950's f32 sqrt is not), as Rust's f32::sqrt.
   dists[o] = (float)__dsqrt_rn((double)key_dist(key));
+}
+
+// ── renumbering into a locality order ──
+__global__ void k_perm_inverse(const uint32_t* __restrict__ perm, uint32_t n,
+                               uint32_t* __restrict__ inv) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) inv[perm[p]] = p;
+}
+// Y[p] = X[perm[p]], rows of dp floats (dp a multiple of 4).
+__global__ void k_perm_rows(const float* __restrict__ X, uint32_t dp, const uint32_t* __restrict__ perm,
+                            uint32_t n, float* __restrict__ Y) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t c4 = dp / 4;
+  if (e >= (size_t)n * c4) return;
+  const uint32_t p = (uint32_t)(e / c4), c = (uint32_t)(e - (size_t)p * c4);
+  reinterpret_cast<float4*>(Y)[e] = reinterpret_cast<const float4*>(X)[(size_t)perm[p] * c4 + c];
+}
+// Gq[p] = G[perm[p]] with every neighbour index renumbered by inv (the rows may
+// then be out of key order at equal distances; the forced merge sorts them).
+__global__ void k_perm_graph(const uint64_t* __restrict__ G, uint32_t k,
+                             const uint32_t* __restrict__ perm, const uint32_t* __restrict__ inv,
+                             uint32_t n, uint64_t* __restrict__ Gq) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)n * k) return;
+  const uint32_t p = (uint32_t)(e / k), s = (uint32_t)(e - (size_t)p * k);
+  uint64_t key = G[(size_t)perm[p] * k + s];
+  if (key != EMPTY_KEY) key = (key & ~0xFFFFFFFEull) | ((uint64_t)inv[key_idx(key)] << 1);
+  Gq[e] = key;
 }
 
 // explicit instantiations used by the host code

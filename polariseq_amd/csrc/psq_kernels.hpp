@@ -33,9 +33,14 @@ __global__ void k_join(const float* Xp, uint32_t n, uint32_t d, uint32_t dp, uin
 template <int NR, int NT>
 __global__ void k_merge(uint64_t* G, uint32_t n, uint32_t k, uint64_t* slots, uint32_t S,
                         uint8_t* touched, float* thr, int force, unsigned long long* changed);
-__global__ void k_row_counts(const uint64_t* G, uint32_t n, uint32_t k, uint32_t ko, uint32_t* cnt);
-__global__ void k_extract(const uint64_t* G, uint32_t n, uint32_t k, uint32_t ko, const uint32_t* off,
-                          uint32_t* indices, float* dists);
+__global__ void k_row_counts(const uint64_t* G, uint32_t n, uint32_t k, uint32_t ko, const uint32_t* inv,
+                             uint32_t* cnt);
+__global__ void k_extract(const uint64_t* G, uint32_t n, uint32_t k, uint32_t ko, const uint32_t* perm,
+                          const uint32_t* inv, const uint32_t* off, uint32_t* indices, float* dists);
+__global__ void k_perm_inverse(const uint32_t* perm, uint32_t n, uint32_t* inv);
+__global__ void k_perm_rows(const float* X, uint32_t dp, const uint32_t* perm, uint32_t n, float* Y);
+__global__ void k_perm_graph(const uint64_t* G, uint32_t k, const uint32_t* perm, const uint32_t* inv,
+                             uint32_t n, uint64_t* Gq);
 
 // RP forest (forest_kernels.hip)
 __global__ void k_forest_init(uint32_t* ids, uint32_t* segv, uint32_t n, uint32_t T, uint32_t leaf_size);

@@ -45,19 +45,21 @@ def nndescent_dense(embedding, k: int, max_iterations: int = 0, seed: int = 0,
                     max_candidates: int = 0, n_trees: int = 0, delta: float = 0.001,
                     out=None, *, precision: str = "fp32", init: str = "rp_forest",
                     device: int = 0, leaf_size: int = 0, slots: int = 0, model=None,
-                    heap_k: int = 0) -> dict:
+                    heap_k: int = 0, reorder: str = "auto") -> dict:
     """Approximate kNN graph by NN-Descent on the GPU (pybind/lib.rs:503-566).
 
     ``max_iterations``, ``max_candidates`` and ``n_trees`` accept 0 for "auto".
     ``heap_k``: entries each row keeps while iterating (0 = k for fp32, 2k for
-    bf16); the graph holds each row's first k.
+    bf16); the graph holds each row's first k. ``reorder``: "auto" (on for bf16
+    with a forest start), "on" or "off": renumber the points in a locality order
+    while iterating (the graph comes back in the caller's numbering).
     Returns ``{"indptr", "indices", "data", "n_obs"}``, or with ``out`` writes the
     three arrays in place and returns ``{"nnz", "n_obs"}``.
     """
     emb = _as_f32_2d(embedding)
     n_obs, n_dim = emb.shape
     opts = N.make_opts(k, max_iterations, seed, max_candidates, n_trees, delta, leaf_size,
-                       precision, init, device, slots, _model_handle(model), heap_k)
+                       precision, init, device, slots, _model_handle(model), heap_k, reorder)
     nnz = ctypes.c_uint64(0)
     L = N.lib()
     if out is not None:
@@ -174,7 +176,7 @@ def _stream_ptr():
 def nndescent_device(x, k: int, max_iterations: int = 0, seed: int = 0, max_candidates: int = 0,
                      n_trees: int = 0, delta: float = 0.001, *, precision: str = "fp32",
                      init: str = "rp_forest", leaf_size: int = 0, slots: int = 0, out=None,
-                     model=None, heap_k: int = 0):
+                     model=None, heap_k: int = 0, reorder: str = "auto"):
     """NN-Descent on a CUDA float32 tensor (n, d). Returns (indptr, indices, dists, nnz)
     as device tensors (indices/dists sized n * effective_k; valid up to nnz)."""
     import torch
@@ -190,7 +192,7 @@ def nndescent_device(x, k: int, max_iterations: int = 0, seed: int = 0, max_cand
         ip, ix, dx = out
     opts = N.make_opts(k, max_iterations, seed, max_candidates, n_trees, delta, leaf_size,
                        precision, init, x.device.index or 0, slots, _model_handle(model),
-                       heap_k)
+                       heap_k, reorder)
     nnz = ctypes.c_uint64(0)
     N.check(N.lib().psq_nndescent_device(x.data_ptr(), n, d, ctypes.byref(opts), ip.data_ptr(),
                                          ip.numel(), ix.data_ptr(), dx.data_ptr(),

@@ -48,3 +48,26 @@ def test_bf16_form_exact_distances_invariants_and_recall(orc, k, heap_k):
     rb = ds.recall(ix[q], ei).mean()
     rp = ds.recall(plain["indices"].reshape(n, k)[q], ei).mean()
     assert rb >= 0.95 and rb >= rp - 0.01, (rb, rp)
+
+
+def test_renumbering_returns_the_callers_numbering(orc):
+    # The BF16 form renumbers the points in a locality order while iterating;
+    # the graph must come back in the caller's numbering, with exact distances.
+    n, k = 12000, 10
+    x = ds.mixture(n, 50, 15, seed=5)
+    on = P.nndescent_dense(x, k, seed=1, precision="bf16", n_trees=6, reorder="on")
+    assert P.last_stats()["reordered"] == 1
+    off = P.nndescent_dense(x, k, seed=1, precision="bf16", n_trees=6, reorder="off")
+    assert P.last_stats()["reordered"] == 0
+    for g in (on, off):
+        _valid(g, n, k)
+        ix, dx = g["indices"].reshape(n, k), g["data"].reshape(n, k)
+        for i in range(0, n, 97):
+            for s in range(k):
+                want = np.float32(np.sqrt(orc.sq_dist(x[i], x[ix[i, s]])))
+                assert dx[i, s].view(np.uint32) == want.view(np.uint32)
+    q = np.arange(0, n, 6, dtype=np.uint32)
+    ei, _ = orc.knn_exact64(x, k, q)
+    r_on = ds.recall(on["indices"].reshape(n, k)[q], ei).mean()
+    r_off = ds.recall(off["indices"].reshape(n, k)[q], ei).mean()
+    assert abs(r_on - r_off) < 0.01 and r_on > 0.95, (r_on, r_off)
