@@ -58,7 +58,7 @@ def main() -> int:
     ap.add_argument("--init", default="rp_forest", choices=["random", "rp_forest"],
                     help="start of the plain and bf16 forms")
     ap.add_argument("--learned-trees", type=int, default=8)
-    ap.add_argument("--train-steps", type=int, default=600)
+    ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--train-subset", type=int, default=200_000)
     ap.add_argument("--recall-queries", type=int, default=100_000)
     ap.add_argument("--cpu-cells", type=int, default=1_000_000)

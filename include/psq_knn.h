@@ -171,6 +171,7 @@ typedef struct psq_nndescent_stats {
   uint64_t cand_rows[64]; /* Σ_i (|new_i| + |old_i|), the join's row gathers, per iteration */
   uint64_t exact_pairs[64]; /* pairs whose exact f32 distance the join computed (all pairs in
                                the FP32 form, the BF16 screen's survivors in BF16_RERANK) */
+  uint64_t proposals[64]; /* (row, neighbour) proposals the join sent to the buckets */
   double ms_init, ms_iterations, ms_extract, ms_total;
   double ms_join[64]; /* join kernel time per iteration (HIP events) */
 } psq_nndescent_stats;

@@ -113,6 +113,7 @@ class NndescentStats(ctypes.Structure):
         ("pairs", ctypes.c_uint64 * 64),
         ("cand_rows", ctypes.c_uint64 * 64),
         ("exact_pairs", ctypes.c_uint64 * 64),
+        ("proposals", ctypes.c_uint64 * 64),
         ("ms_init", ctypes.c_double),
         ("ms_iterations", ctypes.c_double),
         ("ms_extract", ctypes.c_double),
@@ -132,6 +133,7 @@ class NndescentStats(ctypes.Structure):
             "pairs": [int(v) for v in self.pairs[:it]],
             "cand_rows": [int(v) for v in self.cand_rows[:it]],
             "exact_pairs": [int(v) for v in self.exact_pairs[:it]],
+            "proposals": [int(v) for v in self.proposals[:it]],
             "ms_init": float(self.ms_init), "ms_iterations": float(self.ms_iterations),
             "ms_extract": float(self.ms_extract), "ms_total": float(self.ms_total),
             "ms_join": [float(v) for v in self.ms_join[:it]],

@@ -92,7 +92,7 @@ __global__ __launch_bounds__(64) void k_join_bf16(
   __syncthreads();
 
   uint32_t pcnt = 0, scnt = 0;
-  uint64_t pairs = 0, exact = 0;
+  uint64_t pairs = 0, exact = 0, sent = 0;
   // Exact distances of the listed pairs, 64 at a time (all of them when `all`).
   auto drain = [&](bool all) {
     if (scnt < (all ? 1u : (uint32_t)WAVE)) return;
@@ -112,8 +112,8 @@ __global__ __launch_bounds__(64) void k_join_bf16(
         pb = dist < s_thr[b];
         ++exact;
       }
-      pb_append(pb_row, pb_key, pcnt, pa, ia, make_key(dist, ib, 1));
-      pb_append(pb_row, pb_key, pcnt, pb, ib, make_key(dist, ia, 1));
+      pb_append(pb_row, pb_key, pcnt, pa, ia, make_key(dist, ib, 1), sent);
+      pb_append(pb_row, pb_key, pcnt, pb, ib, make_key(dist, ia, 1), sent);
       pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, bmask, seed, iter);
       done += WAVE;
     }
@@ -173,6 +173,7 @@ __global__ __launch_bounds__(64) void k_join_bf16(
     ctr_add(counters, 0, (unsigned long long)tp);
     ctr_add(counters, 1, (unsigned long long)nc);
     ctr_add(counters, 3, (unsigned long long)te);
+    ctr_add(counters, 4, (unsigned long long)sent);
   }
 }
 

@@ -179,6 +179,11 @@ uint32_t round_up(uint32_t v, uint32_t m) { return (v + m - 1) / m * m; }
 // per thousand on centred vectors); passing pairs are re-ranked exactly.
 constexpr float BF16_SLACK = 1.03f;
 constexpr uint32_t BF16_SV_CAP = 320;  // screened-pair list of k_join_bf16
+// BF16 form: for its first iterations the join's threshold is a row's k-th
+// entry (the output size), not its heap_k-th; the rows still keep heap_k
+// entries. Early rows are poor, and the wider threshold mostly buys proposals
+// that later iterations replace anyway.
+constexpr uint64_t BF16_WARM_ITERS = 2;
 
 struct Resolved {
   // ko: neighbours output per row; k: entries each row keeps while iterating.
@@ -237,18 +242,18 @@ int resolve(uint64_t n, uint64_t d, const psq_nndescent_opts* o, Resolved* r) {
   return PSQ_OK;
 }
 
-int launch_merge(const Resolved& R, uint64_t* G, uint64_t* slots, uint8_t* touched, float* thr,
-                 int force, unsigned long long* changed, hipStream_t s) {
+int launch_merge(const Resolved& R, uint64_t* G, uint64_t* slots, uint8_t* touched, int force,
+                 unsigned long long* changed, hipStream_t s) {
   const dim3 grid((unsigned)R.n), block(64);
   const uint32_t n = (uint32_t)R.n, k = (uint32_t)R.k, S = (uint32_t)R.S;
   if (S == 64)
-    hipLaunchKernelGGL((k_merge<1, 2>), grid, block, 0, s, G, n, k, slots, S, touched, thr, force,
+    hipLaunchKernelGGL((k_merge<1, 2>), grid, block, 0, s, G, n, k, slots, S, touched, force,
                        changed);
   else if (S == 128)
-    hipLaunchKernelGGL((k_merge<2, 4>), grid, block, 0, s, G, n, k, slots, S, touched, thr, force,
+    hipLaunchKernelGGL((k_merge<2, 4>), grid, block, 0, s, G, n, k, slots, S, touched, force,
                        changed);
   else if (S == 256)
-    hipLaunchKernelGGL((k_merge<4, 8>), grid, block, 0, s, G, n, k, slots, S, touched, thr, force,
+    hipLaunchKernelGGL((k_merge<4, 8>), grid, block, 0, s, G, n, k, slots, S, touched, force,
                        changed);
   else
     return fail(PSQ_E_UNSUPPORTED, "slots must be 64, 128 or 256");
@@ -601,7 +606,7 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   hipLaunchKernelGGL(k_random_fill, dim3(grid_for(n, 256)), dim3(256), 0, s, Xp, n,
                      (uint32_t)R.d, R.dp, G, k, R.seed, touched);
   HIPC(hipGetLastError());
-  TRY(launch_merge(R, G, slots, touched, thr, 1, ctr, s));
+  TRY(launch_merge(R, G, slots, touched, 1, ctr, s));
   HIPC(hipEventRecord(e_init, s));
 
   // ── iterations ──
@@ -619,7 +624,8 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
                        R.seed, it, roff, rcount, rev);
     HIPC(hipGetLastError());
     hipLaunchKernelGGL(k_sample_select, dim3(n), dim3(64), 0, s, G, n, k, m, R.seed,
-                       it, roff, rev, newl, oldl, ncnt, slots, S);
+                       it, roff, rev, newl, oldl, ncnt, slots, S, thr,
+                       (bf16 && it < BF16_WARM_ITERS) ? (uint32_t)R.ko : k);
     HIPC(hipGetLastError());
     hipEvent_t a, b;
     TRY(evp.make(&a));
@@ -643,11 +649,11 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
     HIPC(hipEventRecord(b, s));
     jb.push_back(a);
     je.push_back(b);
-    TRY(launch_merge(R, G, slots, touched, thr, 0, ctr, s));
+    TRY(launch_merge(R, G, slots, touched, 0, ctr, s));
     unsigned long long* csum = ctr + (size_t)CTR_STRIPES * CTR_WORDS;
     hipLaunchKernelGGL(k_ctr_reduce, dim3(1), dim3(64), 0, s, ctr, csum);
     HIPC(hipGetLastError());
-    HIPC(hipMemcpyAsync(hctr, csum, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(hctr, csum, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
     const uint64_t changed = hctr[2];
     if (it < 64) {
@@ -655,6 +661,7 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
       st.pairs[it] = hctr[0];
       st.cand_rows[it] = hctr[1];
       st.exact_pairs[it] = bf16 ? hctr[3] : hctr[0];
+      st.proposals[it] = hctr[4];
     }
     ++iters;
     if (changed <= R.stop_at) break;

@@ -190,12 +190,13 @@ __global__ void k_rev_scatter(const uint64_t* __restrict__ G, uint32_t n, uint32
 // entry t - k; they are taken 64 at a time. Key: (class << 63) | (prio24 << 32)
 // | neighbour, class 0 = new, so one sort orders new before old.
 // The row's neighbours are also pinned into its buckets for this iteration's
-// join (at most BUCKET per bucket, in any order; the other slots empty).
+// join (at most BUCKET per bucket, in any order; the other slots empty), and
+// the row's threshold for the join is set.
 __global__ __launch_bounds__(64) void k_sample_select(
     uint64_t* __restrict__ G, uint32_t n, uint32_t k, uint32_t m, uint64_t seed, uint64_t iter,
     const uint32_t* __restrict__ roff, const uint64_t* __restrict__ rev,
     uint32_t* __restrict__ newl, uint32_t* __restrict__ oldl, uint32_t* __restrict__ ncnt,
-    uint64_t* __restrict__ slots, uint32_t S) {
+    uint64_t* __restrict__ slots, uint32_t S, float* __restrict__ thr, uint32_t thr_rank) {
   __shared__ uint64_t s_fk[WAVE];     // forward keys
   __shared__ uint64_t s_kkey[WAVE];   // key of the reverse entry that replaced a forward one
   __shared__ uint64_t s_pin[256];     // bucket layout, S <= 256
@@ -206,6 +207,9 @@ __global__ __launch_bounds__(64) void k_sample_select(
   const uint64_t fkey = l < k ? G[(size_t)i * k + l] : EMPTY_KEY;
   const bool fvalid = fkey != EMPTY_KEY;
   const uint32_t me = key_idx(fkey);
+  // the join's threshold: distance of entry thr_rank - 1 (the row's worst when
+  // thr_rank = k), infinite while the row holds fewer entries
+  if (l == thr_rank - 1) thr[i] = fvalid ? key_dist(fkey) : __builtin_inff();
   uint64_t fk = EMPTY_KEY;
   if (fvalid) {
     const uint64_t cls = (fkey & 1) ? 0ull : 1ull;
@@ -337,7 +341,7 @@ __global__ __launch_bounds__(64) void k_join(
   __syncthreads();
   const uint32_t P1 = nn * (nn - 1) / 2, P = P1 + nn * no;
   uint32_t pcnt = 0;
-  uint64_t pairs = 0;
+  uint64_t pairs = 0, sent = 0;
   for (uint32_t base = 0; base < P; base += WAVE) {
     const uint32_t t = base + l;
     bool pa = false, pb = false;
@@ -355,8 +359,8 @@ __global__ __launch_bounds__(64) void k_join(
         pb = dist < s_thr[b];
       }
     }
-    pb_append(pb_row, pb_key, pcnt, pa, ia, make_key(dist, ib, 1));
-    pb_append(pb_row, pb_key, pcnt, pb, ib, make_key(dist, ia, 1));
+    pb_append(pb_row, pb_key, pcnt, pa, ia, make_key(dist, ib, 1), sent);
+    pb_append(pb_row, pb_key, pcnt, pb, ib, make_key(dist, ia, 1), sent);
     pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, bmask, seed, iter);
   }
   pb_flush(pb_row, pb_key, pcnt, true, slots, touched, S, bmask, seed, iter);
@@ -367,6 +371,7 @@ __global__ __launch_bounds__(64) void k_join(
   if (l == 0) {
     ctr_add(counters, 0, (unsigned long long)tot);
     ctr_add(counters, 1, (unsigned long long)nc);
+    ctr_add(counters, 4, (unsigned long long)sent);
   }
 }
 
@@ -384,8 +389,8 @@ __global__ __launch_bounds__(64) void k_join(
 template <int NR, int NT>
 __global__ __launch_bounds__(64) void k_merge(uint64_t* __restrict__ G, uint32_t n, uint32_t k,
                                               uint64_t* __restrict__ slots, uint32_t S,
-                                              uint8_t* __restrict__ touched, float* __restrict__ thr,
-                                              int force, unsigned long long* __restrict__ changed) {
+                                              uint8_t* __restrict__ touched, int force,
+                                              unsigned long long* __restrict__ changed) {
   __shared__ uint64_t s_prop[WAVE];
   const uint32_t l = (uint32_t)lane_id();
   const uint32_t p = blockIdx.x;
@@ -418,10 +423,7 @@ __global__ __launch_bounds__(64) void k_merge(uint64_t* __restrict__ G, uint32_t
     const uint32_t nold =
         nk > 0 ? (uint32_t)__popcll(__ballot(orow != EMPTY_KEY && orow <= kth)) : 0u;
     if (l < k) row[l] = l < nk ? mg : EMPTY_KEY;
-    if (l == 0) {
-      thr[p] = nk == k ? key_dist(kth) : __builtin_inff();
-      nchanged = nk - nold;
-    }
+    if (l == 0) nchanged = nk - nold;
   } else {
     uint64_t u[NT];
 #pragma unroll
@@ -446,14 +448,12 @@ __global__ __launch_bounds__(64) void k_merge(uint64_t* __restrict__ G, uint32_t
       if (keep && rank < k) {
         row[rank] = u[r];
         nchanged += !found;
-        if (rank == k - 1) thr[p] = key_dist(u[r]);
       }
       before += (uint32_t)__popcll(mask);
       lastkey = shfl64(u[r], WAVE - 1);
     }
     const uint32_t nkept = before < k ? before : k;
     for (uint32_t pos = nkept + l; pos < k; pos += WAVE) row[pos] = EMPTY_KEY;
-    if (nkept < k && l == 0) thr[p] = __builtin_inff();
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) nchanged += __shfl_xor(nchanged, o, WAVE);
   }
@@ -530,7 +530,7 @@ template __global__ void k_join<50>(const float*, uint32_t, uint32_t, uint32_t, 
 
 #define PSQ_MERGE_INST(NR, NT)                                                                 \
   template __global__ void k_merge<NR, NT>(uint64_t*, uint32_t, uint32_t, uint64_t*, uint32_t,   \
-                                           uint8_t*, float*, int, unsigned long long*);
+                                           uint8_t*, int, unsigned long long*);
 PSQ_MERGE_INST(1, 2)
 PSQ_MERGE_INST(2, 4)
 PSQ_MERGE_INST(4, 8)

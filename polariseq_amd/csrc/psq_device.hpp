@@ -340,7 +340,7 @@ __host__ __device__ inline void pair_decode(uint32_t t, uint32_t nn, uint32_t no
 // held between calls and one append adds at most 128.
 constexpr uint32_t PB_CAP = 192;
 __device__ inline void pb_append(uint32_t* pb_row, uint64_t* pb_key, uint32_t& cnt, bool has,
-                                 uint32_t row, uint64_t key) {
+                                 uint32_t row, uint64_t key, uint64_t& sent) {
   const uint64_t mask = __ballot(has);
   if (has) {
     const uint32_t pos = cnt + (uint32_t)__popcll(mask & ((1ull << lane_id()) - 1ull));
@@ -348,6 +348,7 @@ __device__ inline void pb_append(uint32_t* pb_row, uint64_t* pb_key, uint32_t& c
     pb_key[pos] = key;
   }
   cnt += (uint32_t)__popcll(mask);
+  sent += (uint64_t)__popcll(mask);
 }
 // Sends every full batch of 64 (all entries when `all`), keeping the rest.
 __device__ inline void pb_flush(uint32_t* pb_row, uint64_t* pb_key, uint32_t& cnt, bool all,

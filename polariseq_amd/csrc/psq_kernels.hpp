@@ -23,7 +23,7 @@ __global__ void k_rev_scatter(const uint64_t* G, uint32_t n, uint32_t k, uint64_
 __global__ void k_sample_select(uint64_t* G, uint32_t n, uint32_t k, uint32_t m, uint64_t seed,
                                 uint64_t iter, const uint32_t* roff, const uint64_t* rev,
                                 uint32_t* newl, uint32_t* oldl, uint32_t* ncnt, uint64_t* slots,
-                                uint32_t S);
+                                uint32_t S, float* thr, uint32_t thr_rank);
 template <int D>
 __global__ void k_join(const float* Xp, uint32_t n, uint32_t d, uint32_t dp, uint32_t RS,
                        const uint32_t* newl,
@@ -32,7 +32,7 @@ __global__ void k_join(const float* Xp, uint32_t n, uint32_t d, uint32_t dp, uin
                        unsigned long long* counters);
 template <int NR, int NT>
 __global__ void k_merge(uint64_t* G, uint32_t n, uint32_t k, uint64_t* slots, uint32_t S,
-                        uint8_t* touched, float* thr, int force, unsigned long long* changed);
+                        uint8_t* touched, int force, unsigned long long* changed);
 __global__ void k_row_counts(const uint64_t* G, uint32_t n, uint32_t k, uint32_t ko, const uint32_t* inv,
                              uint32_t* cnt);
 __global__ void k_extract(const uint64_t* G, uint32_t n, uint32_t k, uint32_t ko, const uint32_t* perm,
