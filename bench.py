@@ -33,6 +33,22 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` on the default workload, from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised in
+    profiles/join_4m_hbm_traffic.md (scaled by the copy-kernel calibration)."""
+    path = os.path.join(ROOT, "profiles", "join_4m_hbm_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 4 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                    return int(parts[3])
+    except OSError:
+        return None
+    return None
+
+
 def host_threads() -> int:
     env = os.environ.get("OMP_NUM_THREADS")
     if env and env.isdigit() and int(env) > 0:
@@ -63,9 +79,13 @@ def main() -> int:
     ap.add_argument("--recall-queries", type=int, default=100_000)
     ap.add_argument("--cpu-cells", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="headline", choices=["headline", "precision", "batched"],
+                    help="headline = the metric above; precision = BF16 form on 4.06M x 50 "
+                         "with 5%% outliers, k=30, distance error on a 50k subset; batched = "
+                         "per-sample exact kNN of 64 samples with doublets in one launch")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="only run 5 device-to-device copies of 1 GiB (the known-size kernel the "
-                         "HBM counters of a profile are scaled by)")
+                    help="only run 5 elementwise copies of 1 GiB (y = x * 1, a kernel of known "
+                         "traffic: 1 GiB read + 1 GiB written) to scale the HBM counters by")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -84,10 +104,15 @@ def main() -> int:
         a = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
         b = torch.empty_like(a)
         for _ in range(5):
-            b.copy_(a)
+            torch.mul(a, 1.0, out=b)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_per_dispatch": a.numel() * 4, "dispatches": 5}), flush=True)
+        print(json.dumps({"read_bytes_per_dispatch": a.numel() * 4,
+                          "write_bytes_per_dispatch": b.numel() * 4, "dispatches": 5}), flush=True)
         return 0
+    if args.workload == "precision":
+        return precision_workload(args, torch, P, ds)
+    if args.workload == "batched":
+        return batched_workload(args, torch, P, ds)
     n, k = args.n, args.k
     log(f"generating {n} x 50 mixture ({args.components} components, seed {args.seed})")
     x = ds.mixture(n, 50, args.components, seed=args.seed)
@@ -176,9 +201,13 @@ def main() -> int:
             join_ms += s_["ms_join"][it]
             launches += 1
     achieved = join_bytes / (join_ms / 1000.0) / 1e9 if join_ms > 0 else 0.0
+    kname = "k_join_bf16<50>" if "bf16" in head else "k_join<50>"
+    default_workload = (n, k, args.components, args.seed, args.init) == (4_060_000, 15, 150, 0,
+                                                                          "rp_forest")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_join_bf16" if "bf16" in head else "k_join<50,1>",
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(kname) if default_workload else None,
+                "kernel": kname,
                 "launches": launches, "avg_launch_ms": round(join_ms / max(launches, 1), 3),
                 "algorithmic_bytes_per_launch": round(join_bytes / max(launches, 1))}
 
@@ -234,6 +263,115 @@ def main() -> int:
                            "steps": train_info["steps"], "subset": args.train_subset,
                            "last_loss": round(train_info["last_loss"], 5),
                            "note": "trained once before the timed steps and reused by every step"}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def _bf16(a: np.ndarray) -> np.ndarray:
+    """float32 -> nearest BF16 value (ties to even), as float32."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def precision_workload(args, torch, P, ds) -> int:
+    """4.06M x 50 with 5% uniform outliers (150 components), k = 30, BF16 form
+    (rows of 2k = 60, each ranked in FP32), seed 0. Reports cells/s, recall@30
+    over 20k queries and the distance error against FP64 on a 50k-cell subset:
+    of the output distances, and of the BF16 screen's squared distance for
+    candidate pairs of the subset's cells (recomputed on the host: operands
+    centred on the cell and rounded to BF16, sums in FP32)."""
+    n, k = args.n, 30
+    x = ds.mixture_with_outliers(n, 50, args.components, seed=args.seed, outlier_frac=0.05)
+    xt = torch.from_numpy(x).cuda()
+    torch.cuda.synchronize()
+    step = lambda: P.nndescent_device(xt, k, seed=args.seed, precision="bf16")  # noqa: E731
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = None
+    for _ in range(args.steps):
+        out = step()
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / args.steps
+    st = P.last_stats()
+    ix = out[1][: n * k].view(n, k)
+    dx = out[2][: n * k].view(n, k)
+    rng = np.random.default_rng(777)
+    q = np.sort(rng.choice(n, 20000, replace=False)).astype(np.int32)
+    qt = torch.from_numpy(q).cuda()
+    ei, _ = P.knn_exact_device(xt, k, qt)
+    rec = ds.recall(ix[qt.long()].cpu().numpy(), ei.cpu().numpy())
+    sub = np.sort(rng.choice(n, 50000, replace=False))
+    si = ix[torch.from_numpy(sub).cuda().long()].cpu().numpy().astype(np.int64)
+    sd = dx[torch.from_numpy(sub).cuda().long()].cpu().numpy().astype(np.float64)
+    xs = x[sub].astype(np.float64)
+    diff = x[si].astype(np.float64) - xs[:, None, :]
+    d64 = np.sqrt((diff * diff).sum(-1))
+    ok = d64 > 0
+    rel_out = np.abs(sd[ok] - d64[ok]) / d64[ok]
+    # BF16 screen, as in a join: two candidates p, q of cell c (here: c's 1st-10th
+    # and 11th-20th output neighbours), both centred on x_c and rounded to BF16;
+    # d~ = |p~|^2 + |q~|^2 - 2 p~.q~ with FP32 sums, against FP64 |x_p - x_q|^2
+    cp = _bf16((x[si[:, :10]] - x[sub][:, None, :]).astype(np.float32))
+    cq = _bf16((x[si[:, 10:20]] - x[sub][:, None, :]).astype(np.float32))
+    dt = ((cp * cp).sum(-1, dtype=np.float32) + (cq * cq).sum(-1, dtype=np.float32)
+          - np.float32(2) * (cp * cq).sum(-1, dtype=np.float32)).astype(np.float64)
+    e = x[si[:, :10]].astype(np.float64) - x[si[:, 10:20]].astype(np.float64)
+    sq64 = (e * e).sum(-1)
+    okp = sq64 > 0
+    rel_screen = np.abs(np.maximum(dt[okp], 0.0) - sq64[okp]) / sq64[okp]
+    line = {
+        "metric": "cells/s to kNN graph, BF16 form, k=30, 4.06M x 50 with 5% outliers",
+        "value": round(n / el, 1), "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1000 * el, 3), "higher_is_better": True,
+        "dtype": "bf16", "data": "synthetic Gaussian mixture + uniform outliers (numpy, seeded)",
+        "config": {"workload": f"BF16 MFMA screen + FP32 re-rank, {n} x 50, k={k}, 5% outliers, "
+                               f"{args.components} components, seed {args.seed}",
+                   "heap_k": st["heap_k"], "iterations": st["iterations"]},
+        "recall_at_30": round(float(rec.mean()), 5),
+        "distance_error_vs_fp64_50k_subset": {
+            "output_max_rel": float(rel_out.max()), "output_p99_rel": float(np.percentile(rel_out, 99)),
+            "edges": int(ok.sum()),
+            "bf16_screen_sq_max_rel": float(rel_screen.max()),
+            "bf16_screen_sq_p99_rel": float(np.percentile(rel_screen, 99)),
+            "bf16_screen_pairs": int(okp.sum())},
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def batched_workload(args, torch, P, ds) -> int:
+    """64 samples, sizes lognormal(10, 1.2) clipped to 2k-200k cells, each an
+    independent 20-component mixture shifted by N(0, I), plus 2x doublets; exact
+    kNN (k = 15) of every point within its sample, all samples in one persistent
+    launch per pass, tiles from the autotuner. Timed after one untimed call (which
+    runs the autotuner)."""
+    samples = ds.batched_samples(64, 50, 20, seed=args.seed)
+    x = np.concatenate([s[0] for s in samples])
+    off = np.zeros(len(samples) + 1, np.uint64)
+    off[1:] = np.cumsum([s[0].shape[0] for s in samples])
+    xt = torch.from_numpy(x).cuda()
+    P.knn_exact_batched_device(xt, off, 15)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        P.knn_exact_batched_device(xt, off, 15)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / args.steps
+    rep = P.batched_report()
+    pairs = float(sum(int(off[i + 1] - off[i]) ** 2 for i in range(len(samples))))
+    line = {
+        "metric": "points/s, per-sample exact kNN (k=15), 64 samples with doublets, one launch",
+        "value": round(x.shape[0] / el, 1), "unit": "points/s", "n_gpus": 1, "steps": args.steps,
+        "ms_per_step": round(1000 * el, 3), "higher_is_better": True, "dtype": "f32",
+        "data": "synthetic per-sample mixtures with doublets (numpy, seeded)",
+        "config": {"workload": f"64 samples, {x.shape[0]} points in total, sizes "
+                               f"{int(np.diff(off).min())}-{int(np.diff(off).max())}, k=15",
+                   "autotuner": rep},
+        "distance_tflops": round(3 * 50 * pairs / el / 1e12, 2),
+    }
     print(json.dumps(line), flush=True)
     return 0
 
