@@ -38,7 +38,8 @@ extern "C" {
 #define PSQ_E_NOMEM (-4)       /* device memory (or the debug budget) exhausted */
 
 #define PSQ_PRECISION_FP32 0        /* plain form: exact f32 sq_dist of the reference */
-#define PSQ_PRECISION_BF16_RERANK 1 /* BF16 MFMA join tiles, FP32 re-rank of each row's top 2k */
+#define PSQ_PRECISION_BF16_RERANK 1 /* BF16 MFMA tiles screen the join's pairs, screened pairs
+                                       get the exact f32 distance; rows keep their top 2k */
 
 #define PSQ_INIT_RP_FOREST 0 /* random-projection forest (the reference's start) */
 #define PSQ_INIT_RANDOM 1    /* random fill only */
@@ -50,7 +51,7 @@ typedef struct psq_nndescent_opts {
   uint64_t k;              /* neighbours per node (>= 1) */
   uint64_t max_candidates; /* 0 -> min(60, 2k) */
   uint64_t max_iterations; /* 0 -> max(5, round(log2 n)) */
-  double delta;            /* stop when changed <= floor(delta * n * k); default 0.001 */
+  double delta;            /* stop when changed <= floor(delta * n * heap_k); default 0.001 */
   uint64_t n_trees;        /* 0 -> min(48, 8 + round(sqrt(n)/10)) */
   uint64_t leaf_size;      /* 0 -> max(10, k) */
   uint64_t gather_max;     /* accepted for compatibility; a memory choice that never changes the graph */
