@@ -201,6 +201,7 @@ __global__ __launch_bounds__(64) void k_sample_select(
   __shared__ uint64_t s_kkey[WAVE];   // key of the reverse entry that replaced a forward one
   __shared__ uint64_t s_pin[256];     // bucket layout, S <= 256
   __shared__ uint32_t s_bcnt[32];     // pinned per bucket
+  __shared__ uint32_t s_ht[128];      // forward neighbour -> lane + 1 (open addressing)
   const uint32_t l = (uint32_t)lane_id();
   const uint32_t i = blockIdx.x;
   if (i >= n) return;  // block-uniform
@@ -219,6 +220,13 @@ __global__ __launch_bounds__(64) void k_sample_select(
   s_fk[l] = fk;
   s_kkey[l] = EMPTY_KEY;
   if (l < 32) s_bcnt[l] = 0;
+  s_ht[l] = 0;
+  s_ht[l + WAVE] = 0;
+  __syncthreads();
+  if (fvalid) {  // forward neighbours are distinct
+    for (uint32_t h = (me * 0x9E3779B1u) >> 25;; h = (h + 1) & 127u)
+      if (atomicCAS(&s_ht[h], 0u, l + 1) == 0u) break;
+  }
   __syncthreads();
   const uint32_t r0 = roff[i], nrev = roff[i + 1] - r0, T = k + nrev;
   const uint32_t nvec = (T + WAVE - 1) / WAVE;
@@ -230,17 +238,20 @@ __global__ __launch_bounds__(64) void k_sample_select(
     const uint64_t rv = rev[r0 + (t - k)];
     const uint32_t src = (uint32_t)rv;
     const uint64_t cp = rv >> 32;  // class and priority
-    for (uint32_t f = 0; f < k; ++f) {
+    // the forward entry with this neighbour, if any (forward neighbours are distinct)
+    for (uint32_t h = (src * 0x9E3779B1u) >> 25;; h = (h + 1) & 127u) {
+      const uint32_t slot = s_ht[h];
+      if (slot == 0) return rv;
+      const uint32_t f = slot - 1;
       const uint64_t fv = s_fk[f];
-      if (fv != EMPTY_KEY && (uint32_t)fv == src && (fv >> 63) == (cp >> 31)) {
-        if ((cp & 0x7FFFFFFFull) < ((fv >> 32) & 0x7FFFFFFFull)) {
-          if (record) s_kkey[f] = rv;
-        } else {
-          return EMPTY_KEY;
-        }
+      if ((uint32_t)fv != src) continue;
+      if ((fv >> 63) != (cp >> 31)) return rv;  // other class: both stay
+      if ((cp & 0x7FFFFFFFull) < ((fv >> 32) & 0x7FFFFFFFull)) {
+        if (record) s_kkey[f] = rv;
+        return rv;
       }
+      return EMPTY_KEY;
     }
-    return rv;
   };
   // pass 1: every replacement is known before a forward key is used
   uint64_t key0 = rev_key(l, true);
