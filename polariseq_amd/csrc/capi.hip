@@ -354,7 +354,7 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
   const uint32_t RSs = join_row_stride(ds);
   const uint32_t cap = ls < 128 ? 256u : 0u;
   const uint32_t NC = cap / (ls + 1) + 2;
-  const size_t fin_smem = ((size_t)7 * cap + 2 + (size_t)13 * NC + (size_t)2 * NC * RSs) * 4;
+  const size_t fin_smem = ((size_t)8 * cap + 2 + (size_t)13 * NC + (size_t)2 * NC * RSs) * 4;
   if (cap > 0 && fin_smem > 65536)
     HIPC(hipFuncSetAttribute((const void*)k_forest_finish,
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)fin_smem));
@@ -377,7 +377,7 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
   TRY(wbuf(ws, "f_lcnt", total_cap, &lcnt_pos));
   uint32_t *ns[2], *nc[2], *nt[2], *nlv, *mode, *nact, *nsm, *aoff, *soff, *child;
   uint32_t *sm_start, *sm_count, *sm_tree, *sm_level, *toff;
-  float *normals, *offsets;
+  float *normals, *offsets, *nabs;
   TRY(wbuf(ws, "f_ns0", node_cap, &ns[0]));
   TRY(wbuf(ws, "f_ns1", node_cap, &ns[1]));
   TRY(wbuf(ws, "f_nc0", node_cap, &nc[0]));
@@ -393,6 +393,15 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
   TRY(wbuf(ws, "f_child", 2 * node_cap, &child));
   TRY(wbuf(ws, "f_normals", node_cap * dps, &normals));
   TRY(wbuf(ws, "f_offsets", node_cap, &offsets));
+  TRY(wbuf(ws, "f_nabs", node_cap, &nabs));
+  // FP16 copy of the split rows for the margin screen (64- or 128-byte rows)
+  const uint32_t dh = ds <= 32 ? 32u : round_up(ds, 64);
+  _Float16* Xh;
+  TRY(wbuf(ws, "f_xh", (size_t)n * dh, &Xh));
+  hipLaunchKernelGGL(k_rows_to_half, dim3(grid_for((size_t)n * dh, 256)), dim3(256), 0, s, Xs, ds,
+                     dps, n, Xh, dh);
+  HIPC(hipGetLastError());
+
   TRY(wbuf(ws, "f_sm_start", node_cap, &sm_start));
   TRY(wbuf(ws, "f_sm_count", node_cap, &sm_count));
   TRY(wbuf(ws, "f_sm_tree", node_cap, &sm_tree));
@@ -433,10 +442,16 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
     for (uint32_t L = 0; A > 0; ++L) {
       const int nx = cur ^ 1;
       hipLaunchKernelGGL(k_forest_planes, dim3(grid_for(A, 4)), dim3(256), 0, s, Xs, ds, dps, n,
-                         ids[cur], ns[cur], nc[cur], nt[cur], A, tseeds + t0, L, normals, offsets);
+                         ids[cur], ns[cur], nc[cur], nt[cur], A, tseeds + t0, L, normals, offsets,
+                         nabs);
       HIPC(hipGetLastError());
       hipLaunchKernelGGL(k_forest_sides, dim3(grid_for(total, 256)), dim3(256), 0, s, Xs, ds, dps, n,
-                         total, ids[cur], segv[cur], nt[cur], tseeds + t0, L, normals, offsets, fl, fr);
+                         Xh, dh, total, ids[cur], segv[cur], nt[cur], tseeds + t0, L, normals,
+                         offsets, nabs, fl, fr);
+      HIPC(hipGetLastError());
+      hipLaunchKernelGGL(k_forest_sides_exact, dim3(grid_for(total, 256)), dim3(256), 0, s, Xs, ds,
+                         dps, total, ids[cur], segv[cur], nt[cur], tseeds + t0, L, normals, offsets,
+                         fl);
       HIPC(hipGetLastError());
       TRY(scan_u32(ws, fl, sl, (uint32_t)total, s));
       TRY(scan_u32(ws, fr, sr, (uint32_t)total, s));

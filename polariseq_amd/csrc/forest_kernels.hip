@@ -23,6 +23,67 @@ namespace psq {
 
 constexpr uint32_t NONE = 0xFFFFFFFFu;
 
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+
+// FP16 copy of the split rows (dh halves per row, zero padded), read by the
+// margin screen below: one 64- or 128-byte line per row instead of two.
+__global__ void k_rows_to_half(const float* __restrict__ X, uint32_t d, uint32_t dp, uint32_t n,
+                               _Float16* __restrict__ Xh, uint32_t dh) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)n * dh) return;
+  const size_t r = e / dh;
+  const uint32_t j = (uint32_t)(e - r * dh);
+  Xh[e] = (_Float16)(j < d ? X[r * dp + j] : 0.0f);
+}
+
+// Which side of the hyperplane (normal nv, offset off, nabs = sum |nv_j|) a
+// point lies on. The reference decides on m = dot_ref(nv, x) + off (f32,
+// simd.rs:22-37 order; |m| < 1e-8 takes hash bit 2, nndescent.rs:439-444).
+// Screen: m~ = sum nv_j * half(x_j) + off from the FP16 copy, A = sum |nv_j *
+// half(x_j)|. FP16 rounding moves each term by at most 2^-11 of itself (2^-25
+// absolutely for subnormals), and both f32 sums are within d * 2^-24 * A of
+// their real values, so |m~ - m| <= 1e-3 A + 1e-7 nabs + 1e-6 |m~| =: E (a
+// factor 2 to spare at d <= 256). When |m~| > E + 1e-8, m has the sign of m~
+// and is not a tie: returns 1 (left) or 0. Otherwise returns 2: the exact
+// margin (forest_side_exact) decides, in a separate pass so that the few
+// points near a plane do not hold up whole waves.
+__device__ inline uint32_t forest_side_screen(const float* nv, float off, float nabs,
+                                              const _Float16* __restrict__ xh, uint32_t d) {
+  // nv and xh are 16-byte aligned; chunks of 8 with 8 independent sums
+  float m[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  float a[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  const uint32_t full = d >> 3;
+  for (uint32_t c = 0; c < full; ++c) {
+    const half8 h = *reinterpret_cast<const half8*>(xh + 8 * c);
+    const float4 n0 = *reinterpret_cast<const float4*>(nv + 8 * c);
+    const float4 n1 = *reinterpret_cast<const float4*>(nv + 8 * c + 4);
+    const float p[8] = {n0.x * (float)h[0], n0.y * (float)h[1], n0.z * (float)h[2],
+                        n0.w * (float)h[3], n1.x * (float)h[4], n1.y * (float)h[5],
+                        n1.z * (float)h[6], n1.w * (float)h[7]};
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      m[u] += p[u];
+      a[u] += fabsf(p[u]);
+    }
+  }
+  float mt = ((m[0] + m[4]) + (m[1] + m[5])) + ((m[2] + m[6]) + (m[3] + m[7]));
+  float A = ((a[0] + a[4]) + (a[1] + a[5])) + ((a[2] + a[6]) + (a[3] + a[7]));
+  for (uint32_t j = full * 8; j < d; ++j) {
+    const float p = nv[j] * (float)xh[j];
+    mt += p;
+    A += fabsf(p);
+  }
+  mt += off;
+  if (fabsf(mt) > 1e-3f * A + 1e-7f * nabs + 1e-6f * fabsf(mt) + 1e-8f) return mt > 0.0f ? 1u : 0u;
+  return 2u;
+}
+__device__ inline uint32_t forest_side_exact(const float* nv, float off, const float* __restrict__ x,
+                                             uint32_t d, uint64_t ts, uint32_t level, uint32_t g) {
+  const float margin = dot_ref(nv, x, (int)d) + off;
+  if (fabsf(margin) < 1e-8f) return (tree_hash(ts, level, g, 2) & 1) == 0 ? 1u : 0u;
+  return margin > 0.0f ? 1u : 0u;
+}
+
 __global__ void k_forest_init(uint32_t* __restrict__ ids, uint32_t* __restrict__ segv, uint32_t n,
                               uint32_t T, uint32_t leaf_size) {
   size_t pos = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -38,7 +99,7 @@ __global__ __launch_bounds__(256) void k_forest_planes(
     const uint32_t* __restrict__ ids, const uint32_t* __restrict__ nstart,
     const uint32_t* __restrict__ ncount, const uint32_t* __restrict__ ntree, uint32_t A,
     const uint64_t* __restrict__ tseeds, uint32_t level, float* __restrict__ normals,
-    float* __restrict__ offsets) {
+    float* __restrict__ offsets, float* __restrict__ nabs) {
   const int w = threadIdx.x / WAVE, l = lane_id();
   const uint32_t v = blockIdx.x * 4 + w;
   if (v >= A) return;
@@ -53,24 +114,28 @@ __global__ __launch_bounds__(256) void k_forest_planes(
   float* nv = normals + (size_t)v * dp;
   for (uint32_t j = l; j < dp; j += WAVE) nv[j] = j < d ? xa[j] - xb[j] : 0.0f;
   if (l == 0) {
-    float s = -0.0f;
+    float s = -0.0f, na = 0.0f;
     for (uint32_t j = 0; j < d; ++j) {
       float nj = xa[j] - xb[j];
       s = s + nj * 0.5f * (xa[j] + xb[j]);
+      na += fabsf(nj);
     }
     offsets[v] = -s;
+    nabs[v] = na;
   }
 }
 
-// One thread per position of every tree: side bits for the hyperplane and for
-// the hash fallback.
+// One thread per position of every tree: side bits for the hyperplane (FP16
+// screen, exact f32 margin near the plane) and for the hash fallback.
 __global__ void k_forest_sides(const float* __restrict__ Xp, uint32_t d, uint32_t dp, uint32_t n,
+                               const _Float16* __restrict__ Xh, uint32_t dh,
                                size_t total, const uint32_t* __restrict__ ids,
                                const uint32_t* __restrict__ segv, const uint32_t* __restrict__ ntree,
                                const uint64_t* __restrict__ tseeds, uint32_t level,
                                const float* __restrict__ normals, const float* __restrict__ offsets,
+                               const float* __restrict__ nabs,
                                uint32_t* __restrict__ fl, uint32_t* __restrict__ fr) {
-  size_t pos = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t pos = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (pos >= total) return;
   const uint32_t v = segv[pos];
   if (v == NONE) {
@@ -80,14 +145,24 @@ __global__ void k_forest_sides(const float* __restrict__ Xp, uint32_t d, uint32_
   }
   const uint32_t i = ids[pos];
   const uint64_t ts = tseeds[ntree[v]];
-  const float margin = dot_ref(normals + (size_t)v * dp, Xp + (size_t)i * dp, (int)d) + offsets[v];
-  bool left;
-  if (fabsf(margin) < 1e-8f)
-    left = (tree_hash(ts, level, i, 2) & 1) == 0;
-  else
-    left = margin > 0.0f;
-  fl[pos] = left ? 1u : 0u;
+  // 2 = undecided, settled by k_forest_sides_exact
+  fl[pos] = forest_side_screen(normals + (size_t)v * dp, offsets[v], nabs[v], Xh + (size_t)i * dh, d);
   fr[pos] = (tree_hash(ts, level, i, 3) & 1) == 0 ? 1u : 0u;
+}
+
+// The positions the screen left undecided (side 2): exact f32 margins.
+__global__ void k_forest_sides_exact(const float* __restrict__ Xp, uint32_t d, uint32_t dp,
+                                     size_t total, const uint32_t* __restrict__ ids,
+                                     const uint32_t* __restrict__ segv,
+                                     const uint32_t* __restrict__ ntree,
+                                     const uint64_t* __restrict__ tseeds, uint32_t level,
+                                     const float* __restrict__ normals,
+                                     const float* __restrict__ offsets, uint32_t* __restrict__ fl) {
+  const size_t pos = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos >= total || fl[pos] != 2u) return;
+  const uint32_t v = segv[pos], i = ids[pos];
+  fl[pos] = forest_side_exact(normals + (size_t)v * dp, offsets[v], Xp + (size_t)i * dp, d,
+                              tseeds[ntree[v]], level, i);
 }
 
 // One thread per active node: split mode and how many children it emits.
@@ -255,20 +330,22 @@ __device__ inline float dot_ref_row4(const float* nv, const float* __restrict__ 
 // kernels would build. The subtree's rows, read once per level, stay in the
 // cache; all bookkeeping is in LDS. Leaves are written as counts at their
 // start positions; the final order of the subtree's positions goes back to ids.
-// LDS (u32 unless noted): gid[C] | lid[2][C] | pn[2][C] | scan_l[C+1] |
-// scan_r[C+1] | nst[2][NC] | ncn[2][NC] | mode[NC] | nl[NC] | child[2 NC] |
-// piv_a[NC] | piv_b[NC] | offsets[NC] f32 | normals[NC][RSs] f32 |
-// terms[NC][RSs] f32. NC bounds the active nodes of one level
-// (C / (leaf_size + 1) + 1).
+// LDS (u32 unless noted): normals[NC][RSs] f32 | terms[NC][RSs] f32 |
+// offsets[NC] f32 | gid[C] | lid[2][C] | pn[2][C] | scan_l[C+1] | scan_r[C+1] |
+// nst[2][NC] | ncn[2][NC] | mode[NC] | nl[NC] | child[2 NC] | piv_a[NC] |
+// piv_b[NC]. NC bounds the active nodes of one level (C / (leaf_size + 1) + 1).
 __global__ __launch_bounds__(64) void k_forest_finish(
     const float* __restrict__ Xs, uint32_t ds, uint32_t dps, uint32_t RSs, uint32_t n,
     uint32_t* __restrict__ ids, const uint32_t* __restrict__ sm_start,
     const uint32_t* __restrict__ sm_count, const uint32_t* __restrict__ sm_tree,
     const uint32_t* __restrict__ sm_level, const uint64_t* __restrict__ tseeds, uint32_t leaf_size,
     uint32_t C, uint32_t NC, uint32_t* __restrict__ lcnt_pos) {
-  extern __shared__ __attribute__((aligned(16))) uint32_t ldsu[];
+  extern __shared__ __attribute__((aligned(16))) float ldsf[];
   __shared__ uint32_t s_nA;
-  uint32_t* gid = ldsu;
+  float* normals = ldsf;                            // NC * RSs, 16-byte aligned rows
+  float* terms = normals + (size_t)NC * RSs;        // NC * RSs
+  float* offs = terms + (size_t)NC * RSs;           // NC
+  uint32_t* gid = reinterpret_cast<uint32_t*>(offs + NC);
   uint32_t* lid[2] = {gid + C, gid + 2 * C};
   uint32_t* pn[2] = {gid + 3 * C, gid + 4 * C};
   uint32_t* scl = gid + 5 * C;
@@ -280,9 +357,6 @@ __global__ __launch_bounds__(64) void k_forest_finish(
   uint32_t* child = nnl + NC;
   uint32_t* piva = child + 2 * NC;
   uint32_t* pivb = piva + NC;
-  float* offs = reinterpret_cast<float*>(pivb + NC);
-  float* normals = offs + NC;
-  float* terms = normals + (size_t)NC * RSs;
   const uint32_t l = (uint32_t)lane_id();
   const uint32_t b = blockIdx.x;
   const uint32_t start = sm_start[b], c = sm_count[b], t = sm_tree[b], L0 = sm_level[b];
@@ -331,19 +405,18 @@ __global__ __launch_bounds__(64) void k_forest_finish(
       offs[v] = -sacc;
     }
     __syncthreads();
-    // sides
+    // sides: the exact f32 margin (the subtree's rows are in cache; this phase
+    // is bound by latency, not bytes)
     for (uint32_t pos = l; pos < c; pos += WAVE) {
       const uint32_t v = pn[cur][pos];
       uint32_t fl = 0, fr = 0;
       if (v != NONE) {
         const uint32_t g = gid[lid[cur][pos]];
         const float margin = dot_ref_row4(normals + (size_t)v * RSs, Xs + (size_t)g * dps, ds) + offs[v];
-        bool left;
         if (fabsf(margin) < 1e-8f)
-          left = (tree_hash(ts, L, g, 2) & 1) == 0;
+          fl = (tree_hash(ts, L, g, 2) & 1) == 0 ? 1u : 0u;
         else
-          left = margin > 0.0f;
-        fl = left ? 1u : 0u;
+          fl = margin > 0.0f ? 1u : 0u;
         fr = (tree_hash(ts, L, g, 3) & 1) == 0 ? 1u : 0u;
       }
       scl[pos] = fl;

@@ -47,11 +47,18 @@ __global__ void k_forest_init(uint32_t* ids, uint32_t* segv, uint32_t n, uint32_
 __global__ void k_forest_planes(const float* Xp, uint32_t d, uint32_t dp, uint32_t n, const uint32_t* ids,
                                 const uint32_t* nstart, const uint32_t* ncount, const uint32_t* ntree,
                                 uint32_t A, const uint64_t* tseeds, uint32_t level, float* normals,
-                                float* offsets);
-__global__ void k_forest_sides(const float* Xp, uint32_t d, uint32_t dp, uint32_t n, size_t total,
-                               const uint32_t* ids, const uint32_t* segv, const uint32_t* ntree,
-                               const uint64_t* tseeds, uint32_t level, const float* normals,
-                               const float* offsets, uint32_t* fl, uint32_t* fr);
+                                float* offsets, float* nabs);
+__global__ void k_rows_to_half(const float* X, uint32_t d, uint32_t dp, uint32_t n, _Float16* Xh,
+                               uint32_t dh);
+__global__ void k_forest_sides(const float* Xp, uint32_t d, uint32_t dp, uint32_t n,
+                               const _Float16* Xh, uint32_t dh, size_t total, const uint32_t* ids,
+                               const uint32_t* segv, const uint32_t* ntree, const uint64_t* tseeds,
+                               uint32_t level, const float* normals, const float* offsets,
+                               const float* nabs, uint32_t* fl, uint32_t* fr);
+__global__ void k_forest_sides_exact(const float* Xp, uint32_t d, uint32_t dp, size_t total,
+                                     const uint32_t* ids, const uint32_t* segv,
+                                     const uint32_t* ntree, const uint64_t* tseeds, uint32_t level,
+                                     const float* normals, const float* offsets, uint32_t* fl);
 __global__ void k_forest_nodes(const uint32_t* nstart, const uint32_t* ncount, const uint32_t* ntree,
                                uint32_t A, uint32_t n, const uint32_t* sl, const uint32_t* sr,
                                uint32_t leaf_size, uint32_t cap, uint32_t level, uint32_t* nl_out,
