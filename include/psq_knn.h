@@ -161,6 +161,40 @@ int psq_encoder_grad(const psq_encoder* e, const float* d_data, uint64_t n, uint
                      double* loss_out);
 int psq_encoder_info(const psq_encoder* e, double* last_loss, double* train_ms, uint64_t* steps);
 
+/* Fuzzy connectivities of a kNN distance graph: the symmetric CSR that clustering
+ * and UMAP share (scanpy's obsp['connectivities']), replacing
+ * fuzzy_connectivities_from (crates/polariseq-core/src/umap.rs:130-139). Each
+ * row's membership strengths come from the bandwidth search of umap.rs:551-608;
+ * row i of the result is the fuzzy union w + w' - w * w' of its own entries with
+ * the entries of the rows that name i (umap.rs:488-537), sorted by column, with
+ * unions <= 1e-5 dropped. The input is a CSR with indptr[0] == 0, non-decreasing
+ * row pointers and every index < n_obs; the result has at most 2 * indptr[n_obs]
+ * entries. A result of 2^32 entries or more fails with PSQ_E_UNSUPPORTED, as the
+ * reference binding does (crates/polariseq-python/src/lib.rs:616-619). */
+
+/* Device form: inputs and outputs on the current device. The outputs hold
+ * n_obs + 1 row pointers and cap entries; cap >= 2 * indptr[n_obs] always
+ * suffices. *nnz_out is set also when cap is too small (then PSQ_E_INVALID and
+ * nothing is written). */
+int psq_fuzzy_connectivities_device(const uint32_t* d_indptr, const uint32_t* d_indices,
+                                    const float* d_dists, uint64_t n_obs, uint32_t* d_out_indptr,
+                                    uint32_t* d_out_indices, float* d_out_weights, uint64_t cap,
+                                    uint64_t* nnz_out, void* stream);
+
+/* Host form, mirroring FuzzyPlan (umap.rs:327-460) as fuzzy_connectivities_into
+ * uses it (crates/polariseq-python/src/lib.rs:590-642): psq_fuzzy_plan_new
+ * computes the graph on the GPU and reports its nnz (FuzzyPlan::new +
+ * row_lens), the caller allocates, psq_fuzzy_plan_fill copies the graph into
+ * its arrays (FuzzyPlan::fill), psq_fuzzy_plan_free releases the plan's device
+ * memory. nnz_in is the length of indices and dists (>= indptr[n_obs]). */
+typedef struct psq_fuzzy_plan psq_fuzzy_plan;
+int psq_fuzzy_plan_new(const uint32_t* indptr, uint64_t indptr_len, const uint32_t* indices,
+                       const float* dists, uint64_t nnz_in, uint64_t n_obs, int32_t device,
+                       psq_fuzzy_plan** plan, uint64_t* nnz_out);
+int psq_fuzzy_plan_fill(const psq_fuzzy_plan* plan, uint32_t* indptr, uint64_t indptr_len,
+                        uint32_t* indices, float* weights, uint64_t cap);
+void psq_fuzzy_plan_free(psq_fuzzy_plan* plan);
+
 /* Per-run statistics of the last NN-Descent call on this thread (for reports). */
 typedef struct psq_nndescent_stats {
   uint64_t n_obs, n_dim, k, max_candidates, max_iterations, n_trees, leaf_size, slots, stop_at;

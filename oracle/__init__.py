@@ -18,6 +18,12 @@ Contents (each C function cites the reference file:line it restates):
 * ``neighbors_exact`` — the reference's exact path (``core/neighbors.rs:101-241``).
 * ``gpu_model_nndescent`` — the GPU algorithm's documented semantics, step for
   step on the CPU. Its output must equal the HIP path's bit for bit.
+* ``fuzzy_strengths`` / ``fuzzy_connectivities`` — membership strengths and the
+  symmetrised fuzzy union of a kNN distance graph (``core/umap.rs:327-460``,
+  ``488-537``, ``551-608``). Pinned by the umap-learn values the reference tests
+  hold (``core/umap.rs:2261-2276``), by the sum-to-log2(k+1) property of the
+  bandwidth search, and by an independent undirected-edge formulation (the
+  reference's HashMap oracle, ``core/umap.rs:1891-1918``) in ``tests/``.
 * scalar ``sq_dist`` / ``dot`` / ``splitmix`` / ``hash_unit`` / ``hash_index``
   (``core/simd.rs:22-81``, ``core/nndescent.rs:80-100``).
 """
@@ -114,6 +120,11 @@ def _declare(L: ctypes.CDLL) -> None:
     L.psq_o_gpu_model_nndescent.argtypes = [_P, _U64, _U64, _U64, _U64, _U64, ctypes.c_double,
                                             _U64, _U64, _U64, _U64, _U64, _U64, _P, ctypes.c_int, _P,
                                             _P, _P, ctypes.POINTER(ModelStats)]
+    L.psq_o_fuzzy_strengths.restype = None
+    L.psq_o_fuzzy_strengths.argtypes = [_P, _P, _U64, _P]
+    L.psq_o_fuzzy_connectivities.restype = ctypes.c_int64
+    L.psq_o_fuzzy_connectivities.argtypes = [_P, _P, _P, _U64, _P, _P, _P, _U64,
+                                             ctypes.POINTER(_U64)]
 
 
 def _f32(x: np.ndarray) -> np.ndarray:
@@ -258,3 +269,41 @@ def gpu_model_nndescent(data: np.ndarray, k: int, max_iterations: int = 0, seed:
     if nnz < 0:
         raise ValueError("invalid input")
     return {"indptr": ip, "indices": ix[:nnz], "data": dx[:nnz], "n_obs": n, "nnz": nnz}, st
+
+
+def _u32(x: np.ndarray) -> np.ndarray:
+    return np.ascontiguousarray(x, dtype=np.uint32)
+
+
+def fuzzy_strengths(indptr: np.ndarray, dists: np.ndarray) -> np.ndarray:
+    """Membership strengths of every entry of a kNN distance graph
+    (``core/umap.rs:551-608`` applied row by row, ``core/umap.rs:343-354``)."""
+    ip = _u32(indptr)
+    dx = _f32(dists)
+    n = len(ip) - 1
+    out = np.zeros(max(int(ip[-1]), 1), np.float32)
+    lib().psq_o_fuzzy_strengths(_ptr(ip), _ptr(dx), n, _ptr(out))
+    return out[: int(ip[-1])]
+
+
+def fuzzy_connectivities(indptr: np.ndarray, indices: np.ndarray, dists: np.ndarray,
+                         n_obs: int) -> dict:
+    """The symmetrised fuzzy connectivities (``core/umap.rs:130-139``): a dict
+    with ``indptr``, ``indices`` and ``data`` (the weights)."""
+    ip = _u32(indptr)
+    ix = _u32(indices)
+    dx = _f32(dists)
+    if len(ip) != n_obs + 1:
+        raise ValueError("indptr must have n_obs + 1 entries")
+    if len(ix) and int(ix.max()) >= n_obs:
+        raise ValueError("index out of range")
+    cap = 2 * int(ip[-1])
+    out_ip = np.zeros(n_obs + 1, np.uint32)
+    out_ix = np.zeros(max(cap, 1), np.uint32)
+    out_w = np.zeros(max(cap, 1), np.float32)
+    need = _U64(0)
+    nnz = lib().psq_o_fuzzy_connectivities(_ptr(ip), _ptr(ix), _ptr(dx), n_obs, _ptr(out_ip),
+                                           _ptr(out_ix), _ptr(out_w), cap, ctypes.byref(need))
+    if nnz < 0:
+        raise ValueError("output capacity too small")
+    return {"indptr": out_ip, "indices": out_ix[:nnz], "data": out_w[:nnz]}

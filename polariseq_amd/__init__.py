@@ -3,7 +3,9 @@
 NN-Descent kNN on a PCA embedding, and an exact tiled kNN, as hand-written HIP
 kernels for gfx950 behind a C-ABI (include/psq_knn.h, lib/libpsq_knn.so).
 The Python functions mirror the reference binding's ``nndescent_dense`` and
-``neighbors_dense`` (crates/polariseq-python/src/lib.rs:471-566).
+``neighbors_dense`` (crates/polariseq-python/src/lib.rs:471-566), and its
+``fuzzy_connectivities`` / ``fuzzy_connectivities_into`` (lib.rs:568-642), the
+graph step's consumer inside ``pp.neighbors``.
 """
 
 from .neighbors import (  # noqa: F401
@@ -20,6 +22,11 @@ from .neighbors import (  # noqa: F401
     release_workspace,
 )
 
+from .fuzzy import (  # noqa: F401,E402
+    fuzzy_connectivities,
+    fuzzy_connectivities_device,
+    fuzzy_connectivities_into,
+)
 from .encoder import Encoder  # noqa: F401,E402
 
 __version__ = "0.1.0"

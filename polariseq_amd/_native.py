@@ -45,6 +45,10 @@ EXPORTS = (
     "psq_knn_exact_batched",
     "psq_knn_exact_batched_device",
     "psq_batched_last_report",
+    "psq_fuzzy_connectivities_device",
+    "psq_fuzzy_plan_new",
+    "psq_fuzzy_plan_fill",
+    "psq_fuzzy_plan_free",
 )
 
 
@@ -204,6 +208,14 @@ def _declare(L: ctypes.CDLL) -> None:
     L.psq_knn_exact_batched_device.argtypes = [P, U64, U64, P, U64, U64, P, P, P]
     L.psq_batched_last_report.restype = ctypes.c_int
     L.psq_batched_last_report.argtypes = [ctypes.POINTER(BatchedReport)]
+    L.psq_fuzzy_connectivities_device.restype = ctypes.c_int
+    L.psq_fuzzy_connectivities_device.argtypes = [P, P, P, U64, P, P, P, U64, ctypes.POINTER(U64), P]
+    L.psq_fuzzy_plan_new.restype = ctypes.c_int
+    L.psq_fuzzy_plan_new.argtypes = [P, U64, P, P, U64, U64, I32, PP, ctypes.POINTER(U64)]
+    L.psq_fuzzy_plan_fill.restype = ctypes.c_int
+    L.psq_fuzzy_plan_fill.argtypes = [P, P, U64, P, P, U64]
+    L.psq_fuzzy_plan_free.restype = None
+    L.psq_fuzzy_plan_free.argtypes = [P]
     L.psq_encoder_info.restype = ctypes.c_int
     L.psq_encoder_info.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                    ctypes.POINTER(U64)]

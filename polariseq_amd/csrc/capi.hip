@@ -718,6 +718,151 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   return PSQ_OK;
 }
 
+// ── fuzzy connectivities (crates/polariseq-core/src/umap.rs:130-139, 327-460) ──
+// fuzzy_core leaves each row's kept entries at the start of its combined
+// segment in "fz_comb" and the row pointers of the packed result in "fz_optr";
+// fuzzy_emit packs them. Both run in one API call, so nothing else reuses the
+// buffers between them.
+struct FuzzyRun {
+  const uint32_t* ip = nullptr;
+  uint32_t n = 0;
+  uint32_t* t_ptr = nullptr;
+  uint64_t* comb = nullptr;
+  uint32_t* optr = nullptr;
+  uint64_t total = 0;
+};
+
+int fuzzy_giant_rows(Workspace* ws, FuzzyRun& R, uint32_t* olen, const uint32_t* giant, uint32_t ng,
+                     hipStream_t s) {
+  uint32_t* lens;
+  TRY(wbuf(ws, "fz_glen", ng, &lens));
+  hipLaunchKernelGGL(k_fuzzy_row_len, dim3(grid_for(ng, 256)), dim3(256), 0, s, R.ip, R.t_ptr, giant,
+                     ng, lens);
+  HIPC(hipGetLastError());
+  std::vector<uint32_t> rows(ng), hl(ng);
+  HIPC(hipMemcpyAsync(rows.data(), giant, ng * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIPC(hipMemcpyAsync(hl.data(), lens, ng * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIPC(hipStreamSynchronize(s));
+  std::map<uint32_t, std::vector<uint32_t>> by_p;  // padded length -> rows
+  for (uint32_t r = 0; r < ng; ++r) {
+    uint32_t P = 1;
+    while (P < hl[r]) P <<= 1;
+    by_p[P].push_back(rows[r]);
+  }
+  for (auto& kv : by_p) {
+    const uint32_t P = kv.first, nr = (uint32_t)kv.second.size();
+    uint32_t* cls;
+    uint64_t* tmp;
+    TRY(wbuf(ws, "fz_gcls", nr, &cls));
+    TRY(wbuf(ws, "fz_gtmp", (size_t)nr * P, &tmp));
+    HIPC(hipMemcpyAsync(cls, kv.second.data(), nr * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    const size_t tot = (size_t)nr * P;
+    hipLaunchKernelGGL(k_fuzzy_gather_pad, dim3(std::min<unsigned>(grid_for(tot, 256), 65536)),
+                       dim3(256), 0, s, R.ip, R.t_ptr, cls, nr, P, R.comb, tmp);
+    HIPC(hipGetLastError());
+    const unsigned gb = std::min<unsigned>(grid_for(tot / 2, 256), 65536);
+    for (uint32_t size = 2; size <= P; size <<= 1)
+      for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+        hipLaunchKernelGGL(k_fuzzy_bitonic_step, dim3(gb), dim3(256), 0, s, tmp, nr, P, size, stride);
+        HIPC(hipGetLastError());
+      }
+    hipLaunchKernelGGL(k_fuzzy_fold_rows, dim3(std::min<uint32_t>(nr, 1024)), dim3(FZ_BLOCK), 0, s,
+                       R.ip, R.t_ptr, cls, nr, P, tmp, R.comb, olen);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(s));  // cls and tmp are reused by the next class
+  }
+  return PSQ_OK;
+}
+
+int fuzzy_core(Workspace* ws, const uint32_t* d_ip, const uint32_t* d_ix, const float* d_dx,
+               uint32_t n, uint64_t m, hipStream_t s, FuzzyRun* out) {
+  FuzzyRun R;
+  R.ip = d_ip;
+  R.n = n;
+  uint32_t *h;
+  TRY(ws->host_counters((void**)&h));
+  uint32_t *flags, *cnt, *cur, *olen, *big, *giant;
+  unsigned long long* tot;
+  float* w;
+  TRY(wbuf(ws, "fz_flags", 4, &flags));  // bad, nbig, ngiant, pad
+  TRY(wbuf(ws, "fz_tot", 1, &tot));
+  TRY(wbuf(ws, "fz_w", std::max<uint64_t>(m, 1), &w));
+  TRY(wbuf(ws, "fz_cnt", n, &cnt));
+  TRY(wbuf(ws, "fz_tptr", (size_t)n + 1, &R.t_ptr));
+  TRY(wbuf(ws, "fz_cur", n, &cur));
+  TRY(wbuf(ws, "fz_comb", std::max<uint64_t>(2 * m, 1), &R.comb));
+  TRY(wbuf(ws, "fz_olen", n, &olen));
+  TRY(wbuf(ws, "fz_big", n, &big));
+  TRY(wbuf(ws, "fz_giant", n, &giant));
+  TRY(wbuf(ws, "fz_optr", (size_t)n + 1, &R.optr));
+  TRY(fill_u32(flags, 4, 0, s));
+  TRY(fill_u64((uint64_t*)tot, 1, 0, s));
+  // Validate before anything indexes by the input: row pointers and indices.
+  hipLaunchKernelGGL(k_fuzzy_check, dim3(std::min<unsigned>(grid_for(std::max<uint64_t>(m, n), 256), 65536)),
+                     dim3(256), 0, s, d_ip, n, d_ix, m, flags);
+  HIPC(hipGetLastError());
+  HIPC(hipMemcpyAsync(h, flags, 4, hipMemcpyDeviceToHost, s));
+  HIPC(hipStreamSynchronize(s));
+  if (h[0] & 1u) return fail(PSQ_E_INVALID, "indptr must be non-decreasing");
+  if (h[0] & 2u) return fail(PSQ_E_INVALID, "an index of the graph is >= n_obs (%u)", n);
+  // Phase 1: membership strengths (umap.rs:343-354).
+  hipLaunchKernelGGL(k_fuzzy_strengths, dim3(grid_for(n, 256)), dim3(256), 0, s, d_ip, d_dx, n, w);
+  HIPC(hipGetLastError());
+  // Phase 2: transpose by counting sort (umap.rs:357-381), into combined segments.
+  TRY(fill_u32(cnt, n, 0, s));
+  hipLaunchKernelGGL(k_fuzzy_in_degree, dim3(std::min<unsigned>(grid_for(m, 256), 65536)), dim3(256),
+                     0, s, d_ix, m, cnt);
+  HIPC(hipGetLastError());
+  TRY(scan_u32(ws, cnt, R.t_ptr, n, s));
+  TRY(fill_u32(cur, n, 0, s));
+  hipLaunchKernelGGL(k_fuzzy_scatter, dim3(grid_for(n, 256)), dim3(256), 0, s, d_ip, d_ix, w, R.t_ptr,
+                     n, cur, R.comb);
+  HIPC(hipGetLastError());
+  // Phase 3: per-row sort, fuzzy union and compaction (umap.rs:488-537), by row size.
+  hipLaunchKernelGGL(k_fuzzy_merge_small, dim3(std::min<uint32_t>(n, 1u << 22)), dim3(WAVE), 0, s,
+                     d_ip, R.t_ptr, n, R.comb, olen, big, flags + 1);
+  HIPC(hipGetLastError());
+  const size_t mid_smem = (size_t)FZ_MID_CAP * sizeof(uint64_t);
+  HIPC(hipFuncSetAttribute((const void*)k_fuzzy_merge_mid, hipFuncAttributeMaxDynamicSharedMemorySize,
+                           (int)mid_smem));
+  hipLaunchKernelGGL(k_fuzzy_merge_mid, dim3(1024), dim3(FZ_BLOCK), mid_smem, s, d_ip, R.t_ptr, R.comb,
+                     olen, big, flags + 1, giant, flags + 2);
+  HIPC(hipGetLastError());
+  HIPC(hipMemcpyAsync(h, flags, 12, hipMemcpyDeviceToHost, s));
+  HIPC(hipStreamSynchronize(s));
+  if (h[2] > 0) TRY(fuzzy_giant_rows(ws, R, olen, giant, h[2], s));
+  hipLaunchKernelGGL(k_fuzzy_total, dim3(std::min<unsigned>(grid_for(n, 256), 2048)), dim3(256), 0, s,
+                     olen, n, tot);
+  HIPC(hipGetLastError());
+  TRY(scan_u32(ws, olen, R.optr, n, s));
+  HIPC(hipMemcpyAsync(h, tot, 8, hipMemcpyDeviceToHost, s));
+  HIPC(hipStreamSynchronize(s));
+  std::memcpy(&R.total, h, 8);
+  if (R.total >= (1ull << 32))
+    return fail(PSQ_E_UNSUPPORTED, "the connectivity graph exceeds 2^32 entries");
+  *out = R;
+  return PSQ_OK;
+}
+
+int fuzzy_emit(const FuzzyRun& R, uint32_t* out_ix, float* out_w, hipStream_t s) {
+  if (R.total == 0) return PSQ_OK;
+  hipLaunchKernelGGL(k_fuzzy_emit, dim3(grid_for((size_t)R.n * FZ_EMIT_LANES, 256)), dim3(256), 0, s,
+                     R.ip, R.t_ptr, R.n, R.comb, R.optr, out_ix, out_w);
+  HIPC(hipGetLastError());
+  return PSQ_OK;
+}
+
+// Reads indptr[0] and indptr[n] of a device CSR; the graph is indptr[0] == 0 based.
+int fuzzy_input_nnz(const uint32_t* d_ip, uint64_t n, uint64_t* m, hipStream_t s) {
+  uint32_t ends[2] = {0, 0};
+  HIPC(hipMemcpyAsync(&ends[0], d_ip, 4, hipMemcpyDeviceToHost, s));
+  HIPC(hipMemcpyAsync(&ends[1], d_ip + n, 4, hipMemcpyDeviceToHost, s));
+  HIPC(hipStreamSynchronize(s));
+  if (ends[0] != 0) return fail(PSQ_E_INVALID, "indptr[0] must be 0 (got %u)", ends[0]);
+  *m = ends[1];
+  return PSQ_OK;
+}
+
 int set_device(int32_t device) {
   int count = 0;
   HIPC(hipGetDeviceCount(&count));
@@ -969,6 +1114,137 @@ int psq_neighbors_exact(const float* embedding, uint64_t n_obs, uint64_t n_dim,
   }
   *nnz_out = n_obs * kk;
   return PSQ_OK;
+}
+
+// ── fuzzy connectivities ──
+struct psq_fuzzy_plan {
+  int device = 0;
+  uint64_t n = 0, nnz = 0;
+  uint32_t* ip = nullptr;
+  uint32_t* ix = nullptr;
+  float* w = nullptr;
+};
+
+int psq_fuzzy_connectivities_device(const uint32_t* d_indptr, const uint32_t* d_indices,
+                                    const float* d_dists, uint64_t n_obs, uint32_t* d_out_indptr,
+                                    uint32_t* d_out_indices, float* d_out_weights, uint64_t cap,
+                                    uint64_t* nnz_out, void* stream) {
+  g_err.clear();
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!d_indptr || !nnz_out) return fail(PSQ_E_INVALID, "NULL pointer argument");
+  if (n_obs >= (1ull << 32)) return fail(PSQ_E_UNSUPPORTED, "n_obs must be < 2^32");
+  uint64_t m = 0;
+  TRY(fuzzy_input_nnz(d_indptr, n_obs, &m, s));
+  if (m > 0 && (!d_indices || !d_dists)) return fail(PSQ_E_INVALID, "NULL pointer argument");
+  Workspace* ws;
+  TRY(workspace(&ws));
+  FuzzyRun R;
+  if (n_obs > 0) TRY(fuzzy_core(ws, d_indptr, d_indices, d_dists, (uint32_t)n_obs, m, s, &R));
+  *nnz_out = R.total;
+  if (!d_out_indptr || (R.total > 0 && (!d_out_indices || !d_out_weights)))
+    return fail(PSQ_E_INVALID, "NULL output pointer");
+  if (cap < R.total)
+    return fail(PSQ_E_INVALID, "outputs must hold %llu entries (cap %llu)",
+                (unsigned long long)R.total, (unsigned long long)cap);
+  if (n_obs == 0) {
+    TRY(fill_u32(d_out_indptr, 1, 0, s));
+  } else {
+    HIPC(hipMemcpyAsync(d_out_indptr, R.optr, (n_obs + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    TRY(fuzzy_emit(R, d_out_indices, d_out_weights, s));
+  }
+  HIPC(hipStreamSynchronize(s));
+  return PSQ_OK;
+}
+
+int psq_fuzzy_plan_new(const uint32_t* indptr, uint64_t indptr_len, const uint32_t* indices,
+                       const float* dists, uint64_t nnz_in, uint64_t n_obs, int32_t device,
+                       psq_fuzzy_plan** plan, uint64_t* nnz_out) {
+  g_err.clear();
+  if (!plan || !nnz_out || !indptr) return fail(PSQ_E_INVALID, "NULL pointer argument");
+  *plan = nullptr;
+  if (indptr_len != n_obs + 1) return fail(PSQ_E_INVALID, "indptr must have n_obs + 1 entries");
+  if (n_obs >= (1ull << 32)) return fail(PSQ_E_UNSUPPORTED, "n_obs must be < 2^32");
+  if (indptr[0] != 0) return fail(PSQ_E_INVALID, "indptr[0] must be 0 (got %u)", indptr[0]);
+  for (uint64_t i = 0; i < n_obs; ++i)
+    if (indptr[i + 1] < indptr[i]) return fail(PSQ_E_INVALID, "indptr must be non-decreasing");
+  const uint64_t m = indptr[n_obs];
+  if (m > nnz_in)
+    return fail(PSQ_E_INVALID, "indptr[n_obs] = %llu exceeds the %llu entries given",
+                (unsigned long long)m, (unsigned long long)nnz_in);
+  if (m > 0 && (!indices || !dists)) return fail(PSQ_E_INVALID, "NULL pointer argument");
+  TRY(set_device(device));
+  auto* P = new psq_fuzzy_plan();
+  P->device = device;
+  P->n = n_obs;
+  auto bail = [&](int rc) {
+    psq_fuzzy_plan_free(P);
+    return rc;
+  };
+  hipError_t e = hipMalloc(&P->ip, (n_obs + 1) * sizeof(uint32_t));
+  if (e != hipSuccess) return bail(fail(PSQ_E_NOMEM, "hipMalloc failed: %s", hipGetErrorString(e)));
+  if (n_obs > 0) {
+    Workspace* ws;
+    int rc = workspace(&ws);
+    if (rc != PSQ_OK) return bail(rc);
+    uint32_t *d_ip, *d_ix;
+    float* d_dx;
+    if ((rc = wbuf(ws, "fz_in_ip", (size_t)n_obs + 1, &d_ip)) != PSQ_OK ||
+        (rc = wbuf(ws, "fz_in_ix", std::max<uint64_t>(m, 1), &d_ix)) != PSQ_OK ||
+        (rc = wbuf(ws, "fz_in_dx", std::max<uint64_t>(m, 1), &d_dx)) != PSQ_OK)
+      return bail(rc);
+    if ((e = hipMemcpy(d_ip, indptr, (n_obs + 1) * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+        (m && (e = hipMemcpy(d_ix, indices, m * 4, hipMemcpyHostToDevice)) != hipSuccess) ||
+        (m && (e = hipMemcpy(d_dx, dists, m * 4, hipMemcpyHostToDevice)) != hipSuccess))
+      return bail(fail(PSQ_E_DEVICE, "hipMemcpy failed: %s", hipGetErrorString(e)));
+    FuzzyRun R;
+    if ((rc = fuzzy_core(ws, d_ip, d_ix, d_dx, (uint32_t)n_obs, m, nullptr, &R)) != PSQ_OK)
+      return bail(rc);
+    P->nnz = R.total;
+    if (R.total > 0) {
+      if ((e = hipMalloc(&P->ix, R.total * sizeof(uint32_t))) != hipSuccess ||
+          (e = hipMalloc(&P->w, R.total * sizeof(float))) != hipSuccess)
+        return bail(fail(PSQ_E_NOMEM, "hipMalloc failed: %s", hipGetErrorString(e)));
+      if ((rc = fuzzy_emit(R, P->ix, P->w, nullptr)) != PSQ_OK) return bail(rc);
+    }
+    if ((e = hipMemcpy(P->ip, R.optr, (n_obs + 1) * 4, hipMemcpyDeviceToDevice)) != hipSuccess)
+      return bail(fail(PSQ_E_DEVICE, "hipMemcpy failed: %s", hipGetErrorString(e)));
+  } else if ((e = hipMemset(P->ip, 0, 4)) != hipSuccess) {
+    return bail(fail(PSQ_E_DEVICE, "hipMemset failed: %s", hipGetErrorString(e)));
+  }
+  if ((e = hipDeviceSynchronize()) != hipSuccess)
+    return bail(fail(PSQ_E_DEVICE, "hipDeviceSynchronize failed: %s", hipGetErrorString(e)));
+  *plan = P;
+  *nnz_out = P->nnz;
+  return PSQ_OK;
+}
+
+int psq_fuzzy_plan_fill(const psq_fuzzy_plan* plan, uint32_t* indptr, uint64_t indptr_len,
+                        uint32_t* indices, float* weights, uint64_t cap) {
+  g_err.clear();
+  if (!plan || !indptr) return fail(PSQ_E_INVALID, "NULL pointer argument");
+  if (indptr_len != plan->n + 1 || cap < plan->nnz)
+    return fail(PSQ_E_INVALID, "alloc returned arrays of the wrong size");
+  if (plan->nnz > 0 && (!indices || !weights)) return fail(PSQ_E_INVALID, "NULL pointer argument");
+  TRY(set_device(plan->device));
+  HIPC(hipMemcpy(indptr, plan->ip, (plan->n + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (plan->nnz > 0) {
+    HIPC(hipMemcpy(indices, plan->ix, plan->nnz * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(weights, plan->w, plan->nnz * sizeof(float), hipMemcpyDeviceToHost));
+  }
+  return PSQ_OK;
+}
+
+void psq_fuzzy_plan_free(psq_fuzzy_plan* plan) {
+  if (!plan) return;
+  int cur = 0;
+  const bool have = hipGetDevice(&cur) == hipSuccess;
+  if (hipSetDevice(plan->device) == hipSuccess) {
+    if (plan->ip) (void)hipFree(plan->ip);
+    if (plan->ix) (void)hipFree(plan->ix);
+    if (plan->w) (void)hipFree(plan->w);
+  }
+  if (have) (void)hipSetDevice(cur);
+  delete plan;
 }
 
 }  // extern "C"

@@ -148,4 +148,34 @@ __global__ void k_bexact_pass2(const float* Xp, uint32_t d, uint32_t dp, const u
                                const float* T, uint32_t* counter, uint32_t* cnt, uint32_t* cand,
                                uint32_t cap);
 
+// Fuzzy connectivities (fuzzy_kernels.hip)
+constexpr int FZ_KR = 16;          // row length held in registers by k_fuzzy_strengths
+constexpr int FZ_BLOCK = 256;      // threads of the block-per-row kernels
+constexpr int FZ_MID_CAP = 8192;   // longest combined row sorted in LDS
+constexpr int FZ_EMIT_LANES = 16;  // lanes per row in k_fuzzy_emit
+__global__ void k_fuzzy_check(const uint32_t* indptr, uint32_t n, const uint32_t* indices, uint64_t m,
+                              uint32_t* bad);
+__global__ void k_fuzzy_strengths(const uint32_t* indptr, const float* dists, uint32_t n, float* w);
+__global__ void k_fuzzy_in_degree(const uint32_t* indices, uint64_t m, uint32_t* cnt);
+__global__ void k_fuzzy_scatter(const uint32_t* indptr, const uint32_t* indices, const float* w,
+                                const uint32_t* t_ptr, uint32_t n, uint32_t* cur, uint64_t* comb);
+__global__ void k_fuzzy_merge_small(const uint32_t* indptr, const uint32_t* t_ptr, uint32_t n,
+                                    uint64_t* comb, uint32_t* olen, uint32_t* big, uint32_t* nbig);
+__global__ void k_fuzzy_merge_mid(const uint32_t* indptr, const uint32_t* t_ptr, uint64_t* comb,
+                                  uint32_t* olen, const uint32_t* big, const uint32_t* nbig,
+                                  uint32_t* giant, uint32_t* ngiant);
+__global__ void k_fuzzy_row_len(const uint32_t* indptr, const uint32_t* t_ptr, const uint32_t* rows,
+                                uint32_t nrows, uint32_t* len);
+__global__ void k_fuzzy_gather_pad(const uint32_t* indptr, const uint32_t* t_ptr, const uint32_t* rows,
+                                   uint32_t nrows, uint32_t P, const uint64_t* comb, uint64_t* tmp);
+__global__ void k_fuzzy_bitonic_step(uint64_t* tmp, uint32_t nrows, uint32_t P, uint32_t size,
+                                     uint32_t stride);
+__global__ void k_fuzzy_fold_rows(const uint32_t* indptr, const uint32_t* t_ptr, const uint32_t* rows,
+                                  uint32_t nrows, uint32_t P, const uint64_t* tmp, uint64_t* comb,
+                                  uint32_t* olen);
+__global__ void k_fuzzy_total(const uint32_t* olen, uint32_t n, unsigned long long* total);
+__global__ void k_fuzzy_emit(const uint32_t* indptr, const uint32_t* t_ptr, uint32_t n,
+                             const uint64_t* comb, const uint32_t* optr, uint32_t* out_ix,
+                             float* out_w);
+
 }  // namespace psq
