@@ -79,10 +79,12 @@ def main() -> int:
     ap.add_argument("--recall-queries", type=int, default=100_000)
     ap.add_argument("--cpu-cells", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="headline", choices=["headline", "precision", "batched"],
+    ap.add_argument("--workload", default="headline",
+                    choices=["headline", "precision", "batched", "fuzzy"],
                     help="headline = the metric above; precision = BF16 form on 4.06M x 50 "
                          "with 5%% outliers, k=30, distance error on a 50k subset; batched = "
-                         "per-sample exact kNN of 64 samples with doublets in one launch")
+                         "per-sample exact kNN of 64 samples with doublets in one launch; "
+                         "fuzzy = fuzzy connectivities of the headline kNN graph")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 elementwise copies of 1 GiB (y = x * 1, a kernel of known "
                          "traffic: 1 GiB read + 1 GiB written) to scale the HBM counters by")
@@ -113,6 +115,8 @@ def main() -> int:
         return precision_workload(args, torch, P, ds)
     if args.workload == "batched":
         return batched_workload(args, torch, P, ds)
+    if args.workload == "fuzzy":
+        return fuzzy_workload(args, torch, P, ds)
     n, k = args.n, args.k
     log(f"generating {n} x 50 mixture ({args.components} components, seed {args.seed})")
     x = ds.mixture(n, 50, args.components, seed=args.seed)
@@ -372,6 +376,78 @@ def batched_workload(args, torch, P, ds) -> int:
                    "autotuner": rep},
         "distance_tflops": round(3 * 50 * pairs / el / 1e12, 2),
     }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def fuzzy_workload(args, torch, P, ds) -> int:
+    """Fuzzy connectivities (the kNN graph's consumer in pp.neighbors) of the
+    headline graph: 4.06M x 50 mixture, plain NN-Descent k = 15, built untimed and
+    left in HBM. A step is one call: strengths, transpose, fuzzy union, the
+    symmetric CSR left in HBM. The roofline is the whole call's: the input CSR
+    read once and the output CSR written once, over the call's time from HIP
+    events on the stream the library launches on. CPU baseline: the oracle's
+    restatement of umap.rs on the full graph, one thread; its output is also the
+    parity check of the GPU result."""
+    n, k = args.n, args.k
+    x = ds.mixture(n, 50, args.components, seed=args.seed)
+    xt = torch.from_numpy(x).cuda()
+    ip, ix, dx, nnz = P.nndescent_device(xt, k, seed=args.seed)
+    del xt
+    torch.cuda.synchronize()
+    cap = 2 * nnz
+    out = (torch.empty(n + 1, dtype=torch.int32, device="cuda"),
+           torch.empty(cap, dtype=torch.int32, device="cuda"),
+           torch.empty(cap, dtype=torch.float32, device="cuda"))
+    for _ in range(args.warmup):
+        P.fuzzy_connectivities_device(ip, ix, dx, n, out=out)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        _, _, _, cnnz = P.fuzzy_connectivities_device(ip, ix, dx, n, out=out)
+    e1.record()
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / args.steps
+    gpu_s = e0.elapsed_time(e1) / 1000.0 / args.steps
+    alg = 2 * (n + 1) * 4 + nnz * 8 + cnnz * 8
+    line = {
+        "metric": "cells/s to fuzzy connectivities of a 4.06M x 15-NN graph",
+        "value": round(n / el, 1), "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1000 * el, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic Gaussian mixture (numpy, seeded); its NN-Descent graph",
+        "config": {"workload": f"fuzzy connectivities of the plain NN-Descent graph of {n} x 50 "
+                               f"FP32 cells, {args.components}-component mixture, k={k}, "
+                               f"seed {args.seed}", "edges_in": nnz, "edges_out": cnnz},
+        "roofline": {"bound": "hbm", "kernel": "psq_fuzzy_connectivities_device (all kernels)",
+                     "achieved": round(alg / gpu_s / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(alg / gpu_s / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None},
+    }
+    if not args.no_cpu:
+        import oracle
+        h_ip = ip.cpu().numpy().view(np.uint32)
+        h_ix = ix.cpu().numpy().view(np.uint32)[:nnz]
+        h_dx = dx.cpu().numpy()[:nnz]
+        log("oracle fuzzy connectivities (one thread) ...")
+        c0 = time.perf_counter()
+        want = oracle.fuzzy_connectivities(h_ip, h_ix, h_dx, n)
+        cs = time.perf_counter() - c0
+        line["cpu_baseline"] = {"value": round(n / cs, 1), "unit": "cells/s", "cores": 1,
+                                "kind": "port",
+                                "sample": f"the full graph ({n} rows x {k}), one thread"}
+        g_ip = out[0].cpu().numpy().view(np.uint32)
+        g_ix = out[1][:cnnz].cpu().numpy().view(np.uint32)
+        g_w = out[2][:cnnz].cpu().numpy()
+        same = np.array_equal(g_ip, want["indptr"]) and np.array_equal(g_ix, want["indices"])
+        ulps = (np.abs(g_w.view(np.int32).astype(np.int64) - want["data"].view(np.int32))
+                if same else None)
+        line["parity_vs_oracle"] = {
+            "indptr_indices_equal": bool(same),
+            "weights_max_ulp": None if ulps is None else int(ulps.max(initial=0)),
+            "weights_bit_exact_frac": None if ulps is None else float((ulps == 0).mean())}
     print(json.dumps(line), flush=True)
     return 0
 

@@ -781,15 +781,15 @@ int fuzzy_core(Workspace* ws, const uint32_t* d_ip, const uint32_t* d_ix, const 
   R.n = n;
   uint32_t *h;
   TRY(ws->host_counters((void**)&h));
-  uint32_t *flags, *cnt, *cur, *olen, *big, *giant;
-  unsigned long long* tot;
+  uint32_t *flags, *cnt, *olen, *big, *giant;
+  unsigned long long *tot, *tpos;
   float* w;
   TRY(wbuf(ws, "fz_flags", 4, &flags));  // bad, nbig, ngiant, pad
   TRY(wbuf(ws, "fz_tot", 1, &tot));
   TRY(wbuf(ws, "fz_w", std::max<uint64_t>(m, 1), &w));
   TRY(wbuf(ws, "fz_cnt", n, &cnt));
   TRY(wbuf(ws, "fz_tptr", (size_t)n + 1, &R.t_ptr));
-  TRY(wbuf(ws, "fz_cur", n, &cur));
+  TRY(wbuf(ws, "fz_tpos", n, &tpos));
   TRY(wbuf(ws, "fz_comb", std::max<uint64_t>(2 * m, 1), &R.comb));
   TRY(wbuf(ws, "fz_olen", n, &olen));
   TRY(wbuf(ws, "fz_big", n, &big));
@@ -814,9 +814,10 @@ int fuzzy_core(Workspace* ws, const uint32_t* d_ip, const uint32_t* d_ix, const 
                      0, s, d_ix, m, cnt);
   HIPC(hipGetLastError());
   TRY(scan_u32(ws, cnt, R.t_ptr, n, s));
-  TRY(fill_u32(cur, n, 0, s));
-  hipLaunchKernelGGL(k_fuzzy_scatter, dim3(grid_for(n, 256)), dim3(256), 0, s, d_ip, d_ix, w, R.t_ptr,
-                     n, cur, R.comb);
+  hipLaunchKernelGGL(k_fuzzy_tpos, dim3(grid_for(n, 256)), dim3(256), 0, s, d_ip, R.t_ptr, n, tpos);
+  HIPC(hipGetLastError());
+  hipLaunchKernelGGL(k_fuzzy_scatter, dim3(grid_for((size_t)n * FZ_SCATTER_LANES, 256)), dim3(256), 0,
+                     s, d_ip, d_ix, w, R.t_ptr, n, tpos, R.comb);
   HIPC(hipGetLastError());
   // Phase 3: per-row sort, fuzzy union and compaction (umap.rs:488-537), by row size.
   hipLaunchKernelGGL(k_fuzzy_merge_small, dim3(std::min<uint32_t>(n, 1u << 22)), dim3(WAVE), 0, s,
@@ -831,13 +832,20 @@ int fuzzy_core(Workspace* ws, const uint32_t* d_ip, const uint32_t* d_ix, const 
   HIPC(hipMemcpyAsync(h, flags, 12, hipMemcpyDeviceToHost, s));
   HIPC(hipStreamSynchronize(s));
   if (h[2] > 0) TRY(fuzzy_giant_rows(ws, R, olen, giant, h[2], s));
-  hipLaunchKernelGGL(k_fuzzy_total, dim3(std::min<unsigned>(grid_for(n, 256), 2048)), dim3(256), 0, s,
-                     olen, n, tot);
-  HIPC(hipGetLastError());
   TRY(scan_u32(ws, olen, R.optr, n, s));
-  HIPC(hipMemcpyAsync(h, tot, 8, hipMemcpyDeviceToHost, s));
-  HIPC(hipStreamSynchronize(s));
-  std::memcpy(&R.total, h, 8);
+  // At most 2m entries are kept: below 2^32 the u32 running sum is exact.
+  if (2 * m < (1ull << 32)) {
+    HIPC(hipMemcpyAsync(h, R.optr + n, 4, hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    R.total = h[0];
+  } else {
+    hipLaunchKernelGGL(k_fuzzy_total, dim3(std::min<unsigned>(grid_for(n, 256), 2048)), dim3(256), 0,
+                       s, olen, n, tot);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(h, tot, 8, hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    std::memcpy(&R.total, h, 8);
+  }
   if (R.total >= (1ull << 32))
     return fail(PSQ_E_UNSUPPORTED, "the connectivity graph exceeds 2^32 entries");
   *out = R;

@@ -199,25 +199,33 @@ __global__ void k_fuzzy_in_degree(const uint32_t* __restrict__ indices, uint64_t
     atomicAdd(&cnt[indices[t]], 1u);
 }
 
-// One thread per row i: its own entries in place, and (i, w) appended to the
-// transposed part of every row it names (umap.rs:368-381; the order within a
-// transposed part is set by the sort that follows, not by this scatter).
+// Start of the transposed part of each row's combined segment.
+__global__ void k_fuzzy_tpos(const uint32_t* __restrict__ indptr, const uint32_t* __restrict__ t_ptr,
+                             uint32_t n, unsigned long long* __restrict__ tpos) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) tpos[j] = (unsigned long long)indptr[j + 1] + t_ptr[j];
+}
+
+// FZ_SCATTER_LANES lanes per row i, one edge (i -> j, w) each: the own entry in
+// place, and (i, w) appended to row j's transposed part (umap.rs:368-381; the
+// order within a transposed part is set by the sort that follows, not by this
+// scatter).
 __global__ __launch_bounds__(256) void k_fuzzy_scatter(const uint32_t* __restrict__ indptr,
                                                        const uint32_t* __restrict__ indices,
                                                        const float* __restrict__ w,
                                                        const uint32_t* __restrict__ t_ptr, uint32_t n,
-                                                       uint32_t* __restrict__ cur,
+                                                       unsigned long long* __restrict__ tpos,
                                                        uint64_t* __restrict__ comb) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = (uint32_t)(g / FZ_SCATTER_LANES), l = (uint32_t)(g % FZ_SCATTER_LANES);
   if (i >= n) return;
   const uint32_t s = indptr[i], e = indptr[i + 1];
   const uint64_t base = (uint64_t)s + t_ptr[i];
-  for (uint32_t q = s; q < e; ++q) {
+  for (uint32_t q = s + l; q < e; q += FZ_SCATTER_LANES) {
     const uint32_t j = indices[q];
     const float wq = w[q];
     comb[base + (q - s)] = fz_key(j, 0, wq);
-    const uint64_t tb = (uint64_t)indptr[j + 1] + t_ptr[j];
-    comb[tb + atomicAdd(&cur[j], 1u)] = fz_key(i, 1, wq);
+    comb[atomicAdd(&tpos[j], 1ull)] = fz_key(i, 1, wq);
   }
 }
 

@@ -153,12 +153,16 @@ constexpr int FZ_KR = 16;          // row length held in registers by k_fuzzy_st
 constexpr int FZ_BLOCK = 256;      // threads of the block-per-row kernels
 constexpr int FZ_MID_CAP = 8192;   // longest combined row sorted in LDS
 constexpr int FZ_EMIT_LANES = 16;  // lanes per row in k_fuzzy_emit
+constexpr int FZ_SCATTER_LANES = 16;  // lanes per row in k_fuzzy_scatter
 __global__ void k_fuzzy_check(const uint32_t* indptr, uint32_t n, const uint32_t* indices, uint64_t m,
                               uint32_t* bad);
 __global__ void k_fuzzy_strengths(const uint32_t* indptr, const float* dists, uint32_t n, float* w);
 __global__ void k_fuzzy_in_degree(const uint32_t* indices, uint64_t m, uint32_t* cnt);
+__global__ void k_fuzzy_tpos(const uint32_t* indptr, const uint32_t* t_ptr, uint32_t n,
+                             unsigned long long* tpos);
 __global__ void k_fuzzy_scatter(const uint32_t* indptr, const uint32_t* indices, const float* w,
-                                const uint32_t* t_ptr, uint32_t n, uint32_t* cur, uint64_t* comb);
+                                const uint32_t* t_ptr, uint32_t n, unsigned long long* tpos,
+                                uint64_t* comb);
 __global__ void k_fuzzy_merge_small(const uint32_t* indptr, const uint32_t* t_ptr, uint32_t n,
                                     uint64_t* comb, uint32_t* olen, uint32_t* big, uint32_t* nbig);
 __global__ void k_fuzzy_merge_mid(const uint32_t* indptr, const uint32_t* t_ptr, uint64_t* comb,
