@@ -781,15 +781,25 @@ int fuzzy_core(Workspace* ws, const uint32_t* d_ip, const uint32_t* d_ix, const 
   R.n = n;
   uint32_t *h;
   TRY(ws->host_counters((void**)&h));
-  uint32_t *flags, *cnt, *olen, *big, *giant;
-  unsigned long long *tot, *tpos;
+  uint32_t shift = FZ_BLOG_MIN;
+  while ((((uint64_t)n + (1ull << shift) - 1) >> shift) > (uint64_t)FZ_NB_MAX) ++shift;
+  const uint32_t nb = (uint32_t)(((uint64_t)n + (1ull << shift) - 1) >> shift);
+  const uint32_t rows_b = 1u << shift;
+  uint32_t *flags, *olen, *big, *giant, *btot, *bstart, *bfill, *pj, *cnt_g;
+  unsigned long long* tot;
+  uint64_t *pay, *srt_g;
   float* w;
   TRY(wbuf(ws, "fz_flags", 4, &flags));  // bad, nbig, ngiant, pad
   TRY(wbuf(ws, "fz_tot", 1, &tot));
   TRY(wbuf(ws, "fz_w", std::max<uint64_t>(m, 1), &w));
-  TRY(wbuf(ws, "fz_cnt", n, &cnt));
+  TRY(wbuf(ws, "fz_btot", (size_t)nb + 1, &btot));
+  TRY(wbuf(ws, "fz_bstart", (size_t)nb + 1, &bstart));
+  TRY(wbuf(ws, "fz_bfill", nb, &bfill));
+  TRY(wbuf(ws, "fz_pay", std::max<uint64_t>(m, 1), &pay));
+  TRY(wbuf(ws, "fz_pj", std::max<uint64_t>(m, 1), &pj));
+  TRY(wbuf(ws, "fz_srt", std::max<uint64_t>(m, 1), &srt_g));
+  TRY(wbuf(ws, "fz_cntg", rows_b > (uint32_t)FZ_BROWS_LDS ? (size_t)nb * 2 * (rows_b + 1) : 1, &cnt_g));
   TRY(wbuf(ws, "fz_tptr", (size_t)n + 1, &R.t_ptr));
-  TRY(wbuf(ws, "fz_tpos", n, &tpos));
   TRY(wbuf(ws, "fz_comb", std::max<uint64_t>(2 * m, 1), &R.comb));
   TRY(wbuf(ws, "fz_olen", n, &olen));
   TRY(wbuf(ws, "fz_big", n, &big));
@@ -808,20 +818,23 @@ int fuzzy_core(Workspace* ws, const uint32_t* d_ip, const uint32_t* d_ix, const 
   // Phase 1: membership strengths (umap.rs:343-354).
   hipLaunchKernelGGL(k_fuzzy_strengths, dim3(grid_for(n, 256)), dim3(256), 0, s, d_ip, d_dx, n, w);
   HIPC(hipGetLastError());
-  // Phase 2: transpose by counting sort (umap.rs:357-381), into combined segments.
-  TRY(fill_u32(cnt, n, 0, s));
-  hipLaunchKernelGGL(k_fuzzy_in_degree, dim3(std::min<unsigned>(grid_for(m, 256), 65536)), dim3(256),
-                     0, s, d_ix, m, cnt);
+  // Phase 2: transpose by bucket (umap.rs:357-381).
+  const unsigned tiles = (unsigned)((n + FZ_TILE_ROWS - 1) / FZ_TILE_ROWS);
+  TRY(fill_u32(btot, (size_t)nb + 1, 0, s));
+  TRY(fill_u32(bfill, nb, 0, s));
+  hipLaunchKernelGGL(k_fz_bucket_count, dim3(tiles), dim3(1024), 0, s, d_ip, d_ix, n, shift, nb, btot);
   HIPC(hipGetLastError());
-  TRY(scan_u32(ws, cnt, R.t_ptr, n, s));
-  hipLaunchKernelGGL(k_fuzzy_tpos, dim3(grid_for(n, 256)), dim3(256), 0, s, d_ip, R.t_ptr, n, tpos);
-  HIPC(hipGetLastError());
-  hipLaunchKernelGGL(k_fuzzy_scatter, dim3(grid_for((size_t)n * FZ_SCATTER_LANES, 256)), dim3(256), 0,
-                     s, d_ip, d_ix, w, R.t_ptr, n, tpos, R.comb);
+  TRY(scan_u32(ws, btot, bstart, nb, s));
+  hipLaunchKernelGGL(k_fz_bucket_scatter, dim3(tiles), dim3(1024), 0, s, d_ip, d_ix, w, n, shift, nb,
+                     bstart, bfill, pay, pj);
   HIPC(hipGetLastError());
   // Phase 3: per-row sort, fuzzy union and compaction (umap.rs:488-537), by row size.
-  hipLaunchKernelGGL(k_fuzzy_merge_small, dim3(std::min<uint32_t>(n, 1u << 22)), dim3(WAVE), 0, s,
-                     d_ip, R.t_ptr, n, R.comb, olen, big, flags + 1);
+  const size_t bm_smem = ((size_t)FZ_BCAP + (size_t)(FZ_MB / WAVE) * 4 * WAVE) * sizeof(uint64_t) +
+                         (2 * (size_t)FZ_BROWS_LDS + 1) * sizeof(uint32_t);
+  HIPC(hipFuncSetAttribute((const void*)k_fz_bucket_merge, hipFuncAttributeMaxDynamicSharedMemorySize,
+                           (int)bm_smem));
+  hipLaunchKernelGGL(k_fz_bucket_merge, dim3(nb), dim3(FZ_MB), bm_smem, s, d_ip, d_ix, w, n, shift, nb,
+                     bstart, pay, pj, srt_g, cnt_g, R.t_ptr, R.comb, olen, big, flags + 1);
   HIPC(hipGetLastError());
   const size_t mid_smem = (size_t)FZ_MID_CAP * sizeof(uint64_t);
   HIPC(hipFuncSetAttribute((const void*)k_fuzzy_merge_mid, hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -192,88 +192,199 @@ __global__ __launch_bounds__(256) void k_fuzzy_strengths(const uint32_t* __restr
   }
 }
 
-__global__ void k_fuzzy_in_degree(const uint32_t* __restrict__ indices, uint64_t m,
-                                  uint32_t* __restrict__ cnt) {
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < m; t += stride)
-    atomicAdd(&cnt[indices[t]], 1u);
+// ── transpose by bucket (umap.rs:357-381) ──
+// Rows are grouped in buckets of 2^shift consecutive rows. k_fz_bucket_count
+// counts the edges that point into each bucket, k_fz_bucket_scatter writes each
+// edge (i -> j, w) into its target bucket's region as the transposed entry of
+// row j, and k_fz_bucket_merge sorts one bucket's entries by row in LDS and
+// merges every row of the bucket. Global atomics are one per (tile, bucket);
+// the per-edge counting runs on LDS atomics.
+
+// Per tile of FZ_TILE_ROWS rows: edges per target bucket, added to btot.
+__global__ __launch_bounds__(1024) void k_fz_bucket_count(const uint32_t* __restrict__ indptr,
+                                                          const uint32_t* __restrict__ indices,
+                                                          uint32_t n, uint32_t shift, uint32_t nb,
+                                                          uint32_t* __restrict__ btot) {
+  __shared__ uint32_t h[FZ_NB_MAX];
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t b = tid; b < nb; b += blockDim.x) h[b] = 0;
+  __syncthreads();
+  const uint32_t r0 = blockIdx.x * FZ_TILE_ROWS, r1 = min(n, r0 + FZ_TILE_ROWS);
+  const uint32_t e0 = indptr[r0], e1 = indptr[r1];
+  for (uint32_t q = e0 + tid; q < e1; q += blockDim.x) atomicAdd(&h[indices[q] >> shift], 1u);
+  __syncthreads();
+  for (uint32_t b = tid; b < nb; b += blockDim.x)
+    if (h[b]) atomicAdd(&btot[b], h[b]);
 }
 
-// Start of the transposed part of each row's combined segment.
-__global__ void k_fuzzy_tpos(const uint32_t* __restrict__ indptr, const uint32_t* __restrict__ t_ptr,
-                             uint32_t n, unsigned long long* __restrict__ tpos) {
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n) tpos[j] = (unsigned long long)indptr[j + 1] + t_ptr[j];
-}
-
-// FZ_SCATTER_LANES lanes per row i, one edge (i -> j, w) each: the own entry in
-// place, and (i, w) appended to row j's transposed part (umap.rs:368-381; the
-// order within a transposed part is set by the sort that follows, not by this
-// scatter).
-__global__ __launch_bounds__(256) void k_fuzzy_scatter(const uint32_t* __restrict__ indptr,
-                                                       const uint32_t* __restrict__ indices,
-                                                       const float* __restrict__ w,
-                                                       const uint32_t* __restrict__ t_ptr, uint32_t n,
-                                                       unsigned long long* __restrict__ tpos,
-                                                       uint64_t* __restrict__ comb) {
-  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t i = (uint32_t)(g / FZ_SCATTER_LANES), l = (uint32_t)(g % FZ_SCATTER_LANES);
-  if (i >= n) return;
-  const uint32_t s = indptr[i], e = indptr[i + 1];
-  const uint64_t base = (uint64_t)s + t_ptr[i];
-  for (uint32_t q = s + l; q < e; q += FZ_SCATTER_LANES) {
-    const uint32_t j = indices[q];
-    const float wq = w[q];
-    comb[base + (q - s)] = fz_key(j, 0, wq);
-    comb[atomicAdd(&tpos[j], 1ull)] = fz_key(i, 1, wq);
+// Same tiles: reserve each bucket's share with one global atomic, then place
+// the tile's edges: pay = transposed entry key of row j (column i, tag 1),
+// pj = j's row within its bucket.
+__global__ __launch_bounds__(1024) void k_fz_bucket_scatter(
+    const uint32_t* __restrict__ indptr, const uint32_t* __restrict__ indices,
+    const float* __restrict__ w, uint32_t n, uint32_t shift, uint32_t nb,
+    const uint32_t* __restrict__ bstart, uint32_t* __restrict__ bfill, uint64_t* __restrict__ pay,
+    uint32_t* __restrict__ pj) {
+  __shared__ uint32_t h[FZ_NB_MAX];
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t b = tid; b < nb; b += blockDim.x) h[b] = 0;
+  __syncthreads();
+  const uint32_t r0 = blockIdx.x * FZ_TILE_ROWS, r1 = min(n, r0 + FZ_TILE_ROWS);
+  const uint32_t e0 = indptr[r0], e1 = indptr[r1];
+  for (uint32_t q = e0 + tid; q < e1; q += blockDim.x) atomicAdd(&h[indices[q] >> shift], 1u);
+  __syncthreads();
+  for (uint32_t b = tid; b < nb; b += blockDim.x)
+    if (h[b]) h[b] = bstart[b] + atomicAdd(&bfill[b], h[b]);
+  __syncthreads();
+  const uint32_t l = tid % FZ_SCATTER_LANES, per = blockDim.x / FZ_SCATTER_LANES;
+  const uint32_t mask = (1u << shift) - 1u;
+  for (uint32_t r = r0 + tid / FZ_SCATTER_LANES; r < r1; r += per) {
+    const uint32_t s = indptr[r], e = indptr[r + 1];
+    for (uint32_t q = s + l; q < e; q += FZ_SCATTER_LANES) {
+      const uint32_t j = indices[q];
+      const uint32_t pos = atomicAdd(&h[j >> shift], 1u);
+      pay[pos] = fz_key(r, 1, w[q]);
+      pj[pos] = j & mask;
+    }
   }
 }
 
-// One wave (a 64-thread block) per row of at most 256 combined entries, rows
-// strided over the grid: sort in registers, fold and compact to the start of
-// the row's segment; olen[i] = kept entries. Longer rows are listed in big[]
-// for k_fuzzy_merge_mid.
-__global__ __launch_bounds__(64) void k_fuzzy_merge_small(const uint32_t* __restrict__ indptr,
-                                                          const uint32_t* __restrict__ t_ptr,
-                                                          uint32_t n, uint64_t* __restrict__ comb,
-                                                          uint32_t* __restrict__ olen,
-                                                          uint32_t* __restrict__ big,
-                                                          uint32_t* __restrict__ nbig) {
-  __shared__ uint64_t s[4 * WAVE];
-  const int l = threadIdx.x;
-  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
-    const uint64_t off = (uint64_t)indptr[i] + t_ptr[i];
-    const uint32_t len = (indptr[i + 1] - indptr[i]) + (t_ptr[i + 1] - t_ptr[i]);
+namespace {
+
+// Exclusive scan in place of v[0, L) by the whole block (blockDim.x == FZ_MB);
+// v[L] = total. v may be in LDS or global memory.
+__device__ inline void fz_block_scan(uint32_t* v, uint32_t L, uint32_t* s_part) {
+  const uint32_t tid = threadIdx.x, l = tid & (WAVE - 1), wv = tid / WAVE;
+  constexpr uint32_t PER = 4, NWV = FZ_MB / WAVE;
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < L; base += FZ_MB * PER) {
+    const uint32_t i0 = base + tid * PER;
+    uint32_t x[PER], sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+      x[k] = i0 + k < L ? v[i0 + k] : 0u;
+      sum += x[k];
+    }
+    uint32_t inc = sum;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+      const uint32_t y = __shfl_up(inc, o, WAVE);
+      if (l >= (uint32_t)o) inc += y;
+    }
+    if (l == WAVE - 1) s_part[wv] = inc;
+    __syncthreads();
+    if (wv == 0) {
+      uint32_t t = l < NWV ? s_part[l] : 0u;
+#pragma unroll
+      for (int o = 1; o < WAVE; o <<= 1) {
+        const uint32_t y = __shfl_up(t, o, WAVE);
+        if (l >= (uint32_t)o) t += y;
+      }
+      if (l < NWV) s_part[l] = t;
+    }
+    __syncthreads();
+    uint32_t run = carry + (wv > 0 ? s_part[wv - 1] : 0u) + (inc - sum);
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k)
+      if (i0 + k < L) {
+        v[i0 + k] = run;
+        run += x[k];
+      }
+    carry += s_part[NWV - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) v[L] = carry;
+  __syncthreads();
+}
+
+__device__ inline void fz_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+}  // namespace
+
+// One block per bucket: counting sort of the bucket's transposed entries by row
+// (in LDS when they fit, else in srt_g at the bucket's region), t_ptr of its
+// rows, then one wave per row: the row's own entries and its transposed ones
+// sorted in registers, folded and compacted to the start of the row's combined
+// segment in comb; olen = kept entries. Rows of more than 256 combined entries
+// are written unsorted to their comb segment and listed in big[] for
+// k_fuzzy_merge_mid.
+__global__ __launch_bounds__(FZ_MB) void k_fz_bucket_merge(
+    const uint32_t* __restrict__ indptr, const uint32_t* __restrict__ indices,
+    const float* __restrict__ w, uint32_t n, uint32_t shift, uint32_t nb,
+    const uint32_t* __restrict__ bstart, const uint64_t* __restrict__ pay,
+    const uint32_t* __restrict__ pj, uint64_t* __restrict__ srt_g, uint32_t* __restrict__ cnt_g,
+    uint32_t* __restrict__ t_ptr, uint64_t* __restrict__ comb, uint32_t* __restrict__ olen,
+    uint32_t* __restrict__ big, uint32_t* __restrict__ nbig) {
+  extern __shared__ uint64_t lds[];
+  uint64_t* s_srt = lds;
+  uint64_t* s_wave = lds + FZ_BCAP;
+  uint32_t* s_off = reinterpret_cast<uint32_t*>(s_wave + (FZ_MB / WAVE) * 4 * WAVE);
+  uint32_t* s_cur = s_off + FZ_BROWS_LDS + 1;
+  __shared__ uint32_t s_part[FZ_MB / WAVE];
+  const uint32_t tid = threadIdx.x, l = tid & (WAVE - 1), wv = tid / WAVE;
+  const uint32_t b = blockIdx.x;
+  const uint32_t rows_b = 1u << shift, row0 = b << shift;
+  const uint32_t rows = min(rows_b, n - row0);
+  const uint32_t p0 = bstart[b], p1 = bstart[b + 1], c = p1 - p0;
+  uint64_t* srt = c <= (uint32_t)FZ_BCAP ? s_srt : srt_g + p0;
+  uint32_t* off = rows_b <= (uint32_t)FZ_BROWS_LDS ? s_off : cnt_g + (size_t)b * 2 * (rows_b + 1);
+  uint32_t* cur = rows_b <= (uint32_t)FZ_BROWS_LDS ? s_cur : off + rows_b + 1;
+  for (uint32_t r = tid; r < rows; r += FZ_MB) off[r] = 0;
+  __syncthreads();
+  for (uint32_t t = tid; t < c; t += FZ_MB) atomicAdd(&off[pj[p0 + t]], 1u);
+  __syncthreads();
+  fz_block_scan(off, rows, s_part);
+  for (uint32_t r = tid; r < rows; r += FZ_MB) {
+    cur[r] = off[r];
+    t_ptr[row0 + r] = p0 + off[r];
+  }
+  if (b == nb - 1 && tid == 0) t_ptr[n] = p1;
+  __syncthreads();
+  for (uint32_t t = tid; t < c; t += FZ_MB) srt[atomicAdd(&cur[pj[p0 + t]], 1u)] = pay[p0 + t];
+  __syncthreads();
+  uint64_t* ws = s_wave + wv * 4 * WAVE;
+  for (uint32_t r = wv; r < rows; r += FZ_MB / WAVE) {
+    const uint32_t row = row0 + r;
+    const uint32_t s = indptr[row], own = indptr[row + 1] - s;
+    const uint32_t ts = off[r], len = own + (off[r + 1] - ts);
+    const uint64_t coff = (uint64_t)s + p0 + ts;
+    auto entry = [&](uint32_t q) -> uint64_t {
+      return q < own ? fz_key(indices[s + q], 0, w[s + q]) : srt[ts + (q - own)];
+    };
     if (len > 4 * WAVE) {
-      if (l == 0) big[atomicAdd(nbig, 1u)] = i;
+      for (uint32_t q = l; q < len; q += WAVE) comb[coff + q] = entry(q);
+      if (l == 0) big[atomicAdd(nbig, 1u)] = row;
       continue;
     }
     if (len <= WAVE) {
-      uint64_t v = (uint32_t)l < len ? comb[off + l] : ~0ull;
-      s[l] = wave_sort64(v);
+      ws[l] = wave_sort64((uint32_t)l < len ? entry(l) : ~0ull);
     } else {
       uint64_t v[4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const uint32_t e = r * WAVE + l;
-        v[r] = e < len ? comb[off + e] : ~0ull;
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t e = q * WAVE + l;
+        v[q] = e < len ? entry(e) : ~0ull;
       }
       wave_sort_n<4>(v);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) s[r * WAVE + l] = v[r];
+      for (int q = 0; q < 4; ++q) ws[q * WAVE + l] = v[q];
     }
-    __syncthreads();
-    uint32_t cnt = 0;
-    for (uint32_t b = 0; b < len; b += WAVE) {
-      const uint32_t e = b + l;
+    fz_wave_sync();
+    uint32_t kept = 0;
+    for (uint32_t base = 0; base < len; base += WAVE) {
+      const uint32_t e = base + l;
       bool keep = false;
       uint64_t outk = 0;
       if (e < len) {
-        const uint32_t col = fz_col(s[e]);
-        if (e == 0 || fz_col(s[e - 1]) != col) {
+        const uint32_t col = fz_col(ws[e]);
+        if (e == 0 || fz_col(ws[e - 1]) != col) {
           float acc = 0.0f;
-          for (uint32_t t = e; t < len && fz_col(s[t]) == col; ++t) {
-            const float x = fz_w(s[t]);
+          for (uint32_t t = e; t < len && fz_col(ws[t]) == col; ++t) {
+            const float x = fz_w(ws[t]);
             acc = acc + x - acc * x;
           }
           keep = acc > 1e-5f;
@@ -281,11 +392,11 @@ __global__ __launch_bounds__(64) void k_fuzzy_merge_small(const uint32_t* __rest
         }
       }
       const uint64_t mask = __ballot(keep);
-      if (keep) comb[off + cnt + (uint32_t)__popcll(mask & ((1ull << l) - 1))] = outk;
-      cnt += (uint32_t)__popcll(mask);
+      if (keep) comb[coff + kept + (uint32_t)__popcll(mask & ((1ull << l) - 1))] = outk;
+      kept += (uint32_t)__popcll(mask);
     }
-    if (l == 0) olen[i] = cnt;
-    __syncthreads();
+    if (l == 0) olen[row] = kept;
+    fz_wave_sync();
   }
 }
 
