@@ -781,24 +781,31 @@ int fuzzy_core(Workspace* ws, const uint32_t* d_ip, const uint32_t* d_ix, const 
   R.n = n;
   uint32_t *h;
   TRY(ws->host_counters((void**)&h));
-  uint32_t shift = FZ_BLOG_MIN;
-  while ((((uint64_t)n + (1ull << shift) - 1) >> shift) > (uint64_t)FZ_NB_MAX) ++shift;
-  const uint32_t nb = (uint32_t)(((uint64_t)n + (1ull << shift) - 1) >> shift);
-  const uint32_t rows_b = 1u << shift;
-  uint32_t *flags, *olen, *big, *giant, *btot, *bstart, *bfill, *pj, *cnt_g;
+  // Super-buckets of 2^sshift rows, at most FZ_SB_MAX of them; buckets of
+  // FZ_BUCKET_ROWS rows.
+  uint32_t sshift = FZ_BUCKET_LOG;
+  while ((((uint64_t)n + (1ull << sshift) - 1) >> sshift) > (uint64_t)FZ_SB_MAX) ++sshift;
+  if (sshift - FZ_BUCKET_LOG > 12)
+    return fail(PSQ_E_UNSUPPORTED, "n_obs = %u exceeds the fuzzy path's limit of 2^26 rows", n);
+  const uint32_t ns = (uint32_t)(((uint64_t)n + (1ull << sshift) - 1) >> sshift);
+  const uint32_t nb = (uint32_t)(((uint64_t)n + FZ_BUCKET_ROWS - 1) >> FZ_BUCKET_LOG);
+  uint32_t *flags, *olen, *big, *giant, *stot, *sstart, *sfill, *btot, *bstart, *bfill, *pj1;
   unsigned long long* tot;
-  uint64_t *pay, *srt_g;
-  float* w;
+  uint64_t *pay0, *pay1, *pay2;
+  uint8_t* pl2;
   TRY(wbuf(ws, "fz_flags", 4, &flags));  // bad, nbig, ngiant, pad
   TRY(wbuf(ws, "fz_tot", 1, &tot));
-  TRY(wbuf(ws, "fz_w", std::max<uint64_t>(m, 1), &w));
+  TRY(wbuf(ws, "fz_pay0", std::max<uint64_t>(m, 1), &pay0));
+  TRY(wbuf(ws, "fz_stot", (size_t)ns + 1, &stot));
+  TRY(wbuf(ws, "fz_sstart", (size_t)ns + 1, &sstart));
+  TRY(wbuf(ws, "fz_sfill", ns, &sfill));
   TRY(wbuf(ws, "fz_btot", (size_t)nb + 1, &btot));
   TRY(wbuf(ws, "fz_bstart", (size_t)nb + 1, &bstart));
   TRY(wbuf(ws, "fz_bfill", nb, &bfill));
-  TRY(wbuf(ws, "fz_pay", std::max<uint64_t>(m, 1), &pay));
-  TRY(wbuf(ws, "fz_pj", std::max<uint64_t>(m, 1), &pj));
-  TRY(wbuf(ws, "fz_srt", std::max<uint64_t>(m, 1), &srt_g));
-  TRY(wbuf(ws, "fz_cntg", rows_b > (uint32_t)FZ_BROWS_LDS ? (size_t)nb * 2 * (rows_b + 1) : 1, &cnt_g));
+  TRY(wbuf(ws, "fz_pay1", std::max<uint64_t>(m, 1), &pay1));
+  TRY(wbuf(ws, "fz_pj1", std::max<uint64_t>(m, 1), &pj1));
+  TRY(wbuf(ws, "fz_pay2", std::max<uint64_t>(m, 1), &pay2));
+  TRY(wbuf(ws, "fz_pl2", std::max<uint64_t>(m, 1), &pl2));
   TRY(wbuf(ws, "fz_tptr", (size_t)n + 1, &R.t_ptr));
   TRY(wbuf(ws, "fz_comb", std::max<uint64_t>(2 * m, 1), &R.comb));
   TRY(wbuf(ws, "fz_olen", n, &olen));
@@ -816,25 +823,39 @@ int fuzzy_core(Workspace* ws, const uint32_t* d_ip, const uint32_t* d_ix, const 
   if (h[0] & 1u) return fail(PSQ_E_INVALID, "indptr must be non-decreasing");
   if (h[0] & 2u) return fail(PSQ_E_INVALID, "an index of the graph is >= n_obs (%u)", n);
   // Phase 1: membership strengths (umap.rs:343-354).
-  hipLaunchKernelGGL(k_fuzzy_strengths, dim3(grid_for(n, 256)), dim3(256), 0, s, d_ip, d_dx, n, w);
+  hipLaunchKernelGGL(k_fuzzy_strengths, dim3(grid_for(n, 256)), dim3(256), 0, s, d_ip, d_dx, n, pay0);
   HIPC(hipGetLastError());
-  // Phase 2: transpose by bucket (umap.rs:357-381).
-  const unsigned tiles = (unsigned)((n + FZ_TILE_ROWS - 1) / FZ_TILE_ROWS);
+  // Phase 2: transpose by bucket (umap.rs:357-381), two partition levels.
+  const unsigned chunks1 = std::max<unsigned>(1, (unsigned)((m + FZ_P1_CHUNK - 1) / FZ_P1_CHUNK));
+  TRY(fill_u32(stot, (size_t)ns + 1, 0, s));
+  TRY(fill_u32(sfill, ns, 0, s));
   TRY(fill_u32(btot, (size_t)nb + 1, 0, s));
   TRY(fill_u32(bfill, nb, 0, s));
-  hipLaunchKernelGGL(k_fz_bucket_count, dim3(tiles), dim3(1024), 0, s, d_ip, d_ix, n, shift, nb, btot);
+  hipLaunchKernelGGL(k_fz_l1_count, dim3(std::max<unsigned>(1, (unsigned)((m + FZ_C1_CHUNK - 1) / FZ_C1_CHUNK))),
+                     dim3(1024), 0, s, d_ix, m, sshift, ns, stot);
+  HIPC(hipGetLastError());
+  TRY(scan_u32(ws, stot, sstart, ns, s));
+  if (m > 0) {
+    hipLaunchKernelGGL(k_fz_l1_scatter, dim3(chunks1), dim3(1024), 0, s, pay0, d_ix, m, sshift, ns,
+                       sstart, sfill, pay1, pj1);
+    HIPC(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_fz_l2_count, dim3(ns, FZ_L2_SHARES), dim3(1024), 0, s, sstart, pj1, sshift,
+                     btot);
   HIPC(hipGetLastError());
   TRY(scan_u32(ws, btot, bstart, nb, s));
-  hipLaunchKernelGGL(k_fz_bucket_scatter, dim3(tiles), dim3(1024), 0, s, d_ip, d_ix, w, n, shift, nb,
-                     bstart, bfill, pay, pj);
+  const size_t l2_smem = (size_t)FZ_P2_CHUNK * (sizeof(uint64_t) + sizeof(uint32_t));
+  HIPC(hipFuncSetAttribute((const void*)k_fz_l2_scatter, hipFuncAttributeMaxDynamicSharedMemorySize,
+                           (int)l2_smem));
+  hipLaunchKernelGGL(k_fz_l2_scatter, dim3(ns, FZ_L2_SHARES), dim3(1024), l2_smem, s, sstart, pay1,
+                     pj1, sshift, bstart, bfill, pay2, pl2);
   HIPC(hipGetLastError());
   // Phase 3: per-row sort, fuzzy union and compaction (umap.rs:488-537), by row size.
-  const size_t bm_smem = ((size_t)FZ_BCAP + (size_t)(FZ_MB / WAVE) * 4 * WAVE) * sizeof(uint64_t) +
-                         (2 * (size_t)FZ_BROWS_LDS + 1) * sizeof(uint32_t);
+  const size_t bm_smem = (size_t)FZ_BCAP * sizeof(uint64_t);
   HIPC(hipFuncSetAttribute((const void*)k_fz_bucket_merge, hipFuncAttributeMaxDynamicSharedMemorySize,
                            (int)bm_smem));
-  hipLaunchKernelGGL(k_fz_bucket_merge, dim3(nb), dim3(FZ_MB), bm_smem, s, d_ip, d_ix, w, n, shift, nb,
-                     bstart, pay, pj, srt_g, cnt_g, R.t_ptr, R.comb, olen, big, flags + 1);
+  hipLaunchKernelGGL(k_fz_bucket_merge, dim3(nb), dim3(FZ_MB), bm_smem, s, d_ip, d_ix, pay0, n, nb, bstart,
+                     pay2, pl2, R.t_ptr, R.comb, olen, big, flags + 1);
   HIPC(hipGetLastError());
   const size_t mid_smem = (size_t)FZ_MID_CAP * sizeof(uint64_t);
   HIPC(hipFuncSetAttribute((const void*)k_fuzzy_merge_mid, hipFuncAttributeMaxDynamicSharedMemorySize,

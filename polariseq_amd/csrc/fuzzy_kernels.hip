@@ -31,8 +31,8 @@ __device__ inline double fz_term(double d, double rho, double mid) {
 
 // umap.rs:551-608 for one row of `len` distances read through get(j).
 // The sums run left to right in f64, as Rust's iterator sum does.
-template <typename Get>
-__device__ inline void fz_smooth(Get get, uint32_t len, float* __restrict__ wr) {
+template <typename Get, typename Put>
+__device__ inline void fz_smooth(Get get, uint32_t len, Put put) {
   const double target = log2((double)(len + 1));
   double rho = 0.0;  // local_connectivity = 1: the nearest non-zero distance
   for (uint32_t j = 0; j < len; ++j) {
@@ -61,7 +61,7 @@ __device__ inline void fz_smooth(Get get, uint32_t len, float* __restrict__ wr) 
   const double sigma = rho > 0.0 ? fmax(mid, 1e-3 * mean) : mid;
   for (uint32_t j = 0; j < len; ++j) {
     const double x = get(j) - rho;
-    wr[j] = (x <= 0.0 || sigma == 0.0) ? 1.0f : (float)exp(-(x / sigma));
+    put(j, (x <= 0.0 || sigma == 0.0) ? 1.0f : (float)exp(-(x / sigma)));
   }
 }
 
@@ -107,6 +107,12 @@ __device__ inline uint32_t fz_fold_block(const uint64_t* s, uint32_t len, uint64
   return cnt;
 }
 
+__device__ inline void fz_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // One bitonic compare-exchange step over pairs t of a P-long sequence.
 __device__ inline void fz_bitonic_pair(uint64_t* s, uint32_t t, uint32_t size, uint32_t stride) {
   const uint32_t lo = 2 * t - (t & (stride - 1));
@@ -135,16 +141,18 @@ __global__ void k_fuzzy_check(const uint32_t* __restrict__ indptr, uint32_t n,
 }
 
 // umap.rs:343-354: one thread per row, distances held in registers when the
-// row has at most FZ_KR entries.
+// row has at most FZ_KR entries. Each strength w of edge (i -> j) is written as
+// the transposed entry it becomes, fz_key(i, 1, w), at the edge's position.
 __global__ __launch_bounds__(256) void k_fuzzy_strengths(const uint32_t* __restrict__ indptr,
                                                          const float* __restrict__ dists, uint32_t n,
-                                                         float* __restrict__ w) {
+                                                         uint64_t* __restrict__ pay0) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t s = indptr[i];
   const uint32_t len = indptr[i + 1] - s;
   if (len == 0) return;
   const float* dr = dists + s;
+  uint64_t* pr = pay0 + s;
   if (len <= FZ_KR) {
     double d[FZ_KR];
 #pragma unroll
@@ -180,224 +188,353 @@ __global__ __launch_bounds__(256) void k_fuzzy_strengths(const uint32_t* __restr
       if ((uint32_t)j < len) sum += d[j];
     const double mean = sum / (double)(len + 1);
     const double sigma = rho > 0.0 ? fmax(mid, 1e-3 * mean) : mid;
-    float* wr = w + s;
 #pragma unroll
     for (int j = 0; j < FZ_KR; ++j)
       if ((uint32_t)j < len) {
         const double x = d[j] - rho;
-        wr[j] = (x <= 0.0 || sigma == 0.0) ? 1.0f : (float)exp(-(x / sigma));
+        pr[j] = fz_key(i, 1, (x <= 0.0 || sigma == 0.0) ? 1.0f : (float)exp(-(x / sigma)));
       }
   } else {
-    fz_smooth([&](uint32_t j) { return (double)dr[j]; }, len, w + s);
+    fz_smooth([&](uint32_t j) { return (double)dr[j]; }, len,
+              [&](uint32_t j, float v) { pr[j] = fz_key(i, 1, v); });
   }
 }
 
 // ── transpose by bucket (umap.rs:357-381) ──
-// Rows are grouped in buckets of 2^shift consecutive rows. k_fz_bucket_count
-// counts the edges that point into each bucket, k_fz_bucket_scatter writes each
-// edge (i -> j, w) into its target bucket's region as the transposed entry of
-// row j, and k_fz_bucket_merge sorts one bucket's entries by row in LDS and
-// merges every row of the bucket. Global atomics are one per (tile, bucket);
-// the per-edge counting runs on LDS atomics.
+// Rows are grouped in buckets of FZ_BUCKET_ROWS consecutive rows, and buckets
+// in super-buckets of 2^sshift rows (at most FZ_SB_MAX of them). Every edge
+// (i -> j, w) becomes the transposed entry (column i, tag 1, w) of row j in two
+// partition passes, each counting in LDS and reserving its output runs with one
+// global atomic per (block, part):
+//   level 1 (k_fz_l1_*): CSR tiles -> super-bucket regions (pay1, full j in pj1);
+//   level 2 (k_fz_l2_*): super-bucket regions -> bucket regions (pay2, pl2 =
+//   j's row within its bucket).
+// k_fz_bucket_merge then counting-sorts one bucket's own and transposed entries
+// by row in LDS and merges every row of the bucket.
 
-// Per tile of FZ_TILE_ROWS rows: edges per target bucket, added to btot.
-__global__ __launch_bounds__(1024) void k_fz_bucket_count(const uint32_t* __restrict__ indptr,
-                                                          const uint32_t* __restrict__ indices,
-                                                          uint32_t n, uint32_t shift, uint32_t nb,
-                                                          uint32_t* __restrict__ btot) {
-  __shared__ uint32_t h[FZ_NB_MAX];
+// Level 1 count: edges per target super-bucket, per chunk of FZ_C1_CHUNK edges.
+__global__ __launch_bounds__(1024) void k_fz_l1_count(const uint32_t* __restrict__ indices,
+                                                      uint64_t m, uint32_t sshift, uint32_t ns,
+                                                      uint32_t* __restrict__ stot) {
+  __shared__ uint32_t h[FZ_SB_MAX];
   const uint32_t tid = threadIdx.x;
-  for (uint32_t b = tid; b < nb; b += blockDim.x) h[b] = 0;
+  if (tid < ns) h[tid] = 0;
   __syncthreads();
-  const uint32_t r0 = blockIdx.x * FZ_TILE_ROWS, r1 = min(n, r0 + FZ_TILE_ROWS);
-  const uint32_t e0 = indptr[r0], e1 = indptr[r1];
-  for (uint32_t q = e0 + tid; q < e1; q += blockDim.x) atomicAdd(&h[indices[q] >> shift], 1u);
+  const uint64_t e0 = (uint64_t)blockIdx.x * FZ_C1_CHUNK;
+  const uint64_t e1 = min(m, e0 + FZ_C1_CHUNK);
+  for (uint64_t e = e0 + tid; e < e1; e += blockDim.x) atomicAdd(&h[indices[e] >> sshift], 1u);
   __syncthreads();
-  for (uint32_t b = tid; b < nb; b += blockDim.x)
-    if (h[b]) atomicAdd(&btot[b], h[b]);
-}
-
-// Same tiles: reserve each bucket's share with one global atomic, then place
-// the tile's edges: pay = transposed entry key of row j (column i, tag 1),
-// pj = j's row within its bucket.
-__global__ __launch_bounds__(1024) void k_fz_bucket_scatter(
-    const uint32_t* __restrict__ indptr, const uint32_t* __restrict__ indices,
-    const float* __restrict__ w, uint32_t n, uint32_t shift, uint32_t nb,
-    const uint32_t* __restrict__ bstart, uint32_t* __restrict__ bfill, uint64_t* __restrict__ pay,
-    uint32_t* __restrict__ pj) {
-  __shared__ uint32_t h[FZ_NB_MAX];
-  const uint32_t tid = threadIdx.x;
-  for (uint32_t b = tid; b < nb; b += blockDim.x) h[b] = 0;
-  __syncthreads();
-  const uint32_t r0 = blockIdx.x * FZ_TILE_ROWS, r1 = min(n, r0 + FZ_TILE_ROWS);
-  const uint32_t e0 = indptr[r0], e1 = indptr[r1];
-  for (uint32_t q = e0 + tid; q < e1; q += blockDim.x) atomicAdd(&h[indices[q] >> shift], 1u);
-  __syncthreads();
-  for (uint32_t b = tid; b < nb; b += blockDim.x)
-    if (h[b]) h[b] = bstart[b] + atomicAdd(&bfill[b], h[b]);
-  __syncthreads();
-  const uint32_t l = tid % FZ_SCATTER_LANES, per = blockDim.x / FZ_SCATTER_LANES;
-  const uint32_t mask = (1u << shift) - 1u;
-  for (uint32_t r = r0 + tid / FZ_SCATTER_LANES; r < r1; r += per) {
-    const uint32_t s = indptr[r], e = indptr[r + 1];
-    for (uint32_t q = s + l; q < e; q += FZ_SCATTER_LANES) {
-      const uint32_t j = indices[q];
-      const uint32_t pos = atomicAdd(&h[j >> shift], 1u);
-      pay[pos] = fz_key(r, 1, w[q]);
-      pj[pos] = j & mask;
-    }
-  }
+  if (tid < ns && h[tid]) atomicAdd(&stot[tid], h[tid]);
 }
 
 namespace {
 
-// Exclusive scan in place of v[0, L) by the whole block (blockDim.x == FZ_MB);
-// v[L] = total. v may be in LDS or global memory.
-__device__ inline void fz_block_scan(uint32_t* v, uint32_t L, uint32_t* s_part) {
+// Exclusive scan in place of v[0, L), L <= 4 * blockDim.x, by the whole
+// block (each thread four consecutive values); v[L] = total.
+__device__ inline void fz_block_scan_small(uint32_t* v, uint32_t L, uint32_t* s_part) {
   const uint32_t tid = threadIdx.x, l = tid & (WAVE - 1), wv = tid / WAVE;
-  constexpr uint32_t PER = 4, NWV = FZ_MB / WAVE;
-  uint32_t carry = 0;
-  for (uint32_t base = 0; base < L; base += FZ_MB * PER) {
-    const uint32_t i0 = base + tid * PER;
-    uint32_t x[PER], sum = 0;
+  const uint32_t nwv = blockDim.x / WAVE;
+  uint32_t x[4], sum = 0;
 #pragma unroll
-    for (uint32_t k = 0; k < PER; ++k) {
-      x[k] = i0 + k < L ? v[i0 + k] : 0u;
-      sum += x[k];
-    }
-    uint32_t inc = sum;
+  for (uint32_t k = 0; k < 4; ++k) {
+    x[k] = 4 * tid + k < L ? v[4 * tid + k] : 0u;
+    sum += x[k];
+  }
+  uint32_t inc = sum;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const uint32_t y = __shfl_up(inc, o, WAVE);
+    if (l >= (uint32_t)o) inc += y;
+  }
+  if (l == WAVE - 1) s_part[wv] = inc;
+  __syncthreads();
+  if (wv == 0) {
+    uint32_t t = l < nwv ? s_part[l] : 0u;
 #pragma unroll
     for (int o = 1; o < WAVE; o <<= 1) {
-      const uint32_t y = __shfl_up(inc, o, WAVE);
-      if (l >= (uint32_t)o) inc += y;
+      const uint32_t y = __shfl_up(t, o, WAVE);
+      if (l >= (uint32_t)o) t += y;
     }
-    if (l == WAVE - 1) s_part[wv] = inc;
-    __syncthreads();
-    if (wv == 0) {
-      uint32_t t = l < NWV ? s_part[l] : 0u;
-#pragma unroll
-      for (int o = 1; o < WAVE; o <<= 1) {
-        const uint32_t y = __shfl_up(t, o, WAVE);
-        if (l >= (uint32_t)o) t += y;
-      }
-      if (l < NWV) s_part[l] = t;
-    }
-    __syncthreads();
-    uint32_t run = carry + (wv > 0 ? s_part[wv - 1] : 0u) + (inc - sum);
-#pragma unroll
-    for (uint32_t k = 0; k < PER; ++k)
-      if (i0 + k < L) {
-        v[i0 + k] = run;
-        run += x[k];
-      }
-    carry += s_part[NWV - 1];
-    __syncthreads();
+    if (l < nwv) s_part[l] = t;
   }
-  if (threadIdx.x == 0) v[L] = carry;
+  __syncthreads();
+  uint32_t run = (wv > 0 ? s_part[wv - 1] : 0u) + inc - sum;
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k)
+    if (4 * tid + k < L) {
+      v[4 * tid + k] = run;
+      run += x[k];
+    }
+  if (tid == 0) v[L] = s_part[nwv - 1];
   __syncthreads();
 }
 
-__device__ inline void fz_wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+// One staged partition of up to NPT * blockDim.x entries (key[t], tgt[t]) into
+// parts part(tgt) < np: counted in LDS, reserved in global memory with one
+// atomic per part (base = start[part0 + p] + atomicAdd(fill[part0 + p], count)),
+// sorted by part in LDS, then written out so that consecutive threads write
+// consecutive addresses of one part's run. out_j(tgt) is what the second
+// output array receives.
+template <int NPT, typename PartF, typename OutJ, typename JT>
+__device__ inline void fz_stage_partition(const uint64_t* key, const uint32_t* tgt, uint32_t cnt,
+                                          uint32_t np, uint32_t part0, PartF part,
+                                          const uint32_t* __restrict__ start,
+                                          uint32_t* __restrict__ fill, uint64_t* __restrict__ out_key,
+                                          JT* __restrict__ out_j, OutJ out_jv, uint64_t* s_key,
+                                          uint32_t* s_tgt, uint32_t* s_cnt, uint32_t* s_base,
+                                          uint32_t* s_part) {
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t p = tid; p < np; p += blockDim.x) s_cnt[p] = 0;
+  __syncthreads();
+  uint64_t k[NPT];
+  uint32_t t[NPT], rank[NPT];
+#pragma unroll
+  for (int q = 0; q < NPT; ++q) {
+    const uint32_t e = q * blockDim.x + tid;
+    if (e < cnt) {
+      k[q] = key[e];
+      t[q] = tgt[e];
+      rank[q] = atomicAdd(&s_cnt[part(t[q])], 1u);
+    }
+  }
+  __syncthreads();
+  for (uint32_t p = tid; p < np; p += blockDim.x) {
+    const uint32_t c = s_cnt[p];
+    s_base[p] = c ? start[part0 + p] + atomicAdd(&fill[part0 + p], c) : 0u;
+  }
+  fz_block_scan_small(s_cnt, np, s_part);  // s_cnt becomes the LDS offsets
+#pragma unroll
+  for (int q = 0; q < NPT; ++q) {
+    const uint32_t e = q * blockDim.x + tid;
+    if (e < cnt) {
+      const uint32_t pos = s_cnt[part(t[q])] + rank[q];
+      s_key[pos] = k[q];
+      s_tgt[pos] = t[q];
+    }
+  }
+  __syncthreads();
+  for (uint32_t e = tid; e < cnt; e += blockDim.x) {
+    const uint32_t tg = s_tgt[e], p = part(tg);
+    const uint32_t pos = s_base[p] + (e - s_cnt[p]);
+    out_key[pos] = s_key[e];
+    out_j[pos] = out_jv(tg);
+  }
+  __syncthreads();
 }
 
 }  // namespace
 
-// One block per bucket: counting sort of the bucket's transposed entries by row
-// (in LDS when they fit, else in srt_g at the bucket's region), t_ptr of its
-// rows, then one wave per row: the row's own entries and its transposed ones
-// sorted in registers, folded and compacted to the start of the row's combined
-// segment in comb; olen = kept entries. Rows of more than 256 combined entries
-// are written unsorted to their comb segment and listed in big[] for
-// k_fuzzy_merge_mid.
-__global__ __launch_bounds__(FZ_MB) void k_fz_bucket_merge(
+// Level 1 placement, per chunk of FZ_P1_CHUNK edges: pay1 = the transposed
+// entry key, pj1 = the full target row j, in super-bucket order.
+__global__ __launch_bounds__(1024) void k_fz_l1_scatter(
+    const uint64_t* __restrict__ pay0, const uint32_t* __restrict__ indices, uint64_t m,
+    uint32_t sshift, uint32_t ns, const uint32_t* __restrict__ sstart, uint32_t* __restrict__ sfill,
+    uint64_t* __restrict__ pay1, uint32_t* __restrict__ pj1) {
+  __shared__ uint64_t s_key[FZ_P1_CHUNK];
+  __shared__ uint32_t s_tgt[FZ_P1_CHUNK];
+  __shared__ uint32_t s_cnt[FZ_SB_MAX + 1], s_base[FZ_SB_MAX], s_part[1024 / WAVE];
+  const uint64_t e0 = (uint64_t)blockIdx.x * FZ_P1_CHUNK;
+  const uint32_t cnt = (uint32_t)min((uint64_t)FZ_P1_CHUNK, m - e0);
+  fz_stage_partition<FZ_P1_CHUNK / 1024>(
+      pay0 + e0, indices + e0, cnt, ns, 0u, [&](uint32_t j) { return j >> sshift; }, sstart, sfill,
+      pay1, pj1, [](uint32_t j) { return j; }, s_key, s_tgt, s_cnt, s_base, s_part);
+}
+
+// Level 2 count: block (s, c) takes the c-th of gridDim.y strided shares of
+// super-bucket s's region; buckets of s are local ids 0 .. 2^(sshift - 8).
+__global__ __launch_bounds__(1024) void k_fz_l2_count(const uint32_t* __restrict__ sstart,
+                                                      const uint32_t* __restrict__ pj1,
+                                                      uint32_t sshift, uint32_t* __restrict__ btot) {
+  __shared__ uint32_t h[FZ_BPS_MAX];
+  const uint32_t tid = threadIdx.x, sb = blockIdx.x;
+  const uint32_t nbk = 1u << (sshift - FZ_BUCKET_LOG), bk0 = sb * nbk;
+  for (uint32_t b = tid; b < nbk; b += blockDim.x) h[b] = 0;
+  __syncthreads();
+  const uint32_t q0 = sstart[sb], q1 = sstart[sb + 1];
+  const uint32_t stride = gridDim.y * FZ_P2_CHUNK;
+  for (uint32_t c0 = q0 + blockIdx.y * FZ_P2_CHUNK; c0 < q1; c0 += stride)
+    for (uint32_t q = c0 + tid; q < min(q1, c0 + FZ_P2_CHUNK); q += blockDim.x)
+      atomicAdd(&h[(pj1[q] >> FZ_BUCKET_LOG) - bk0], 1u);
+  __syncthreads();
+  for (uint32_t b = tid; b < nbk; b += blockDim.x)
+    if (h[b]) atomicAdd(&btot[bk0 + b], h[b]);
+}
+
+// Level 2 placement, same shares, chunk by chunk: pay2 = the entry key, pl2 =
+// j's row within its bucket, in bucket order.
+__global__ __launch_bounds__(1024) void k_fz_l2_scatter(
+    const uint32_t* __restrict__ sstart, const uint64_t* __restrict__ pay1,
+    const uint32_t* __restrict__ pj1, uint32_t sshift, const uint32_t* __restrict__ bstart,
+    uint32_t* __restrict__ bfill, uint64_t* __restrict__ pay2, uint8_t* __restrict__ pl2) {
+  extern __shared__ uint64_t s_key[];  // FZ_P2_CHUNK
+  uint32_t* s_tgt = reinterpret_cast<uint32_t*>(s_key + FZ_P2_CHUNK);
+  __shared__ uint32_t s_cnt[FZ_BPS_MAX + 1], s_base[FZ_BPS_MAX], s_part[1024 / WAVE];
+  const uint32_t sb = blockIdx.x;
+  const uint32_t nbk = 1u << (sshift - FZ_BUCKET_LOG), bk0 = sb * nbk;
+  const uint32_t q0 = sstart[sb], q1 = sstart[sb + 1];
+  const uint32_t stride = gridDim.y * FZ_P2_CHUNK;
+  for (uint32_t c0 = q0 + blockIdx.y * FZ_P2_CHUNK; c0 < q1; c0 += stride)
+    fz_stage_partition<FZ_P2_CHUNK / 1024>(
+        pay1 + c0, pj1 + c0, min((uint32_t)FZ_P2_CHUNK, q1 - c0), nbk, bk0,
+        [&](uint32_t j) { return (j >> FZ_BUCKET_LOG) - bk0; }, bstart, bfill, pay2, pl2,
+        [](uint32_t j) { return (uint8_t)(j & (FZ_BUCKET_ROWS - 1)); }, s_key, s_tgt, s_cnt,
+        s_base, s_part);
+}
+
+
+// One block per bucket of FZ_BUCKET_ROWS rows: the bucket's own entries (a
+// contiguous CSR range) and its transposed entries are counting-sorted by row
+// into one array, in LDS when they fit (FZ_BCAP), else in the bucket's own
+// region of comb. t_ptr of the bucket's rows is written. Then one wave per row
+// sorts the row's segment in registers, writes it back, folds and compacts it
+// to the start of the row's combined segment in comb; olen = kept entries.
+// Rows of more than 256 combined entries are written unsorted to their comb
+// segment and listed in big[] for k_fuzzy_merge_mid.
+__global__ __launch_bounds__(FZ_MB, 2) void k_fz_bucket_merge(
     const uint32_t* __restrict__ indptr, const uint32_t* __restrict__ indices,
-    const float* __restrict__ w, uint32_t n, uint32_t shift, uint32_t nb,
-    const uint32_t* __restrict__ bstart, const uint64_t* __restrict__ pay,
-    const uint32_t* __restrict__ pj, uint64_t* __restrict__ srt_g, uint32_t* __restrict__ cnt_g,
+    const uint64_t* __restrict__ pay0, uint32_t n, uint32_t nb, const uint32_t* __restrict__ bstart,
+    const uint64_t* __restrict__ pay2, const uint8_t* __restrict__ pl2,
     uint32_t* __restrict__ t_ptr, uint64_t* __restrict__ comb, uint32_t* __restrict__ olen,
     uint32_t* __restrict__ big, uint32_t* __restrict__ nbig) {
   extern __shared__ uint64_t lds[];
-  uint64_t* s_srt = lds;
-  uint64_t* s_wave = lds + FZ_BCAP;
-  uint32_t* s_off = reinterpret_cast<uint32_t*>(s_wave + (FZ_MB / WAVE) * 4 * WAVE);
-  uint32_t* s_cur = s_off + FZ_BROWS_LDS + 1;
+  uint64_t* s_srt = lds;  // FZ_BCAP
+  __shared__ uint32_t s_ip[FZ_BUCKET_ROWS + 1], s_off[FZ_BUCKET_ROWS + 1], s_cur[FZ_BUCKET_ROWS];
   __shared__ uint32_t s_part[FZ_MB / WAVE];
+  __shared__ uint32_t s_short[FZ_BUCKET_ROWS], s_long[FZ_BUCKET_ROWS], s_ncls[2];
   const uint32_t tid = threadIdx.x, l = tid & (WAVE - 1), wv = tid / WAVE;
   const uint32_t b = blockIdx.x;
-  const uint32_t rows_b = 1u << shift, row0 = b << shift;
-  const uint32_t rows = min(rows_b, n - row0);
+  if (tid < 2) s_ncls[tid] = 0;
+  const uint32_t row0 = b * FZ_BUCKET_ROWS, rows = min((uint32_t)FZ_BUCKET_ROWS, n - row0);
   const uint32_t p0 = bstart[b], p1 = bstart[b + 1], c = p1 - p0;
-  uint64_t* srt = c <= (uint32_t)FZ_BCAP ? s_srt : srt_g + p0;
-  uint32_t* off = rows_b <= (uint32_t)FZ_BROWS_LDS ? s_off : cnt_g + (size_t)b * 2 * (rows_b + 1);
-  uint32_t* cur = rows_b <= (uint32_t)FZ_BROWS_LDS ? s_cur : off + rows_b + 1;
-  for (uint32_t r = tid; r < rows; r += FZ_MB) off[r] = 0;
+  for (uint32_t r = tid; r <= rows; r += FZ_MB) s_ip[r] = indptr[row0 + r];
   __syncthreads();
-  for (uint32_t t = tid; t < c; t += FZ_MB) atomicAdd(&off[pj[p0 + t]], 1u);
+  const uint32_t a0 = s_ip[0], own_all = s_ip[rows] - a0, total = own_all + c;
+  const uint64_t cbase = (uint64_t)a0 + p0;  // the bucket's region of comb
+  for (uint32_t r = tid; r < rows; r += FZ_MB) s_off[r] = s_ip[r + 1] - s_ip[r];
   __syncthreads();
-  fz_block_scan(off, rows, s_part);
+  for (uint32_t t = tid; t < c; t += FZ_MB) atomicAdd(&s_off[pl2[p0 + t]], 1u);
+  __syncthreads();
+  fz_block_scan_small(s_off, rows, s_part);
   for (uint32_t r = tid; r < rows; r += FZ_MB) {
-    cur[r] = off[r];
-    t_ptr[row0 + r] = p0 + off[r];
+    const uint32_t own_r = s_ip[r + 1] - s_ip[r];
+    s_cur[r] = s_off[r] + own_r;
+    t_ptr[row0 + r] = p0 + s_off[r] - (s_ip[r] - a0);
   }
   if (b == nb - 1 && tid == 0) t_ptr[n] = p1;
-  __syncthreads();
-  for (uint32_t t = tid; t < c; t += FZ_MB) srt[atomicAdd(&cur[pj[p0 + t]], 1u)] = pay[p0 + t];
-  __syncthreads();
-  uint64_t* ws = s_wave + wv * 4 * WAVE;
-  for (uint32_t r = wv; r < rows; r += FZ_MB / WAVE) {
-    const uint32_t row = row0 + r;
-    const uint32_t s = indptr[row], own = indptr[row + 1] - s;
-    const uint32_t ts = off[r], len = own + (off[r + 1] - ts);
-    const uint64_t coff = (uint64_t)s + p0 + ts;
-    auto entry = [&](uint32_t q) -> uint64_t {
-      return q < own ? fz_key(indices[s + q], 0, w[s + q]) : srt[ts + (q - own)];
-    };
-    if (len > 4 * WAVE) {
-      for (uint32_t q = l; q < len; q += WAVE) comb[coff + q] = entry(q);
-      if (l == 0) big[atomicAdd(nbig, 1u)] = row;
-      continue;
-    }
-    if (len <= WAVE) {
-      ws[l] = wave_sort64((uint32_t)l < len ? entry(l) : ~0ull);
-    } else {
-      uint64_t v[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const uint32_t e = q * WAVE + l;
-        v[q] = e < len ? entry(e) : ~0ull;
+  // The rest runs on the sorted array in LDS or, for a bucket too large for
+  // it, in the bucket's comb region; each call below sees one address space.
+  auto rest = [&](uint64_t* srt) __attribute__((always_inline)) {
+    // own entries: the bucket's contiguous CSR range, each entry's row found by
+    // binary search over the row pointers
+    for (uint32_t t = tid; t < own_all; t += FZ_MB) {
+      const uint32_t q = a0 + t;
+      uint32_t lo = 0, hi = rows;  // s_ip[lo] <= q < s_ip[hi]
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_ip[mid] <= q) lo = mid;
+        else hi = mid;
       }
-      wave_sort_n<4>(v);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) ws[q * WAVE + l] = v[q];
+      srt[s_off[lo] + (q - s_ip[lo])] = ((uint64_t)indices[q] << 32) | ((uint32_t)pay0[q] & 0x7fffffffu);
     }
-    fz_wave_sync();
-    uint32_t kept = 0;
-    for (uint32_t base = 0; base < len; base += WAVE) {
-      const uint32_t e = base + l;
+    __syncthreads();
+    for (uint32_t t = tid; t < c; t += FZ_MB) srt[atomicAdd(&s_cur[pl2[p0 + t]], 1u)] = pay2[p0 + t];
+    __syncthreads();
+    // Rows of at most 32 entries are taken in pairs, a half wave each; longer
+    // rows one per wave.
+    constexpr uint32_t NW = FZ_MB / WAVE;
+    for (uint32_t r = tid; r < rows; r += FZ_MB) {
+      const uint32_t len = s_off[r + 1] - s_off[r];
+      if (len <= WAVE / 2) s_short[atomicAdd(&s_ncls[0], 1u)] = r;
+      else s_long[atomicAdd(&s_ncls[1], 1u)] = r;
+    }
+    __syncthreads();
+    const uint32_t nshort = s_ncls[0], nlong = s_ncls[1];
+    for (uint32_t pr = wv; 2 * pr < nshort; pr += NW) {
+      const uint32_t h = l / (WAVE / 2), el = l % (WAVE / 2);
+      const bool has = 2 * pr + h < nshort;
+      const uint32_t r = has ? s_short[2 * pr + h] : 0u;
+      const uint32_t so = has ? s_off[r] : 0u;
+      const uint32_t len = has ? s_off[r + 1] - so : 0u;
+      uint64_t* seg = srt + so;
+      const uint64_t v = wave_sort32x2(el < len ? seg[el] : ~0ull);
+      if (el < len) seg[el] = v;
+      fz_wave_sync();
       bool keep = false;
       uint64_t outk = 0;
-      if (e < len) {
-        const uint32_t col = fz_col(ws[e]);
-        if (e == 0 || fz_col(ws[e - 1]) != col) {
+      if (el < len) {
+        const uint32_t col = fz_col(seg[el]);
+        if (el == 0 || fz_col(seg[el - 1]) != col) {
           float acc = 0.0f;
-          for (uint32_t t = e; t < len && fz_col(ws[t]) == col; ++t) {
-            const float x = fz_w(ws[t]);
+          for (uint32_t t = el; t < len && fz_col(seg[t]) == col; ++t) {
+            const float x = fz_w(seg[t]);
             acc = acc + x - acc * x;
           }
           keep = acc > 1e-5f;
           outk = ((uint64_t)col << 32) | __float_as_uint(acc);
         }
       }
-      const uint64_t mask = __ballot(keep);
-      if (keep) comb[coff + kept + (uint32_t)__popcll(mask & ((1ull << l) - 1))] = outk;
-      kept += (uint32_t)__popcll(mask);
+      const uint32_t hm = (uint32_t)(__ballot(keep) >> (h * (WAVE / 2)));
+      fz_wave_sync();
+      if (keep) comb[cbase + so + (uint32_t)__popc(hm & ((1u << el) - 1u))] = outk;
+      if (el == 0 && has) olen[row0 + r] = (uint32_t)__popc(hm);
+      fz_wave_sync();
     }
-    if (l == 0) olen[row] = kept;
-    fz_wave_sync();
-  }
+    for (uint32_t li = wv; li < nlong; li += NW) {
+      const uint32_t r = s_long[li];
+      const uint32_t row = row0 + r;
+      const uint32_t so = s_off[r], len = s_off[r + 1] - so;
+      uint64_t* seg = srt + so;
+      const uint64_t coff = cbase + so;
+      if (len > 4 * WAVE) {
+        if (srt != comb + cbase)
+          for (uint32_t q = l; q < len; q += WAVE) comb[coff + q] = seg[q];
+        if (l == 0) big[atomicAdd(nbig, 1u)] = row;
+        continue;
+      }
+      if (len <= WAVE) {
+        const uint64_t v = wave_sort64((uint32_t)l < len ? seg[l] : ~0ull);
+        if ((uint32_t)l < len) seg[l] = v;
+      } else {
+        uint64_t v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t e = q * WAVE + l;
+          v[q] = e < len ? seg[e] : ~0ull;
+        }
+        wave_sort_n<4>(v);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if ((uint32_t)(q * WAVE + l) < len) seg[q * WAVE + l] = v[q];
+      }
+      fz_wave_sync();
+      uint32_t kept = 0;
+      for (uint32_t base = 0; base < len; base += WAVE) {
+        const uint32_t e = base + l;
+        bool keep = false;
+        uint64_t outk = 0;
+        if (e < len) {
+          const uint32_t col = fz_col(seg[e]);
+          if (e == 0 || fz_col(seg[e - 1]) != col) {
+            float acc = 0.0f;
+            for (uint32_t t = e; t < len && fz_col(seg[t]) == col; ++t) {
+              const float x = fz_w(seg[t]);
+              acc = acc + x - acc * x;
+            }
+            keep = acc > 1e-5f;
+            outk = ((uint64_t)col << 32) | __float_as_uint(acc);
+          }
+        }
+        const uint64_t mask = __ballot(keep);
+        fz_wave_sync();
+        if (keep) comb[coff + kept + (uint32_t)__popcll(mask & ((1ull << l) - 1))] = outk;
+        kept += (uint32_t)__popcll(mask);
+        fz_wave_sync();
+      }
+      if (l == 0) olen[row] = kept;
+    }
+  };
+  if (total <= (uint32_t)FZ_BCAP) rest(s_srt);
+  else rest(comb + cbase);
 }
 
 // Rows of 257 .. FZ_MID_CAP combined entries: one block per row (looping over

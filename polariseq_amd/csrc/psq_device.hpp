@@ -78,6 +78,23 @@ __device__ inline uint64_t wave_sort64(uint64_t v) {
   }
   return v;
 }
+// Two independent bitonic sorts of 32: lanes 0-31 and lanes 32-63, each
+// ascending by lane.
+__device__ inline uint64_t wave_sort32x2(uint64_t v) {
+  const int l = lane_id() & (WAVE / 2 - 1);
+#pragma unroll
+  for (int size = 2; size <= WAVE / 2; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      uint64_t o = shfl_xor64(v, stride);
+      bool up = ((l & size) == 0);
+      bool lower = ((l & stride) == 0);
+      uint64_t mn = v < o ? v : o, mx = v < o ? o : v;
+      v = (up == lower) ? mn : mx;
+    }
+  }
+  return v;
+}
 // v holds an ascending-by-lane bitonic sequence's first half property: sort a
 // bitonic sequence of 64 ascending.
 __device__ inline uint64_t wave_bitonic_merge64(uint64_t v) {

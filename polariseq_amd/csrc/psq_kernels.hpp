@@ -153,26 +153,35 @@ constexpr int FZ_KR = 16;          // row length held in registers by k_fuzzy_st
 constexpr int FZ_BLOCK = 256;      // threads of the block-per-row kernels
 constexpr int FZ_MID_CAP = 8192;   // longest combined row sorted in LDS
 constexpr int FZ_EMIT_LANES = 16;  // lanes per row in k_fuzzy_emit
-constexpr int FZ_SCATTER_LANES = 16;  // lanes per row in k_fz_bucket_scatter
-constexpr int FZ_TILE_ROWS = 4096;     // rows per block of k_fz_bucket_count / _scatter
-constexpr int FZ_NB_MAX = 16384;       // most buckets (LDS histogram of those kernels)
-constexpr int FZ_BLOG_MIN = 9;         // at least 512 rows per bucket
-constexpr int FZ_BROWS_LDS = 4096;     // bucket row counters kept in LDS up to this many rows
-constexpr int FZ_BCAP = 10240;         // bucket entries sorted in LDS
+constexpr int FZ_C1_CHUNK = 65536;     // edges per block of k_fz_l1_count
+constexpr int FZ_P1_CHUNK = 4096;      // edges per block of k_fz_l1_scatter
+constexpr int FZ_P2_CHUNK = 8192;      // entries per chunk of the level-2 placement
+constexpr int FZ_BUCKET_LOG = 8;       // 256 rows per bucket
+constexpr int FZ_BUCKET_ROWS = 1 << FZ_BUCKET_LOG;
+constexpr int FZ_SB_MAX = 64;          // most super-buckets
+constexpr int FZ_BPS_MAX = 4096;       // most buckets per super-bucket
+constexpr int FZ_L2_SHARES = 64;       // blocks per super-bucket in level 2
+constexpr int FZ_BCAP = 9216;          // bucket entries counting-sorted in LDS
 constexpr int FZ_MB = 1024;            // threads of k_fz_bucket_merge
 __global__ void k_fuzzy_check(const uint32_t* indptr, uint32_t n, const uint32_t* indices, uint64_t m,
                               uint32_t* bad);
-__global__ void k_fuzzy_strengths(const uint32_t* indptr, const float* dists, uint32_t n, float* w);
-__global__ void k_fz_bucket_count(const uint32_t* indptr, const uint32_t* indices, uint32_t n,
-                                  uint32_t shift, uint32_t nb, uint32_t* btot);
-__global__ void k_fz_bucket_scatter(const uint32_t* indptr, const uint32_t* indices, const float* w,
-                                    uint32_t n, uint32_t shift, uint32_t nb, const uint32_t* bstart,
-                                    uint32_t* bfill, uint64_t* pay, uint32_t* pj);
-__global__ void k_fz_bucket_merge(const uint32_t* indptr, const uint32_t* indices, const float* w,
-                                  uint32_t n, uint32_t shift, uint32_t nb, const uint32_t* bstart,
-                                  const uint64_t* pay, const uint32_t* pj, uint64_t* srt_g,
-                                  uint32_t* cnt_g, uint32_t* t_ptr, uint64_t* comb, uint32_t* olen,
-                                  uint32_t* big, uint32_t* nbig);
+__global__ void k_fuzzy_strengths(const uint32_t* indptr, const float* dists, uint32_t n,
+                                  uint64_t* pay0);
+__global__ void k_fz_l1_count(const uint32_t* indices, uint64_t m, uint32_t sshift, uint32_t ns,
+                              uint32_t* stot);
+__global__ void k_fz_l1_scatter(const uint64_t* pay0, const uint32_t* indices, uint64_t m,
+                                uint32_t sshift, uint32_t ns, const uint32_t* sstart,
+                                uint32_t* sfill, uint64_t* pay1, uint32_t* pj1);
+__global__ void k_fz_l2_count(const uint32_t* sstart, const uint32_t* pj1, uint32_t sshift,
+                              uint32_t* btot);
+__global__ void k_fz_l2_scatter(const uint32_t* sstart, const uint64_t* pay1, const uint32_t* pj1,
+                                uint32_t sshift, const uint32_t* bstart, uint32_t* bfill,
+                                uint64_t* pay2, uint8_t* pl2);
+__global__ void k_fz_bucket_merge(const uint32_t* indptr, const uint32_t* indices,
+                                  const uint64_t* pay0, uint32_t n, uint32_t nb,
+                                  const uint32_t* bstart, const uint64_t* pay2, const uint8_t* pl2,
+                                  uint32_t* t_ptr, uint64_t* comb, uint32_t* olen, uint32_t* big,
+                                  uint32_t* nbig);
 __global__ void k_fuzzy_merge_mid(const uint32_t* indptr, const uint32_t* t_ptr, uint64_t* comb,
                                   uint32_t* olen, const uint32_t* big, const uint32_t* nbig,
                                   uint32_t* giant, uint32_t* ngiant);
