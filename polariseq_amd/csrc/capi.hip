@@ -16,6 +16,7 @@
 
 #include "../../include/psq_knn.h"
 #include "psq_kernels.hpp"
+#include "psq_partition.hpp"
 
 using namespace psq;
 
@@ -173,6 +174,77 @@ int scan_u32(Workspace* ws, const uint32_t* in, uint32_t* out, uint32_t n, hipSt
 }
 
 uint32_t round_up(uint32_t v, uint32_t m) { return (v + m - 1) / m * m; }
+
+// ── two-level partition (psq_partition.hpp) ──
+struct PartBufs {
+  uint32_t *stot, *sstart, *sfill, *btot, *bstart, *bfill, *pj1;
+  uint64_t *pay1, *pay2;
+  uint16_t* pl2;
+};
+
+int part_bufs(Workspace* ws, const std::string& pre, const PartShape& P, uint64_t m, PartBufs* B) {
+  const uint64_t mm = std::max<uint64_t>(m, 1);
+  TRY(wbuf(ws, (pre + "_stot").c_str(), (size_t)P.ns + 1, &B->stot));
+  TRY(wbuf(ws, (pre + "_sstart").c_str(), (size_t)P.ns + 1, &B->sstart));
+  TRY(wbuf(ws, (pre + "_sfill").c_str(), P.ns, &B->sfill));
+  TRY(wbuf(ws, (pre + "_btot").c_str(), (size_t)P.nb + 1, &B->btot));
+  TRY(wbuf(ws, (pre + "_bstart").c_str(), (size_t)P.nb + 1, &B->bstart));
+  TRY(wbuf(ws, (pre + "_bfill").c_str(), P.nb, &B->bfill));
+  TRY(wbuf(ws, (pre + "_pay1").c_str(), mm, &B->pay1));
+  TRY(wbuf(ws, (pre + "_pj1").c_str(), mm, &B->pj1));
+  TRY(wbuf(ws, (pre + "_pay2").c_str(), mm, &B->pay2));
+  TRY(wbuf(ws, (pre + "_pl2").c_str(), mm, &B->pl2));
+  return PSQ_OK;
+}
+
+int part_clear(const PartShape& P, PartBufs& B, hipStream_t s) {
+  TRY(fill_u32(B.stot, (size_t)P.ns + 1, 0, s));
+  TRY(fill_u32(B.sfill, P.ns, 0, s));
+  TRY(fill_u32(B.btot, (size_t)P.nb + 1, 0, s));
+  TRY(fill_u32(B.bfill, P.nb, 0, s));
+  return PSQ_OK;
+}
+
+// Level 2: super-bucket regions (pay1, pj1 from level 1, sstart) -> bucket
+// regions (pay2, pl2) with bstart.
+int part_level2(Workspace* ws, const PartShape& P, PartBufs& B, hipStream_t s) {
+  hipLaunchKernelGGL(k_part_l2_count, dim3(P.ns, PART_L2_SHARES), dim3(1024), 0, s, B.sstart, B.pj1,
+                     P.sshift, P.blog, B.btot);
+  HIPC(hipGetLastError());
+  TRY(scan_u32(ws, B.btot, B.bstart, P.nb, s));
+  const size_t l2_smem = (size_t)PART_P2_CHUNK * (sizeof(uint64_t) + sizeof(uint32_t));
+  HIPC(hipFuncSetAttribute((const void*)k_part_l2_scatter,
+                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)l2_smem));
+  hipLaunchKernelGGL(k_part_l2_scatter, dim3(P.ns, PART_L2_SHARES), dim3(1024), l2_smem, s, B.sstart,
+                     B.pay1, B.pj1, P.sshift, P.blog, B.bstart, B.bfill, B.pay2, B.pl2);
+  HIPC(hipGetLastError());
+  return PSQ_OK;
+}
+
+// Reverse lists of the rows G (n rows of k) grouped by neighbour: rev, roff.
+int build_reverse(Workspace* ws, const PartShape& P, PartBufs& B, const uint64_t* G, uint32_t n,
+                  uint32_t k, uint64_t seed, uint64_t iter, uint32_t* roff, uint64_t* rev,
+                  hipStream_t s) {
+  const uint64_t nk = (uint64_t)n * k;
+  TRY(part_clear(P, B, s));
+  hipLaunchKernelGGL(k_rv_l1_count, dim3((unsigned)((nk + PART_C1_CHUNK - 1) / PART_C1_CHUNK)),
+                     dim3(1024), 0, s, G, nk, P.sshift, P.ns, B.stot);
+  HIPC(hipGetLastError());
+  TRY(scan_u32(ws, B.stot, B.sstart, P.ns, s));
+  hipLaunchKernelGGL(k_rv_l1_scatter, dim3((unsigned)((nk + PART_P1_CHUNK - 1) / PART_P1_CHUNK)),
+                     dim3(1024), 0, s, G, nk, k, seed, iter, P.sshift, P.ns, B.sstart, B.sfill,
+                     B.pay1, B.pj1);
+  HIPC(hipGetLastError());
+  TRY(part_level2(ws, P, B, s));
+  const size_t smem = (2 * ((size_t)1 << P.blog) + 1) * sizeof(uint32_t);
+  if (smem > 65536)
+    HIPC(hipFuncSetAttribute((const void*)k_rv_bucket_sort,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+  hipLaunchKernelGGL(k_rv_bucket_sort, dim3(P.nb), dim3(1024), smem, s, B.bstart, B.pay2, B.pl2, n,
+                     P.blog, P.nb, roff, rev);
+  HIPC(hipGetLastError());
+  return PSQ_OK;
+}
 
 // BF16 join: a pair passes the screen when its approximate distance is below an
 // endpoint's threshold times this factor (the approximation error is a few parts
@@ -541,14 +613,13 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   uint64_t *G, *slots, *rev;
   float* thr;
   uint8_t* touched;
-  uint32_t *rcount, *roff, *newl, *oldl, *ncnt, *rowcnt;
+  uint32_t *roff, *newl, *oldl, *ncnt, *rowcnt;
   unsigned long long* ctr;
   TRY(wbuf(ws, "Xp", (size_t)n * R.dp, &Xp));
   TRY(wbuf(ws, "G", (size_t)n * k, &G));
   TRY(wbuf(ws, "slots", (size_t)n * S, &slots));
   TRY(wbuf(ws, "thr", n, &thr));
   TRY(wbuf(ws, "touched", n, &touched));
-  TRY(wbuf(ws, "rcount", n, &rcount));
   TRY(wbuf(ws, "roff", (size_t)n + 1, &roff));
   TRY(wbuf(ws, "rev", (size_t)n * k, &rev));
   TRY(wbuf(ws, "newl", (size_t)n * m, &newl));
@@ -556,6 +627,9 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   TRY(wbuf(ws, "ncnt", n, &ncnt));
   TRY(wbuf(ws, "rowcnt", n, &rowcnt));
   TRY(wbuf(ws, "counters", (size_t)(CTR_STRIPES + 1) * CTR_WORDS, &ctr));
+  const PartShape RP = part_shape(n, 8);
+  PartBufs RB;
+  TRY(part_bufs(ws, "rv", RP, (uint64_t)n * k, &RB));
   void* hp;
   TRY(ws->host_counters(&hp));
   unsigned long long* hctr = static_cast<unsigned long long*>(hp);
@@ -574,7 +648,6 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   TRY(fill_u64(G, (size_t)n * k, EMPTY_KEY, s));
   TRY(fill_u64(slots, (size_t)n * S, EMPTY_KEY, s));
   TRY(fill_u8(touched, n, 0, s));
-  TRY(fill_u32(rcount, n, 0, s));
   TRY(fill_u64(reinterpret_cast<uint64_t*>(ctr), (size_t)(CTR_STRIPES + 1) * CTR_WORDS, 0, s));
   {
     float inf = __builtin_inff();
@@ -631,13 +704,7 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   uint64_t iters = 0;
   for (uint64_t it = 0; it < R.n_iters; ++it) {
     TRY(fill_u64(reinterpret_cast<uint64_t*>(ctr), (size_t)CTR_STRIPES * CTR_WORDS, 0, s));
-    hipLaunchKernelGGL(k_rev_count, dim3(grid_for((size_t)n * k, 256)), dim3(256), 0, s, G,
-                       (size_t)n * k, rcount);
-    HIPC(hipGetLastError());
-    TRY(scan_u32(ws, rcount, roff, n, s));
-    hipLaunchKernelGGL(k_rev_scatter, dim3(grid_for((size_t)n * k, 256)), dim3(256), 0, s, G, n, k,
-                       R.seed, it, roff, rcount, rev);
-    HIPC(hipGetLastError());
+    TRY(build_reverse(ws, RP, RB, G, n, k, R.seed, it, roff, rev, s));
     hipLaunchKernelGGL(k_sample_select, dim3(n), dim3(64), 0, s, G, n, k, m, R.seed,
                        it, roff, rev, newl, oldl, ncnt, slots, S, thr,
                        (bf16 && it < BF16_WARM_ITERS) ? (uint32_t)R.ko : k);
@@ -781,31 +848,19 @@ int fuzzy_core(Workspace* ws, const uint32_t* d_ip, const uint32_t* d_ix, const 
   R.n = n;
   uint32_t *h;
   TRY(ws->host_counters((void**)&h));
-  // Super-buckets of 2^sshift rows, at most FZ_SB_MAX of them; buckets of
-  // FZ_BUCKET_ROWS rows.
-  uint32_t sshift = FZ_BUCKET_LOG;
-  while ((((uint64_t)n + (1ull << sshift) - 1) >> sshift) > (uint64_t)FZ_SB_MAX) ++sshift;
-  if (sshift - FZ_BUCKET_LOG > 12)
+  // Super-buckets of at most PART_SB_MAX; buckets of FZ_BUCKET_ROWS rows.
+  const PartShape P = part_shape(n, FZ_BUCKET_LOG);
+  if (P.blog != FZ_BUCKET_LOG)
     return fail(PSQ_E_UNSUPPORTED, "n_obs = %u exceeds the fuzzy path's limit of 2^26 rows", n);
-  const uint32_t ns = (uint32_t)(((uint64_t)n + (1ull << sshift) - 1) >> sshift);
-  const uint32_t nb = (uint32_t)(((uint64_t)n + FZ_BUCKET_ROWS - 1) >> FZ_BUCKET_LOG);
-  uint32_t *flags, *olen, *big, *giant, *stot, *sstart, *sfill, *btot, *bstart, *bfill, *pj1;
+  const uint32_t nb = P.nb;
+  uint32_t *flags, *olen, *big, *giant;
   unsigned long long* tot;
-  uint64_t *pay0, *pay1, *pay2;
-  uint8_t* pl2;
+  uint64_t* pay0;
+  PartBufs B;
   TRY(wbuf(ws, "fz_flags", 4, &flags));  // bad, nbig, ngiant, pad
   TRY(wbuf(ws, "fz_tot", 1, &tot));
   TRY(wbuf(ws, "fz_pay0", std::max<uint64_t>(m, 1), &pay0));
-  TRY(wbuf(ws, "fz_stot", (size_t)ns + 1, &stot));
-  TRY(wbuf(ws, "fz_sstart", (size_t)ns + 1, &sstart));
-  TRY(wbuf(ws, "fz_sfill", ns, &sfill));
-  TRY(wbuf(ws, "fz_btot", (size_t)nb + 1, &btot));
-  TRY(wbuf(ws, "fz_bstart", (size_t)nb + 1, &bstart));
-  TRY(wbuf(ws, "fz_bfill", nb, &bfill));
-  TRY(wbuf(ws, "fz_pay1", std::max<uint64_t>(m, 1), &pay1));
-  TRY(wbuf(ws, "fz_pj1", std::max<uint64_t>(m, 1), &pj1));
-  TRY(wbuf(ws, "fz_pay2", std::max<uint64_t>(m, 1), &pay2));
-  TRY(wbuf(ws, "fz_pl2", std::max<uint64_t>(m, 1), &pl2));
+  TRY(part_bufs(ws, "fz", P, m, &B));
   TRY(wbuf(ws, "fz_tptr", (size_t)n + 1, &R.t_ptr));
   TRY(wbuf(ws, "fz_comb", std::max<uint64_t>(2 * m, 1), &R.comb));
   TRY(wbuf(ws, "fz_olen", n, &olen));
@@ -826,36 +881,24 @@ int fuzzy_core(Workspace* ws, const uint32_t* d_ip, const uint32_t* d_ix, const 
   hipLaunchKernelGGL(k_fuzzy_strengths, dim3(grid_for(n, 256)), dim3(256), 0, s, d_ip, d_dx, n, pay0);
   HIPC(hipGetLastError());
   // Phase 2: transpose by bucket (umap.rs:357-381), two partition levels.
-  const unsigned chunks1 = std::max<unsigned>(1, (unsigned)((m + FZ_P1_CHUNK - 1) / FZ_P1_CHUNK));
-  TRY(fill_u32(stot, (size_t)ns + 1, 0, s));
-  TRY(fill_u32(sfill, ns, 0, s));
-  TRY(fill_u32(btot, (size_t)nb + 1, 0, s));
-  TRY(fill_u32(bfill, nb, 0, s));
-  hipLaunchKernelGGL(k_fz_l1_count, dim3(std::max<unsigned>(1, (unsigned)((m + FZ_C1_CHUNK - 1) / FZ_C1_CHUNK))),
-                     dim3(1024), 0, s, d_ix, m, sshift, ns, stot);
+  TRY(part_clear(P, B, s));
+  hipLaunchKernelGGL(k_fz_l1_count, dim3(std::max<unsigned>(1, (unsigned)((m + PART_C1_CHUNK - 1) / PART_C1_CHUNK))),
+                     dim3(1024), 0, s, d_ix, m, P.sshift, P.ns, B.stot);
   HIPC(hipGetLastError());
-  TRY(scan_u32(ws, stot, sstart, ns, s));
+  TRY(scan_u32(ws, B.stot, B.sstart, P.ns, s));
   if (m > 0) {
-    hipLaunchKernelGGL(k_fz_l1_scatter, dim3(chunks1), dim3(1024), 0, s, pay0, d_ix, m, sshift, ns,
-                       sstart, sfill, pay1, pj1);
+    hipLaunchKernelGGL(k_fz_l1_scatter, dim3((unsigned)((m + PART_P1_CHUNK - 1) / PART_P1_CHUNK)),
+                       dim3(1024), 0, s, pay0, d_ix, m, P.sshift, P.ns, B.sstart, B.sfill, B.pay1,
+                       B.pj1);
     HIPC(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_fz_l2_count, dim3(ns, FZ_L2_SHARES), dim3(1024), 0, s, sstart, pj1, sshift,
-                     btot);
-  HIPC(hipGetLastError());
-  TRY(scan_u32(ws, btot, bstart, nb, s));
-  const size_t l2_smem = (size_t)FZ_P2_CHUNK * (sizeof(uint64_t) + sizeof(uint32_t));
-  HIPC(hipFuncSetAttribute((const void*)k_fz_l2_scatter, hipFuncAttributeMaxDynamicSharedMemorySize,
-                           (int)l2_smem));
-  hipLaunchKernelGGL(k_fz_l2_scatter, dim3(ns, FZ_L2_SHARES), dim3(1024), l2_smem, s, sstart, pay1,
-                     pj1, sshift, bstart, bfill, pay2, pl2);
-  HIPC(hipGetLastError());
+  TRY(part_level2(ws, P, B, s));
   // Phase 3: per-row sort, fuzzy union and compaction (umap.rs:488-537), by row size.
   const size_t bm_smem = (size_t)FZ_BCAP * sizeof(uint64_t);
   HIPC(hipFuncSetAttribute((const void*)k_fz_bucket_merge, hipFuncAttributeMaxDynamicSharedMemorySize,
                            (int)bm_smem));
-  hipLaunchKernelGGL(k_fz_bucket_merge, dim3(nb), dim3(FZ_MB), bm_smem, s, d_ip, d_ix, pay0, n, nb, bstart,
-                     pay2, pl2, R.t_ptr, R.comb, olen, big, flags + 1);
+  hipLaunchKernelGGL(k_fz_bucket_merge, dim3(nb), dim3(FZ_MB), bm_smem, s, d_ip, d_ix, pay0, n, nb,
+                     B.bstart, B.pay2, B.pl2, R.t_ptr, R.comb, olen, big, flags + 1);
   HIPC(hipGetLastError());
   const size_t mid_smem = (size_t)FZ_MID_CAP * sizeof(uint64_t);
   HIPC(hipFuncSetAttribute((const void*)k_fuzzy_merge_mid, hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -13,6 +13,7 @@
 // them in. Weights are >= +0 or NaN, so the sign bit carries no information
 // (a NaN weight folds to NaN and is dropped either way).
 #include "psq_kernels.hpp"
+#include "psq_partition.hpp"
 
 namespace psq {
 
@@ -201,186 +202,49 @@ __global__ __launch_bounds__(256) void k_fuzzy_strengths(const uint32_t* __restr
 }
 
 // ── transpose by bucket (umap.rs:357-381) ──
-// Rows are grouped in buckets of FZ_BUCKET_ROWS consecutive rows, and buckets
-// in super-buckets of 2^sshift rows (at most FZ_SB_MAX of them). Every edge
-// (i -> j, w) becomes the transposed entry (column i, tag 1, w) of row j in two
-// partition passes, each counting in LDS and reserving its output runs with one
-// global atomic per (block, part):
-//   level 1 (k_fz_l1_*): CSR tiles -> super-bucket regions (pay1, full j in pj1);
-//   level 2 (k_fz_l2_*): super-bucket regions -> bucket regions (pay2, pl2 =
-//   j's row within its bucket).
-// k_fz_bucket_merge then counting-sorts one bucket's own and transposed entries
-// by row in LDS and merges every row of the bucket.
+// Every edge (i -> j, w) becomes the transposed entry (column i, tag 1, w) of
+// row j through the two staged partitions of psq_partition.hpp: level 1 here
+// (edges -> super-buckets), level 2 by k_part_l2_* (-> buckets of
+// FZ_BUCKET_ROWS rows). k_fz_bucket_merge then counting-sorts one bucket's own
+// and transposed entries by row in LDS and merges every row of the bucket.
 
-// Level 1 count: edges per target super-bucket, per chunk of FZ_C1_CHUNK edges.
+// Level-1 count: edges per target super-bucket.
 __global__ __launch_bounds__(1024) void k_fz_l1_count(const uint32_t* __restrict__ indices,
                                                       uint64_t m, uint32_t sshift, uint32_t ns,
                                                       uint32_t* __restrict__ stot) {
-  __shared__ uint32_t h[FZ_SB_MAX];
+  __shared__ uint32_t h[PART_SB_MAX];
   const uint32_t tid = threadIdx.x;
   if (tid < ns) h[tid] = 0;
   __syncthreads();
-  const uint64_t e0 = (uint64_t)blockIdx.x * FZ_C1_CHUNK;
-  const uint64_t e1 = min(m, e0 + FZ_C1_CHUNK);
+  const uint64_t e0 = (uint64_t)blockIdx.x * PART_C1_CHUNK;
+  const uint64_t e1 = min(m, e0 + PART_C1_CHUNK);
   for (uint64_t e = e0 + tid; e < e1; e += blockDim.x) atomicAdd(&h[indices[e] >> sshift], 1u);
   __syncthreads();
   if (tid < ns && h[tid]) atomicAdd(&stot[tid], h[tid]);
 }
 
-namespace {
-
-// Exclusive scan in place of v[0, L), L <= 4 * blockDim.x, by the whole
-// block (each thread four consecutive values); v[L] = total.
-__device__ inline void fz_block_scan_small(uint32_t* v, uint32_t L, uint32_t* s_part) {
-  const uint32_t tid = threadIdx.x, l = tid & (WAVE - 1), wv = tid / WAVE;
-  const uint32_t nwv = blockDim.x / WAVE;
-  uint32_t x[4], sum = 0;
-#pragma unroll
-  for (uint32_t k = 0; k < 4; ++k) {
-    x[k] = 4 * tid + k < L ? v[4 * tid + k] : 0u;
-    sum += x[k];
-  }
-  uint32_t inc = sum;
-#pragma unroll
-  for (int o = 1; o < WAVE; o <<= 1) {
-    const uint32_t y = __shfl_up(inc, o, WAVE);
-    if (l >= (uint32_t)o) inc += y;
-  }
-  if (l == WAVE - 1) s_part[wv] = inc;
-  __syncthreads();
-  if (wv == 0) {
-    uint32_t t = l < nwv ? s_part[l] : 0u;
-#pragma unroll
-    for (int o = 1; o < WAVE; o <<= 1) {
-      const uint32_t y = __shfl_up(t, o, WAVE);
-      if (l >= (uint32_t)o) t += y;
-    }
-    if (l < nwv) s_part[l] = t;
-  }
-  __syncthreads();
-  uint32_t run = (wv > 0 ? s_part[wv - 1] : 0u) + inc - sum;
-#pragma unroll
-  for (uint32_t k = 0; k < 4; ++k)
-    if (4 * tid + k < L) {
-      v[4 * tid + k] = run;
-      run += x[k];
-    }
-  if (tid == 0) v[L] = s_part[nwv - 1];
-  __syncthreads();
-}
-
-// One staged partition of up to NPT * blockDim.x entries (key[t], tgt[t]) into
-// parts part(tgt) < np: counted in LDS, reserved in global memory with one
-// atomic per part (base = start[part0 + p] + atomicAdd(fill[part0 + p], count)),
-// sorted by part in LDS, then written out so that consecutive threads write
-// consecutive addresses of one part's run. out_j(tgt) is what the second
-// output array receives.
-template <int NPT, typename PartF, typename OutJ, typename JT>
-__device__ inline void fz_stage_partition(const uint64_t* key, const uint32_t* tgt, uint32_t cnt,
-                                          uint32_t np, uint32_t part0, PartF part,
-                                          const uint32_t* __restrict__ start,
-                                          uint32_t* __restrict__ fill, uint64_t* __restrict__ out_key,
-                                          JT* __restrict__ out_j, OutJ out_jv, uint64_t* s_key,
-                                          uint32_t* s_tgt, uint32_t* s_cnt, uint32_t* s_base,
-                                          uint32_t* s_part) {
-  const uint32_t tid = threadIdx.x;
-  for (uint32_t p = tid; p < np; p += blockDim.x) s_cnt[p] = 0;
-  __syncthreads();
-  uint64_t k[NPT];
-  uint32_t t[NPT], rank[NPT];
-#pragma unroll
-  for (int q = 0; q < NPT; ++q) {
-    const uint32_t e = q * blockDim.x + tid;
-    if (e < cnt) {
-      k[q] = key[e];
-      t[q] = tgt[e];
-      rank[q] = atomicAdd(&s_cnt[part(t[q])], 1u);
-    }
-  }
-  __syncthreads();
-  for (uint32_t p = tid; p < np; p += blockDim.x) {
-    const uint32_t c = s_cnt[p];
-    s_base[p] = c ? start[part0 + p] + atomicAdd(&fill[part0 + p], c) : 0u;
-  }
-  fz_block_scan_small(s_cnt, np, s_part);  // s_cnt becomes the LDS offsets
-#pragma unroll
-  for (int q = 0; q < NPT; ++q) {
-    const uint32_t e = q * blockDim.x + tid;
-    if (e < cnt) {
-      const uint32_t pos = s_cnt[part(t[q])] + rank[q];
-      s_key[pos] = k[q];
-      s_tgt[pos] = t[q];
-    }
-  }
-  __syncthreads();
-  for (uint32_t e = tid; e < cnt; e += blockDim.x) {
-    const uint32_t tg = s_tgt[e], p = part(tg);
-    const uint32_t pos = s_base[p] + (e - s_cnt[p]);
-    out_key[pos] = s_key[e];
-    out_j[pos] = out_jv(tg);
-  }
-  __syncthreads();
-}
-
-}  // namespace
-
-// Level 1 placement, per chunk of FZ_P1_CHUNK edges: pay1 = the transposed
-// entry key, pj1 = the full target row j, in super-bucket order.
+// Level-1 placement: pay1 = the transposed entry key, pj1 = the full target
+// row j, in super-bucket order.
 __global__ __launch_bounds__(1024) void k_fz_l1_scatter(
     const uint64_t* __restrict__ pay0, const uint32_t* __restrict__ indices, uint64_t m,
     uint32_t sshift, uint32_t ns, const uint32_t* __restrict__ sstart, uint32_t* __restrict__ sfill,
     uint64_t* __restrict__ pay1, uint32_t* __restrict__ pj1) {
-  __shared__ uint64_t s_key[FZ_P1_CHUNK];
-  __shared__ uint32_t s_tgt[FZ_P1_CHUNK];
-  __shared__ uint32_t s_cnt[FZ_SB_MAX + 1], s_base[FZ_SB_MAX], s_part[1024 / WAVE];
-  const uint64_t e0 = (uint64_t)blockIdx.x * FZ_P1_CHUNK;
-  const uint32_t cnt = (uint32_t)min((uint64_t)FZ_P1_CHUNK, m - e0);
-  fz_stage_partition<FZ_P1_CHUNK / 1024>(
-      pay0 + e0, indices + e0, cnt, ns, 0u, [&](uint32_t j) { return j >> sshift; }, sstart, sfill,
-      pay1, pj1, [](uint32_t j) { return j; }, s_key, s_tgt, s_cnt, s_base, s_part);
+  __shared__ uint64_t s_key[PART_P1_CHUNK];
+  __shared__ uint32_t s_tgt[PART_P1_CHUNK];
+  __shared__ uint32_t s_cnt[PART_SB_MAX + 1], s_base[PART_SB_MAX], s_part[1024 / WAVE];
+  const uint64_t e0 = (uint64_t)blockIdx.x * PART_P1_CHUNK;
+  const uint32_t cnt = (uint32_t)min((uint64_t)PART_P1_CHUNK, m - e0);
+  const uint64_t* kin = pay0 + e0;
+  const uint32_t* tin = indices + e0;
+  stage_partition<PART_P1_CHUNK / 1024>(
+      [&](uint32_t e, uint64_t& kk, uint32_t& tt) {
+        kk = kin[e];
+        tt = tin[e];
+        return true;
+      },
+      cnt, ns, 0u, [&](uint32_t j) { return j >> sshift; }, sstart, sfill, pay1, pj1,
+      [](uint32_t j) { return j; }, s_key, s_tgt, s_cnt, s_base, s_part);
 }
-
-// Level 2 count: block (s, c) takes the c-th of gridDim.y strided shares of
-// super-bucket s's region; buckets of s are local ids 0 .. 2^(sshift - 8).
-__global__ __launch_bounds__(1024) void k_fz_l2_count(const uint32_t* __restrict__ sstart,
-                                                      const uint32_t* __restrict__ pj1,
-                                                      uint32_t sshift, uint32_t* __restrict__ btot) {
-  __shared__ uint32_t h[FZ_BPS_MAX];
-  const uint32_t tid = threadIdx.x, sb = blockIdx.x;
-  const uint32_t nbk = 1u << (sshift - FZ_BUCKET_LOG), bk0 = sb * nbk;
-  for (uint32_t b = tid; b < nbk; b += blockDim.x) h[b] = 0;
-  __syncthreads();
-  const uint32_t q0 = sstart[sb], q1 = sstart[sb + 1];
-  const uint32_t stride = gridDim.y * FZ_P2_CHUNK;
-  for (uint32_t c0 = q0 + blockIdx.y * FZ_P2_CHUNK; c0 < q1; c0 += stride)
-    for (uint32_t q = c0 + tid; q < min(q1, c0 + FZ_P2_CHUNK); q += blockDim.x)
-      atomicAdd(&h[(pj1[q] >> FZ_BUCKET_LOG) - bk0], 1u);
-  __syncthreads();
-  for (uint32_t b = tid; b < nbk; b += blockDim.x)
-    if (h[b]) atomicAdd(&btot[bk0 + b], h[b]);
-}
-
-// Level 2 placement, same shares, chunk by chunk: pay2 = the entry key, pl2 =
-// j's row within its bucket, in bucket order.
-__global__ __launch_bounds__(1024) void k_fz_l2_scatter(
-    const uint32_t* __restrict__ sstart, const uint64_t* __restrict__ pay1,
-    const uint32_t* __restrict__ pj1, uint32_t sshift, const uint32_t* __restrict__ bstart,
-    uint32_t* __restrict__ bfill, uint64_t* __restrict__ pay2, uint8_t* __restrict__ pl2) {
-  extern __shared__ uint64_t s_key[];  // FZ_P2_CHUNK
-  uint32_t* s_tgt = reinterpret_cast<uint32_t*>(s_key + FZ_P2_CHUNK);
-  __shared__ uint32_t s_cnt[FZ_BPS_MAX + 1], s_base[FZ_BPS_MAX], s_part[1024 / WAVE];
-  const uint32_t sb = blockIdx.x;
-  const uint32_t nbk = 1u << (sshift - FZ_BUCKET_LOG), bk0 = sb * nbk;
-  const uint32_t q0 = sstart[sb], q1 = sstart[sb + 1];
-  const uint32_t stride = gridDim.y * FZ_P2_CHUNK;
-  for (uint32_t c0 = q0 + blockIdx.y * FZ_P2_CHUNK; c0 < q1; c0 += stride)
-    fz_stage_partition<FZ_P2_CHUNK / 1024>(
-        pay1 + c0, pj1 + c0, min((uint32_t)FZ_P2_CHUNK, q1 - c0), nbk, bk0,
-        [&](uint32_t j) { return (j >> FZ_BUCKET_LOG) - bk0; }, bstart, bfill, pay2, pl2,
-        [](uint32_t j) { return (uint8_t)(j & (FZ_BUCKET_ROWS - 1)); }, s_key, s_tgt, s_cnt,
-        s_base, s_part);
-}
-
 
 // One block per bucket of FZ_BUCKET_ROWS rows: the bucket's own entries (a
 // contiguous CSR range) and its transposed entries are counting-sorted by row
@@ -393,7 +257,7 @@ __global__ __launch_bounds__(1024) void k_fz_l2_scatter(
 __global__ __launch_bounds__(FZ_MB, 2) void k_fz_bucket_merge(
     const uint32_t* __restrict__ indptr, const uint32_t* __restrict__ indices,
     const uint64_t* __restrict__ pay0, uint32_t n, uint32_t nb, const uint32_t* __restrict__ bstart,
-    const uint64_t* __restrict__ pay2, const uint8_t* __restrict__ pl2,
+    const uint64_t* __restrict__ pay2, const uint16_t* __restrict__ pl2,
     uint32_t* __restrict__ t_ptr, uint64_t* __restrict__ comb, uint32_t* __restrict__ olen,
     uint32_t* __restrict__ big, uint32_t* __restrict__ nbig) {
   extern __shared__ uint64_t lds[];
@@ -414,7 +278,7 @@ __global__ __launch_bounds__(FZ_MB, 2) void k_fz_bucket_merge(
   __syncthreads();
   for (uint32_t t = tid; t < c; t += FZ_MB) atomicAdd(&s_off[pl2[p0 + t]], 1u);
   __syncthreads();
-  fz_block_scan_small(s_off, rows, s_part);
+  block_scan4(s_off, rows, s_part);
   for (uint32_t r = tid; r < rows; r += FZ_MB) {
     const uint32_t own_r = s_ip[r + 1] - s_ip[r];
     s_cur[r] = s_off[r] + own_r;

@@ -156,31 +156,6 @@ __global__ void k_random_fill(const float* __restrict__ Xp, uint32_t n, uint32_t
 }
 
 // ── sampling (nndescent.rs:853-909) ──
-__global__ void k_rev_count(const uint64_t* __restrict__ G, size_t nk, uint32_t* __restrict__ rcount) {
-  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nk) return;
-  uint64_t key = G[e];
-  if (key != EMPTY_KEY) atomicAdd(&rcount[key_idx(key)], 1u);
-}
-
-// Reverse proposals grouped by destination. The order inside a group depends on
-// atomic arrival and is irrelevant: the consumer is a set function.
-// Entry: (class << 63) | (prio24 << 32) | src, class 0 = new, 1 = old.
-__global__ void k_rev_scatter(const uint64_t* __restrict__ G, uint32_t n, uint32_t k, uint64_t seed,
-                              uint64_t iter, const uint32_t* __restrict__ roff,
-                              uint32_t* __restrict__ rcount, uint64_t* __restrict__ rev) {
-  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)n * k) return;
-  uint64_t key = G[e];
-  if (key == EMPTY_KEY) return;
-  uint32_t i = (uint32_t)(e / k);
-  uint32_t j = key_idx(key);
-  uint64_t cls = (key & 1) ? 0ull : 1ull;
-  uint64_t pr = hash_prio24(seed, iter, e);
-  uint32_t pos = roff[j] + atomicSub(&rcount[j], 1u) - 1u;
-  rev[pos] = (cls << 63) | (pr << 32) | (uint64_t)i;
-}
-
 // One wave (one block) per row i. Candidates: the row's own entries (forward,
 // priority of slot i*k+s) and the reverse entries rev[roff[i]..roff[i+1]).
 // Per flag class, a neighbour seen twice keeps its smaller priority (a forward

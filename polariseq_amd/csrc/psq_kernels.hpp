@@ -17,9 +17,6 @@ __global__ void k_scan_add(uint32_t* out, uint32_t n, const uint32_t* sums);
 __global__ void k_random_fill(const float* Xp, uint32_t n, uint32_t d, uint32_t dp, uint64_t* G,
                               uint32_t k, uint64_t seed, uint8_t* touched);
 __global__ void k_ctr_reduce(const unsigned long long* ctr, unsigned long long* out);
-__global__ void k_rev_count(const uint64_t* G, size_t nk, uint32_t* rcount);
-__global__ void k_rev_scatter(const uint64_t* G, uint32_t n, uint32_t k, uint64_t seed,
-                              uint64_t iter, const uint32_t* roff, uint32_t* rcount, uint64_t* rev);
 __global__ void k_sample_select(uint64_t* G, uint32_t n, uint32_t k, uint32_t m, uint64_t seed,
                                 uint64_t iter, const uint32_t* roff, const uint64_t* rev,
                                 uint32_t* newl, uint32_t* oldl, uint32_t* ncnt, uint64_t* slots,
@@ -148,19 +145,28 @@ __global__ void k_bexact_pass2(const float* Xp, uint32_t d, uint32_t dp, const u
                                const float* T, uint32_t* counter, uint32_t* cnt, uint32_t* cand,
                                uint32_t cap);
 
+// Level-2 partition and reverse lists (partition.hip; layout in psq_partition.hpp)
+__global__ void k_part_l2_count(const uint32_t* sstart, const uint32_t* pj1, uint32_t sshift,
+                                uint32_t blog, uint32_t* btot);
+__global__ void k_part_l2_scatter(const uint32_t* sstart, const uint64_t* pay1, const uint32_t* pj1,
+                                  uint32_t sshift, uint32_t blog, const uint32_t* bstart,
+                                  uint32_t* bfill, uint64_t* pay2, uint16_t* pl2);
+__global__ void k_rv_l1_count(const uint64_t* G, uint64_t nk, uint32_t sshift, uint32_t ns,
+                              uint32_t* stot);
+__global__ void k_rv_l1_scatter(const uint64_t* G, uint64_t nk, uint32_t k, uint64_t seed,
+                                uint64_t iter, uint32_t sshift, uint32_t ns, const uint32_t* sstart,
+                                uint32_t* sfill, uint64_t* pay1, uint32_t* pj1);
+__global__ void k_rv_bucket_sort(const uint32_t* bstart, const uint64_t* pay2, const uint16_t* pl2,
+                                 uint32_t n, uint32_t blog, uint32_t nb, uint32_t* roff,
+                                 uint64_t* rev);
+
 // Fuzzy connectivities (fuzzy_kernels.hip)
 constexpr int FZ_KR = 16;          // row length held in registers by k_fuzzy_strengths
 constexpr int FZ_BLOCK = 256;      // threads of the block-per-row kernels
 constexpr int FZ_MID_CAP = 8192;   // longest combined row sorted in LDS
 constexpr int FZ_EMIT_LANES = 16;  // lanes per row in k_fuzzy_emit
-constexpr int FZ_C1_CHUNK = 65536;     // edges per block of k_fz_l1_count
-constexpr int FZ_P1_CHUNK = 4096;      // edges per block of k_fz_l1_scatter
-constexpr int FZ_P2_CHUNK = 8192;      // entries per chunk of the level-2 placement
 constexpr int FZ_BUCKET_LOG = 8;       // 256 rows per bucket
 constexpr int FZ_BUCKET_ROWS = 1 << FZ_BUCKET_LOG;
-constexpr int FZ_SB_MAX = 64;          // most super-buckets
-constexpr int FZ_BPS_MAX = 4096;       // most buckets per super-bucket
-constexpr int FZ_L2_SHARES = 64;       // blocks per super-bucket in level 2
 constexpr int FZ_BCAP = 9216;          // bucket entries counting-sorted in LDS
 constexpr int FZ_MB = 1024;            // threads of k_fz_bucket_merge
 __global__ void k_fuzzy_check(const uint32_t* indptr, uint32_t n, const uint32_t* indices, uint64_t m,
@@ -172,14 +178,9 @@ __global__ void k_fz_l1_count(const uint32_t* indices, uint64_t m, uint32_t sshi
 __global__ void k_fz_l1_scatter(const uint64_t* pay0, const uint32_t* indices, uint64_t m,
                                 uint32_t sshift, uint32_t ns, const uint32_t* sstart,
                                 uint32_t* sfill, uint64_t* pay1, uint32_t* pj1);
-__global__ void k_fz_l2_count(const uint32_t* sstart, const uint32_t* pj1, uint32_t sshift,
-                              uint32_t* btot);
-__global__ void k_fz_l2_scatter(const uint32_t* sstart, const uint64_t* pay1, const uint32_t* pj1,
-                                uint32_t sshift, const uint32_t* bstart, uint32_t* bfill,
-                                uint64_t* pay2, uint8_t* pl2);
 __global__ void k_fz_bucket_merge(const uint32_t* indptr, const uint32_t* indices,
                                   const uint64_t* pay0, uint32_t n, uint32_t nb,
-                                  const uint32_t* bstart, const uint64_t* pay2, const uint8_t* pl2,
+                                  const uint32_t* bstart, const uint64_t* pay2, const uint16_t* pl2,
                                   uint32_t* t_ptr, uint64_t* comb, uint32_t* olen, uint32_t* big,
                                   uint32_t* nbig);
 __global__ void k_fuzzy_merge_mid(const uint32_t* indptr, const uint32_t* t_ptr, uint64_t* comb,
