@@ -28,13 +28,26 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr uint32_t SV_CAP = 320;  // screened pairs held: < 64 between tiles + 256 per tile
 
 // 8 centred dimensions [t0, t0 + 8) of an LDS row, rounded to BF16 (0 past d).
-__device__ inline bf16x8 centred_bf16x8(const float* row, const float* xi, uint32_t t0, uint32_t d) {
+// t0 is a multiple of 8 and rows are 16-byte aligned: two 16-byte reads per
+// operand, so the 16 rows of a tile, 52 floats apart, fill all 64 banks.
+// Reads stop at the row's padded stride rs (>= the last dimension read).
+__device__ inline bf16x8 centred_bf16x8(const float* row, const float* xi, uint32_t t0, uint32_t d,
+                                        uint32_t rs) {
+  float r[8], x[8];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const uint32_t t = t0 + 4 * h;
+    float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
+    if (t < rs) {
+      a = *reinterpret_cast<const float4*>(row + t);
+      b = *reinterpret_cast<const float4*>(xi + t);
+    }
+    r[4 * h] = a.x; r[4 * h + 1] = a.y; r[4 * h + 2] = a.z; r[4 * h + 3] = a.w;
+    x[4 * h] = b.x; x[4 * h + 1] = b.y; x[4 * h + 2] = b.z; x[4 * h + 3] = b.w;
+  }
   bf16x8 v;
 #pragma unroll
-  for (int u = 0; u < 8; ++u) {
-    const uint32_t t = t0 + u;
-    v[u] = (__bf16)(t < d ? row[t] - xi[t] : 0.0f);
-  }
+  for (int u = 0; u < 8; ++u) v[u] = (__bf16)(t0 + u < d ? r[u] - x[u] : 0.0f);
   return v;
 }
 
@@ -83,9 +96,16 @@ __global__ __launch_bounds__(64) void k_join_bf16(
   for (uint32_t s = l; s < nc; s += WAVE) {
     const float* row = rows + (size_t)s * RS;
     float nrm = 0.0f;
-    for (uint32_t t = 0; t < d; ++t) {
-      const float b = (float)(__bf16)(row[t] - xi[t]);
-      nrm = fmaf(b, b, nrm);
+    for (uint32_t t = 0; t < d; t += 4) {
+      const float4 a = *reinterpret_cast<const float4*>(row + t);
+      const float4 x = *reinterpret_cast<const float4*>(xi + t);
+      const float av[4] = {a.x, a.y, a.z, a.w}, xv[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (t + u < d) {
+          const float b = (float)(__bf16)(av[u] - xv[u]);
+          nrm = fmaf(b, b, nrm);
+        }
     }
     s_nrm[s] = nrm;
   }
@@ -133,15 +153,15 @@ __global__ __launch_bounds__(64) void k_join_bf16(
   const uint32_t lr = l % 16, lk = (l / 16) * 8;
   for (uint32_t rt = 0; rt < rtiles; ++rt) {
     const float* arow = rows + (size_t)(rt * 16 + lr) * RS;
-    const bf16x8 a0 = centred_bf16x8(arow, xi, lk, d);
-    const bf16x8 a1 = centred_bf16x8(arow, xi, 32 + lk, d);
+    const bf16x8 a0 = centred_bf16x8(arow, xi, lk, d, RS);
+    const bf16x8 a1 = centred_bf16x8(arow, xi, 32 + lk, d, RS);
     for (uint32_t ct = 0; ct < ctiles; ++ct) {
       // skip tiles entirely below the new x new diagonal
       if (ct * 16 + 15 < rt * 16 && ct * 16 + 15 < nn) continue;
       const float* brow = rows + (size_t)(ct * 16 + lr) * RS;
       f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, centred_bf16x8(brow, xi, lk, d), acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, centred_bf16x8(brow, xi, 32 + lk, d), acc, 0,
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, centred_bf16x8(brow, xi, lk, d, RS), acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, centred_bf16x8(brow, xi, 32 + lk, d, RS), acc, 0,
                                                     0, 0);
       const uint32_t c = ct * 16 + lr;
 #pragma unroll

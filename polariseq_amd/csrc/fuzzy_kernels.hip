@@ -108,11 +108,6 @@ __device__ inline uint32_t fz_fold_block(const uint64_t* s, uint32_t len, uint64
   return cnt;
 }
 
-__device__ inline void fz_wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // One bitonic compare-exchange step over pairs t of a P-long sequence.
 __device__ inline void fz_bitonic_pair(uint64_t* s, uint32_t t, uint32_t size, uint32_t stride) {
@@ -322,7 +317,7 @@ __global__ __launch_bounds__(FZ_MB, 2) void k_fz_bucket_merge(
       uint64_t* seg = srt + so;
       const uint64_t v = wave_sort32x2(el < len ? seg[el] : ~0ull);
       if (el < len) seg[el] = v;
-      fz_wave_sync();
+      wave_sync();
       bool keep = false;
       uint64_t outk = 0;
       if (el < len) {
@@ -338,10 +333,10 @@ __global__ __launch_bounds__(FZ_MB, 2) void k_fz_bucket_merge(
         }
       }
       const uint32_t hm = (uint32_t)(__ballot(keep) >> (h * (WAVE / 2)));
-      fz_wave_sync();
+      wave_sync();
       if (keep) comb[cbase + so + (uint32_t)__popc(hm & ((1u << el) - 1u))] = outk;
       if (el == 0 && has) olen[row0 + r] = (uint32_t)__popc(hm);
-      fz_wave_sync();
+      wave_sync();
     }
     for (uint32_t li = wv; li < nlong; li += NW) {
       const uint32_t r = s_long[li];
@@ -370,7 +365,7 @@ __global__ __launch_bounds__(FZ_MB, 2) void k_fz_bucket_merge(
         for (int q = 0; q < 4; ++q)
           if ((uint32_t)(q * WAVE + l) < len) seg[q * WAVE + l] = v[q];
       }
-      fz_wave_sync();
+      wave_sync();
       uint32_t kept = 0;
       for (uint32_t base = 0; base < len; base += WAVE) {
         const uint32_t e = base + l;
@@ -389,10 +384,10 @@ __global__ __launch_bounds__(FZ_MB, 2) void k_fz_bucket_merge(
           }
         }
         const uint64_t mask = __ballot(keep);
-        fz_wave_sync();
+        wave_sync();
         if (keep) comb[coff + kept + (uint32_t)__popcll(mask & ((1ull << l) - 1))] = outk;
         kept += (uint32_t)__popcll(mask);
-        fz_wave_sync();
+        wave_sync();
       }
       if (l == 0) olen[row] = kept;
     }

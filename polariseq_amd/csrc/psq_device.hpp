@@ -352,6 +352,13 @@ __host__ __device__ inline void pair_decode(uint32_t t, uint32_t nn, uint32_t no
 
 // Proposals a wave finds are appended to an LDS list and sent 64 at a time,
 // one per lane, so a batch's bucket reads and atomics are in flight together.
+// Orders one wave's LDS accesses when the other waves of its block are not
+// synchronised with it: a wave executes its LDS operations in order, so waiting
+// for the outstanding ones (lgkmcnt) makes every earlier LDS write visible to
+// every lane's later reads; the memory clobber keeps the compiler from moving
+// accesses across. Global memory operations in flight are not waited for.
+__device__ inline void wave_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
 // The join kernels run one wave per block, so __syncthreads() is a wave barrier
 // and every call below is made by the whole wave. Capacity: < 64 entries are
 // held between calls and one append adds at most 128.
