@@ -48,10 +48,24 @@ __device__ inline uint32_t key_idx(uint64_t key) { return ((uint32_t)key) >> 1; 
 __device__ inline float key_dist(uint64_t key) { return __uint_as_float((uint32_t)(key >> 32)); }
 
 // ── wave primitives (64 lanes) ──
+// Lane l receives lane l ^ m. For the masks of the sorting networks (constant
+// after unrolling) the exchange avoids the LDS crossbar's addressed permute:
+// xor 1 and 2 are DPP quad permutations, xor 8 a DPP rotation by 8 within each
+// row of 16 lanes, xor 4 and 16 ds_swizzle in bit mode (and 0x1f, xor m) within
+// each half of 32 lanes; other masks use ds_bpermute.
+__device__ inline uint32_t xor_lane32(uint32_t v, int m) {
+  const int x = (int)v;
+  switch (m) {
+    case 1: return (uint32_t)__builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);
+    case 2: return (uint32_t)__builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);
+    case 4: return (uint32_t)__builtin_amdgcn_ds_swizzle(x, 0x101F);
+    case 8: return (uint32_t)__builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);
+    case 16: return (uint32_t)__builtin_amdgcn_ds_swizzle(x, 0x401F);
+    default: return (uint32_t)__shfl_xor(x, m, WAVE);
+  }
+}
 __device__ inline uint64_t shfl_xor64(uint64_t v, int m) {
-  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
-  lo = __shfl_xor(lo, m, WAVE);
-  hi = __shfl_xor(hi, m, WAVE);
+  const uint32_t lo = xor_lane32((uint32_t)v, m), hi = xor_lane32((uint32_t)(v >> 32), m);
   return ((uint64_t)hi << 32) | lo;
 }
 __device__ inline uint64_t shfl64(uint64_t v, int src) {
