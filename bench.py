@@ -6,8 +6,9 @@ Default workload: 4.06M cells x 50 PCs FP32, 150-component Gaussian mixture,
 k = 15, seed 0 (polariseq_amd.datasets.mixture). Two forms are timed: plain (the
 drop-in: FP32 join, 48-tree forest, the reference's defaults) and learned+BF16,
 the headline, at the operating point --learned-trees / --learned-heap-k /
---learned-delta (6 trees on the learned codes, rows of 24 while iterating,
-early stop at 0.012 n h changes), chosen for throughput at recall@15 >= 0.95 and
+--learned-delta / --learned-slots (6 trees on the learned codes, rows of 24 while
+iterating, early stop at 0.012 n h changes, 64 proposal slots per row), chosen for
+throughput at recall@15 >= 0.95 and
 reported with the measured recall. After the timed steps:
   * recall@15 of the graph against the exact top-15 over all cells, for 100k
     random query cells (exact kNN on the GPU, FP64-ranked);
@@ -83,6 +84,8 @@ def main() -> int:
                     help="entries per row while iterating, learned form (the API default is 2k)")
     ap.add_argument("--learned-delta", type=float, default=0.012,
                     help="early-stop fraction of the learned form (the API default is 0.001)")
+    ap.add_argument("--learned-slots", type=int, default=64,
+                    help="proposal slots per row, learned form (0 = the API default, 128 here)")
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--train-subset", type=int, default=200_000)
     ap.add_argument("--recall-queries", type=int, default=100_000)
@@ -150,7 +153,8 @@ def main() -> int:
                                               precision="bf16")
         return lambda: P.nndescent_device(xt, k, seed=args.seed, init="learned", model=enc,
                                           precision="bf16", n_trees=args.learned_trees,
-                                          heap_k=args.learned_heap_k, delta=args.learned_delta)
+                                          heap_k=args.learned_heap_k, delta=args.learned_delta,
+                                          slots=args.learned_slots)
 
     results = {}
     for form in forms:
@@ -253,6 +257,7 @@ def main() -> int:
             "n_trees": s_["n_trees"], "heap_k": s_["heap_k"]}
         if form == "learned_bf16":
             forms_out[form]["delta"] = args.learned_delta
+            forms_out[form]["slots"] = s_["slots"]
     desc = {"plain": "plain GPU NN-Descent (FP32 join, RP forest start)",
             "bf16": "BF16 MFMA join + FP32 re-rank, RP forest start",
             "learned_bf16": "learned seeding + BF16 MFMA join with FP32 re-rank"}[head]
