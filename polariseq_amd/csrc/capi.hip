@@ -251,6 +251,7 @@ int build_reverse(Workspace* ws, const PartShape& P, PartBufs& B, const uint64_t
 // per thousand on centred vectors); passing pairs are re-ranked exactly.
 constexpr float BF16_SLACK = 1.03f;
 constexpr uint32_t BF16_SV_CAP = 320;  // screened-pair list of k_join_bf16
+constexpr uint64_t EX_BIG_QMIN = 256;   // smallest query chunk of the large-k exact path
 // BF16 form: for its first iterations the join's threshold is a row's k-th
 // entry (the output size), not its heap_k-th; the rows still keep heap_k
 // entries. Early rows are poor, and the wider threshold mostly buys proposals
@@ -960,6 +961,99 @@ int set_device(int32_t device) {
 
 }  // namespace
 
+// Exact kNN for 64 < k <= EX_KBIG_MAX (Scrublet's k_adj): pass 1 with 64 per
+// segment and at least k/16 segments, a radix-select tau per query, pass 2 into
+// a candidate buffer of 2k + 256, an LDS sort per query; queries in chunks that
+// bound the pass-1 and candidate buffers.
+int knn_exact_big(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d_queries,
+                  uint64_t nq, uint64_t k, uint32_t* d_idx, float* d_dist, hipStream_t s) {
+  Workspace* ws;
+  TRY(workspace(&ws));
+  const uint32_t dp = round_up((uint32_t)d, 32);
+  uint32_t nseg = (uint32_t)std::min<uint64_t>((n + 63) / 64, std::min<uint64_t>(256, (k + 15) / 16 * 2));
+  nseg = std::max<uint32_t>(nseg, 1);
+  const uint32_t seg_len = round_up((uint32_t)((n + nseg - 1) / nseg), 64);
+  nseg = (uint32_t)((n + seg_len - 1) / seg_len);
+  const uint32_t cap = round_up((uint32_t)(2 * k + 256), 64);
+  uint32_t pmax = 1;
+  while (pmax < cap) pmax <<= 1;
+  // chunk of queries: pass-1 lists and candidates within about 4 GiB
+  const uint64_t per_q = (uint64_t)nseg * 64 * 4 + (uint64_t)cap * 4 + 16;
+  const uint32_t qc = (uint32_t)std::max<uint64_t>(
+      EX_BIG_QMIN, std::min<uint64_t>(nq, (4ull << 30) / per_q / 256 * 256));
+  float *Xp, *partial, *T;
+  uint32_t *cnt, *cand, *ovf, *novf, *qidx = nullptr;
+  TRY(wbuf(ws, "Xp", (size_t)n * dp, &Xp));
+  TRY(wbuf(ws, "exb_partial", (size_t)qc * nseg * 64, &partial));
+  TRY(wbuf(ws, "ex_T", qc, &T));
+  TRY(wbuf(ws, "ex_cnt", qc, &cnt));
+  TRY(wbuf(ws, "exb_cand", (size_t)qc * cap, &cand));
+  TRY(wbuf(ws, "ex_ovf", qc, &ovf));
+  TRY(wbuf(ws, "ex_novf", 1, &novf));
+  if (!d_queries) TRY(wbuf(ws, "exb_q", qc, &qidx));
+  hipLaunchKernelGGL(k_pad_rows, dim3(std::min<unsigned>(grid_for((size_t)n * dp, 256), 65536)),
+                     dim3(256), 0, s, d_data, (uint32_t)n, (uint32_t)d, dp, Xp);
+  HIPC(hipGetLastError());
+  const size_t smem = (size_t)64 * dp * sizeof(float);
+  const bool d50 = d == 50;
+  void (*p1)(const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t, uint32_t,
+             uint32_t, float*) = d50 ? k_exact_pass1<50, 64> : k_exact_pass1<0, 64>;
+  void (*p2)(const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t, uint32_t,
+             const float*, uint32_t*, uint32_t*, uint32_t) = d50 ? k_exact_pass2<50> : k_exact_pass2<0>;
+  if (smem > 65536) {
+    HIPC(hipFuncSetAttribute((const void*)p1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    HIPC(hipFuncSetAttribute((const void*)p2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+  }
+  const size_t tau_smem = (size_t)nseg * 64 * 4;
+  if (tau_smem > 65536)
+    HIPC(hipFuncSetAttribute((const void*)k_exact_tau_big, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)tau_smem));
+  const size_t rr_smem = (size_t)pmax * 12;
+  if (rr_smem > 65536)
+    HIPC(hipFuncSetAttribute((const void*)k_exact_rerank_big,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)rr_smem));
+  for (uint64_t c0 = 0; c0 < nq; c0 += qc) {
+    const uint32_t qn = (uint32_t)std::min<uint64_t>(qc, nq - c0);
+    const uint32_t* qp = d_queries ? d_queries + c0 : qidx;
+    if (!d_queries) {
+      hipLaunchKernelGGL(k_iota_u32, dim3(grid_for(qn, 256)), dim3(256), 0, s, qidx, (uint32_t)c0, qn);
+      HIPC(hipGetLastError());
+    }
+    const uint32_t qblocks = (qn + 255) / 256;
+    hipLaunchKernelGGL(p1, dim3(qblocks, nseg), dim3(256), smem, s, Xp, (uint32_t)n, (uint32_t)d, dp,
+                       qp, qn, seg_len, nseg, partial);
+    HIPC(hipGetLastError());
+    hipLaunchKernelGGL(k_exact_tau_big, dim3(qn), dim3(256), tau_smem, s, partial, qn, nseg,
+                       (uint32_t)k, (uint32_t)d, T);
+    HIPC(hipGetLastError());
+    TRY(fill_u32(cnt, qn, 0, s));
+    TRY(fill_u32(novf, 1, 0, s));
+    hipLaunchKernelGGL(p2, dim3(qblocks, nseg), dim3(256), smem, s, Xp, (uint32_t)n, (uint32_t)d, dp,
+                       qp, qn, seg_len, T, cnt, cand, cap);
+    HIPC(hipGetLastError());
+    uint32_t* io = d_idx + c0 * k;
+    float* dq = d_dist + c0 * k;
+    hipLaunchKernelGGL(k_exact_rerank_big, dim3(qn), dim3(256), rr_smem, s, Xp, (uint32_t)n,
+                       (uint32_t)d, dp, qp, qn, (uint32_t)k, cnt, cand, cap, pmax, io, dq, ovf, novf);
+    HIPC(hipGetLastError());
+    uint32_t h_novf = 0;
+    HIPC(hipMemcpyAsync(&h_novf, novf, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    if (h_novf > 0) {
+      double* sd;
+      uint32_t* si;
+      TRY(wbuf(ws, "ex_fb_d", (size_t)h_novf * k, &sd));
+      TRY(wbuf(ws, "ex_fb_i", (size_t)h_novf * k, &si));
+      hipLaunchKernelGGL(k_exact_fallback, dim3(grid_for(h_novf, 64)), dim3(64), 0, s, Xp,
+                         (uint32_t)n, (uint32_t)d, dp, qp, (uint32_t)k, T, ovf, h_novf, sd, si, io, dq,
+                         nullptr, 0u);
+      HIPC(hipGetLastError());
+      HIPC(hipStreamSynchronize(s));
+    }
+  }
+  return PSQ_OK;
+}
+
 int psq_host::knn_exact_device_impl(const float* d_data, uint64_t n, uint64_t d,
                                     const uint32_t* d_queries, uint64_t nq, uint64_t k,
                                     uint32_t* d_idx, float* d_dist, hipStream_t s) {
@@ -967,11 +1061,13 @@ int psq_host::knn_exact_device_impl(const float* d_data, uint64_t n, uint64_t d,
                                     (unsigned long long)n, (unsigned long long)d);
   if (k == 0) return fail(PSQ_E_INVALID, "k must be >= 1");
   if (!d_queries && nq != n) return fail(PSQ_E_INVALID, "queries is NULL but n_q != n");
-  if (k > 64) return fail(PSQ_E_UNSUPPORTED, "k > 64 is not supported by the exact GPU path");
+  if (k > EX_KBIG_MAX)
+    return fail(PSQ_E_UNSUPPORTED, "k > %u is not supported by the exact GPU path", EX_KBIG_MAX);
   if (d > 256) return fail(PSQ_E_UNSUPPORTED, "n_dim > 256 is not supported on the GPU path");
   if (n >= (1ull << 31) || nq >= (1ull << 31)) return fail(PSQ_E_UNSUPPORTED, "n must be < 2^31");
   if (!d_data || !d_idx || !d_dist) return fail(PSQ_E_INVALID, "NULL pointer argument");
   if (nq == 0) return PSQ_OK;
+  if (k > 64) return knn_exact_big(d_data, n, d, d_queries, nq, k, d_idx, d_dist, s);
   Workspace* ws;
   TRY(workspace(&ws));
   const uint32_t dp = round_up((uint32_t)d, 32);

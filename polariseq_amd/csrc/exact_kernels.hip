@@ -332,6 +332,127 @@ __global__ void k_exact_fallback(const float* __restrict__ Xp, uint32_t n, uint3
   }
 }
 
+// ── large k (64 < k <= EX_KBIG_MAX; Scrublet's k_adj, doublets.rs:1083-1090) ──
+// Pass 1 keeps each segment's 64 smallest FP32 distances as above, with enough
+// segments that a query's true top k rarely crowd one segment. The k-th
+// smallest of the union of the segments' lists is never below the true k-th
+// smallest FP32 distance (the union is a subset of all distances), so the
+// threshold built from it keeps the proof of pass 2. The selection and the
+// re-rank run per query in LDS.
+
+// tau: the k-th smallest of a query's nseg * 64 pass-1 values, by radix select
+// on the FP32 bits (four 8-bit digits; the values are >= 0, so bit order is
+// value order). One block per query; dynamic LDS: nseg * 64 values.
+__global__ __launch_bounds__(256) void k_exact_tau_big(const float* __restrict__ partial,
+                                                       uint32_t nq, uint32_t nseg, uint32_t k,
+                                                       uint32_t d, float* __restrict__ T) {
+  extern __shared__ uint32_t sv[];
+  __shared__ uint32_t hist[256];
+  __shared__ uint32_t s_prefix, s_mask, s_rank;
+  const uint32_t q = blockIdx.x, tid = threadIdx.x;
+  const uint32_t L = nseg * 64;
+  const float* src = partial + (size_t)q * L;
+  for (uint32_t e = tid; e < L; e += blockDim.x) sv[e] = __float_as_uint(src[e]);
+  if (tid == 0) {
+    s_prefix = 0;
+    s_mask = 0;
+    s_rank = k;  // 1-based rank still to find under the current prefix
+  }
+  __syncthreads();
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    hist[tid] = 0;
+    __syncthreads();
+    const uint32_t prefix = s_prefix, mask = s_mask;
+    for (uint32_t e = tid; e < L; e += blockDim.x) {
+      const uint32_t v = sv[e];
+      if ((v & mask) == prefix) atomicAdd(&hist[(v >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t r = s_rank, b = 0;
+      for (; b < 255u && hist[b] < r; ++b) r -= hist[b];
+      s_rank = r;
+      s_prefix = prefix | (b << shift);
+      s_mask = mask | (255u << shift);
+    }
+    __syncthreads();
+  }
+  if (tid != 0) return;
+  const float tau = __uint_as_float(s_prefix);
+  if (!(tau < __builtin_inff())) {
+    T[q] = __builtin_inff();
+    return;
+  }
+  double F = 1.0 + 4.0 * ((double)d + 8.0) * 5.9604644775390625e-08;
+  double Td = (double)tau * F + 1e-30;
+  float Tf = (float)Td;
+  if ((double)Tf < Td) Tf = nextafterf(Tf, __builtin_inff());
+  T[q] = Tf;
+}
+
+// One block per query: the FP64 distance of every candidate, a bitonic sort of
+// (d64 bits, index) in LDS, the first k written. Dynamic LDS: P u64 + P u32,
+// P = cap rounded up to a power of two.
+__global__ __launch_bounds__(256) void k_exact_rerank_big(
+    const float* __restrict__ Xp, uint32_t n, uint32_t d, uint32_t dp,
+    const uint32_t* __restrict__ queries, uint32_t nq, uint32_t k, const uint32_t* __restrict__ cnt,
+    const uint32_t* __restrict__ cand, uint32_t cap, uint32_t pmax, uint32_t* __restrict__ idx_out,
+    float* __restrict__ dist_out, uint32_t* __restrict__ overflow,
+    uint32_t* __restrict__ n_overflow) {
+  extern __shared__ uint64_t s_d[];
+  uint32_t* s_i = reinterpret_cast<uint32_t*>(s_d + pmax);
+  const uint32_t qs = blockIdx.x, tid = threadIdx.x;
+  const uint32_t c = cnt[qs];
+  if (c > cap) {
+    if (tid == 0) overflow[atomicAdd(n_overflow, 1u)] = qs;
+    return;
+  }
+  const uint32_t qi = queries ? queries[qs] : qs;
+  const float* qrow = Xp + (size_t)qi * dp;
+  uint32_t P = 1;
+  while (P < c) P <<= 1;
+  for (uint32_t e = tid; e < P; e += blockDim.x) {
+    if (e < c) {
+      const uint32_t i = cand[(size_t)qs * cap + e];
+      s_d[e] = (uint64_t)__double_as_longlong(dist64(qrow, Xp + (size_t)i * dp, (int)d));
+      s_i[e] = i;
+    } else {
+      s_d[e] = ~0ull;
+      s_i[e] = EMPTY;
+    }
+  }
+  __syncthreads();
+  for (uint32_t size = 2; size <= P; size <<= 1)
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      for (uint32_t t = tid; t < P / 2; t += blockDim.x) {
+        const uint32_t lo = 2 * t - (t & (stride - 1)), hi = lo + stride;
+        const bool up = (lo & size) == 0;
+        const uint64_t da = s_d[lo], db = s_d[hi];
+        const uint32_t ia = s_i[lo], ib = s_i[hi];
+        const bool gt = da > db || (da == db && ia > ib);
+        if (gt == up) {
+          s_d[lo] = db;
+          s_d[hi] = da;
+          s_i[lo] = ib;
+          s_i[hi] = ia;
+        }
+      }
+      __syncthreads();
+    }
+  for (uint32_t pos = tid; pos < k; pos += blockDim.x) {
+    const bool has = pos < c;
+    idx_out[(size_t)qs * k + pos] = has ? s_i[pos] : EMPTY;
+    dist_out[(size_t)qs * k + pos] =
+        has ? (float)__dsqrt_rn(__longlong_as_double((long long)s_d[pos])) : __builtin_inff();
+  }
+}
+
+// q[i] = q0 + i
+__global__ void k_iota_u32(uint32_t* __restrict__ q, uint32_t q0, uint32_t count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) q[i] = q0 + i;
+}
+
 template __global__ void k_exact_pass1<0, 16>(const float*, uint32_t, uint32_t, uint32_t,
                                               const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
 template __global__ void k_exact_pass1<0, 32>(const float*, uint32_t, uint32_t, uint32_t,

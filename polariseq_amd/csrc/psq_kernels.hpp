@@ -126,6 +126,15 @@ __global__ void k_exact_rerank(const float* Xp, uint32_t n, uint32_t d, uint32_t
                                const uint32_t* cnt, const uint32_t* cand, uint32_t cap,
                                uint32_t* idx_out, float* dist_out, uint32_t* overflow,
                                uint32_t* n_overflow);
+constexpr uint32_t EX_KBIG_MAX = 2048;  // largest k of the exact path
+__global__ void k_exact_tau_big(const float* partial, uint32_t nq, uint32_t nseg, uint32_t k,
+                                uint32_t d, float* T);
+__global__ void k_exact_rerank_big(const float* Xp, uint32_t n, uint32_t d, uint32_t dp,
+                                   const uint32_t* queries, uint32_t nq, uint32_t k,
+                                   const uint32_t* cnt, const uint32_t* cand, uint32_t cap,
+                                   uint32_t pmax, uint32_t* idx_out, float* dist_out,
+                                   uint32_t* overflow, uint32_t* n_overflow);
+__global__ void k_iota_u32(uint32_t* q, uint32_t q0, uint32_t count);
 __global__ void k_exact_fallback(const float* Xp, uint32_t n, uint32_t d, uint32_t dp,
                                  const uint32_t* queries, uint32_t k, const float* T,
                                  const uint32_t* overflow, uint32_t n_overflow, double* sd,

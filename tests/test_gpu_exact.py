@@ -1,7 +1,9 @@
 """GPU parity of the exact tiled kNN: indices bit-exact against the FP64 oracle
 (order = distance then index, self excluded; core/doublets.rs:544-651), distances
 within 1e-5 relative. Covers the reference KATs, duplicate points (exact ties),
-rows with fewer than k others, the d = 50 register path and the generic path."""
+rows with fewer than k others, the d = 50 register path and the generic path, and
+the large-k path (64 < k <= 2048) that Scrublet's k_adj needs
+(core/doublets.rs:1083-1090: round(0.5 sqrt(n)) * (1 + n_sim / n))."""
 
 from __future__ import annotations
 
@@ -107,3 +109,41 @@ def test_100k_subset_of_the_1p3m_workload(orc):
     ei, ed = orc.knn_exact64(xs, 15, rows)
     assert np.array_equal(idx[rows], ei)
     assert np.allclose(dist[rows], np.sqrt(ed), rtol=RTOL, atol=0)
+
+
+# ── large k (64 < k <= 2048) ──
+
+@pytest.mark.parametrize("n,d,k", [(3000, 50, 65), (3000, 50, 474), (2500, 20, 1161),
+                                   (2300, 50, 2048), (900, 7, 300)])
+def test_large_k_bit_exact(orc, n, d, k):
+    rng = np.random.default_rng(n + k)
+    x = ds.mixture(n, d, 6, seed=k) if d >= 8 else rng.random((n, d), dtype=np.float32)
+    for i in range(0, n, 89):  # exact duplicates
+        x[(i + 1) % n] = x[i]
+    q = np.arange(0, n, 7, dtype=np.uint32)
+    print(f"exact large k n={n} d={d} k={k}", flush=True)
+    _check(x, k, q, orc)
+
+
+def test_large_k_scrublet_shape_all_rows(orc):
+    """Scrublet's exact search: cells plus twice as many simulated doublets
+    (means of two random cells), k_adj = round(0.5 sqrt(n)) * 3, every row."""
+    rng = np.random.default_rng(5)
+    cells = ds.mixture(3000, 30, 8, seed=5)
+    a, b = rng.integers(0, 3000, (2, 6000))
+    dbl = ((cells[a] + cells[b]) / 2).astype(np.float32)
+    x = np.concatenate([cells, dbl])
+    k = int(np.round(0.5 * np.sqrt(3000))) * 3
+    assert k > 64
+    print(f"exact scrublet shape k={k}", flush=True)
+    _check(x, k, None, orc)
+
+
+def test_large_k_fewer_points_and_overflow(orc):
+    x = np.random.default_rng(1).random((90, 6), dtype=np.float32)
+    idx, dist = _check(x, 120, None, orc)
+    assert (idx[:, 89:] == 0xFFFFFFFF).all() and np.isinf(dist[:, 89:]).all()
+    # 700 copies of one point: ties beyond the candidate buffer go to the fallback
+    y = np.random.default_rng(2).random((1500, 50), dtype=np.float32)
+    y[200:900] = y[200]
+    _check(y, 100, np.array([200, 201, 650, 899, 0, 1400], np.uint32), orc)
