@@ -163,7 +163,7 @@ extern "C" int psq_knn_exact_batched_device(const float* d_data, uint64_t n_tota
   const uint32_t dp = (uint32_t)((d + 31) / 32 * 32);
   const uint32_t K = k <= 16 ? 16 : (k <= 32 ? 32 : 64);
   const uint32_t cap = 128;
-  const bool d50 = d == 50;
+  const uint32_t dfix = d == 50 ? 50u : (d == 30 ? 30u : 0u);
   std::vector<uint64_t> off(offsets, offsets + n_samples + 1);
   uint64_t maxs = 0;
   for (uint64_t i = 0; i < n_samples; ++i) maxs = std::max(maxs, off[i + 1] - off[i]);
@@ -247,8 +247,10 @@ extern "C" int psq_knn_exact_batched_device(const float* d_data, uint64_t n_tota
       unsigned grid;
       BTRY(device_grid(t.block, &grid));
       BHIPC(hipEventRecord(a, s));
-      if (d50)
+      if (dfix == 50)
         BTRY(dispatch_pass1<50>(K, Xprobe, (uint32_t)d, dp, d_poff2, d_units2, U, t, d_nseg2, d_pofs, counter, d_part2, grid, s));
+      else if (dfix == 30)
+        BTRY(dispatch_pass1<30>(K, Xprobe, (uint32_t)d, dp, d_poff2, d_units2, U, t, d_nseg2, d_pofs, counter, d_part2, grid, s));
       else
         BTRY(dispatch_pass1<0>(K, Xprobe, (uint32_t)d, dp, d_poff2, d_units2, U, t, d_nseg2, d_pofs, counter, d_part2, grid, s));
       BHIPC(hipEventRecord(b, s));
@@ -291,8 +293,10 @@ extern "C" int psq_knn_exact_batched_device(const float* d_data, uint64_t n_tota
   BHIPC(hipEventCreate(&e2));
   BHIPC(hipEventCreate(&e3));
   BHIPC(hipEventRecord(e0, s));
-  if (d50)
+  if (dfix == 50)
     BTRY(dispatch_pass1<50>(K, Xp, (uint32_t)d, dp, d_off, d_units, U, best, d_nseg, d_poff, counter, d_partial, grid, s));
+  else if (dfix == 30)
+    BTRY(dispatch_pass1<30>(K, Xp, (uint32_t)d, dp, d_off, d_units, U, best, d_nseg, d_poff, counter, d_partial, grid, s));
   else
     BTRY(dispatch_pass1<0>(K, Xp, (uint32_t)d, dp, d_off, d_units, U, best, d_nseg, d_poff, counter, d_partial, grid, s));
   BHIPC(hipEventRecord(e1, s));
@@ -301,7 +305,7 @@ extern "C" int psq_knn_exact_batched_device(const float* d_data, uint64_t n_tota
   BHIPC(hipGetLastError());
   {
     const size_t smem = (size_t)best.tile * dp * sizeof(float);
-    auto kern = d50 ? k_bexact_pass2<50> : k_bexact_pass2<0>;
+    auto kern = dfix == 50 ? k_bexact_pass2<50> : (dfix == 30 ? k_bexact_pass2<30> : k_bexact_pass2<0>);
     if (smem > 65536)
       BHIPC(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     BHIPC(hipMemsetAsync(counter, 0, 4, s));

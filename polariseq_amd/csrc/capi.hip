@@ -995,11 +995,11 @@ int knn_exact_big(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d
                      dim3(256), 0, s, d_data, (uint32_t)n, (uint32_t)d, dp, Xp);
   HIPC(hipGetLastError());
   const size_t smem = (size_t)64 * dp * sizeof(float);
-  const bool d50 = d == 50;
+  const uint32_t dfix = d == 50 ? 50u : (d == 30 ? 30u : 0u);
   void (*p1)(const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t, uint32_t,
-             uint32_t, float*) = d50 ? k_exact_pass1<50, 64> : k_exact_pass1<0, 64>;
+             uint32_t, float*) = dfix == 50 ? k_exact_pass1<50, 64> : (dfix == 30 ? k_exact_pass1<30, 64> : k_exact_pass1<0, 64>);
   void (*p2)(const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t, uint32_t,
-             const float*, uint32_t*, uint32_t*, uint32_t) = d50 ? k_exact_pass2<50> : k_exact_pass2<0>;
+             const float*, uint32_t*, uint32_t*, uint32_t) = dfix == 50 ? k_exact_pass2<50> : (dfix == 30 ? k_exact_pass2<30> : k_exact_pass2<0>);
   if (smem > 65536) {
     HIPC(hipFuncSetAttribute((const void*)p1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     HIPC(hipFuncSetAttribute((const void*)p2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
@@ -1092,13 +1092,13 @@ int psq_host::knn_exact_device_impl(const float* d_data, uint64_t n, uint64_t d,
   TRY(fill_u32(cnt, nq, 0, s));
   TRY(fill_u32(novf, 1, 0, s));
   const size_t smem = (size_t)64 * dp * sizeof(float);
-  const bool d50 = d == 50;
+  const uint32_t dfix = d == 50 ? 50u : (d == 30 ? 30u : 0u);
   dim3 g1(qblocks, nseg);
   void (*p1)(const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t, uint32_t,
              uint32_t, float*);
-  if (K == 16) p1 = d50 ? k_exact_pass1<50, 16> : k_exact_pass1<0, 16>;
-  else if (K == 32) p1 = d50 ? k_exact_pass1<50, 32> : k_exact_pass1<0, 32>;
-  else p1 = d50 ? k_exact_pass1<50, 64> : k_exact_pass1<0, 64>;
+  if (K == 16) p1 = dfix == 50 ? k_exact_pass1<50, 16> : (dfix == 30 ? k_exact_pass1<30, 16> : k_exact_pass1<0, 16>);
+  else if (K == 32) p1 = dfix == 50 ? k_exact_pass1<50, 32> : (dfix == 30 ? k_exact_pass1<30, 32> : k_exact_pass1<0, 32>);
+  else p1 = dfix == 50 ? k_exact_pass1<50, 64> : (dfix == 30 ? k_exact_pass1<30, 64> : k_exact_pass1<0, 64>);
   if (smem > 65536) HIPC(hipFuncSetAttribute((const void*)p1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
   hipLaunchKernelGGL(p1, g1, dim3(256), smem, s, Xp, (uint32_t)n, (uint32_t)d, dp, d_queries,
                      (uint32_t)nq, seg_len, nseg, partial);
@@ -1107,7 +1107,7 @@ int psq_host::knn_exact_device_impl(const float* d_data, uint64_t n, uint64_t d,
                      nseg, K, (uint32_t)k, (uint32_t)d, T);
   HIPC(hipGetLastError());
   void (*p2)(const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t, uint32_t,
-             const float*, uint32_t*, uint32_t*, uint32_t) = d50 ? k_exact_pass2<50> : k_exact_pass2<0>;
+             const float*, uint32_t*, uint32_t*, uint32_t) = dfix == 50 ? k_exact_pass2<50> : (dfix == 30 ? k_exact_pass2<30> : k_exact_pass2<0>);
   if (smem > 65536) HIPC(hipFuncSetAttribute((const void*)p2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
   hipLaunchKernelGGL(p2, g1, dim3(256), smem, s, Xp, (uint32_t)n, (uint32_t)d, dp, d_queries,
                      (uint32_t)nq, seg_len, T, cnt, cand, cap);

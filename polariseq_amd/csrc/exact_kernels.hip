@@ -8,8 +8,10 @@
 //
 // pass 1  one lane per query, points streamed through LDS and read by
 //         broadcast; each lane keeps its K smallest FP32 distances per point
-//         segment (registers, sorted insertion).
-// tau     the k-th smallest over all segments -> threshold T = tau * F, where
+//         segment (registers, sorted insertion). For d = 50 (the kNN graph's
+//         PCA width) and d = 30 (Scrublet's) the query row is held in
+//         registers; other widths read it through L1 for every point.
+// tau    the k-th smallest over all segments -> threshold T = tau * F, where
 //         F = 1 + 4 (d + 8) 2^-24 bounds the FP32 error of both ranks (each
 //         FP32 distance is within (d + 4) u of the exact value, relative,
 //         because every term is non-negative).
@@ -459,6 +461,12 @@ template __global__ void k_exact_pass1<0, 32>(const float*, uint32_t, uint32_t, 
                                               const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
 template __global__ void k_exact_pass1<0, 64>(const float*, uint32_t, uint32_t, uint32_t,
                                               const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+template __global__ void k_exact_pass1<30, 16>(const float*, uint32_t, uint32_t, uint32_t,
+                                               const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+template __global__ void k_exact_pass1<30, 32>(const float*, uint32_t, uint32_t, uint32_t,
+                                               const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+template __global__ void k_exact_pass1<30, 64>(const float*, uint32_t, uint32_t, uint32_t,
+                                               const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
 template __global__ void k_exact_pass1<50, 16>(const float*, uint32_t, uint32_t, uint32_t,
                                                const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
 template __global__ void k_exact_pass1<50, 32>(const float*, uint32_t, uint32_t, uint32_t,
@@ -468,6 +476,9 @@ template __global__ void k_exact_pass1<50, 64>(const float*, uint32_t, uint32_t,
 template __global__ void k_exact_pass2<0>(const float*, uint32_t, uint32_t, uint32_t,
                                           const uint32_t*, uint32_t, uint32_t, const float*,
                                           uint32_t*, uint32_t*, uint32_t);
+template __global__ void k_exact_pass2<30>(const float*, uint32_t, uint32_t, uint32_t,
+                                           const uint32_t*, uint32_t, uint32_t, const float*,
+                                           uint32_t*, uint32_t*, uint32_t);
 template __global__ void k_exact_pass2<50>(const float*, uint32_t, uint32_t, uint32_t,
                                            const uint32_t*, uint32_t, uint32_t, const float*,
                                            uint32_t*, uint32_t*, uint32_t);
@@ -654,6 +665,15 @@ template __global__ void k_bexact_pass1<0, 32>(const float*, uint32_t, uint32_t,
 template __global__ void k_bexact_pass1<0, 64>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
                                                uint32_t, uint32_t, uint32_t, const uint32_t*, const uint64_t*,
                                                uint32_t*, float*);
+template __global__ void k_bexact_pass1<30, 16>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
+                                                uint32_t, uint32_t, uint32_t, const uint32_t*, const uint64_t*,
+                                                uint32_t*, float*);
+template __global__ void k_bexact_pass1<30, 32>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
+                                                uint32_t, uint32_t, uint32_t, const uint32_t*, const uint64_t*,
+                                                uint32_t*, float*);
+template __global__ void k_bexact_pass1<30, 64>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
+                                                uint32_t, uint32_t, uint32_t, const uint32_t*, const uint64_t*,
+                                                uint32_t*, float*);
 template __global__ void k_bexact_pass1<50, 16>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
                                                 uint32_t, uint32_t, uint32_t, const uint32_t*, const uint64_t*,
                                                 uint32_t*, float*);
@@ -666,6 +686,9 @@ template __global__ void k_bexact_pass1<50, 64>(const float*, uint32_t, uint32_t
 template __global__ void k_bexact_pass2<0>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
                                            uint32_t, uint32_t, uint32_t, const float*, uint32_t*, uint32_t*,
                                            uint32_t*, uint32_t);
+template __global__ void k_bexact_pass2<30>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
+                                            uint32_t, uint32_t, uint32_t, const float*, uint32_t*, uint32_t*,
+                                            uint32_t*, uint32_t);
 template __global__ void k_bexact_pass2<50>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
                                             uint32_t, uint32_t, uint32_t, const float*, uint32_t*, uint32_t*,
                                             uint32_t*, uint32_t);

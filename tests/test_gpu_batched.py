@@ -34,6 +34,21 @@ def test_batched_samples_bit_exact_and_tuned_once(orc):
     assert P.batched_report()["tuned_now"] == 0
 
 
+def test_batched_scrublet_width(orc):
+    """d = 30, Scrublet's default number of principal components, k = 20."""
+    samples = ds.batched_samples(n_samples=4, d=30, seed=8, lo=300, hi=2500, mu_log=6.5,
+                                 sigma_log=0.8)
+    pts = [p for p, _ in samples]
+    off = np.concatenate([[0], np.cumsum([len(p) for p in pts])]).astype(np.uint64)
+    x = np.ascontiguousarray(np.concatenate(pts, axis=0))
+    idx, dist = P.knn_exact_batched(x, off, 20)
+    for s in range(len(pts)):
+        a, b = int(off[s]), int(off[s + 1])
+        ei, ed = orc.knn_exact64(pts[s], 20)
+        assert np.array_equal(idx[a:b] - a, ei), f"sample {s}"
+        assert np.allclose(dist[a:b], np.sqrt(ed), rtol=1e-5, atol=0)
+
+
 def test_batched_single_tiny_samples_and_generic_dims(orc):
     rng = np.random.default_rng(1)
     sizes = [5, 17, 1, 40, 3]

@@ -47,7 +47,7 @@ def test_four_cells_kat(orc):
 
 
 @pytest.mark.parametrize("n,d,k", [(2000, 50, 15), (2000, 50, 30), (1500, 50, 64), (3000, 8, 10),
-                                   (1200, 20, 1), (700, 30, 33), (300, 3, 16)])
+                                   (1200, 20, 1), (700, 30, 33), (1800, 30, 12), (300, 3, 16)])
 def test_random_with_ties_bit_exact(orc, n, d, k):
     rng = np.random.default_rng(n * 31 + d)
     x = ds.mixture(n, d, 6, seed=n + d) if d >= 8 else rng.random((n, d), dtype=np.float32)
