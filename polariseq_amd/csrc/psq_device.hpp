@@ -273,12 +273,16 @@ __device__ inline uint64_t bucket_to_row(uint64_t b) {
   return (b == EMPTY_KEY || (b >> 32) == 0) ? EMPTY_KEY : b - (1ull << 32);
 }
 
-// Insert a proposal (row key `key`, distance finite) into its bucket. The
-// unpinned part of a bucket holds the smallest distinct proposal keys in
-// ascending order: a key starts at its rank among the slots read (slots only
-// decrease, so the smaller ones stay smaller), then an atomic min either places
-// it and carries the displaced key down, or carries it on. Equal keys stop the
-// cascade, so the content is a set function of the proposals.
+// Insert a proposal (row key `key`, distance finite) into its bucket. A bucket
+// holds the smallest distinct keys offered to it (pinned keys included), in no
+// particular order: a key that is absent and below the bucket's largest slot
+// replaces that slot with one compare-and-swap (the first such slot when
+// several are equal, e.g. empty). A failed swap returns the slot's current
+// value and the choice is made again. Slots only ever decrease, so every value
+// read is the slot's value at some earlier time; two inserts of one key then
+// cannot both succeed in different slots (each would have to see the other's
+// slot as the larger, older one), and the content after all proposals is the
+// set of the 8 smallest distinct keys, whatever the order of arrival.
 __device__ inline void propose(uint64_t* __restrict__ slots, uint8_t* __restrict__ touched,
                                uint32_t row, uint64_t key, uint32_t S, uint32_t bmask,
                                uint64_t seed, uint64_t iter) {
@@ -287,21 +291,25 @@ __device__ inline void propose(uint64_t* __restrict__ slots, uint8_t* __restrict
   uint64_t* bk = slots + (size_t)row * S + (size_t)b * BUCKET;
   const ulonglong2* b2 = reinterpret_cast<const ulonglong2*>(bk);
   const uint64_t bkey = bucket_key(key), pin = pin_key(q);
-  ulonglong2 v01 = b2[3];
-  if (bkey >= v01.y) return;  // 8 smaller keys already present (or this one)
-  ulonglong2 a = b2[0], bb = b2[1], c = b2[2];
-  uint64_t v[8] = {a.x, a.y, bb.x, bb.y, c.x, c.y, v01.x, v01.y};
-  int r = 0;
+  const ulonglong2 a = b2[0], bb = b2[1], c = b2[2], e = b2[3];
+  uint64_t v[8] = {a.x, a.y, bb.x, bb.y, c.x, c.y, e.x, e.y};
+  for (;;) {
+    uint64_t mx = v[0];
+    int jm = 0;
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    if (v[j] == bkey || v[j] == pin) return;
-    r += (v[j] < bkey);
-  }
-  uint64_t cur = bkey;
-  for (int j = r; j < BUCKET; ++j) {
-    uint64_t old = atomicMin((unsigned long long*)&bk[j], (unsigned long long)cur);
-    if (old == cur || old == EMPTY_KEY) break;
-    cur = old > cur ? old : cur;
+    for (int j = 0; j < 8; ++j) {
+      if (v[j] == bkey || v[j] == pin) return;
+      if (v[j] > mx) {
+        mx = v[j];
+        jm = j;
+      }
+    }
+    if (bkey >= mx) return;  // 8 smaller keys present
+    const uint64_t old = atomicCAS((unsigned long long*)&bk[jm], (unsigned long long)mx,
+                                   (unsigned long long)bkey);
+    if (old == mx) break;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = j == jm ? old : v[j];
   }
   touched[row] = 1;
 }
