@@ -366,8 +366,8 @@ __global__ __launch_bounds__(64) void k_join(
 // becomes the top k of (row ∪ proposals). Word 2 of the striped counters
 // `changed` counts entries of the new row absent from the old.
 // Proposals never repeat a neighbour of the row (pinned slots reject those)
-// and are distinct (a neighbour has one bucket, whose cascade stops on equal
-// keys), so when they fit one 64-lane vector the merge is: compact them, sort,
+// and are distinct (a neighbour has one bucket, which never holds a key twice),
+// so when they fit one 64-lane vector the merge is: compact them, sort,
 // and keep the 64 smallest of (row, proposals) with one bitonic merge (the row
 // is sorted, except before the forced first merge, which sorts it). Otherwise
 // every slot and the row are sorted together and deduplicated on (dist, idx),
