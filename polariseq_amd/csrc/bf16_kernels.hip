@@ -72,7 +72,7 @@ __global__ __launch_bounds__(64) void k_join_bf16(
   uint32_t* pb_row = reinterpret_cast<uint32_t*>(s_nrm + R);
   uint32_t* sv = pb_row + PB_CAP;
   const uint32_t l = (uint32_t)lane_id();
-  const uint32_t i = blockIdx.x;
+  const uint32_t i = xcd_row();
   if (i >= n) return;  // block-uniform
   const uint32_t cnt = ncnt[i];
   const uint32_t nn = cnt & 0xFFFF, no = cnt >> 16, nc = nn + no;

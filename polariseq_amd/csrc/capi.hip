@@ -270,7 +270,8 @@ int resolve(uint64_t n, uint64_t d, const psq_nndescent_opts* o, Resolved* r) {
   if (n == 0 || d == 0) return fail(PSQ_E_INVALID, "empty data: n_obs=%llu n_dim=%llu",
                                     (unsigned long long)n, (unsigned long long)d);
   if (o->k == 0) return fail(PSQ_E_INVALID, "k must be >= 1");
-  if (n >= (1ull << 31)) return fail(PSQ_E_UNSUPPORTED, "n_obs must be < 2^31");
+  // one 64-lane workgroup per row, and a dispatch counts fewer than 2^32 lanes
+  if (n > (1ull << 26) - XCDS) return fail(PSQ_E_UNSUPPORTED, "n_obs must be <= 2^26 - 8");
   if (d > 256) return fail(PSQ_E_UNSUPPORTED, "n_dim > 256 is not supported on the GPU path");
   r->n = n;
   r->d = d;
@@ -349,7 +350,7 @@ int launch_join(const Resolved& R, const float* Xp, const uint32_t* newl, const 
       R.d == 50 ? k_join<50> : k_join<0>;
   if (smem > 65536)
     HIPC(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-  hipLaunchKernelGGL(kern, dim3((unsigned)R.n), dim3(64), smem, s, Xp, (uint32_t)R.n, (uint32_t)R.d,
+  hipLaunchKernelGGL(kern, dim3(xcd_grid(R.n)), dim3(64), smem, s, Xp, (uint32_t)R.n, (uint32_t)R.d,
                      R.dp, RS, newl, oldl, ncnt, (uint32_t)R.m, thr, slots, touched, (uint32_t)R.S,
                      R.seed, iter, counters);
   HIPC(hipGetLastError());
@@ -723,7 +724,7 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
       if (smem > 65536)
         HIPC(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)smem));
-      hipLaunchKernelGGL(kern, dim3(n), dim3(64), smem, s, Xp, n, (uint32_t)R.d, R.dp, RS, newl,
+      hipLaunchKernelGGL(kern, dim3(xcd_grid(n)), dim3(64), smem, s, Xp, n, (uint32_t)R.d, R.dp, RS, newl,
                          oldl, ncnt, m, thr, BF16_SLACK, slots, touched, S, R.seed, it, ctr);
       HIPC(hipGetLastError());
     } else {

@@ -306,7 +306,7 @@ __global__ __launch_bounds__(64) void k_join(
   float* s_thr = reinterpret_cast<float*>(s_id + 2 * m);
   uint32_t* pb_row = reinterpret_cast<uint32_t*>(s_thr + 2 * m);
   const uint32_t l = (uint32_t)lane_id();
-  const uint32_t i = blockIdx.x;
+  const uint32_t i = xcd_row();
   if (i >= n) return;  // block-uniform
   const uint32_t cnt = ncnt[i];
   const uint32_t nn = cnt & 0xFFFF, no = cnt >> 16, nc = nn + no;

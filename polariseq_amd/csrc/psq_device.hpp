@@ -76,6 +76,17 @@ __device__ inline uint64_t shfl64(uint64_t v, int src) {
 }
 __device__ inline int lane_id() { return (int)(threadIdx.x & (WAVE - 1)); }
 
+// Row of workgroup blockIdx.x for the one-row-per-workgroup kernels whose rows
+// share data with nearby rows (the candidate rows of a join). The hardware
+// deals workgroups to the XCDS dies round-robin and each die has its own L2,
+// so XCD x takes the contiguous rows [x * c, (x + 1) * c), c = gridDim.x /
+// XCDS, in order. The grid is XCDS * ceil(n / XCDS); rows >= n do nothing.
+constexpr uint32_t XCDS = 8;
+__device__ inline uint32_t xcd_row() {
+  return (blockIdx.x % XCDS) * (gridDim.x / XCDS) + blockIdx.x / XCDS;
+}
+__host__ inline uint32_t xcd_grid(uint64_t n) { return (uint32_t)((n + XCDS - 1) / XCDS * XCDS); }
+
 // Bitonic sort of one u64 per lane, ascending by lane.
 __device__ inline uint64_t wave_sort64(uint64_t v) {
   const int l = lane_id();
