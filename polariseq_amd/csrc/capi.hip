@@ -252,11 +252,21 @@ int build_reverse(Workspace* ws, const PartShape& P, PartBufs& B, const uint64_t
 constexpr float BF16_SLACK = 1.03f;
 constexpr uint32_t BF16_SV_CAP = 320;  // screened-pair list of k_join_bf16
 constexpr uint64_t EX_BIG_QMIN = 256;   // smallest query chunk of the large-k exact path
-// BF16 form: for its first iterations the join's threshold is a row's k-th
-// entry (the output size), not its heap_k-th; the rows still keep heap_k
-// entries. Early rows are poor, and the wider threshold mostly buys proposals
-// that later iterations replace anyway.
-constexpr uint64_t BF16_WARM_ITERS = 2;
+// BF16 form: the join's threshold is a row's entry of rank bf16_thr_rank (1 =
+// its nearest), not its heap_k-th, in the first two iterations: k (the output
+// size) in the second and, when rows keep at least 1.5 k entries, 0.4 k (rounded
+// up) in the first; the rows still keep heap_k entries. Early rows are poor,
+// and the wider thresholds mostly buy proposals that later iterations replace
+// anyway (on 4.06M x 50, k = 15, heap_k = 24, the first join sends 2.7x fewer
+// proposals for the same recall and iteration count). A pair skipped in the
+// first join is not joined again, so rows with no spare entries (heap_k = k)
+// keep the k-th entry there: without the spares the narrow first threshold
+// costs recall.
+uint32_t bf16_thr_rank(uint64_t it, uint64_t ko, uint64_t k) {
+  if (it == 0 && 2 * k >= 3 * ko) return (uint32_t)std::max<uint64_t>(1, (2 * ko + 4) / 5);
+  if (it <= 1) return (uint32_t)ko;
+  return (uint32_t)k;
+}
 
 struct Resolved {
   // ko: neighbours output per row; k: entries each row keeps while iterating.
@@ -709,7 +719,7 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
     TRY(build_reverse(ws, RP, RB, G, n, k, R.seed, it, roff, rev, s));
     hipLaunchKernelGGL(k_sample_select, dim3(n), dim3(64), 0, s, G, n, k, m, R.seed,
                        it, roff, rev, newl, oldl, ncnt, slots, S, thr,
-                       (bf16 && it < BF16_WARM_ITERS) ? (uint32_t)R.ko : k);
+                       bf16 ? bf16_thr_rank(it, R.ko, k) : k);
     HIPC(hipGetLastError());
     hipEvent_t a, b;
     TRY(evp.make(&a));
