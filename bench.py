@@ -6,10 +6,9 @@ Default workload: 4.06M cells x 50 PCs FP32, 150-component Gaussian mixture,
 k = 15, seed 0 (polariseq_amd.datasets.mixture). Two forms are timed: plain (the
 drop-in: FP32 join, 48-tree forest, the reference's defaults) and learned+BF16,
 the headline, at the operating point --learned-trees / --learned-heap-k /
---learned-delta / --learned-slots (6 trees on the learned codes, rows of 24 while
-iterating, early stop at 0.012 n h changes, 64 proposal slots per row), chosen for
-throughput at recall@15 >= 0.95 and
-reported with the measured recall. After the timed steps:
+--learned-delta / --learned-slots (5 trees on the learned codes, rows of 26 while
+iterating, early stop at 0.07 n h changes, 64 proposal slots per row), chosen for
+throughput at recall@15 >= 0.95 and reported with the measured recall. After the timed steps:
   * recall@15 of the graph against the exact top-15 over all cells, for 100k
     random query cells (exact kNN on the GPU, FP64-ranked);
   * the join kernel's roofline from HIP events recorded on its stream;
@@ -79,10 +78,10 @@ def main() -> int:
                          "learned_bf16 = learned seeding + BF16; all = plain and learned_bf16")
     ap.add_argument("--init", default="rp_forest", choices=["random", "rp_forest"],
                     help="start of the plain and bf16 forms")
-    ap.add_argument("--learned-trees", type=int, default=6)
-    ap.add_argument("--learned-heap-k", type=int, default=24,
+    ap.add_argument("--learned-trees", type=int, default=5)
+    ap.add_argument("--learned-heap-k", type=int, default=26,
                     help="entries per row while iterating, learned form (the API default is 2k)")
-    ap.add_argument("--learned-delta", type=float, default=0.012,
+    ap.add_argument("--learned-delta", type=float, default=0.07,
                     help="early-stop fraction of the learned form (the API default is 0.001)")
     ap.add_argument("--learned-slots", type=int, default=64,
                     help="proposal slots per row, learned form (0 = the API default, 128 here)")

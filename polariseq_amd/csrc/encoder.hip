@@ -4,6 +4,7 @@
 // raw embedding.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -43,6 +44,7 @@ struct psq_encoder {
   float* wr = nullptr;  // BF16-rounded copy used by the products
   float* m1 = nullptr;
   float* m2 = nullptr;
+  float* packed = nullptr;  // wr laid out as the encode kernel's per-lane operands
   uint64_t steps_done = 0;
   double last_loss = 0.0;
   double train_ms = 0.0;
@@ -84,6 +86,7 @@ int psq_encoder_create(psq_encoder** out, uint64_t seed, int32_t device) {
   if (rc == hipSuccess) rc = hipMalloc(&e->wr, NP * sizeof(float));
   if (rc == hipSuccess) rc = hipMalloc(&e->m1, NP * sizeof(float));
   if (rc == hipSuccess) rc = hipMalloc(&e->m2, NP * sizeof(float));
+  if (rc == hipSuccess) rc = hipMalloc(&e->packed, ENC_PACKED * sizeof(float));
   if (rc == hipSuccess) rc = hipMemcpy(e->w, p.data(), NP * sizeof(float), hipMemcpyHostToDevice);
   if (rc == hipSuccess) rc = hipMemset(e->m1, 0, NP * sizeof(float));
   if (rc == hipSuccess) rc = hipMemset(e->m2, 0, NP * sizeof(float));
@@ -92,6 +95,7 @@ int psq_encoder_create(psq_encoder** out, uint64_t seed, int32_t device) {
     (void)hipFree(e->wr);
     (void)hipFree(e->m1);
     (void)hipFree(e->m2);
+    (void)hipFree(e->packed);
     delete e;
     return fail(PSQ_E_DEVICE, "encoder allocation failed: %s", hipGetErrorString(rc));
   }
@@ -108,6 +112,7 @@ void psq_encoder_destroy(psq_encoder* e) {
   (void)hipFree(e->wr);
   (void)hipFree(e->m1);
   (void)hipFree(e->m2);
+  (void)hipFree(e->packed);
   delete e;
 }
 
@@ -292,8 +297,15 @@ int psq_encoder_encode(const psq_encoder* e, const float* d_data, uint64_t n, ui
   if (d != 50) return fail(PSQ_E_UNSUPPORTED, "the encoder takes 50 dimensions");
   if (n == 0) return PSQ_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(k_mlp_forward, dim3(blocks_for(n, 16)), dim3(128), 0, s, d_data, 50u, nullptr,
-                     (uint32_t)n, e->wr, e->mu, e->inv_scale, nullptr, d_codes);
+  if (n >= (1ull << 32) - 16) return fail(PSQ_E_UNSUPPORTED, "n must be < 2^32 - 16");
+  hipLaunchKernelGGL(k_encode_pack, dim3(blocks_for(ENC_PACKED, 256)), dim3(256), 0, s, e->wr,
+                     e->packed);
+  EHIPC(hipGetLastError());
+  // persistent blocks: a few per CU, each stages the operands in LDS once
+  const unsigned tiles = (unsigned)((n + 15) / 16);
+  const unsigned grid = std::min<unsigned>(blocks_for(tiles, ENC_WAVES), 4096u);
+  hipLaunchKernelGGL(k_encode_mfma, dim3(grid), dim3(ENC_WAVES * 64), 0, s, d_data, (uint32_t)n,
+                     e->packed, e->wr, e->mu, e->inv_scale, d_codes);
   EHIPC(hipGetLastError());
   return PSQ_OK;
 }

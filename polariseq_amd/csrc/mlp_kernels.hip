@@ -92,6 +92,97 @@ __global__ __launch_bounds__(128) void k_mlp_forward(
   }
 }
 
+// ── encode on the FP32 matrix cores ──
+// z for every row of X (stride 50) with v_mfma_f32_16x16x4f32, 16 rows per
+// wave-tile. MFMA 16x16x4 operands: lane l (c = l % 16, g = l / 16) holds
+// A[c][g] and B[g][c]; D[4g + i][c] is its i-th output.
+//   layer 1, transposed: H^T (hidden x row) = W1 x X^T, tile t of 16 hidden
+//     units, 13 k-steps over the 50 inputs (52 with zeros): lane l holds
+//     H[row c][hidden 16t + 4g + i] in D.
+//   layer 2: Z = gelu(H) x W2^T with the hidden index permuted so that each
+//     lane's A operand is one of its own layer-1 outputs: k-step s = 4t + i
+//     contracts hidden 16t + 4g + i in lane group g (every hidden unit once).
+// The per-lane weight operands are packed once (k_encode_pack) into
+//   A1[t][s][l] = W1[16t + c][4s + g]   (t < 8, s < 13; 0 past input 50)
+//   B2[u][s][l] = W2[16u + c][16(s / 4) + 4g + s % 4]   (u < 2, s < 32)
+// and staged in LDS per block. Products are FP32; sums are ordered by the
+// matrix core, so z agrees with the VALU forward to FP32 rounding.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int ENC_KS1 = 13, ENC_KS2 = 32;
+constexpr int ENC_A1 = 8 * ENC_KS1 * 64, ENC_B2 = 2 * ENC_KS2 * 64;
+static_assert(ENC_A1 + ENC_B2 == ENC_PACKED, "packed encoder operands");
+
+__global__ void k_encode_pack(const float* __restrict__ pr, float* __restrict__ packed) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (uint32_t)(ENC_A1 + ENC_B2)) return;
+  const uint32_t l = e % 64, c = l % 16, g = l / 16;
+  if (e < (uint32_t)ENC_A1) {
+    const uint32_t s = (e / 64) % ENC_KS1, t = e / (64 * ENC_KS1);
+    const uint32_t in = 4 * s + g;
+    packed[e] = in < (uint32_t)MLP_IN ? pr[OFF_W1 + (16 * t + c) * MLP_IN + in] : 0.0f;
+  } else {
+    const uint32_t f = e - ENC_A1;
+    const uint32_t s = (f / 64) % ENC_KS2, u = f / (64 * ENC_KS2);
+    const uint32_t h = 16 * (s / 4) + 4 * g + s % 4;
+    packed[e] = pr[OFF_W2 + (16 * u + c) * MLP_H + h];
+  }
+}
+
+__global__ __launch_bounds__(ENC_WAVES * 64) void k_encode_mfma(
+    const float* __restrict__ X, uint32_t rows, const float* __restrict__ packed,
+    const float* __restrict__ pr, float mu, float inv_scale, float* __restrict__ z_out) {
+  __shared__ float s_w[ENC_A1 + ENC_B2];
+  __shared__ float s_b[MLP_H + MLP_OUT];
+  for (uint32_t e = threadIdx.x; e < (uint32_t)(ENC_A1 + ENC_B2); e += blockDim.x) s_w[e] = packed[e];
+  for (uint32_t e = threadIdx.x; e < (uint32_t)(MLP_H + MLP_OUT); e += blockDim.x)
+    s_b[e] = pr[e < (uint32_t)MLP_H ? OFF_B1 + e : OFF_B2 + (e - MLP_H)];
+  __syncthreads();
+  const uint32_t l = threadIdx.x % 64, c = l % 16, g = l / 16;
+  const uint32_t ntiles = (rows + 15) / 16;
+  const uint32_t stride = gridDim.x * ENC_WAVES;
+  for (uint32_t tile = blockIdx.x * ENC_WAVES + threadIdx.x / 64; tile < ntiles; tile += stride) {
+    // keeps the weight reads from LDS inside the loop: hoisted, they would take
+    // 168 registers per lane and leave one wave per SIMD
+    asm volatile("" ::: "memory");
+    const uint32_t r0 = tile * 16;
+    float xb[ENC_KS1];
+    const bool rv = r0 + c < rows;
+#pragma unroll
+    for (int s = 0; s < ENC_KS1; ++s) {
+      const uint32_t in = 4 * s + g;
+      xb[s] = (rv && in < (uint32_t)MLP_IN) ? (X[(size_t)(r0 + c) * MLP_IN + in] - mu) * inv_scale : 0.0f;
+    }
+    float h[8][4];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      f32x4 acc;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[i] = s_b[16 * t + 4 * g + i];
+#pragma unroll
+      for (int s = 0; s < ENC_KS1; ++s)
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(s_w[(t * ENC_KS1 + s) * 64 + l], xb[s], acc, 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) h[t][i] = gelu(acc[i]);
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      f32x4 acc;
+      const float b2 = s_b[MLP_H + 16 * u + c];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[i] = b2;
+#pragma unroll
+      for (int s = 0; s < ENC_KS2; ++s)
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(h[s / 4][s % 4],
+                                                   s_w[ENC_A1 + (u * ENC_KS2 + s) * 64 + l], acc, 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const uint32_t r = r0 + 4 * g + i;
+        if (r < rows) z_out[(size_t)r * MLP_OUT + 16 * u + c] = acc[i];
+      }
+    }
+  }
+}
+
 // Triplet loss and dL/dz for a batch of B triplets laid out as rows
 // [a_0..a_{B-1}, p_0..p_{B-1}, n_0..n_{B-1}] of z. One thread per triplet.
 __global__ void k_triplet_loss(const float* __restrict__ z, uint32_t B, float margin,

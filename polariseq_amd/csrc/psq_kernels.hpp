@@ -98,6 +98,13 @@ constexpr int MLP_N_PARAMS = 128 * 50 + 128 + 32 * 128 + 32;
 __global__ void k_mlp_round(const float* w, float* wr, uint32_t count);
 __global__ void k_mlp_forward(const float* X, uint32_t ldx, const uint32_t* idx, uint32_t rows,
                               const float* pr, float mu, float inv_scale, float* hpre_out, float* z_out);
+// Encode on the FP32 matrix cores: operands packed per lane (ENC_PACKED floats),
+// ENC_WAVES wave-tiles of 16 rows per block.
+constexpr int ENC_PACKED = 10752;
+constexpr int ENC_WAVES = 4;
+__global__ void k_encode_pack(const float* pr, float* packed);
+__global__ void k_encode_mfma(const float* X, uint32_t rows, const float* packed, const float* pr,
+                              float mu, float inv_scale, float* z_out);
 __global__ void k_triplet_loss(const float* z, uint32_t B, float margin, float* dz, float* loss_terms);
 __global__ void k_mlp_backward(const float* X, uint32_t ldx, const uint32_t* idx, uint32_t rows,
                                const float* pr, float mu, float inv_scale, const float* hpre,

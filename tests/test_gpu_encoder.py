@@ -58,6 +58,29 @@ def test_backward_matches_the_fp64_oracle():
         assert np.abs(g64[a:b]).max() > 0
 
 
+@pytest.mark.parametrize("n", [5003, 16, 1])
+def test_encode_matches_the_fp64_oracle_forward(n):
+    """The matrix-core encode (k_encode_mfma) against the oracle's forward with
+    the same BF16-rounded weights: FP32 rounding only. Row counts that are not
+    a multiple of the 16-row tile exercise the tail."""
+    import torch
+    x = ds.mixture(max(n, 2), 50, 10, seed=6)[:n]
+    p = _params(9)
+    mu, inv = -0.05, 0.3
+    enc = P.Encoder(seed=0)
+    enc.set_params(p, mu, inv)
+    z = enc.encode(torch.from_numpy(np.ascontiguousarray(x)).cuda()).cpu().numpy()
+    _, _, _, z64 = M.forward(p, x, mu, inv)
+    scale = np.abs(z64).max()
+    err = np.abs(z - z64).max() / scale
+    assert err < 1e-5, err
+    # a hidden unit dropped or a transposed weight block would not pass
+    q = p.copy()
+    q[M.OFF_W2:M.OFF_B2] = q[M.OFF_W2:M.OFF_B2].reshape(M.OUT, M.H)[:, ::-1].ravel()
+    _, _, _, zq = M.forward(q, x, mu, inv)
+    assert np.abs(z - zq).max() / scale > 1e-2
+
+
 def test_training_lowers_the_loss_and_is_deterministic():
     import torch
     x = ds.mixture(30000, 50, 20, seed=2)
