@@ -764,7 +764,8 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
 
   // ── extraction ──
   const uint32_t ko = (uint32_t)R.ko;
-  hipLaunchKernelGGL(k_row_counts, dim3(grid_for(n, 256)), dim3(256), 0, s, G, n, k, ko, inv, rowcnt);
+  hipLaunchKernelGGL(k_row_counts, dim3(grid_for((size_t)n * ko, 256)), dim3(256), 0, s, G, n, k, ko, inv,
+                     rowcnt);
   HIPC(hipGetLastError());
   TRY(scan_u32(ws, rowcnt, d_indptr, n, s));
   hipLaunchKernelGGL(k_extract, dim3(grid_for((size_t)n * ko, 256)), dim3(256), 0, s, G, n, k,

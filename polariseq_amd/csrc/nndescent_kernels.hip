@@ -385,6 +385,13 @@ __global__ __launch_bounds__(64) void k_merge(uint64_t* __restrict__ G, uint32_t
   uint64_t* row = G + (size_t)p * k;
   const uint64_t* sl = slots + (size_t)p * S;
   const uint64_t orow = l < k ? row[l] : EMPTY_KEY;
+  if (force && !touched[p]) {
+    // the forced pass only sorts (no proposals exist yet): a row the random
+    // fill left alone is sorted unless the renumbering reordered equal
+    // distances, so sorted rows are left as they are
+    const uint64_t next = shfl64(orow, (int)((l + 1) & (WAVE - 1)));
+    if (!__ballot(l + 1 < k && orow > next)) return;
+  }
   if (l == 0) touched[p] = 0;
   uint64_t v[NR];
   uint32_t total = 0;
@@ -453,20 +460,25 @@ __global__ __launch_bounds__(64) void k_merge(uint64_t* __restrict__ G, uint32_t
 // perm; otherwise perm and inv are NULL.
 __global__ void k_row_counts(const uint64_t* __restrict__ G, uint32_t n, uint32_t k, uint32_t ko,
                              const uint32_t* __restrict__ inv, uint32_t* __restrict__ cnt) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  // one thread per entry s < ko of output row i; a row's entries are a sorted
+  // prefix followed by EMPTY, so the last non-empty one (or an empty first)
+  // gives the count. n * ko < 2^32.
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * ko) return;
+  const uint32_t i = e / ko, s = e - i * ko;
   const uint64_t* r = G + (size_t)(inv ? inv[i] : i) * k;
-  uint32_t c = 0;
-  while (c < ko && r[c] != EMPTY_KEY) ++c;
-  cnt[i] = c;
+  const bool here = r[s] != EMPTY_KEY;
+  const bool next = s + 1 < ko && r[s + 1] != EMPTY_KEY;
+  if (s == 0 && !here) cnt[i] = 0;
+  if (here && !next) cnt[i] = s + 1;
 }
 __global__ void k_extract(const uint64_t* __restrict__ G, uint32_t n, uint32_t k, uint32_t ko,
                           const uint32_t* __restrict__ perm, const uint32_t* __restrict__ inv,
                           const uint32_t* __restrict__ off, uint32_t* __restrict__ indices,
                           float* __restrict__ dists) {
-  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)n * ko) return;
-  uint32_t i = (uint32_t)(e / ko), s = (uint32_t)(e - (size_t)i * ko);
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;  // n * ko < 2^32
+  if (e >= n * ko) return;
+  const uint32_t i = e / ko, s = e - i * ko;
   uint64_t key = G[(size_t)(inv ? inv[i] : i) * k + s];
   if (key == EMPTY_KEY) return;
   uint32_t o = off[i] + s;
@@ -496,9 +508,9 @@ __global__ void k_perm_rows(const float* __restrict__ X, uint32_t dp, const uint
 __global__ void k_perm_graph(const uint64_t* __restrict__ G, uint32_t k,
                              const uint32_t* __restrict__ perm, const uint32_t* __restrict__ inv,
                              uint32_t n, uint64_t* __restrict__ Gq) {
-  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)n * k) return;
-  const uint32_t p = (uint32_t)(e / k), s = (uint32_t)(e - (size_t)p * k);
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;  // n * k < 2^32
+  if (e >= n * k) return;
+  const uint32_t p = e / k, s = e - p * k;
   uint64_t key = G[(size_t)perm[p] * k + s];
   if (key != EMPTY_KEY) key = (key & ~0xFFFFFFFEull) | ((uint64_t)inv[key_idx(key)] << 1);
   Gq[e] = key;
