@@ -424,9 +424,20 @@ int launch_leaf_merge(uint32_t KM, uint32_t LG, const float* Xp, uint32_t d, uin
 // are merged into the rows, tree after tree (nndescent.rs:767-817).
 // Xs (ds dims, row stride dps) is what the hyperplanes split: the embedding
 // itself, or the learned codes. Leaf pairs are measured on Xp (the embedding).
+// Renumbering of the points in the first tree's leaf order (perm[p] = original
+// index of point p, inv its inverse, Xq the rows in the new order), made by
+// build_forest before the leaf pairs so that the graph is built in the new
+// numbering: neighbours get nearby indices, so rows, buckets and candidate rows
+// touched together share cache.
+struct Renumbering {
+  uint32_t* perm = nullptr;
+  uint32_t* inv = nullptr;
+  float* Xq = nullptr;
+};
+
 int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds, uint32_t dps,
                  const float* Xp, uint64_t* G, hipStream_t s, uint32_t* hbuf,
-                 const uint32_t** order0 = nullptr) {
+                 Renumbering* rn = nullptr) {
   const uint32_t n = (uint32_t)R.n, d = (uint32_t)R.d, dp = R.dp, k = (uint32_t)R.k;
   const uint32_t ls = (uint32_t)R.leaf_size;
   const uint64_t T = R.n_trees;
@@ -581,21 +592,33 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
     std::vector<uint32_t> hoff(tb + 1, 0);
     HIPC(hipMemcpyAsync(hoff.data(), toff, (tb + 1) * 4, hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
-    if (t0 == 0 && order0) {  // tree 0: the points in leaf order
-      uint32_t* o;
-      TRY(wbuf(ws, "order0", n, &o));
-      HIPC(hipMemcpyAsync(o, ids[cur], (size_t)n * 4, hipMemcpyDeviceToDevice, s));
-      *order0 = o;
+    if (rn) {
+      if (t0 == 0) {  // tree 0: the points in leaf order
+        TRY(wbuf(ws, "perm", n, &rn->perm));
+        TRY(wbuf(ws, "inv", n, &rn->inv));
+        TRY(wbuf(ws, "Xq", (size_t)n * dp, &rn->Xq));
+        HIPC(hipMemcpyAsync(rn->perm, ids[cur], (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_perm_inverse, dim3(grid_for(n, 256)), dim3(256), 0, s, rn->perm, n,
+                           rn->inv);
+        HIPC(hipGetLastError());
+        hipLaunchKernelGGL(k_perm_rows, dim3(grid_for((size_t)n * (dp / 4), 256)), dim3(256), 0, s,
+                           Xp, dp, rn->perm, n, rn->Xq);
+        HIPC(hipGetLastError());
+      }
+      hipLaunchKernelGGL(k_renumber, dim3(grid_for(total, 256)), dim3(256), 0, s, ids[cur], total,
+                         rn->inv);
+      HIPC(hipGetLastError());
     }
+    const float* Xl = rn ? rn->Xq : Xp;
     // leaf pairs, tree after tree
     for (uint32_t t = 0; t < tb; ++t) {
       const uint32_t nl = hoff[t + 1] - hoff[t];
       if (nl == 0) continue;
       const uint32_t* ids_t = ids[cur] + (size_t)t * n;
       if (d == 50)
-        TRY(launch_leaf_merge<50>(KM, LG, Xp, d, dp, n, ids_t, gs + hoff[t], gc + hoff[t], nl, G, k, s));
+        TRY(launch_leaf_merge<50>(KM, LG, Xl, d, dp, n, ids_t, gs + hoff[t], gc + hoff[t], nl, G, k, s));
       else
-        TRY(launch_leaf_merge<0>(KM, LG, Xp, d, dp, n, ids_t, gs + hoff[t], gc + hoff[t], nl, G, k, s));
+        TRY(launch_leaf_merge<0>(KM, LG, Xl, d, dp, n, ids_t, gs + hoff[t], gc + hoff[t], nl, G, k, s));
     }
   }
   return PSQ_OK;
@@ -672,37 +695,20 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   const bool bf16 = opts->precision == PSQ_PRECISION_BF16_RERANK;
   const bool forest = opts->init != PSQ_INIT_RANDOM && R.n_trees > 0;
   const bool reorder = forest && (opts->reorder == 1 || (opts->reorder == 0 && bf16));
-  const uint32_t* perm = nullptr;
-  uint32_t* inv = nullptr;
+  Renumbering rn;
   if (opts->init == PSQ_INIT_RP_FOREST && R.n_trees > 0)
     TRY(build_forest(ws, R, Xp, (uint32_t)R.d, R.dp, Xp, G, s, static_cast<uint32_t*>(hp) + 64,
-                     reorder ? &perm : nullptr));
+                     reorder ? &rn : nullptr));
   if (opts->init == PSQ_INIT_LEARNED && R.n_trees > 0) {
     float* codes;
     TRY(wbuf(ws, "codes", (size_t)n * 32, &codes));
     TRY(psq_encoder_encode(static_cast<const psq_encoder*>(opts->model), d_data, n, R.d, codes, s));
     TRY(build_forest(ws, R, codes, 32u, 32u, Xp, G, s, static_cast<uint32_t*>(hp) + 64,
-                     reorder ? &perm : nullptr));
+                     reorder ? &rn : nullptr));
   }
-  if (perm) {
-    // renumber the points in the first tree's leaf order: neighbours get nearby
-    // indices, so rows, buckets and candidate rows touched together share cache
-    float* Xq;
-    uint64_t* Gq;
-    TRY(wbuf(ws, "inv", n, &inv));
-    TRY(wbuf(ws, "Xq", (size_t)n * R.dp, &Xq));
-    TRY(wbuf(ws, "Gq", (size_t)n * k, &Gq));
-    hipLaunchKernelGGL(k_perm_inverse, dim3(grid_for(n, 256)), dim3(256), 0, s, perm, n, inv);
-    HIPC(hipGetLastError());
-    hipLaunchKernelGGL(k_perm_rows, dim3(grid_for((size_t)n * (R.dp / 4), 256)), dim3(256), 0, s, Xp,
-                       R.dp, perm, n, Xq);
-    HIPC(hipGetLastError());
-    hipLaunchKernelGGL(k_perm_graph, dim3(grid_for((size_t)n * k, 256)), dim3(256), 0, s, G, k, perm,
-                       inv, n, Gq);
-    HIPC(hipGetLastError());
-    Xp = Xq;
-    G = Gq;
-  }
+  const uint32_t* perm = rn.perm;
+  const uint32_t* inv = rn.inv;
+  if (rn.Xq) Xp = rn.Xq;
   hipLaunchKernelGGL(k_random_fill, dim3(grid_for(n, 256)), dim3(256), 0, s, Xp, n,
                      (uint32_t)R.d, R.dp, G, k, R.seed, touched);
   HIPC(hipGetLastError());

@@ -386,9 +386,9 @@ __global__ __launch_bounds__(64) void k_merge(uint64_t* __restrict__ G, uint32_t
   const uint64_t* sl = slots + (size_t)p * S;
   const uint64_t orow = l < k ? row[l] : EMPTY_KEY;
   if (force && !touched[p]) {
-    // the forced pass only sorts (no proposals exist yet): a row the random
-    // fill left alone is sorted unless the renumbering reordered equal
-    // distances, so sorted rows are left as they are
+    // the forced pass only sorts (no proposals exist yet): the leaf pairs
+    // leave rows sorted and the random fill marks the rows it extends, so a
+    // sorted row is left as it is
     const uint64_t next = shfl64(orow, (int)((l + 1) & (WAVE - 1)));
     if (!__ballot(l + 1 < k && orow > next)) return;
   }
@@ -503,17 +503,11 @@ __global__ void k_perm_rows(const float* __restrict__ X, uint32_t dp, const uint
   const uint32_t p = (uint32_t)(e / c4), c = (uint32_t)(e - (size_t)p * c4);
   reinterpret_cast<float4*>(Y)[e] = reinterpret_cast<const float4*>(X)[(size_t)perm[p] * c4 + c];
 }
-// Gq[p] = G[perm[p]] with every neighbour index renumbered by inv (the rows may
-// then be out of key order at equal distances; the forced merge sorts them).
-__global__ void k_perm_graph(const uint64_t* __restrict__ G, uint32_t k,
-                             const uint32_t* __restrict__ perm, const uint32_t* __restrict__ inv,
-                             uint32_t n, uint64_t* __restrict__ Gq) {
-  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;  // n * k < 2^32
-  if (e >= n * k) return;
-  const uint32_t p = e / k, s = e - p * k;
-  uint64_t key = G[(size_t)perm[p] * k + s];
-  if (key != EMPTY_KEY) key = (key & ~0xFFFFFFFEull) | ((uint64_t)inv[key_idx(key)] << 1);
-  Gq[e] = key;
+// ids[e] = inv[ids[e]]: point indices of the forest's leaf lists in the new
+// numbering.
+__global__ void k_renumber(uint32_t* __restrict__ ids, size_t total, const uint32_t* __restrict__ inv) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < total) ids[e] = inv[ids[e]];
 }
 
 // explicit instantiations used by the host code

@@ -36,8 +36,7 @@ __global__ void k_extract(const uint64_t* G, uint32_t n, uint32_t k, uint32_t ko
                           const uint32_t* inv, const uint32_t* off, uint32_t* indices, float* dists);
 __global__ void k_perm_inverse(const uint32_t* perm, uint32_t n, uint32_t* inv);
 __global__ void k_perm_rows(const float* X, uint32_t dp, const uint32_t* perm, uint32_t n, float* Y);
-__global__ void k_perm_graph(const uint64_t* G, uint32_t k, const uint32_t* perm, const uint32_t* inv,
-                             uint32_t n, uint64_t* Gq);
+__global__ void k_renumber(uint32_t* ids, size_t total, const uint32_t* inv);
 
 // RP forest (forest_kernels.hip)
 __global__ void k_forest_init(uint32_t* ids, uint32_t* segv, uint32_t n, uint32_t T, uint32_t leaf_size);
