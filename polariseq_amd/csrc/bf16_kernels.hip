@@ -84,7 +84,7 @@ __global__ __launch_bounds__(64) void k_join_bf16(
     s_thr[s] = thr[v];
   }
   __syncthreads();
-  const uint32_t c4 = (d + 3) / 4;
+  const uint32_t c4 = D > 0 ? (uint32_t)(D + 3) / 4 : (d + 3) / 4;  // a constant divisor for D > 0
   for (uint32_t e = l; e < (nc + 1) * c4; e += WAVE) {
     const uint32_t s = e / c4, c = e - s * c4;
     const uint32_t src = s < nc ? s_id[s] : i;
@@ -155,6 +155,16 @@ __global__ __launch_bounds__(64) void k_join_bf16(
     const float* arow = rows + (size_t)(rt * 16 + lr) * RS;
     const bf16x8 a0 = centred_bf16x8(arow, xi, lk, d, RS);
     const bf16x8 a1 = centred_bf16x8(arow, xi, 32 + lk, d, RS);
+    // the 4 rows of the tile this lane's outputs belong to: id, norm, slack * threshold
+    uint32_t rid[4];
+    float rnrm[4], rthr[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const uint32_t r = min(rt * 16 + (l / 16) * 4 + v, R - 1);
+      rid[v] = s_id[r];
+      rnrm[v] = s_nrm[r];
+      rthr[v] = slack * s_thr[r];
+    }
     for (uint32_t ct = 0; ct < ctiles; ++ct) {
       // skip tiles entirely below the new x new diagonal
       if (ct * 16 + 15 < rt * 16 && ct * 16 + 15 < nn) continue;
@@ -164,15 +174,17 @@ __global__ __launch_bounds__(64) void k_join_bf16(
       acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, centred_bf16x8(brow, xi, 32 + lk, d, RS), acc, 0,
                                                     0, 0);
       const uint32_t c = ct * 16 + lr;
+      const uint32_t cid = s_id[c];
+      const float cnrm = s_nrm[c], cthr = slack * s_thr[c];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const uint32_t r = rt * 16 + (l / 16) * 4 + v;
         bool keep = false;
-        if (r < nn && c < nc && ((c < nn) ? (c > r) : (s_id[c] != s_id[r]))) {
+        if (r < nn && c < nc && ((c < nn) ? (c > r) : (cid != rid[v]))) {
           ++pairs;
-          float dt = s_nrm[r] + s_nrm[c] - 2.0f * acc[v];
+          float dt = rnrm[v] + cnrm - 2.0f * acc[v];
           dt = dt > 0.0f ? dt : 0.0f;
-          keep = dt < slack * s_thr[r] || dt < slack * s_thr[c];
+          keep = dt < rthr[v] || dt < cthr;
         }
         const uint64_t mask = __ballot(keep);
         if (keep) sv[scnt + (uint32_t)__popcll(mask & ((1ull << l) - 1ull))] = r | (c << 16);
