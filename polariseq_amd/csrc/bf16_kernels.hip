@@ -27,28 +27,34 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr uint32_t SV_CAP = 320;  // screened pairs held: < 64 between tiles + 256 per tile
 
-// 8 centred dimensions [t0, t0 + 8) of an LDS row, rounded to BF16 (0 past d).
+// 8 centred dimensions [t0, t0 + 8) of an LDS row, rounded to BF16 (0 past d),
+// with x_i's 8 dimensions in registers (xr, from load8).
 // t0 is a multiple of 8 and rows are 16-byte aligned: two 16-byte reads per
 // operand, so the 16 rows of a tile, 52 floats apart, fill all 64 banks.
 // Reads stop at the row's padded stride rs (>= the last dimension read).
-__device__ inline bf16x8 centred_bf16x8(const float* row, const float* xi, uint32_t t0, uint32_t d,
-                                        uint32_t rs) {
-  float r[8], x[8];
+__device__ inline bf16x8 centred_bf16x8_r(const float* row, const float (&xr)[8], uint32_t t0,
+                                          uint32_t d, uint32_t rs) {
+  float r[8];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const uint32_t t = t0 + 4 * h;
-    float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
-    if (t < rs) {
-      a = *reinterpret_cast<const float4*>(row + t);
-      b = *reinterpret_cast<const float4*>(xi + t);
-    }
+    float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (t < rs) a = *reinterpret_cast<const float4*>(row + t);
     r[4 * h] = a.x; r[4 * h + 1] = a.y; r[4 * h + 2] = a.z; r[4 * h + 3] = a.w;
-    x[4 * h] = b.x; x[4 * h + 1] = b.y; x[4 * h + 2] = b.z; x[4 * h + 3] = b.w;
   }
   bf16x8 v;
 #pragma unroll
-  for (int u = 0; u < 8; ++u) v[u] = (__bf16)(t0 + u < d ? r[u] - x[u] : 0.0f);
+  for (int u = 0; u < 8; ++u) v[u] = (__bf16)(t0 + u < d ? r[u] - xr[u] : 0.0f);
   return v;
+}
+__device__ inline void load8(const float* p, uint32_t t0, uint32_t rs, float (&out)[8]) {
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const uint32_t t = t0 + 4 * h;
+    float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (t < rs) a = *reinterpret_cast<const float4*>(p + t);
+    out[4 * h] = a.x; out[4 * h + 1] = a.y; out[4 * h + 2] = a.z; out[4 * h + 3] = a.w;
+  }
 }
 
 // One wave (one block) per cell. LDS: pb_key[PB_CAP] u64 | rows[R + 1][RS] f32
@@ -151,10 +157,14 @@ __global__ __launch_bounds__(64) void k_join_bf16(
 
   const uint32_t rtiles = (nn + 15) / 16, ctiles = (nc + 15) / 16;
   const uint32_t lr = l % 16, lk = (l / 16) * 8;
+  // this lane's dimensions of x_i for both MFMA steps, read once
+  float xr0[8], xr1[8];
+  load8(xi, lk, RS, xr0);
+  load8(xi, 32 + lk, RS, xr1);
   for (uint32_t rt = 0; rt < rtiles; ++rt) {
     const float* arow = rows + (size_t)(rt * 16 + lr) * RS;
-    const bf16x8 a0 = centred_bf16x8(arow, xi, lk, d, RS);
-    const bf16x8 a1 = centred_bf16x8(arow, xi, 32 + lk, d, RS);
+    const bf16x8 a0 = centred_bf16x8_r(arow, xr0, lk, d, RS);
+    const bf16x8 a1 = centred_bf16x8_r(arow, xr1, 32 + lk, d, RS);
     // the 4 rows of the tile this lane's outputs belong to: id, norm, slack * threshold
     uint32_t rid[4];
     float rnrm[4], rthr[4];
@@ -170,9 +180,10 @@ __global__ __launch_bounds__(64) void k_join_bf16(
       if (ct * 16 + 15 < rt * 16 && ct * 16 + 15 < nn) continue;
       const float* brow = rows + (size_t)(ct * 16 + lr) * RS;
       f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, centred_bf16x8(brow, xi, lk, d, RS), acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, centred_bf16x8(brow, xi, 32 + lk, d, RS), acc, 0,
-                                                    0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, centred_bf16x8_r(brow, xr0, lk, d, RS), acc, 0, 0,
+                                                    0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, centred_bf16x8_r(brow, xr1, 32 + lk, d, RS), acc,
+                                                    0, 0, 0);
       const uint32_t c = ct * 16 + lr;
       const uint32_t cid = s_id[c];
       const float cnrm = s_nrm[c], cthr = slack * s_thr[c];
