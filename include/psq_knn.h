@@ -95,7 +95,9 @@ int psq_nndescent_device(const float* d_data, uint64_t n_obs, uint64_t n_dim,
  * queries: u32 indices into data, or NULL for all n points (n_q must then be n).
  * Outputs idx_out u32[n_q][k] and dist_out f32[n_q][k] (Euclidean, from the FP64
  * squared distance). Rows with fewer than k other points are padded with index
- * 0xFFFFFFFF and distance +inf. */
+ * 0xFFFFFFFF and distance +inf. k <= 2048 (k > 64 covers Scrublet's
+ * k_adj = round(0.5 sqrt(n_cells)) * 3, doublets.rs:1083-1090); larger k returns
+ * PSQ_E_UNSUPPORTED. */
 int psq_knn_exact(const float* data, uint64_t n, uint64_t d, const uint32_t* queries, uint64_t n_q,
                   uint64_t k, uint32_t* idx_out, float* dist_out, int32_t device);
 int psq_knn_exact_device(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d_queries,
