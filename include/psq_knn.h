@@ -136,6 +136,17 @@ int psq_neighbors_exact(const float* embedding, uint64_t n_obs, uint64_t n_dim,
                         uint64_t n_neighbors, int32_t metric, uint32_t* indptr, uint32_t* indices,
                         float* dists, uint64_t cap, uint64_t* nnz_out, int32_t device);
 
+/* pp.neighbors' cosine metric on the approximate path
+ * (python/polariseq/__init__.py:2540-2552): NN-Descent searches unit rows, on which
+ * Euclidean order is cosine order, and the distances become 1 - cos.
+ * psq_unit_rows_device: d_out[i] = d_x[i] / max(|d_x[i]|, 1e-12), bit for bit
+ * as numpy's float32 `emb / np.maximum(np.linalg.norm(emb, axis=1, keepdims=True),
+ * 1e-12)` (numpy's pairwise summation order for the norm); n_dim <= 256.
+ * psq_cosine_distances_device: d_dist[e] = float32(float64(d_dist[e])^2 / 2), as
+ * numpy's `(d.astype(np.float64) ** 2 / 2.0).astype(np.float32)`. In place. */
+int psq_unit_rows_device(const float* d_x, uint64_t n, uint64_t d, float* d_out, void* stream);
+int psq_cosine_distances_device(float* d_dist, uint64_t count, void* stream);
+
 /* ── learned encoder (no reference counterpart; used by PSQ_INIT_LEARNED) ──
  * A 50 -> 128 -> 32 MLP (GELU, BF16-rounded weights, FP32 accumulation),
  * trained with a triplet margin loss on exact-kNN triplets of a subset. */

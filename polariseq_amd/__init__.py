@@ -5,7 +5,8 @@ kernels for gfx950 behind a C-ABI (include/psq_knn.h, lib/libpsq_knn.so).
 The Python functions mirror the reference binding's ``nndescent_dense`` and
 ``neighbors_dense`` (crates/polariseq-python/src/lib.rs:471-566), and its
 ``fuzzy_connectivities`` / ``fuzzy_connectivities_into`` (lib.rs:568-642), the
-graph step's consumer inside ``pp.neighbors``.
+graph step's consumer inside ``pp.neighbors``; ``neighbors_graph`` is the graph
+computation of ``pp.neighbors`` itself (python/polariseq/__init__.py:2514-2600).
 """
 
 from .neighbors import (  # noqa: F401
@@ -28,5 +29,10 @@ from .fuzzy import (  # noqa: F401,E402
     fuzzy_connectivities_into,
 )
 from .encoder import Encoder  # noqa: F401,E402
+from .graph import (  # noqa: F401,E402
+    cosine_distances_device,
+    neighbors_graph,
+    unit_rows_device,
+)
 
 __version__ = "0.1.0"

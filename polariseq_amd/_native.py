@@ -49,6 +49,8 @@ EXPORTS = (
     "psq_fuzzy_plan_new",
     "psq_fuzzy_plan_fill",
     "psq_fuzzy_plan_free",
+    "psq_unit_rows_device",
+    "psq_cosine_distances_device",
 )
 
 
@@ -216,6 +218,10 @@ def _declare(L: ctypes.CDLL) -> None:
     L.psq_fuzzy_plan_fill.argtypes = [P, P, U64, P, P, U64]
     L.psq_fuzzy_plan_free.restype = None
     L.psq_fuzzy_plan_free.argtypes = [P]
+    L.psq_unit_rows_device.restype = ctypes.c_int
+    L.psq_unit_rows_device.argtypes = [P, U64, U64, P, P]
+    L.psq_cosine_distances_device.restype = ctypes.c_int
+    L.psq_cosine_distances_device.argtypes = [P, U64, P]
     L.psq_encoder_info.restype = ctypes.c_int
     L.psq_encoder_info.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                    ctypes.POINTER(U64)]

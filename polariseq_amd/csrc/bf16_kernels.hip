@@ -25,7 +25,6 @@ namespace psq {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr uint32_t SV_CAP = 320;  // screened pairs held: < 64 between tiles + 256 per tile
 
 // 8 centred dimensions [t0, t0 + 8) of an LDS row, rounded to BF16 (0 past d),
 // with x_i's 8 dimensions in registers (xr, from load8).
@@ -58,7 +57,8 @@ __device__ inline void load8(const float* p, uint32_t t0, uint32_t rs, float (&o
 }
 
 // One wave (one block) per cell. LDS: pb_key[PB_CAP] u64 | rows[R + 1][RS] f32
-// (row R is x_i) | ids[R] | thr[R] | nrm[R] | pb_row[PB_CAP] | sv[SV_CAP],
+// (row R is x_i) | ids[R] | thr[R] | nrm[R] | pb_row[PB_CAP] | sv[320] (screened
+// pairs: < 64 held between tiles + 256 per tile; the host sizes it),
 // R = 2m rounded up to 16. d <= 64 (two MFMA steps of depth 32).
 template <int D>
 __global__ __launch_bounds__(64) void k_join_bf16(

@@ -1269,6 +1269,29 @@ int psq_knn_exact(const float* data, uint64_t n, uint64_t d, const uint32_t* que
   return PSQ_OK;
 }
 
+// pp.neighbors' cosine metric on the approximate path (python/polariseq/__init__.py:2540-2552).
+int psq_unit_rows_device(const float* d_x, uint64_t n, uint64_t d, float* d_out, void* stream) {
+  if (n == 0) return PSQ_OK;
+  if (!d_x || !d_out) return fail(PSQ_E_INVALID, "NULL pointer argument");
+  if (d == 0) return fail(PSQ_E_INVALID, "n_dim must be >= 1");
+  if (d > ROWS_D_MAX) return fail(PSQ_E_UNSUPPORTED, "n_dim > %u is not supported", ROWS_D_MAX);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_unit_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_x, n,
+                     (uint32_t)d, d_out);
+  HIPC(hipGetLastError());
+  return PSQ_OK;
+}
+
+int psq_cosine_distances_device(float* d_dist, uint64_t count, void* stream) {
+  if (count == 0) return PSQ_OK;
+  if (!d_dist) return fail(PSQ_E_INVALID, "NULL pointer argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_half_square, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, d_dist,
+                     count);
+  HIPC(hipGetLastError());
+  return PSQ_OK;
+}
+
 int psq_neighbors_exact(const float* embedding, uint64_t n_obs, uint64_t n_dim,
                         uint64_t n_neighbors, int32_t metric, uint32_t* indptr, uint32_t* indices,
                         float* dists, uint64_t cap, uint64_t* nnz_out, int32_t device) {

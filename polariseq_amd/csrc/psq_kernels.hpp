@@ -97,6 +97,12 @@ constexpr int MLP_N_PARAMS = 128 * 50 + 128 + 32 * 128 + 32;
 __global__ void k_mlp_round(const float* w, float* wr, uint32_t count);
 __global__ void k_mlp_forward(const float* X, uint32_t ldx, const uint32_t* idx, uint32_t rows,
                               const float* pr, float mu, float inv_scale, float* hpre_out, float* z_out);
+// pp.neighbors' cosine metric on the approximate path: unit rows as numpy
+// scales them, and 1 - cos distances from the unit-row Euclidean ones.
+constexpr uint32_t ROWS_D_MAX = 256;
+__global__ void k_unit_rows(const float* x, uint64_t n, uint32_t d, float* out);
+__global__ void k_half_square(float* dist, uint64_t count);
+
 // Encode on the FP32 matrix cores: operands packed per lane (ENC_PACKED floats),
 // ENC_WAVES wave-tiles of 16 rows per block.
 constexpr int ENC_PACKED = 10752;
