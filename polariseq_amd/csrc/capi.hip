@@ -988,7 +988,15 @@ int knn_exact_big(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d
   Workspace* ws;
   TRY(workspace(&ws));
   const uint32_t dp = round_up((uint32_t)d, 32);
-  uint32_t nseg = (uint32_t)std::min<uint64_t>((n + 63) / 64, std::min<uint64_t>(256, (k + 15) / 16 * 2));
+  // Pass 1 keeps the K smallest of each segment. With many points per segment
+  // (n >= 32 k) the 32 smallest of 4 ceil(k / 16) segments (2k values or more)
+  // bound the k-th as well as the 64 smallest of half as many, at half the
+  // insertion network; fewer points keep 64 per segment so that the union of
+  // the lists still holds k values.
+  const bool k32 = n >= 32 * k;
+  const uint32_t K = k32 ? 32u : 64u;
+  uint32_t nseg = k32 ? (uint32_t)std::min<uint64_t>(512, (k + 15) / 16 * 4)
+                      : (uint32_t)std::min<uint64_t>((n + 63) / 64, std::min<uint64_t>(256, (k + 15) / 16 * 2));
   nseg = std::max<uint32_t>(nseg, 1);
   const uint32_t seg_len = round_up((uint32_t)((n + nseg - 1) / nseg), 64);
   nseg = (uint32_t)((n + seg_len - 1) / seg_len);
@@ -996,13 +1004,13 @@ int knn_exact_big(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d
   uint32_t pmax = 1;
   while (pmax < cap) pmax <<= 1;
   // chunk of queries: pass-1 lists and candidates within about 4 GiB
-  const uint64_t per_q = (uint64_t)nseg * 64 * 4 + (uint64_t)cap * 4 + 16;
+  const uint64_t per_q = (uint64_t)nseg * K * 4 + (uint64_t)cap * 4 + 16;
   const uint32_t qc = (uint32_t)std::max<uint64_t>(
       EX_BIG_QMIN, std::min<uint64_t>(nq, (4ull << 30) / per_q / 256 * 256));
   float *Xp, *partial, *T;
   uint32_t *cnt, *cand, *ovf, *novf, *qidx = nullptr;
   TRY(wbuf(ws, "Xp", (size_t)n * dp, &Xp));
-  TRY(wbuf(ws, "exb_partial", (size_t)qc * nseg * 64, &partial));
+  TRY(wbuf(ws, "exb_partial", (size_t)qc * nseg * K, &partial));
   TRY(wbuf(ws, "ex_T", qc, &T));
   TRY(wbuf(ws, "ex_cnt", qc, &cnt));
   TRY(wbuf(ws, "exb_cand", (size_t)qc * cap, &cand));
@@ -1015,14 +1023,16 @@ int knn_exact_big(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d
   const size_t smem = (size_t)64 * dp * sizeof(float);
   const uint32_t dfix = d == 50 ? 50u : (d == 30 ? 30u : 0u);
   void (*p1)(const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t, uint32_t,
-             uint32_t, float*) = dfix == 50 ? k_exact_pass1<50, 64> : (dfix == 30 ? k_exact_pass1<30, 64> : k_exact_pass1<0, 64>);
+             uint32_t, float*) =
+      k32 ? (dfix == 50 ? k_exact_pass1<50, 32> : (dfix == 30 ? k_exact_pass1<30, 32> : k_exact_pass1<0, 32>))
+          : (dfix == 50 ? k_exact_pass1<50, 64> : (dfix == 30 ? k_exact_pass1<30, 64> : k_exact_pass1<0, 64>));
   void (*p2)(const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t, uint32_t,
              const float*, uint32_t*, uint32_t*, uint32_t) = dfix == 50 ? k_exact_pass2<50> : (dfix == 30 ? k_exact_pass2<30> : k_exact_pass2<0>);
   if (smem > 65536) {
     HIPC(hipFuncSetAttribute((const void*)p1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     HIPC(hipFuncSetAttribute((const void*)p2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
   }
-  const size_t tau_smem = (size_t)nseg * 64 * 4;
+  const size_t tau_smem = (size_t)nseg * K * 4;
   if (tau_smem > 65536)
     HIPC(hipFuncSetAttribute((const void*)k_exact_tau_big, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)tau_smem));
@@ -1041,7 +1051,7 @@ int knn_exact_big(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d
     hipLaunchKernelGGL(p1, dim3(qblocks, nseg), dim3(256), smem, s, Xp, (uint32_t)n, (uint32_t)d, dp,
                        qp, qn, seg_len, nseg, partial);
     HIPC(hipGetLastError());
-    hipLaunchKernelGGL(k_exact_tau_big, dim3(qn), dim3(256), tau_smem, s, partial, qn, nseg,
+    hipLaunchKernelGGL(k_exact_tau_big, dim3(qn), dim3(256), tau_smem, s, partial, qn, nseg, K,
                        (uint32_t)k, (uint32_t)d, T);
     HIPC(hipGetLastError());
     TRY(fill_u32(cnt, qn, 0, s));

@@ -342,17 +342,18 @@ __global__ void k_exact_fallback(const float* __restrict__ Xp, uint32_t n, uint3
 // threshold built from it keeps the proof of pass 2. The selection and the
 // re-rank run per query in LDS.
 
-// tau: the k-th smallest of a query's nseg * 64 pass-1 values, by radix select
+// tau: the k-th smallest of a query's nseg * K pass-1 values, by radix select
 // on the FP32 bits (four 8-bit digits; the values are >= 0, so bit order is
-// value order). One block per query; dynamic LDS: nseg * 64 values.
+// value order). One block per query; dynamic LDS: nseg * K values.
 __global__ __launch_bounds__(256) void k_exact_tau_big(const float* __restrict__ partial,
-                                                       uint32_t nq, uint32_t nseg, uint32_t k,
-                                                       uint32_t d, float* __restrict__ T) {
+                                                       uint32_t nq, uint32_t nseg, uint32_t K,
+                                                       uint32_t k, uint32_t d,
+                                                       float* __restrict__ T) {
   extern __shared__ uint32_t sv[];
   __shared__ uint32_t hist[256];
   __shared__ uint32_t s_prefix, s_mask, s_rank;
   const uint32_t q = blockIdx.x, tid = threadIdx.x;
-  const uint32_t L = nseg * 64;
+  const uint32_t L = nseg * K;
   const float* src = partial + (size_t)q * L;
   for (uint32_t e = tid; e < L; e += blockDim.x) sv[e] = __float_as_uint(src[e]);
   if (tid == 0) {

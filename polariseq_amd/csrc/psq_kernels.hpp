@@ -139,7 +139,7 @@ __global__ void k_exact_rerank(const float* Xp, uint32_t n, uint32_t d, uint32_t
                                uint32_t* idx_out, float* dist_out, uint32_t* overflow,
                                uint32_t* n_overflow);
 constexpr uint32_t EX_KBIG_MAX = 2048;  // largest k of the exact path
-__global__ void k_exact_tau_big(const float* partial, uint32_t nq, uint32_t nseg, uint32_t k,
+__global__ void k_exact_tau_big(const float* partial, uint32_t nq, uint32_t nseg, uint32_t K, uint32_t k,
                                 uint32_t d, float* T);
 __global__ void k_exact_rerank_big(const float* Xp, uint32_t n, uint32_t d, uint32_t dp,
                                    const uint32_t* queries, uint32_t nq, uint32_t k,

@@ -114,7 +114,9 @@ def test_100k_subset_of_the_1p3m_workload(orc):
 # ── large k (64 < k <= 2048) ──
 
 @pytest.mark.parametrize("n,d,k", [(3000, 50, 65), (3000, 50, 474), (2500, 20, 1161),
-                                   (2300, 50, 2048), (900, 7, 300)])
+                                   (2300, 50, 2048), (900, 7, 300),
+                                   # n >= 32 k: pass 1 keeps 32 per segment
+                                   (8000, 30, 100), (20000, 30, 474)])
 def test_large_k_bit_exact(orc, n, d, k):
     rng = np.random.default_rng(n + k)
     x = ds.mixture(n, d, 6, seed=k) if d >= 8 else rng.random((n, d), dtype=np.float32)
@@ -137,6 +139,15 @@ def test_large_k_scrublet_shape_all_rows(orc):
     assert k > 64
     print(f"exact scrublet shape k={k}", flush=True)
     _check(x, k, None, orc)
+
+
+def test_large_k_32_per_segment_with_duplicates(orc):
+    """n >= 32 k (32 smallest per segment in pass 1) with 600 copies of one
+    point: ties beyond the candidate buffer go to the fallback."""
+    y = np.random.default_rng(3).random((7000, 30), dtype=np.float32)
+    y[1000:1600] = y[1000]
+    print("exact large k, 32 per segment, duplicates", flush=True)
+    _check(y, 120, np.array([1000, 1001, 1599, 0, 6999, 3500], np.uint32), orc)
 
 
 def test_large_k_fewer_points_and_overflow(orc):
