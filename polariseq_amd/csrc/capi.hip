@@ -988,15 +988,15 @@ int knn_exact_big(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d
   Workspace* ws;
   TRY(workspace(&ws));
   const uint32_t dp = round_up((uint32_t)d, 32);
-  // Pass 1 keeps the K smallest of each segment. With many points per segment
-  // (n >= 32 k) the 32 smallest of 4 ceil(k / 16) segments (2k values or more)
-  // bound the k-th as well as the 64 smallest of half as many, at half the
-  // insertion network; fewer points keep 64 per segment so that the union of
-  // the lists still holds k values.
-  const bool k32 = n >= 32 * k;
-  const uint32_t K = k32 ? 32u : 64u;
-  uint32_t nseg = k32 ? (uint32_t)std::min<uint64_t>(512, (k + 15) / 16 * 4)
-                      : (uint32_t)std::min<uint64_t>((n + 63) / 64, std::min<uint64_t>(256, (k + 15) / 16 * 2));
+  // Pass 1 keeps the K smallest of each segment, over 128 ceil(k / 16) / K
+  // segments (about 8k values in all). With many points per segment (n >= 4 K k
+  // for K = 16 or 32) the shorter lists bound the k-th as well as 64 per
+  // segment, with a shorter insertion network and fewer insertions; fewer
+  // points keep 64 per segment (and at most one segment per 64 points) so that
+  // the union of the lists still holds k values.
+  const uint32_t K = n >= 64 * k ? 16u : (n >= 32 * k ? 32u : 64u);
+  uint32_t nseg = (uint32_t)((k + 15) / 16 * 128 / K);
+  if (K == 64) nseg = (uint32_t)std::min<uint64_t>((n + 63) / 64, nseg);
   nseg = std::max<uint32_t>(nseg, 1);
   const uint32_t seg_len = round_up((uint32_t)((n + nseg - 1) / nseg), 64);
   nseg = (uint32_t)((n + seg_len - 1) / seg_len);
@@ -1024,8 +1024,9 @@ int knn_exact_big(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d
   const uint32_t dfix = d == 50 ? 50u : (d == 30 ? 30u : 0u);
   void (*p1)(const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t, uint32_t,
              uint32_t, float*) =
-      k32 ? (dfix == 50 ? k_exact_pass1<50, 32> : (dfix == 30 ? k_exact_pass1<30, 32> : k_exact_pass1<0, 32>))
-          : (dfix == 50 ? k_exact_pass1<50, 64> : (dfix == 30 ? k_exact_pass1<30, 64> : k_exact_pass1<0, 64>));
+      K == 16   ? (dfix == 50 ? k_exact_pass1<50, 16> : (dfix == 30 ? k_exact_pass1<30, 16> : k_exact_pass1<0, 16>))
+      : K == 32 ? (dfix == 50 ? k_exact_pass1<50, 32> : (dfix == 30 ? k_exact_pass1<30, 32> : k_exact_pass1<0, 32>))
+                : (dfix == 50 ? k_exact_pass1<50, 64> : (dfix == 30 ? k_exact_pass1<30, 64> : k_exact_pass1<0, 64>));
   void (*p2)(const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t, uint32_t,
              const float*, uint32_t*, uint32_t*, uint32_t) = dfix == 50 ? k_exact_pass2<50> : (dfix == 30 ? k_exact_pass2<30> : k_exact_pass2<0>);
   if (smem > 65536) {

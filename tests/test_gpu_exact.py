@@ -115,7 +115,7 @@ def test_100k_subset_of_the_1p3m_workload(orc):
 
 @pytest.mark.parametrize("n,d,k", [(3000, 50, 65), (3000, 50, 474), (2500, 20, 1161),
                                    (2300, 50, 2048), (900, 7, 300),
-                                   # n >= 32 k: pass 1 keeps 32 per segment
+                                   # n >= 64 k / 32 k: pass 1 keeps 16 / 32 per segment
                                    (8000, 30, 100), (20000, 30, 474)])
 def test_large_k_bit_exact(orc, n, d, k):
     rng = np.random.default_rng(n + k)
@@ -141,13 +141,14 @@ def test_large_k_scrublet_shape_all_rows(orc):
     _check(x, k, None, orc)
 
 
-def test_large_k_32_per_segment_with_duplicates(orc):
-    """n >= 32 k (32 smallest per segment in pass 1) with 600 copies of one
-    point: ties beyond the candidate buffer go to the fallback."""
-    y = np.random.default_rng(3).random((7000, 30), dtype=np.float32)
+@pytest.mark.parametrize("n", [7000, 16000])
+def test_large_k_short_lists_with_duplicates(orc, n):
+    """n >= 32 k (32 smallest per segment in pass 1) and n >= 64 k (16) with 600
+    copies of one point: ties beyond the candidate buffer go to the fallback."""
+    y = np.random.default_rng(3).random((n, 30), dtype=np.float32)
     y[1000:1600] = y[1000]
-    print("exact large k, 32 per segment, duplicates", flush=True)
-    _check(y, 120, np.array([1000, 1001, 1599, 0, 6999, 3500], np.uint32), orc)
+    print(f"exact large k, short lists, duplicates, n={n}", flush=True)
+    _check(y, 120, np.array([1000, 1001, 1599, 0, n - 1, 3500], np.uint32), orc)
 
 
 def test_large_k_fewer_points_and_overflow(orc):
