@@ -108,6 +108,17 @@ void make_plan(const std::vector<uint64_t>& off, const Tile& t, uint32_t K, Plan
   }
 }
 
+// Persistent grid: as many blocks of `kern` as are resident on every CU at once
+// (registers and LDS permitting), so each CU runs the most waves it can.
+int occupancy_grid(const void* kern, unsigned block, size_t smem, unsigned* grid) {
+  int dev = 0, cus = 0, per_cu = 0;
+  BHIPC(hipGetDevice(&dev));
+  BHIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  BHIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, (int)block, smem));
+  *grid = (unsigned)cus * (unsigned)std::max(per_cu, 1);
+  return PSQ_OK;
+}
+
 template <int D, int K>
 int run_pass1(const float* Xp, uint32_t d, uint32_t dp, const uint64_t* d_off, const uint32_t* d_units,
               uint32_t U, const Tile& t, const uint32_t* d_nseg, const uint64_t* d_poff,
@@ -116,6 +127,7 @@ int run_pass1(const float* Xp, uint32_t d, uint32_t dp, const uint64_t* d_off, c
   auto kern = k_bexact_pass1<D, K>;
   if (smem > 65536)
     BHIPC(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+  BTRY(occupancy_grid((const void*)kern, t.block, smem, &grid));
   BHIPC(hipMemsetAsync(d_counter, 0, 4, s));
   hipLaunchKernelGGL(kern, dim3(grid), dim3(t.block), smem, s, Xp, d, dp, d_off, d_units, U, t.seg_len,
                      t.tile, d_nseg, d_poff, d_counter, d_partial);
@@ -133,14 +145,6 @@ int dispatch_pass1(uint32_t K, const float* Xp, uint32_t d, uint32_t dp, const u
   return run_pass1<D, 64>(Xp, d, dp, d_off, d_units, U, t, d_nseg, d_poff, d_counter, d_partial, grid, s);
 }
 
-int device_grid(unsigned block, unsigned* grid) {
-  int dev = 0, cus = 0;
-  BHIPC(hipGetDevice(&dev));
-  BHIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  // two resident blocks of 256 lanes per CU keep 8 waves per CU busy
-  *grid = (unsigned)cus * (block >= 256 ? 2u : (block >= 128 ? 4u : 8u));
-  return PSQ_OK;
-}
 
 }  // namespace
 
@@ -244,8 +248,7 @@ extern "C" int psq_knn_exact_batched_device(const float* d_data, uint64_t n_tota
       BHIPC(hipMemcpyAsync(d_units2, P.units.data(), P.units.size() * 4, hipMemcpyHostToDevice, s));
       BHIPC(hipMemcpyAsync(d_nseg2, P.nseg.data(), 4, hipMemcpyHostToDevice, s));
       BHIPC(hipMemcpyAsync(d_pofs, P.part_off.data(), 8, hipMemcpyHostToDevice, s));
-      unsigned grid;
-      BTRY(device_grid(t.block, &grid));
+      unsigned grid = 0;  // sized per kernel at its launch
       BHIPC(hipEventRecord(a, s));
       if (dfix == 50)
         BTRY(dispatch_pass1<50>(K, Xprobe, (uint32_t)d, dp, d_poff2, d_units2, U, t, d_nseg2, d_pofs, counter, d_part2, grid, s));
@@ -285,8 +288,7 @@ extern "C" int psq_knn_exact_batched_device(const float* d_data, uint64_t n_tota
   BHIPC(hipMemcpyAsync(d_units, P.units.data(), P.units.size() * 4, hipMemcpyHostToDevice, s));
   BHIPC(hipMemcpyAsync(d_nseg, P.nseg.data(), n_samples * 4, hipMemcpyHostToDevice, s));
   BHIPC(hipMemcpyAsync(d_poff, P.part_off.data(), n_samples * 8, hipMemcpyHostToDevice, s));
-  unsigned grid;
-  BTRY(device_grid(best.block, &grid));
+  unsigned grid = 0;  // sized per kernel at its launch
   hipEvent_t e0, e1, e2, e3;
   BHIPC(hipEventCreate(&e0));
   BHIPC(hipEventCreate(&e1));
@@ -308,6 +310,7 @@ extern "C" int psq_knn_exact_batched_device(const float* d_data, uint64_t n_tota
     auto kern = dfix == 50 ? k_bexact_pass2<50> : (dfix == 30 ? k_bexact_pass2<30> : k_bexact_pass2<0>);
     if (smem > 65536)
       BHIPC(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    BTRY(occupancy_grid((const void*)kern, best.block, smem, &grid));
     BHIPC(hipMemsetAsync(counter, 0, 4, s));
     hipLaunchKernelGGL(kern, dim3(grid), dim3(best.block), smem, s, Xp, (uint32_t)d, dp, d_off, d_units, U,
                        best.seg_len, best.tile, T, counter, cnt, cand, cap);
