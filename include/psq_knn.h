@@ -147,6 +147,43 @@ int psq_neighbors_exact(const float* embedding, uint64_t n_obs, uint64_t n_dim,
 int psq_unit_rows_device(const float* d_x, uint64_t n, uint64_t d, float* d_out, void* stream);
 int psq_cosine_distances_device(float* d_dist, uint64_t count, void* stream);
 
+/* PCA of a CSR matrix (n_obs x n_features; indptr u64[n_obs + 1], indices u32,
+ * data f32) by the exact Gram route. Replaces pca_csr_into / pca_scaled_blocks_into
+ * (crates/polariseq-core/src/pca.rs:255-290, 396-431) as bound by pca_sparse and
+ * pca_sparse_scaled (crates/polariseq-python/src/lib.rs:395-467).
+ * scale = 0: centre when center != 0 (implicitly; the matrix is never changed).
+ * scale = 1: the pp.scale view (ddof-1 sd, 0 -> 1, values clipped to
+ *   +-max_value; max_value = +inf for none, else >= 0), always centred.
+ * Outputs: embedding f32[n_obs][n_components] (columns signed so that each
+ * column's first largest-magnitude entry is positive), variance_ratio
+ * f32[n_components] = max(eigenvalue, 0) / trace of the centred Gram, and, if
+ * components is not NULL, the principal axes f32[n_features][n_components].
+ * n_features > 4096 (where the reference uses randomized SVD) returns
+ * PSQ_E_UNSUPPORTED; n_obs or n_features 0, or n_components 0 or above
+ * min(n_obs, n_features), PSQ_E_INVALID with the reference's message. A run
+ * repeats bit for bit. psq_pca_csr_device takes device pointers and runs on
+ * the given stream of the current device. */
+int psq_pca_csr(const uint64_t* indptr, const uint32_t* indices, const float* data, uint64_t n_obs,
+                uint64_t n_features, uint64_t n_components, int32_t center, int32_t scale,
+                double max_value, float* embedding, float* variance_ratio, float* components,
+                int32_t device);
+int psq_pca_csr_device(const uint64_t* d_indptr, const uint32_t* d_indices, const float* d_data,
+                       uint64_t n_obs, uint64_t n_features, uint64_t n_components, int32_t center,
+                       int32_t scale, double max_value, float* d_embedding, float* d_variance_ratio,
+                       float* d_components, void* stream);
+/* Phases of this thread's last successful PCA call, from HIP events on its
+ * stream: the scaled view's statistics and values, the Gram pass (densified row
+ * chunks, column sums and SYRK; ms_syrk is the SYRK launches alone), the
+ * eigensolver, and projection + signs + ratios. */
+typedef struct psq_pca_report {
+  uint64_t iterations; /* subspace steps */
+  uint64_t block;      /* subspace block b */
+  uint64_t chunks;     /* dense row chunks of the Gram pass */
+  uint64_t rows_per_chunk;
+  double ms_scale, ms_gram, ms_syrk, ms_eigen, ms_project, ms_total;
+} psq_pca_report;
+int psq_pca_last_report(psq_pca_report* out);
+
 /* ── learned encoder (no reference counterpart; used by PSQ_INIT_LEARNED) ──
  * A 50 -> 128 -> 32 MLP (GELU, BF16-rounded weights, FP32 accumulation),
  * trained with a triplet margin loss on exact-kNN triplets of a subset. */

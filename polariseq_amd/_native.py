@@ -51,6 +51,9 @@ EXPORTS = (
     "psq_fuzzy_plan_free",
     "psq_unit_rows_device",
     "psq_cosine_distances_device",
+    "psq_pca_csr",
+    "psq_pca_csr_device",
+    "psq_pca_last_report",
 )
 
 
@@ -65,6 +68,18 @@ class BatchedReport(ctypes.Structure):
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
+
+
+class PcaReport(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_uint64), ("block", ctypes.c_uint64), ("chunks", ctypes.c_uint64),
+        ("rows_per_chunk", ctypes.c_uint64), ("ms_scale", ctypes.c_double),
+        ("ms_gram", ctypes.c_double), ("ms_syrk", ctypes.c_double), ("ms_eigen", ctypes.c_double),
+        ("ms_project", ctypes.c_double), ("ms_total", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class EncoderTrainOpts(ctypes.Structure):
@@ -222,6 +237,13 @@ def _declare(L: ctypes.CDLL) -> None:
     L.psq_unit_rows_device.argtypes = [P, U64, U64, P, P]
     L.psq_cosine_distances_device.restype = ctypes.c_int
     L.psq_cosine_distances_device.argtypes = [P, U64, P]
+    D = ctypes.c_double
+    L.psq_pca_csr.restype = ctypes.c_int
+    L.psq_pca_csr.argtypes = [P, P, P, U64, U64, U64, I32, I32, D, P, P, P, I32]
+    L.psq_pca_csr_device.restype = ctypes.c_int
+    L.psq_pca_csr_device.argtypes = [P, P, P, U64, U64, U64, I32, I32, D, P, P, P, P]
+    L.psq_pca_last_report.restype = ctypes.c_int
+    L.psq_pca_last_report.argtypes = [ctypes.POINTER(PcaReport)]
     L.psq_encoder_info.restype = ctypes.c_int
     L.psq_encoder_info.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                    ctypes.POINTER(U64)]

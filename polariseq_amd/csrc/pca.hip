@@ -1,0 +1,883 @@
+// PCA of a CSR matrix by the exact Gram method, on the GPU
+// (crates/polariseq-core/src/pca.rs:396-545, rsvd.rs:155-177, 241-283, 356-390, 439-456):
+//
+//   optional scaled view (pp.scale semantics, pca.rs:396-480): stored value x of
+//     gene j becomes clip((x - mu_j) / sd_j) - b_j, b_j = clip(-mu_j / sd_j), the
+//     gene statistics in f64 with ddof 1 (preprocess.rs:314-358); scaling centres;
+//   C = S^T S in f64, then C_ij -= (mu_i n) mu_j with mu the f32 column means and
+//     i >= j (the triangle the reference's solver reads), mirrored;
+//   the top k eigenpairs of C (f64); V = eigenvectors as f32 (row-major m x k);
+//   embedding = S V - 1 (mu^T V), f32, each row summed over its stored entries in
+//     order with separate multiply and add (rsvd.rs:241-283); every column
+//     flipped so that its first largest-magnitude entry is positive
+//     (rsvd.rs:439-456);
+//   variance ratio = max(lambda_j, 0) / trace(C).
+//
+// The reference takes the eigendecomposition with a dense solver; here the top
+// k come from subspace iteration on C + sigma I (sigma = 1e-4 trace(C) / m keeps
+// every block full rank; the eigenvectors are C's) with block b = k +
+// max(16, k / 2), orthonormalised each step by shifted CholeskyQR3 (Fukaya,
+// Kannan, Nakatsukasa, Yamamoto, Yanagisawa, SIAM J. Sci. Comput. 42 (2020)
+// A477), with a Rayleigh-Ritz step (cyclic Jacobi in round-robin order on the
+// b x b projection, one workgroup) every 4 steps, until every wanted residual
+// is below 1e-11 of the largest eigenvalue. The Gram of densified row chunks,
+// C Q and the small products are rocBLAS f64 calls with atomics disabled; every
+// other reduction has a fixed order, so a run repeats bit for bit.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "../../include/psq_knn.h"
+#include "psq_kernels.hpp"
+
+namespace psq_host {
+int fail(int code, const char* fmt, ...);
+}
+using psq_host::fail;
+
+#define PHIPC(call)                                                                                  \
+  do {                                                                                               \
+    hipError_t e_ = (call);                                                                          \
+    if (e_ != hipSuccess)                                                                            \
+      return fail(e_ == hipErrorOutOfMemory ? PSQ_E_NOMEM : PSQ_E_DEVICE, "%s failed: %s", #call, \
+                  hipGetErrorString(e_));                                                            \
+  } while (0)
+#define PBLAS(call)                                                                     \
+  do {                                                                                  \
+    rocblas_status s_ = (call);                                                         \
+    if (s_ != rocblas_status_success)                                                   \
+      return fail(PSQ_E_DEVICE, "%s failed: %s", #call, rocblas_status_to_string(s_)); \
+  } while (0)
+#define PTRY(expr)                 \
+  do {                             \
+    int rc_ = (expr);              \
+    if (rc_ != PSQ_OK) return rc_; \
+  } while (0)
+
+namespace psq {
+
+constexpr uint32_t PCA_GRAM_MAX_FEATURES = 4096;  // pca.rs:201
+constexpr uint32_t PCA_MAX_BLOCK = 1024;          // subspace block b
+constexpr uint32_t PCA_LDS_BLOCK = 80;            // b x b f64 (50 KB) kept in LDS
+constexpr uint32_t PCA_SEGMENTS = 64;             // row segments of the column sums
+
+// ── input check ──
+
+// bad |= 1 for a column index >= m, 2 for a decreasing indptr.
+__global__ void k_pca_check(const uint64_t* __restrict__ indptr, uint64_t n,
+                            const uint32_t* __restrict__ indices, uint64_t nnz, uint32_t m,
+                            uint32_t* __restrict__ bad) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nnz && indices[t] >= m) atomicOr(bad, 1u);
+  if (t < n && indptr[t + 1] < indptr[t]) atomicOr(bad, 2u);
+}
+
+// ── column sums and the scaled view ──
+
+// Rows [r0, r0 + R) as dense f64 rows of m, stored values added in order (the
+// chunk must be zero on entry). One thread per row.
+__global__ void k_pca_densify(const uint64_t* __restrict__ indptr, const uint32_t* __restrict__ indices,
+                              const float* __restrict__ val, uint64_t r0, uint32_t R, uint32_t m,
+                              double* __restrict__ D) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  double* row = D + (size_t)r * m;
+  for (uint64_t e = indptr[r0 + r]; e < indptr[r0 + r + 1]; ++e) row[indices[e]] += (double)val[e];
+}
+
+// Column sums (and sums of squares when ps_q is set) of a dense chunk, per row
+// segment: part[p][j] over rows [p seg, (p + 1) seg). Grid (ceil(m / 256), P).
+__global__ void k_pca_colsums_part(const double* __restrict__ D, uint32_t R, uint32_t m, uint32_t seg,
+                                   double* __restrict__ ps_s, double* __restrict__ ps_q) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t p = blockIdx.y;
+  if (j >= m) return;
+  const uint32_t lo = p * seg, hi = lo + seg < R ? lo + seg : R;
+  double a = 0.0, b = 0.0;
+  for (uint32_t r = lo; r < hi; ++r) {
+    const double x = D[(size_t)r * m + j];
+    a += x;
+    b += x * x;
+  }
+  ps_s[(size_t)p * m + j] = a;
+  if (ps_q) ps_q[(size_t)p * m + j] = b;
+}
+// acc[j] += the P partials in segment order.
+__global__ void k_pca_colsums_fold(const double* __restrict__ ps_s, const double* __restrict__ ps_q,
+                                   uint32_t P, uint32_t m, double* __restrict__ s, double* __restrict__ q) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  double a = 0.0, b = 0.0;
+  for (uint32_t p = 0; p < P; ++p) {
+    a += ps_s[(size_t)p * m + j];
+    if (ps_q) b += ps_q[(size_t)p * m + j];
+  }
+  s[j] += a;
+  if (q) q[j] += b;
+}
+
+// Gene statistics -> the scaled view's mean, 1/sd and clipped zero
+// (pca.rs:403-414; preprocess.rs:344-358: var = (q/n - mean^2) n/(n-1), >= 0).
+__global__ void k_pca_scale_stats(const double* __restrict__ s, const double* __restrict__ q, uint32_t m,
+                                  uint64_t n, double clip, double* __restrict__ mean,
+                                  double* __restrict__ inv_sd, double* __restrict__ zero) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const double nf = (double)n;
+  const double mu = s[j] / nf;
+  const double var = n < 2 ? 0.0 : fmax((q[j] / nf - mu * mu) * nf / (nf - 1.0), 0.0);
+  const double sd = sqrt(var);
+  const double inv = sd == 0.0 ? 1.0 : 1.0 / sd;
+  mean[j] = mu;
+  inv_sd[j] = inv;
+  zero[j] = fmin(fmax(-mu * inv, -clip), clip);
+}
+
+// Stored values of the scaled view (pca.rs:450-460).
+__global__ void k_pca_scale_values(const uint32_t* __restrict__ indices, const float* __restrict__ x,
+                                   uint64_t nnz, const double* __restrict__ mean,
+                                   const double* __restrict__ inv_sd, const double* __restrict__ zero,
+                                   double clip, float* __restrict__ out) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nnz) return;
+  const uint32_t j = indices[e];
+  const double z = fmin(fmax(((double)x[e] - mean[j]) * inv_sd[j], -clip), clip);
+  out[e] = (float)(z - zero[j]);
+}
+
+// ── Gram matrix ──
+
+__global__ void k_pca_means(const double* __restrict__ s, uint32_t m, uint64_t n, float* __restrict__ mu) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < m) mu[j] = (float)(s[j] / (double)n);
+}
+
+// C (column-major, upper triangle from SYRK) -> full centred matrix. Element
+// (row, col), row <= col, is the reference's lower-triangle element (col, row):
+// C - (mu_col n) mu_row.
+__global__ void k_pca_center(double* __restrict__ C, uint32_t m, uint64_t n, const float* __restrict__ mu) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (uint64_t)m * m) return;
+  const uint32_t col = (uint32_t)(t / m), row = (uint32_t)(t - (uint64_t)col * m);
+  if (row > col) return;
+  double v = C[t];
+  if (mu) v -= ((double)mu[col] * (double)n) * (double)mu[row];
+  C[t] = v;
+  C[(size_t)row * m + col] = v;
+}
+
+// trace(C) in index order (pca.rs:510) and sigma = 1e-4 trace / m.
+__global__ void k_pca_trace(const double* __restrict__ C, uint32_t m, double* __restrict__ out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double t = 0.0;
+  for (uint32_t i = 0; i < m; ++i) t += C[(size_t)i * m + i];
+  out[0] = t;
+  out[1] = 1e-4 * t / (double)m;
+}
+
+// ── eigensolver ──
+
+// Start block Q (column-major m x b), uniform in [-1, 1) from a hash of the
+// element index.
+__global__ void k_pca_start(double* __restrict__ Q, uint32_t m, uint32_t b) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (uint64_t)m * b) return;
+  Q[t] = (double)(splitmix(0x9E3779B97F4A7C15ull ^ splitmix(t + 0x5EEDull)) >> 11) * 0x1.0p-52 - 1.0;
+}
+
+// Z += sigma Q (the shifted operator), elementwise.
+__global__ void k_pca_shift(double* __restrict__ Z, const double* __restrict__ Q, uint64_t count,
+                            const double* __restrict__ tv) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) Z[t] += tv[1] * Q[t];
+}
+
+// One workgroup: upper Cholesky factor of the b x b matrix G (column-major,
+// upper triangle read) plus shift I, in place; the strict lower triangle is
+// zeroed. shifted: shift = 11 (m b + b (b + 1)) u trace(G) (the shifted
+// CholeskyQR bound, u = 2^-53). Pivots are floored at 1e-28 of the largest
+// diagonal (never reached for a full-rank block). With b <= PCA_LDS_BLOCK the
+// factorisation runs in LDS.
+__global__ __launch_bounds__(256) void k_pca_chol(double* __restrict__ G, uint32_t b, uint32_t m,
+                                                  int shifted) {
+  extern __shared__ double lds[];
+  __shared__ double s_piv, s_shift, s_floor;
+  const uint32_t tid = threadIdx.x;
+  const bool in_lds = b <= PCA_LDS_BLOCK;
+  double* A = in_lds ? lds : G;
+  if (in_lds)
+    for (uint32_t t = tid; t < b * b; t += blockDim.x) A[t] = G[t];
+  __syncthreads();
+  if (tid == 0) {
+    double tr = 0.0, mx = 0.0;
+    for (uint32_t i = 0; i < b; ++i) {
+      tr += A[(size_t)i * b + i];
+      mx = fmax(mx, A[(size_t)i * b + i]);
+    }
+    s_shift = shifted ? 11.0 * ((double)m * b + (double)b * (b + 1)) * 0x1.0p-53 * tr : 0.0;
+    s_floor = fmax(mx * 1e-28, 1e-290);
+  }
+  __syncthreads();
+  const double shift = s_shift, floor_ = s_floor;
+  for (uint32_t k = 0; k < b; ++k) {
+    if (tid == 0) {
+      double d = A[(size_t)k * b + k] + shift;
+      d = d > floor_ ? d : floor_;
+      const double r = sqrt(d);
+      A[(size_t)k * b + k] = r;
+      s_piv = r;
+    }
+    __syncthreads();
+    const double r = s_piv;
+    for (uint32_t j = k + 1 + tid; j < b; j += blockDim.x) A[(size_t)j * b + k] /= r;  // R[k][j]
+    __syncthreads();
+    const uint32_t rem = b - k - 1;
+    for (uint32_t t = tid; t < rem * rem; t += blockDim.x) {
+      const uint32_t i = k + 1 + t / rem, j = k + 1 + t % rem;
+      if (i <= j) A[(size_t)j * b + i] -= A[(size_t)i * b + k] * A[(size_t)j * b + k];
+    }
+    __syncthreads();
+  }
+  for (uint32_t t = tid; t < b * b; t += blockDim.x) {
+    const uint32_t col = t / b, row = t % b;
+    G[t] = row > col ? 0.0 : A[t];
+  }
+}
+
+// One workgroup: eigendecomposition of the symmetric b x b matrix T
+// (column-major; symmetrised first as (T + T^T) / 2) by cyclic Jacobi in
+// round-robin order (b/2 disjoint rotations per round, rows, then columns and
+// the accumulated rotations W), until the off-diagonal mass is at most 1e-30 of
+// the total or 50 sweeps. The eigenvalues, sorted descending (ties by index),
+// go to lam, the matching columns of W to Wout. T and W are overwritten. With
+// b <= PCA_LDS_BLOCK, T lives in LDS.
+__global__ __launch_bounds__(256) void k_pca_jacobi(double* __restrict__ Tg, uint32_t b,
+                                                    double* __restrict__ W, double* __restrict__ Wout,
+                                                    double* __restrict__ lam, double* __restrict__ cs) {
+  extern __shared__ double lds[];
+  __shared__ double s_part[256];
+  __shared__ int s_done;
+  __shared__ uint32_t s_order[PCA_MAX_BLOCK];
+  const uint32_t tid = threadIdx.x;
+  const bool in_lds = b <= PCA_LDS_BLOCK;
+  double* T = in_lds ? lds : Tg;
+  for (uint32_t t = tid; t < b * b; t += blockDim.x) {
+    const uint32_t col = t / b, row = t % b;
+    if (in_lds) T[t] = 0.5 * (Tg[t] + Tg[(size_t)row * b + col]);
+    W[t] = row == col ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  if (!in_lds) {
+    // symmetrise in place: the thread of (row, col), row < col, owns both halves
+    for (uint32_t t = tid; t < b * b; t += blockDim.x) {
+      const uint32_t col = t / b, row = t % b;
+      if (row < col) {
+        const double v = 0.5 * (T[t] + T[(size_t)row * b + col]);
+        T[t] = v;
+        T[(size_t)row * b + col] = v;
+      }
+    }
+    __syncthreads();
+  }
+  const uint32_t bb = b + (b & 1);  // an even number of players; index b is a dummy
+  const uint32_t half = bb / 2;
+  for (int sweep = 0; sweep < 50; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (uint32_t t = tid; t < b * b; t += blockDim.x) {
+      const double v = T[t];
+      tot += v * v;
+      if (t / b != t % b) off += v * v;
+    }
+    s_part[tid] = off;
+    __syncthreads();
+    if (tid == 0) {
+      double a = 0.0;
+      for (uint32_t i = 0; i < blockDim.x; ++i) a += s_part[i];
+      s_part[0] = a;
+    }
+    __syncthreads();
+    off = s_part[0];
+    __syncthreads();
+    s_part[tid] = tot;
+    __syncthreads();
+    if (tid == 0) {
+      double a = 0.0;
+      for (uint32_t i = 0; i < blockDim.x; ++i) a += s_part[i];
+      s_done = !(off > 1e-30 * a);
+    }
+    __syncthreads();
+    if (s_done) break;
+    for (uint32_t rnd = 0; rnd + 1 < bb; ++rnd) {
+      for (uint32_t i = tid; i < half; i += blockDim.x) {
+        uint32_t p = i == 0 ? 0u : 1u + (i - 1 + rnd) % (bb - 1);
+        uint32_t q = 1u + (bb - 2 - i + rnd) % (bb - 1);
+        if (p > q) {
+          const uint32_t x = p;
+          p = q;
+          q = x;
+        }
+        double c = 1.0, s = 0.0;
+        if (q < b) {
+          const double apq = T[(size_t)q * b + p];
+          if (apq != 0.0) {
+            const double app = T[(size_t)p * b + p], aqq = T[(size_t)q * b + q];
+            const double tau = (aqq - app) / (2.0 * apq);
+            const double tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+            c = 1.0 / sqrt(1.0 + tt * tt);
+            s = tt * c;
+          }
+        }
+        cs[4 * i] = (double)p;
+        cs[4 * i + 1] = (double)q;
+        cs[4 * i + 2] = c;
+        cs[4 * i + 3] = s;
+      }
+      __syncthreads();
+      for (uint32_t t = tid; t < half * b; t += blockDim.x) {  // rows: T <- J^T T
+        const uint32_t i = t / b, k = t % b;
+        const uint32_t p = (uint32_t)cs[4 * i], q = (uint32_t)cs[4 * i + 1];
+        if (q >= b) continue;
+        const double c = cs[4 * i + 2], s = cs[4 * i + 3];
+        const double tp = T[(size_t)k * b + p], tq = T[(size_t)k * b + q];
+        T[(size_t)k * b + p] = c * tp - s * tq;
+        T[(size_t)k * b + q] = s * tp + c * tq;
+      }
+      __syncthreads();
+      for (uint32_t t = tid; t < half * b; t += blockDim.x) {  // columns: T <- T J, W <- W J
+        const uint32_t i = t / b, k = t % b;
+        const uint32_t p = (uint32_t)cs[4 * i], q = (uint32_t)cs[4 * i + 1];
+        if (q >= b) continue;
+        const double c = cs[4 * i + 2], s = cs[4 * i + 3];
+        const double tp = T[(size_t)p * b + k], tq = T[(size_t)q * b + k];
+        T[(size_t)p * b + k] = c * tp - s * tq;
+        T[(size_t)q * b + k] = s * tp + c * tq;
+        const double wp = W[(size_t)p * b + k], wq = W[(size_t)q * b + k];
+        W[(size_t)p * b + k] = c * wp - s * wq;
+        W[(size_t)q * b + k] = s * wp + c * wq;
+      }
+      __syncthreads();
+    }
+  }
+  if (tid == 0) {
+    for (uint32_t i = 0; i < b; ++i) s_order[i] = i;
+    for (uint32_t i = 1; i < b; ++i) {  // insertion sort, descending, ties by index
+      const uint32_t x = s_order[i];
+      const double v = T[(size_t)x * b + x];
+      uint32_t j = i;
+      while (j > 0) {
+        const uint32_t y = s_order[j - 1];
+        const double u = T[(size_t)y * b + y];
+        if (u > v || (u == v && y < x)) break;
+        s_order[j] = y;
+        --j;
+      }
+      s_order[j] = x;
+    }
+    for (uint32_t i = 0; i < b; ++i) lam[i] = T[(size_t)s_order[i] * b + s_order[i]];
+  }
+  __syncthreads();
+  for (uint32_t t = tid; t < b * b; t += blockDim.x) Wout[t] = W[(size_t)s_order[t / b] * b + t % b];
+}
+
+// Residual norms |Z_c - lam_c Q_c| of the first k Ritz pairs (column-major m x b),
+// one block per pair, fixed-order tree sum.
+__global__ __launch_bounds__(256) void k_pca_residuals(const double* __restrict__ Z,
+                                                       const double* __restrict__ Q,
+                                                       const double* __restrict__ lam, uint32_t m,
+                                                       double* __restrict__ res) {
+  const uint32_t c = blockIdx.x;
+  __shared__ double part[256];
+  double a = 0.0;
+  for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+    const double r = Z[(size_t)c * m + i] - lam[c] * Q[(size_t)c * m + i];
+    a += r * r;
+  }
+  part[threadIdx.x] = a;
+  __syncthreads();
+  for (uint32_t o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) res[c] = sqrt(part[0]);
+}
+
+// The k unit vectors e_{m-1-j} with eigenvalue 0: what a dense solver returns
+// for the zero matrix (ascending order, reversed).
+__global__ void k_pca_unit_block(double* __restrict__ Q, double* __restrict__ lam, uint32_t m, uint32_t k) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (uint64_t)m * k) return;
+  const uint32_t c = (uint32_t)(t / m), i = (uint32_t)(t % m);
+  Q[t] = i == m - 1 - c ? 1.0 : 0.0;
+  if (i == 0) lam[c] = 0.0;
+}
+
+// ── projection ──
+
+// V (row-major m x k f32) from the first k columns of Q (column-major f64).
+__global__ void k_pca_vecs(const double* __restrict__ Q, uint32_t m, uint32_t k, float* __restrict__ V) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (uint64_t)m * k) return;
+  const uint32_t j = (uint32_t)(t / k), c = (uint32_t)(t % k);
+  V[t] = (float)Q[(size_t)c * m + j];
+}
+// mub[c] = sum_j mu_j V[j][c] in f64, j in order, as f32 (rsvd.rs:252-262).
+__global__ void k_pca_mub(const float* __restrict__ mu, const float* __restrict__ V, uint32_t m, uint32_t k,
+                          float* __restrict__ mub) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= k) return;
+  double a = 0.0;
+  for (uint32_t j = 0; j < m; ++j) a += (double)mu[j] * (double)V[(size_t)j * k + c];
+  mub[c] = (float)a;
+}
+
+// out[i][c] = (sum over row i's stored entries e, in order, of s_e V[j_e][c])
+// - mub[c], f32 with separate multiply and add (rsvd.rs:265-280). One thread per
+// (row, component).
+__global__ void k_pca_project(const uint64_t* __restrict__ indptr, const uint32_t* __restrict__ indices,
+                              const float* __restrict__ s, uint64_t n, const float* __restrict__ V,
+                              uint32_t k, const float* __restrict__ mub, float* __restrict__ out) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * k) return;
+  const uint64_t i = t / k;
+  const uint32_t c = (uint32_t)(t - i * k);
+  float acc = 0.0f;
+  for (uint64_t e = indptr[i]; e < indptr[i + 1]; ++e) acc = acc + s[e] * V[(size_t)indices[e] * k + c];
+  if (mub) acc = acc - mub[c];
+  out[t] = acc;
+}
+
+// Column signs (rsvd.rs:439-456): the first entry of largest magnitude (over
+// magnitudes > 0; row 0 if none) becomes positive.
+// Pass 1: block g scans rows [g seg, (g + 1) seg); thread t takes column t % k
+// and rows t / k + rps i. Partial (largest |u|, first row) per (block, column).
+__global__ void k_pca_sign_part(const float* __restrict__ U, uint64_t n, uint32_t k, uint32_t rps,
+                                uint64_t seg, float* __restrict__ pv, uint64_t* __restrict__ pi) {
+  __shared__ float s_v[1024];
+  __shared__ uint64_t s_i[1024];
+  const uint32_t tid = threadIdx.x, c = tid % k, r0 = tid / k;
+  const uint64_t lo = (uint64_t)blockIdx.x * seg, hi = lo + seg < n ? lo + seg : n;
+  float best = 0.0f;
+  uint64_t bi = ~0ull;
+  for (uint64_t r = lo + r0; r < hi; r += rps) {
+    const float a = fabsf(U[r * k + c]);
+    if (a > best) {
+      best = a;
+      bi = r;
+    }
+  }
+  s_v[tid] = best;
+  s_i[tid] = bi;
+  __syncthreads();
+  if (r0 != 0) return;
+  for (uint32_t q = 1; q < rps; ++q) {
+    const float v = s_v[q * k + c];
+    const uint64_t i = s_i[q * k + c];
+    if (v > best || (v == best && v > 0.0f && i < bi)) {
+      best = v;
+      bi = i;
+    }
+  }
+  pv[(size_t)blockIdx.x * k + c] = best;
+  pi[(size_t)blockIdx.x * k + c] = bi;
+}
+// Pass 2: per column, the partials in block order (earlier rows first).
+__global__ void k_pca_sign_pick(const float* __restrict__ U, uint32_t k, uint32_t G,
+                                const float* __restrict__ pv, const uint64_t* __restrict__ pi,
+                                uint32_t* __restrict__ flip) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= k) return;
+  float best = 0.0f;
+  uint64_t bi = 0;
+  for (uint32_t g = 0; g < G; ++g) {
+    const float v = pv[(size_t)g * k + c];
+    if (v > best) {
+      best = v;
+      bi = pi[(size_t)g * k + c];
+    }
+  }
+  flip[c] = U[bi * k + c] < 0.0f;
+}
+__global__ void k_pca_sign_apply(float* __restrict__ U, uint64_t count, uint32_t k,
+                                 const uint32_t* __restrict__ flip) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count && flip[t % k]) U[t] = -U[t];
+}
+
+// ratio[j] = max(lam_j - sigma, 0) / trace (lam holds eigenvalues of C + sigma I).
+__global__ void k_pca_ratio(const double* __restrict__ lam, const double* __restrict__ tv, uint32_t k,
+                            float* __restrict__ ratio) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < k) ratio[j] = (float)(fmax(lam[j] - tv[1], 0.0) / tv[0]);
+}
+
+}  // namespace psq
+
+using namespace psq;
+
+namespace {
+
+unsigned blocks(uint64_t n, unsigned b) { return (unsigned)std::max<uint64_t>(1, (n + b - 1) / b); }
+
+thread_local psq_pca_report g_pca_report;
+
+// Events recorded on the call's stream; destroyed with the call.
+struct Events {
+  std::vector<hipEvent_t> ev;
+  ~Events() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
+  int record(hipStream_t s, size_t* id) {
+    hipEvent_t e;
+    PHIPC(hipEventCreate(&e));
+    ev.push_back(e);
+    *id = ev.size() - 1;
+    PHIPC(hipEventRecord(e, s));
+    return PSQ_OK;
+  }
+  double ms(size_t a, size_t b) const {
+    float t = 0.0f;
+    return hipEventElapsedTime(&t, ev[a], ev[b]) == hipSuccess ? (double)t : 0.0;
+  }
+};
+
+struct Dev {
+  std::vector<void*> ptrs;
+  ~Dev() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <typename T>
+  int alloc(size_t count, T** out) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess)
+      return fail(e == hipErrorOutOfMemory ? PSQ_E_NOMEM : PSQ_E_DEVICE,
+                  "PCA allocation of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+    ptrs.push_back(p);
+    *out = static_cast<T*>(p);
+    return PSQ_OK;
+  }
+};
+
+// One rocBLAS handle per device for the life of the process (created on first
+// use; creation loads the library's kernels).
+std::mutex g_blas_mu;
+std::map<int, rocblas_handle> g_blas;
+
+int blas_handle(hipStream_t s, rocblas_handle* out) {
+  int dev = 0;
+  PHIPC(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> g(g_blas_mu);
+  rocblas_handle& h = g_blas[dev];
+  if (!h) {
+    PBLAS(rocblas_create_handle(&h));
+    PBLAS(rocblas_set_pointer_mode(h, rocblas_pointer_mode_host));
+    PBLAS(rocblas_set_atomics_mode(h, rocblas_atomics_not_allowed));  // repeatable sums
+  }
+  PBLAS(rocblas_set_stream(h, s));
+  *out = h;
+  return PSQ_OK;
+}
+
+size_t small_lds(uint32_t b) { return b <= PCA_LDS_BLOCK ? (size_t)b * b * sizeof(double) : 0; }
+
+// Q (m x b, column-major) <- an orthonormal basis of its columns: shifted
+// CholeskyQR, then CholeskyQR twice (G = Q^T Q, R = chol(G), Q <- Q R^-1).
+int orthonormalise(rocblas_handle h, double* Q, uint32_t m, uint32_t b, double* G, hipStream_t s) {
+  const double one = 1.0, zero = 0.0;
+  for (int pass = 0; pass < 3; ++pass) {
+    PBLAS(rocblas_dsyrk(h, rocblas_fill_upper, rocblas_operation_transpose, b, m, &one, Q, m, &zero, G, b));
+    hipLaunchKernelGGL(k_pca_chol, dim3(1), dim3(256), small_lds(b), s, G, b, m, pass == 0 ? 1 : 0);
+    PHIPC(hipGetLastError());
+    PBLAS(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                        rocblas_diagonal_non_unit, m, b, &one, G, b, Q, m));
+  }
+  return PSQ_OK;
+}
+
+int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t n, uint64_t m64, uint64_t k64,
+               int32_t center, int32_t scale, double max_value, float* d_emb, float* d_ratio, float* d_comp,
+               hipStream_t s) {
+  const uint32_t m = (uint32_t)m64, kk = (uint32_t)k64;
+  const double clip = max_value;  // +inf: no clipping
+  if (scale) center = 1;          // pp.scale always centres (pca.rs:416-419)
+
+  uint64_t ends[2] = {0, 0};
+  PHIPC(hipMemcpyAsync(&ends[0], ip, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  PHIPC(hipMemcpyAsync(&ends[1], ip + n, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  PHIPC(hipStreamSynchronize(s));
+  if (ends[0] != 0) return fail(PSQ_E_INVALID, "indptr[0] must be 0");
+  const uint64_t nnz = ends[1];
+  if (nnz > 0 && (!ix || !x)) return fail(PSQ_E_INVALID, "NULL pointer argument");
+
+  rocblas_handle h = nullptr;
+  PTRY(blas_handle(s, &h));
+  Dev dv;
+  Events evs;
+  psq_pca_report rep{};
+  size_t ev_start, ev_scaled, ev_gram, ev_eigen, ev_end;
+  std::vector<std::pair<size_t, size_t>> ev_syrk;
+  PTRY(evs.record(s, &ev_start));
+  {
+    uint32_t* d_bad;
+    PTRY(dv.alloc(1, &d_bad));
+    PHIPC(hipMemsetAsync(d_bad, 0, 4, s));
+    hipLaunchKernelGGL(k_pca_check, dim3(blocks(std::max(nnz, n), 256)), dim3(256), 0, s, ip, n, ix, nnz, m,
+                       d_bad);
+    PHIPC(hipGetLastError());
+    uint32_t bad = 0;
+    PHIPC(hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, s));
+    PHIPC(hipStreamSynchronize(s));
+    if (bad & 2) return fail(PSQ_E_INVALID, "indptr must not decrease");
+    if (bad & 1) return fail(PSQ_E_INVALID, "a column index is >= n_features");
+  }
+
+  // dense f64 row chunks of at most 2^25 elements (256 MB)
+  const uint32_t R = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(n, (1ull << 25) / m));
+  const uint32_t P = std::min<uint32_t>(PCA_SEGMENTS, R);
+  double *D, *acc_s, *acc_q = nullptr, *ps_s, *ps_q = nullptr, *C;
+  PTRY(dv.alloc((size_t)R * m, &D));
+  PTRY(dv.alloc(m, &acc_s));
+  PTRY(dv.alloc((size_t)P * m, &ps_s));
+  PTRY(dv.alloc((size_t)m * m, &C));
+  const double one = 1.0, zero = 0.0;
+  auto chunk_sums = [&](uint64_t r0, uint32_t rc, const float* vals, double* q, double* psq) -> int {
+    PHIPC(hipMemsetAsync(D, 0, (size_t)rc * m * sizeof(double), s));
+    hipLaunchKernelGGL(k_pca_densify, dim3(blocks(rc, 256)), dim3(256), 0, s, ip, ix, vals, r0, rc, m, D);
+    const uint32_t Pc = std::min<uint32_t>(P, rc), seg = (rc + Pc - 1) / Pc;
+    hipLaunchKernelGGL(k_pca_colsums_part, dim3(blocks(m, 256), Pc), dim3(256), 0, s, D, rc, m, seg, ps_s, psq);
+    hipLaunchKernelGGL(k_pca_colsums_fold, dim3(blocks(m, 256)), dim3(256), 0, s, ps_s, psq, Pc, m, acc_s, q);
+    PHIPC(hipGetLastError());
+    return PSQ_OK;
+  };
+
+  const float* S = x;  // the stored values the Gram and projection read
+  if (scale) {
+    double *mean, *inv, *zr;
+    float* sv;
+    PTRY(dv.alloc(m, &acc_q));
+    PTRY(dv.alloc((size_t)P * m, &ps_q));
+    PTRY(dv.alloc(m, &mean));
+    PTRY(dv.alloc(m, &inv));
+    PTRY(dv.alloc(m, &zr));
+    PTRY(dv.alloc(nnz, &sv));
+    PHIPC(hipMemsetAsync(acc_s, 0, m * sizeof(double), s));
+    PHIPC(hipMemsetAsync(acc_q, 0, m * sizeof(double), s));
+    for (uint64_t r0 = 0; r0 < n; r0 += R) {
+      const uint32_t rc = (uint32_t)std::min<uint64_t>(R, n - r0);
+      PTRY(chunk_sums(r0, rc, x, acc_q, ps_q));
+    }
+    hipLaunchKernelGGL(k_pca_scale_stats, dim3(blocks(m, 256)), dim3(256), 0, s, acc_s, acc_q, m, n, clip, mean,
+                       inv, zr);
+    if (nnz)
+      hipLaunchKernelGGL(k_pca_scale_values, dim3(blocks(nnz, 256)), dim3(256), 0, s, ix, x, nnz, mean, inv, zr,
+                         clip, sv);
+    PHIPC(hipGetLastError());
+    S = sv;
+  }
+  PTRY(evs.record(s, &ev_scaled));
+
+  // Gram of the stored values and their column sums
+  PHIPC(hipMemsetAsync(acc_s, 0, m * sizeof(double), s));
+  PHIPC(hipMemsetAsync(C, 0, (size_t)m * m * sizeof(double), s));
+  for (uint64_t r0 = 0; r0 < n; r0 += R) {
+    const uint32_t rc = (uint32_t)std::min<uint64_t>(R, n - r0);
+    PTRY(chunk_sums(r0, rc, S, nullptr, nullptr));
+    // D is row-major rc x m, i.e. column-major m x rc: C += D^T D (upper)
+    size_t a, b2;
+    PTRY(evs.record(s, &a));
+    PBLAS(rocblas_dsyrk(h, rocblas_fill_upper, rocblas_operation_none, m, rc, &one, D, m, &one, C, m));
+    PTRY(evs.record(s, &b2));
+    ev_syrk.emplace_back(a, b2);
+    ++rep.chunks;
+  }
+  rep.rows_per_chunk = R;
+  float* mu = nullptr;
+  if (center) {
+    PTRY(dv.alloc(m, &mu));
+    hipLaunchKernelGGL(k_pca_means, dim3(blocks(m, 256)), dim3(256), 0, s, acc_s, m, n, mu);
+  }
+  hipLaunchKernelGGL(k_pca_center, dim3(blocks((uint64_t)m * m, 256)), dim3(256), 0, s, C, m, n, mu);
+  double* tv;
+  PTRY(dv.alloc(2, &tv));
+  hipLaunchKernelGGL(k_pca_trace, dim3(1), dim3(1), 0, s, C, m, tv);
+  PHIPC(hipGetLastError());
+  PTRY(evs.record(s, &ev_gram));
+  double htv[2];
+  PHIPC(hipMemcpyAsync(htv, tv, sizeof(htv), hipMemcpyDeviceToHost, s));
+  PHIPC(hipStreamSynchronize(s));
+  if (!std::isfinite(htv[0])) return fail(PSQ_E_INVALID, "the matrix holds non-finite values");
+
+  // top k eigenpairs
+  const uint32_t b = (uint32_t)std::min<uint64_t>(m, kk + std::max<uint32_t>(16, kk / 2));
+  if (b > PCA_MAX_BLOCK) return fail(PSQ_E_UNSUPPORTED, "n_components > 680 is not supported");
+  double *Q, *Z, *Tmp, *G, *Tb, *Wb, *Ws, *lam, *cs, *res;
+  PTRY(dv.alloc((size_t)m * b, &Q));
+  PTRY(dv.alloc((size_t)m * b, &Z));
+  PTRY(dv.alloc((size_t)m * b, &Tmp));
+  PTRY(dv.alloc((size_t)b * b, &G));
+  PTRY(dv.alloc((size_t)b * b, &Tb));
+  PTRY(dv.alloc((size_t)b * b, &Wb));
+  PTRY(dv.alloc((size_t)b * b, &Ws));
+  PTRY(dv.alloc(b, &lam));
+  PTRY(dv.alloc((size_t)4 * (b / 2 + 1), &cs));
+  PTRY(dv.alloc(b, &res));
+  if (htv[0] <= 0.0) {
+    // C = 0 (a PSD matrix with zero trace): no direction is preferred
+    hipLaunchKernelGGL(k_pca_unit_block, dim3(blocks((uint64_t)m * kk, 256)), dim3(256), 0, s, Q, lam, m, kk);
+    PHIPC(hipGetLastError());
+  } else {
+    hipLaunchKernelGGL(k_pca_start, dim3(blocks((uint64_t)m * b, 256)), dim3(256), 0, s, Q, m, b);
+    PHIPC(hipGetLastError());
+    PTRY(orthonormalise(h, Q, m, b, G, s));
+    std::vector<double> hres(kk), hlam(b);
+    bool converged = false;
+    for (int it = 1; it <= 4000; ++it) {
+      rep.iterations = it;
+      PBLAS(rocblas_dsymm(h, rocblas_side_left, rocblas_fill_upper, m, b, &one, C, m, Q, m, &zero, Z, m));
+      hipLaunchKernelGGL(k_pca_shift, dim3(blocks((uint64_t)m * b, 256)), dim3(256), 0, s, Z, Q,
+                         (uint64_t)m * b, tv);
+      PHIPC(hipGetLastError());
+      if (it % 4 == 0 || m == b) {
+        // Rayleigh-Ritz: T = Q^T Z = W L W^T; Q <- Q W, Z <- Z W
+        PBLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, b, b, m, &one, Q, m, Z, m,
+                            &zero, Tb, b));
+        hipLaunchKernelGGL(k_pca_jacobi, dim3(1), dim3(256), small_lds(b), s, Tb, b, Wb, Ws, lam, cs);
+        PHIPC(hipGetLastError());
+        PBLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, b, b, &one, Q, m, Ws, b, &zero,
+                            Tmp, m));
+        std::swap(Q, Tmp);
+        PBLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, b, b, &one, Z, m, Ws, b, &zero,
+                            Tmp, m));
+        std::swap(Z, Tmp);
+        hipLaunchKernelGGL(k_pca_residuals, dim3(kk), dim3(256), 0, s, Z, Q, lam, m, res);
+        PHIPC(hipGetLastError());
+        PHIPC(hipMemcpyAsync(hres.data(), res, kk * sizeof(double), hipMemcpyDeviceToHost, s));
+        PHIPC(hipMemcpyAsync(hlam.data(), lam, b * sizeof(double), hipMemcpyDeviceToHost, s));
+        PHIPC(hipStreamSynchronize(s));
+        const double top = std::max(std::fabs(hlam[0]), 1e-300);
+        converged = true;
+        for (uint32_t i = 0; i < kk; ++i)
+          if (!(hres[i] <= 1e-11 * top)) converged = false;
+        if (m == b) converged = true;  // the whole space: the projection's eigenpairs are C's
+        if (converged) break;
+      }
+      std::swap(Q, Z);
+      PTRY(orthonormalise(h, Q, m, b, G, s));
+    }
+    if (!converged) return fail(PSQ_E_DEVICE, "the eigensolver did not converge in 4000 iterations");
+  }
+
+  PTRY(evs.record(s, &ev_eigen));
+  rep.block = b;
+
+  // projection, signs, ratios
+  float *V = d_comp, *mub = nullptr, *ppv;
+  uint64_t* ppi;
+  uint32_t* flip;
+  if (!V) PTRY(dv.alloc((size_t)m * kk, &V));
+  hipLaunchKernelGGL(k_pca_vecs, dim3(blocks((uint64_t)m * kk, 256)), dim3(256), 0, s, Q, m, kk, V);
+  if (center) {
+    PTRY(dv.alloc(kk, &mub));
+    hipLaunchKernelGGL(k_pca_mub, dim3(blocks(kk, 64)), dim3(64), 0, s, mu, V, m, kk, mub);
+  }
+  hipLaunchKernelGGL(k_pca_project, dim3(blocks(n * kk, 256)), dim3(256), 0, s, ip, ix, S, n, V, kk, mub, d_emb);
+  const uint32_t rps = std::max<uint32_t>(1, 256 / kk);
+  const uint32_t Gs = (uint32_t)std::min<uint64_t>(2048, (n + 255) / 256);
+  const uint64_t seg = (n + Gs - 1) / Gs;
+  PTRY(dv.alloc((size_t)Gs * kk, &ppv));
+  PTRY(dv.alloc((size_t)Gs * kk, &ppi));
+  PTRY(dv.alloc(kk, &flip));
+  hipLaunchKernelGGL(k_pca_sign_part, dim3(Gs), dim3(kk * rps), 0, s, d_emb, n, kk, rps, seg, ppv, ppi);
+  hipLaunchKernelGGL(k_pca_sign_pick, dim3(blocks(kk, 64)), dim3(64), 0, s, d_emb, kk, Gs, ppv, ppi, flip);
+  hipLaunchKernelGGL(k_pca_sign_apply, dim3(blocks(n * kk, 256)), dim3(256), 0, s, d_emb, n * kk, kk, flip);
+  hipLaunchKernelGGL(k_pca_ratio, dim3(blocks(kk, 64)), dim3(64), 0, s, lam, tv, kk, d_ratio);
+  PHIPC(hipGetLastError());
+  PTRY(evs.record(s, &ev_end));
+  PHIPC(hipStreamSynchronize(s));  // the buffers freed on return are no longer in use
+  rep.ms_scale = evs.ms(ev_start, ev_scaled);
+  rep.ms_gram = evs.ms(ev_scaled, ev_gram);
+  for (const auto& p : ev_syrk) rep.ms_syrk += evs.ms(p.first, p.second);
+  rep.ms_eigen = evs.ms(ev_gram, ev_eigen);
+  rep.ms_project = evs.ms(ev_eigen, ev_end);
+  rep.ms_total = evs.ms(ev_start, ev_end);
+  g_pca_report = rep;
+  return PSQ_OK;
+}
+
+int pca_validate(uint64_t n, uint64_t m, uint64_t k, int32_t scale, double max_value) {
+  if (n == 0 || m == 0)
+    return fail(PSQ_E_INVALID, "input matrix has zero rows (%llu) or columns (%llu)", (unsigned long long)n,
+                (unsigned long long)m);
+  const uint64_t most = std::min(n, m);
+  if (k > most)
+    return fail(PSQ_E_INVALID, "n_components (%llu) exceeds min(n_obs, n_features) (%llu)",
+                (unsigned long long)k, (unsigned long long)most);
+  if (k == 0) return fail(PSQ_E_INVALID, "n_components must be >= 1");
+  if (scale && !(max_value >= 0.0)) return fail(PSQ_E_INVALID, "max_value must be >= 0 (or +inf for none)");
+  if (m > PCA_GRAM_MAX_FEATURES)
+    return fail(PSQ_E_UNSUPPORTED,
+                "n_features (%llu) > %u: the reference switches to randomized SVD there, which this build "
+                "does not provide",
+                (unsigned long long)m, PCA_GRAM_MAX_FEATURES);
+  if (n >= (1ull << 40) / k) return fail(PSQ_E_UNSUPPORTED, "n_obs * n_components must be < 2^40");
+  return PSQ_OK;
+}
+
+}  // namespace
+
+extern "C" int psq_pca_csr_device(const uint64_t* d_indptr, const uint32_t* d_indices, const float* d_data,
+                                  uint64_t n_obs, uint64_t n_features, uint64_t n_components, int32_t center,
+                                  int32_t scale, double max_value, float* d_embedding, float* d_variance_ratio,
+                                  float* d_components, void* stream) {
+  PTRY(pca_validate(n_obs, n_features, n_components, scale, max_value));
+  if (!d_indptr || !d_embedding || !d_variance_ratio) return fail(PSQ_E_INVALID, "NULL pointer argument");
+  return pca_device(d_indptr, d_indices, d_data, n_obs, n_features, n_components, center, scale, max_value,
+                    d_embedding, d_variance_ratio, d_components, (hipStream_t)stream);
+}
+
+extern "C" int psq_pca_csr(const uint64_t* indptr, const uint32_t* indices, const float* data, uint64_t n_obs,
+                           uint64_t n_features, uint64_t n_components, int32_t center, int32_t scale,
+                           double max_value, float* embedding, float* variance_ratio, float* components,
+                           int32_t device) {
+  PTRY(pca_validate(n_obs, n_features, n_components, scale, max_value));
+  if (!indptr || !embedding || !variance_ratio) return fail(PSQ_E_INVALID, "NULL pointer argument");
+  const uint64_t n = n_obs, m = n_features, k = n_components, nnz = indptr[n];
+  if (nnz > 0 && (!indices || !data)) return fail(PSQ_E_INVALID, "NULL pointer argument");
+  int count = 0;
+  PHIPC(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count) return fail(PSQ_E_INVALID, "device %d out of range", device);
+  PHIPC(hipSetDevice(device));
+  hipStream_t s = nullptr;
+  Dev dv;
+  uint64_t* d_ip;
+  uint32_t* d_ix;
+  float *d_x, *d_emb, *d_ratio, *d_comp = nullptr;
+  PTRY(dv.alloc(n + 1, &d_ip));
+  PTRY(dv.alloc(nnz, &d_ix));
+  PTRY(dv.alloc(nnz, &d_x));
+  PTRY(dv.alloc(n * k, &d_emb));
+  PTRY(dv.alloc(k, &d_ratio));
+  if (components) PTRY(dv.alloc(m * k, &d_comp));
+  PHIPC(hipMemcpyAsync(d_ip, indptr, (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+  if (nnz) {
+    PHIPC(hipMemcpyAsync(d_ix, indices, nnz * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    PHIPC(hipMemcpyAsync(d_x, data, nnz * sizeof(float), hipMemcpyHostToDevice, s));
+  }
+  PTRY(pca_device(d_ip, d_ix, d_x, n, m, k, center, scale, max_value, d_emb, d_ratio, d_comp, s));
+  PHIPC(hipMemcpyAsync(embedding, d_emb, n * k * sizeof(float), hipMemcpyDeviceToHost, s));
+  PHIPC(hipMemcpyAsync(variance_ratio, d_ratio, k * sizeof(float), hipMemcpyDeviceToHost, s));
+  if (components) PHIPC(hipMemcpyAsync(components, d_comp, m * k * sizeof(float), hipMemcpyDeviceToHost, s));
+  PHIPC(hipStreamSynchronize(s));
+  return PSQ_OK;
+}
+
+extern "C" int psq_pca_last_report(psq_pca_report* out) {
+  if (!out) return fail(PSQ_E_INVALID, "NULL pointer argument");
+  *out = g_pca_report;
+  return PSQ_OK;
+}

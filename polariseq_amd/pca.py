@@ -1,0 +1,142 @@
+"""PCA of a sparse matrix on the GPU: the reference binding's ``pca_sparse`` and
+``pca_sparse_scaled`` (crates/polariseq-python/src/lib.rs:395-467), the calls
+``pp.pca`` makes for sparse input (python/polariseq/__init__.py:2421-2447).
+
+Both take the exact Gram route of the reference (crates/polariseq-core/src/pca.rs:487-535):
+C = S^T S - n mu mu^T in float64, its top ``n_components`` eigenvectors, the
+projection of the cells (float32, each row summed over its stored entries in
+order), column signs fixed so that each column's largest-magnitude entry is
+positive, and variance ratios max(lambda, 0) / trace(C). ``pca_sparse_scaled``
+first applies ``pp.scale`` semantics to the stored values (pca.rs:396-480).
+
+Where the reference differs:
+
+- For fewer than 1,000 cells or genes, or n_components * 4 >= min(n, m), the
+  reference's ``pca_sparse`` densifies and takes a thin SVD (pca.rs:143-146,
+  261-263), whose column signs are arbitrary; here the Gram route runs at every
+  size, so the columns carry canonical signs. The two agree to the tolerance
+  the reference's own test states (pca.rs:861-905: variance ratios within 1e-4,
+  |cos| > 0.9999 per column).
+- More than 4,096 genes: the reference switches to randomized block Krylov SVD
+  (``oversample``, ``n_iter`` and ``seed`` configure it); this build raises
+  ``RuntimeError`` there. The three arguments are accepted for the signature's
+  sake.
+
+Errors follow the binding: ``RuntimeError("PCA failed: ...")`` for an empty
+matrix or too many components (the ``PcaError`` cases), ``ValueError`` for an
+``out`` array of the wrong shape or type.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import _native as N
+
+
+def _csr_parts(matrix):
+    """(indptr u64, indices u32, data f32, n_obs, n_features) of a scipy CSR
+    matrix or an (indptr, indices, data, (n_obs, n_features)) tuple."""
+    if isinstance(matrix, tuple):
+        indptr, indices, data, shape = matrix
+    else:
+        fmt = getattr(matrix, "format", None)
+        if fmt != "csr":
+            raise ValueError("matrix must be a scipy CSR matrix or an (indptr, indices, data, shape) tuple")
+        indptr, indices, data, shape = matrix.indptr, matrix.indices, matrix.data, matrix.shape
+    n, m = int(shape[0]), int(shape[1])
+    indptr = np.ascontiguousarray(indptr, dtype=np.uint64)
+    indices = np.ascontiguousarray(indices, dtype=np.uint32)
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    if indptr.shape != (n + 1,):
+        raise ValueError(f"indptr must have n_obs + 1 = {n + 1} entries")
+    nnz = int(indptr[-1]) if n + 1 > 0 else 0
+    if indices.shape[0] < nnz or data.shape[0] < nnz:
+        raise ValueError("indices and data must hold indptr[-1] entries")
+    return indptr, indices, data, n, m
+
+
+def _run(matrix, n_components: int, center: bool, scale: bool, max_value, out, components, device: int):
+    indptr, indices, data, n, m = _csr_parts(matrix)
+    k = int(n_components)
+    if out is not None:
+        if not (isinstance(out, np.ndarray) and out.dtype == np.float32 and out.shape == (n, k)
+                and out.flags.c_contiguous and out.flags.writeable):
+            raise ValueError(f"out must be a writable C-contiguous float32 array of shape ({n}, {k})")
+        emb = out
+    else:
+        emb = np.empty((n, max(k, 0)), dtype=np.float32)
+    if components is not None and not (isinstance(components, np.ndarray) and components.dtype == np.float32
+                                       and components.shape == (m, k) and components.flags.c_contiguous):
+        raise ValueError(f"components must be a C-contiguous float32 array of shape ({m}, {k})")
+    ratio = np.empty(max(k, 0), dtype=np.float32)
+    mv = math.inf if max_value is None else float(max_value)
+    L = N.lib()
+    rc = L.psq_pca_csr(indptr.ctypes.data, indices.ctypes.data, data.ctypes.data, n, m, max(k, 0),
+                       int(bool(center)), int(bool(scale)), mv, emb.ctypes.data, ratio.ctypes.data,
+                       None if components is None else components.ctypes.data, int(device))
+    if rc != N.PSQ_OK:
+        msg = L.psq_last_error().decode("utf-8", "replace")
+        raise N.PsqError(rc, f"PCA failed: {msg}")
+    return {"embedding": None if out is not None else emb, "variance_ratio": ratio, "n_components": k}
+
+
+def pca_sparse(matrix, n_components: int, center: bool, oversample: int, n_iter: int, seed: int,
+               out=None, *, components=None, device: int = 0) -> dict:
+    """lib.rs:395-436: PCA of a CSR matrix with implicit centering (``center``).
+    Returns ``{"embedding": (n_obs, n_components) float32 or None when written
+    into ``out``, "variance_ratio": float32[n_components], "n_components": k}``.
+    ``components`` (optional, float32 (n_features, n_components)) receives the
+    principal axes the embedding was projected on."""
+    del oversample, n_iter, seed  # parameters of the randomized route (> 4,096 genes)
+    return _run(matrix, n_components, center, False, None, out, components, device)
+
+
+def pca_sparse_scaled(matrix, n_components: int, max_value=None, seed: int = 0, out=None, *,
+                      components=None, device: int = 0) -> dict:
+    """lib.rs:438-467: PCA of the per-gene standardised matrix (``pp.scale``,
+    ddof 1, a constant gene divided by 1, values clipped to +-max_value), centred,
+    without densifying."""
+    del seed
+    if max_value is not None and not float(max_value) >= 0.0:
+        raise ValueError("max_value must be >= 0")
+    return _run(matrix, n_components, True, True, max_value, out, components, device)
+
+
+def pca_last_report() -> dict:
+    """Phase times (HIP events on the call's stream) and solver counts of this
+    thread's last successful PCA call."""
+    import ctypes
+    r = N.PcaReport()
+    N.check(N.lib().psq_pca_last_report(ctypes.byref(r)))
+    return r.as_dict()
+
+
+def pca_csr_device(indptr, indices, data, n_features: int, n_components: int, *, center: bool = True,
+                   scale: bool = False, max_value=None, out=None, components=None):
+    """The same computation on CUDA tensors already on the device (uint64
+    indptr as int64, int32 indices, float32 data); returns (embedding,
+    variance_ratio) tensors on the current stream."""
+    import torch
+
+    from .neighbors import _stream_ptr
+    for t, dt in ((indptr, torch.int64), (indices, torch.int32), (data, torch.float32)):
+        if not (t.is_cuda and t.dtype == dt and t.is_contiguous() and t.dim() == 1):
+            raise ValueError("indptr/indices/data must be contiguous 1-D CUDA tensors "
+                             "(int64, int32, float32)")
+    n = indptr.numel() - 1
+    k = int(n_components)
+    if out is None:
+        out = torch.empty((n, k), dtype=torch.float32, device=indptr.device)
+    ratio = torch.empty(k, dtype=torch.float32, device=indptr.device)
+    mv = math.inf if max_value is None else float(max_value)
+    L = N.lib()
+    rc = L.psq_pca_csr_device(indptr.data_ptr(), indices.data_ptr(), data.data_ptr(), n, int(n_features), k,
+                              int(bool(center)), int(bool(scale)), mv, out.data_ptr(), ratio.data_ptr(),
+                              None if components is None else components.data_ptr(), _stream_ptr())
+    if rc != N.PSQ_OK:
+        msg = L.psq_last_error().decode("utf-8", "replace")
+        raise N.PsqError(rc, f"PCA failed: {msg}")
+    return out, ratio

@@ -1,0 +1,257 @@
+"""GPU parity of the sparse PCA (polariseq_amd/csrc/pca.hip) with the oracle
+(oracle/pca.py, the reference's Gram route, crates/polariseq-core/src/pca.rs:396-535).
+
+* Axes and variance ratios against the oracle's LAPACK eigendecomposition
+  (both come from the same float64 Gram matrix; the eigensolvers differ):
+  ratios within 1e-6 absolute; every axis an eigenvector of the oracle's Gram
+  matrix to float32 rounding (residual <= 1e-6 of the largest eigenvalue), and
+  each axis whose eigenvalue is separated by more than 1e-3 of the largest
+  within 2e-6 of the oracle's axis elementwise, up to sign.
+* Everything after the axes bit for bit: the oracle, given the GPU's axes,
+  projects, centres and signs the cells exactly as the GPU did (count data,
+  whose float64 column sums are exact in any order).
+* The reference's own test shapes and tolerances (pca.rs:754-905), a run that
+  repeats bit for bit, the device entry equal to the host one, the full space
+  (block = n_features), rank-deficient input, an all-zero matrix, 4,096 genes,
+  and the argument errors.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import pca as P
+from polariseq_amd import pca as gp
+
+pytestmark = pytest.mark.gpu
+
+
+def counts(n, m, density, seed, hi=6):
+    rng = np.random.default_rng(seed)
+    keep = rng.random((n, m)) < density
+    vals = np.floor(rng.random((n, m)) * hi) + 1.0
+    return sp.csr_matrix(np.where(keep, vals, 0.0).astype(np.float32))
+
+
+def structured_counts(n, m, rank, density, seed):
+    """Counts with a few strong directions (clusters of cells over genes)."""
+    rng = np.random.default_rng(seed)
+    load = rng.gamma(1.0, 1.0, size=(rank, m))
+    score = rng.gamma(0.7, 1.0, size=(n, rank))
+    lam = score @ load * (3.0 / rank)
+    x = rng.poisson(lam) * (rng.random((n, m)) < density)
+    return sp.csr_matrix(x.astype(np.float32))
+
+
+def parts(mat):
+    mat = mat.tocsr()
+    mat.sort_indices()
+    return mat.indptr.astype(np.uint64), mat.indices.astype(np.uint32), mat.data.astype(np.float32)
+
+
+def run(mat, k, center=True, scale=False, max_value=None):
+    n, m = mat.shape
+    comp = np.zeros((m, k), np.float32)
+    if scale:
+        r = gp.pca_sparse_scaled(mat, k, max_value, 0, components=comp)
+    else:
+        r = gp.pca_sparse(mat, k, center, 10, 4, 0, components=comp)
+    return r["embedding"], r["variance_ratio"], comp
+
+
+def check_axes(comp, want, tol=2e-6):
+    """Every axis is an eigenvector of the oracle's Gram matrix to float32
+    rounding (a residual check, valid whatever the eigenvalue gaps); axes whose
+    eigenvalue is separated from its neighbours by more than 1e-3 of the
+    largest also match the oracle's axis elementwise, up to sign."""
+    lam, vecs, c = want["eigenvalues"], want["components"], want["gram"]
+    top = abs(lam[0])
+    k = comp.shape[1]
+    for j in range(k):
+        v = comp[:, j].astype(np.float64)
+        rho = v @ c @ v / (v @ v)
+        assert np.linalg.norm(c @ v - rho * v) <= 1e-6 * top, j
+        gaps = [abs(lam[j] - lam[i]) for i in (j - 1, j + 1) if 0 <= i < len(lam)]
+        if gaps and min(gaps) < 1e-3 * top:
+            continue
+        b = vecs[:, j].astype(np.float64)
+        s = 1.0 if v @ b >= 0 else -1.0
+        assert np.max(np.abs(v - s * b)) < tol, (j, np.max(np.abs(v - s * b)))
+
+
+def oracle(ip, ix, dx, n, m, k, center=True, scaled=False, max_value=None):
+    """The oracle's result for k + 1 components (the gap after the last axis)
+    with the Gram matrix it decomposed."""
+    vals = P.scaled_values(ip, ix, dx, n, m, max_value) if scaled else dx
+    kk = min(k + 1, min(n, m))
+    r = P.pca_gram(ip, ix, vals, n, m, kk, center or scaled)
+    r["gram"] = P.centred_gram(ip, ix, vals, n, m, center or scaled)[0]
+    r["variance_ratio"] = r["variance_ratio"][:k]
+    return r
+
+
+@pytest.mark.parametrize("n,m,k,center,seed", [
+    (600, 40, 6, True, 3),     # gram_pca_equals_exact_thin_svd's shape
+    (600, 40, 6, False, 4),
+    (2000, 300, 30, True, 5),
+    (5000, 1000, 50, True, 6),
+])
+def test_pca_matches_oracle(n, m, k, center, seed):
+    print(f"pca n={n} m={m} k={k} center={center} seed={seed}", flush=True)
+    mat = structured_counts(n, m, 12, 0.3, seed)
+    ip, ix, dx = parts(mat)
+    emb, vr, comp = run(mat, k, center)
+    want = oracle(ip, ix, dx, n, m, k, center)
+    assert np.max(np.abs(vr - want["variance_ratio"])) < 1e-6
+    check_axes(comp, want)
+    # after the axes: bit for bit
+    same = P.pca_gram(ip, ix, dx, n, m, k, center, components=comp)
+    assert np.array_equal(emb, same["embedding"])
+
+
+@pytest.mark.parametrize("max_value", [None, 3.0, 10.0])
+def test_scaled_pca_matches_oracle(max_value):
+    print(f"scaled max_value={max_value}", flush=True)
+    rng = np.random.default_rng(11)
+    n, m, k = 500, 30, 5
+    dense = np.zeros((n, m), np.float32)
+    keep = rng.random((n, m - 2)) < 0.25
+    dense[:, : m - 2] = np.where(keep, np.floor(rng.random((n, m - 2)) * 5.0) + 1.0, 0.0)
+    dense[:, m - 2] = np.where(np.arange(n) % 250 == 0, 0.0, 5.0 + rng.random(n) * 0.01)
+    dense[::97, 3] = 400.0
+    mat = sp.csr_matrix(dense)
+    ip, ix, dx = parts(mat)
+    emb, vr, comp = run(mat, k, scale=True, max_value=max_value)
+    want = oracle(ip, ix, dx, n, m, k, scaled=True, max_value=max_value)
+    assert np.max(np.abs(vr - want["variance_ratio"])) < 1e-6
+    check_axes(comp, want)
+    same = P.pca_scaled(ip, ix, dx, n, m, k, max_value, components=comp)
+    # the scaled values' column means are float64 sums of non-integers, whose
+    # order may move the last bit: within 2 ulp of the column's largest entry
+    for j in range(k):
+        col = same["embedding"][:, j]
+        assert np.max(np.abs(emb[:, j] - col)) <= 2 * np.spacing(np.max(np.abs(col)))
+
+
+def test_scaled_large_matches_oracle():
+    mat = structured_counts(4000, 800, 10, 0.25, 21)
+    ip, ix, dx = parts(mat)
+    emb, vr, comp = run(mat, 40, scale=True, max_value=10.0)
+    want = oracle(ip, ix, dx, 4000, 800, 40, scaled=True, max_value=10.0)
+    assert np.max(np.abs(vr - want["variance_ratio"])) < 1e-6
+    check_axes(comp, want, 1e-5)
+
+
+def test_runs_repeat_bit_for_bit_and_device_entry_agrees():
+    import torch
+    mat = structured_counts(3000, 500, 8, 0.3, 7)
+    a = run(mat, 20)
+    b = run(mat, 20)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    ip, ix, dx = parts(mat)
+    dev = torch.device("cuda:0")
+    comp = torch.empty((500, 20), dtype=torch.float32, device=dev)
+    e, r = gp.pca_csr_device(torch.from_numpy(ip.astype(np.int64)).to(dev),
+                             torch.from_numpy(ix.astype(np.int32)).to(dev),
+                             torch.from_numpy(dx).to(dev), 500, 20, components=comp)
+    torch.cuda.synchronize()
+    assert np.array_equal(e.cpu().numpy(), a[0])
+    assert np.array_equal(r.cpu().numpy(), a[1])
+    assert np.array_equal(comp.cpu().numpy(), a[2])
+
+
+def test_out_array_receives_the_embedding():
+    mat = structured_counts(700, 60, 5, 0.3, 8)
+    out = np.zeros((700, 7), np.float32)
+    r = gp.pca_sparse(mat, 7, True, 10, 4, 0, out=out)
+    assert r["embedding"] is None and r["n_components"] == 7
+    emb, vr, _ = run(mat, 7)
+    assert np.array_equal(out, emb) and np.array_equal(r["variance_ratio"], vr)
+
+
+def test_reference_fixtures():
+    # dense_pca_recovers_low_rank_structure / sparse_pca_matches_dense_on_same_data (pca.rs:708-796)
+    d = np.zeros((50, 5), np.float32)
+    t = np.arange(50, dtype=np.float32)
+    d[:, 0] = t
+    d[:, 1] = t * np.float32(-2.0) + (np.arange(50) % 3).astype(np.float32) * np.float32(0.001)
+    d[:, 2] = np.sin(t * np.float32(0.7)).astype(np.float32) * np.float32(1e-6)
+    mat = sp.csr_matrix(d)
+    emb, vr, _ = run(mat, 2, True)
+    assert vr.sum() > 0.999 and vr[0] > vr[1] and emb.shape == (50, 2)
+    _, vr_u, _ = run(mat, 2, False)
+    assert vr_u.sum() > 0.95
+    # against a dense thin SVD, the reference's tolerance (pca.rs:861-905)
+    mat = counts(600, 40, 0.2, 3)
+    emb, vr, _ = run(mat, 6)
+    x = mat.toarray().astype(np.float64)
+    u, s, _ = np.linalg.svd(x - x.mean(axis=0), full_matrices=False)
+    assert np.max(np.abs(vr - s[:6] ** 2 / (s ** 2).sum())) < 1e-4
+    for j in range(6):
+        ref = u[:, j] * s[j]
+        c = abs(emb[:, j].astype(np.float64) @ ref) / (np.linalg.norm(emb[:, j]) * np.linalg.norm(ref))
+        assert c > 0.9999
+
+
+def test_full_space_and_rank_deficient():
+    # block = n_features: the Rayleigh-Ritz step alone decides
+    mat = structured_counts(400, 20, 4, 0.5, 9)
+    ip, ix, dx = parts(mat)
+    emb, vr, comp = run(mat, 10)
+    want = oracle(ip, ix, dx, 400, 20, 10)
+    assert np.max(np.abs(vr - want["variance_ratio"])) < 1e-6
+    check_axes(comp, want)
+    # 8 informative genes of 200, 20 components: the block (36) exceeds the rank
+    rng = np.random.default_rng(10)
+    x = np.zeros((300, 200), np.float32)
+    x[:, :8] = rng.poisson(2.0, size=(300, 8))
+    mat = sp.csr_matrix(x)
+    ip, ix, dx = parts(mat)
+    emb, vr, comp = run(mat, 20)
+    want = oracle(ip, ix, dx, 300, 200, 20)
+    assert np.max(np.abs(vr - want["variance_ratio"])) < 1e-6
+    check_axes(comp, want)
+    assert np.all(vr[8:] < 1e-6)
+    assert np.max(np.abs(emb[:, 8:])) < 1e-3
+
+
+def test_all_zero_matrix():
+    mat = sp.csr_matrix((30, 12), dtype=np.float32)
+    emb, vr, comp = run(mat, 3)
+    assert np.all(np.isnan(vr))  # 0 / 0, as the reference's max(0, 0) / trace
+    assert np.array_equal(emb, np.zeros((30, 3), np.float32))
+    assert np.array_equal(comp, np.eye(12, dtype=np.float32)[:, ::-1][:, :3])
+
+
+def test_widest_gene_count():
+    print("pca m=4096", flush=True)
+    mat = structured_counts(3000, 4096, 10, 0.08, 12)
+    ip, ix, dx = parts(mat)
+    emb, vr, comp = run(mat, 30)
+    want = oracle(ip, ix, dx, 3000, 4096, 30)
+    assert np.max(np.abs(vr - want["variance_ratio"])) < 1e-6
+    check_axes(comp, want, 1e-5)
+
+
+def test_argument_errors():
+    mat = counts(100, 20, 0.3, 1)
+    with pytest.raises(RuntimeError, match="exceeds min"):
+        gp.pca_sparse(mat, 21, True, 10, 4, 0)
+    with pytest.raises(RuntimeError, match="4096"):
+        gp.pca_sparse(counts(10, 4097, 0.01, 2), 2, True, 10, 4, 0)
+    ip, ix, dx = parts(mat)
+    bad = ix.copy()
+    bad[3] = 20
+    with pytest.raises(RuntimeError, match="column index"):
+        gp.pca_sparse((ip, bad, dx, (100, 20)), 3, True, 10, 4, 0)
+    with pytest.raises(RuntimeError, match="non-finite"):
+        d = dx.copy()
+        d[0] = np.inf
+        gp.pca_sparse((ip, ix, d, (100, 20)), 3, True, 10, 4, 0)
+    e, v, _ = run(mat, 3)
+    assert np.all(np.isfinite(e)) and np.all(np.isfinite(v)) and not math.isnan(float(v.sum()))
