@@ -173,14 +173,14 @@ int psq_pca_csr_device(const uint64_t* d_indptr, const uint32_t* d_indices, cons
                        float* d_components, void* stream);
 /* Phases of this thread's last successful PCA call, from HIP events on its
  * stream: the scaled view's statistics and values, the Gram pass (densified row
- * chunks, column sums and SYRK; ms_syrk is the SYRK launches alone), the
- * eigensolver, and projection + signs + ratios. */
+ * chunks, column sums and the DGEMM products; ms_gemm is those launches alone),
+ * the eigensolver, and projection + signs + ratios. */
 typedef struct psq_pca_report {
-  uint64_t iterations; /* subspace steps */
+  uint64_t iterations; /* block products with C in the eigensolver */
   uint64_t block;      /* subspace block b */
   uint64_t chunks;     /* dense row chunks of the Gram pass */
   uint64_t rows_per_chunk;
-  double ms_scale, ms_gram, ms_syrk, ms_eigen, ms_project, ms_total;
+  double ms_scale, ms_gram, ms_gemm, ms_eigen, ms_project, ms_total;
 } psq_pca_report;
 int psq_pca_last_report(psq_pca_report* out);
 

@@ -74,7 +74,7 @@ class PcaReport(ctypes.Structure):
     _fields_ = [
         ("iterations", ctypes.c_uint64), ("block", ctypes.c_uint64), ("chunks", ctypes.c_uint64),
         ("rows_per_chunk", ctypes.c_uint64), ("ms_scale", ctypes.c_double),
-        ("ms_gram", ctypes.c_double), ("ms_syrk", ctypes.c_double), ("ms_eigen", ctypes.c_double),
+        ("ms_gram", ctypes.c_double), ("ms_gemm", ctypes.c_double), ("ms_eigen", ctypes.c_double),
         ("ms_project", ctypes.c_double), ("ms_total", ctypes.c_double),
     ]
 

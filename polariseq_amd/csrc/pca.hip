@@ -14,15 +14,19 @@
 //   variance ratio = max(lambda_j, 0) / trace(C).
 //
 // The reference takes the eigendecomposition with a dense solver; here the top
-// k come from subspace iteration on C + sigma I (sigma = 1e-4 trace(C) / m keeps
-// every block full rank; the eigenvectors are C's) with block b = k +
-// max(16, k / 2), orthonormalised each step by shifted CholeskyQR3 (Fukaya,
-// Kannan, Nakatsukasa, Yamamoto, Yanagisawa, SIAM J. Sci. Comput. 42 (2020)
-// A477), with a Rayleigh-Ritz step (cyclic Jacobi in round-robin order on the
-// b x b projection, one workgroup) every 4 steps, until every wanted residual
+// k come from Chebyshev-accelerated subspace iteration (Rutishauser's
+// simultaneous iteration, Numer. Math. 16 (1970) 205-223) on A = C + sigma I
+// (sigma = 1e-4 trace(C) / m keeps every block full rank; the eigenvectors are
+// C's), block b = k + max(16, k / 2): a Rayleigh-Ritz step (cyclic Jacobi in
+// round-robin order on the b x b projection, one workgroup), then a Chebyshev
+// filter T_d(A / e) on [-e, e], e the block's smallest Ritz value, with the
+// converged leading Ritz pairs locked (left out of the filter and deflated from
+// A), then shifted CholeskyQR3 (Fukaya, Kannan, Nakatsukasa, Yamamoto,
+// Yanagisawa, SIAM J. Sci. Comput. 42 (2020) A477), until every wanted residual
 // is below 1e-11 of the largest eigenvalue. The Gram of densified row chunks,
-// C Q and the small products are rocBLAS f64 calls with atomics disabled; every
-// other reduction has a fixed order, so a run repeats bit for bit.
+// the products with A and the small products are rocBLAS f64 calls with
+// atomics disabled; every other reduction has a fixed order, so a run repeats
+// bit for bit.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 
@@ -65,6 +69,8 @@ constexpr uint32_t PCA_GRAM_MAX_FEATURES = 4096;  // pca.rs:201
 constexpr uint32_t PCA_MAX_BLOCK = 1024;          // subspace block b
 constexpr uint32_t PCA_LDS_BLOCK = 80;            // b x b f64 (50 KB) kept in LDS
 constexpr uint32_t PCA_SEGMENTS = 64;             // row segments of the column sums
+constexpr uint32_t PCA_MAX_PRODUCTS = 20000;      // block products with C before giving up
+constexpr int PCA_MAX_DEGREE = 24;                // Chebyshev filter degree cap
 
 // ── input check ──
 
@@ -157,7 +163,7 @@ __global__ void k_pca_means(const double* __restrict__ s, uint32_t m, uint64_t n
   if (j < m) mu[j] = (float)(s[j] / (double)n);
 }
 
-// C (column-major, upper triangle from SYRK) -> full centred matrix. Element
+// C (column-major; the upper triangle of the Gram product) -> full centred matrix. Element
 // (row, col), row <= col, is the reference's lower-triangle element (col, row):
 // C - (mu_col n) mu_row.
 __global__ void k_pca_center(double* __restrict__ C, uint32_t m, uint64_t n, const float* __restrict__ mu) {
@@ -188,6 +194,19 @@ __global__ void k_pca_start(double* __restrict__ Q, uint32_t m, uint32_t b) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (uint64_t)m * b) return;
   Q[t] = (double)(splitmix(0x9E3779B97F4A7C15ull ^ splitmix(t + 0x5EEDull)) >> 11) * 0x1.0p-52 - 1.0;
+}
+
+// Y = a Y + b X, elementwise (X may be Y).
+__global__ void k_pca_axpby(double* Y, double a, const double* X, double b, uint64_t count) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) Y[t] = a * Y[t] + b * X[t];
+}
+
+// H[i][j] *= lam[i] (column-major nl x nb, count = nl nb).
+__global__ void k_pca_scale_rows(double* __restrict__ H, const double* __restrict__ lam, uint32_t nl,
+                                 uint64_t count) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) H[t] *= lam[t % nl];
 }
 
 // Z += sigma Q (the shifted operator), elementwise.
@@ -620,7 +639,7 @@ int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t 
   Events evs;
   psq_pca_report rep{};
   size_t ev_start, ev_scaled, ev_gram, ev_eigen, ev_end;
-  std::vector<std::pair<size_t, size_t>> ev_syrk;
+  std::vector<std::pair<size_t, size_t>> ev_gemm;
   PTRY(evs.record(s, &ev_start));
   {
     uint32_t* d_bad;
@@ -687,12 +706,15 @@ int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t 
   for (uint64_t r0 = 0; r0 < n; r0 += R) {
     const uint32_t rc = (uint32_t)std::min<uint64_t>(R, n - r0);
     PTRY(chunk_sums(r0, rc, S, nullptr, nullptr));
-    // D is row-major rc x m, i.e. column-major m x rc: C += D^T D (upper)
+    // D is row-major rc x m, i.e. column-major A = m x rc: C += A A^T. The full
+    // product (DGEMM, N/T) runs near the FP64 matrix peak where DSYRK's half
+    // product does not; only the upper triangle is read afterwards.
     size_t a, b2;
     PTRY(evs.record(s, &a));
-    PBLAS(rocblas_dsyrk(h, rocblas_fill_upper, rocblas_operation_none, m, rc, &one, D, m, &one, C, m));
+    PBLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, m, m, rc, &one, D, m, D, m, &one,
+                        C, m));
     PTRY(evs.record(s, &b2));
-    ev_syrk.emplace_back(a, b2);
+    ev_gemm.emplace_back(a, b2);
     ++rep.chunks;
   }
   rep.rows_per_chunk = R;
@@ -734,42 +756,108 @@ int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t 
     hipLaunchKernelGGL(k_pca_start, dim3(blocks((uint64_t)m * b, 256)), dim3(256), 0, s, Q, m, b);
     PHIPC(hipGetLastError());
     PTRY(orthonormalise(h, Q, m, b, G, s));
+    double *Y = nullptr, *H = nullptr;  // the filter's fourth block; lead x b products
+    PTRY(dv.alloc((size_t)m * b, &Y));
+    PTRY(dv.alloc((size_t)b * b, &H));
+    // Zo <- (C + sigma I) X for nb columns, minus V_L diag(lam_L) V_L^T X for the
+    // first nl columns of Q (locked Ritz pairs, deflated to eigenvalue 0)
+    auto apply = [&](const double* X, double* Zo, uint32_t nb, const double* VL, uint32_t nl) -> int {
+      const uint64_t cnt = (uint64_t)m * nb;
+      PBLAS(rocblas_dsymm(h, rocblas_side_left, rocblas_fill_upper, m, nb, &one, C, m, X, m, &zero, Zo, m));
+      hipLaunchKernelGGL(k_pca_shift, dim3(blocks(cnt, 256)), dim3(256), 0, s, Zo, X, cnt, tv);
+      PHIPC(hipGetLastError());
+      if (nl) {
+        const double minus = -1.0;
+        PBLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nl, nb, m, &one, VL, m, X, m,
+                            &zero, H, nl));
+        hipLaunchKernelGGL(k_pca_scale_rows, dim3(blocks((uint64_t)nl * nb, 256)), dim3(256), 0, s, H, lam, nl,
+                           (uint64_t)nl * nb);
+        PHIPC(hipGetLastError());
+        PBLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, nb, nl, &minus, VL, m, H, nl,
+                            &one, Zo, m));
+      }
+      ++rep.iterations;
+      return PSQ_OK;
+    };
     std::vector<double> hres(kk), hlam(b);
     bool converged = false;
-    for (int it = 1; it <= 4000; ++it) {
-      rep.iterations = it;
-      PBLAS(rocblas_dsymm(h, rocblas_side_left, rocblas_fill_upper, m, b, &one, C, m, Q, m, &zero, Z, m));
-      hipLaunchKernelGGL(k_pca_shift, dim3(blocks((uint64_t)m * b, 256)), dim3(256), 0, s, Z, Q,
-                         (uint64_t)m * b, tv);
+    while (rep.iterations < PCA_MAX_PRODUCTS) {
+      PTRY(apply(Q, Z, b, nullptr, 0));
+      // Rayleigh-Ritz: T = Q^T Z = W L W^T; Q <- Q W, Z <- Z W
+      PBLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, b, b, m, &one, Q, m, Z, m,
+                          &zero, Tb, b));
+      hipLaunchKernelGGL(k_pca_jacobi, dim3(1), dim3(256), small_lds(b), s, Tb, b, Wb, Ws, lam, cs);
       PHIPC(hipGetLastError());
-      if (it % 4 == 0 || m == b) {
-        // Rayleigh-Ritz: T = Q^T Z = W L W^T; Q <- Q W, Z <- Z W
-        PBLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, b, b, m, &one, Q, m, Z, m,
-                            &zero, Tb, b));
-        hipLaunchKernelGGL(k_pca_jacobi, dim3(1), dim3(256), small_lds(b), s, Tb, b, Wb, Ws, lam, cs);
+      PBLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, b, b, &one, Q, m, Ws, b, &zero,
+                          Tmp, m));
+      std::swap(Q, Tmp);
+      PBLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, b, b, &one, Z, m, Ws, b, &zero,
+                          Tmp, m));
+      std::swap(Z, Tmp);
+      hipLaunchKernelGGL(k_pca_residuals, dim3(kk), dim3(256), 0, s, Z, Q, lam, m, res);
+      PHIPC(hipGetLastError());
+      PHIPC(hipMemcpyAsync(hres.data(), res, kk * sizeof(double), hipMemcpyDeviceToHost, s));
+      PHIPC(hipMemcpyAsync(hlam.data(), lam, b * sizeof(double), hipMemcpyDeviceToHost, s));
+      PHIPC(hipStreamSynchronize(s));
+      const double top = std::max(std::fabs(hlam[0]), 1e-300);
+      converged = true;
+      uint32_t lead = kk;  // the converged leading Ritz pairs are locked
+      for (uint32_t i = 0; i < kk; ++i)
+        if (!(hres[i] <= 1e-11 * top)) {
+          converged = false;
+          if (lead == kk) lead = i;
+        }
+      if (m == b) converged = true;  // the whole space: the projection's eigenpairs are C's
+      if (converged) break;
+      // Chebyshev filter T_d(A / e) on the unlocked columns, A = C + sigma I
+      // deflated by the locked pairs, e = the block's smallest Ritz value:
+      // components at or below e (the locked ones, now at 0, included) stay
+      // bounded by 1, the wanted ones grow by cosh(d acosh(lambda / e)); d keeps
+      // the growth of the first unlocked pair under 1e7, so the filtered block
+      // stays well conditioned for CholeskyQR.
+      const double e = hlam[b - 1];
+      const double lu = hlam[lead];
+      int d = PCA_MAX_DEGREE;
+      if (e > 0.0 && lu > e) d = (int)std::floor(std::acosh(1e7) / std::acosh(lu / e));
+      d = std::max(1, std::min(d, PCA_MAX_DEGREE));
+      if (!(e > 0.0)) d = 1;
+      const uint32_t nb = b - lead;
+      const size_t off = (size_t)lead * m;
+      // Y0 = Q, Y1 = Z / e, Y_{i+1} = (2 / e) A Y_i - Y_{i-1}, on columns [lead, b)
+      // (Z = A Q needs no deflation: the Ritz vectors are orthogonal to the locked ones)
+      double *prev = Q, *cur = Z, *nxt = Tmp, *spare = Y;
+      hipLaunchKernelGGL(k_pca_axpby, dim3(blocks((uint64_t)m * nb, 256)), dim3(256), 0, s, cur + off, 1.0 / e,
+                         cur + off, 0.0, (uint64_t)m * nb);
+      PHIPC(hipGetLastError());
+      for (int i = 2; i <= d; ++i) {
+        PTRY(apply(cur + off, nxt + off, nb, Q, lead));
+        hipLaunchKernelGGL(k_pca_axpby, dim3(blocks((uint64_t)m * nb, 256)), dim3(256), 0, s, nxt + off,
+                           2.0 / e, prev + off, -1.0, (uint64_t)m * nb);
         PHIPC(hipGetLastError());
-        PBLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, b, b, &one, Q, m, Ws, b, &zero,
-                            Tmp, m));
-        std::swap(Q, Tmp);
-        PBLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, b, b, &one, Z, m, Ws, b, &zero,
-                            Tmp, m));
-        std::swap(Z, Tmp);
-        hipLaunchKernelGGL(k_pca_residuals, dim3(kk), dim3(256), 0, s, Z, Q, lam, m, res);
-        PHIPC(hipGetLastError());
-        PHIPC(hipMemcpyAsync(hres.data(), res, kk * sizeof(double), hipMemcpyDeviceToHost, s));
-        PHIPC(hipMemcpyAsync(hlam.data(), lam, b * sizeof(double), hipMemcpyDeviceToHost, s));
-        PHIPC(hipStreamSynchronize(s));
-        const double top = std::max(std::fabs(hlam[0]), 1e-300);
-        converged = true;
-        for (uint32_t i = 0; i < kk; ++i)
-          if (!(hres[i] <= 1e-11 * top)) converged = false;
-        if (m == b) converged = true;  // the whole space: the projection's eigenpairs are C's
-        if (converged) break;
+        double* t = prev;
+        prev = cur;
+        cur = nxt;
+        if (i == 2) {
+          nxt = spare;
+          spare = t;  // Q's buffer: the locked columns stay readable there
+        } else {
+          nxt = t;
+        }
       }
-      std::swap(Q, Z);
+      // the new block: the locked columns, then the filtered ones
+      if (lead) PHIPC(hipMemcpyAsync(cur, Q, off * sizeof(double), hipMemcpyDeviceToDevice, s));
+      double* rest[3];
+      int r = 0;
+      for (double* p : {Q, Z, Tmp, Y})
+        if (p != cur) rest[r++] = p;
+      Q = cur;
+      Z = rest[0];
+      Tmp = rest[1];
+      Y = rest[2];
       PTRY(orthonormalise(h, Q, m, b, G, s));
     }
-    if (!converged) return fail(PSQ_E_DEVICE, "the eigensolver did not converge in 4000 iterations");
+    if (!converged)
+      return fail(PSQ_E_DEVICE, "the eigensolver did not converge in %u products", PCA_MAX_PRODUCTS);
   }
 
   PTRY(evs.record(s, &ev_eigen));
@@ -801,7 +889,7 @@ int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t 
   PHIPC(hipStreamSynchronize(s));  // the buffers freed on return are no longer in use
   rep.ms_scale = evs.ms(ev_start, ev_scaled);
   rep.ms_gram = evs.ms(ev_scaled, ev_gram);
-  for (const auto& p : ev_syrk) rep.ms_syrk += evs.ms(p.first, p.second);
+  for (const auto& p : ev_gemm) rep.ms_gemm += evs.ms(p.first, p.second);
   rep.ms_eigen = evs.ms(ev_gram, ev_eigen);
   rep.ms_project = evs.ms(ev_eigen, ev_end);
   rep.ms_total = evs.ms(ev_start, ev_end);
