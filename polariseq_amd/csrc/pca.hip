@@ -85,20 +85,24 @@ __global__ void k_pca_check(const uint64_t* __restrict__ indptr, uint64_t n,
 
 // ── column sums and the scaled view ──
 
-// Rows [r0, r0 + R) as dense f64 rows of m, stored values added in order (the
-// chunk must be zero on entry). One thread per row.
+// Rows [r0, r0 + R) as dense f64 rows of ld >= m + 1 elements: the stored
+// values at their columns, 1 at column m (its Gram row holds the column sums),
+// the chunk zero on entry. One wave per row, lanes over its entries; a column
+// stored twice in a row (not canonical CSR) gets both values added.
 __global__ void k_pca_densify(const uint64_t* __restrict__ indptr, const uint32_t* __restrict__ indices,
-                              const float* __restrict__ val, uint64_t r0, uint32_t R, uint32_t m,
+                              const float* __restrict__ val, uint64_t r0, uint32_t R, uint32_t m, uint32_t ld,
                               double* __restrict__ D) {
-  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t r = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64, lane = threadIdx.x % 64;
   if (r >= R) return;
-  double* row = D + (size_t)r * m;
-  for (uint64_t e = indptr[r0 + r]; e < indptr[r0 + r + 1]; ++e) row[indices[e]] += (double)val[e];
+  double* row = D + (size_t)r * ld;
+  if (lane == 0) row[m] = 1.0;
+  for (uint64_t e = indptr[r0 + r] + lane; e < indptr[r0 + r + 1]; e += 64)
+    unsafeAtomicAdd(&row[indices[e]], (double)val[e]);
 }
 
-// Column sums (and sums of squares when ps_q is set) of a dense chunk, per row
+// Column sums and sums of squares of a dense chunk (rows of ld), per row
 // segment: part[p][j] over rows [p seg, (p + 1) seg). Grid (ceil(m / 256), P).
-__global__ void k_pca_colsums_part(const double* __restrict__ D, uint32_t R, uint32_t m, uint32_t seg,
+__global__ void k_pca_colsums_part(const double* __restrict__ D, uint32_t R, uint32_t m, uint32_t ld, uint32_t seg,
                                    double* __restrict__ ps_s, double* __restrict__ ps_q) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t p = blockIdx.y;
@@ -106,12 +110,12 @@ __global__ void k_pca_colsums_part(const double* __restrict__ D, uint32_t R, uin
   const uint32_t lo = p * seg, hi = lo + seg < R ? lo + seg : R;
   double a = 0.0, b = 0.0;
   for (uint32_t r = lo; r < hi; ++r) {
-    const double x = D[(size_t)r * m + j];
+    const double x = D[(size_t)r * ld + j];
     a += x;
     b += x * x;
   }
   ps_s[(size_t)p * m + j] = a;
-  if (ps_q) ps_q[(size_t)p * m + j] = b;
+  ps_q[(size_t)p * m + j] = b;
 }
 // acc[j] += the P partials in segment order.
 __global__ void k_pca_colsums_fold(const double* __restrict__ ps_s, const double* __restrict__ ps_q,
@@ -121,10 +125,21 @@ __global__ void k_pca_colsums_fold(const double* __restrict__ ps_s, const double
   double a = 0.0, b = 0.0;
   for (uint32_t p = 0; p < P; ++p) {
     a += ps_s[(size_t)p * m + j];
-    if (ps_q) b += ps_q[(size_t)p * m + j];
+    b += ps_q[(size_t)p * m + j];
   }
   s[j] += a;
-  if (q) q[j] += b;
+  q[j] += b;
+}
+
+// The m x m Gram matrix and the column sums out of the (ld x ld) product of
+// the chunks with their ones column: C[i][j] = G[i][j], s[j] = G[m][j].
+__global__ void k_pca_gram_unpack(const double* __restrict__ G, uint32_t m, uint32_t ld, double* __restrict__ C,
+                                  double* __restrict__ s) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (uint64_t)m * m) return;
+  const uint32_t col = (uint32_t)(t / m), row = (uint32_t)(t % m);
+  C[t] = G[(size_t)col * ld + row];
+  if (row == 0) s[col] = G[(size_t)col * ld + m];
 }
 
 // Gene statistics -> the scaled view's mean, 1/sd and clipped zero
@@ -655,21 +670,20 @@ int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t 
     if (bad & 1) return fail(PSQ_E_INVALID, "a column index is >= n_features");
   }
 
-  // dense f64 row chunks of at most 2^25 elements (256 MB)
-  const uint32_t R = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(n, (1ull << 25) / m));
+  // dense f64 row chunks of at most 2^25 elements (256 MB), rows of ld = m + 1
+  // (the ones column) rounded up to 8
+  const uint32_t ld = (m + 1 + 7) & ~7u;
+  const uint32_t R = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(n, (1ull << 25) / ld));
   const uint32_t P = std::min<uint32_t>(PCA_SEGMENTS, R);
-  double *D, *acc_s, *acc_q = nullptr, *ps_s, *ps_q = nullptr, *C;
-  PTRY(dv.alloc((size_t)R * m, &D));
+  double *D, *acc_s, *acc_q = nullptr, *ps_s = nullptr, *ps_q = nullptr, *C, *Gb;
+  PTRY(dv.alloc((size_t)R * ld, &D));
   PTRY(dv.alloc(m, &acc_s));
-  PTRY(dv.alloc((size_t)P * m, &ps_s));
   PTRY(dv.alloc((size_t)m * m, &C));
+  PTRY(dv.alloc((size_t)ld * ld, &Gb));
   const double one = 1.0, zero = 0.0;
-  auto chunk_sums = [&](uint64_t r0, uint32_t rc, const float* vals, double* q, double* psq) -> int {
-    PHIPC(hipMemsetAsync(D, 0, (size_t)rc * m * sizeof(double), s));
-    hipLaunchKernelGGL(k_pca_densify, dim3(blocks(rc, 256)), dim3(256), 0, s, ip, ix, vals, r0, rc, m, D);
-    const uint32_t Pc = std::min<uint32_t>(P, rc), seg = (rc + Pc - 1) / Pc;
-    hipLaunchKernelGGL(k_pca_colsums_part, dim3(blocks(m, 256), Pc), dim3(256), 0, s, D, rc, m, seg, ps_s, psq);
-    hipLaunchKernelGGL(k_pca_colsums_fold, dim3(blocks(m, 256)), dim3(256), 0, s, ps_s, psq, Pc, m, acc_s, q);
+  auto densify = [&](uint64_t r0, uint32_t rc, const float* vals) -> int {
+    PHIPC(hipMemsetAsync(D, 0, (size_t)rc * ld * sizeof(double), s));
+    hipLaunchKernelGGL(k_pca_densify, dim3(blocks(rc, 4)), dim3(256), 0, s, ip, ix, vals, r0, rc, m, ld, D);
     PHIPC(hipGetLastError());
     return PSQ_OK;
   };
@@ -679,6 +693,7 @@ int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t 
     double *mean, *inv, *zr;
     float* sv;
     PTRY(dv.alloc(m, &acc_q));
+    PTRY(dv.alloc((size_t)P * m, &ps_s));
     PTRY(dv.alloc((size_t)P * m, &ps_q));
     PTRY(dv.alloc(m, &mean));
     PTRY(dv.alloc(m, &inv));
@@ -688,7 +703,13 @@ int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t 
     PHIPC(hipMemsetAsync(acc_q, 0, m * sizeof(double), s));
     for (uint64_t r0 = 0; r0 < n; r0 += R) {
       const uint32_t rc = (uint32_t)std::min<uint64_t>(R, n - r0);
-      PTRY(chunk_sums(r0, rc, x, acc_q, ps_q));
+      PTRY(densify(r0, rc, x));
+      const uint32_t Pc = std::min<uint32_t>(P, rc), seg = (rc + Pc - 1) / Pc;
+      hipLaunchKernelGGL(k_pca_colsums_part, dim3(blocks(m, 256), Pc), dim3(256), 0, s, D, rc, m, ld, seg, ps_s,
+                         ps_q);
+      hipLaunchKernelGGL(k_pca_colsums_fold, dim3(blocks(m, 256)), dim3(256), 0, s, ps_s, ps_q, Pc, m, acc_s,
+                         acc_q);
+      PHIPC(hipGetLastError());
     }
     hipLaunchKernelGGL(k_pca_scale_stats, dim3(blocks(m, 256)), dim3(256), 0, s, acc_s, acc_q, m, n, clip, mean,
                        inv, zr);
@@ -700,24 +721,25 @@ int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t 
   }
   PTRY(evs.record(s, &ev_scaled));
 
-  // Gram of the stored values and their column sums
-  PHIPC(hipMemsetAsync(acc_s, 0, m * sizeof(double), s));
-  PHIPC(hipMemsetAsync(C, 0, (size_t)m * m * sizeof(double), s));
+  // Gram of the stored values; its row m (the ones column) holds the column sums
+  PHIPC(hipMemsetAsync(Gb, 0, (size_t)ld * ld * sizeof(double), s));
   for (uint64_t r0 = 0; r0 < n; r0 += R) {
     const uint32_t rc = (uint32_t)std::min<uint64_t>(R, n - r0);
-    PTRY(chunk_sums(r0, rc, S, nullptr, nullptr));
-    // D is row-major rc x m, i.e. column-major A = m x rc: C += A A^T. The full
+    PTRY(densify(r0, rc, S));
+    // D is row-major rc x ld, i.e. column-major A = ld x rc: G += A A^T. The full
     // product (DGEMM, N/T) runs near the FP64 matrix peak where DSYRK's half
-    // product does not; only the upper triangle is read afterwards.
+    // product does not.
     size_t a, b2;
     PTRY(evs.record(s, &a));
-    PBLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, m, m, rc, &one, D, m, D, m, &one,
-                        C, m));
+    PBLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, ld, ld, rc, &one, D, ld, D, ld,
+                        &one, Gb, ld));
     PTRY(evs.record(s, &b2));
     ev_gemm.emplace_back(a, b2);
     ++rep.chunks;
   }
   rep.rows_per_chunk = R;
+  hipLaunchKernelGGL(k_pca_gram_unpack, dim3(blocks((uint64_t)m * m, 256)), dim3(256), 0, s, Gb, m, ld, C, acc_s);
+  PHIPC(hipGetLastError());
   float* mu = nullptr;
   if (center) {
     PTRY(dv.alloc(m, &mu));
