@@ -65,7 +65,9 @@ using psq_host::fail;
 
 namespace psq {
 
-constexpr uint32_t PCA_GRAM_MAX_FEATURES = 4096;  // pca.rs:201
+// The reference takes the Gram route up to 4096 genes (pca.rs:201) and randomized
+// SVD above; this build stays exact up to 16384 (C: 2 GiB of f64 in HBM).
+constexpr uint32_t PCA_MAX_FEATURES = 16384;
 constexpr uint32_t PCA_MAX_BLOCK = 1024;          // subspace block b
 constexpr uint32_t PCA_LDS_BLOCK = 80;            // b x b f64 (50 KB) kept in LDS
 constexpr uint32_t PCA_SEGMENTS = 64;             // row segments of the column sums
@@ -670,10 +672,11 @@ int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t 
     if (bad & 1) return fail(PSQ_E_INVALID, "a column index is >= n_features");
   }
 
-  // dense f64 row chunks of at most 2^25 elements (256 MB), rows of ld = m + 1
-  // (the ones column) rounded up to 8
+  // dense f64 row chunks of at least 2^25 elements (256 MB) and 4096 rows, rows
+  // of ld = m + 1 (the ones column) rounded up to 8
   const uint32_t ld = (m + 1 + 7) & ~7u;
-  const uint32_t R = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(n, (1ull << 25) / ld));
+  const uint32_t R = (uint32_t)std::max<uint64_t>(
+      1, std::min<uint64_t>(n, std::max<uint64_t>(4096, (1ull << 25) / ld)));
   const uint32_t P = std::min<uint32_t>(PCA_SEGMENTS, R);
   double *D, *acc_s, *acc_q = nullptr, *ps_s = nullptr, *ps_q = nullptr, *C, *Gb;
   PTRY(dv.alloc((size_t)R * ld, &D));
@@ -929,11 +932,9 @@ int pca_validate(uint64_t n, uint64_t m, uint64_t k, int32_t scale, double max_v
                 (unsigned long long)k, (unsigned long long)most);
   if (k == 0) return fail(PSQ_E_INVALID, "n_components must be >= 1");
   if (scale && !(max_value >= 0.0)) return fail(PSQ_E_INVALID, "max_value must be >= 0 (or +inf for none)");
-  if (m > PCA_GRAM_MAX_FEATURES)
-    return fail(PSQ_E_UNSUPPORTED,
-                "n_features (%llu) > %u: the reference switches to randomized SVD there, which this build "
-                "does not provide",
-                (unsigned long long)m, PCA_GRAM_MAX_FEATURES);
+  if (m > PCA_MAX_FEATURES)
+    return fail(PSQ_E_UNSUPPORTED, "n_features (%llu) > %u is not supported (exact Gram route only)",
+                (unsigned long long)m, PCA_MAX_FEATURES);
   if (n >= (1ull << 40) / k) return fail(PSQ_E_UNSUPPORTED, "n_obs * n_components must be < 2^40");
   return PSQ_OK;
 }

@@ -12,7 +12,7 @@
   whose float64 column sums are exact in any order).
 * The reference's own test shapes and tolerances (pca.rs:754-905), a run that
   repeats bit for bit, the device entry equal to the host one, the full space
-  (block = n_features), rank-deficient input, an all-zero matrix, 4,096 genes,
+  (block = n_features), rank-deficient input, an all-zero matrix, 4,096 and 5,000 genes,
   and the argument errors.
 """
 
@@ -228,12 +228,15 @@ def test_all_zero_matrix():
     assert np.array_equal(comp, np.eye(12, dtype=np.float32)[:, ::-1][:, :3])
 
 
-def test_widest_gene_count():
-    print("pca m=4096", flush=True)
-    mat = structured_counts(3000, 4096, 10, 0.08, 12)
+@pytest.mark.parametrize("m", [4096, 5000])
+def test_wide_gene_counts(m):
+    # 4096: the reference's last Gram width; 5000: past it (the reference's
+    # randomized route), still exact here
+    print(f"pca m={m}", flush=True)
+    mat = structured_counts(3000, m, 10, 0.08, 12)
     ip, ix, dx = parts(mat)
     emb, vr, comp = run(mat, 30)
-    want = oracle(ip, ix, dx, 3000, 4096, 30)
+    want = oracle(ip, ix, dx, 3000, m, 30)
     assert np.max(np.abs(vr - want["variance_ratio"])) < 1e-6
     check_axes(comp, want, 1e-5)
 
@@ -242,8 +245,8 @@ def test_argument_errors():
     mat = counts(100, 20, 0.3, 1)
     with pytest.raises(RuntimeError, match="exceeds min"):
         gp.pca_sparse(mat, 21, True, 10, 4, 0)
-    with pytest.raises(RuntimeError, match="4096"):
-        gp.pca_sparse(counts(10, 4097, 0.01, 2), 2, True, 10, 4, 0)
+    with pytest.raises(RuntimeError, match="16384"):
+        gp.pca_sparse(counts(10, 16385, 0.01, 2), 2, True, 10, 4, 0)
     ip, ix, dx = parts(mat)
     bad = ix.copy()
     bad[3] = 20
