@@ -9,6 +9,7 @@ graph step's consumer inside ``pp.neighbors``; ``neighbors_graph`` is the graph
 computation of ``pp.neighbors`` itself (python/polariseq/__init__.py:2514-2600).
 ``pca_sparse`` / ``pca_sparse_scaled`` are the binding's sparse PCA calls
 (lib.rs:395-467), the step that produces the embedding the graph is built on.
+``RocmBackend`` implements the reference's ``ComputeBackend`` trait (pca, neighbors).
 """
 
 from .neighbors import (  # noqa: F401
@@ -37,5 +38,6 @@ from .graph import (  # noqa: F401,E402
     unit_rows_device,
 )
 from .pca import pca_csr_device, pca_last_report, pca_sparse, pca_sparse_scaled  # noqa: F401,E402
+from .backend import RocmBackend  # noqa: F401,E402
 
 __version__ = "0.1.0"

@@ -1324,27 +1324,31 @@ int psq_neighbors_exact(const float* embedding, uint64_t n_obs, uint64_t n_dim,
     *nnz_out = 0;
     return PSQ_OK;
   }
-  std::vector<float> unit;
-  const float* src = embedding;
+  TRY(set_device(device));
+  Workspace* ws;
+  TRY(workspace(&ws));
+  float *d_x, *d_src, *d_dist;
+  uint32_t* d_idx;
+  const size_t cnt = (size_t)n_obs * kk;
+  TRY(wbuf(ws, "ne_data", (size_t)n_obs * n_dim, &d_x));
+  TRY(wbuf(ws, "ne_indices", cnt, &d_idx));
+  TRY(wbuf(ws, "ne_dists", cnt, &d_dist));
+  HIPC(hipMemcpy(d_x, embedding, (size_t)n_obs * n_dim * sizeof(float), hipMemcpyHostToDevice));
+  d_src = d_x;
   if (metric == 1) {  // cosine: unit rows; 1 - cos = |a - b|^2 / 2 on unit vectors
-    unit.resize(n_obs * n_dim);
-    for (uint64_t i = 0; i < n_obs; ++i) {
-      double s = 0;
-      for (uint64_t t = 0; t < n_dim; ++t) s += (double)embedding[i * n_dim + t] * embedding[i * n_dim + t];
-      double nr = std::max(std::sqrt(s), 1e-12);
-      for (uint64_t t = 0; t < n_dim; ++t) unit[i * n_dim + t] = (float)(embedding[i * n_dim + t] / nr);
-    }
-    src = unit.data();
+    TRY(wbuf(ws, "ne_unit", (size_t)n_obs * n_dim, &d_src));
+    hipLaunchKernelGGL(k_unit_rows64, dim3((unsigned)((n_obs + 255) / 256)), dim3(256), 0, nullptr, d_x, n_obs,
+                       (uint32_t)n_dim, d_src);
+    HIPC(hipGetLastError());
   }
-  std::vector<float> dd(n_obs * kk);
-  TRY(psq_knn_exact(src, n_obs, n_dim, nullptr, n_obs, kk, indices, dd.data(), device));
-  for (uint64_t i = 0; i < n_obs; ++i) {
-    for (uint64_t s = 0; s < kk; ++s) {
-      float v = dd[i * kk + s];
-      dists[i * kk + s] = metric == 1 ? (float)((double)v * (double)v / 2.0) : v;
-    }
-    indptr[i + 1] = (uint32_t)((i + 1) * kk);
+  TRY(knn_exact_device_impl(d_src, n_obs, n_dim, nullptr, n_obs, kk, d_idx, d_dist, nullptr));
+  if (metric == 1) {
+    hipLaunchKernelGGL(k_half_square, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, nullptr, d_dist, cnt);
+    HIPC(hipGetLastError());
   }
+  HIPC(hipMemcpy(indices, d_idx, cnt * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  HIPC(hipMemcpy(dists, d_dist, cnt * sizeof(float), hipMemcpyDeviceToHost));
+  for (uint64_t i = 0; i < n_obs; ++i) indptr[i + 1] = (uint32_t)((i + 1) * kk);
   *nnz_out = n_obs * kk;
   return PSQ_OK;
 }

@@ -102,6 +102,7 @@ __global__ void k_mlp_forward(const float* X, uint32_t ldx, const uint32_t* idx,
 constexpr uint32_t ROWS_D_MAX = 256;
 __global__ void k_unit_rows(const float* x, uint64_t n, uint32_t d, float* out);
 __global__ void k_half_square(float* dist, uint64_t count);
+__global__ void k_unit_rows64(const float* x, uint64_t n, uint32_t d, float* out);
 
 // Encode on the FP32 matrix cores: operands packed per lane (ENC_PACKED floats),
 // ENC_WAVES wave-tiles of 16 rows per block.
