@@ -37,7 +37,13 @@ from .graph import (  # noqa: F401,E402
     neighbors_graph,
     unit_rows_device,
 )
-from .pca import pca_csr_device, pca_last_report, pca_sparse, pca_sparse_scaled  # noqa: F401,E402
+from .pca import (  # noqa: F401,E402
+    pca_csr_device,
+    pca_dense,
+    pca_last_report,
+    pca_sparse,
+    pca_sparse_scaled,
+)
 from .backend import RocmBackend  # noqa: F401,E402
 
 __version__ = "0.1.0"

@@ -108,6 +108,22 @@ def pca_sparse_scaled(matrix, n_components: int, max_value=None, seed: int = 0, 
     return _run(matrix, n_components, True, True, max_value, out, components, device)
 
 
+def pca_dense(data, n_components: int, center: bool, seed: int, *, components=None,
+              device: int = 0) -> dict:
+    """lib.rs:355-388: PCA of a dense float32 (n_obs, n_features) array, the call
+    ``pp.pca`` makes for dense input (python/polariseq/__init__.py:2426-2434,
+    2448-2453). The reference densely factorises (pca.rs:176-223, a thin SVD
+    with arbitrary column signs, or randomized SVD when large); here the matrix
+    goes through the same exact Gram route as the sparse calls, as the CSR of
+    its nonzero entries, so the columns carry canonical signs."""
+    del seed
+    import scipy.sparse as sp
+    x = np.asarray(data, dtype=np.float32)
+    if x.ndim != 2:
+        raise ValueError("data must be a 2-D float32 array")
+    return _run(sp.csr_matrix(x), n_components, center, False, None, None, components, device)
+
+
 def pca_last_report() -> dict:
     """Phase times (HIP events on the call's stream) and solver counts of this
     thread's last successful PCA call."""

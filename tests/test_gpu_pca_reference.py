@@ -130,3 +130,18 @@ def test_embedding_stays_in_hbm_for_the_graph():
     assert np.array_equal(g_ip.cpu().numpy().view(np.uint32), want["indptr"])
     assert np.array_equal(g_ix[:nnz].cpu().numpy().view(np.uint32), want["indices"])
     assert np.array_equal(g_dx[:nnz].cpu().numpy(), want["data"])
+
+
+def test_dense_input():
+    """pca_dense (lib.rs:355-388): the dense form agrees with the sparse call on the
+    same matrix bit for bit, and with the exact SVD at the reference's bounds
+    (its dense route is a thin SVD, pca.rs:567-650)."""
+    x = structured_counts(n=2000, m=600, seed=8)
+    d = x.toarray()
+    r = gp.pca_dense(d, 8, True, 0)
+    s_ = gp.pca_sparse(x, 8, True, 10, 4, 0)
+    assert np.array_equal(r["embedding"], s_["embedding"])
+    xc = d.astype(np.float64) - d.astype(np.float64).mean(axis=0)
+    u, s, _ = np.linalg.svd(xc, full_matrices=False)
+    np.testing.assert_allclose(r["variance_ratio"], s[:8] ** 2 / (s ** 2).sum(), rtol=1e-5)
+    assert principal_angles_deg(r["embedding"].astype(np.float64), u[:, :8] * s[:8]).max() < 1e-3
