@@ -549,28 +549,31 @@ def scrublet_workload(args, torch, P, ds) -> int:
 
 
 def pca_counts(torch, n, m, density, seed, chunk=65536):
-    """A synthetic log-normalised sparse expression matrix on the device: 20
-    cell groups, each with raised expression in its own 5% of the genes; entries
-    kept with probability `density`; values log1p(1 + floor(u * (1 + 4 g_j))),
-    tripled in the group's genes. CSR with sorted column indices per row."""
+    """A synthetic log-normalised sparse expression matrix on the device with the
+    structure PCA is for: ten gene programs of decaying strength (cell scores
+    N(0, s_p), s = 6 ... 0.8; gene loadings N(0, 1/10)) on top of gene-specific
+    count noise; entries kept with probability `density`; values
+    log1p(floor(u * (1 + 4 g_j)) + 1 + max(program signal, 0)). CSR with sorted
+    column indices per row."""
     gen = torch.Generator(device="cuda")
     gen.manual_seed(seed)
     gscale = 1.0 + 4.0 * torch.rand(m, generator=gen, device="cuda")
-    module = torch.randint(0, 20, (m,), generator=gen, device="cuda")
+    strength = torch.tensor([6.0, 5.0, 4.0, 3.0, 2.5, 2.0, 1.5, 1.2, 1.0, 0.8], device="cuda")
+    load = torch.randn((10, m), generator=gen, device="cuda") / 10 ** 0.5
     counts, idx, val = [], [], []
     for r0 in range(0, n, chunk):
         rc = min(chunk, n - r0)
-        group = torch.randint(0, 20, (rc, 1), generator=gen, device="cuda")
+        score = torch.randn((rc, 10), generator=gen, device="cuda") * strength
         keep = torch.rand((rc, m), generator=gen, device="cuda") < density
         nz = keep.nonzero()
         rows, cols = nz[:, 0], nz[:, 1]
         u = torch.rand(rows.numel(), generator=gen, device="cuda")
-        v = torch.floor(u * gscale[cols]) + 1.0
-        v = torch.where(module[cols] == group[rows, 0], 3.0 * v, v)
+        sig = (score[rows] * load[:, cols].t()).sum(dim=1).clamp_min(0.0)
+        v = torch.floor(u * gscale[cols]) + 1.0 + sig
         counts.append(torch.bincount(rows, minlength=rc))
         idx.append(cols.to(torch.int32))
         val.append(torch.log1p(v).to(torch.float32))
-        del keep, nz
+        del keep, nz, sig
     indptr = torch.zeros(n + 1, dtype=torch.int64, device="cuda")
     indptr[1:] = torch.cumsum(torch.cat(counts), 0)
     return indptr, torch.cat(idx), torch.cat(val)
@@ -613,7 +616,7 @@ def pca_workload(args, torch, P, ds) -> int:
         "value": round(n / el, 1), "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(1000 * el, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic log-normalised sparse counts (torch, seeded), 20 groups",
+        "data": "synthetic log-normalised sparse counts with 10 gene programs (torch, seeded)",
         "config": {"workload": f"{n} cells x {m} genes, {nnz} stored ({dens:.0%}), {k} components, "
                                "centred, CSR resident in HBM", "nnz": nnz},
         "phases_ms": {kk: round(v, 3) for kk, v in rep.items() if kk.startswith("ms_")},
