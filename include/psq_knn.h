@@ -164,7 +164,10 @@ int psq_cosine_distances_device(float* d_dist, uint64_t count, void* stream);
  * n_features > 16384 returns PSQ_E_UNSUPPORTED; n_obs or n_features 0, or n_components 0 or above
  * min(n_obs, n_features), PSQ_E_INVALID with the reference's message. A run
  * repeats bit for bit. psq_pca_csr_device takes device pointers and runs on
- * the given stream of the current device. */
+ * the given stream of the current device. Unlike the neighbour-graph entries,
+ * PCA allocates its device buffers per call (freed before it returns, outside
+ * the psq_set_device_budget cap); the rocBLAS handle it creates on a device's
+ * first call lives until the process ends. */
 int psq_pca_csr(const uint64_t* indptr, const uint32_t* indices, const float* data, uint64_t n_obs,
                 uint64_t n_features, uint64_t n_components, int32_t center, int32_t scale,
                 double max_value, float* embedding, float* variance_ratio, float* components,
