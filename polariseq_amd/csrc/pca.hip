@@ -134,14 +134,15 @@ __global__ void k_pca_colsums_fold(const double* __restrict__ ps_s, const double
 }
 
 // The m x m Gram matrix and the column sums out of the (ld x ld) product of
-// the chunks with their ones column: C[i][j] = G[i][j], s[j] = G[m][j].
+// the chunks with their ones column (only its upper triangle is computed, and
+// only C's upper triangle is read later): C[i][j] = G[i][j], s[j] = G[j][m].
 __global__ void k_pca_gram_unpack(const double* __restrict__ G, uint32_t m, uint32_t ld, double* __restrict__ C,
                                   double* __restrict__ s) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (uint64_t)m * m) return;
   const uint32_t col = (uint32_t)(t / m), row = (uint32_t)(t % m);
   C[t] = G[(size_t)col * ld + row];
-  if (row == 0) s[col] = G[(size_t)col * ld + m];
+  if (row == 0) s[col] = G[(size_t)m * ld + col];  // (col, m): upper triangle
 }
 
 // Gene statistics -> the scaled view's mean, 1/sd and clipped zero
@@ -734,7 +735,7 @@ int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t 
     // product does not.
     size_t a, b2;
     PTRY(evs.record(s, &a));
-    PBLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, ld, ld, rc, &one, D, ld, D, ld,
+PBLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, ld, ld, rc, &one, D, ld, D, ld,
                         &one, Gb, ld));
     PTRY(evs.record(s, &b2));
     ev_gemm.emplace_back(a, b2);
