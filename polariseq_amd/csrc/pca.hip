@@ -725,7 +725,7 @@ int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t 
   }
   PTRY(evs.record(s, &ev_scaled));
 
-  // Gram of the stored values; its row m (the ones column) holds the column sums
+  // Gram of the stored values; the ones column's entries hold the column sums
   PHIPC(hipMemsetAsync(Gb, 0, (size_t)ld * ld * sizeof(double), s));
   for (uint64_t r0 = 0; r0 < n; r0 += R) {
     const uint32_t rc = (uint32_t)std::min<uint64_t>(R, n - r0);
@@ -735,7 +735,7 @@ int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t 
     // product does not.
     size_t a, b2;
     PTRY(evs.record(s, &a));
-PBLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, ld, ld, rc, &one, D, ld, D, ld,
+    PBLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, ld, ld, rc, &one, D, ld, D, ld,
                         &one, Gb, ld));
     PTRY(evs.record(s, &b2));
     ev_gemm.emplace_back(a, b2);
