@@ -38,11 +38,11 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` on the default workload, from the
-    rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised in
-    profiles/join_4m_hbm_traffic.md (scaled by the copy-kernel calibration)."""
-    path = os.path.join(ROOT, "profiles", "join_4m_hbm_traffic.md")
+def measured_traffic(kernel: str, summary: str = "join_4m_hbm_traffic.md"):
+    """HBM bytes per launch of `kernel` on its workload, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes summarised in profiles/<summary> (scaled by
+    the copy-kernel calibration)."""
+    path = os.path.join(ROOT, "profiles", summary)
     try:
         with open(path) as f:
             for line in f:
@@ -624,7 +624,10 @@ def pca_workload(args, torch, P, ds) -> int:
                    "rows_per_chunk": rep["rows_per_chunk"]},
         "roofline": {"bound": "mfma", "kernel": "Gram DGEMM (rocBLAS, FP64; algorithmic flops n m (m + 1), the product computes twice that)",
                      "achieved": round(flops / gemm_s / 1e12, 2), "peak": 78.6, "unit": "TFLOP/s",
-                     "frac": round(flops / gemm_s / 1e12 / 78.6, 4), "traffic": None},
+                     "frac": round(flops / gemm_s / 1e12 / 78.6, 4),
+                     "traffic": (measured_traffic("gram_dgemm", "pca_gram_hbm_traffic.md")
+                                 if m == 2000 else None),
+                     "traffic_note": "HBM bytes per DGEMM launch (one 16710-row chunk)"},
         "variance_ratio_head": [round(float(v), 6) for v in vr[:5].cpu().numpy()],
     }
     if not args.no_cpu:
