@@ -128,13 +128,27 @@ typedef struct psq_batched_report {
 } psq_batched_report;
 int psq_batched_last_report(psq_batched_report* out);
 
-/* Replaces neighbors() / neighbors_dense (crates/polariseq-core/src/neighbors.rs:101-147,
- * crates/polariseq-python/src/lib.rs:474-501): k_with_self = min(n_neighbors, n_obs);
- * each row holds k_with_self - 1 neighbours, self excluded. metric 0 = euclidean
- * (sqrt distance), 1 = cosine (1 - cos). Output capacity n_obs * (k_with_self - 1). */
+/* Replaces neighbors() / neighbors_dense (crates/polariseq-core/src/neighbors.rs:101-241,
+ * crates/polariseq-python/src/lib.rs:474-501), with the reference's results bit for bit:
+ * k_with_self = min(n_neighbors, n_obs); each query keeps a bounded max-heap of
+ * k_with_self entries over every point in index order (a point replaces the root only
+ * when strictly closer), the heap array is sorted stably by distance and its entry 0 is
+ * dropped as self (so a duplicate of the query can take that place and the query stay in
+ * its own row). Distances are the reference's f32 values: metric 0 = euclidean, the sqrt
+ * of the 8-lane sq_dist; 1 = cosine, 1 - dot/(sqrt(dot(a,a)) sqrt(dot(b,b))), and 1.0
+ * when that denominator is 0. Each row holds k_with_self - 1 entries; capacity
+ * n_obs * (k_with_self - 1). k_with_self > 16384 returns PSQ_E_UNSUPPORTED. Rows of
+ * distances that contain NaN are ordered with NaN last (the reference's order is
+ * unspecified there). */
 int psq_neighbors_exact(const float* embedding, uint64_t n_obs, uint64_t n_dim,
                         uint64_t n_neighbors, int32_t metric, uint32_t* indptr, uint32_t* indices,
                         float* dists, uint64_t cap, uint64_t* nnz_out, int32_t device);
+/* The same on device pointers: embedding f32[n_obs][n_dim], indptr u32[n_obs + 1],
+ * indices / dists of cap entries, on the current device and `stream`. */
+int psq_neighbors_exact_device(const float* d_embedding, uint64_t n_obs, uint64_t n_dim,
+                               uint64_t n_neighbors, int32_t metric, uint32_t* d_indptr,
+                               uint32_t* d_indices, float* d_dists, uint64_t cap,
+                               uint64_t* nnz_out, void* stream);
 
 /* pp.neighbors' cosine metric on the approximate path
  * (python/polariseq/__init__.py:2540-2552): NN-Descent searches unit rows, on which

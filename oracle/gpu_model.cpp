@@ -207,6 +207,17 @@ int64_t psq_o_gpu_model_nndescent(const float* data, uint64_t n_obs, uint64_t n_
                                   int threads, uint32_t* indptr_out, uint32_t* indices_out,
                                   float* dists_out, psq_o_model_stats* stats) {
   if (n_obs == 0 || n_dim == 0 || k_req == 0) return -1;
+  if (n_obs == 1) {
+    // nndescent.rs:727 makes k = 1; the random fill's only draw is the point
+    // itself (hash_index % 1 = 0, and (0 + 1) % 1 = 0, nndescent.rs:836-839),
+    // pushed with its own distance into the empty row; nothing else changes it.
+    indptr_out[0] = 0;
+    indptr_out[1] = 1;
+    indices_out[0] = 0;
+    dists_out[0] = std::sqrt(sq_dist(data, data, n_dim));
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    return 1;
+  }
   Params P;
   P.n = n_obs;
   P.d = n_dim;

@@ -23,6 +23,7 @@ from .neighbors import (  # noqa: F401
     nndescent_dense,
     nndescent_device,
     neighbors_dense,
+    neighbors_dense_device,
     release_workspace,
 )
 

@@ -30,6 +30,7 @@ EXPORTS = (
     "psq_knn_exact",
     "psq_knn_exact_device",
     "psq_neighbors_exact",
+    "psq_neighbors_exact_device",
     "psq_last_stats",
     "psq_last_error",
     "psq_release_workspace",
@@ -196,6 +197,9 @@ def _declare(L: ctypes.CDLL) -> None:
     L.psq_knn_exact_device.argtypes = [P, U64, U64, P, U64, U64, P, P, P]
     L.psq_neighbors_exact.restype = ctypes.c_int
     L.psq_neighbors_exact.argtypes = [P, U64, U64, U64, I32, P, P, P, U64, ctypes.POINTER(U64), I32]
+    L.psq_neighbors_exact_device.restype = ctypes.c_int
+    L.psq_neighbors_exact_device.argtypes = [P, U64, U64, U64, I32, P, P, P, U64,
+                                             ctypes.POINTER(U64), P]
     L.psq_last_stats.restype = ctypes.c_int
     L.psq_last_stats.argtypes = [ctypes.POINTER(NndescentStats)]
     L.psq_last_error.restype = ctypes.c_char_p

@@ -640,6 +640,18 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
                 (unsigned long long)(n_obs + 1), (unsigned long long)need);
   if (!d_data || !d_indptr || !d_indices || !d_dists || !nnz_out)
     return fail(PSQ_E_INVALID, "NULL pointer argument");
+  if (n_obs == 1) {  // the reference's one-point graph: the point itself, at its own distance
+    hipLaunchKernelGGL(k_single_point, dim3(1), dim3(64), 0, s, d_data, (uint32_t)n_dim, d_indptr,
+                       d_indices, d_dists);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(s));
+    *nnz_out = 1;
+    g_stats = psq_nndescent_stats{};
+    g_stats.n_obs = 1;
+    g_stats.n_dim = n_dim;
+    g_stats.k = 1;
+    return PSQ_OK;
+  }
 
   Workspace* ws;
   TRY(workspace(&ws));
@@ -967,6 +979,85 @@ int fuzzy_input_nnz(const uint32_t* d_ip, uint64_t n, uint64_t* m, hipStream_t s
   return PSQ_OK;
 }
 
+// neighbors.rs:107-118 and the binding's checks (crates/polariseq-python/src/lib.rs:474-501).
+int neighbors_exact_check(uint64_t n_obs, uint64_t n_dim, uint64_t n_neighbors, int32_t metric,
+                          uint64_t cap) {
+  if (n_obs == 0 || n_dim == 0)
+    return fail(PSQ_E_INVALID, "embedding has zero rows (%llu) or dims (%llu)",
+                (unsigned long long)n_obs, (unsigned long long)n_dim);
+  if (n_neighbors == 0) return fail(PSQ_E_INVALID, "n_neighbors must be \xe2\x89\xa5 1, got 0");
+  if (metric != 0 && metric != 1) return fail(PSQ_E_INVALID, "unknown metric %d", metric);
+  const uint64_t kws = std::min<uint64_t>(n_neighbors, n_obs);
+  if (cap < n_obs * (kws - 1))
+    return fail(PSQ_E_INVALID, "outputs must hold %llu neighbours",
+                (unsigned long long)(n_obs * (kws - 1)));
+  if (n_obs >= (1ull << 31)) return fail(PSQ_E_UNSUPPORTED, "n_obs must be < 2^31");
+  if (kws > NBR_K_MAX)
+    return fail(PSQ_E_UNSUPPORTED, "min(n_neighbors, n_obs) > %u is not supported", NBR_K_MAX);
+  if (n_obs * (kws - 1) >= (1ull << 32))
+    return fail(PSQ_E_UNSUPPORTED, "the graph would hold 2^32 entries or more");
+  return PSQ_OK;
+}
+
+// neighbors.rs:101-241 on the device: one heap per query, run as the reference
+// runs it (graph_exact.hip), then the stable sort and entry 0 dropped.
+int neighbors_exact_device_impl(const float* d_x, uint64_t n_obs, uint64_t n_dim,
+                                uint64_t n_neighbors, int32_t metric, uint32_t* d_indptr,
+                                uint32_t* d_indices, float* d_dists, uint64_t cap,
+                                uint64_t* nnz_out, hipStream_t s) {
+  TRY(neighbors_exact_check(n_obs, n_dim, n_neighbors, metric, cap));
+  if (!d_x || !d_indptr || !nnz_out) return fail(PSQ_E_INVALID, "NULL pointer argument");
+  const uint32_t n = (uint32_t)n_obs, d = (uint32_t)n_dim;
+  const uint32_t kws = (uint32_t)std::min<uint64_t>(n_neighbors, n_obs);
+  const uint32_t per = kws - 1;
+  hipLaunchKernelGGL(k_nbr_indptr, dim3(grid_for((size_t)n + 1, 256)), dim3(256), 0, s, d_indptr, n,
+                     per);
+  HIPC(hipGetLastError());
+  if (per == 0) {
+    HIPC(hipStreamSynchronize(s));
+    *nnz_out = 0;
+    return PSQ_OK;
+  }
+  if (!d_indices || !d_dists) return fail(PSQ_E_INVALID, "NULL pointer argument");
+  Workspace* ws;
+  TRY(workspace(&ws));
+  float *rs = nullptr, *hd;
+  uint32_t* hi;
+  TRY(wbuf(ws, "nbr_hd", (size_t)n * kws, &hd));
+  TRY(wbuf(ws, "nbr_hi", (size_t)n * kws, &hi));
+  if (metric == 1) {
+    TRY(wbuf(ws, "nbr_rs", n, &rs));
+    hipLaunchKernelGGL(k_nbr_row_norms, dim3(grid_for(n, 256)), dim3(256), 0, s, d_x, n, d, rs);
+    HIPC(hipGetLastError());
+  }
+  const uint32_t lds = kws <= NBR_LDS_K ? 1u : 0u;
+  const size_t hsmem = lds ? (size_t)kws * 64 * (sizeof(float) + sizeof(uint32_t)) : 0;
+  const int dfix = d == 50 ? 50 : (d == 30 ? 30 : 0);
+  void (*heap)(const float*, uint32_t, uint32_t, uint32_t, const float*, float*, uint32_t*,
+               uint32_t) =
+      metric == 0 ? (dfix == 50 ? k_nbr_heap<50, 0> : dfix == 30 ? k_nbr_heap<30, 0> : k_nbr_heap<0, 0>)
+                  : (dfix == 50 ? k_nbr_heap<50, 1> : dfix == 30 ? k_nbr_heap<30, 1> : k_nbr_heap<0, 1>);
+  hipLaunchKernelGGL(heap, dim3((n + 63) / 64), dim3(64), hsmem, s, d_x, n, d, kws, rs, hd, hi, lds);
+  HIPC(hipGetLastError());
+  if (kws <= 64) {
+    hipLaunchKernelGGL(k_nbr_sort64, dim3(n), dim3(64), 0, s, hd, hi, n, kws, (int)metric, d_indices,
+                       d_dists);
+  } else {
+    uint32_t P = 1;
+    while (P < kws) P <<= 1;
+    const size_t ssmem = (size_t)P * sizeof(uint64_t);
+    if (ssmem > 65536)
+      HIPC(hipFuncSetAttribute((const void*)k_nbr_sort_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)ssmem));
+    hipLaunchKernelGGL(k_nbr_sort_lds, dim3(n), dim3(256), ssmem, s, hd, hi, n, kws, P, (int)metric,
+                       d_indices, d_dists);
+  }
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(s));
+  *nnz_out = (uint64_t)n * per;
+  return PSQ_OK;
+}
+
 int set_device(int32_t device) {
   int count = 0;
   HIPC(hipGetDeviceCount(&count));
@@ -1083,38 +1174,70 @@ int knn_exact_big(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d
   return PSQ_OK;
 }
 
-int psq_host::knn_exact_device_impl(const float* d_data, uint64_t n, uint64_t d,
-                                    const uint32_t* d_queries, uint64_t nq, uint64_t k,
-                                    uint32_t* d_idx, float* d_dist, hipStream_t s) {
-  if (n == 0 || d == 0) return fail(PSQ_E_INVALID, "empty data: n=%llu d=%llu",
-                                    (unsigned long long)n, (unsigned long long)d);
-  if (k == 0) return fail(PSQ_E_INVALID, "k must be >= 1");
-  if (!d_queries && nq != n) return fail(PSQ_E_INVALID, "queries is NULL but n_q != n");
-  if (k > EX_KBIG_MAX)
-    return fail(PSQ_E_UNSUPPORTED, "k > %u is not supported by the exact GPU path", EX_KBIG_MAX);
-  if (d > 256) return fail(PSQ_E_UNSUPPORTED, "n_dim > 256 is not supported on the GPU path");
-  if (n >= (1ull << 31) || nq >= (1ull << 31)) return fail(PSQ_E_UNSUPPORTED, "n must be < 2^31");
-  if (!d_data || !d_idx || !d_dist) return fail(PSQ_E_INVALID, "NULL pointer argument");
-  if (nq == 0) return PSQ_OK;
-  if (k > 64) return knn_exact_big(d_data, n, d, d_queries, nq, k, d_idx, d_dist, s);
-  Workspace* ws;
-  TRY(workspace(&ws));
-  const uint32_t dp = round_up((uint32_t)d, 32);
-  const uint32_t K = k <= 16 ? 16 : (k <= 32 ? 32 : 64);
-  const uint32_t cap = 128;
+// Device buffers of one exact-kNN call: the library's grow-only workspace, or
+// a fixed arena carved front to back at 256-byte alignment (the entry point
+// that allocates nothing).
+struct BufSrc {
+  Workspace* ws = nullptr;
+  uint8_t* base = nullptr;
+  size_t cap = 0, used = 0;
+  static size_t span(size_t count, size_t elem) {
+    return (std::max<size_t>(count, 1) * elem + 255) & ~(size_t)255;
+  }
+  template <typename T>
+  int take(const char* name, size_t count, T** out) {
+    if (ws) return wbuf(ws, name, count, out);
+    const size_t b = span(count, sizeof(T));
+    if (used + b > cap)
+      return fail(PSQ_E_INVALID, "scratch of %zu bytes cannot hold %s (%zu more bytes)", cap, name, b);
+    *out = reinterpret_cast<T*>(base + used);
+    used += b;
+    return PSQ_OK;
+  }
+};
+
+constexpr uint32_t EX_CAND_CAP = 128;  // pass-2 candidates per query (k <= 64 path)
+
+// Segments of pass 1 for the k <= 64 path: about 2048 blocks in all, at least
+// 64 points per segment. Any count gives the same result (each segment keeps
+// K >= k values, so the k-th of the union bounds the true k-th from above).
+uint32_t exact_small_nseg(uint64_t n, uint64_t nq) {
   const uint32_t qblocks = (uint32_t)((nq + 255) / 256);
-  uint32_t nseg = std::max<uint32_t>(1, (uint32_t)std::min<uint64_t>((n + 63) / 64, (2048 + qblocks - 1) / qblocks));
+  return std::max<uint32_t>(1, (uint32_t)std::min<uint64_t>((n + 63) / 64, (2048 + qblocks - 1) / qblocks));
+}
+uint32_t exact_small_K(uint64_t k) { return k <= 16 ? 16 : (k <= 32 ? 32 : 64); }
+
+// Bytes BufSrc hands out to knn_exact_small with nseg segments, the
+// fallback's lists reserved for every query.
+size_t exact_small_bytes(uint64_t n, uint64_t d, uint64_t nq, uint64_t k, uint32_t nseg) {
+  const size_t dp = round_up((uint32_t)d, 32);
+  return BufSrc::span(n * dp, 4) + BufSrc::span(nq * nseg * exact_small_K(k), 4) +
+         3 * BufSrc::span(nq, 4) + BufSrc::span(nq * EX_CAND_CAP, 4) + BufSrc::span(1, 4) +
+         BufSrc::span(nq * k, 8) + BufSrc::span(nq * k, 4);
+}
+
+// The k <= 64 path: pass 1 (per-segment top K in FP32), tau, pass 2 (every
+// point under the threshold), FP64 re-rank, and the fallback for overflowed
+// queries.
+int knn_exact_small(BufSrc& B, const float* d_data, uint64_t n, uint64_t d, const uint32_t* d_queries,
+                    uint64_t nq, uint64_t k, uint32_t nseg_in, uint32_t* d_idx, float* d_dist,
+                    hipStream_t s) {
+  const uint32_t dp = round_up((uint32_t)d, 32);
+  const uint32_t K = exact_small_K(k);
+  const uint32_t cap = EX_CAND_CAP;
+  const uint32_t qblocks = (uint32_t)((nq + 255) / 256);
+  uint32_t nseg = std::max<uint32_t>(1, nseg_in);
   uint32_t seg_len = round_up((uint32_t)((n + nseg - 1) / nseg), 64);
   nseg = (uint32_t)((n + seg_len - 1) / seg_len);
   float *Xp, *partial, *T;
   uint32_t *cnt, *cand, *ovf, *novf;
-  TRY(wbuf(ws, "Xp", (size_t)n * dp, &Xp));
-  TRY(wbuf(ws, "ex_partial", (size_t)nq * nseg * K, &partial));
-  TRY(wbuf(ws, "ex_T", nq, &T));
-  TRY(wbuf(ws, "ex_cnt", nq, &cnt));
-  TRY(wbuf(ws, "ex_cand", (size_t)nq * cap, &cand));
-  TRY(wbuf(ws, "ex_ovf", nq, &ovf));
-  TRY(wbuf(ws, "ex_novf", 1, &novf));
+  TRY(B.take("Xp", (size_t)n * dp, &Xp));
+  TRY(B.take("ex_partial", (size_t)nq * nseg * K, &partial));
+  TRY(B.take("ex_T", nq, &T));
+  TRY(B.take("ex_cnt", nq, &cnt));
+  TRY(B.take("ex_cand", (size_t)nq * cap, &cand));
+  TRY(B.take("ex_ovf", nq, &ovf));
+  TRY(B.take("ex_novf", 1, &novf));
   hipLaunchKernelGGL(k_pad_rows, dim3(std::min<unsigned>(grid_for((size_t)n * dp, 256), 65536)),
                      dim3(256), 0, s, d_data, (uint32_t)n, (uint32_t)d, dp, Xp);
   HIPC(hipGetLastError());
@@ -1151,8 +1274,8 @@ int psq_host::knn_exact_device_impl(const float* d_data, uint64_t n, uint64_t d,
   if (h_novf > 0) {
     double* sd;
     uint32_t* si;
-    TRY(wbuf(ws, "ex_fb_d", (size_t)h_novf * k, &sd));
-    TRY(wbuf(ws, "ex_fb_i", (size_t)h_novf * k, &si));
+    TRY(B.take("ex_fb_d", (size_t)h_novf * k, &sd));
+    TRY(B.take("ex_fb_i", (size_t)h_novf * k, &si));
     hipLaunchKernelGGL(k_exact_fallback, dim3(grid_for(h_novf, 64)), dim3(64), 0, s, Xp,
                        (uint32_t)n, (uint32_t)d, dp, d_queries, (uint32_t)k, T, ovf, h_novf, sd, si,
                        d_idx, d_dist, nullptr, 0u);
@@ -1160,6 +1283,33 @@ int psq_host::knn_exact_device_impl(const float* d_data, uint64_t n, uint64_t d,
     HIPC(hipStreamSynchronize(s));
   }
   return PSQ_OK;
+}
+
+// Arguments of psq_knn_exact_device and its no-allocation variant.
+int exact_check(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d_queries, uint64_t nq,
+                uint64_t k, const uint32_t* d_idx, const float* d_dist) {
+  if (n == 0 || d == 0) return fail(PSQ_E_INVALID, "empty data: n=%llu d=%llu",
+                                    (unsigned long long)n, (unsigned long long)d);
+  if (k == 0) return fail(PSQ_E_INVALID, "k must be >= 1");
+  if (!d_queries && nq != n) return fail(PSQ_E_INVALID, "queries is NULL but n_q != n");
+  if (k > EX_KBIG_MAX)
+    return fail(PSQ_E_UNSUPPORTED, "k > %u is not supported by the exact GPU path", EX_KBIG_MAX);
+  if (d > 256) return fail(PSQ_E_UNSUPPORTED, "n_dim > 256 is not supported on the GPU path");
+  if (n >= (1ull << 31) || nq >= (1ull << 31)) return fail(PSQ_E_UNSUPPORTED, "n must be < 2^31");
+  if (!d_data || !d_idx || !d_dist) return fail(PSQ_E_INVALID, "NULL pointer argument");
+  return PSQ_OK;
+}
+
+int psq_host::knn_exact_device_impl(const float* d_data, uint64_t n, uint64_t d,
+                                    const uint32_t* d_queries, uint64_t nq, uint64_t k,
+                                    uint32_t* d_idx, float* d_dist, hipStream_t s) {
+  TRY(exact_check(d_data, n, d, d_queries, nq, k, d_idx, d_dist));
+  if (nq == 0) return PSQ_OK;
+  if (k > 64) return knn_exact_big(d_data, n, d, d_queries, nq, k, d_idx, d_dist, s);
+  BufSrc B;
+  TRY(workspace(&B.ws));
+  return knn_exact_small(B, d_data, n, d, d_queries, nq, k, exact_small_nseg(n, nq), d_idx, d_dist,
+                         s);
 }
 
 using psq_host::knn_exact_device_impl;
@@ -1303,53 +1453,47 @@ int psq_cosine_distances_device(float* d_dist, uint64_t count, void* stream) {
   return PSQ_OK;
 }
 
+int psq_neighbors_exact_device(const float* d_embedding, uint64_t n_obs, uint64_t n_dim,
+                               uint64_t n_neighbors, int32_t metric, uint32_t* d_indptr,
+                               uint32_t* d_indices, float* d_dists, uint64_t cap,
+                               uint64_t* nnz_out, void* stream) {
+  g_err.clear();
+  return neighbors_exact_device_impl(d_embedding, n_obs, n_dim, n_neighbors, metric, d_indptr,
+                                     d_indices, d_dists, cap, nnz_out,
+                                     static_cast<hipStream_t>(stream));
+}
+
 int psq_neighbors_exact(const float* embedding, uint64_t n_obs, uint64_t n_dim,
                         uint64_t n_neighbors, int32_t metric, uint32_t* indptr, uint32_t* indices,
                         float* dists, uint64_t cap, uint64_t* nnz_out, int32_t device) {
   g_err.clear();
-  // neighbors.rs:107-115
-  if (n_obs == 0 || n_dim == 0)
-    return fail(PSQ_E_INVALID, "embedding has zero rows (%llu) or dims (%llu)",
-                (unsigned long long)n_obs, (unsigned long long)n_dim);
-  if (n_neighbors == 0) return fail(PSQ_E_INVALID, "n_neighbors must be >= 1, got 0");
-  if (metric != 0 && metric != 1) return fail(PSQ_E_INVALID, "unknown metric %d", metric);
+  TRY(neighbors_exact_check(n_obs, n_dim, n_neighbors, metric, cap));
   if (!embedding || !indptr || !nnz_out) return fail(PSQ_E_INVALID, "NULL pointer argument");
-  const uint64_t kws = std::min<uint64_t>(n_neighbors, n_obs);
-  const uint64_t kk = kws - 1;
-  if (cap < n_obs * kk)
-    return fail(PSQ_E_INVALID, "outputs must hold %llu neighbours", (unsigned long long)(n_obs * kk));
-  indptr[0] = 0;
+  const uint64_t kk = std::min<uint64_t>(n_neighbors, n_obs) - 1;
   if (kk == 0) {
     for (uint64_t i = 0; i <= n_obs; ++i) indptr[i] = 0;
     *nnz_out = 0;
     return PSQ_OK;
   }
+  if (!indices || !dists) return fail(PSQ_E_INVALID, "NULL pointer argument");
   TRY(set_device(device));
   Workspace* ws;
   TRY(workspace(&ws));
-  float *d_x, *d_src, *d_dist;
-  uint32_t* d_idx;
+  float *d_x, *d_dist;
+  uint32_t *d_ip, *d_idx;
   const size_t cnt = (size_t)n_obs * kk;
   TRY(wbuf(ws, "ne_data", (size_t)n_obs * n_dim, &d_x));
+  TRY(wbuf(ws, "ne_indptr", (size_t)n_obs + 1, &d_ip));
   TRY(wbuf(ws, "ne_indices", cnt, &d_idx));
   TRY(wbuf(ws, "ne_dists", cnt, &d_dist));
   HIPC(hipMemcpy(d_x, embedding, (size_t)n_obs * n_dim * sizeof(float), hipMemcpyHostToDevice));
-  d_src = d_x;
-  if (metric == 1) {  // cosine: unit rows; 1 - cos = |a - b|^2 / 2 on unit vectors
-    TRY(wbuf(ws, "ne_unit", (size_t)n_obs * n_dim, &d_src));
-    hipLaunchKernelGGL(k_unit_rows64, dim3((unsigned)((n_obs + 255) / 256)), dim3(256), 0, nullptr, d_x, n_obs,
-                       (uint32_t)n_dim, d_src);
-    HIPC(hipGetLastError());
-  }
-  TRY(knn_exact_device_impl(d_src, n_obs, n_dim, nullptr, n_obs, kk, d_idx, d_dist, nullptr));
-  if (metric == 1) {
-    hipLaunchKernelGGL(k_half_square, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, nullptr, d_dist, cnt);
-    HIPC(hipGetLastError());
-  }
+  uint64_t nnz = 0;
+  TRY(neighbors_exact_device_impl(d_x, n_obs, n_dim, n_neighbors, metric, d_ip, d_idx, d_dist, cnt,
+                                  &nnz, nullptr));
+  HIPC(hipMemcpy(indptr, d_ip, (size_t)(n_obs + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
   HIPC(hipMemcpy(indices, d_idx, cnt * sizeof(uint32_t), hipMemcpyDeviceToHost));
   HIPC(hipMemcpy(dists, d_dist, cnt * sizeof(float), hipMemcpyDeviceToHost));
-  for (uint64_t i = 0; i < n_obs; ++i) indptr[i + 1] = (uint32_t)((i + 1) * kk);
-  *nnz_out = n_obs * kk;
+  *nnz_out = nnz;
   return PSQ_OK;
 }
 

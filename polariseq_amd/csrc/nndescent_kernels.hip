@@ -155,6 +155,18 @@ __global__ void k_random_fill(const float* __restrict__ Xp, uint32_t n, uint32_t
   if (wrote) touched[i] = 1;
 }
 
+// One point (nndescent.rs:727, 836-839): k is 1, the random fill's only draw
+// is the point itself (hash_index % 1 = 0, (0 + 1) % 1 = 0), pushed with its own
+// distance into the empty row, and no later step changes the row.
+__global__ void k_single_point(const float* __restrict__ X, uint32_t d, uint32_t* __restrict__ indptr,
+                               uint32_t* __restrict__ indices, float* __restrict__ dists) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  indptr[0] = 0;
+  indptr[1] = 1;
+  indices[0] = 0;
+  dists[0] = (float)__dsqrt_rn((double)sq_dist_ref(X, X, (int)d));
+}
+
 // ── sampling (nndescent.rs:853-909) ──
 // One wave (one block) per row i. Candidates: the row's own entries (forward,
 // priority of slot i*k+s) and the reverse entries rev[roff[i]..roff[i+1]).

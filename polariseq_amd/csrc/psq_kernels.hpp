@@ -102,7 +102,6 @@ __global__ void k_mlp_forward(const float* X, uint32_t ldx, const uint32_t* idx,
 constexpr uint32_t ROWS_D_MAX = 256;
 __global__ void k_unit_rows(const float* x, uint64_t n, uint32_t d, float* out);
 __global__ void k_half_square(float* dist, uint64_t count);
-__global__ void k_unit_rows64(const float* x, uint64_t n, uint32_t d, float* out);
 
 // Encode on the FP32 matrix cores: operands packed per lane (ENC_PACKED floats),
 // ENC_WAVES wave-tiles of 16 rows per block.
@@ -221,5 +220,20 @@ __global__ void k_fuzzy_total(const uint32_t* olen, uint32_t n, unsigned long lo
 __global__ void k_fuzzy_emit(const uint32_t* indptr, const uint32_t* t_ptr, uint32_t n,
                              const uint64_t* comb, const uint32_t* optr, uint32_t* out_ix,
                              float* out_w);
+
+// Exact neighbour graph of neighbors.rs:101-241 (graph_exact.hip)
+constexpr uint32_t NBR_LDS_K = 128;    // heaps of at most this many entries live in LDS
+constexpr uint32_t NBR_K_MAX = 16384;  // largest min(n_neighbors, n_obs): one LDS sort per row
+__global__ void k_nbr_row_norms(const float* X, uint32_t n, uint32_t d, float* rs);
+template <int D, int M>
+__global__ void k_nbr_heap(const float* X, uint32_t n, uint32_t d, uint32_t k, const float* rs,
+                           float* hd, uint32_t* hi, uint32_t lds_heap);
+__global__ void k_nbr_sort64(const float* hd, const uint32_t* hi, uint32_t n, uint32_t k, int metric,
+                             uint32_t* out_idx, float* out_d);
+__global__ void k_nbr_sort_lds(const float* hd, const uint32_t* hi, uint32_t n, uint32_t k, uint32_t P,
+                               int metric, uint32_t* out_idx, float* out_d);
+__global__ void k_nbr_indptr(uint32_t* indptr, uint32_t n, uint32_t per);
+__global__ void k_single_point(const float* X, uint32_t d, uint32_t* indptr, uint32_t* indices,
+                               float* dists);
 
 }  // namespace psq

@@ -59,20 +59,6 @@ __global__ void k_unit_rows(const float* __restrict__ x, uint64_t n, uint32_t d,
   for (uint32_t t = 0; t < d; ++t) oi[t] = xi[t] / nrm;
 }
 
-// The exact path's cosine rows (neighbors.rs:101-241 through psq_neighbors_exact):
-// out = float32(x / max(sqrt(sum x^2), 1e-12)), the sum, root and quotient in
-// float64, the sum left to right. One thread per row.
-__global__ void k_unit_rows64(const float* __restrict__ x, uint64_t n, uint32_t d, float* __restrict__ out) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float* xi = x + i * d;
-  double s = 0.0;
-  for (uint32_t t = 0; t < d; ++t) s += (double)xi[t] * (double)xi[t];
-  const double nr = fmax(__dsqrt_rn(s), 1e-12);
-  float* oi = out + i * d;
-  for (uint32_t t = 0; t < d; ++t) oi[t] = (float)((double)xi[t] / nr);
-}
-
 // dist = float32(float64(dist)^2 / 2): the square of a float32 is exact in
 // float64 and the halving is exact, so one rounding, as numpy's.
 __global__ void k_half_square(float* __restrict__ dist, uint64_t count) {

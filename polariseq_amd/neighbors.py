@@ -110,6 +110,33 @@ def neighbors_dense(embedding, n_neighbors: int, metric: str = "euclidean", *,
     return {"indptr": ip, "indices": ix[:c], "data": dx[:c], "n_obs": n_obs}
 
 
+def neighbors_dense_device(x, n_neighbors: int, metric: str = "euclidean"):
+    """``neighbors_dense`` on a contiguous float32 CUDA tensor (n_obs, n_dim): the
+    graph stays on the device. Returns (indptr, indices, data) CUDA tensors (int32
+    views of the u32 arrays, float32) with n_obs * (min(n_neighbors, n_obs) - 1)
+    entries."""
+    import torch
+    if not (x.is_cuda and x.dtype == torch.float32 and x.dim() == 2 and x.is_contiguous()):
+        raise ValueError("x must be a contiguous float32 CUDA tensor of shape (n, d)")
+    n_obs, n_dim = x.shape
+    m = {"euclidean": 0, "cosine": 1}.get(str(metric).lower())
+    if m is None:
+        raise RuntimeError(f"unknown metric: {metric}")
+    kk = max(0, min(int(n_neighbors), n_obs) - 1)
+    ip = torch.empty(n_obs + 1, dtype=torch.int32, device=x.device)
+    ix = torch.empty(max(n_obs * kk, 1), dtype=torch.int32, device=x.device)
+    dx = torch.empty(max(n_obs * kk, 1), dtype=torch.float32, device=x.device)
+    nnz = ctypes.c_uint64(0)
+    rc = N.lib().psq_neighbors_exact_device(x.data_ptr(), n_obs, n_dim, int(n_neighbors), m,
+                                            ip.data_ptr(), ix.data_ptr(), dx.data_ptr(),
+                                            n_obs * kk, ctypes.byref(nnz), _stream_ptr())
+    if rc != N.PSQ_OK:
+        msg = N.lib().psq_last_error().decode()
+        raise RuntimeError(f"neighbors failed: {msg}")
+    c = int(nnz.value)
+    return ip, ix[:c], dx[:c]
+
+
 def knn_exact(data, k: int, queries=None, *, device: int = 0):
     """Exact k nearest (self excluded) by (FP64 distance, index).
 

@@ -33,7 +33,7 @@ def test_errors_carry_the_trait_texts():
 
 
 @pytest.mark.gpu
-def test_backend_equals_direct_calls():
+def test_backend_equals_direct_calls(orc):
     rng = np.random.default_rng(0)
     x = np.where(rng.random((1500, 300)) < 0.2, rng.poisson(3.0, (1500, 300)), 0).astype(np.float32)
     b = B.RocmBackend(0)
@@ -48,3 +48,7 @@ def test_backend_equals_direct_calls():
         want = neighbors_dense(r.embedding, 10, metric)
         assert np.array_equal(g.indptr, want["indptr"]) and np.array_equal(g.indices, want["indices"])
         assert np.array_equal(g.data, want["data"]) and g.n_obs == 1500
+        # and the reference's exact path (neighbors.rs:101-241), restated by the oracle
+        ref = orc.neighbors_exact(r.embedding, 10, metric)
+        assert np.array_equal(g.indptr, ref["indptr"]) and np.array_equal(g.indices, ref["indices"])
+        assert np.array_equal(g.data.view(np.uint32), ref["data"].view(np.uint32))

@@ -106,6 +106,20 @@ def test_tiny_inputs_work():
     assert (np.diff(g["indptr"]) == 4).all()
 
 
+def test_one_point_is_the_reference_graph(orc):
+    # nndescent.rs:727, 836-839: k = 1 and the random fill's only draw is the
+    # point itself, so the graph is one self entry at distance 0.
+    x = np.array([[0.5, -2.0, 3.0]], np.float32)
+    want, _ = orc.nndescent_port(x, 15)
+    for init in ("rp_forest", "random"):
+        g = P.nndescent_dense(x, 15, init=init)
+        assert g["indptr"].tolist() == want["indptr"].tolist() == [0, 1]
+        assert g["indices"].tolist() == want["indices"].tolist() == [0]
+        assert g["data"].tolist() == want["data"].tolist() == [0.0]
+    m, _ = orc.gpu_model_nndescent(x, 15)
+    assert m["indices"].tolist() == [0] and m["data"].tolist() == [0.0]
+
+
 def test_distances_are_the_reference_sq_dist(orc):
     x = ds.mixture(3000, 50, 6, seed=3)
     g = P.nndescent_dense(x, 15, init="random")

@@ -176,6 +176,66 @@ def test_reference_exact_path_kats(orc):
     assert g["indices"][0] == 1 and abs(g["data"][0]) < 1e-5
 
 
+def _row(g, i):
+    s, e = g["indptr"][i], g["indptr"][i + 1]
+    return g["indices"][s:e].tolist(), g["data"][s:e].tolist()
+
+
+def test_reference_exact_path_heap_traces(orc):
+    # Hand traces of knn_for_one (neighbors.rs:151-223) on a line.
+    # x = 0, -1, 1, 0.5; query 0 with k_with_self = 3: points 0, 1, 2 are pushed
+    # (heap [(1,1), (0,0), (1,2)] after the sift-ups); point 3 (0.25) replaces the
+    # root (1,1) and sifts below (1,2). Point 1 is evicted at the boundary tie and
+    # point 2, the later index, survives; sorted: (0,0), (0.25,3), (1,2).
+    x = np.array([[0.0], [-1.0], [1.0], [0.5]], np.float32)
+    ix, dx = _row(orc.neighbors_exact(x, 3), 0)
+    assert ix == [3, 2] and dx == [0.5, 1.0]
+    # Duplicates x0 = x1: both distances 0 tie, the stable sort keeps the heap's
+    # array order [(0,0), (0,1)] and entry 0 (point 0) is dropped as "self", so
+    # row 1 keeps the query itself and row 0 keeps its duplicate. Query 2 pushes
+    # (1,0), (1,1), then replaces the root (1,0) with itself: point 1 remains.
+    x = np.array([[0.0, 0.0], [0.0, 0.0], [1.0, 0.0], [2.0, 0.0]], np.float32)
+    g = orc.neighbors_exact(x, 2)
+    assert _row(g, 0) == ([1], [0.0]) and _row(g, 1) == ([1], [0.0])
+    assert _row(g, 2) == ([1], [1.0]) and _row(g, 3) == ([2], [1.0])
+
+
+def test_reference_exact_path_cosine_zero_rows(orc):
+    # cosine_dist (neighbors.rs:232-241) is 1.0 whenever a norm is 0.
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(40, 6)).astype(np.float32)
+    x[[5, 17]] = 0.0
+    g = orc.neighbors_exact(x, 8, "cosine")
+    for i in (5, 17):
+        _, dx = _row(g, i)
+        assert dx == [1.0] * 7
+    for i in range(40):
+        ix, dx = _row(g, i)
+        for j, v in zip(ix, dx):
+            if j in (5, 17):
+                assert v == 1.0
+
+
+def test_reference_exact_path_without_ties_is_the_sorted_top(orc):
+    # With distinct distances the heap holds the k smallest: the row is the query's
+    # stable argsort of the reference's f32 distances (sq_dist, pinned by its own
+    # tests) after the query itself, with square roots.
+    rng = np.random.default_rng(11)
+    x = rng.normal(size=(120, 9)).astype(np.float32)
+    g = orc.neighbors_exact(x, 6)
+    for i in range(0, 120, 7):
+        d = np.array([orc.sq_dist(x[i], x[j]) for j in range(120)], np.float32)
+        assert len(np.unique(d)) == 120
+        order = np.argsort(d, kind="stable")[:6]
+        assert order[0] == i
+        ix, dx = _row(g, i)
+        assert ix == order[1:].tolist()
+        assert np.array_equal(np.array(dx, np.float32), np.sqrt(d[order[1:]]))
+    # n_neighbors above n_obs: every other point, each row n - 1 long
+    g = orc.neighbors_exact(x[:10], 25)
+    assert (np.diff(g["indptr"]) == 9).all()
+
+
 # ── the reference NN-Descent restated (nndescent.rs:1059-1199) ──
 
 def _valid(g, n, k):
