@@ -104,6 +104,24 @@ int psq_knn_exact_device(const float* d_data, uint64_t n, uint64_t d, const uint
                          uint64_t n_q, uint64_t k, uint32_t* d_idx_out, float* d_dist_out,
                          void* stream);
 
+/* No counterpart in the reference: psq_knn_exact_device for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). It calls no hipMalloc, hipFree or allocating
+ * library routine: its buffers live in a static device array of
+ * PSQ_NOALLOC_SCRATCH_BYTES per GPU, held from the library's load until the
+ * process ends. Results equal psq_knn_exact_device's bit for bit. k <= 64
+ * (larger k returns PSQ_E_UNSUPPORTED); inputs whose buffers do not fit the
+ * array return PSQ_E_INVALID before any work. They need at least
+ * n * round_up(d, 32) * 4 + n_q * (524 + 4 K + 12 k) bytes (K the smallest of
+ * 16, 32, 64 that is >= k; each term rounded up to 256 bytes): 64 MiB holds,
+ * e.g., 100,000 points of 50 dims with 40,000 queries at k = 15. Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU race, so the
+ * caller serialises them (one stream, or its own lock). */
+#define PSQ_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int psq_knn_exact_device_noalloc(const float* d_data, uint64_t n, uint64_t d,
+                                 const uint32_t* d_queries, uint64_t n_q, uint64_t k,
+                                 uint32_t* d_idx_out, float* d_dist_out, void* stream);
+
 /* Batched exact kNN: n_samples independent samples stored as contiguous row
  * ranges [offsets[s], offsets[s+1]) of data (offsets is a HOST array of
  * n_samples + 1 entries, offsets[0] = 0, offsets[n_samples] = n_total). Every

@@ -29,6 +29,7 @@ EXPORTS = (
     "psq_nndescent_device",
     "psq_knn_exact",
     "psq_knn_exact_device",
+    "psq_knn_exact_device_noalloc",
     "psq_neighbors_exact",
     "psq_neighbors_exact_device",
     "psq_last_stats",
@@ -195,6 +196,8 @@ def _declare(L: ctypes.CDLL) -> None:
     L.psq_knn_exact.argtypes = [P, U64, U64, P, U64, U64, P, P, I32]
     L.psq_knn_exact_device.restype = ctypes.c_int
     L.psq_knn_exact_device.argtypes = [P, U64, U64, P, U64, U64, P, P, P]
+    L.psq_knn_exact_device_noalloc.restype = ctypes.c_int
+    L.psq_knn_exact_device_noalloc.argtypes = [P, U64, U64, P, U64, U64, P, P, P]
     L.psq_neighbors_exact.restype = ctypes.c_int
     L.psq_neighbors_exact.argtypes = [P, U64, U64, U64, I32, P, P, P, U64, ctypes.POINTER(U64), I32]
     L.psq_neighbors_exact_device.restype = ctypes.c_int

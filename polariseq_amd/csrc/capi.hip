@@ -1312,6 +1312,31 @@ int psq_host::knn_exact_device_impl(const float* d_data, uint64_t n, uint64_t d,
                          s);
 }
 
+// ── exact kNN without allocation ──
+// The scratch of psq_knn_exact_device_noalloc: one array per device, held from
+// the library's load until the process ends. Host code never names it beyond
+// hipGetSymbolAddress: its device address, looked up once per device on the
+// first call there, is what the kernels receive.
+static __device__ __align__(256) uint8_t scratch[PSQ_NOALLOC_SCRATCH_BYTES];
+std::mutex g_scratch_mu;
+std::map<int, uint8_t*> g_scratch_ptr;
+
+int scratch_ptr(uint8_t** out) {
+  int dev = 0;
+  HIPC(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it == g_scratch_ptr.end()) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(PSQ_E_DEVICE, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    it = g_scratch_ptr.emplace(dev, static_cast<uint8_t*>(p)).first;
+  }
+  *out = it->second;
+  return PSQ_OK;
+}
+
 using psq_host::knn_exact_device_impl;
 
 extern "C" {
@@ -1399,6 +1424,30 @@ int psq_knn_exact_device(const float* d_data, uint64_t n, uint64_t d, const uint
   g_err.clear();
   return knn_exact_device_impl(d_data, n, d, d_queries, n_q, k, d_idx_out, d_dist_out,
                                static_cast<hipStream_t>(stream));
+}
+
+int psq_knn_exact_device_noalloc(const float* d_data, uint64_t n, uint64_t d,
+                                 const uint32_t* d_queries, uint64_t n_q, uint64_t k,
+                                 uint32_t* d_idx_out, float* d_dist_out, void* stream) {
+  g_err.clear();
+  TRY(exact_check(d_data, n, d, d_queries, n_q, k, d_idx_out, d_dist_out));
+  if (k > 64) return fail(PSQ_E_UNSUPPORTED, "the allocation-free exact kNN takes k <= 64");
+  if (n_q == 0) return PSQ_OK;
+  // the standard segment count, halved until the buffers fit the scratch
+  uint32_t nseg = exact_small_nseg(n, n_q);
+  while (nseg > 1 && exact_small_bytes(n, d, n_q, k, nseg) > PSQ_NOALLOC_SCRATCH_BYTES) nseg /= 2;
+  const size_t need = exact_small_bytes(n, d, n_q, k, nseg);
+  if (need > PSQ_NOALLOC_SCRATCH_BYTES)
+    return fail(PSQ_E_INVALID,
+                "n = %llu, d = %llu, n_q = %llu, k = %llu need %zu bytes of scratch; the "
+                "allocation-free entry has %llu",
+                (unsigned long long)n, (unsigned long long)d, (unsigned long long)n_q,
+                (unsigned long long)k, need, (unsigned long long)PSQ_NOALLOC_SCRATCH_BYTES);
+  BufSrc B;
+  TRY(scratch_ptr(&B.base));
+  B.cap = PSQ_NOALLOC_SCRATCH_BYTES;
+  return knn_exact_small(B, d_data, n, d, d_queries, n_q, k, nseg, d_idx_out, d_dist_out,
+                         static_cast<hipStream_t>(stream));
 }
 
 int psq_knn_exact(const float* data, uint64_t n, uint64_t d, const uint32_t* queries, uint64_t n_q,
