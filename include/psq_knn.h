@@ -227,15 +227,20 @@ int psq_pca_last_report(psq_pca_report* out);
 typedef struct psq_encoder psq_encoder;
 typedef struct psq_encoder_train_opts {
   uint64_t subset; /* cells in the training subset (0 -> 200000), evenly spaced */
-  uint64_t k_pos;  /* positives drawn from the exact top-k_pos (0 -> 15) */
+  uint64_t k_pos;  /* positives drawn from the exact top-k_pos within the subset (0 -> 15) */
   uint64_t batch;  /* triplets per step (0 -> 2048) */
-  uint64_t steps;  /* Adam steps (0 -> 1000) */
+  uint64_t steps;  /* Adam steps (0 -> 300) */
   float lr;        /* 0 -> 1e-3 */
   float margin;    /* triplet margin on squared code distances (0 -> 1.0) */
   uint64_t seed;
+  uint64_t anchors; /* anchor pool: evenly spaced subset cells whose exact top-k_pos is
+                       mined (0 -> 16384); anchors are drawn from it, negatives from the subset */
 } psq_encoder_train_opts;
 int psq_encoder_create(psq_encoder** out, uint64_t seed, int32_t device);
 void psq_encoder_destroy(psq_encoder* e);
+/* Back to the untrained state of psq_encoder_create(seed): initial weights, zero Adam
+ * moments, no standardisation; the device buffers are kept. */
+int psq_encoder_reset(psq_encoder* e, uint64_t seed);
 int psq_encoder_train(psq_encoder* e, const float* d_data, uint64_t n, uint64_t d,
                       const psq_encoder_train_opts* o, void* stream);
 int psq_encoder_encode(const psq_encoder* e, const float* d_data, uint64_t n, uint64_t d,
@@ -247,6 +252,9 @@ int psq_encoder_grad(const psq_encoder* e, const float* d_data, uint64_t n, uint
                      const uint32_t* rows, uint64_t B, float margin, float* host_grad,
                      double* loss_out);
 int psq_encoder_info(const psq_encoder* e, double* last_loss, double* train_ms, uint64_t* steps);
+/* HIP-event time of the last training's mining part (subset, moments, exact kNN of the
+ * anchor pool); train_ms includes it. */
+int psq_encoder_mine_ms(const psq_encoder* e, double* mine_ms);
 
 /* Fuzzy connectivities of a kNN distance graph: the symmetric CSR that clustering
  * and UMAP share (scanpy's obsp['connectivities']), replacing

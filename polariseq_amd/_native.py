@@ -38,12 +38,14 @@ EXPORTS = (
     "psq_set_device_budget",
     "psq_encoder_create",
     "psq_encoder_destroy",
+    "psq_encoder_reset",
     "psq_encoder_train",
     "psq_encoder_encode",
     "psq_encoder_params",
     "psq_encoder_set_params",
     "psq_encoder_grad",
     "psq_encoder_info",
+    "psq_encoder_mine_ms",
     "psq_knn_exact_batched",
     "psq_knn_exact_batched_device",
     "psq_batched_last_report",
@@ -93,6 +95,7 @@ class EncoderTrainOpts(ctypes.Structure):
         ("lr", ctypes.c_float),
         ("margin", ctypes.c_float),
         ("seed", ctypes.c_uint64),
+        ("anchors", ctypes.c_uint64),
     ]
 
 
@@ -216,6 +219,8 @@ def _declare(L: ctypes.CDLL) -> None:
     L.psq_encoder_create.argtypes = [PP, U64, I32]
     L.psq_encoder_destroy.restype = None
     L.psq_encoder_destroy.argtypes = [P]
+    L.psq_encoder_reset.restype = ctypes.c_int
+    L.psq_encoder_reset.argtypes = [P, U64]
     L.psq_encoder_train.restype = ctypes.c_int
     L.psq_encoder_train.argtypes = [P, P, U64, U64, ctypes.POINTER(EncoderTrainOpts), P]
     L.psq_encoder_encode.restype = ctypes.c_int
@@ -251,6 +256,8 @@ def _declare(L: ctypes.CDLL) -> None:
     L.psq_pca_csr_device.argtypes = [P, P, P, U64, U64, U64, I32, I32, D, P, P, P, P]
     L.psq_pca_last_report.restype = ctypes.c_int
     L.psq_pca_last_report.argtypes = [ctypes.POINTER(PcaReport)]
+    L.psq_encoder_mine_ms.restype = ctypes.c_int
+    L.psq_encoder_mine_ms.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
     L.psq_encoder_info.restype = ctypes.c_int
     L.psq_encoder_info.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                    ctypes.POINTER(U64)]

@@ -48,6 +48,13 @@ struct psq_encoder {
   uint64_t steps_done = 0;
   double last_loss = 0.0;
   double train_ms = 0.0;
+  double mine_ms = 0.0;  // subset, moments and the exact kNN of the anchor pool
+  // training buffers, grown on demand and kept until destroy
+  struct Buf {
+    void* p = nullptr;
+    size_t bytes = 0;
+  };
+  Buf sub, x, pool, knn, knd, partial, lt, mom, rows, grad;
 };
 
 namespace {
@@ -67,6 +74,32 @@ void init_params(std::vector<float>& p, uint64_t seed) {
 }
 
 unsigned blocks_for(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+template <typename T>
+int ebuf(psq_encoder::Buf& b, size_t count, T** out) {
+  const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  if (b.bytes < bytes) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    const hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess)
+      return fail(e == hipErrorOutOfMemory ? PSQ_E_NOMEM : PSQ_E_DEVICE,
+                  "encoder buffer of %zu bytes: %s", bytes, hipGetErrorString(e));
+    b.bytes = bytes;
+  }
+  *out = static_cast<T*>(b.p);
+  return PSQ_OK;
+}
+
+void free_bufs(psq_encoder* e) {
+  for (psq_encoder::Buf* b : {&e->sub, &e->x, &e->pool, &e->knn, &e->knd, &e->partial, &e->lt,
+                              &e->mom, &e->rows, &e->grad}) {
+    if (b->p) (void)hipFree(b->p);
+    b->p = nullptr;
+    b->bytes = 0;
+  }
+}
 
 }  // namespace
 
@@ -106,8 +139,28 @@ int psq_encoder_create(psq_encoder** out, uint64_t seed, int32_t device) {
   return PSQ_OK;
 }
 
+int psq_encoder_reset(psq_encoder* e, uint64_t seed) {
+  if (!e) return fail(PSQ_E_INVALID, "NULL encoder");
+  EHIPC(hipSetDevice(e->device));
+  std::vector<float> p;
+  init_params(p, seed);
+  EHIPC(hipMemcpy(e->w, p.data(), NP * sizeof(float), hipMemcpyHostToDevice));
+  EHIPC(hipMemset(e->m1, 0, NP * sizeof(float)));
+  EHIPC(hipMemset(e->m2, 0, NP * sizeof(float)));
+  hipLaunchKernelGGL(k_mlp_round, dim3(blocks_for(NP, 256)), dim3(256), 0, 0, e->w, e->wr, NP);
+  EHIPC(hipGetLastError());
+  EHIPC(hipDeviceSynchronize());
+  e->mu = 0.0f;
+  e->inv_scale = 1.0f;
+  e->trained = false;
+  e->steps_done = 0;
+  e->last_loss = e->train_ms = e->mine_ms = 0.0;
+  return PSQ_OK;
+}
+
 void psq_encoder_destroy(psq_encoder* e) {
   if (!e) return;
+  free_bufs(e);
   (void)hipFree(e->w);
   (void)hipFree(e->wr);
   (void)hipFree(e->m1);
@@ -149,25 +202,20 @@ int psq_encoder_grad(const psq_encoder* e, const float* d_data, uint64_t n, uint
   if (B == 0) return fail(PSQ_E_INVALID, "empty batch");
   for (uint64_t i = 0; i < 3 * B; ++i)
     if (rows[i] >= n) return fail(PSQ_E_INVALID, "row %llu out of range", (unsigned long long)i);
-  const uint32_t R = (uint32_t)(3 * B);
-  const uint32_t nb = blocks_for(R, 16);
+  psq_encoder* me = const_cast<psq_encoder*>(e);
+  const uint32_t nb = blocks_for(B, MLP_STEP_T);
   uint32_t* d_rows;
-  float *hpre, *z, *dz, *lt, *partial, *grad;
-  EHIPC(hipMalloc(&d_rows, R * 4));
-  EHIPC(hipMalloc(&hpre, (size_t)R * 128 * 4));
-  EHIPC(hipMalloc(&z, (size_t)R * 32 * 4));
-  EHIPC(hipMalloc(&dz, (size_t)R * 32 * 4));
-  EHIPC(hipMalloc(&lt, B * 4));
-  EHIPC(hipMalloc(&partial, (size_t)nb * NP * 4));
-  EHIPC(hipMalloc(&grad, NP * 4));
-  EHIPC(hipMemcpy(d_rows, rows, R * 4, hipMemcpyHostToDevice));
-  hipLaunchKernelGGL(k_mlp_forward, dim3(nb), dim3(128), 0, 0, d_data, 50u, d_rows, R, e->wr, e->mu,
-                     e->inv_scale, hpre, z);
-  hipLaunchKernelGGL(k_triplet_loss, dim3(blocks_for(B, 128)), dim3(128), 0, 0, z, (uint32_t)B,
-                     margin, dz, lt);
-  hipLaunchKernelGGL(k_mlp_backward, dim3(nb), dim3(128), 0, 0, d_data, 50u, d_rows, R, e->wr, e->mu,
-                     e->inv_scale, hpre, dz, partial);
-  hipLaunchKernelGGL(k_mlp_reduce, dim3(blocks_for(NP, 256)), dim3(256), 0, 0, partial, nb, grad);
+  float *lt, *partial, *grad;
+  ETRY(ebuf(me->rows, 3 * B, &d_rows));
+  ETRY(ebuf(me->lt, B, &lt));
+  ETRY(ebuf(me->partial, (size_t)nb * NP, &partial));
+  ETRY(ebuf(me->grad, NP, &grad));
+  EHIPC(hipMemcpy(d_rows, rows, 3 * B * 4, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_mlp_step, dim3(nb), dim3(128), 0, 0, d_data, d_rows, (uint32_t)B, nullptr, 0u,
+                     nullptr, 0u, 0u, 0ull, 0ull, e->wr, e->mu, e->inv_scale, margin, partial, lt);
+  EHIPC(hipGetLastError());
+  hipLaunchKernelGGL(k_mlp_reduce_adam, dim3(blocks_for(NP, 32)), dim3(256), 0, 0, partial, nb,
+                     nullptr, nullptr, nullptr, nullptr, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, grad);
   EHIPC(hipGetLastError());
   std::vector<float> hl(B);
   EHIPC(hipMemcpy(host_grad, grad, NP * 4, hipMemcpyDeviceToHost));
@@ -175,55 +223,54 @@ int psq_encoder_grad(const psq_encoder* e, const float* d_data, uint64_t n, uint
   double L = 0.0;
   for (float v : hl) L += v;
   *loss_out = L / (double)B;
-  (void)hipFree(d_rows);
-  (void)hipFree(hpre);
-  (void)hipFree(z);
-  (void)hipFree(dz);
-  (void)hipFree(lt);
-  (void)hipFree(partial);
-  (void)hipFree(grad);
   return PSQ_OK;
 }
 
-// Train on triplets from the exact kNN of a subset of d_data (device, n x 50):
-// subset = every (n / subset)-th cell; positives from its exact top-k_pos.
+// Train on triplets from the exact kNN within a subset of d_data (device,
+// n x 50): subset = every (n / subset)-th cell; anchors = an evenly spaced pool
+// of the subset, whose exact top-k_pos within the subset gives the positives;
+// negatives uniform over the subset. Each step is two launches: k_mlp_step
+// (sampling, forward, loss, backward, per-block partial gradients) and
+// k_mlp_reduce_adam (the ordered sum and the Adam step).
 int psq_encoder_train(psq_encoder* e, const float* d_data, uint64_t n, uint64_t d,
                       const psq_encoder_train_opts* o, void* stream) {
   if (!e || !d_data || !o) return fail(PSQ_E_INVALID, "NULL argument");
   if (d != 50) return fail(PSQ_E_UNSUPPORTED, "the encoder takes 50 dimensions");
   if (n < 2) return fail(PSQ_E_INVALID, "need at least 2 cells");
+  if (n >= (1ull << 31)) return fail(PSQ_E_UNSUPPORTED, "n must be < 2^31");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  hipEvent_t ev0, ev1;
+  hipEvent_t ev0, ev1, ev2;
   EHIPC(hipEventCreate(&ev0));
   EHIPC(hipEventCreate(&ev1));
+  EHIPC(hipEventCreate(&ev2));
   EHIPC(hipEventRecord(ev0, s));
   const uint64_t ns = std::min<uint64_t>(n, o->subset ? o->subset : 200000);
+  const uint64_t A = std::min<uint64_t>(ns, o->anchors ? o->anchors : 16384);
   const uint32_t kp = (uint32_t)std::min<uint64_t>(o->k_pos ? o->k_pos : 15, ns - 1);
   const uint32_t B = (uint32_t)(o->batch ? o->batch : 2048);
-  const uint64_t steps = o->steps ? o->steps : 1000;
+  const uint64_t steps = o->steps ? o->steps : 300;
   const float margin = o->margin > 0 ? o->margin : 1.0f;
   const float lr = o->lr > 0 ? o->lr : 1e-3f;
-  // subset: evenly spaced cells
-  std::vector<uint32_t> hsub(ns);
+  if (B > (1u << 24)) return fail(PSQ_E_INVALID, "batch must be <= 2^24");
+  // subset: evenly spaced cells; pool: evenly spaced subset rows
+  std::vector<uint32_t> hsub(ns), hpool(A);
   for (uint64_t i = 0; i < ns; ++i) hsub[i] = (uint32_t)((i * n) / ns);
-  uint32_t *d_sub, *d_knn, *d_rows;
-  float *d_x, *d_knd, *hpre, *z, *dz, *lt, *partial, *grad;
-  double* mom;
-  const uint32_t R = 3 * B, nb = blocks_for(R, 16);
+  for (uint64_t a = 0; a < A; ++a) hpool[a] = (uint32_t)((a * ns) / A);
+  const uint32_t nb = blocks_for(B, MLP_STEP_T);
   const uint32_t chunk = 1024, nmb = blocks_for(ns, chunk);
-  EHIPC(hipMalloc(&d_sub, ns * 4));
-  EHIPC(hipMalloc(&d_x, ns * 50 * 4));
-  EHIPC(hipMalloc(&d_knn, ns * kp * 4));
-  EHIPC(hipMalloc(&d_knd, ns * kp * 4));
-  EHIPC(hipMalloc(&d_rows, R * 4));
-  EHIPC(hipMalloc(&hpre, (size_t)R * 128 * 4));
-  EHIPC(hipMalloc(&z, (size_t)R * 32 * 4));
-  EHIPC(hipMalloc(&dz, (size_t)R * 32 * 4));
-  EHIPC(hipMalloc(&lt, B * 4));
-  EHIPC(hipMalloc(&partial, (size_t)nb * NP * 4));
-  EHIPC(hipMalloc(&grad, NP * 4));
-  EHIPC(hipMalloc(&mom, nmb * 2 * sizeof(double)));
+  uint32_t *d_sub, *d_pool, *d_knn;
+  float *d_x, *d_knd, *lt, *partial;
+  double* mom;
+  ETRY(ebuf(e->sub, ns, &d_sub));
+  ETRY(ebuf(e->x, ns * 50, &d_x));
+  ETRY(ebuf(e->pool, A, &d_pool));
+  ETRY(ebuf(e->knn, A * kp, &d_knn));
+  ETRY(ebuf(e->knd, A * kp, &d_knd));
+  ETRY(ebuf(e->lt, B, &lt));
+  ETRY(ebuf(e->partial, (size_t)nb * NP, &partial));
+  ETRY(ebuf(e->mom, nmb * 2, &mom));
   EHIPC(hipMemcpyAsync(d_sub, hsub.data(), ns * 4, hipMemcpyHostToDevice, s));
+  EHIPC(hipMemcpyAsync(d_pool, hpool.data(), A * 4, hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_gather_rows, dim3(blocks_for(ns * 50, 256)), dim3(256), 0, s, d_data, 50u,
                      d_sub, (uint32_t)ns, d_x);
   EHIPC(hipGetLastError());
@@ -242,43 +289,40 @@ int psq_encoder_train(psq_encoder* e, const float* d_data, uint64_t n, uint64_t 
   const double mean = s1 / cnt, var = std::max(1e-12, s2 / cnt - mean * mean);
   e->mu = (float)mean;
   e->inv_scale = (float)(1.0 / std::sqrt(var));
-  // exact kNN of the subset (positives)
-  ETRY(psq_host::knn_exact_device_impl(d_x, ns, 50, nullptr, ns, kp, d_knn, d_knd, s));
+  // exact top-kp of every anchor within the subset (the positives)
+  ETRY(psq_host::knn_exact_device_impl(d_x, ns, 50, d_pool, A, kp, d_knn, d_knd, s));
+  EHIPC(hipEventRecord(ev1, s));
   // Adam
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   double c1 = 1.0, c2 = 1.0;
   for (uint64_t t = 0; t < steps; ++t) {
-    hipLaunchKernelGGL(k_triplets, dim3(blocks_for(B, 256)), dim3(256), 0, s, d_knn, kp, (uint32_t)ns,
-                       B, o->seed, e->steps_done + t, d_rows);
-    hipLaunchKernelGGL(k_mlp_forward, dim3(nb), dim3(128), 0, s, d_x, 50u, d_rows, R, e->wr, e->mu,
-                       e->inv_scale, hpre, z);
-    hipLaunchKernelGGL(k_triplet_loss, dim3(blocks_for(B, 128)), dim3(128), 0, s, z, B, margin, dz, lt);
-    hipLaunchKernelGGL(k_mlp_backward, dim3(nb), dim3(128), 0, s, d_x, 50u, d_rows, R, e->wr, e->mu,
-                       e->inv_scale, hpre, dz, partial);
-    hipLaunchKernelGGL(k_mlp_reduce, dim3(blocks_for(NP, 256)), dim3(256), 0, s, partial, nb, grad);
+    hipLaunchKernelGGL(k_mlp_step, dim3(nb), dim3(128), 0, s, d_x, nullptr, B, d_pool, (uint32_t)A,
+                       d_knn, kp, (uint32_t)ns, o->seed, e->steps_done + t, e->wr, e->mu,
+                       e->inv_scale, margin, partial, lt);
     c1 *= b1;
     c2 *= b2;
-    hipLaunchKernelGGL(k_adam, dim3(blocks_for(NP, 256)), dim3(256), 0, s, e->w, e->wr, grad, e->m1,
-                       e->m2, lr, b1, b2, eps, (float)(1.0 - c1), (float)(1.0 - c2));
+    hipLaunchKernelGGL(k_mlp_reduce_adam, dim3(blocks_for(NP, 32)), dim3(256), 0, s, partial, nb,
+                       e->w, e->wr, e->m1, e->m2, lr, b1, b2, eps, (float)(1.0 - c1),
+                       (float)(1.0 - c2), nullptr);
     EHIPC(hipGetLastError());
   }
   std::vector<float> hl(B);
   EHIPC(hipMemcpyAsync(hl.data(), lt, B * 4, hipMemcpyDeviceToHost, s));
-  EHIPC(hipEventRecord(ev1, s));
+  EHIPC(hipEventRecord(ev2, s));
   EHIPC(hipStreamSynchronize(s));
   double L = 0;
   for (float v : hl) L += v;
-  e->last_loss = L / B;
-  float ms = 0;
-  (void)hipEventElapsedTime(&ms, ev0, ev1);
+  e->last_loss = steps > 0 ? L / B : 0.0;
+  float ms = 0, ms_mine = 0;
+  (void)hipEventElapsedTime(&ms, ev0, ev2);
+  (void)hipEventElapsedTime(&ms_mine, ev0, ev1);
   e->train_ms = ms;
+  e->mine_ms = ms_mine;
   e->steps_done += steps;
   e->trained = true;
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
-  for (void* p : {(void*)d_sub, (void*)d_x, (void*)d_knn, (void*)d_knd, (void*)d_rows, (void*)hpre,
-                  (void*)z, (void*)dz, (void*)lt, (void*)partial, (void*)grad, (void*)mom})
-    (void)hipFree(p);
+  (void)hipEventDestroy(ev2);
   return PSQ_OK;
 }
 
@@ -287,6 +331,12 @@ int psq_encoder_info(const psq_encoder* e, double* last_loss, double* train_ms, 
   if (last_loss) *last_loss = e->last_loss;
   if (train_ms) *train_ms = e->train_ms;
   if (steps) *steps = e->steps_done;
+  return PSQ_OK;
+}
+
+int psq_encoder_mine_ms(const psq_encoder* e, double* mine_ms) {
+  if (!e || !mine_ms) return fail(PSQ_E_INVALID, "NULL argument");
+  *mine_ms = e->mine_ms;
   return PSQ_OK;
 }
 

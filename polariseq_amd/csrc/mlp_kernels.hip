@@ -42,56 +42,6 @@ constexpr int OFF_W2 = OFF_B1 + MLP_H;
 constexpr int OFF_B2 = OFF_W2 + MLP_OUT * MLP_H;
 constexpr int N_PARAMS = OFF_B2 + MLP_OUT;
 
-// Forward for `rows` points given by index (or 0..rows-1 when idx is NULL) of
-// X (row stride ldx). One block of 128 threads per 16 rows: thread j computes
-// hidden unit j for the 16 rows, then 32 threads compute the outputs.
-// Writes h_pre (optional, for backward) and z.
-__global__ __launch_bounds__(128) void k_mlp_forward(
-    const float* __restrict__ X, uint32_t ldx, const uint32_t* __restrict__ idx, uint32_t rows,
-    const float* __restrict__ pr, float mu, float inv_scale, float* __restrict__ hpre_out,
-    float* __restrict__ z_out) {
-  __shared__ float s_x[16][MLP_IN];
-  __shared__ float s_h[16][MLP_H];
-  const uint32_t r0 = blockIdx.x * 16;
-  for (uint32_t e = threadIdx.x; e < 16 * MLP_IN; e += blockDim.x) {
-    const uint32_t rr = e / MLP_IN, t = e % MLP_IN;
-    const uint32_t r = r0 + rr;
-    float v = 0.0f;
-    if (r < rows) {
-      const uint32_t src = idx ? idx[r] : r;
-      v = (X[(size_t)src * ldx + t] - mu) * inv_scale;
-    }
-    s_x[rr][t] = v;
-  }
-  __syncthreads();
-  const uint32_t j = threadIdx.x;  // hidden unit
-  const float* w1 = pr + OFF_W1 + j * MLP_IN;
-  float wreg[MLP_IN];
-#pragma unroll
-  for (int t = 0; t < MLP_IN; ++t) wreg[t] = w1[t];
-  const float b1 = pr[OFF_B1 + j];
-  for (int rr = 0; rr < 16; ++rr) {
-    float acc = b1;
-#pragma unroll
-    for (int t = 0; t < MLP_IN; ++t) acc = fmaf(wreg[t], s_x[rr][t], acc);
-    const uint32_t r = r0 + rr;
-    if (hpre_out && r < rows) hpre_out[(size_t)r * MLP_H + j] = acc;
-    s_h[rr][j] = gelu(acc);
-  }
-  __syncthreads();
-  // outputs: 128 threads = 32 outputs x 4 row groups of 4 rows
-  const uint32_t o = threadIdx.x % MLP_OUT, rg = threadIdx.x / MLP_OUT;
-  const float* w2 = pr + OFF_W2 + o * MLP_H;
-  const float b2 = pr[OFF_B2 + o];
-  for (int q = 0; q < 4; ++q) {
-    const uint32_t rr = rg * 4 + q;
-    float acc = b2;
-    for (int t = 0; t < MLP_H; ++t) acc = fmaf(w2[t], s_h[rr][t], acc);
-    const uint32_t r = r0 + rr;
-    if (r < rows) z_out[(size_t)r * MLP_OUT + o] = acc;
-  }
-}
-
 // ── encode on the FP32 matrix cores ──
 // z for every row of X (stride 50) with v_mfma_f32_16x16x4f32, 16 rows per
 // wave-tile. MFMA 16x16x4 operands: lane l (c = l % 16, g = l / 16) holds
@@ -183,117 +133,188 @@ __global__ __launch_bounds__(ENC_WAVES * 64) void k_encode_mfma(
   }
 }
 
-// Triplet loss and dL/dz for a batch of B triplets laid out as rows
-// [a_0..a_{B-1}, p_0..p_{B-1}, n_0..n_{B-1}] of z. One thread per triplet.
-__global__ void k_triplet_loss(const float* __restrict__ z, uint32_t B, float margin,
-                               float* __restrict__ dz, float* __restrict__ loss_terms) {
-  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
-  const float* za = z + (size_t)b * MLP_OUT;
-  const float* zp = z + (size_t)(B + b) * MLP_OUT;
-  const float* zn = z + (size_t)(2 * B + b) * MLP_OUT;
-  float dap = 0.0f, dan = 0.0f;
-  for (int t = 0; t < MLP_OUT; ++t) {
-    const float u = za[t] - zp[t], v = za[t] - zn[t];
-    dap = fmaf(u, u, dap);
-    dan = fmaf(v, v, dan);
-  }
-  const float l = dap - dan + margin;
-  const bool active = l > 0.0f;
-  loss_terms[b] = active ? l : 0.0f;
-  const float s = active ? 2.0f / (float)B : 0.0f;
-  float* ga = dz + (size_t)b * MLP_OUT;
-  float* gp = dz + (size_t)(B + b) * MLP_OUT;
-  float* gn = dz + (size_t)(2 * B + b) * MLP_OUT;
-  for (int t = 0; t < MLP_OUT; ++t) {
-    ga[t] = s * (zn[t] - zp[t]);
-    gp[t] = -s * (za[t] - zp[t]);
-    gn[t] = s * (za[t] - zn[t]);
-  }
+// ── one training step in two launches ──
+// k_mlp_step: a block of MLP_STEP_T triplets (3 T rows), forward, triplet loss,
+// backward, its partial gradient (N_PARAMS floats per block). Triplets come from
+// `rows` ([anchors | positives | negatives] of the whole batch) or, when rows is
+// NULL, are sampled here: anchor = pool[hash % A] (cells whose exact top-kk is
+// in knn), positive uniform over that top-kk, negative uniform over the subset.
+// k_mlp_reduce_adam: one thread per parameter sums the blocks' partials in
+// block order and takes the Adam step (deterministic for a seed).
+constexpr int STEP_R = 3 * MLP_STEP_T;  // rows per block
+
+__device__ inline uint32_t step_row(const uint32_t* __restrict__ rows, uint32_t B, uint32_t b,
+                                    uint32_t role, const uint32_t* __restrict__ pool, uint32_t A,
+                                    const uint32_t* __restrict__ knn, uint32_t kk, uint32_t ns,
+                                    uint64_t seed, uint64_t step) {
+  if (rows) return rows[role * B + b];
+  const uint64_t key = step * 0x100000000ull + b;
+  const uint32_t a = (uint32_t)hash_index(seed, key, 1, A);
+  if (role == 0) return pool[a];
+  if (role == 1) return knn[(size_t)a * kk + (uint32_t)hash_index(seed, key, 2, kk)];
+  return (uint32_t)hash_index(seed, key, 3, ns);
 }
 
-// Backward, per block of 16 rows: partial weight gradients for the block.
-// partial layout per block: N_PARAMS floats. Thread j handles hidden unit j.
-__global__ __launch_bounds__(128) void k_mlp_backward(
-    const float* __restrict__ X, uint32_t ldx, const uint32_t* __restrict__ idx, uint32_t rows,
-    const float* __restrict__ pr, float mu, float inv_scale, const float* __restrict__ hpre,
-    const float* __restrict__ dz, float* __restrict__ partial) {
-  __shared__ float s_x[16][MLP_IN];
-  __shared__ float s_dz[16][MLP_OUT];
-  __shared__ float s_dpre[16][MLP_H];
-  const uint32_t r0 = blockIdx.x * 16;
-  for (uint32_t e = threadIdx.x; e < 16 * MLP_IN; e += blockDim.x) {
-    const uint32_t rr = e / MLP_IN, t = e % MLP_IN;
-    const uint32_t r = r0 + rr;
-    float v = 0.0f;
-    if (r < rows) {
-      const uint32_t src = idx ? idx[r] : r;
-      v = (X[(size_t)src * ldx + t] - mu) * inv_scale;
-    }
-    s_x[rr][t] = v;
+__global__ __launch_bounds__(128) void k_mlp_step(
+    const float* __restrict__ X, const uint32_t* __restrict__ rows, uint32_t B,
+    const uint32_t* __restrict__ pool, uint32_t A, const uint32_t* __restrict__ knn, uint32_t kk,
+    uint32_t ns, uint64_t seed, uint64_t step, const float* __restrict__ pr, float mu,
+    float inv_scale, float margin, float* __restrict__ partial, float* __restrict__ loss_terms) {
+  __shared__ float s_x[STEP_R][MLP_IN + 1];
+  __shared__ float s_hp[STEP_R][MLP_H + 1];  // h_pre, then dL/dh_pre
+  __shared__ float s_h[STEP_R][MLP_H + 1];   // gelu(h_pre)
+  __shared__ float s_z[STEP_R][MLP_OUT + 1];  // z, then dL/dz
+  __shared__ uint32_t s_row[STEP_R];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t b0 = blockIdx.x * MLP_STEP_T;
+  if (tid < (uint32_t)STEP_R) {
+    const uint32_t role = tid / MLP_STEP_T, b = b0 + tid % MLP_STEP_T;
+    s_row[tid] = b < B ? step_row(rows, B, b, role, pool, A, knn, kk, ns, seed, step) : 0xFFFFFFFFu;
   }
-  for (uint32_t e = threadIdx.x; e < 16 * MLP_OUT; e += blockDim.x) {
-    const uint32_t rr = e / MLP_OUT, t = e % MLP_OUT;
-    const uint32_t r = r0 + rr;
-    s_dz[rr][t] = r < rows ? dz[(size_t)r * MLP_OUT + t] : 0.0f;
+  __syncthreads();
+  for (uint32_t e = tid; e < (uint32_t)(STEP_R * MLP_IN); e += blockDim.x) {
+    const uint32_t rr = e / MLP_IN, t = e % MLP_IN;
+    const uint32_t src = s_row[rr];
+    s_x[rr][t] = src != 0xFFFFFFFFu ? (X[(size_t)src * MLP_IN + t] - mu) * inv_scale : 0.0f;
+  }
+  __syncthreads();
+  // layer 1: thread j = hidden unit j for every row
+  const uint32_t j = tid;
+  {
+    float w1[MLP_IN];
+#pragma unroll
+    for (int t = 0; t < MLP_IN; ++t) w1[t] = pr[OFF_W1 + j * MLP_IN + t];
+    const float b1 = pr[OFF_B1 + j];
+#pragma unroll 2
+    for (int rr = 0; rr < STEP_R; ++rr) {
+      float acc = b1;
+#pragma unroll
+      for (int t = 0; t < MLP_IN; ++t) acc = fmaf(w1[t], s_x[rr][t], acc);
+      s_hp[rr][j] = acc;
+      s_h[rr][j] = gelu(acc);
+    }
+  }
+  __syncthreads();
+  // layer 2: thread (o, g) = output o for rows g, g + 4, ...
+  const uint32_t o = tid % MLP_OUT, g = tid / MLP_OUT;
+  {
+    const float* w2 = pr + OFF_W2 + o * MLP_H;
+    const float b2 = pr[OFF_B2 + o];
+    for (int rr = g; rr < STEP_R; rr += 4) {
+      float acc = b2;
+#pragma unroll 8
+      for (int t = 0; t < MLP_H; ++t) acc = fmaf(w2[t], s_h[rr][t], acc);
+      s_z[rr][o] = acc;
+    }
+  }
+  __syncthreads();
+  // loss and dL/dz: 4 lanes per triplet (8 outputs each), written over z
+  {
+    float dap = 0.0f, dan = 0.0f;
+    const uint32_t tt = tid / 4, part = tid % 4;
+    const bool mine = tid < 4u * MLP_STEP_T;
+    const bool live = mine && b0 + tt < B;
+    float za[8], zp[8], zn[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      za[c] = mine ? s_z[tt][part * 8 + c] : 0.0f;
+      zp[c] = mine ? s_z[MLP_STEP_T + tt][part * 8 + c] : 0.0f;
+      zn[c] = mine ? s_z[2 * MLP_STEP_T + tt][part * 8 + c] : 0.0f;
+      const float u = za[c] - zp[c], v = za[c] - zn[c];
+      dap = fmaf(u, u, dap);
+      dan = fmaf(v, v, dan);
+    }
+    // the 4 lanes of a triplet are consecutive: combine in a fixed order
+    dap += __shfl_xor(dap, 1);
+    dan += __shfl_xor(dan, 1);
+    dap += __shfl_xor(dap, 2);
+    dan += __shfl_xor(dan, 2);
+    __syncthreads();  // every z read before dz overwrites it
+    if (mine) {
+      const float l = dap - dan + margin;
+      const bool active = live && l > 0.0f;
+      if (live && part == 0) loss_terms[b0 + tt] = active ? l : 0.0f;
+      const float sc = active ? 2.0f / (float)B : 0.0f;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        s_z[tt][part * 8 + c] = sc * (zn[c] - zp[c]);
+        s_z[MLP_STEP_T + tt][part * 8 + c] = -sc * (za[c] - zp[c]);
+        s_z[2 * MLP_STEP_T + tt][part * 8 + c] = sc * (za[c] - zn[c]);
+      }
+    }
   }
   __syncthreads();
   float* out = partial + (size_t)blockIdx.x * N_PARAMS;
-  const uint32_t j = threadIdx.x;
-  // dh_j = sum_o W2[o][j] dz_o ; dpre_j = dh_j * gelu'(hpre_j)
-  // dW2[o][j] = sum_r dz[r][o] * gelu(hpre[r][j])
-  float dw2[MLP_OUT];
+  // dW2[o][j] = sum_r dz[r][o] h[r][j]; dh_pre[r][j] = (sum_o W2[o][j] dz[r][o]) gelu'(h_pre)
+  {
+    float w2c[MLP_OUT];
 #pragma unroll
-  for (int o = 0; o < MLP_OUT; ++o) dw2[o] = 0.0f;
-  float db1 = 0.0f;
-  for (int rr = 0; rr < 16; ++rr) {
-    const uint32_t r = r0 + rr;
-    const float hp = r < rows ? hpre[(size_t)r * MLP_H + j] : 0.0f;
-    const float h = r < rows ? gelu(hp) : 0.0f;
-    float dh = 0.0f;
+    for (int q = 0; q < MLP_OUT; ++q) w2c[q] = pr[OFF_W2 + q * MLP_H + j];
+    float dw2[MLP_OUT];
 #pragma unroll
-    for (int o = 0; o < MLP_OUT; ++o) {
-      dh = fmaf(pr[OFF_W2 + o * MLP_H + j], s_dz[rr][o], dh);
-      dw2[o] = fmaf(s_dz[rr][o], h, dw2[o]);
+    for (int q = 0; q < MLP_OUT; ++q) dw2[q] = 0.0f;
+    float db1 = 0.0f;
+#pragma unroll 1
+    for (int rr = 0; rr < STEP_R; ++rr) {
+      const float h = s_h[rr][j];
+      float dh = 0.0f;
+#pragma unroll
+      for (int q = 0; q < MLP_OUT; ++q) {
+        const float dzv = s_z[rr][q];
+        dh = fmaf(w2c[q], dzv, dh);
+        dw2[q] = fmaf(dzv, h, dw2[q]);
+      }
+      const float dp = dh * gelu_grad(s_hp[rr][j]);
+      db1 += dp;
+      s_hp[rr][j] = dp;
     }
-    const float dp = r < rows ? dh * gelu_grad(hp) : 0.0f;
-    s_dpre[rr][j] = dp;
-    db1 += dp;
-  }
 #pragma unroll
-  for (int o = 0; o < MLP_OUT; ++o) out[OFF_W2 + o * MLP_H + j] = dw2[o];
-  out[OFF_B1 + j] = db1;
-  __syncthreads();
-  // dW1[j][t] = sum_r dpre[r][j] * x[r][t]
-  for (int t = 0; t < MLP_IN; ++t) {
-    float acc = 0.0f;
-    for (int rr = 0; rr < 16; ++rr) acc = fmaf(s_dpre[rr][j], s_x[rr][t], acc);
-    out[OFF_W1 + j * MLP_IN + t] = acc;
+    for (int q = 0; q < MLP_OUT; ++q) out[OFF_W2 + q * MLP_H + j] = dw2[q];
+    out[OFF_B1 + j] = db1;
   }
-  if (j < MLP_OUT) {
+  // dW1[j][t] = sum_r dh_pre[r][j] x[r][t] (thread j's own column of s_hp: no barrier)
+  {
+    float dw1[MLP_IN];
+#pragma unroll
+    for (int t = 0; t < MLP_IN; ++t) dw1[t] = 0.0f;
+#pragma unroll 1
+    for (int rr = 0; rr < STEP_R; ++rr) {
+      const float dp = s_hp[rr][j];
+#pragma unroll
+      for (int t = 0; t < MLP_IN; ++t) dw1[t] = fmaf(dp, s_x[rr][t], dw1[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < MLP_IN; ++t) out[OFF_W1 + j * MLP_IN + t] = dw1[t];
+  }
+  if (j < (uint32_t)MLP_OUT) {
     float acc = 0.0f;
-    for (int rr = 0; rr < 16; ++rr) acc += s_dz[rr][j];
+    for (int rr = 0; rr < STEP_R; ++rr) acc += s_z[rr][j];
     out[OFF_B2 + j] = acc;
   }
 }
 
-// Ordered sum of the per-block partials: grad[p] = sum_b partial[b][p].
-__global__ void k_mlp_reduce(const float* __restrict__ partial, uint32_t nblocks,
-                             float* __restrict__ grad) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= (uint32_t)N_PARAMS) return;
-  float s = 0.0f;
-  for (uint32_t b = 0; b < nblocks; ++b) s += partial[(size_t)b * N_PARAMS + p];
-  grad[p] = s;
-}
-
-// Adam step on the FP32 master weights, then BF16 rounding for the next pass.
-__global__ void k_adam(float* __restrict__ w, float* __restrict__ wr, const float* __restrict__ g,
-                       float* __restrict__ m1, float* __restrict__ m2, float lr, float b1, float b2,
-                       float eps, float c1, float c2) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= (uint32_t)N_PARAMS) return;
-  const float gg = g[p];
+// Block of 256 threads = 32 parameters x 8 groups: group g sums the partials
+// of blocks g, g + 8, ... in order, then the 8 group sums are added in a fixed
+// tree.
+__global__ __launch_bounds__(256) void k_mlp_reduce_adam(
+    const float* __restrict__ partial, uint32_t nblocks, float* __restrict__ w,
+    float* __restrict__ wr, float* __restrict__ m1, float* __restrict__ m2, float lr, float b1,
+    float b2, float eps, float c1, float c2, float* __restrict__ grad_out) {
+  __shared__ float s_g[8][33];
+  const uint32_t pl = threadIdx.x % 32, grp = threadIdx.x / 32;
+  const uint32_t p = blockIdx.x * 32 + pl;
+  float part = 0.0f;
+  if (p < (uint32_t)N_PARAMS)
+    for (uint32_t b = grp; b < nblocks; b += 8) part += partial[(size_t)b * N_PARAMS + p];
+  s_g[grp][pl] = part;
+  __syncthreads();
+  if (grp != 0 || p >= (uint32_t)N_PARAMS) return;
+  const float gg = ((s_g[0][pl] + s_g[1][pl]) + (s_g[2][pl] + s_g[3][pl])) +
+                   ((s_g[4][pl] + s_g[5][pl]) + (s_g[6][pl] + s_g[7][pl]));
+  if (grad_out) {  // gradient only (no step)
+    grad_out[p] = gg;
+    return;
+  }
   const float a = b1 * m1[p] + (1.0f - b1) * gg;
   const float v = b2 * m2[p] + (1.0f - b2) * gg * gg;
   m1[p] = a;
@@ -339,22 +360,6 @@ __global__ void k_gather_rows(const float* __restrict__ src, uint32_t d, const u
   if (e >= (size_t)rows * d) return;
   const uint32_t r = (uint32_t)(e / d), t = (uint32_t)(e - (size_t)r * d);
   dst[e] = src[(size_t)idx[r] * d + t];
-}
-
-// Triplet sampling for one step: anchors uniform over the subset, positive
-// uniform over the anchor's exact top-kk, negative uniform over the subset.
-// Rows written as [anchors | positives | negatives] (indices into the subset).
-__global__ void k_triplets(const uint32_t* __restrict__ knn, uint32_t kk, uint32_t ns, uint32_t B,
-                           uint64_t seed, uint64_t step, uint32_t* __restrict__ rows) {
-  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
-  const uint64_t key = step * 0x100000000ull + b;
-  const uint32_t a = (uint32_t)hash_index(seed, key, 1, ns);
-  const uint32_t pi = (uint32_t)hash_index(seed, key, 2, kk);
-  const uint32_t ng = (uint32_t)hash_index(seed, key, 3, ns);
-  rows[b] = a;
-  rows[B + b] = knn[(size_t)a * kk + pi];
-  rows[2 * B + b] = ng;
 }
 
 }  // namespace psq

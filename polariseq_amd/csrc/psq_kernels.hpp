@@ -95,8 +95,6 @@ __global__ void k_join_bf16(const float* Xp, uint32_t n, uint32_t d, uint32_t dp
 // learned encoder (mlp_kernels.hip)
 constexpr int MLP_N_PARAMS = 128 * 50 + 128 + 32 * 128 + 32;
 __global__ void k_mlp_round(const float* w, float* wr, uint32_t count);
-__global__ void k_mlp_forward(const float* X, uint32_t ldx, const uint32_t* idx, uint32_t rows,
-                              const float* pr, float mu, float inv_scale, float* hpre_out, float* z_out);
 // pp.neighbors' cosine metric on the approximate path: unit rows as numpy
 // scales them, and 1 - cos distances from the unit-row Euclidean ones.
 constexpr uint32_t ROWS_D_MAX = 256;
@@ -110,15 +108,14 @@ constexpr int ENC_WAVES = 4;
 __global__ void k_encode_pack(const float* pr, float* packed);
 __global__ void k_encode_mfma(const float* X, uint32_t rows, const float* packed, const float* pr,
                               float mu, float inv_scale, float* z_out);
-__global__ void k_triplet_loss(const float* z, uint32_t B, float margin, float* dz, float* loss_terms);
-__global__ void k_mlp_backward(const float* X, uint32_t ldx, const uint32_t* idx, uint32_t rows,
-                               const float* pr, float mu, float inv_scale, const float* hpre,
-                               const float* dz, float* partial);
-__global__ void k_mlp_reduce(const float* partial, uint32_t nblocks, float* grad);
-__global__ void k_adam(float* w, float* wr, const float* g, float* m1, float* m2, float lr, float b1,
-                       float b2, float eps, float c1, float c2);
-__global__ void k_triplets(const uint32_t* knn, uint32_t kk, uint32_t ns, uint32_t B, uint64_t seed,
-                           uint64_t step, uint32_t* rows);
+constexpr int MLP_STEP_T = 8;  // triplets per block of k_mlp_step
+__global__ void k_mlp_step(const float* X, const uint32_t* rows, uint32_t B, const uint32_t* pool,
+                           uint32_t A, const uint32_t* knn, uint32_t kk, uint32_t ns, uint64_t seed,
+                           uint64_t step, const float* pr, float mu, float inv_scale, float margin,
+                           float* partial, float* loss_terms);
+__global__ void k_mlp_reduce_adam(const float* partial, uint32_t nblocks, float* w, float* wr,
+                                  float* m1, float* m2, float lr, float b1, float b2, float eps,
+                                  float c1, float c2, float* grad_out);
 __global__ void k_gather_rows(const float* src, uint32_t d, const uint32_t* idx, uint32_t rows, float* dst);
 __global__ void k_moments(const float* X, uint32_t n, uint32_t d, uint32_t chunk, double* partial);
 

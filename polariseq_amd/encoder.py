@@ -41,12 +41,18 @@ class Encoder:
         except Exception:
             pass
 
+    def reset(self, seed: int = 0) -> None:
+        """Back to the untrained weights of ``Encoder(seed)`` (buffers kept)."""
+        N.check(N.lib().psq_encoder_reset(self._h, int(seed)))
+
     def train(self, x, *, subset: int = 200_000, k_pos: int = 15, batch: int = 2048,
-              steps: int = 1000, lr: float = 1e-3, margin: float = 1.0, seed: int = 0) -> dict:
-        """Train on a CUDA float32 tensor (n, 50) resident on the device."""
+              steps: int = 300, lr: float = 1e-3, margin: float = 1.0, seed: int = 0,
+              anchors: int = 16384) -> dict:
+        """Train on a CUDA float32 tensor (n, 50) resident on the device: triplets
+        from the exact top-k_pos (within an evenly spaced subset) of an anchor pool."""
         import torch
         _check_dev(x)
-        o = N.EncoderTrainOpts(subset, k_pos, batch, steps, lr, margin, seed)
+        o = N.EncoderTrainOpts(subset, k_pos, batch, steps, lr, margin, seed, anchors)
         N.check(N.lib().psq_encoder_train(self._h, x.data_ptr(), x.shape[0], x.shape[1],
                                           ctypes.byref(o),
                                           ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
@@ -56,7 +62,10 @@ class Encoder:
         loss, ms, steps = ctypes.c_double(), ctypes.c_double(), ctypes.c_uint64()
         N.check(N.lib().psq_encoder_info(self._h, ctypes.byref(loss), ctypes.byref(ms),
                                          ctypes.byref(steps)))
-        return {"last_loss": loss.value, "train_ms": ms.value, "steps": int(steps.value)}
+        mine = ctypes.c_double()
+        N.check(N.lib().psq_encoder_mine_ms(self._h, ctypes.byref(mine)))
+        return {"last_loss": loss.value, "train_ms": ms.value, "mine_ms": mine.value,
+                "steps": int(steps.value)}
 
     def encode(self, x):
         """Codes (n, 32) float32 on the device for a CUDA tensor (n, 50)."""
