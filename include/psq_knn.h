@@ -13,9 +13,11 @@
  *    run on `stream` (NULL = the default stream). They return after the
  *    results are complete in device memory.
  *  - The library owns its device working set. It grows as needed, is kept
- *    between calls, and is released by psq_release_workspace(). One call at a
- *    time per device: calls that overlap on one device must be serialised by
- *    the caller.
+ *    between calls, and is released by psq_release_workspace(). Calls are
+ *    reentrant: every entry point that uses a device's working set holds that
+ *    device's lock for the whole call, so calls from several threads on one
+ *    device run one after the other (the reference's calls are reentrant too).
+ *    The exception is psq_knn_exact_device_noalloc (see there).
  *  - Graph layout (crates/polariseq-core/src/backend.rs:37-46): CSR with indptr
  *    u32[n_obs+1], indices u32[nnz], dists f32[nnz]. Rows are ascending by
  *    distance, ties by index. Distances are Euclidean (sqrt of the squared
@@ -189,7 +191,8 @@ int psq_cosine_distances_device(float* d_dist, uint64_t count, void* stream);
  * Outputs: embedding f32[n_obs][n_components] (columns signed so that each
  * column's first largest-magnitude entry is positive), variance_ratio
  * f32[n_components] = max(eigenvalue, 0) / trace of the centred Gram, and, if
- * components is not NULL, the principal axes f32[n_features][n_components].
+ * components is not NULL, the principal axes f32[n_features][n_components], with the
+ * embedding's column signs (the centred matrix times the axes is the embedding).
  * Up to 4096 genes this is the reference's own route; from 4097 to 16384 the
  * reference switches to randomized SVD and this build stays exact (its result
  * meets the reference's acceptance bounds against the exact SVD, and more);

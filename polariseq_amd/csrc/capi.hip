@@ -33,7 +33,33 @@ int fail(int code, const char* fmt, ...) {
 }
 int knn_exact_device_impl(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d_queries,
                           uint64_t nq, uint64_t k, uint32_t* d_idx, float* d_dist, hipStream_t s);
+
+// One lock per device, held for a whole call of every entry point that uses
+// the device's cached working set: calls from several threads on one device
+// run one after the other instead of sharing buffers. Recursive, so an entry
+// that calls another (training mines triplets with the exact kNN) keeps it.
+std::mutex g_lock_mu;
+std::map<int, std::recursive_mutex> g_dev_locks;
+int lock_device(std::unique_lock<std::recursive_mutex>* out) {
+  int dev = 0;
+  const hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(PSQ_E_DEVICE, "hipGetDevice failed: %s", hipGetErrorString(e));
+  std::recursive_mutex* m;
+  {
+    std::lock_guard<std::mutex> g(g_lock_mu);
+    m = &g_dev_locks[dev];
+  }
+  *out = std::unique_lock<std::recursive_mutex>(*m);
+  return PSQ_OK;
+}
 }  // namespace psq_host
+
+#define LOCK_DEVICE()                                  \
+  std::unique_lock<std::recursive_mutex> dev_lock_;    \
+  do {                                                 \
+    int rc_lk_ = psq_host::lock_device(&dev_lock_);    \
+    if (rc_lk_ != PSQ_OK) return rc_lk_;               \
+  } while (0)
 
 using psq_host::fail;
 using psq_host::g_err;
@@ -1367,6 +1393,7 @@ int psq_last_stats(psq_nndescent_stats* out) {
 void psq_set_device_budget(uint64_t bytes) { g_budget = bytes; }
 
 int psq_release_workspace(void) {
+  LOCK_DEVICE();
   Workspace* ws;
   TRY(workspace(&ws));
   std::lock_guard<std::mutex> g(g_ws_mu);
@@ -1379,6 +1406,7 @@ int psq_nndescent_device(const float* d_data, uint64_t n_obs, uint64_t n_dim,
                          uint32_t* d_indices, float* d_dists, uint64_t cap, uint64_t* nnz_out,
                          void* stream) {
   g_err.clear();
+  LOCK_DEVICE();
   return nndescent_device_impl(d_data, n_obs, n_dim, opts, d_indptr, indptr_len, d_indices, d_dists,
                                cap, nnz_out, static_cast<hipStream_t>(stream));
 }
@@ -1398,6 +1426,7 @@ int psq_nndescent_into(const float* data, uint64_t n_obs, uint64_t n_dim,
     return fail(PSQ_E_INVALID, "outputs must hold %llu row pointers and %llu neighbours",
                 (unsigned long long)(n_obs + 1), (unsigned long long)(n_obs * R.ko));
   TRY(set_device(opts->device));
+  LOCK_DEVICE();
   Workspace* ws;
   TRY(workspace(&ws));
   float* d_data;
@@ -1422,6 +1451,7 @@ int psq_knn_exact_device(const float* d_data, uint64_t n, uint64_t d, const uint
                          uint64_t n_q, uint64_t k, uint32_t* d_idx_out, float* d_dist_out,
                          void* stream) {
   g_err.clear();
+  LOCK_DEVICE();
   return knn_exact_device_impl(d_data, n, d, d_queries, n_q, k, d_idx_out, d_dist_out,
                                static_cast<hipStream_t>(stream));
 }
@@ -1462,6 +1492,7 @@ int psq_knn_exact(const float* data, uint64_t n, uint64_t d, const uint32_t* que
     for (uint64_t i = 0; i < n_q; ++i)
       if (queries[i] >= n) return fail(PSQ_E_INVALID, "query %llu is out of range", (unsigned long long)i);
   TRY(set_device(device));
+  LOCK_DEVICE();
   Workspace* ws;
   TRY(workspace(&ws));
   float* d_data;
@@ -1507,6 +1538,7 @@ int psq_neighbors_exact_device(const float* d_embedding, uint64_t n_obs, uint64_
                                uint32_t* d_indices, float* d_dists, uint64_t cap,
                                uint64_t* nnz_out, void* stream) {
   g_err.clear();
+  LOCK_DEVICE();
   return neighbors_exact_device_impl(d_embedding, n_obs, n_dim, n_neighbors, metric, d_indptr,
                                      d_indices, d_dists, cap, nnz_out,
                                      static_cast<hipStream_t>(stream));
@@ -1526,6 +1558,7 @@ int psq_neighbors_exact(const float* embedding, uint64_t n_obs, uint64_t n_dim,
   }
   if (!indices || !dists) return fail(PSQ_E_INVALID, "NULL pointer argument");
   TRY(set_device(device));
+  LOCK_DEVICE();
   Workspace* ws;
   TRY(workspace(&ws));
   float *d_x, *d_dist;
@@ -1566,6 +1599,7 @@ int psq_fuzzy_connectivities_device(const uint32_t* d_indptr, const uint32_t* d_
   uint64_t m = 0;
   TRY(fuzzy_input_nnz(d_indptr, n_obs, &m, s));
   if (m > 0 && (!d_indices || !d_dists)) return fail(PSQ_E_INVALID, "NULL pointer argument");
+  LOCK_DEVICE();
   Workspace* ws;
   TRY(workspace(&ws));
   FuzzyRun R;
@@ -1603,6 +1637,7 @@ int psq_fuzzy_plan_new(const uint32_t* indptr, uint64_t indptr_len, const uint32
                 (unsigned long long)m, (unsigned long long)nnz_in);
   if (m > 0 && (!indices || !dists)) return fail(PSQ_E_INVALID, "NULL pointer argument");
   TRY(set_device(device));
+  LOCK_DEVICE();
   auto* P = new psq_fuzzy_plan();
   P->device = device;
   P->n = n_obs;

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -19,6 +20,7 @@ namespace psq_host {
 int fail(int code, const char* fmt, ...);
 int knn_exact_device_impl(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d_queries,
                           uint64_t nq, uint64_t k, uint32_t* d_idx, float* d_dist, hipStream_t s);
+int lock_device(std::unique_lock<std::recursive_mutex>* out);
 }  // namespace psq_host
 
 using psq_host::fail;
@@ -238,6 +240,8 @@ int psq_encoder_train(psq_encoder* e, const float* d_data, uint64_t n, uint64_t 
   if (d != 50) return fail(PSQ_E_UNSUPPORTED, "the encoder takes 50 dimensions");
   if (n < 2) return fail(PSQ_E_INVALID, "need at least 2 cells");
   if (n >= (1ull << 31)) return fail(PSQ_E_UNSUPPORTED, "n must be < 2^31");
+  std::unique_lock<std::recursive_mutex> dev_lock;  // the exact kNN below uses the workspace
+  ETRY(psq_host::lock_device(&dev_lock));
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipEvent_t ev0, ev1, ev2;
   EHIPC(hipEventCreate(&ev0));

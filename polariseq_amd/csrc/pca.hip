@@ -80,9 +80,11 @@ constexpr int PCA_MAX_DEGREE = 24;                // Chebyshev filter degree cap
 __global__ void k_pca_check(const uint64_t* __restrict__ indptr, uint64_t n,
                             const uint32_t* __restrict__ indices, uint64_t nnz, uint32_t m,
                             uint32_t* __restrict__ bad) {
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < nnz && indices[t] >= m) atomicOr(bad, 1u);
-  if (t < n && indptr[t + 1] < indptr[t]) atomicOr(bad, 2u);
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nnz || t < n; t += step) {
+    if (t < nnz && indices[t] >= m) atomicOr(bad, 1u);
+    if (t < n && indptr[t + 1] < indptr[t]) atomicOr(bad, 2u);
+  }
 }
 
 // ── column sums and the scaled view ──
@@ -167,11 +169,12 @@ __global__ void k_pca_scale_values(const uint32_t* __restrict__ indices, const f
                                    uint64_t nnz, const double* __restrict__ mean,
                                    const double* __restrict__ inv_sd, const double* __restrict__ zero,
                                    double clip, float* __restrict__ out) {
-  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nnz) return;
-  const uint32_t j = indices[e];
-  const double z = fmin(fmax(((double)x[e] - mean[j]) * inv_sd[j], -clip), clip);
-  out[e] = (float)(z - zero[j]);
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += step) {
+    const uint32_t j = indices[e];
+    const double z = fmin(fmax(((double)x[e] - mean[j]) * inv_sd[j], -clip), clip);
+    out[e] = (float)(z - zero[j]);
+  }
 }
 
 // ── Gram matrix ──
@@ -478,14 +481,15 @@ __global__ void k_pca_mub(const float* __restrict__ mu, const float* __restrict_
 __global__ void k_pca_project(const uint64_t* __restrict__ indptr, const uint32_t* __restrict__ indices,
                               const float* __restrict__ s, uint64_t n, const float* __restrict__ V,
                               uint32_t k, const float* __restrict__ mub, float* __restrict__ out) {
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n * k) return;
-  const uint64_t i = t / k;
-  const uint32_t c = (uint32_t)(t - i * k);
-  float acc = 0.0f;
-  for (uint64_t e = indptr[i]; e < indptr[i + 1]; ++e) acc = acc + s[e] * V[(size_t)indices[e] * k + c];
-  if (mub) acc = acc - mub[c];
-  out[t] = acc;
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n * k; t += step) {
+    const uint64_t i = t / k;
+    const uint32_t c = (uint32_t)(t - i * k);
+    float acc = 0.0f;
+    for (uint64_t e = indptr[i]; e < indptr[i + 1]; ++e) acc = acc + s[e] * V[(size_t)indices[e] * k + c];
+    if (mub) acc = acc - mub[c];
+    out[t] = acc;
+  }
 }
 
 // Column signs (rsvd.rs:439-456): the first entry of largest magnitude (over
@@ -541,8 +545,9 @@ __global__ void k_pca_sign_pick(const float* __restrict__ U, uint32_t k, uint32_
 }
 __global__ void k_pca_sign_apply(float* __restrict__ U, uint64_t count, uint32_t k,
                                  const uint32_t* __restrict__ flip) {
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < count && flip[t % k]) U[t] = -U[t];
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += step)
+    if (flip[t % k]) U[t] = -U[t];
 }
 
 // ratio[j] = max(lam_j - sigma, 0) / trace (lam holds eigenvalues of C + sigma I).
@@ -559,6 +564,9 @@ using namespace psq;
 namespace {
 
 unsigned blocks(uint64_t n, unsigned b) { return (unsigned)std::max<uint64_t>(1, (n + b - 1) / b); }
+// Grids of the kernels that loop over n- or nnz-sized ranges: a dispatch holds
+// fewer than 2^32 work-items, so the grid is capped and the kernel strides.
+unsigned stride_blocks(uint64_t n, unsigned b) { return std::min<unsigned>(blocks(n, b), 1u << 20); }
 
 thread_local psq_pca_report g_pca_report;
 
@@ -663,7 +671,7 @@ int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t 
     uint32_t* d_bad;
     PTRY(dv.alloc(1, &d_bad));
     PHIPC(hipMemsetAsync(d_bad, 0, 4, s));
-    hipLaunchKernelGGL(k_pca_check, dim3(blocks(std::max(nnz, n), 256)), dim3(256), 0, s, ip, n, ix, nnz, m,
+    hipLaunchKernelGGL(k_pca_check, dim3(stride_blocks(std::max(nnz, n), 256)), dim3(256), 0, s, ip, n, ix, nnz, m,
                        d_bad);
     PHIPC(hipGetLastError());
     uint32_t bad = 0;
@@ -718,7 +726,7 @@ int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t 
     hipLaunchKernelGGL(k_pca_scale_stats, dim3(blocks(m, 256)), dim3(256), 0, s, acc_s, acc_q, m, n, clip, mean,
                        inv, zr);
     if (nnz)
-      hipLaunchKernelGGL(k_pca_scale_values, dim3(blocks(nnz, 256)), dim3(256), 0, s, ix, x, nnz, mean, inv, zr,
+      hipLaunchKernelGGL(k_pca_scale_values, dim3(stride_blocks(nnz, 256)), dim3(256), 0, s, ix, x, nnz, mean, inv, zr,
                          clip, sv);
     PHIPC(hipGetLastError());
     S = sv;
@@ -899,7 +907,7 @@ int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t 
     PTRY(dv.alloc(kk, &mub));
     hipLaunchKernelGGL(k_pca_mub, dim3(blocks(kk, 64)), dim3(64), 0, s, mu, V, m, kk, mub);
   }
-  hipLaunchKernelGGL(k_pca_project, dim3(blocks(n * kk, 256)), dim3(256), 0, s, ip, ix, S, n, V, kk, mub, d_emb);
+  hipLaunchKernelGGL(k_pca_project, dim3(stride_blocks(n * kk, 256)), dim3(256), 0, s, ip, ix, S, n, V, kk, mub, d_emb);
   const uint32_t rps = std::max<uint32_t>(1, 256 / kk);
   const uint32_t Gs = (uint32_t)std::min<uint64_t>(2048, (n + 255) / 256);
   const uint64_t seg = (n + Gs - 1) / Gs;
@@ -908,7 +916,11 @@ int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t 
   PTRY(dv.alloc(kk, &flip));
   hipLaunchKernelGGL(k_pca_sign_part, dim3(Gs), dim3(kk * rps), 0, s, d_emb, n, kk, rps, seg, ppv, ppi);
   hipLaunchKernelGGL(k_pca_sign_pick, dim3(blocks(kk, 64)), dim3(64), 0, s, d_emb, kk, Gs, ppv, ppi, flip);
-  hipLaunchKernelGGL(k_pca_sign_apply, dim3(blocks(n * kk, 256)), dim3(256), 0, s, d_emb, n * kk, kk, flip);
+  hipLaunchKernelGGL(k_pca_sign_apply, dim3(stride_blocks(n * kk, 256)), dim3(256), 0, s, d_emb, n * kk, kk, flip);
+  // the axes take the embedding's column signs: (X - mu) V reproduces the embedding
+  if (d_comp)
+    hipLaunchKernelGGL(k_pca_sign_apply, dim3(stride_blocks((uint64_t)m * kk, 256)), dim3(256), 0, s, d_comp,
+                       (uint64_t)m * kk, kk, flip);
   hipLaunchKernelGGL(k_pca_ratio, dim3(blocks(kk, 64)), dim3(64), 0, s, lam, tv, kk, d_ratio);
   PHIPC(hipGetLastError());
   PTRY(evs.record(s, &ev_end));

@@ -147,8 +147,19 @@ def pca_csr_device(indptr, indices, data, n_features: int, n_components: int, *,
                              "(int64, int32, float32)")
     n = indptr.numel() - 1
     k = int(n_components)
+
+    def _check_out(t, shape, name):
+        # the C-ABI takes no buffer sizes: a wrong tensor would be written past its end
+        if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()
+                and tuple(t.shape) == shape and t.device == indptr.device):
+            raise ValueError(f"{name} must be a contiguous float32 CUDA tensor of shape {shape} "
+                             "on the input's device")
     if out is None:
         out = torch.empty((n, k), dtype=torch.float32, device=indptr.device)
+    else:
+        _check_out(out, (n, k), "out")
+    if components is not None:
+        _check_out(components, (int(n_features), k), "components")
     ratio = torch.empty(k, dtype=torch.float32, device=indptr.device)
     mv = math.inf if max_value is None else float(max_value)
     L = N.lib()

@@ -120,6 +120,42 @@ def test_one_point_is_the_reference_graph(orc):
     assert m["indices"].tolist() == [0] and m["data"].tolist() == [0.0]
 
 
+def test_calls_from_two_threads_give_the_graphs_of_single_calls():
+    # ctypes releases the GIL, so the two threads' calls overlap on the one
+    # device; the library holds the device for each whole call.
+    import threading
+    x = ds.mixture(60000, 50, 20, seed=4)
+    y = ds.mixture(30000, 50, 10, seed=5)
+    want = {"x": P.nndescent_dense(x, 15, seed=1), "y": P.nndescent_dense(y, 10, seed=2),
+            "q": P.knn_exact(x, 15, np.arange(0, 60000, 7, dtype=np.uint32))}
+    got, errors = {}, []
+
+    def run(name):
+        try:
+            for r in range(3):
+                if name == "x":
+                    got[(name, r)] = P.nndescent_dense(x, 15, seed=1)
+                elif name == "y":
+                    got[(name, r)] = P.nndescent_dense(y, 10, seed=2)
+                else:
+                    got[(name, r)] = P.knn_exact(x, 15, np.arange(0, 60000, 7, dtype=np.uint32))
+        except Exception as e:  # re-raised below
+            errors.append(e)
+
+    ts = [threading.Thread(target=run, args=(nm,)) for nm in ("x", "y", "q")]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errors, errors
+    for (name, r), g in got.items():
+        if name == "q":
+            assert np.array_equal(g[0], want["q"][0]) and np.array_equal(g[1], want["q"][1])
+        else:
+            for key in ("indptr", "indices", "data"):
+                assert np.array_equal(g[key], want[name][key]), (name, r, key)
+
+
 def test_distances_are_the_reference_sq_dist(orc):
     x = ds.mixture(3000, 50, 6, seed=3)
     g = P.nndescent_dense(x, 15, init="random")

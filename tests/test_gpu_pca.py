@@ -111,6 +111,10 @@ def test_pca_matches_oracle(n, m, k, center, seed):
     # after the axes: bit for bit
     same = P.pca_gram(ip, ix, dx, n, m, k, center, components=comp)
     assert np.array_equal(emb, same["embedding"])
+    # the axes carry the embedding's signs: projecting on them needs no flip
+    mu = P.centred_gram(ip, ix, dx, n, m, center)[2]
+    raw = P.project(ip, ix, dx, n, comp, P.mu_b(mu, comp) if center else None)
+    assert np.array_equal(emb, raw)
 
 
 @pytest.mark.parametrize("max_value", [None, 3.0, 10.0])
