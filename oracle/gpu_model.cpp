@@ -13,10 +13,14 @@
 //    reference's heap keeps the same k smallest; here exact ties go to the lower
 //    index, not to the first arrival (nndescent.rs:176-181).
 //  * Candidate sampling keeps, per row and flag class, the m smallest
-//    (priority, neighbour) keys after deduplicating on neighbour, keeping the
-//    smaller priority. The reference keeps the first arrival
-//    (nndescent.rs:244-249). Priorities are the same hash_unit numerators
-//    (nndescent.rs:872), keyed by the row's sorted slot.
+//    (priority, neighbour) keys after deduplicating on neighbour. Of the two
+//    copies of a neighbour j of row i in one class (i's own entry, and j's entry
+//    naming i), the one from the earlier source row survives (the forward copy
+//    when i < j): the reference applies the candidate updates in source-row
+//    order and rejects a duplicate of a neighbour already held
+//    (nndescent.rs:244-249, 856-892), so there the first arrival keeps its
+//    priority (unless it was already evicted). Priorities are the same
+//    hash_unit numerators (nndescent.rs:872), keyed by the row's sorted slot.
 //  * Join proposals for a row are routed by a hash of the neighbour to one of
 //    S/8 buckets of 8 slots. The row's current neighbours occupy ("pin") the
 //    slots of their buckets (at most 8 per bucket), and a proposal of a
@@ -344,31 +348,37 @@ int64_t psq_o_gpu_model_nndescent(const float* data, uint64_t n_obs, uint64_t n_
           rev[cur[key_idx(key)]++] = (cls << 63) | (pr << 32) | (uint64_t)i;
         }
     }
-    // 2b. per row: dedup on neighbour per class (smaller priority wins), keep m smallest.
+    // 2b. per row: dedup on neighbour per class (the copy from the earlier source
+    // row wins), keep m smallest.
     pool.parallel_for(n, 256, [&](size_t i) {
-      std::vector<uint64_t> all;
+      // (key, from_reverse)
+      std::vector<std::pair<uint64_t, int>> all;
       for (size_t s = 0; s < k; ++s) {
         uint64_t key = G[i * k + s];
         if (key == EMPTY_KEY) continue;
         uint64_t cls = (key & 1) ? 0 : 1;
         uint64_t pr = hash_prio24(seed, iter, i * k + s);
-        all.push_back((cls << 63) | (pr << 32) | key_idx(key));
+        all.emplace_back((cls << 63) | (pr << 32) | key_idx(key), 0);
       }
-      for (uint32_t r = roff[i]; r < roff[i + 1]; ++r) all.push_back(rev[r]);
-      // dedup on (class, nbr) keeping the smallest priority
-      std::sort(all.begin(), all.end(), [](uint64_t a, uint64_t b) {
+      for (uint32_t r = roff[i]; r < roff[i + 1]; ++r) all.emplace_back(rev[r], 1);
+      // group by (class, nbr); within a group the surviving copy first
+      std::sort(all.begin(), all.end(), [i](const std::pair<uint64_t, int>& x,
+                                           const std::pair<uint64_t, int>& y) {
+        const uint64_t a = x.first, b = y.first;
         uint64_t ca = a >> 63, cb = b >> 63;
         uint32_t na = (uint32_t)a, nb = (uint32_t)b;
         if (ca != cb) return ca < cb;
         if (na != nb) return na < nb;
-        return a < b;
+        // forward (source i) before reverse (source na) when i < na
+        const bool fwd_first = i < na;
+        return fwd_first ? x.second < y.second : x.second > y.second;
       });
       std::vector<uint64_t> uniq;
       for (size_t t = 0; t < all.size(); ++t) {
-        if (t > 0 && (all[t] >> 63) == (all[t - 1] >> 63) &&
-            (uint32_t)all[t] == (uint32_t)all[t - 1])
+        if (t > 0 && (all[t].first >> 63) == (all[t - 1].first >> 63) &&
+            (uint32_t)all[t].first == (uint32_t)all[t - 1].first)
           continue;
-        uniq.push_back(all[t]);
+        uniq.push_back(all[t].first);
       }
       std::sort(uniq.begin(), uniq.end());
       size_t nn = 0, no = 0;

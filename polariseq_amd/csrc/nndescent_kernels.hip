@@ -170,8 +170,10 @@ __global__ void k_single_point(const float* __restrict__ X, uint32_t d, uint32_t
 // ── sampling (nndescent.rs:853-909) ──
 // One wave (one block) per row i. Candidates: the row's own entries (forward,
 // priority of slot i*k+s) and the reverse entries rev[roff[i]..roff[i+1]).
-// Per flag class, a neighbour seen twice keeps its smaller priority (a forward
-// entry wins ties); then the m smallest (priority, neighbour) keys form the new
+// Per flag class, a neighbour j seen twice keeps the copy from the earlier
+// source row (the forward one when i < j; the reference applies candidate
+// updates in source order and rejects later duplicates, nndescent.rs:244-249,
+// 856-892); then the m smallest (priority, neighbour) keys form the new
 // and old lists. Entries of the row that made the new list lose their new flag
 // (nndescent.rs:894-909). Candidate t < k is forward entry t, t >= k reverse
 // entry t - k; they are taken 64 at a time. Key: (class << 63) | (prio24 << 32)
@@ -218,8 +220,9 @@ __global__ __launch_bounds__(64) void k_sample_select(
   const uint32_t r0 = roff[i], nrev = roff[i + 1] - r0, T = k + nrev;
   const uint32_t nvec = (T + WAVE - 1) / WAVE;
   // Reverse candidate t of vector j (t >= k): its key, or EMPTY when a forward
-  // entry of the same class and neighbour has a priority <= its own. When its
-  // priority is smaller, it replaces that forward entry (record = true).
+  // entry of the same class names the same neighbour and comes from the earlier
+  // source row (i before the neighbour). When the reverse copy's source (the
+  // neighbour) comes first, it replaces that forward entry (record = true).
   auto rev_key = [&](uint32_t t, bool record) -> uint64_t {
     if (t < k || t >= T) return EMPTY_KEY;
     const uint64_t rv = rev[r0 + (t - k)];
@@ -233,7 +236,7 @@ __global__ __launch_bounds__(64) void k_sample_select(
       const uint64_t fv = s_fk[f];
       if ((uint32_t)fv != src) continue;
       if ((fv >> 63) != (cp >> 31)) return rv;  // other class: both stay
-      if ((cp & 0x7FFFFFFFull) < ((fv >> 32) & 0x7FFFFFFFull)) {
+      if (src < i) {
         if (record) s_kkey[f] = rv;
         return rv;
       }

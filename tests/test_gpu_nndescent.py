@@ -167,19 +167,20 @@ def test_distances_are_the_reference_sq_dist(orc):
 
 
 @pytest.mark.parametrize("init", ["rp_forest", "random"])
-def test_recall_at_least_the_reference_restatement(orc, init):
+def test_recall_at_least_the_reference_restatement(orc, init, not_below):
     # 20k x 50 cells, 20-component mixture, k = 15, seed 0 (the CPU-runnable case).
+    # Every cell is a query; three seeds of both searches.
     x = ds.mixture(20000, 50, 20, seed=0)
-    q = np.random.default_rng(1).choice(20000, 4000, replace=False).astype(np.uint32)
-    ei, _ = orc.knn_exact64(x, 15, q)
-    gp, _ = orc.nndescent_port(x, 15, seed=0)
-    g = P.nndescent_dense(x, 15, seed=0, init=init)
-    rp = ds.recall(gp["indices"].reshape(-1, 15)[q], ei).mean()
-    rg = ds.recall(g["indices"].reshape(-1, 15)[q], ei).mean()
-    assert rg >= 0.95
-    # equal within the sampling noise of 4000 queries (a paired difference
-    # of 0.002 is about two standard errors)
-    assert rg >= rp - 0.002, (rg, rp)
+    ei, _ = orc.knn_exact64(x, 15)
+    diffs = []
+    for seed in (0, 1, 2):
+        gp, _ = orc.nndescent_port(x, 15, seed=seed)
+        g = P.nndescent_dense(x, 15, seed=seed, init=init)
+        rp = ds.recall(gp["indices"].reshape(-1, 15), ei)
+        rg = ds.recall(g["indices"].reshape(-1, 15), ei)
+        assert rg.mean() >= 0.95
+        diffs.append(rg - rp)
+    not_below(np.concatenate(diffs))
 
 
 def test_device_entry_and_workspace_reuse(orc):
