@@ -486,35 +486,45 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
   HIPC(hipMemcpyAsync(tseeds, hseeds.data(), T * sizeof(uint64_t), hipMemcpyHostToDevice, s));
   const size_t total_cap = (size_t)TB * n;
   const size_t node_cap = (size_t)TB * (n / (ls + 1) + 1);
-  uint32_t *ids[2], *segv[2], *fl, *fr, *sl, *sr, *lcnt_pos;
-  TRY(wbuf(ws, "f_ids0", total_cap, &ids[0]));
-  TRY(wbuf(ws, "f_ids1", total_cap, &ids[1]));
-  TRY(wbuf(ws, "f_seg0", total_cap, &segv[0]));
-  TRY(wbuf(ws, "f_seg1", total_cap, &segv[1]));
+  // active nodes hold more than cap points (more than leaf_size without a finish)
+  const size_t act_cap = std::min(node_cap, cap > 0 ? total_cap / (cap + 1) + TB : node_cap);
+  const size_t tile_cap = total_cap / FT_TILE + act_cap + 1;
+  uint32_t *ids, *tmp, *fl, *sl, *lcnt_pos;
+  TRY(wbuf(ws, "f_ids", total_cap, &ids));
+  TRY(wbuf(ws, "f_tmp", total_cap, &tmp));
   TRY(wbuf(ws, "f_fl", total_cap, &fl));
-  TRY(wbuf(ws, "f_fr", total_cap, &fr));
   TRY(wbuf(ws, "f_sl", total_cap + 1, &sl));
-  TRY(wbuf(ws, "f_sr", total_cap + 1, &sr));
   TRY(wbuf(ws, "f_lcnt", total_cap, &lcnt_pos));
   uint32_t *ns[2], *nc[2], *nt[2], *nlv, *mode, *nact, *nsm, *aoff, *soff, *child;
   uint32_t *sm_start, *sm_count, *sm_tree, *sm_level, *toff;
+  uint32_t *ntile, *tileoff, *tnode, *tcl, *tcr, *scl, *scr;
+  uint64_t *flmask, *frmask;
   float *normals, *offsets, *nabs;
-  TRY(wbuf(ws, "f_ns0", node_cap, &ns[0]));
-  TRY(wbuf(ws, "f_ns1", node_cap, &ns[1]));
-  TRY(wbuf(ws, "f_nc0", node_cap, &nc[0]));
-  TRY(wbuf(ws, "f_nc1", node_cap, &nc[1]));
-  TRY(wbuf(ws, "f_nt0", node_cap, &nt[0]));
-  TRY(wbuf(ws, "f_nt1", node_cap, &nt[1]));
-  TRY(wbuf(ws, "f_nl", node_cap, &nlv));
-  TRY(wbuf(ws, "f_mode", node_cap, &mode));
-  TRY(wbuf(ws, "f_nact", node_cap, &nact));
-  TRY(wbuf(ws, "f_nsm", node_cap, &nsm));
-  TRY(wbuf(ws, "f_aoff", node_cap + 1, &aoff));
-  TRY(wbuf(ws, "f_soff", node_cap + 1, &soff));
-  TRY(wbuf(ws, "f_child", 2 * node_cap, &child));
-  TRY(wbuf(ws, "f_normals", node_cap * dps, &normals));
-  TRY(wbuf(ws, "f_offsets", node_cap, &offsets));
-  TRY(wbuf(ws, "f_nabs", node_cap, &nabs));
+  TRY(wbuf(ws, "f_ns0", act_cap, &ns[0]));
+  TRY(wbuf(ws, "f_ns1", act_cap, &ns[1]));
+  TRY(wbuf(ws, "f_nc0", act_cap, &nc[0]));
+  TRY(wbuf(ws, "f_nc1", act_cap, &nc[1]));
+  TRY(wbuf(ws, "f_nt0", act_cap, &nt[0]));
+  TRY(wbuf(ws, "f_nt1", act_cap, &nt[1]));
+  TRY(wbuf(ws, "f_nl", act_cap, &nlv));
+  TRY(wbuf(ws, "f_mode", act_cap, &mode));
+  TRY(wbuf(ws, "f_nact", act_cap, &nact));
+  TRY(wbuf(ws, "f_nsm", act_cap, &nsm));
+  TRY(wbuf(ws, "f_aoff", act_cap + 1, &aoff));
+  TRY(wbuf(ws, "f_soff", act_cap + 1, &soff));
+  TRY(wbuf(ws, "f_child", 2 * act_cap, &child));
+  TRY(wbuf(ws, "f_normals", act_cap * dps, &normals));
+  TRY(wbuf(ws, "f_offsets", act_cap, &offsets));
+  TRY(wbuf(ws, "f_nabs", act_cap, &nabs));
+  TRY(wbuf(ws, "f_ntile", act_cap, &ntile));
+  TRY(wbuf(ws, "f_tileoff", act_cap + 1, &tileoff));
+  TRY(wbuf(ws, "f_tnode", tile_cap, &tnode));
+  TRY(wbuf(ws, "f_tcl", tile_cap, &tcl));
+  TRY(wbuf(ws, "f_tcr", tile_cap, &tcr));
+  TRY(wbuf(ws, "f_scl", tile_cap + 1, &scl));
+  TRY(wbuf(ws, "f_scr", tile_cap + 1, &scr));
+  TRY(wbuf(ws, "f_flmask", tile_cap * (FT_TILE / 64), &flmask));
+  TRY(wbuf(ws, "f_frmask", tile_cap * (FT_TILE / 64), &frmask));
   // FP16 copy of the split rows for the margin screen (64- or 128-byte rows)
   const uint32_t dh = ds <= 32 ? 32u : round_up(ds, 64);
   _Float16* Xh;
@@ -536,8 +546,7 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
   for (uint64_t t0 = 0; t0 < T; t0 += TB) {
     const uint32_t tb = (uint32_t)std::min<uint64_t>(TB, T - t0);
     const size_t total = (size_t)tb * n;
-    hipLaunchKernelGGL(k_forest_init, dim3(grid_for(total, 256)), dim3(256), 0, s, ids[0], segv[0],
-                       n, tb, ls);
+    hipLaunchKernelGGL(k_forest_init, dim3(grid_for(total, 256)), dim3(256), 0, s, ids, n, tb);
     HIPC(hipGetLastError());
     TRY(fill_u32(lcnt_pos, total, 0, s));
     std::vector<uint32_t> h0s(tb, 0), h0c(tb, n), h0t(tb), h0l(tb, 0);
@@ -562,22 +571,29 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
     int cur = 0;
     for (uint32_t L = 0; A > 0; ++L) {
       const int nx = cur ^ 1;
+      // tiles of the active nodes
+      hipLaunchKernelGGL(k_ftile_count, dim3(grid_for(A, 256)), dim3(256), 0, s, nc[cur], A, ntile);
+      HIPC(hipGetLastError());
+      TRY(scan_u32(ws, ntile, tileoff, A, s));
+      HIPC(hipMemcpyAsync(hbuf, tileoff + A, 4, hipMemcpyDeviceToHost, s));
+      HIPC(hipStreamSynchronize(s));
+      const uint32_t NT = hbuf[0];
+      if ((size_t)NT > tile_cap)
+        return fail(PSQ_E_DEVICE, "forest bookkeeping overflow (tiles %u)", NT);
+      hipLaunchKernelGGL(k_ftile_map, dim3(grid_for(NT, 256)), dim3(256), 0, s, tileoff, A, NT, tnode);
+      HIPC(hipGetLastError());
       hipLaunchKernelGGL(k_forest_planes, dim3(grid_for(A, 4)), dim3(256), 0, s, Xs, ds, dps, n,
-                         ids[cur], ns[cur], nc[cur], nt[cur], A, tseeds + t0, L, normals, offsets,
+                         ids, ns[cur], nc[cur], nt[cur], A, tseeds + t0, L, normals, offsets,
                          nabs);
       HIPC(hipGetLastError());
-      hipLaunchKernelGGL(k_forest_sides, dim3(grid_for(total, 256)), dim3(256), 0, s, Xs, ds, dps, n,
-                         Xh, dh, total, ids[cur], segv[cur], nt[cur], tseeds + t0, L, normals,
-                         offsets, nabs, fl, fr);
+      hipLaunchKernelGGL(k_forest_sides_tile, dim3(NT), dim3(256), 0, s, Xs, ds, dps, n, Xh, dh, ids,
+                         ns[cur], nc[cur], nt[cur], tileoff, tnode, tseeds + t0, L, normals,
+                         offsets, nabs, flmask, frmask, tcl, tcr);
       HIPC(hipGetLastError());
-      hipLaunchKernelGGL(k_forest_sides_exact, dim3(grid_for(total, 256)), dim3(256), 0, s, Xs, ds,
-                         dps, total, ids[cur], segv[cur], nt[cur], tseeds + t0, L, normals, offsets,
-                         fl);
-      HIPC(hipGetLastError());
-      TRY(scan_u32(ws, fl, sl, (uint32_t)total, s));
-      TRY(scan_u32(ws, fr, sr, (uint32_t)total, s));
-      hipLaunchKernelGGL(k_forest_nodes, dim3(grid_for(A, 256)), dim3(256), 0, s, ns[cur], nc[cur],
-                         nt[cur], A, n, sl, sr, ls, cap, L, nlv, mode, nact, nsm);
+      TRY(scan_u32(ws, tcl, scl, NT, s));
+      TRY(scan_u32(ws, tcr, scr, NT, s));
+      hipLaunchKernelGGL(k_forest_nodes_tile, dim3(grid_for(A, 256)), dim3(256), 0, s, nc[cur], A,
+                         tileoff, scl, scr, ls, cap, L, nlv, mode, nact, nsm);
       HIPC(hipGetLastError());
       TRY(scan_u32(ws, nact, aoff, A, s));
       TRY(scan_u32(ws, nsm, soff, A, s));
@@ -585,16 +601,18 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
       HIPC(hipMemcpyAsync(hbuf + 1, soff + A, 4, hipMemcpyDeviceToHost, s));
       HIPC(hipStreamSynchronize(s));
       const uint32_t nextA = hbuf[0], newsmall = hbuf[1];
-      if ((size_t)nextA > node_cap || (size_t)small_base + newsmall > node_cap)
+      if ((size_t)nextA > act_cap || (size_t)small_base + newsmall > node_cap)
         return fail(PSQ_E_DEVICE, "forest bookkeeping overflow (nodes %u, subtrees %u)", nextA,
                     small_base + newsmall);
       hipLaunchKernelGGL(k_forest_emit, dim3(grid_for(A, 256)), dim3(256), 0, s, ns[cur], nc[cur],
                          nt[cur], A, n, nlv, mode, aoff, soff, ls, cap, L, ns[nx], nc[nx], nt[nx],
                          child, lcnt_pos, small_base, sm_start, sm_count, sm_tree, sm_level);
       HIPC(hipGetLastError());
-      hipLaunchKernelGGL(k_forest_scatter, dim3(grid_for(total, 256)), dim3(256), 0, s, total, n,
-                         ids[cur], segv[cur], ns[cur], nt[cur], nlv, mode, child, sl, sr, ids[nx],
-                         segv[nx]);
+      hipLaunchKernelGGL(k_forest_scatter_tile, dim3(NT), dim3(256), 0, s, n, ids, ns[cur], nc[cur],
+                         nt[cur], tileoff, tnode, nlv, mode, flmask, frmask, scl, scr, tmp);
+      HIPC(hipGetLastError());
+      hipLaunchKernelGGL(k_forest_copyback_tile, dim3(NT), dim3(256), 0, s, n, ns[cur], nc[cur],
+                         nt[cur], tileoff, tnode, tmp, ids);
       HIPC(hipGetLastError());
       cur = nx;
       A = nextA;
@@ -602,7 +620,7 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
     }
     if (small_base > 0) {
       hipLaunchKernelGGL(k_forest_finish, dim3(small_base), dim3(64), fin_smem, s, Xs, ds, dps,
-                         RSs, n, ids[cur], sm_start, sm_count, sm_tree, sm_level, tseeds + t0, ls,
+                         RSs, n, ids, sm_start, sm_count, sm_tree, sm_level, tseeds + t0, ls,
                          cap, NC, lcnt_pos);
       HIPC(hipGetLastError());
     }
@@ -623,7 +641,7 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
         TRY(wbuf(ws, "perm", n, &rn->perm));
         TRY(wbuf(ws, "inv", n, &rn->inv));
         TRY(wbuf(ws, "Xq", (size_t)n * dp, &rn->Xq));
-        HIPC(hipMemcpyAsync(rn->perm, ids[cur], (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+        HIPC(hipMemcpyAsync(rn->perm, ids, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
         hipLaunchKernelGGL(k_perm_inverse, dim3(grid_for(n, 256)), dim3(256), 0, s, rn->perm, n,
                            rn->inv);
         HIPC(hipGetLastError());
@@ -631,7 +649,7 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
                            Xp, dp, rn->perm, n, rn->Xq);
         HIPC(hipGetLastError());
       }
-      hipLaunchKernelGGL(k_renumber, dim3(grid_for(total, 256)), dim3(256), 0, s, ids[cur], total,
+      hipLaunchKernelGGL(k_renumber, dim3(grid_for(total, 256)), dim3(256), 0, s, ids, total,
                          rn->inv);
       HIPC(hipGetLastError());
     }
@@ -640,7 +658,7 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
     for (uint32_t t = 0; t < tb; ++t) {
       const uint32_t nl = hoff[t + 1] - hoff[t];
       if (nl == 0) continue;
-      const uint32_t* ids_t = ids[cur] + (size_t)t * n;
+      const uint32_t* ids_t = ids + (size_t)t * n;
       if (d == 50)
         TRY(launch_leaf_merge<50>(KM, LG, Xl, d, dp, n, ids_t, gs + hoff[t], gc + hoff[t], nl, G, k, s));
       else

@@ -84,13 +84,11 @@ __device__ inline uint32_t forest_side_exact(const float* nv, float off, const f
   return margin > 0.0f ? 1u : 0u;
 }
 
-__global__ void k_forest_init(uint32_t* __restrict__ ids, uint32_t* __restrict__ segv, uint32_t n,
-                              uint32_t T, uint32_t leaf_size) {
+__global__ void k_forest_init(uint32_t* __restrict__ ids, uint32_t n, uint32_t T) {
   size_t pos = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (pos >= (size_t)n * T) return;
-  uint32_t t = (uint32_t)(pos / n), p = (uint32_t)(pos - (size_t)t * n);
-  ids[pos] = p;
-  segv[pos] = n > leaf_size ? t : NONE;
+  uint32_t t = (uint32_t)(pos / n);
+  ids[pos] = (uint32_t)(pos - (size_t)t * n);
 }
 
 // One wave per active node: pivots, normal and offset.
@@ -125,62 +123,94 @@ __global__ __launch_bounds__(256) void k_forest_planes(
   }
 }
 
-// One thread per position of every tree: side bits for the hyperplane (FP16
-// screen, exact f32 margin near the plane) and for the hash fallback.
-__global__ void k_forest_sides(const float* __restrict__ Xp, uint32_t d, uint32_t dp, uint32_t n,
-                               const _Float16* __restrict__ Xh, uint32_t dh,
-                               size_t total, const uint32_t* __restrict__ ids,
-                               const uint32_t* __restrict__ segv, const uint32_t* __restrict__ ntree,
-                               const uint64_t* __restrict__ tseeds, uint32_t level,
-                               const float* __restrict__ normals, const float* __restrict__ offsets,
-                               const float* __restrict__ nabs,
-                               uint32_t* __restrict__ fl, uint32_t* __restrict__ fr) {
-  const size_t pos = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pos >= total) return;
-  const uint32_t v = segv[pos];
-  if (v == NONE) {
-    fl[pos] = 0;
-    fr[pos] = 0;
-    return;
+// ── level passes over the positions of active nodes only ──
+// An active node (start, count, tree) is cut into tiles of FT_TILE positions;
+// toff (exclusive scan of the tile counts) numbers the tiles node after node.
+// Per tile: the side bits of its positions (hyperplane side in fl, hash
+// fallback in fr) as bit masks in position order, and their counts, whose
+// scans over the tiles give every tile's rank base inside its node.
+
+// Tiles per active node.
+__global__ void k_ftile_count(const uint32_t* __restrict__ ncount, uint32_t A,
+                              uint32_t* __restrict__ ntile) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < A) ntile[v] = (ncount[v] + FT_TILE - 1) / FT_TILE;
+}
+// Tile -> node (binary search in toff[0..A]).
+__global__ void k_ftile_map(const uint32_t* __restrict__ toff, uint32_t A, uint32_t NT,
+                            uint32_t* __restrict__ tnode) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= NT) return;
+  uint32_t lo = 0, hi = A;  // toff[lo] <= t < toff[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (toff[mid] <= t) lo = mid; else hi = mid;
   }
-  const uint32_t i = ids[pos];
-  const uint64_t ts = tseeds[ntree[v]];
-  // 2 = undecided, settled by k_forest_sides_exact
-  fl[pos] = forest_side_screen(normals + (size_t)v * dp, offsets[v], nabs[v], Xh + (size_t)i * dh, d);
-  fr[pos] = (tree_hash(ts, level, i, 3) & 1) == 0 ? 1u : 0u;
+  tnode[t] = lo;
 }
 
-// The positions the screen left undecided (side 2): exact f32 margins.
-__global__ void k_forest_sides_exact(const float* __restrict__ Xp, uint32_t d, uint32_t dp,
-                                     size_t total, const uint32_t* __restrict__ ids,
-                                     const uint32_t* __restrict__ segv,
-                                     const uint32_t* __restrict__ ntree,
-                                     const uint64_t* __restrict__ tseeds, uint32_t level,
-                                     const float* __restrict__ normals,
-                                     const float* __restrict__ offsets, uint32_t* __restrict__ fl) {
-  const size_t pos = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pos >= total || fl[pos] != 2u) return;
-  const uint32_t v = segv[pos], i = ids[pos];
-  fl[pos] = forest_side_exact(normals + (size_t)v * dp, offsets[v], Xp + (size_t)i * dp, d,
-                              tseeds[ntree[v]], level, i);
+// One block of 256 per tile, position e = tid + 256 u of the tile (u < 4).
+__global__ __launch_bounds__(256) void k_forest_sides_tile(
+    const float* __restrict__ Xp, uint32_t d, uint32_t dp, uint32_t n,
+    const _Float16* __restrict__ Xh, uint32_t dh, const uint32_t* __restrict__ ids,
+    const uint32_t* __restrict__ nstart, const uint32_t* __restrict__ ncount,
+    const uint32_t* __restrict__ ntree, const uint32_t* __restrict__ toff,
+    const uint32_t* __restrict__ tnode, const uint64_t* __restrict__ tseeds, uint32_t level,
+    const float* __restrict__ normals, const float* __restrict__ offsets,
+    const float* __restrict__ nabs, uint64_t* __restrict__ flmask, uint64_t* __restrict__ frmask,
+    uint32_t* __restrict__ tcl, uint32_t* __restrict__ tcr) {
+  __shared__ uint32_t s_cl[4], s_cr[4];
+  const uint32_t t = blockIdx.x, tid = threadIdx.x, w = tid / WAVE;
+  const uint32_t v = tnode[t];
+  const uint32_t s0 = nstart[v], c = ncount[v], tr = ntree[v];
+  const uint32_t e0 = (t - toff[v]) * FT_TILE;
+  const uint64_t ts = tseeds[tr];
+  const float* nv = normals + (size_t)v * dp;
+  const float off = offsets[v], na = nabs[v];
+  const uint32_t* tids = ids + (size_t)tr * n + s0;
+  uint32_t cl = 0, cr = 0;
+#pragma unroll
+  for (int u = 0; u < FT_TILE / 256; ++u) {
+    const uint32_t e = tid + 256 * u, pin = e0 + e;
+    uint32_t fl = 0, fr = 0;
+    if (pin < c) {
+      const uint32_t i = tids[pin];
+      fl = forest_side_screen(nv, off, na, Xh + (size_t)i * dh, d);
+      if (fl == 2u) fl = forest_side_exact(nv, off, Xp + (size_t)i * dp, d, ts, level, i);
+      fr = (tree_hash(ts, level, i, 3) & 1) == 0 ? 1u : 0u;
+    }
+    const uint64_t ml = __ballot(fl != 0), mr = __ballot(fr != 0);
+    if (lane_id() == 0) {
+      flmask[(size_t)t * (FT_TILE / 64) + e / 64] = ml;
+      frmask[(size_t)t * (FT_TILE / 64) + e / 64] = mr;
+    }
+    cl += (uint32_t)__popcll(ml);
+    cr += (uint32_t)__popcll(mr);
+  }
+  if (lane_id() == 0) {
+    s_cl[w] = cl;
+    s_cr[w] = cr;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    tcl[t] = (s_cl[0] + s_cl[1]) + (s_cl[2] + s_cl[3]);
+    tcr[t] = (s_cr[0] + s_cr[1]) + (s_cr[2] + s_cr[3]);
+  }
 }
 
-// One thread per active node: split mode and how many children it emits.
-// mode 0 = hyperplane, 1 = hash bits, 2 = no split (the node is a leaf).
-// A child is a leaf (at most leaf_size points, or past depth 100), small (at
-// most cap points: finished in LDS by k_forest_finish) or active.
-__global__ void k_forest_nodes(const uint32_t* __restrict__ nstart, const uint32_t* __restrict__ ncount,
-                               const uint32_t* __restrict__ ntree, uint32_t A, uint32_t n,
-                               const uint32_t* __restrict__ sl, const uint32_t* __restrict__ sr,
-                               uint32_t leaf_size, uint32_t cap, uint32_t level,
-                               uint32_t* __restrict__ nl_out, uint32_t* __restrict__ mode_out,
-                               uint32_t* __restrict__ nact, uint32_t* __restrict__ nsmall) {
+// Split mode per active node from the tile scans: mode 0 = hyperplane, 1 =
+// hash bits, 2 = no split (a leaf); and how many active / small children.
+__global__ void k_forest_nodes_tile(const uint32_t* __restrict__ ncount, uint32_t A,
+                                    const uint32_t* __restrict__ toff,
+                                    const uint32_t* __restrict__ scl, const uint32_t* __restrict__ scr,
+                                    uint32_t leaf_size, uint32_t cap, uint32_t level,
+                                    uint32_t* __restrict__ nl_out, uint32_t* __restrict__ mode_out,
+                                    uint32_t* __restrict__ nact, uint32_t* __restrict__ nsmall) {
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= A) return;
-  const size_t g0 = (size_t)ntree[v] * n + nstart[v];
   const uint32_t c = ncount[v];
-  const uint32_t cl = sl[g0 + c] - sl[g0];
-  const uint32_t rl = sr[g0 + c] - sr[g0];
+  const uint32_t cl = scl[toff[v + 1]] - scl[toff[v]];
+  const uint32_t rl = scr[toff[v + 1]] - scr[toff[v]];
   uint32_t mode, nl;
   if (cl != 0 && cl != c) {
     mode = 0;
@@ -203,13 +233,80 @@ __global__ void k_forest_nodes(const uint32_t* __restrict__ nstart, const uint32
   uint32_t a = 0, sm = 0;
   const uint32_t cc[2] = {nl, c - nl};
 #pragma unroll
-  for (int s = 0; s < 2; ++s) {
-    if (child_deep || cc[s] <= leaf_size) continue;
-    if (cc[s] <= cap) ++sm;
+  for (int q = 0; q < 2; ++q) {
+    if (child_deep || cc[q] <= leaf_size) continue;
+    if (cc[q] <= cap) ++sm;
     else ++a;
   }
   nact[v] = a;
   nsmall[v] = sm;
+}
+
+// Stable partition of each tile's positions into tmp (same tree offsets as
+// ids): left members first, in position order. One block of 256 per tile.
+__global__ __launch_bounds__(256) void k_forest_scatter_tile(
+    uint32_t n, const uint32_t* __restrict__ ids, const uint32_t* __restrict__ nstart,
+    const uint32_t* __restrict__ ncount, const uint32_t* __restrict__ ntree,
+    const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tnode,
+    const uint32_t* __restrict__ nl_in, const uint32_t* __restrict__ mode_in,
+    const uint64_t* __restrict__ flmask, const uint64_t* __restrict__ frmask,
+    const uint32_t* __restrict__ scl, const uint32_t* __restrict__ scr, uint32_t* __restrict__ tmp) {
+  constexpr uint32_t W = FT_TILE / 64;
+  __shared__ uint64_t s_m[W];
+  __shared__ uint32_t s_pre[W];
+  const uint32_t t = blockIdx.x, tid = threadIdx.x;
+  const uint32_t v = tnode[t];
+  const uint32_t s0 = nstart[v], c = ncount[v], tr = ntree[v];
+  const uint32_t e0 = (t - toff[v]) * FT_TILE;
+  const uint32_t mode = mode_in[v];
+  const uint32_t* X = mode == 1 ? scr : scl;
+  const uint64_t* M = mode == 1 ? frmask : flmask;
+  if (tid < W) s_m[tid] = M[(size_t)t * W + tid];
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t acc = 0;
+    for (uint32_t q = 0; q < W; ++q) {
+      s_pre[q] = acc;
+      acc += (uint32_t)__popcll(s_m[q]);
+    }
+  }
+  __syncthreads();
+  const uint32_t base = X[t] - X[toff[v]];  // left members of the node before this tile
+  const uint32_t nl = nl_in[v];
+  const uint32_t* tids = ids + (size_t)tr * n + s0;
+  uint32_t* out = tmp + (size_t)tr * n + s0;
+#pragma unroll
+  for (int u = 0; u < FT_TILE / 256; ++u) {
+    const uint32_t e = tid + 256 * u, pin = e0 + e;
+    if (pin >= c) continue;
+    const uint32_t id = tids[pin];
+    if (mode == 2) {
+      out[pin] = id;
+      continue;
+    }
+    const uint64_t word = s_m[e / 64];
+    const uint32_t b = e % 64;
+    const uint32_t rank = base + s_pre[e / 64] + (uint32_t)__popcll(word & ((1ull << b) - 1ull));
+    const bool left = (word >> b) & 1ull;
+    out[left ? rank : nl + (pin - rank)] = id;
+  }
+}
+
+// tmp -> ids for the positions of every tile.
+__global__ __launch_bounds__(256) void k_forest_copyback_tile(
+    uint32_t n, const uint32_t* __restrict__ nstart, const uint32_t* __restrict__ ncount,
+    const uint32_t* __restrict__ ntree, const uint32_t* __restrict__ toff,
+    const uint32_t* __restrict__ tnode, const uint32_t* __restrict__ tmp, uint32_t* __restrict__ ids) {
+  const uint32_t t = blockIdx.x, tid = threadIdx.x;
+  const uint32_t v = tnode[t];
+  const uint32_t s0 = nstart[v], c = ncount[v], tr = ntree[v];
+  const uint32_t e0 = (t - toff[v]) * FT_TILE;
+  const size_t g0 = (size_t)tr * n + s0;
+#pragma unroll
+  for (int u = 0; u < FT_TILE / 256; ++u) {
+    const uint32_t pin = e0 + tid + 256 * u;
+    if (pin < c) ids[g0 + pin] = tmp[g0 + pin];
+  }
 }
 
 // One thread per active node: next-level nodes, small subtrees, leaves (the
@@ -258,46 +355,6 @@ __global__ void k_forest_emit(const uint32_t* __restrict__ nstart, const uint32_
       ++ao;
     }
   }
-}
-
-// One thread per position: stable partition into the ping-pong buffers.
-__global__ void k_forest_scatter(size_t total, uint32_t n, const uint32_t* __restrict__ ids_in,
-                                 const uint32_t* __restrict__ segv_in,
-                                 const uint32_t* __restrict__ nstart, const uint32_t* __restrict__ ntree,
-                                 const uint32_t* __restrict__ nl_in, const uint32_t* __restrict__ mode_in,
-                                 const uint32_t* __restrict__ child, const uint32_t* __restrict__ sl,
-                                 const uint32_t* __restrict__ sr, uint32_t* __restrict__ ids_out,
-                                 uint32_t* __restrict__ segv_out) {
-  size_t pos = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pos >= total) return;
-  const uint32_t v = segv_in[pos];
-  const uint32_t id = ids_in[pos];
-  if (v == NONE) {
-    ids_out[pos] = id;
-    segv_out[pos] = NONE;
-    return;
-  }
-  const uint32_t mode = mode_in[v];
-  if (mode == 2) {
-    ids_out[pos] = id;
-    segv_out[pos] = NONE;
-    return;
-  }
-  const uint32_t* X = mode == 0 ? sl : sr;
-  const size_t g0 = (size_t)ntree[v] * n + nstart[v];
-  const uint32_t rank_left = X[pos] - X[g0];
-  const bool left = (X[pos + 1] - X[pos]) != 0;
-  size_t np;
-  uint32_t ch;
-  if (left) {
-    np = g0 + rank_left;
-    ch = child[2 * v];
-  } else {
-    np = g0 + nl_in[v] + ((uint32_t)(pos - g0) - rank_left);
-    ch = child[2 * v + 1];
-  }
-  ids_out[np] = id;
-  segv_out[np] = ch;
 }
 
 // dot_ref (simd.rs:22-37 order) of an LDS normal with a global row whose

@@ -39,26 +39,13 @@ __global__ void k_perm_rows(const float* X, uint32_t dp, const uint32_t* perm, u
 __global__ void k_renumber(uint32_t* ids, size_t total, const uint32_t* inv);
 
 // RP forest (forest_kernels.hip)
-__global__ void k_forest_init(uint32_t* ids, uint32_t* segv, uint32_t n, uint32_t T, uint32_t leaf_size);
+__global__ void k_forest_init(uint32_t* ids, uint32_t n, uint32_t T);
 __global__ void k_forest_planes(const float* Xp, uint32_t d, uint32_t dp, uint32_t n, const uint32_t* ids,
                                 const uint32_t* nstart, const uint32_t* ncount, const uint32_t* ntree,
                                 uint32_t A, const uint64_t* tseeds, uint32_t level, float* normals,
                                 float* offsets, float* nabs);
 __global__ void k_rows_to_half(const float* X, uint32_t d, uint32_t dp, uint32_t n, _Float16* Xh,
                                uint32_t dh);
-__global__ void k_forest_sides(const float* Xp, uint32_t d, uint32_t dp, uint32_t n,
-                               const _Float16* Xh, uint32_t dh, size_t total, const uint32_t* ids,
-                               const uint32_t* segv, const uint32_t* ntree, const uint64_t* tseeds,
-                               uint32_t level, const float* normals, const float* offsets,
-                               const float* nabs, uint32_t* fl, uint32_t* fr);
-__global__ void k_forest_sides_exact(const float* Xp, uint32_t d, uint32_t dp, size_t total,
-                                     const uint32_t* ids, const uint32_t* segv,
-                                     const uint32_t* ntree, const uint64_t* tseeds, uint32_t level,
-                                     const float* normals, const float* offsets, uint32_t* fl);
-__global__ void k_forest_nodes(const uint32_t* nstart, const uint32_t* ncount, const uint32_t* ntree,
-                               uint32_t A, uint32_t n, const uint32_t* sl, const uint32_t* sr,
-                               uint32_t leaf_size, uint32_t cap, uint32_t level, uint32_t* nl_out,
-                               uint32_t* mode_out, uint32_t* nact, uint32_t* nsmall);
 __global__ void k_forest_emit(const uint32_t* nstart, const uint32_t* ncount, const uint32_t* ntree,
                               uint32_t A, uint32_t n, const uint32_t* nl_in, const uint32_t* mode_in,
                               const uint32_t* act_off, const uint32_t* small_off, uint32_t leaf_size,
@@ -66,15 +53,34 @@ __global__ void k_forest_emit(const uint32_t* nstart, const uint32_t* ncount, co
                               uint32_t* x_tree, uint32_t* child, uint32_t* lcnt_pos,
                               uint32_t small_base, uint32_t* sm_start, uint32_t* sm_count,
                               uint32_t* sm_tree, uint32_t* sm_level);
-__global__ void k_forest_scatter(size_t total, uint32_t n, const uint32_t* ids_in, const uint32_t* segv_in,
-                                 const uint32_t* nstart, const uint32_t* ntree, const uint32_t* nl_in,
-                                 const uint32_t* mode_in, const uint32_t* child, const uint32_t* sl,
-                                 const uint32_t* sr, uint32_t* ids_out, uint32_t* segv_out);
 __global__ void k_forest_finish(const float* Xs, uint32_t ds, uint32_t dps, uint32_t RSs, uint32_t n,
                                 uint32_t* ids, const uint32_t* sm_start, const uint32_t* sm_count,
                                 const uint32_t* sm_tree, const uint32_t* sm_level,
                                 const uint64_t* tseeds, uint32_t leaf_size, uint32_t C, uint32_t NC,
                                 uint32_t* lcnt_pos);
+constexpr uint32_t FT_TILE = 1024;  // positions per tile of the level passes
+__global__ void k_ftile_count(const uint32_t* ncount, uint32_t A, uint32_t* ntile);
+__global__ void k_ftile_map(const uint32_t* toff, uint32_t A, uint32_t NT, uint32_t* tnode);
+__global__ void k_forest_sides_tile(const float* Xp, uint32_t d, uint32_t dp, uint32_t n,
+                                    const _Float16* Xh, uint32_t dh, const uint32_t* ids,
+                                    const uint32_t* nstart, const uint32_t* ncount,
+                                    const uint32_t* ntree, const uint32_t* toff, const uint32_t* tnode,
+                                    const uint64_t* tseeds, uint32_t level, const float* normals,
+                                    const float* offsets, const float* nabs, uint64_t* flmask,
+                                    uint64_t* frmask, uint32_t* tcl, uint32_t* tcr);
+__global__ void k_forest_nodes_tile(const uint32_t* ncount, uint32_t A, const uint32_t* toff,
+                                    const uint32_t* scl, const uint32_t* scr, uint32_t leaf_size,
+                                    uint32_t cap, uint32_t level, uint32_t* nl_out, uint32_t* mode_out,
+                                    uint32_t* nact, uint32_t* nsmall);
+__global__ void k_forest_scatter_tile(uint32_t n, const uint32_t* ids, const uint32_t* nstart,
+                                      const uint32_t* ncount, const uint32_t* ntree,
+                                      const uint32_t* toff, const uint32_t* tnode,
+                                      const uint32_t* nl_in, const uint32_t* mode_in,
+                                      const uint64_t* flmask, const uint64_t* frmask,
+                                      const uint32_t* scl, const uint32_t* scr, uint32_t* tmp);
+__global__ void k_forest_copyback_tile(uint32_t n, const uint32_t* nstart, const uint32_t* ncount,
+                                       const uint32_t* ntree, const uint32_t* toff,
+                                       const uint32_t* tnode, const uint32_t* tmp, uint32_t* ids);
 __global__ void k_leaf_flags(const uint32_t* lcnt_pos, size_t total, uint32_t* flag);
 __global__ void k_leaf_compact(const uint32_t* lcnt_pos, size_t total, uint32_t n, const uint32_t* off,
                                uint32_t* g_start, uint32_t* g_count);
