@@ -219,7 +219,11 @@ def main() -> int:
             f"cells/s; iterations {st['iterations']}, ms init {st['ms_init']:.1f} iter "
             f"{st['ms_iterations']:.1f} extract {st['ms_extract']:.1f}, changed "
             f"{st['changed'][:st['iterations']]}, join ms "
-            f"{[round(v, 2) for v in st['ms_join'][:st['iterations']]]}")
+            f"{[round(v, 2) for v in st['ms_join'][:st['iterations']]]}, per cell: pairs "
+            f"{[round(v / n, 1) for v in st['pairs'][:st['iterations']]]} exact "
+            f"{[round(v / n, 1) for v in st['exact_pairs'][:st['iterations']]]} proposals "
+            f"{[round(v / n, 1) for v in st['proposals'][:st['iterations']]]} rows "
+            f"{[round(v / n, 1) for v in st['cand_rows'][:st['iterations']]]}")
         results[form] = {"elapsed": elapsed, "stats": stats, "out": out,
                          "train": list(train_log) if form == "learned_bf16" else None}
 

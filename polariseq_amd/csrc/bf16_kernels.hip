@@ -58,17 +58,18 @@ __device__ inline void load8(const float* p, uint32_t t0, uint32_t rs, float (&o
 
 // One wave (one block) per cell. LDS: pb_key[PB_CAP] u64 | rows[R + 1][RS] f32
 // (row R is x_i) | ids[R] | thr[R] | nrm[R] | pb_row[PB_CAP] | sv[320] (screened
-// pairs: < 64 held between tiles + 256 per tile; the host sizes it),
-// R = 2m rounded up to 16. d <= 64 (two MFMA steps of depth 32).
+// pairs: < 64 held between tiles + 256 per tile; the host sizes it). A block
+// holds R candidate rows (a multiple of 16); a cell with more is left to a
+// launch with more (jc). d <= 64 (two MFMA steps of depth 32).
 template <int D>
 __global__ __launch_bounds__(64) void k_join_bf16(
     const float* __restrict__ Xp, uint32_t n, uint32_t d, uint32_t dp, uint32_t RS,
     const uint32_t* __restrict__ newl, const uint32_t* __restrict__ oldl,
-    const uint32_t* __restrict__ ncnt, uint32_t m, const float* __restrict__ thr, float slack,
-    uint64_t* __restrict__ slots, uint8_t* __restrict__ touched, uint32_t S, uint64_t seed,
-    uint64_t iter, unsigned long long* __restrict__ counters) {
+    const uint32_t* __restrict__ ncnt, uint32_t m, uint32_t R, JoinCells jc,
+    const float* __restrict__ thr, float slack, uint64_t* __restrict__ slots,
+    uint8_t* __restrict__ touched, uint32_t S, uint64_t seed, uint64_t iter,
+    unsigned long long* __restrict__ counters) {
   extern __shared__ __attribute__((aligned(16))) uint64_t lds64[];
-  const uint32_t R = (2 * m + 15) / 16 * 16;
   uint64_t* pb_key = lds64;
   float* rows = reinterpret_cast<float*>(lds64 + PB_CAP);
   float* xi = rows + (size_t)R * RS;
@@ -78,143 +79,135 @@ __global__ __launch_bounds__(64) void k_join_bf16(
   uint32_t* pb_row = reinterpret_cast<uint32_t*>(s_nrm + R);
   uint32_t* sv = pb_row + PB_CAP;
   const uint32_t l = (uint32_t)lane_id();
-  const uint32_t i = xcd_row();
-  if (i >= n) return;  // block-uniform
-  const uint32_t cnt = ncnt[i];
-  const uint32_t nn = cnt & 0xFFFF, no = cnt >> 16, nc = nn + no;
-  if (nn == 0) return;
   const uint32_t bmask = S / BUCKET - 1;
-  for (uint32_t s = l; s < nc; s += WAVE) {
-    const uint32_t v = s < nn ? newl[(size_t)i * m + s] : oldl[(size_t)i * m + (s - nn)];
-    s_id[s] = v;
-    s_thr[s] = thr[v];
-  }
-  __syncthreads();
-  const uint32_t c4 = D > 0 ? (uint32_t)(D + 3) / 4 : (d + 3) / 4;  // a constant divisor for D > 0
-  for (uint32_t e = l; e < (nc + 1) * c4; e += WAVE) {
-    const uint32_t s = e / c4, c = e - s * c4;
-    const uint32_t src = s < nc ? s_id[s] : i;
-    const float4 v = *reinterpret_cast<const float4*>(Xp + (size_t)src * dp + 4 * c);
-    float* dst = s < nc ? rows + (size_t)s * RS : xi;
-    *reinterpret_cast<float4*>(dst + 4 * c) = v;
-  }
-  __syncthreads();
-  for (uint32_t s = l; s < nc; s += WAVE) {
-    const float* row = rows + (size_t)s * RS;
-    float nrm = 0.0f;
-    for (uint32_t t = 0; t < d; t += 4) {
-      const float4 a = *reinterpret_cast<const float4*>(row + t);
-      const float4 x = *reinterpret_cast<const float4*>(xi + t);
-      const float av[4] = {a.x, a.y, a.z, a.w}, xv[4] = {x.x, x.y, x.z, x.w};
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (t + u < d) {
-          const float b = (float)(__bf16)(av[u] - xv[u]);
-          nrm = fmaf(b, b, nrm);
-        }
-    }
-    s_nrm[s] = nrm;
-  }
-  __syncthreads();
-
-  uint32_t pcnt = 0, scnt = 0;
-  uint64_t pairs = 0, exact = 0, sent = 0;
-  // Exact distances of the listed pairs, 64 at a time (all of them when `all`).
-  auto drain = [&](bool all) {
-    if (scnt < (all ? 1u : (uint32_t)WAVE)) return;
-    __syncthreads();
-    uint32_t done = 0;
-    while (done < scnt && (all || done + WAVE <= scnt)) {
-      const uint32_t e = done + l;
-      bool pa = false, pb = false;
-      uint32_t ia = 0, ib = 0;
-      float dist = 0.0f;
-      if (e < scnt) {
-        const uint32_t pr = sv[e], a = pr & 0xFFFF, b = pr >> 16;
-        ia = s_id[a];
-        ib = s_id[b];
-        dist = lds_sq_dist<D>(rows + (size_t)a * RS, rows + (size_t)b * RS, d);
-        pa = dist < s_thr[a];
-        pb = dist < s_thr[b];
-        ++exact;
-      }
-      pb_append(pb_row, pb_key, pcnt, pa, ia, make_key(dist, ib, 1), sent);
-      pb_append(pb_row, pb_key, pcnt, pb, ib, make_key(dist, ia, 1), sent);
-      pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, bmask, seed, iter);
-      done += WAVE;
-    }
-    if (all) {
-      scnt = 0;
+  uint64_t pairs = 0, exact = 0, sent = 0, rows_in = 0;
+  uint32_t i;
+  if (!join_cell_of_block(jc, n, i)) return;
+  [&] {
+    uint32_t nn, no;
+    if (!stage_cell<D>(Xp, d, dp, RS, newl, oldl, ncnt, m, thr, i, R, s_id, s_thr, rows,
+                       xi, nn, no))
       return;
-    }
-    const uint32_t rest = scnt - done;  // < 64
-    const uint32_t keep = l < rest ? sv[done + l] : 0u;
+    const uint32_t nc = nn + no;
+    rows_in += nc;
     __syncthreads();
-    if (l < rest) sv[l] = keep;
-    __syncthreads();
-    scnt = rest;
-  };
-
-  const uint32_t rtiles = (nn + 15) / 16, ctiles = (nc + 15) / 16;
-  const uint32_t lr = l % 16, lk = (l / 16) * 8;
-  // this lane's dimensions of x_i for both MFMA steps, read once
-  float xr0[8], xr1[8];
-  load8(xi, lk, RS, xr0);
-  load8(xi, 32 + lk, RS, xr1);
-  for (uint32_t rt = 0; rt < rtiles; ++rt) {
-    const float* arow = rows + (size_t)(rt * 16 + lr) * RS;
-    const bf16x8 a0 = centred_bf16x8_r(arow, xr0, lk, d, RS);
-    const bf16x8 a1 = centred_bf16x8_r(arow, xr1, 32 + lk, d, RS);
-    // the 4 rows of the tile this lane's outputs belong to: id, norm, slack * threshold
-    uint32_t rid[4];
-    float rnrm[4], rthr[4];
+    for (uint32_t s = l; s < nc; s += WAVE) {
+      const float* row = rows + (size_t)s * RS;
+      float nrm = 0.0f;
+      for (uint32_t t = 0; t < d; t += 4) {
+        const float4 a = *reinterpret_cast<const float4*>(row + t);
+        const float4 x = *reinterpret_cast<const float4*>(xi + t);
+        const float av[4] = {a.x, a.y, a.z, a.w}, xv[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const uint32_t r = min(rt * 16 + (l / 16) * 4 + v, R - 1);
-      rid[v] = s_id[r];
-      rnrm[v] = s_nrm[r];
-      rthr[v] = slack * s_thr[r];
+        for (int u = 0; u < 4; ++u)
+          if (t + u < d) {
+            const float b = (float)(__bf16)(av[u] - xv[u]);
+            nrm = fmaf(b, b, nrm);
+          }
+      }
+      s_nrm[s] = nrm;
     }
-    for (uint32_t ct = 0; ct < ctiles; ++ct) {
-      // skip tiles entirely below the new x new diagonal
-      if (ct * 16 + 15 < rt * 16 && ct * 16 + 15 < nn) continue;
-      const float* brow = rows + (size_t)(ct * 16 + lr) * RS;
-      f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, centred_bf16x8_r(brow, xr0, lk, d, RS), acc, 0, 0,
-                                                    0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, centred_bf16x8_r(brow, xr1, 32 + lk, d, RS), acc,
-                                                    0, 0, 0);
-      const uint32_t c = ct * 16 + lr;
-      const uint32_t cid = s_id[c];
-      const float cnrm = s_nrm[c], cthr = slack * s_thr[c];
+    __syncthreads();
+
+    uint32_t pcnt = 0, scnt = 0;
+    // Exact distances of the listed pairs, 64 at a time (all of them when `all`).
+    auto drain = [&](bool all) {
+      if (scnt < (all ? 1u : (uint32_t)WAVE)) return;
+      __syncthreads();
+      uint32_t done = 0;
+      while (done < scnt && (all || done + WAVE <= scnt)) {
+        const uint32_t e = done + l;
+        bool pa = false, pb = false;
+        uint32_t ia = 0, ib = 0;
+        float dist = 0.0f;
+        if (e < scnt) {
+          const uint32_t pr = sv[e], a = pr & 0xFFFF, b = pr >> 16;
+          ia = s_id[a];
+          ib = s_id[b];
+          dist = lds_sq_dist<D>(rows + (size_t)a * RS, rows + (size_t)b * RS, d);
+          pa = dist < s_thr[a];
+          pb = dist < s_thr[b];
+          ++exact;
+        }
+        pb_append(pb_row, pb_key, pcnt, pa, ia, make_key(dist, ib, 1), sent);
+        pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, bmask, seed, iter);
+        pb_append(pb_row, pb_key, pcnt, pb, ib, make_key(dist, ia, 1), sent);
+        pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, bmask, seed, iter);
+        done += WAVE;
+      }
+      if (all) {
+        scnt = 0;
+        return;
+      }
+      const uint32_t rest = scnt - done;  // < 64
+      const uint32_t keep = l < rest ? sv[done + l] : 0u;
+      __syncthreads();
+      if (l < rest) sv[l] = keep;
+      __syncthreads();
+      scnt = rest;
+    };
+
+    const uint32_t rtiles = (nn + 15) / 16, ctiles = (nc + 15) / 16;
+    const uint32_t lr = l % 16, lk = (l / 16) * 8;
+    // this lane's dimensions of x_i for both MFMA steps, read once
+    float xr0[8], xr1[8];
+    load8(xi, lk, RS, xr0);
+    load8(xi, 32 + lk, RS, xr1);
+    for (uint32_t rt = 0; rt < rtiles; ++rt) {
+      const float* arow = rows + (size_t)(rt * 16 + lr) * RS;
+      const bf16x8 a0 = centred_bf16x8_r(arow, xr0, lk, d, RS);
+      const bf16x8 a1 = centred_bf16x8_r(arow, xr1, 32 + lk, d, RS);
+      // the 4 rows of the tile this lane's outputs belong to: id, norm, slack * threshold
+      uint32_t rid[4];
+      float rnrm[4], rthr[4];
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const uint32_t r = rt * 16 + (l / 16) * 4 + v;
-        bool keep = false;
-        if (r < nn && c < nc && ((c < nn) ? (c > r) : (cid != rid[v]))) {
-          ++pairs;
-          float dt = rnrm[v] + cnrm - 2.0f * acc[v];
-          dt = dt > 0.0f ? dt : 0.0f;
-          keep = dt < rthr[v] || dt < cthr;
-        }
-        const uint64_t mask = __ballot(keep);
-        if (keep) sv[scnt + (uint32_t)__popcll(mask & ((1ull << l) - 1ull))] = r | (c << 16);
-        scnt += (uint32_t)__popcll(mask);
+        const uint32_t r = min(rt * 16 + (l / 16) * 4 + v, R - 1);
+        rid[v] = s_id[r];
+        rnrm[v] = s_nrm[r];
+        rthr[v] = slack * s_thr[r];
       }
-      drain(false);
+      for (uint32_t ct = 0; ct < ctiles; ++ct) {
+        // skip tiles entirely below the new x new diagonal
+        if (ct * 16 + 15 < rt * 16 && ct * 16 + 15 < nn) continue;
+        const float* brow = rows + (size_t)(ct * 16 + lr) * RS;
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, centred_bf16x8_r(brow, xr0, lk, d, RS), acc, 0,
+                                                      0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, centred_bf16x8_r(brow, xr1, 32 + lk, d, RS),
+                                                      acc, 0, 0, 0);
+        const uint32_t c = ct * 16 + lr;
+        const uint32_t cid = s_id[c];
+        const float cnrm = s_nrm[c], cthr = slack * s_thr[c];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const uint32_t r = rt * 16 + (l / 16) * 4 + v;
+          bool keep = false;
+          if (r < nn && c < nc && ((c < nn) ? (c > r) : (cid != rid[v]))) {
+            ++pairs;
+            float dt = rnrm[v] + cnrm - 2.0f * acc[v];
+            dt = dt > 0.0f ? dt : 0.0f;
+            keep = dt < rthr[v] || dt < cthr;
+          }
+          const uint64_t mask = __ballot(keep);
+          if (keep) sv[scnt + (uint32_t)__popcll(mask & ((1ull << l) - 1ull))] = r | (c << 16);
+          scnt += (uint32_t)__popcll(mask);
+        }
+        drain(false);
+      }
     }
-  }
-  drain(true);
-  pb_flush(pb_row, pb_key, pcnt, true, slots, touched, S, bmask, seed, iter);
+    drain(true);
+    pb_flush(pb_row, pb_key, pcnt, true, slots, touched, S, bmask, seed, iter);
+  }();
   uint64_t tp = pairs, te = exact;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     tp += shfl_xor64(tp, o);
     te += shfl_xor64(te, o);
   }
-  if (l == 0) {
+  if (l == 0 && rows_in > 0) {
     ctr_add(counters, 0, (unsigned long long)tp);
-    ctr_add(counters, 1, (unsigned long long)nc);
+    ctr_add(counters, 1, (unsigned long long)rows_in);
     ctr_add(counters, 3, (unsigned long long)te);
     ctr_add(counters, 4, (unsigned long long)sent);
   }
@@ -222,11 +215,11 @@ __global__ __launch_bounds__(64) void k_join_bf16(
 
 template __global__ void k_join_bf16<0>(const float*, uint32_t, uint32_t, uint32_t, uint32_t,
                                         const uint32_t*, const uint32_t*, const uint32_t*, uint32_t,
-                                        const float*, float, uint64_t*, uint8_t*, uint32_t, uint64_t,
-                                        uint64_t, unsigned long long*);
+                                        uint32_t, JoinCells, const float*, float, uint64_t*, uint8_t*,
+                                        uint32_t, uint64_t, uint64_t, unsigned long long*);
 template __global__ void k_join_bf16<50>(const float*, uint32_t, uint32_t, uint32_t, uint32_t,
                                          const uint32_t*, const uint32_t*, const uint32_t*, uint32_t,
-                                         const float*, float, uint64_t*, uint8_t*, uint32_t, uint64_t,
-                                         uint64_t, unsigned long long*);
+                                         uint32_t, JoinCells, const float*, float, uint64_t*, uint8_t*,
+                                         uint32_t, uint64_t, uint64_t, unsigned long long*);
 
 }  // namespace psq

@@ -375,22 +375,71 @@ int launch_merge(const Resolved& R, uint64_t* G, uint64_t* slots, uint8_t* touch
 // number of 16-byte units so that consecutive rows start on different banks.
 uint32_t join_row_stride(uint64_t d) { return (uint32_t)(round_up((uint32_t)d, 4) | 4u); }
 
-int launch_join(const Resolved& R, const float* Xp, const uint32_t* newl, const uint32_t* oldl,
-                const uint32_t* ncnt, const float* thr, uint64_t* slots, uint8_t* touched,
-                uint64_t iter, unsigned long long* counters, hipStream_t s) {
-  const uint32_t RS = join_row_stride(R.d);
-  const size_t smem = (size_t)PB_CAP * 12 + (size_t)2 * R.m * (RS * 4 + 8);
-  void (*kern)(const float*, uint32_t, uint32_t, uint32_t, uint32_t, const uint32_t*,
-               const uint32_t*, const uint32_t*, uint32_t, const float*, uint64_t*, uint8_t*,
-               uint32_t, uint64_t, uint64_t, unsigned long long*) =
-      R.d == 50 ? k_join<50> : k_join<0>;
-  if (smem > 65536)
-    HIPC(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-  hipLaunchKernelGGL(kern, dim3(xcd_grid(R.n)), dim3(64), smem, s, Xp, (uint32_t)R.n, (uint32_t)R.d,
-                     R.dp, RS, newl, oldl, ncnt, (uint32_t)R.m, thr, slots, touched, (uint32_t)R.S,
-                     R.seed, iter, counters);
+// Rows a join block holds in the launch over every cell. Fewer rows than 2m
+// mean less LDS per block and more resident blocks per CU: the join waits on
+// memory, and its time per cell falls with the blocks in flight. The cells with
+// more candidates are deferred to a second launch with room for 2m rows.
+constexpr uint32_t JOIN_SMALL_ROWS = 32;
+
+struct JoinArgs {
+  const float* Xp;
+  const uint32_t *newl, *oldl, *ncnt;
+  const float* thr;
+  uint64_t* slots;
+  uint8_t* touched;
+  uint32_t* flag;   // n: cells the first launch leaves out
+  uint32_t* pos;    // n + 1: their exclusive scan
+  uint32_t* list;   // n: those cells in order
+  uint32_t* hcount; // pinned host word: their number
+  hipEvent_t counted;
+  unsigned long long* ctr;
+  bool bf16;
+};
+
+size_t join_smem(bool bf16, uint32_t rows, uint32_t RS) {
+  return bf16 ? (size_t)PB_CAP * 12 + (size_t)(rows + 1) * RS * 4 + (size_t)rows * 12 +
+                    (size_t)BF16_SV_CAP * 4
+              : (size_t)PB_CAP * 12 + (size_t)rows * (RS * 4 + 8);
+}
+
+int launch_join(Workspace* ws, const Resolved& R, const JoinArgs& a, uint64_t iter, hipStream_t s) {
+  const uint32_t n = (uint32_t)R.n, m = (uint32_t)R.m, RS = join_row_stride(R.d);
+  const uint32_t full = a.bf16 ? (2 * m + 15) / 16 * 16 : 2 * m;
+  auto bk = R.d == 50 ? k_join_bf16<50> : k_join_bf16<0>;
+  auto pk = R.d == 50 ? k_join<50> : k_join<0>;
+  const void* kern = a.bf16 ? (const void*)bk : (const void*)pk;
+  auto launch = [&](uint32_t rows, JoinCells jc, uint32_t grid) -> int {
+    const size_t smem = join_smem(a.bf16, rows, RS);
+    if (smem > 65536)
+      HIPC(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    if (a.bf16)
+      hipLaunchKernelGGL(bk, dim3(grid), dim3(64), smem, s, a.Xp, n, (uint32_t)R.d, R.dp, RS, a.newl,
+                         a.oldl, a.ncnt, m, rows, jc, a.thr, BF16_SLACK, a.slots, a.touched,
+                         (uint32_t)R.S, R.seed, iter, a.ctr);
+    else
+      hipLaunchKernelGGL(pk, dim3(grid), dim3(64), smem, s, a.Xp, n, (uint32_t)R.d, R.dp, RS, a.newl,
+                         a.oldl, a.ncnt, m, rows, jc, a.thr, a.slots, a.touched, (uint32_t)R.S,
+                         R.seed, iter, a.ctr);
+    HIPC(hipGetLastError());
+    return PSQ_OK;
+  };
+  if (full <= JOIN_SMALL_ROWS) return launch(full, JoinCells{nullptr, 0}, xcd_grid(n));
+  // the cells with more than JOIN_SMALL_ROWS candidates, listed in cell order
+  hipLaunchKernelGGL(k_join_defer_flags, dim3(grid_for(n, 256)), dim3(256), 0, s, a.ncnt, n,
+                     JOIN_SMALL_ROWS, a.flag);
   HIPC(hipGetLastError());
-  return PSQ_OK;
+  TRY(scan_u32(ws, a.flag, a.pos, n, s));
+  hipLaunchKernelGGL(k_join_defer_list, dim3(grid_for(n, 256)), dim3(256), 0, s, a.flag, a.pos, n,
+                     a.list);
+  HIPC(hipGetLastError());
+  HIPC(hipMemcpyAsync(a.hcount, a.pos + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIPC(hipEventRecord(a.counted, s));
+  TRY(launch(JOIN_SMALL_ROWS, JoinCells{nullptr, 0}, xcd_grid(n)));
+  // the count is read while the first launch runs
+  HIPC(hipEventSynchronize(a.counted));
+  const uint32_t cnt = *a.hcount;
+  if (cnt == 0) return PSQ_OK;
+  return launch(full, JoinCells{a.list, cnt}, xcd_grid(cnt));
 }
 
 double ms_between(hipEvent_t a, hipEvent_t b) {
@@ -704,7 +753,7 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   uint64_t *G, *slots, *rev;
   float* thr;
   uint8_t* touched;
-  uint32_t *roff, *newl, *oldl, *ncnt, *rowcnt;
+  uint32_t *roff, *newl, *oldl, *ncnt, *rowcnt, *jflag, *jpos, *jlist;
   unsigned long long* ctr;
   TRY(wbuf(ws, "Xp", (size_t)n * R.dp, &Xp));
   TRY(wbuf(ws, "G", (size_t)n * k, &G));
@@ -716,6 +765,9 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   TRY(wbuf(ws, "newl", (size_t)n * m, &newl));
   TRY(wbuf(ws, "oldl", (size_t)n * m, &oldl));
   TRY(wbuf(ws, "ncnt", n, &ncnt));
+  TRY(wbuf(ws, "jflag", n, &jflag));
+  TRY(wbuf(ws, "jpos", (size_t)n + 1, &jpos));
+  TRY(wbuf(ws, "jlist", n, &jlist));
   TRY(wbuf(ws, "rowcnt", n, &rowcnt));
   TRY(wbuf(ws, "counters", (size_t)(CTR_STRIPES + 1) * CTR_WORDS, &ctr));
   const PartShape RP = part_shape(n, 8);
@@ -787,21 +839,11 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
     TRY(evp.make(&a));
     TRY(evp.make(&b));
     HIPC(hipEventRecord(a, s));
-    if (bf16) {
-      const uint32_t RS = join_row_stride(R.d);
-      const uint32_t Rr = (2 * m + 15) / 16 * 16;
-      const size_t smem = (size_t)PB_CAP * 12 + (size_t)(Rr + 1) * RS * 4 + (size_t)Rr * 12 +
-                          (size_t)BF16_SV_CAP * 4;
-      auto kern = R.d == 50 ? k_join_bf16<50> : k_join_bf16<0>;
-      if (smem > 65536)
-        HIPC(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)smem));
-      hipLaunchKernelGGL(kern, dim3(xcd_grid(n)), dim3(64), smem, s, Xp, n, (uint32_t)R.d, R.dp, RS, newl,
-                         oldl, ncnt, m, thr, BF16_SLACK, slots, touched, S, R.seed, it, ctr);
-      HIPC(hipGetLastError());
-    } else {
-      TRY(launch_join(R, Xp, newl, oldl, ncnt, thr, slots, touched, it, ctr, s));
-    }
+    hipEvent_t counted;
+    TRY(evp.make(&counted));
+    const JoinArgs ja{Xp,    newl,  oldl,  ncnt,  thr,     slots, touched, jflag, jpos,
+                      jlist, static_cast<uint32_t*>(hp) + 960, counted,   ctr,   bf16};
+    TRY(launch_join(ws, R, ja, it, s));
     HIPC(hipEventRecord(b, s));
     jb.push_back(a);
     je.push_back(b);

@@ -305,75 +305,85 @@ __global__ __launch_bounds__(64) void k_sample_select(
 // whose distance beats an endpoint's threshold (snapshot of that row's worst)
 // proposes the other endpoint to it; proposals are batched through the wave's
 // LDS list. D = 50 fixes the dimension at compile time; D = 0 reads it at run
-// time. Both use the reference's operation order (simd.rs:45-62).
+// time. Both use the reference's operation order (simd.rs:45-62). A block holds
+// R candidate rows; a cell with more is left to a launch with more (jc).
 template <int D>
 __global__ __launch_bounds__(64) void k_join(
     const float* __restrict__ Xp, uint32_t n, uint32_t d, uint32_t dp, uint32_t RS,
     const uint32_t* __restrict__ newl, const uint32_t* __restrict__ oldl,
-    const uint32_t* __restrict__ ncnt, uint32_t m, const float* __restrict__ thr,
-    uint64_t* __restrict__ slots, uint8_t* __restrict__ touched, uint32_t S, uint64_t seed,
-    uint64_t iter, unsigned long long* __restrict__ counters) {
-  // LDS: pb_key[PB_CAP] u64 | rows[2m][RS] f32 | ids[2m] | thr[2m] | pb_row[PB_CAP]
+    const uint32_t* __restrict__ ncnt, uint32_t m, uint32_t R, JoinCells jc,
+    const float* __restrict__ thr, uint64_t* __restrict__ slots, uint8_t* __restrict__ touched,
+    uint32_t S, uint64_t seed, uint64_t iter, unsigned long long* __restrict__ counters) {
+  // LDS: pb_key[PB_CAP] u64 | rows[R][RS] f32 | ids[R] | thr[R] | pb_row[PB_CAP]
   extern __shared__ __attribute__((aligned(16))) float lds[];
   uint64_t* pb_key = reinterpret_cast<uint64_t*>(lds);
   float* rows = lds + 2 * PB_CAP;
-  uint32_t* s_id = reinterpret_cast<uint32_t*>(rows + (size_t)2 * m * RS);
-  float* s_thr = reinterpret_cast<float*>(s_id + 2 * m);
-  uint32_t* pb_row = reinterpret_cast<uint32_t*>(s_thr + 2 * m);
+  uint32_t* s_id = reinterpret_cast<uint32_t*>(rows + (size_t)R * RS);
+  float* s_thr = reinterpret_cast<float*>(s_id + R);
+  uint32_t* pb_row = reinterpret_cast<uint32_t*>(s_thr + R);
   const uint32_t l = (uint32_t)lane_id();
-  const uint32_t i = xcd_row();
-  if (i >= n) return;  // block-uniform
-  const uint32_t cnt = ncnt[i];
-  const uint32_t nn = cnt & 0xFFFF, no = cnt >> 16, nc = nn + no;
-  if (nn == 0) return;
   const uint32_t bmask = S / BUCKET - 1;
-  for (uint32_t s = l; s < nc; s += WAVE) {
-    const uint32_t v = s < nn ? newl[(size_t)i * m + s] : oldl[(size_t)i * m + (s - nn)];
-    s_id[s] = v;
-    s_thr[s] = thr[v];
-  }
-  __syncthreads();
-  const uint32_t c4 = D > 0 ? (uint32_t)((D + 3) / 4) : (d + 3) / 4;
-  for (uint32_t e = l; e < nc * c4; e += WAVE) {
-    const uint32_t s = e / c4, c = e - s * c4;
-    const float4 v = *reinterpret_cast<const float4*>(Xp + (size_t)s_id[s] * dp + 4 * c);
-    *reinterpret_cast<float4*>(rows + (size_t)s * RS + 4 * c) = v;
-  }
-  __syncthreads();
-  const uint32_t P1 = nn * (nn - 1) / 2, P = P1 + nn * no;
-  uint32_t pcnt = 0;
-  uint64_t pairs = 0, sent = 0;
-  for (uint32_t base = 0; base < P; base += WAVE) {
-    const uint32_t t = base + l;
-    bool pa = false, pb = false;
-    uint32_t ia = 0, ib = 0;
-    float dist = 0.0f;
-    if (t < P) {
-      uint32_t a, b;
-      pair_decode(t, nn, no, P1, a, b);
-      ia = s_id[a];
-      ib = s_id[b];
-      if (ia != ib) {
-        ++pairs;
-        dist = lds_sq_dist<D>(rows + (size_t)a * RS, rows + (size_t)b * RS, d);
-        pa = dist < s_thr[a];
-        pb = dist < s_thr[b];
+  uint64_t pairs = 0, sent = 0, rows_in = 0;
+  uint32_t i;
+  if (!join_cell_of_block(jc, n, i)) return;
+  [&] {
+    uint32_t nn, no;
+    if (!stage_cell<D>(Xp, d, dp, RS, newl, oldl, ncnt, m, thr, i, R, s_id, s_thr, rows,
+                       nullptr, nn, no))
+      return;
+    rows_in += nn + no;
+    __syncthreads();
+    const uint32_t P1 = nn * (nn - 1) / 2, P = P1 + nn * no;
+    uint32_t pcnt = 0;
+    for (uint32_t base = 0; base < P; base += WAVE) {
+      const uint32_t t = base + l;
+      bool pa = false, pb = false;
+      uint32_t ia = 0, ib = 0;
+      float dist = 0.0f;
+      if (t < P) {
+        uint32_t a, b;
+        pair_decode(t, nn, no, P1, a, b);
+        ia = s_id[a];
+        ib = s_id[b];
+        if (ia != ib) {
+          ++pairs;
+          dist = lds_sq_dist<D>(rows + (size_t)a * RS, rows + (size_t)b * RS, d);
+          pa = dist < s_thr[a];
+          pb = dist < s_thr[b];
+        }
       }
+      pb_append(pb_row, pb_key, pcnt, pa, ia, make_key(dist, ib, 1), sent);
+      pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, bmask, seed, iter);
+      pb_append(pb_row, pb_key, pcnt, pb, ib, make_key(dist, ia, 1), sent);
+      pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, bmask, seed, iter);
     }
-    pb_append(pb_row, pb_key, pcnt, pa, ia, make_key(dist, ib, 1), sent);
-    pb_append(pb_row, pb_key, pcnt, pb, ib, make_key(dist, ia, 1), sent);
-    pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, bmask, seed, iter);
-  }
-  pb_flush(pb_row, pb_key, pcnt, true, slots, touched, S, bmask, seed, iter);
-  // counters[0] += pairs, counters[1] += nn + no (candidate rows gathered)
+    pb_flush(pb_row, pb_key, pcnt, true, slots, touched, S, bmask, seed, iter);
+  }();
+  // counters[0] += pairs, [1] += candidate rows gathered, [4] += proposals sent
   uint64_t tot = pairs;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) tot += shfl_xor64(tot, o);
-  if (l == 0) {
+  if (l == 0 && rows_in > 0) {
     ctr_add(counters, 0, (unsigned long long)tot);
-    ctr_add(counters, 1, (unsigned long long)nc);
+    ctr_add(counters, 1, (unsigned long long)rows_in);
     ctr_add(counters, 4, (unsigned long long)sent);
   }
+}
+
+// Cells a join launch of R rows per block leaves out: flag[i] = 1 when cell i
+// has a new candidate and more than R candidates.
+__global__ void k_join_defer_flags(const uint32_t* __restrict__ ncnt, uint32_t n, uint32_t R,
+                                   uint32_t* __restrict__ flag) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t c = ncnt[i], nn = c & 0xFFFF, nc = nn + (c >> 16);
+  flag[i] = (nn > 0 && nc > R) ? 1u : 0u;
+}
+// The flagged cells in cell order: list[pos[i]] = i (pos = exclusive scan of flag).
+__global__ void k_join_defer_list(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
+                                  uint32_t n, uint32_t* __restrict__ list) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && flag[i]) list[pos[i]] = i;
 }
 
 // ── merge (the apply of nndescent.rs:307-324 as a set function) ──
@@ -528,12 +538,12 @@ __global__ void k_renumber(uint32_t* __restrict__ ids, size_t total, const uint3
 // explicit instantiations used by the host code
 template __global__ void k_join<0>(const float*, uint32_t, uint32_t, uint32_t, uint32_t,
                                    const uint32_t*, const uint32_t*, const uint32_t*, uint32_t,
-                                   const float*, uint64_t*, uint8_t*, uint32_t, uint64_t, uint64_t,
-                                   unsigned long long*);
+                                   uint32_t, JoinCells, const float*, uint64_t*, uint8_t*, uint32_t,
+                                   uint64_t, uint64_t, unsigned long long*);
 template __global__ void k_join<50>(const float*, uint32_t, uint32_t, uint32_t, uint32_t,
                                     const uint32_t*, const uint32_t*, const uint32_t*, uint32_t,
-                                    const float*, uint64_t*, uint8_t*, uint32_t, uint64_t, uint64_t,
-                                    unsigned long long*);
+                                    uint32_t, JoinCells, const float*, uint64_t*, uint8_t*, uint32_t,
+                                    uint64_t, uint64_t, unsigned long long*);
 
 #define PSQ_MERGE_INST(NR, NT)                                                                 \
   template __global__ void k_merge<NR, NT>(uint64_t*, uint32_t, uint32_t, uint64_t*, uint32_t,   \

@@ -392,10 +392,105 @@ __host__ __device__ inline void pair_decode(uint32_t t, uint32_t nn, uint32_t no
 // accesses across. Global memory operations in flight are not waited for.
 __device__ inline void wave_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
+// Cell i's candidates staged in LDS for the join kernels (one wave per block,
+// the whole wave calls; m <= 64, so nc <= 128 and lane l holds candidates l and
+// l + 64). Loads that do not depend on each other are issued together, so a cell
+// waits on two memory round trips, not one per 64 items: (1) its counts and both
+// lists, read whole (m entries each; entries past the counts are read and
+// dropped), (2) the candidates' thresholds and rows, STAGE_BATCH 16-byte loads
+// per lane in flight. Candidate s (new list, then old list) goes to s_id[s],
+// s_thr[s] and row s of `rows` (stride RS floats); with `xi`, row i goes there.
+// Returns false, with nothing staged, when the cell has no new candidate or its
+// nc candidates exceed the rcap rows the caller holds (another launch takes it).
+// The caller synchronises before reading what was staged.
+constexpr int STAGE_BATCH = 8;
+template <int D>
+__device__ inline bool stage_cell(const float* __restrict__ Xp, uint32_t d, uint32_t dp, uint32_t RS,
+                                  const uint32_t* __restrict__ newl, const uint32_t* __restrict__ oldl,
+                                  const uint32_t* __restrict__ ncnt, uint32_t m,
+                                  const float* __restrict__ thr, uint32_t i, uint32_t rcap,
+                                  uint32_t* s_id, float* s_thr, float* rows, float* xi, uint32_t& nn,
+                                  uint32_t& no) {
+  const uint32_t l = (uint32_t)lane_id();
+  const size_t lb = (size_t)i * m;
+  // loads are unconditional, on clamped in-bounds addresses, so the compiler
+  // issues them back to back and waits once
+  const uint32_t lm = l < m ? l : m - 1;
+  const uint32_t nv = newl[lb + lm];
+  const uint32_t ov = oldl[lb + lm];
+  const uint32_t cnt = ncnt[i];
+  nn = cnt & 0xFFFF;
+  no = cnt >> 16;
+  const uint32_t nc = nn + no;
+  if (nn == 0 || nc > rcap) return false;
+  // every lane takes part in each shuffle; the selects come after
+  const uint32_t o0 = (uint32_t)__shfl((int)ov, (int)((l - nn) & (WAVE - 1)), WAVE);
+  const uint32_t v0 = l < nn ? nv : o0;
+  const uint32_t v1 = (uint32_t)__shfl((int)ov, (int)((l + WAVE - nn) & (WAVE - 1)), WAVE);
+  const float t0 = thr[l < nc ? v0 : i];
+  const float t1 = thr[l + WAVE < nc ? v1 : i];
+  const uint32_t c4 = D > 0 ? (uint32_t)(D + 3) / 4 : (d + 3) / 4;
+  const uint32_t tot = (nc + (xi != nullptr ? 1u : 0u)) * c4;
+  for (uint32_t b0 = 0; b0 < tot; b0 += STAGE_BATCH * WAVE) {
+    float4 v[STAGE_BATCH];
+#pragma unroll
+    for (int u = 0; u < STAGE_BATCH; ++u) {
+      const uint32_t e = b0 + (uint32_t)u * WAVE + l;
+      const uint32_t s = e / c4, c = e < tot ? e - s * c4 : 0u;
+      const uint32_t a = (uint32_t)__shfl((int)v0, (int)(s & (WAVE - 1)), WAVE);
+      const uint32_t b = (uint32_t)__shfl((int)v1, (int)(s & (WAVE - 1)), WAVE);
+      const uint32_t src = s >= nc ? i : (s < WAVE ? a : b);
+      v[u] = *reinterpret_cast<const float4*>(Xp + (size_t)src * dp + 4 * c);
+    }
+    // the values are pinned here, so no load is sunk into its (conditional) store
+#pragma unroll
+    for (int u = 0; u < STAGE_BATCH; ++u)
+      asm volatile("" : "+v"(v[u].x), "+v"(v[u].y), "+v"(v[u].z), "+v"(v[u].w));
+#pragma unroll
+    for (int u = 0; u < STAGE_BATCH; ++u) {
+      const uint32_t e = b0 + (uint32_t)u * WAVE + l;
+      const uint32_t s = e / c4, c = e - s * c4;
+      if (e < tot) *reinterpret_cast<float4*>((s < nc ? rows + (size_t)s * RS : xi) + 4 * c) = v[u];
+    }
+  }
+  if (l < nc) {
+    s_id[l] = v0;
+    s_thr[l] = t0;
+  }
+  if (l + WAVE < nc) {
+    s_id[l + WAVE] = v1;
+    s_thr[l + WAVE] = t1;
+  }
+  return true;
+}
+
+// A join launch holds rcap candidate rows per block in LDS. Where 2m > rcap,
+// the launch over every cell skips the cells with more candidates, and a second
+// launch, with room for 2m rows, takes those from a list in cell order, one per
+// block. Both map blocks to cells with xcd_row(), so each XCD joins a
+// contiguous run of cells (and its L2 holds their neighbourhood). Cells are
+// independent (every proposal goes through the order-free bucket insert).
+struct JoinCells {
+  const uint32_t* list;  // NULL: cells 0..n-1
+  uint32_t count;        // cells in the list
+};
+// This block's cell, or false when it has none (block-uniform).
+__device__ inline bool join_cell_of_block(const JoinCells& jc, uint32_t n, uint32_t& i) {
+  const uint32_t e = xcd_row();
+  if (jc.list) {
+    if (e >= jc.count) return false;
+    i = jc.list[e];
+    return true;
+  }
+  i = e;
+  return i < n;
+}
+
 // The join kernels run one wave per block, so __syncthreads() is a wave barrier
 // and every call below is made by the whole wave. Capacity: < 64 entries are
-// held between calls and one append adds at most 128.
-constexpr uint32_t PB_CAP = 192;
+// held between calls and one append adds at most 64 (callers flush after each
+// append).
+constexpr uint32_t PB_CAP = 128;
 __device__ inline void pb_append(uint32_t* pb_row, uint64_t* pb_key, uint32_t& cnt, bool has,
                                  uint32_t row, uint64_t key, uint64_t& sent) {
   const uint64_t mask = __ballot(has);

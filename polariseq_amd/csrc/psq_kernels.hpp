@@ -23,10 +23,11 @@ __global__ void k_sample_select(uint64_t* G, uint32_t n, uint32_t k, uint32_t m,
                                 uint32_t S, float* thr, uint32_t thr_rank);
 template <int D>
 __global__ void k_join(const float* Xp, uint32_t n, uint32_t d, uint32_t dp, uint32_t RS,
-                       const uint32_t* newl,
-                       const uint32_t* oldl, const uint32_t* ncnt, uint32_t m, const float* thr,
-                       uint64_t* slots, uint8_t* touched, uint32_t S, uint64_t seed, uint64_t iter,
-                       unsigned long long* counters);
+                       const uint32_t* newl, const uint32_t* oldl, const uint32_t* ncnt, uint32_t m,
+                       uint32_t R, JoinCells jc, const float* thr, uint64_t* slots, uint8_t* touched,
+                       uint32_t S, uint64_t seed, uint64_t iter, unsigned long long* counters);
+__global__ void k_join_defer_flags(const uint32_t* ncnt, uint32_t n, uint32_t R, uint32_t* flag);
+__global__ void k_join_defer_list(const uint32_t* flag, const uint32_t* pos, uint32_t n, uint32_t* list);
 template <int NR, int NT>
 __global__ void k_merge(uint64_t* G, uint32_t n, uint32_t k, uint64_t* slots, uint32_t S,
                         uint8_t* touched, int force, unsigned long long* changed);
@@ -94,9 +95,9 @@ __global__ void k_leaf_merge(const float* Xp, uint32_t d, uint32_t dp, uint32_t 
 template <int D>
 __global__ void k_join_bf16(const float* Xp, uint32_t n, uint32_t d, uint32_t dp, uint32_t RS,
                             const uint32_t* newl, const uint32_t* oldl, const uint32_t* ncnt,
-                            uint32_t m, const float* thr, float slack, uint64_t* slots,
-                            uint8_t* touched, uint32_t S, uint64_t seed, uint64_t iter,
-                            unsigned long long* counters);
+                            uint32_t m, uint32_t R, JoinCells jc, const float* thr, float slack,
+                            uint64_t* slots, uint8_t* touched, uint32_t S, uint64_t seed,
+                            uint64_t iter, unsigned long long* counters);
 
 // learned encoder (mlp_kernels.hip)
 constexpr int MLP_N_PARAMS = 128 * 50 + 128 + 32 * 128 + 32;
