@@ -59,16 +59,15 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic(kernel: str, summary: str = "join_4m_hbm_traffic.md"):
-    """HBM bytes per launch of `kernel` on its workload, from the rocprofv3
-    FETCH_SIZE / WRITE_SIZE passes summarised in profiles/<summary> (scaled by
-    the copy-kernel calibration)."""
+def measured_traffic(kernel: str, summary: str = "join_4m_counters.md"):
+    """HBM bytes per join pass of `kernel` on its workload, from the rocprofv3
+    request-size counter passes summarised in profiles/<summary>."""
     path = os.path.join(ROOT, "profiles", summary)
     try:
         with open(path) as f:
             for line in f:
                 parts = line.split()
-                if len(parts) == 4 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                if len(parts) == 4 and parts[0] == "traffic_bytes_per_pass" and parts[1] == kernel:
                     return int(parts[3])
     except OSError:
         return None
@@ -253,8 +252,10 @@ def main() -> int:
     recall = recalls[head][0]
 
     # roofline of the dominant kernel of the head form (its local join), over every
-    # timed launch: algorithmic bytes = candidate rows gathered x (d x 4 B + 4 B
-    # threshold) + the two candidate lists (2m x 4 B per cell) + the count word.
+    # timed join pass (one dispatch over all cells with room for 32 candidate rows,
+    # one over the cells with more; HIP events on the join's stream bracket both):
+    # algorithmic bytes = candidate rows gathered x (d x 4 B + 4 B threshold) + the
+    # two candidate lists (2m x 4 B per cell) + the count word.
     m = st["max_candidates"]
     join_bytes = 0.0
     join_ms = 0.0
@@ -272,8 +273,10 @@ def main() -> int:
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": measured_traffic(kname) if default_workload else None,
                 "kernel": kname,
-                "launches": launches, "avg_launch_ms": round(join_ms / max(launches, 1), 3),
-                "algorithmic_bytes_per_launch": round(join_bytes / max(launches, 1))}
+                "per": "join pass (the kernel's dispatch over every cell + its dispatch over "
+                       "the cells with more than 32 candidates)",
+                "passes": launches, "avg_pass_ms": round(join_ms / max(launches, 1), 3),
+                "algorithmic_bytes_per_pass": round(join_bytes / max(launches, 1))}
 
     cpu = None
     cpu_c1 = None

@@ -28,11 +28,10 @@ WEIGHT_ULPS = 2
 
 
 def _mixture(n: int, d: int = 30, seed: int = 0) -> np.ndarray:
-    """The reference test's mixture (test_neighbors.py:19-23)."""
-    rng = np.random.default_rng(seed)
-    centers = rng.normal(size=(25, d)) * 5
-    lab = rng.integers(0, 25, size=n)
-    return (centers[lab] + rng.normal(size=(n, d))).astype(np.float32)
+    """A 25-component Gaussian mixture (the reference's pp.neighbors tests use
+    one of this size, test_neighbors.py:19-23; this is polariseq_amd.datasets')."""
+    from polariseq_amd import datasets as ds
+    return ds.mixture(n, d, 25, seed=seed)
 
 
 @pytest.mark.parametrize("d", [1, 5, 8, 30, 50, 128, 129, 200, 250, 256])
