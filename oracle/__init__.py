@@ -38,7 +38,8 @@ from pathlib import Path
 import numpy as np
 
 _HERE = Path(__file__).resolve().parent
-_LIB_PATH = _HERE / "libpsq_oracle.so"
+# PSQ_ORACLE_LIB selects another build of the same sources (the sanitizer build)
+_LIB_PATH = Path(os.environ.get("PSQ_ORACLE_LIB", str(_HERE / "libpsq_oracle.so")))
 _lib = None
 
 
