@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -162,6 +163,23 @@ inline unsigned grid_for(size_t count, unsigned block) {
   return (unsigned)std::max<size_t>(1, std::min<size_t>(g, 1u << 30));
 }
 
+// Grid of `blocks` blocks for the kernels that number their blocks with
+// block_id(): one dimension up to GRID_X_BLOCKS blocks, rows of GRID_X_BLOCKS
+// (a multiple of XCDS) beyond. The environment variable PSQ_GRID_X_BLOCKS (a
+// multiple of 8) narrows the rows, so the two-dimensional layout can be tested
+// at small sizes.
+constexpr uint32_t GRID_X_BLOCKS = 1u << 22;
+dim3 block_grid(uint64_t blocks) {
+  uint64_t gx = GRID_X_BLOCKS;
+  if (const char* e = std::getenv("PSQ_GRID_X_BLOCKS")) {
+    const unsigned long long v = std::strtoull(e, nullptr, 10);
+    if (v >= XCDS && v <= GRID_X_BLOCKS && v % XCDS == 0) gx = v;
+  }
+  blocks = std::max<uint64_t>(blocks, 1);
+  if (blocks <= gx) return dim3((unsigned)blocks);
+  return dim3((unsigned)gx, (unsigned)((blocks + gx - 1) / gx));
+}
+
 int fill_u64(uint64_t* p, size_t count, uint64_t v, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_u64, dim3(std::min<unsigned>(grid_for(count, 256), 65536)), dim3(256),
                      0, s, p, count, v);
@@ -306,8 +324,8 @@ int resolve(uint64_t n, uint64_t d, const psq_nndescent_opts* o, Resolved* r) {
   if (n == 0 || d == 0) return fail(PSQ_E_INVALID, "empty data: n_obs=%llu n_dim=%llu",
                                     (unsigned long long)n, (unsigned long long)d);
   if (o->k == 0) return fail(PSQ_E_INVALID, "k must be >= 1");
-  // one 64-lane workgroup per row, and a dispatch counts fewer than 2^32 lanes
-  if (n > (1ull << 26) - XCDS) return fail(PSQ_E_UNSUPPORTED, "n_obs must be <= 2^26 - 8");
+  // row keys hold a 31-bit index
+  if (n >= (1ull << 31)) return fail(PSQ_E_UNSUPPORTED, "n_obs must be < 2^31");
   if (d > 256) return fail(PSQ_E_UNSUPPORTED, "n_dim > 256 is not supported on the GPU path");
   r->n = n;
   r->d = d;
@@ -337,6 +355,10 @@ int resolve(uint64_t n, uint64_t d, const psq_nndescent_opts* o, Resolved* r) {
   r->seed = o->seed;
   r->dp = round_up((uint32_t)d, 32);
   if (r->k > 64) return fail(PSQ_E_UNSUPPORTED, "k and heap_k > 64 are not supported on the GPU path");
+  // the reverse lists' offsets and the graph's row offsets are 32-bit
+  if (n * r->k > 0xFFFFFF00ull)
+    return fail(PSQ_E_UNSUPPORTED, "n_obs x heap_k must be below 2^32 (%llu x %llu)",
+                (unsigned long long)n, (unsigned long long)r->k);
   if (r->m > 64) return fail(PSQ_E_UNSUPPORTED, "max_candidates > 64 is not supported on the GPU path");
   if (r->S > 256) return fail(PSQ_E_UNSUPPORTED, "slots > 256 is not supported");
   if (o->precision != PSQ_PRECISION_FP32 && o->precision != PSQ_PRECISION_BF16_RERANK)
@@ -354,7 +376,7 @@ int resolve(uint64_t n, uint64_t d, const psq_nndescent_opts* o, Resolved* r) {
 
 int launch_merge(const Resolved& R, uint64_t* G, uint64_t* slots, uint8_t* touched, int force,
                  unsigned long long* changed, hipStream_t s) {
-  const dim3 grid((unsigned)R.n), block(64);
+  const dim3 grid = block_grid(R.n), block(64);
   const uint32_t n = (uint32_t)R.n, k = (uint32_t)R.k, S = (uint32_t)R.S;
   if (S == 64)
     hipLaunchKernelGGL((k_merge<1, 2>), grid, block, 0, s, G, n, k, slots, S, touched, force,
@@ -413,11 +435,11 @@ int launch_join(Workspace* ws, const Resolved& R, const JoinArgs& a, uint64_t it
     if (smem > 65536)
       HIPC(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     if (a.bf16)
-      hipLaunchKernelGGL(bk, dim3(grid), dim3(64), smem, s, a.Xp, n, (uint32_t)R.d, R.dp, RS, a.newl,
+      hipLaunchKernelGGL(bk, block_grid(grid), dim3(64), smem, s, a.Xp, n, (uint32_t)R.d, R.dp, RS, a.newl,
                          a.oldl, a.ncnt, m, rows, jc, a.thr, BF16_SLACK, a.slots, a.touched,
                          (uint32_t)R.S, R.seed, iter, a.ctr);
     else
-      hipLaunchKernelGGL(pk, dim3(grid), dim3(64), smem, s, a.Xp, n, (uint32_t)R.d, R.dp, RS, a.newl,
+      hipLaunchKernelGGL(pk, block_grid(grid), dim3(64), smem, s, a.Xp, n, (uint32_t)R.d, R.dp, RS, a.newl,
                          a.oldl, a.ncnt, m, rows, jc, a.thr, a.slots, a.touched, (uint32_t)R.S,
                          R.seed, iter, a.ctr);
     HIPC(hipGetLastError());
@@ -487,7 +509,7 @@ int launch_leaf_merge(uint32_t KM, uint32_t LG, const float* Xp, uint32_t d, uin
   else if (KM == 64 && LG == 32) kern = k_leaf_merge<D, 64, 32>;
   else kern = k_leaf_merge<D, 64, 64>;
   if (smem > 65536) HIPC(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(bt), smem, s, Xp, d, dp, RS, n, ids_t, gs, gc, nl, G, k);
+  hipLaunchKernelGGL(kern, block_grid(grid), dim3(bt), smem, s, Xp, d, dp, RS, n, ids_t, gs, gc, nl, G, k);
   HIPC(hipGetLastError());
   return PSQ_OK;
 }
@@ -668,9 +690,9 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
       small_base += newsmall;
     }
     if (small_base > 0) {
-      hipLaunchKernelGGL(k_forest_finish, dim3(small_base), dim3(64), fin_smem, s, Xs, ds, dps,
-                         RSs, n, ids, sm_start, sm_count, sm_tree, sm_level, tseeds + t0, ls,
-                         cap, NC, lcnt_pos);
+      hipLaunchKernelGGL(k_forest_finish, block_grid(small_base), dim3(64), fin_smem, s, Xs, ds,
+                         dps, RSs, n, ids, sm_start, sm_count, sm_tree, sm_level, tseeds + t0, ls,
+                         cap, NC, lcnt_pos, small_base);
       HIPC(hipGetLastError());
     }
     // leaf list, grouped by tree
@@ -831,7 +853,7 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   for (uint64_t it = 0; it < R.n_iters; ++it) {
     TRY(fill_u64(reinterpret_cast<uint64_t*>(ctr), (size_t)CTR_STRIPES * CTR_WORDS, 0, s));
     TRY(build_reverse(ws, RP, RB, G, n, k, R.seed, it, roff, rev, s));
-    hipLaunchKernelGGL(k_sample_select, dim3(n), dim3(64), 0, s, G, n, k, m, R.seed,
+    hipLaunchKernelGGL(k_sample_select, block_grid(n), dim3(64), 0, s, G, n, k, m, R.seed,
                        it, roff, rev, newl, oldl, ncnt, slots, S, thr,
                        bf16 ? bf16_thr_rank(it, R.ko, k) : k);
     HIPC(hipGetLastError());
@@ -1126,7 +1148,7 @@ int neighbors_exact_device_impl(const float* d_x, uint64_t n_obs, uint64_t n_dim
   hipLaunchKernelGGL(heap, dim3((n + 63) / 64), dim3(64), hsmem, s, d_x, n, d, kws, rs, hd, hi, lds);
   HIPC(hipGetLastError());
   if (kws <= 64) {
-    hipLaunchKernelGGL(k_nbr_sort64, dim3(n), dim3(64), 0, s, hd, hi, n, kws, (int)metric, d_indices,
+    hipLaunchKernelGGL(k_nbr_sort64, block_grid(n), dim3(64), 0, s, hd, hi, n, kws, (int)metric, d_indices,
                        d_dists);
   } else {
     uint32_t P = 1;
@@ -1135,7 +1157,7 @@ int neighbors_exact_device_impl(const float* d_x, uint64_t n_obs, uint64_t n_dim
     if (ssmem > 65536)
       HIPC(hipFuncSetAttribute((const void*)k_nbr_sort_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)ssmem));
-    hipLaunchKernelGGL(k_nbr_sort_lds, dim3(n), dim3(256), ssmem, s, hd, hi, n, kws, P, (int)metric,
+    hipLaunchKernelGGL(k_nbr_sort_lds, block_grid(n), dim3(256), ssmem, s, hd, hi, n, kws, P, (int)metric,
                        d_indices, d_dists);
   }
   HIPC(hipGetLastError());

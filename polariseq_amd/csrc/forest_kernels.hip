@@ -396,7 +396,7 @@ __global__ __launch_bounds__(64) void k_forest_finish(
     uint32_t* __restrict__ ids, const uint32_t* __restrict__ sm_start,
     const uint32_t* __restrict__ sm_count, const uint32_t* __restrict__ sm_tree,
     const uint32_t* __restrict__ sm_level, const uint64_t* __restrict__ tseeds, uint32_t leaf_size,
-    uint32_t C, uint32_t NC, uint32_t* __restrict__ lcnt_pos) {
+    uint32_t C, uint32_t NC, uint32_t* __restrict__ lcnt_pos, uint32_t nsmall) {
   extern __shared__ __attribute__((aligned(16))) float ldsf[];
   __shared__ uint32_t s_nA;
   float* normals = ldsf;                            // NC * RSs, 16-byte aligned rows
@@ -415,7 +415,8 @@ __global__ __launch_bounds__(64) void k_forest_finish(
   uint32_t* piva = child + 2 * NC;
   uint32_t* pivb = piva + NC;
   const uint32_t l = (uint32_t)lane_id();
-  const uint32_t b = blockIdx.x;
+  const uint32_t b = block_id();
+  if (b >= nsmall) return;  // block-uniform
   const uint32_t start = sm_start[b], c = sm_count[b], t = sm_tree[b], L0 = sm_level[b];
   const uint64_t ts = tseeds[t];
   uint32_t* tids = ids + (size_t)t * n + start;
@@ -651,7 +652,7 @@ __global__ __launch_bounds__(256) void k_leaf_merge(
   __shared__ uint32_t s_rounds;
   const uint32_t GPB = blockDim.x / LG;
   const int g = threadIdx.x / LG, j = threadIdx.x % LG;
-  const uint32_t li = blockIdx.x * GPB + g;
+  const uint32_t li = block_id() * GPB + g;
   const bool lvalid = li < nleaves;
   const uint32_t start = lvalid ? g_start[li] : 0u, cnt = lvalid ? g_count[li] : 0u;
   const uint32_t nch = (cnt + LG - 1) / LG;

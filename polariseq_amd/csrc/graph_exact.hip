@@ -180,7 +180,8 @@ __global__ __launch_bounds__(64) void k_nbr_sort64(const float* __restrict__ hd,
                                                    uint32_t k, int metric,
                                                    uint32_t* __restrict__ out_idx,
                                                    float* __restrict__ out_d) {
-  const uint32_t q = blockIdx.x, l = threadIdx.x;
+  const uint32_t q = block_id(), l = threadIdx.x;
+  if (q >= n) return;  // block-uniform
   uint64_t key = l < k ? nbr_key(hd[(size_t)l * n + q], l) : EMPTY_KEY;
   key = wave_sort64(key);
   if (l >= 1 && l < k) nbr_emit(hd, hi, n, q, k, metric, l, key, out_idx, out_d);
@@ -193,7 +194,8 @@ __global__ __launch_bounds__(256) void k_nbr_sort_lds(const float* __restrict__ 
                                                       uint32_t* __restrict__ out_idx,
                                                       float* __restrict__ out_d) {
   extern __shared__ uint64_t nbr_keys[];
-  const uint32_t q = blockIdx.x;
+  const uint32_t q = block_id();
+  if (q >= n) return;  // block-uniform
   for (uint32_t s = threadIdx.x; s < P; s += blockDim.x)
     nbr_keys[s] = s < k ? nbr_key(hd[(size_t)s * n + q], s) : EMPTY_KEY;
   __syncthreads();

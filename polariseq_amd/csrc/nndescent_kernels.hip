@@ -192,7 +192,7 @@ __global__ __launch_bounds__(64) void k_sample_select(
   __shared__ uint32_t s_bcnt[32];     // pinned per bucket
   __shared__ uint32_t s_ht[128];      // forward neighbour -> lane + 1 (open addressing)
   const uint32_t l = (uint32_t)lane_id();
-  const uint32_t i = blockIdx.x;
+  const uint32_t i = block_id();
   if (i >= n) return;  // block-uniform
   const uint64_t fkey = l < k ? G[(size_t)i * k + l] : EMPTY_KEY;
   const bool fvalid = fkey != EMPTY_KEY;
@@ -404,7 +404,7 @@ __global__ __launch_bounds__(64) void k_merge(uint64_t* __restrict__ G, uint32_t
                                               unsigned long long* __restrict__ changed) {
   __shared__ uint64_t s_prop[WAVE];
   const uint32_t l = (uint32_t)lane_id();
-  const uint32_t p = blockIdx.x;
+  const uint32_t p = block_id();
   if (p >= n) return;
   if (!force && !touched[p]) return;
   uint64_t* row = G + (size_t)p * k;

@@ -76,14 +76,23 @@ __device__ inline uint64_t shfl64(uint64_t v, int src) {
 }
 __device__ inline int lane_id() { return (int)(threadIdx.x & (WAVE - 1)); }
 
-// Row of workgroup blockIdx.x for the one-row-per-workgroup kernels whose rows
-// share data with nearby rows (the candidate rows of a join). The hardware
-// deals workgroups to the XCDS dies round-robin and each die has its own L2,
-// so XCD x takes the contiguous rows [x * c, (x + 1) * c), c = gridDim.x /
-// XCDS, in order. The grid is XCDS * ceil(n / XCDS); rows >= n do nothing.
+// Block b of a one-block-per-item launch. A dispatch counts fewer than 2^32
+// work-items per grid dimension, so launches of many blocks are two-dimensional
+// (the host's block_grid): b = blockIdx.y * gridDim.x + blockIdx.x, with
+// gridDim.x a multiple of XCDS whenever gridDim.y > 1. Blocks past the item
+// count do nothing.
+__device__ inline uint32_t block_id() { return blockIdx.y * gridDim.x + blockIdx.x; }
+__device__ inline uint32_t grid_blocks() { return gridDim.y * gridDim.x; }
+
+// Row of block b for the one-row-per-block kernels whose rows share data with
+// nearby rows (the candidate rows of a join). The hardware deals blocks to the
+// XCDS dies round-robin in the order of b and each die has its own L2, so XCD x
+// takes the contiguous rows [x * c, (x + 1) * c), c = grid_blocks() / XCDS, in
+// order. The grid holds a multiple of XCDS blocks; rows >= n do nothing.
 constexpr uint32_t XCDS = 8;
 __device__ inline uint32_t xcd_row() {
-  return (blockIdx.x % XCDS) * (gridDim.x / XCDS) + blockIdx.x / XCDS;
+  const uint32_t b = block_id();
+  return (b % XCDS) * (grid_blocks() / XCDS) + b / XCDS;
 }
 __host__ inline uint32_t xcd_grid(uint64_t n) { return (uint32_t)((n + XCDS - 1) / XCDS * XCDS); }
 

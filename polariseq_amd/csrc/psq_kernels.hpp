@@ -58,7 +58,7 @@ __global__ void k_forest_finish(const float* Xs, uint32_t ds, uint32_t dps, uint
                                 uint32_t* ids, const uint32_t* sm_start, const uint32_t* sm_count,
                                 const uint32_t* sm_tree, const uint32_t* sm_level,
                                 const uint64_t* tseeds, uint32_t leaf_size, uint32_t C, uint32_t NC,
-                                uint32_t* lcnt_pos);
+                                uint32_t* lcnt_pos, uint32_t nsmall);
 constexpr uint32_t FT_TILE = 1024;  // positions per tile of the level passes
 __global__ void k_ftile_count(const uint32_t* ncount, uint32_t A, uint32_t* ntile);
 __global__ void k_ftile_map(const uint32_t* toff, uint32_t A, uint32_t NT, uint32_t* tnode);
