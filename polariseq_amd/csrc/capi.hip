@@ -354,13 +354,13 @@ int resolve(uint64_t n, uint64_t d, const psq_nndescent_opts* o, Resolved* r) {
   r->stop_at = (uint64_t)(o->delta * (double)n * (double)r->k);
   r->seed = o->seed;
   r->dp = round_up((uint32_t)d, 32);
-  if (r->k > 64) return fail(PSQ_E_UNSUPPORTED, "k and heap_k > 64 are not supported on the GPU path");
+  if (r->k > 256) return fail(PSQ_E_UNSUPPORTED, "k and heap_k > 256 are not supported on the GPU path");
   // the reverse lists' offsets and the graph's row offsets are 32-bit
   if (n * r->k > 0xFFFFFF00ull)
     return fail(PSQ_E_UNSUPPORTED, "n_obs x heap_k must be below 2^32 (%llu x %llu)",
                 (unsigned long long)n, (unsigned long long)r->k);
   if (r->m > 64) return fail(PSQ_E_UNSUPPORTED, "max_candidates > 64 is not supported on the GPU path");
-  if (r->S > 256) return fail(PSQ_E_UNSUPPORTED, "slots > 256 is not supported");
+  if (r->S > 1024) return fail(PSQ_E_UNSUPPORTED, "slots > 1024 is not supported");
   if (o->precision != PSQ_PRECISION_FP32 && o->precision != PSQ_PRECISION_BF16_RERANK)
     return fail(PSQ_E_INVALID, "unknown precision %d", o->precision);
   if (o->precision == PSQ_PRECISION_BF16_RERANK && d > 64)
@@ -378,7 +378,11 @@ int launch_merge(const Resolved& R, uint64_t* G, uint64_t* slots, uint8_t* touch
                  unsigned long long* changed, hipStream_t s) {
   const dim3 grid = block_grid(R.n), block(64);
   const uint32_t n = (uint32_t)R.n, k = (uint32_t)R.k, S = (uint32_t)R.S;
-  if (S == 64)
+  if (k > 64 || S > 256) {  // rows or slots past one wave's registers: sort in LDS
+    const uint32_t need = k + S;
+    auto wk = need <= 512 ? k_merge_wide<512> : (need <= 1024 ? k_merge_wide<1024> : k_merge_wide<2048>);
+    hipLaunchKernelGGL(wk, grid, dim3(256), 0, s, G, n, k, slots, S, touched, force, changed);
+  } else if (S == 64)
     hipLaunchKernelGGL((k_merge<1, 2>), grid, block, 0, s, G, n, k, slots, S, touched, force,
                        changed);
   else if (S == 128)
@@ -495,9 +499,19 @@ template <int D>
 int launch_leaf_merge(uint32_t KM, uint32_t LG, const float* Xp, uint32_t d, uint32_t dp, uint32_t n,
                       const uint32_t* ids_t, const uint32_t* gs, const uint32_t* gc, uint32_t nl,
                       uint64_t* G, uint32_t k, hipStream_t s) {
+  const uint32_t RS = join_row_stride(d);
+  if (KM > 64) {  // rows of more than 64 entries: one block per leaf
+    const size_t wsmem = (size_t)LG * (RS + 1) * sizeof(float);
+    auto wk = KM == 128 ? k_leaf_merge_wide<D, 2> : k_leaf_merge_wide<D, 4>;
+    if (wsmem > 65536)
+      HIPC(hipFuncSetAttribute((const void*)wk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)wsmem));
+    hipLaunchKernelGGL(wk, block_grid(nl), dim3(256), wsmem, s, Xp, d, dp, RS, n, ids_t, gs, gc, nl, G,
+                       k, LG);
+    HIPC(hipGetLastError());
+    return PSQ_OK;
+  }
   const unsigned bt = dp <= 128 ? 256 : 64;
   if (bt < LG) return fail(PSQ_E_UNSUPPORTED, "leaf group larger than the block");
-  const uint32_t RS = join_row_stride(d);
   const size_t smem = (size_t)bt * (RS + 1) * sizeof(float);
   const unsigned grid = grid_for(nl, bt / LG);
   void (*kern)(const float*, uint32_t, uint32_t, uint32_t, uint32_t, const uint32_t*,
@@ -540,7 +554,7 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
   const uint64_t T = R.n_trees;
   const uint64_t TB = std::max<uint64_t>(1, std::min<uint64_t>(T, ((1ull << 31) - 2) / n));
   const uint32_t LG = leaf_group(ls);
-  uint32_t KM = k <= 16 ? 16 : (k <= 32 ? 32 : 64);
+  uint32_t KM = k <= 16 ? 16 : (k <= 32 ? 32 : (k <= 64 ? 64 : (k <= 128 ? 128 : 256)));
   if (LG > KM) KM = LG;
   // subtrees of at most `cap` points are finished by one wave each
   const uint32_t RSs = join_row_stride(ds);
@@ -853,9 +867,10 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   for (uint64_t it = 0; it < R.n_iters; ++it) {
     TRY(fill_u64(reinterpret_cast<uint64_t*>(ctr), (size_t)CTR_STRIPES * CTR_WORDS, 0, s));
     TRY(build_reverse(ws, RP, RB, G, n, k, R.seed, it, roff, rev, s));
-    hipLaunchKernelGGL(k_sample_select, block_grid(n), dim3(64), 0, s, G, n, k, m, R.seed,
-                       it, roff, rev, newl, oldl, ncnt, slots, S, thr,
-                       bf16 ? bf16_thr_rank(it, R.ko, k) : k);
+    auto sk = (k <= 64 && S <= 256) ? k_sample_select<1>
+                                    : ((k <= 128 && S <= 512) ? k_sample_select<2> : k_sample_select<4>);
+    hipLaunchKernelGGL(sk, block_grid(n), dim3(64), 0, s, G, n, k, m, R.seed, it, roff, rev, newl, oldl,
+                       ncnt, slots, S, thr, bf16 ? bf16_thr_rank(it, R.ko, k) : k);
     HIPC(hipGetLastError());
     hipEvent_t a, b;
     TRY(evp.make(&a));

@@ -729,6 +729,82 @@ __global__ __launch_bounds__(256) void k_leaf_merge(
   }
 }
 
+// Leaf pairs for rows of more than 64 entries (k <= 64 KV): one block of 256
+// threads per leaf, the leaf taken in consecutive chunks of LG members as above.
+// The chunk's rows are staged in LDS; wave w takes members j = w, w + 4, ...:
+// lane q computes the key of chunk-mate q (q != j), and the row of j and those
+// keys are sorted together in registers, equal (dist, idx) keep their first
+// (the row's entry when present) and the row takes the first k. Same set
+// function as the per-lane insertion of k_leaf_merge.
+template <int D, int KV>
+__global__ __launch_bounds__(256) void k_leaf_merge_wide(
+    const float* __restrict__ Xp, uint32_t d, uint32_t dp, uint32_t RS, uint32_t n,
+    const uint32_t* __restrict__ ids_t, const uint32_t* __restrict__ g_start,
+    const uint32_t* __restrict__ g_count, uint32_t nleaves, uint64_t* __restrict__ G, uint32_t k,
+    uint32_t LG) {
+  constexpr int NT = KV * 2;  // row vectors + the candidates, rounded up to a power of two
+  extern __shared__ __attribute__((aligned(16))) float s_rows[];  // [LG][RS] | ids[LG]
+  const uint32_t li = block_id();
+  if (li >= nleaves) return;  // block-uniform
+  const uint32_t tid = threadIdx.x, l = (uint32_t)lane_id(), w = tid / WAVE;
+  uint32_t* gids = reinterpret_cast<uint32_t*>(s_rows + (size_t)LG * RS);
+  const uint32_t start = g_start[li], cnt = g_count[li];
+  const uint32_t c4 = (d + 3) / 4;
+  for (uint32_t c0 = 0; c0 < cnt; c0 += LG) {
+    const uint32_t cc = min(LG, cnt - c0);
+    __syncthreads();
+    if (tid < cc) gids[tid] = ids_t[start + c0 + tid];
+    __syncthreads();
+    for (uint32_t e = tid; e < cc * c4; e += blockDim.x) {
+      const uint32_t rr = e / c4, c = e - rr * c4;
+      *reinterpret_cast<float4*>(s_rows + (size_t)rr * RS + 4 * c) =
+          *reinterpret_cast<const float4*>(Xp + (size_t)gids[rr] * dp + 4 * c);
+    }
+    __syncthreads();
+    for (uint32_t j = w; j < cc; j += blockDim.x / WAVE) {
+      const float* mine = s_rows + (size_t)j * RS;
+      uint64_t u[NT];
+      uint64_t* gr = G + (size_t)gids[j] * k;
+#pragma unroll
+      for (int r = 0; r < KV; ++r) u[r] = (uint32_t)r * WAVE + l < k ? gr[r * WAVE + l] : EMPTY_KEY;
+      u[KV] = EMPTY_KEY;
+      if (l < cc && l != j) {
+        const float dist = lds_sq_dist<D>(mine, s_rows + (size_t)l * RS, d);
+        u[KV] = make_key(dist, gids[l], 1);
+      }
+#pragma unroll
+      for (int r = KV + 1; r < NT; ++r) u[r] = EMPTY_KEY;
+      wave_sort_n<NT>(u);
+      uint32_t before = 0;
+      uint64_t lastkey = EMPTY_KEY;
+#pragma unroll
+      for (int r = 0; r < NT; ++r) {
+        uint64_t prev = shfl64(u[r], (int)((l + WAVE - 1) & (WAVE - 1)));
+        if (l == 0) prev = lastkey;
+        const bool keep = u[r] != EMPTY_KEY && !((r > 0 || l > 0) && (u[r] >> 1) == (prev >> 1));
+        const uint64_t mask = __ballot(keep);
+        const uint32_t rank = before + (uint32_t)__popcll(mask & ((1ull << l) - 1ull));
+        if (keep && rank < k) gr[rank] = u[r];
+        before += (uint32_t)__popcll(mask);
+        lastkey = shfl64(u[r], WAVE - 1);
+      }
+      for (uint32_t pos = (before < k ? before : k) + l; pos < k; pos += WAVE) gr[pos] = EMPTY_KEY;
+    }
+  }
+}
+template __global__ void k_leaf_merge_wide<0, 2>(const float*, uint32_t, uint32_t, uint32_t, uint32_t,
+                                                 const uint32_t*, const uint32_t*, const uint32_t*,
+                                                 uint32_t, uint64_t*, uint32_t, uint32_t);
+template __global__ void k_leaf_merge_wide<0, 4>(const float*, uint32_t, uint32_t, uint32_t, uint32_t,
+                                                 const uint32_t*, const uint32_t*, const uint32_t*,
+                                                 uint32_t, uint64_t*, uint32_t, uint32_t);
+template __global__ void k_leaf_merge_wide<50, 2>(const float*, uint32_t, uint32_t, uint32_t, uint32_t,
+                                                  const uint32_t*, const uint32_t*, const uint32_t*,
+                                                  uint32_t, uint64_t*, uint32_t, uint32_t);
+template __global__ void k_leaf_merge_wide<50, 4>(const float*, uint32_t, uint32_t, uint32_t, uint32_t,
+                                                  const uint32_t*, const uint32_t*, const uint32_t*,
+                                                  uint32_t, uint64_t*, uint32_t, uint32_t);
+
 #define PSQ_LEAF_INST(D, KM, LG)                                                              \
   template __global__ void k_leaf_merge<D, KM, LG>(const float*, uint32_t, uint32_t, uint32_t, \
                                                    uint32_t, const uint32_t*, const uint32_t*, \

@@ -181,41 +181,55 @@ __global__ void k_single_point(const float* __restrict__ X, uint32_t d, uint32_t
 // The row's neighbours are also pinned into its buckets for this iteration's
 // join (at most BUCKET per bucket, in any order; the other slots empty), and
 // the row's threshold for the join is set.
+// KV: forward vectors, k <= 64 KV and S <= 256 KV; lane l holds forward entries
+// l, l + 64, ..., l + 64 (KV - 1).
+template <int KV>
 __global__ __launch_bounds__(64) void k_sample_select(
     uint64_t* __restrict__ G, uint32_t n, uint32_t k, uint32_t m, uint64_t seed, uint64_t iter,
     const uint32_t* __restrict__ roff, const uint64_t* __restrict__ rev,
     uint32_t* __restrict__ newl, uint32_t* __restrict__ oldl, uint32_t* __restrict__ ncnt,
     uint64_t* __restrict__ slots, uint32_t S, float* __restrict__ thr, uint32_t thr_rank) {
-  __shared__ uint64_t s_fk[WAVE];     // forward keys
-  __shared__ uint64_t s_kkey[WAVE];   // key of the reverse entry that replaced a forward one
-  __shared__ uint64_t s_pin[256];     // bucket layout, S <= 256
-  __shared__ uint32_t s_bcnt[32];     // pinned per bucket
-  __shared__ uint32_t s_ht[128];      // forward neighbour -> lane + 1 (open addressing)
+  constexpr int KF = KV * WAVE;  // forward entries
+  constexpr int HB = KV == 1 ? 7 : (KV == 2 ? 8 : 9);  // hash table of 2 KF slots
+  constexpr uint32_t HM = (1u << HB) - 1;
+  __shared__ uint64_t s_fk[KF];            // forward keys
+  __shared__ uint64_t s_kkey[KF];          // key of the reverse entry that replaced a forward one
+  __shared__ uint64_t s_pin[256 * KV];     // bucket layout
+  __shared__ uint32_t s_bcnt[32 * KV];     // pinned per bucket
+  __shared__ uint32_t s_ht[2 * KF];        // forward neighbour -> entry + 1 (open addressing)
   const uint32_t l = (uint32_t)lane_id();
   const uint32_t i = block_id();
   if (i >= n) return;  // block-uniform
-  const uint64_t fkey = l < k ? G[(size_t)i * k + l] : EMPTY_KEY;
-  const bool fvalid = fkey != EMPTY_KEY;
-  const uint32_t me = key_idx(fkey);
-  // the join's threshold: distance of entry thr_rank - 1 (the row's worst when
-  // thr_rank = k), infinite while the row holds fewer entries
-  if (l == thr_rank - 1) thr[i] = fvalid ? key_dist(fkey) : __builtin_inff();
-  uint64_t fk = EMPTY_KEY;
-  if (fvalid) {
-    const uint64_t cls = (fkey & 1) ? 0ull : 1ull;
-    const uint64_t pr = hash_prio24(seed, iter, (uint64_t)i * k + l);
-    fk = (cls << 63) | (pr << 32) | me;
+  uint64_t fkey[KV], fk[KV];
+  bool fvalid[KV];
+  uint32_t me[KV];
+#pragma unroll
+  for (int v = 0; v < KV; ++v) {
+    const uint32_t f = v * WAVE + l;
+    fkey[v] = f < k ? G[(size_t)i * k + f] : EMPTY_KEY;
+    fvalid[v] = fkey[v] != EMPTY_KEY;
+    me[v] = key_idx(fkey[v]);
+    // the join's threshold: distance of entry thr_rank - 1 (the row's worst when
+    // thr_rank = k), infinite while the row holds fewer entries
+    if (f == thr_rank - 1) thr[i] = fvalid[v] ? key_dist(fkey[v]) : __builtin_inff();
+    fk[v] = EMPTY_KEY;
+    if (fvalid[v]) {
+      const uint64_t cls = (fkey[v] & 1) ? 0ull : 1ull;
+      const uint64_t pr = hash_prio24(seed, iter, (uint64_t)i * k + f);
+      fk[v] = (cls << 63) | (pr << 32) | me[v];
+    }
+    s_fk[f] = fk[v];
+    s_kkey[f] = EMPTY_KEY;
   }
-  s_fk[l] = fk;
-  s_kkey[l] = EMPTY_KEY;
-  if (l < 32) s_bcnt[l] = 0;
-  s_ht[l] = 0;
-  s_ht[l + WAVE] = 0;
+  for (uint32_t e = l; e < 32 * KV; e += WAVE) s_bcnt[e] = 0;
+  for (uint32_t e = l; e < 2 * KF; e += WAVE) s_ht[e] = 0;
   __syncthreads();
-  if (fvalid) {  // forward neighbours are distinct
-    for (uint32_t h = (me * 0x9E3779B1u) >> 25;; h = (h + 1) & 127u)
-      if (atomicCAS(&s_ht[h], 0u, l + 1) == 0u) break;
-  }
+#pragma unroll
+  for (int v = 0; v < KV; ++v)
+    if (fvalid[v]) {  // forward neighbours are distinct
+      for (uint32_t h = (me[v] * 0x9E3779B1u) >> (32 - HB);; h = (h + 1) & HM)
+        if (atomicCAS(&s_ht[h], 0u, v * WAVE + l + 1) == 0u) break;
+    }
   __syncthreads();
   const uint32_t r0 = roff[i], nrev = roff[i + 1] - r0, T = k + nrev;
   const uint32_t nvec = (T + WAVE - 1) / WAVE;
@@ -229,7 +243,7 @@ __global__ __launch_bounds__(64) void k_sample_select(
     const uint32_t src = (uint32_t)rv;
     const uint64_t cp = rv >> 32;  // class and priority
     // the forward entry with this neighbour, if any (forward neighbours are distinct)
-    for (uint32_t h = (src * 0x9E3779B1u) >> 25;; h = (h + 1) & 127u) {
+    for (uint32_t h = (src * 0x9E3779B1u) >> (32 - HB);; h = (h + 1) & HM) {
       const uint32_t slot = s_ht[h];
       if (slot == 0) return rv;
       const uint32_t f = slot - 1;
@@ -244,15 +258,27 @@ __global__ __launch_bounds__(64) void k_sample_select(
     }
   };
   // pass 1: every replacement is known before a forward key is used
-  uint64_t key0 = rev_key(l, true);
-  for (uint32_t j = 1; j < nvec; ++j) (void)rev_key(j * WAVE + l, true);
+  for (uint32_t j = 0; j < nvec; ++j) (void)rev_key(j * WAVE + l, true);
   __syncthreads();
-  if (l < k) key0 = (fvalid && s_kkey[l] == EMPTY_KEY) ? fk : EMPTY_KEY;
   // pass 2: selection
   uint64_t bestN = EMPTY_KEY, bestO = EMPTY_KEY;
   uint32_t nn = 0, no = 0;
   for (uint32_t j = 0; j < nvec; ++j) {
-    const uint64_t key = j == 0 ? key0 : rev_key(j * WAVE + l, false);
+    const uint32_t t = j * WAVE + l;
+    uint64_t key;
+    if (t < k) {
+      uint64_t f = EMPTY_KEY;
+      bool fv = false;
+#pragma unroll
+      for (int v = 0; v < KV; ++v)
+        if ((uint32_t)v == j) {
+          f = fk[v];
+          fv = fvalid[v];
+        }
+      key = (fv && s_kkey[t] == EMPTY_KEY) ? f : EMPTY_KEY;
+    } else {
+      key = rev_key(t, false);
+    }
     const uint64_t sk = wave_sort64(key);
     const uint32_t cN = (uint32_t)__popcll(__ballot(sk != EMPTY_KEY && (sk >> 63) == 0));
     const uint32_t cO = (uint32_t)__popcll(__ballot(sk != EMPTY_KEY && (sk >> 63) == 1));
@@ -279,22 +305,37 @@ __global__ __launch_bounds__(64) void k_sample_select(
   // flag update: the surviving new-class key of this neighbour (the entry's own
   // or the reverse entry that replaced it) is among the nn selected
   const uint64_t lastN = shfl64(bestN, (int)(nn > 0 ? nn - 1 : 0));
-  if (fvalid && (fkey & 1) && nn > 0) {
-    const uint64_t kk = s_kkey[l] != EMPTY_KEY ? s_kkey[l] : fk;
-    if (kk <= lastN) G[(size_t)i * k + l] = fkey & ~1ull;
+#pragma unroll
+  for (int v = 0; v < KV; ++v) {
+    const uint32_t f = v * WAVE + l;
+    if (fvalid[v] && (fkey[v] & 1) && nn > 0) {
+      const uint64_t kk = s_kkey[f] != EMPTY_KEY ? s_kkey[f] : fk[v];
+      if (kk <= lastN) G[(size_t)i * k + f] = fkey[v] & ~1ull;
+    }
   }
   // bucket layout
   const uint32_t bmask = S / BUCKET - 1;
   for (uint32_t e = l; e < S; e += WAVE) s_pin[e] = EMPTY_KEY;
   __syncthreads();
-  if (fvalid) {
-    const uint32_t b = slot_hash(seed, iter, me) & bmask;
-    const uint32_t r = atomicAdd(&s_bcnt[b], 1u);
-    if (r < BUCKET) s_pin[b * BUCKET + r] = pin_key(me);
-  }
+#pragma unroll
+  for (int v = 0; v < KV; ++v)
+    if (fvalid[v]) {
+      const uint32_t b = slot_hash(seed, iter, me[v]) & bmask;
+      const uint32_t r = atomicAdd(&s_bcnt[b], 1u);
+      if (r < BUCKET) s_pin[b * BUCKET + r] = pin_key(me[v]);
+    }
   __syncthreads();
   for (uint32_t e = l; e < S; e += WAVE) slots[(size_t)i * S + e] = s_pin[e];
 }
+template __global__ void k_sample_select<1>(uint64_t*, uint32_t, uint32_t, uint32_t, uint64_t, uint64_t,
+                                            const uint32_t*, const uint64_t*, uint32_t*, uint32_t*,
+                                            uint32_t*, uint64_t*, uint32_t, float*, uint32_t);
+template __global__ void k_sample_select<2>(uint64_t*, uint32_t, uint32_t, uint32_t, uint64_t, uint64_t,
+                                            const uint32_t*, const uint64_t*, uint32_t*, uint32_t*,
+                                            uint32_t*, uint64_t*, uint32_t, float*, uint32_t);
+template __global__ void k_sample_select<4>(uint64_t*, uint32_t, uint32_t, uint32_t, uint64_t, uint64_t,
+                                            const uint32_t*, const uint64_t*, uint32_t*, uint32_t*,
+                                            uint32_t*, uint64_t*, uint32_t, float*, uint32_t);
 
 // ── local join (nndescent.rs:912-969) ──
 // One wave (one block) per cell i. The candidate rows (new list, then old
@@ -477,6 +518,100 @@ __global__ __launch_bounds__(64) void k_merge(uint64_t* __restrict__ G, uint32_t
   }
   if (l == 0 && nchanged) ctr_add(changed, 2, (unsigned long long)nchanged);
 }
+
+// Merge for rows of more than 64 entries or more than 256 slots: one block of
+// 256 threads per touched row sorts the row and every slot together in LDS
+// (bitonic, P >= k + S keys), keeps the first of each run of equal (dist, idx)
+// (an entry already in the row keeps its flag, its key being the smaller), and
+// writes the first k; the same set function as k_merge.
+template <int P>
+__global__ __launch_bounds__(256) void k_merge_wide(uint64_t* __restrict__ G, uint32_t n, uint32_t k,
+                                                    uint64_t* __restrict__ slots, uint32_t S,
+                                                    uint8_t* __restrict__ touched, int force,
+                                                    unsigned long long* __restrict__ changed) {
+  constexpr int NTH = 256, EPT = P / NTH;
+  __shared__ uint64_t s_key[P];
+  __shared__ uint32_t s_wsum[NTH / WAVE];
+  __shared__ uint64_t s_kth;
+  const uint32_t tid = threadIdx.x, l = (uint32_t)lane_id(), w = tid / WAVE;
+  const uint32_t p = block_id();
+  if (p >= n) return;  // block-uniform
+  const bool was = touched[p] != 0;
+  if (!force && !was) return;
+  uint64_t* row = G + (size_t)p * k;
+  const uint64_t* sl = slots + (size_t)p * S;
+  const uint64_t orow = tid < k ? row[tid] : EMPTY_KEY;  // k <= 256
+  if (force && !was) {
+    // the forced pass only sorts: a sorted row is left as it is
+    const bool desc = tid + 1 < k && orow > row[tid + 1];
+    if (!__syncthreads_or(desc)) return;
+  }
+  __syncthreads();
+  if (tid == 0) touched[p] = 0;
+  for (uint32_t e = tid; e < P; e += NTH)
+    s_key[e] = e < k ? row[e] : (e < k + S ? bucket_to_row(sl[e - k]) : EMPTY_KEY);
+  for (uint32_t size = 2; size <= P; size <<= 1)
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      __syncthreads();
+      for (uint32_t e = tid; e < P / 2; e += NTH) {
+        const uint32_t lo = 2 * stride * (e / stride) + (e % stride), hi = lo + stride;
+        const bool up = (lo & size) == 0;
+        const uint64_t a = s_key[lo], b = s_key[hi];
+        if ((a > b) == up) {
+          s_key[lo] = b;
+          s_key[hi] = a;
+        }
+      }
+    }
+  __syncthreads();
+  // dedup on (dist, idx) and compact: thread tid takes elements [tid EPT, (tid + 1) EPT)
+  bool keep[EPT];
+  uint32_t mine = 0;
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const uint32_t e = tid * EPT + q;
+    const uint64_t key = s_key[e];
+    keep[q] = key != EMPTY_KEY && (e == 0 || (key >> 1) != (s_key[e - 1] >> 1));
+    mine += keep[q];
+  }
+  uint32_t incl = mine;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o, WAVE);
+    if (l >= (uint32_t)o) incl += y;
+  }
+  if (l == WAVE - 1) s_wsum[w] = incl;
+  __syncthreads();
+  uint32_t base = 0, total = 0;
+  for (uint32_t q = 0; q < NTH / WAVE; ++q) {
+    base += q < w ? s_wsum[q] : 0u;
+    total += s_wsum[q];
+  }
+  uint32_t rank = base + incl - mine;
+  const uint32_t nk = total < k ? total : k;
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    if (keep[q]) {
+      const uint64_t key = s_key[tid * EPT + q];
+      if (rank < k) row[rank] = key;
+      if (rank + 1 == nk) s_kth = key;
+      ++rank;
+    }
+  }
+  for (uint32_t pos = nk + tid; pos < k; pos += NTH) row[pos] = EMPTY_KEY;
+  __syncthreads();
+  // entries of the new row absent from the old: nk minus the old entries kept
+  const uint64_t kth = s_kth;
+  const bool kept_old = nk > 0 && orow != EMPTY_KEY && (orow >> 1) <= (kth >> 1);
+  const uint32_t nold = (uint32_t)__syncthreads_count(kept_old);
+  if (tid == 0 && nk > nold) ctr_add(changed, 2, (unsigned long long)(nk - nold));
+}
+template __global__ void k_merge_wide<512>(uint64_t*, uint32_t, uint32_t, uint64_t*, uint32_t, uint8_t*,
+                                           int, unsigned long long*);
+template __global__ void k_merge_wide<1024>(uint64_t*, uint32_t, uint32_t, uint64_t*, uint32_t, uint8_t*,
+                                            int, unsigned long long*);
+template __global__ void k_merge_wide<2048>(uint64_t*, uint32_t, uint32_t, uint64_t*, uint32_t, uint8_t*,
+                                            int, unsigned long long*);
 
 // ── extraction (nndescent.rs:977-1002) ──
 // Rows hold k entries; the graph takes the first ko (ko <= k) of each. With a

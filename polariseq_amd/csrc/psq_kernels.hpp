@@ -17,6 +17,7 @@ __global__ void k_scan_add(uint32_t* out, uint32_t n, const uint32_t* sums);
 __global__ void k_random_fill(const float* Xp, uint32_t n, uint32_t d, uint32_t dp, uint64_t* G,
                               uint32_t k, uint64_t seed, uint8_t* touched);
 __global__ void k_ctr_reduce(const unsigned long long* ctr, unsigned long long* out);
+template <int KV>
 __global__ void k_sample_select(uint64_t* G, uint32_t n, uint32_t k, uint32_t m, uint64_t seed,
                                 uint64_t iter, const uint32_t* roff, const uint64_t* rev,
                                 uint32_t* newl, uint32_t* oldl, uint32_t* ncnt, uint64_t* slots,
@@ -28,6 +29,9 @@ __global__ void k_join(const float* Xp, uint32_t n, uint32_t d, uint32_t dp, uin
                        uint32_t S, uint64_t seed, uint64_t iter, unsigned long long* counters);
 __global__ void k_join_defer_flags(const uint32_t* ncnt, uint32_t n, uint32_t R, uint32_t* flag);
 __global__ void k_join_defer_list(const uint32_t* flag, const uint32_t* pos, uint32_t n, uint32_t* list);
+template <int P>
+__global__ void k_merge_wide(uint64_t* G, uint32_t n, uint32_t k, uint64_t* slots, uint32_t S,
+                             uint8_t* touched, int force, unsigned long long* changed);
 template <int NR, int NT>
 __global__ void k_merge(uint64_t* G, uint32_t n, uint32_t k, uint64_t* slots, uint32_t S,
                         uint8_t* touched, int force, unsigned long long* changed);
@@ -86,6 +90,10 @@ __global__ void k_leaf_flags(const uint32_t* lcnt_pos, size_t total, uint32_t* f
 __global__ void k_leaf_compact(const uint32_t* lcnt_pos, size_t total, uint32_t n, const uint32_t* off,
                                uint32_t* g_start, uint32_t* g_count);
 __global__ void k_tree_offsets(const uint32_t* off, uint32_t n, uint32_t tb, uint32_t* out);
+template <int D, int KV>
+__global__ void k_leaf_merge_wide(const float* Xp, uint32_t d, uint32_t dp, uint32_t RS, uint32_t n,
+                                  const uint32_t* ids_t, const uint32_t* g_start, const uint32_t* g_count,
+                                  uint32_t nleaves, uint64_t* G, uint32_t k, uint32_t LG);
 template <int D, int KMAX, int LG>
 __global__ void k_leaf_merge(const float* Xp, uint32_t d, uint32_t dp, uint32_t RS, uint32_t n,
                              const uint32_t* ids_t, const uint32_t* g_start, const uint32_t* g_count,

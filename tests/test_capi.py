@@ -78,9 +78,9 @@ def test_invalid_inputs_return_error_codes_before_any_device_work():
 
 
 def test_unsupported_options_are_reported_not_ignored():
-    x = np.zeros((100, 4), np.float32)
-    rc, msg = _call_into(x, 80)
-    assert rc == N.PSQ_E_UNSUPPORTED and "k > 64" in msg
+    x = np.zeros((400, 4), np.float32)
+    rc, msg = _call_into(x, 300)
+    assert rc == N.PSQ_E_UNSUPPORTED and "heap_k > 256" in msg
     rc, msg = _call_into(np.zeros((10, 300), np.float32), 5)
     assert rc == N.PSQ_E_UNSUPPORTED and "n_dim" in msg
 

@@ -65,8 +65,9 @@ def test_cosine_conversion_bit_exact_with_numpy():
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
 
 
-@pytest.mark.parametrize("metric", ["euclidean", "cosine"])
-def test_approximate_pipeline_matches_the_oracle_pipeline(orc, metric):
+@pytest.mark.parametrize("metric,K", [("euclidean", 15), ("cosine", 15), ("euclidean", 100),
+                                      ("cosine", 100)])
+def test_approximate_pipeline_matches_the_oracle_pipeline(orc, metric, K):
     x = _mixture(3000, d=20, seed=7)
     x *= np.random.default_rng(8).uniform(0.5, 2.0, size=(3000, 1)).astype(np.float32)
 
@@ -74,11 +75,11 @@ def test_approximate_pipeline_matches_the_oracle_pipeline(orc, metric):
         g, _ = orc.gpu_model_nndescent(search, k, seed=seed)
         return g["indptr"], g["indices"], g["data"]
 
-    want = C.neighbors_graph(x, 15, metric, True, 3, nnd, orc.neighbors_exact,
+    want = C.neighbors_graph(x, K, metric, True, 3, nnd, orc.neighbors_exact,
                              orc.fuzzy_connectivities)
     wd = want["distances"]
-    print(f"pipeline {metric}", flush=True)
-    got = P.neighbors_graph(x, 15, metric=metric, approximate=True, seed=3)
+    print(f"pipeline {metric} n_neighbors={K}", flush=True)
+    got = P.neighbors_graph(x, K, metric=metric, approximate=True, seed=3)
     gd, gc = got["distances"], got["connectivities"]
     assert got["uns"]["method"] == "nndescent"
     assert np.array_equal(gd["indptr"], wd["indptr"])

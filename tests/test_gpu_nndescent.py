@@ -78,6 +78,27 @@ def test_wider_rows_bit_exact_against_the_oracle(orc, init, model_init):
     _same(g, m)
 
 
+@pytest.mark.parametrize("n,d,k,seed,init,heap_k", [(4000, 50, 65, 1, 0, 0), (5000, 20, 99, 2, 0, 0),
+                                                     (3000, 50, 99, 3, 1, 0), (3000, 16, 150, 4, 0, 0),
+                                                     (2500, 50, 200, 5, 1, 0), (4000, 50, 40, 6, 0, 130),
+                                                     (3000, 50, 256, 7, 0, 0)])
+def test_rows_wider_than_a_wave_bit_exact_against_the_oracle(orc, n, d, k, seed, init, heap_k):
+    # rows of more than 64 entries (sampling with 2 or 4 forward vectors, the
+    # LDS-sorted merge, one block per leaf for the leaf pairs); pp.neighbors with
+    # n_neighbors = 100 asks for k = 99 (nndescent.rs accepts any k)
+    x = ds.mixture(n, d, 8, seed=seed)
+    print(f"wide rows n={n} d={d} k={k} init={init} heap_k={heap_k}", flush=True)
+    g = P.nndescent_dense(x, k, seed=seed, init="rp_forest" if init == 0 else "random", n_trees=4,
+                          heap_k=heap_k)
+    m, _ = orc.gpu_model_nndescent(x, k, seed=seed, init=init, n_trees=4, heap_k=heap_k)
+    _same(g, m)
+    ip, ix = g["indptr"], g["indices"]
+    assert (np.diff(ip) == k).all()
+    for i in range(0, n, 97):
+        row = ix[ip[i]:ip[i + 1]]
+        assert i not in row and len(set(row.tolist())) == k
+
+
 def test_structure_determinism_and_in_place_form():
     x = ds.clustered(2000, 8, 5, seed=1)
     a = P.nndescent_dense(x, 10, seed=1, init="random")
