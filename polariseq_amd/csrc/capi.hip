@@ -1002,10 +1002,12 @@ int fuzzy_core(Workspace* ws, const uint32_t* d_ip, const uint32_t* d_ix, const 
   R.n = n;
   uint32_t *h;
   TRY(ws->host_counters((void**)&h));
-  // Super-buckets of at most PART_SB_MAX; buckets of FZ_BUCKET_ROWS rows.
-  const PartShape P = part_shape(n, FZ_BUCKET_LOG);
-  if (P.blog != FZ_BUCKET_LOG)
-    return fail(PSQ_E_UNSUPPORTED, "n_obs = %u exceeds the fuzzy path's limit of 2^26 rows", n);
+  // Buckets of FZ_BUCKET_ROWS rows; super-buckets of at most PART_SB_MAX (2^26
+  // rows), else PART_SB_WIDE. The emit pass counts FZ_EMIT_LANES lanes a row.
+  const bool wide = n > (1u << 26);
+  const PartShape P = part_shape(n, FZ_BUCKET_LOG, wide ? PART_SB_WIDE : PART_SB_MAX);
+  if (P.blog != FZ_BUCKET_LOG || (uint64_t)n * FZ_EMIT_LANES >= (1ull << 32))
+    return fail(PSQ_E_UNSUPPORTED, "n_obs = %u exceeds the fuzzy path's limit of 2^28 rows", n);
   const uint32_t nb = P.nb;
   uint32_t *flags, *olen, *big, *giant;
   unsigned long long* tot;
@@ -1036,12 +1038,12 @@ int fuzzy_core(Workspace* ws, const uint32_t* d_ip, const uint32_t* d_ix, const 
   HIPC(hipGetLastError());
   // Phase 2: transpose by bucket (umap.rs:357-381), two partition levels.
   TRY(part_clear(P, B, s));
-  hipLaunchKernelGGL(k_fz_l1_count, dim3(std::max<unsigned>(1, (unsigned)((m + PART_C1_CHUNK - 1) / PART_C1_CHUNK))),
+  hipLaunchKernelGGL(wide ? k_fz_l1_count<PART_SB_WIDE> : k_fz_l1_count<PART_SB_MAX>, dim3(std::max<unsigned>(1, (unsigned)((m + PART_C1_CHUNK - 1) / PART_C1_CHUNK))),
                      dim3(1024), 0, s, d_ix, m, P.sshift, P.ns, B.stot);
   HIPC(hipGetLastError());
   TRY(scan_u32(ws, B.stot, B.sstart, P.ns, s));
   if (m > 0) {
-    hipLaunchKernelGGL(k_fz_l1_scatter, dim3((unsigned)((m + PART_P1_CHUNK - 1) / PART_P1_CHUNK)),
+    hipLaunchKernelGGL(wide ? k_fz_l1_scatter<PART_SB_WIDE> : k_fz_l1_scatter<PART_SB_MAX>, dim3((unsigned)((m + PART_P1_CHUNK - 1) / PART_P1_CHUNK)),
                        dim3(1024), 0, s, pay0, d_ix, m, P.sshift, P.ns, B.sstart, B.sfill, B.pay1,
                        B.pj1);
     HIPC(hipGetLastError());

@@ -203,11 +203,12 @@ __global__ __launch_bounds__(256) void k_fuzzy_strengths(const uint32_t* __restr
 // FZ_BUCKET_ROWS rows). k_fz_bucket_merge then counting-sorts one bucket's own
 // and transposed entries by row in LDS and merges every row of the bucket.
 
-// Level-1 count: edges per target super-bucket.
+// Level-1 count: edges per target super-bucket (at most SBM <= 1024 of them).
+template <int SBM>
 __global__ __launch_bounds__(1024) void k_fz_l1_count(const uint32_t* __restrict__ indices,
                                                       uint64_t m, uint32_t sshift, uint32_t ns,
                                                       uint32_t* __restrict__ stot) {
-  __shared__ uint32_t h[PART_SB_MAX];
+  __shared__ uint32_t h[SBM];
   const uint32_t tid = threadIdx.x;
   if (tid < ns) h[tid] = 0;
   __syncthreads();
@@ -220,13 +221,14 @@ __global__ __launch_bounds__(1024) void k_fz_l1_count(const uint32_t* __restrict
 
 // Level-1 placement: pay1 = the transposed entry key, pj1 = the full target
 // row j, in super-bucket order.
+template <int SBM>
 __global__ __launch_bounds__(1024) void k_fz_l1_scatter(
     const uint64_t* __restrict__ pay0, const uint32_t* __restrict__ indices, uint64_t m,
     uint32_t sshift, uint32_t ns, const uint32_t* __restrict__ sstart, uint32_t* __restrict__ sfill,
     uint64_t* __restrict__ pay1, uint32_t* __restrict__ pj1) {
   __shared__ uint64_t s_key[PART_P1_CHUNK];
   __shared__ uint32_t s_tgt[PART_P1_CHUNK];
-  __shared__ uint32_t s_cnt[PART_SB_MAX + 1], s_base[PART_SB_MAX], s_part[1024 / WAVE];
+  __shared__ uint32_t s_cnt[SBM + 1], s_base[SBM], s_part[1024 / WAVE];
   const uint64_t e0 = (uint64_t)blockIdx.x * PART_P1_CHUNK;
   const uint32_t cnt = (uint32_t)min((uint64_t)PART_P1_CHUNK, m - e0);
   const uint64_t* kin = pay0 + e0;
@@ -240,6 +242,14 @@ __global__ __launch_bounds__(1024) void k_fz_l1_scatter(
       cnt, ns, 0u, [&](uint32_t j) { return j >> sshift; }, sstart, sfill, pay1, pj1,
       [](uint32_t j) { return j; }, s_key, s_tgt, s_cnt, s_base, s_part);
 }
+template __global__ void k_fz_l1_count<PART_SB_MAX>(const uint32_t*, uint64_t, uint32_t, uint32_t, uint32_t*);
+template __global__ void k_fz_l1_count<PART_SB_WIDE>(const uint32_t*, uint64_t, uint32_t, uint32_t, uint32_t*);
+template __global__ void k_fz_l1_scatter<PART_SB_MAX>(const uint64_t*, const uint32_t*, uint64_t, uint32_t,
+                                                      uint32_t, const uint32_t*, uint32_t*, uint64_t*,
+                                                      uint32_t*);
+template __global__ void k_fz_l1_scatter<PART_SB_WIDE>(const uint64_t*, const uint32_t*, uint64_t, uint32_t,
+                                                       uint32_t, const uint32_t*, uint32_t*, uint64_t*,
+                                                       uint32_t*);
 
 // One block per bucket of FZ_BUCKET_ROWS rows: the bucket's own entries (a
 // contiguous CSR range) and its transposed entries are counting-sorted by row

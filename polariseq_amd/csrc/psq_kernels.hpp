@@ -206,8 +206,10 @@ __global__ void k_fuzzy_check(const uint32_t* indptr, uint32_t n, const uint32_t
                               uint32_t* bad);
 __global__ void k_fuzzy_strengths(const uint32_t* indptr, const float* dists, uint32_t n,
                                   uint64_t* pay0);
+template <int SBM>
 __global__ void k_fz_l1_count(const uint32_t* indices, uint64_t m, uint32_t sshift, uint32_t ns,
                               uint32_t* stot);
+template <int SBM>
 __global__ void k_fz_l1_scatter(const uint64_t* pay0, const uint32_t* indices, uint64_t m,
                                 uint32_t sshift, uint32_t ns, const uint32_t* sstart,
                                 uint32_t* sfill, uint64_t* pay1, uint32_t* pj1);

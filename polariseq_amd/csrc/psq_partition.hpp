@@ -18,6 +18,7 @@
 namespace psq {
 
 constexpr int PART_SB_MAX = 64;      // most super-buckets
+constexpr int PART_SB_WIDE = 1024;   // most super-buckets of the wide level-1 kernels
 constexpr int PART_BPS_MAX = 4096;   // most buckets per super-bucket
 constexpr int PART_C1_CHUNK = 65536; // entries per block of a level-1 count
 constexpr int PART_P1_CHUNK = 4096;  // entries per block of a level-1 placement
@@ -120,15 +121,16 @@ __device__ inline void stage_partition(Load load, uint32_t cnt, uint32_t np, uin
 }
 
 // Super-bucket shift and bucket size for n target rows: super-buckets of
-// 2^sshift rows, at most PART_SB_MAX; buckets of 2^blog rows, blog >= blog_min
-// and at most PART_BPS_MAX buckets per super-bucket.
+// 2^sshift rows, at most sb_max (PART_SB_MAX, or PART_SB_WIDE for the wide
+// level-1 kernels); buckets of 2^blog rows, blog >= blog_min and at most
+// PART_BPS_MAX buckets per super-bucket.
 struct PartShape {
   uint32_t sshift, ns, blog, nb;
 };
-__host__ inline PartShape part_shape(uint64_t n, uint32_t blog_min) {
+__host__ inline PartShape part_shape(uint64_t n, uint32_t blog_min, uint32_t sb_max = PART_SB_MAX) {
   PartShape p;
   p.sshift = blog_min;
-  while (((n + (1ull << p.sshift) - 1) >> p.sshift) > (uint64_t)PART_SB_MAX) ++p.sshift;
+  while (((n + (1ull << p.sshift) - 1) >> p.sshift) > (uint64_t)sb_max) ++p.sshift;
   p.blog = p.sshift > blog_min + 12 ? p.sshift - 12 : blog_min;
   p.ns = (uint32_t)((n + (1ull << p.sshift) - 1) >> p.sshift);
   p.nb = (uint32_t)((n + (1ull << p.blog) - 1) >> p.blog);

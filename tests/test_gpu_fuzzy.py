@@ -169,3 +169,49 @@ def test_invariants_at_a_million_rows():
     t.sort_indices()
     assert np.array_equal(t.indptr, m.indptr) and np.array_equal(t.indices, m.indices)
     assert np.array_equal(t.data, m.data)
+
+
+@pytest.mark.timeout(600)
+def test_more_rows_than_the_narrow_partition_holds(orc):
+    """A circulant kNN graph over 2^26 + 4096 rows (row i -> i+1 .. i+8 mod n,
+    the same distances in every row), past the 64 super-buckets of 2^20 rows
+    of the narrow transpose: every row's fuzzy union is row i +- s with the
+    weight of offset s, which the oracle gives on a 64-row circulant graph."""
+    import torch
+    n, k = (1 << 26) + 4096, 8
+    dist = (0.5 + 0.25 * np.arange(1, k + 1)).astype(np.float32)
+
+    def circulant(nn):
+        off = np.arange(1, k + 1)
+        return ((np.arange(nn)[:, None] + off[None, :]) % nn).astype(np.uint32)
+
+    ns = 64
+    small = orc.fuzzy_connectivities(np.arange(ns + 1, dtype=np.uint32) * k, circulant(ns).ravel(),
+                                     np.tile(dist, ns), ns)
+    r = 32
+    sl = slice(int(small["indptr"][r]), int(small["indptr"][r + 1]))
+    want = {int(j) - r: w for j, w in zip(small["indices"][sl], small["data"][sl])}
+    assert sorted(want) == [s for s in range(-k, k + 1) if s != 0]
+    print(f"fuzzy large n={n}", flush=True)
+    dev = torch.device("cuda:0")
+    ar = torch.arange(n, dtype=torch.int64, device=dev)
+    ix = ((ar[:, None] + torch.arange(1, k + 1, device=dev)[None, :]) % n).to(torch.int32).reshape(-1)
+    ip = (torch.arange(n + 1, dtype=torch.int64, device=dev) * k).to(torch.int32)
+    dx = torch.from_numpy(dist).to(dev).repeat(n)
+    del ar
+    cip, cix, cw, nnz = P.fuzzy_connectivities_device(ip, ix, dx, n)
+    torch.cuda.synchronize()
+    assert nnz == 2 * k * n
+    rng = np.random.default_rng(3)
+    rows = np.concatenate([np.arange(0, 1024), np.arange(n - 2048, n), rng.integers(0, n, 2048)])
+    h_ip = cip.cpu().numpy().view(np.uint32).astype(np.int64)
+    assert np.array_equal(h_ip[rows], rows * 2 * k)
+    sel = (torch.from_numpy(rows).to(dev)[:, None] * (2 * k) + torch.arange(2 * k, device=dev)).reshape(-1)
+    gi = cix[sel].cpu().numpy().view(np.uint32).astype(np.int64).reshape(len(rows), 2 * k)
+    gw = cw[sel].cpu().numpy().reshape(len(rows), 2 * k)
+    for t, i in enumerate(rows):
+        assert (np.diff(gi[t]) > 0).all()
+        for j, w in zip(gi[t], gw[t]):
+            o = (j - i) % n
+            o = o - n if o > n // 2 else o
+            assert o in want and w.view(np.uint32) == want[o].view(np.uint32), (i, j, o)
