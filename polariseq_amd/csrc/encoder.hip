@@ -213,7 +213,9 @@ int psq_encoder_grad(const psq_encoder* e, const float* d_data, uint64_t n, uint
   ETRY(ebuf(me->partial, (size_t)nb * NP, &partial));
   ETRY(ebuf(me->grad, NP, &grad));
   EHIPC(hipMemcpy(d_rows, rows, 3 * B * 4, hipMemcpyHostToDevice));
-  hipLaunchKernelGGL(k_mlp_step, dim3(nb), dim3(128), 0, 0, d_data, d_rows, (uint32_t)B, nullptr, 0u,
+  EHIPC(hipFuncSetAttribute((const void*)k_mlp_step, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)mlp_step_smem()));
+  hipLaunchKernelGGL(k_mlp_step, dim3(nb), dim3(256), mlp_step_smem(), 0, d_data, d_rows, (uint32_t)B, nullptr, 0u,
                      nullptr, 0u, 0u, 0ull, 0ull, e->wr, e->mu, e->inv_scale, margin, partial, lt);
   EHIPC(hipGetLastError());
   hipLaunchKernelGGL(k_mlp_reduce_adam, dim3(blocks_for(NP, 32)), dim3(256), 0, 0, partial, nb,
@@ -299,8 +301,10 @@ int psq_encoder_train(psq_encoder* e, const float* d_data, uint64_t n, uint64_t 
   // Adam
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   double c1 = 1.0, c2 = 1.0;
+  EHIPC(hipFuncSetAttribute((const void*)k_mlp_step, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)mlp_step_smem()));
   for (uint64_t t = 0; t < steps; ++t) {
-    hipLaunchKernelGGL(k_mlp_step, dim3(nb), dim3(128), 0, s, d_x, nullptr, B, d_pool, (uint32_t)A,
+    hipLaunchKernelGGL(k_mlp_step, dim3(nb), dim3(256), mlp_step_smem(), s, d_x, nullptr, B, d_pool, (uint32_t)A,
                        d_knn, kp, (uint32_t)ns, o->seed, e->steps_done + t, e->wr, e->mu,
                        e->inv_scale, margin, partial, lt);
     c1 *= b1;

@@ -134,14 +134,27 @@ __global__ __launch_bounds__(ENC_WAVES * 64) void k_encode_mfma(
 }
 
 // ── one training step in two launches ──
-// k_mlp_step: a block of MLP_STEP_T triplets (3 T rows), forward, triplet loss,
-// backward, its partial gradient (N_PARAMS floats per block). Triplets come from
-// `rows` ([anchors | positives | negatives] of the whole batch) or, when rows is
-// NULL, are sampled here: anchor = pool[hash % A] (cells whose exact top-kk is
-// in knn), positive uniform over that top-kk, negative uniform over the subset.
-// k_mlp_reduce_adam: one thread per parameter sums the blocks' partials in
-// block order and takes the Adam step (deterministic for a seed).
+// k_mlp_step: a block of MLP_STEP_T triplets (3 T = 48 rows: three 16-row
+// tiles), forward, triplet loss, backward, its partial gradient (N_PARAMS
+// floats per block). Triplets come from `rows` ([anchors | positives |
+// negatives] of the whole batch) or, when rows is NULL, are sampled here:
+// anchor = pool[hash % A] (cells whose exact top-kk is in knn), positive
+// uniform over that top-kk, negative uniform over the subset.
+// The five products run on the FP32 matrix cores (v_mfma_f32_16x16x4f32, the
+// operand layout of k_encode_mfma: lane l, c = l % 16, g = l / 16, holds A[c][g]
+// and B[g][c], and D[4g + i][c] is its i-th output), with FP32 activations and
+// the BF16-valued weights:
+//   H_pre = X W1^T + b1      (48 x 128, k = 52)      Z = gelu(H_pre) W2^T + b2 (48 x 32, k = 128)
+//   dW2 = dZ^T gelu(H_pre)   (32 x 128, k = 48)      dH = dZ W2              (48 x 128, k = 32)
+//   dW1 = dH_pre^T X         (128 x 52, k = 48),     dH_pre = dH * gelu'(H_pre)
+// Activations live in LDS at row strides of 20 mod 64 words, so the 16 rows of
+// a tile read by c fall on distinct banks. Four waves take the tiles of each
+// product in turn.
+// k_mlp_reduce_adam: sums the blocks' partials in a fixed order and takes the
+// Adam step (deterministic for a seed).
 constexpr int STEP_R = 3 * MLP_STEP_T;  // rows per block
+constexpr int SX = 84, SH = 148, SZ = 52;  // LDS row strides (floats)
+constexpr size_t STEP_SMEM = (size_t)STEP_R * (SX + 2 * SH + SZ) * sizeof(float) + STEP_R * 4;
 
 __device__ inline uint32_t step_row(const uint32_t* __restrict__ rows, uint32_t B, uint32_t b,
                                     uint32_t role, const uint32_t* __restrict__ pool, uint32_t A,
@@ -155,57 +168,65 @@ __device__ inline uint32_t step_row(const uint32_t* __restrict__ rows, uint32_t 
   return (uint32_t)hash_index(seed, key, 3, ns);
 }
 
-__global__ __launch_bounds__(128) void k_mlp_step(
+__global__ __launch_bounds__(256) void k_mlp_step(
     const float* __restrict__ X, const uint32_t* __restrict__ rows, uint32_t B,
     const uint32_t* __restrict__ pool, uint32_t A, const uint32_t* __restrict__ knn, uint32_t kk,
     uint32_t ns, uint64_t seed, uint64_t step, const float* __restrict__ pr, float mu,
     float inv_scale, float margin, float* __restrict__ partial, float* __restrict__ loss_terms) {
-  __shared__ float s_x[STEP_R][MLP_IN + 1];
-  __shared__ float s_hp[STEP_R][MLP_H + 1];  // h_pre, then dL/dh_pre
-  __shared__ float s_h[STEP_R][MLP_H + 1];   // gelu(h_pre)
-  __shared__ float s_z[STEP_R][MLP_OUT + 1];  // z, then dL/dz
-  __shared__ uint32_t s_row[STEP_R];
-  const uint32_t tid = threadIdx.x;
+  extern __shared__ float step_lds[];
+  float* s_x = step_lds;                  // [48][SX]: x standardised, columns 50..63 zero
+  float* s_hp = s_x + STEP_R * SX;        // [48][SH]: h_pre, then dL/dh_pre
+  float* s_h = s_hp + STEP_R * SH;        // [48][SH]: gelu(h_pre)
+  float* s_z = s_h + STEP_R * SH;         // [48][SZ]: z, then dL/dz
+  uint32_t* s_row = reinterpret_cast<uint32_t*>(s_z + STEP_R * SZ);
+  const uint32_t tid = threadIdx.x, l = tid % 64, w = tid / 64, c = l % 16, g = l / 16;
   const uint32_t b0 = blockIdx.x * MLP_STEP_T;
   if (tid < (uint32_t)STEP_R) {
     const uint32_t role = tid / MLP_STEP_T, b = b0 + tid % MLP_STEP_T;
     s_row[tid] = b < B ? step_row(rows, B, b, role, pool, A, knn, kk, ns, seed, step) : 0xFFFFFFFFu;
   }
   __syncthreads();
-  for (uint32_t e = tid; e < (uint32_t)(STEP_R * MLP_IN); e += blockDim.x) {
-    const uint32_t rr = e / MLP_IN, t = e % MLP_IN;
+  for (uint32_t e = tid; e < (uint32_t)(STEP_R * 64); e += blockDim.x) {
+    const uint32_t rr = e / 64, t = e % 64;
     const uint32_t src = s_row[rr];
-    s_x[rr][t] = src != 0xFFFFFFFFu ? (X[(size_t)src * MLP_IN + t] - mu) * inv_scale : 0.0f;
+    s_x[rr * SX + t] =
+        (t < (uint32_t)MLP_IN && src != 0xFFFFFFFFu) ? (X[(size_t)src * MLP_IN + t] - mu) * inv_scale : 0.0f;
   }
   __syncthreads();
-  // layer 1: thread j = hidden unit j for every row
-  const uint32_t j = tid;
-  {
-    float w1[MLP_IN];
+  const float* W1 = pr + OFF_W1;
+  const float* W2 = pr + OFF_W2;
+  // layer 1: 3 x 8 tiles of (16 rows, 16 hidden units)
+  for (uint32_t tile = w; tile < 24; tile += 4) {
+    const uint32_t rt = tile / 8, ht = tile % 8, hj = 16 * ht + c;
+    f32x4 acc;
 #pragma unroll
-    for (int t = 0; t < MLP_IN; ++t) w1[t] = pr[OFF_W1 + j * MLP_IN + t];
-    const float b1 = pr[OFF_B1 + j];
-#pragma unroll 2
-    for (int rr = 0; rr < STEP_R; ++rr) {
-      float acc = b1;
+    for (int i = 0; i < 4; ++i) acc[i] = pr[OFF_B1 + hj];
 #pragma unroll
-      for (int t = 0; t < MLP_IN; ++t) acc = fmaf(w1[t], s_x[rr][t], acc);
-      s_hp[rr][j] = acc;
-      s_h[rr][j] = gelu(acc);
+    for (int s = 0; s < ENC_KS1; ++s) {
+      const uint32_t t = 4 * s + g;
+      const float wv = t < (uint32_t)MLP_IN ? W1[hj * MLP_IN + t] : 0.0f;
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(s_x[(16 * rt + c) * SX + t], wv, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t r = 16 * rt + 4 * g + i;
+      s_hp[r * SH + hj] = acc[i];
+      s_h[r * SH + hj] = gelu(acc[i]);
     }
   }
   __syncthreads();
-  // layer 2: thread (o, g) = output o for rows g, g + 4, ...
-  const uint32_t o = tid % MLP_OUT, g = tid / MLP_OUT;
-  {
-    const float* w2 = pr + OFF_W2 + o * MLP_H;
-    const float b2 = pr[OFF_B2 + o];
-    for (int rr = g; rr < STEP_R; rr += 4) {
-      float acc = b2;
+  // layer 2: 3 x 2 tiles of (16 rows, 16 outputs)
+  for (uint32_t tile = w; tile < 6; tile += 4) {
+    const uint32_t rt = tile / 2, ot = tile % 2, oc = 16 * ot + c;
+    f32x4 acc;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[i] = pr[OFF_B2 + oc];
 #pragma unroll 8
-      for (int t = 0; t < MLP_H; ++t) acc = fmaf(w2[t], s_h[rr][t], acc);
-      s_z[rr][o] = acc;
-    }
+    for (int s = 0; s < MLP_H / 4; ++s)
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(s_h[(16 * rt + c) * SH + 4 * s + g],
+                                                 W2[oc * MLP_H + 4 * s + g], acc, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) s_z[(16 * rt + 4 * g + i) * SZ + oc] = acc[i];
   }
   __syncthreads();
   // loss and dL/dz: 4 lanes per triplet (8 outputs each), written over z
@@ -216,11 +237,11 @@ __global__ __launch_bounds__(128) void k_mlp_step(
     const bool live = mine && b0 + tt < B;
     float za[8], zp[8], zn[8];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      za[c] = mine ? s_z[tt][part * 8 + c] : 0.0f;
-      zp[c] = mine ? s_z[MLP_STEP_T + tt][part * 8 + c] : 0.0f;
-      zn[c] = mine ? s_z[2 * MLP_STEP_T + tt][part * 8 + c] : 0.0f;
-      const float u = za[c] - zp[c], v = za[c] - zn[c];
+    for (int q = 0; q < 8; ++q) {
+      za[q] = mine ? s_z[tt * SZ + part * 8 + q] : 0.0f;
+      zp[q] = mine ? s_z[(MLP_STEP_T + tt) * SZ + part * 8 + q] : 0.0f;
+      zn[q] = mine ? s_z[(2 * MLP_STEP_T + tt) * SZ + part * 8 + q] : 0.0f;
+      const float u = za[q] - zp[q], v = za[q] - zn[q];
       dap = fmaf(u, u, dap);
       dan = fmaf(v, v, dan);
     }
@@ -231,67 +252,78 @@ __global__ __launch_bounds__(128) void k_mlp_step(
     dan += __shfl_xor(dan, 2);
     __syncthreads();  // every z read before dz overwrites it
     if (mine) {
-      const float l = dap - dan + margin;
-      const bool active = live && l > 0.0f;
-      if (live && part == 0) loss_terms[b0 + tt] = active ? l : 0.0f;
+      const float lv = dap - dan + margin;
+      const bool active = live && lv > 0.0f;
+      if (live && part == 0) loss_terms[b0 + tt] = active ? lv : 0.0f;
       const float sc = active ? 2.0f / (float)B : 0.0f;
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        s_z[tt][part * 8 + c] = sc * (zn[c] - zp[c]);
-        s_z[MLP_STEP_T + tt][part * 8 + c] = -sc * (za[c] - zp[c]);
-        s_z[2 * MLP_STEP_T + tt][part * 8 + c] = sc * (za[c] - zn[c]);
+      for (int q = 0; q < 8; ++q) {
+        s_z[tt * SZ + part * 8 + q] = sc * (zn[q] - zp[q]);
+        s_z[(MLP_STEP_T + tt) * SZ + part * 8 + q] = -sc * (za[q] - zp[q]);
+        s_z[(2 * MLP_STEP_T + tt) * SZ + part * 8 + q] = sc * (za[q] - zn[q]);
       }
     }
   }
   __syncthreads();
   float* out = partial + (size_t)blockIdx.x * N_PARAMS;
-  // dW2[o][j] = sum_r dz[r][o] h[r][j]; dh_pre[r][j] = (sum_o W2[o][j] dz[r][o]) gelu'(h_pre)
-  {
-    float w2c[MLP_OUT];
+  // dW2 = dZ^T gelu(H_pre): 2 x 8 tiles of (16 outputs, 16 hidden), k over the 48 rows
+  for (uint32_t tile = w; tile < 16; tile += 4) {
+    const uint32_t ot = tile / 8, jt = tile % 8;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int q = 0; q < MLP_OUT; ++q) w2c[q] = pr[OFF_W2 + q * MLP_H + j];
-    float dw2[MLP_OUT];
-#pragma unroll
-    for (int q = 0; q < MLP_OUT; ++q) dw2[q] = 0.0f;
-    float db1 = 0.0f;
-#pragma unroll 1
-    for (int rr = 0; rr < STEP_R; ++rr) {
-      const float h = s_h[rr][j];
-      float dh = 0.0f;
-#pragma unroll
-      for (int q = 0; q < MLP_OUT; ++q) {
-        const float dzv = s_z[rr][q];
-        dh = fmaf(w2c[q], dzv, dh);
-        dw2[q] = fmaf(dzv, h, dw2[q]);
-      }
-      const float dp = dh * gelu_grad(s_hp[rr][j]);
-      db1 += dp;
-      s_hp[rr][j] = dp;
+    for (int s = 0; s < STEP_R / 4; ++s) {
+      const uint32_t r = 4 * s + g;
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(s_z[r * SZ + 16 * ot + c], s_h[r * SH + 16 * jt + c], acc,
+                                                 0, 0, 0);
     }
 #pragma unroll
-    for (int q = 0; q < MLP_OUT; ++q) out[OFF_W2 + q * MLP_H + j] = dw2[q];
-    out[OFF_B1 + j] = db1;
+    for (int i = 0; i < 4; ++i) out[OFF_W2 + (16 * ot + 4 * g + i) * MLP_H + 16 * jt + c] = acc[i];
   }
-  // dW1[j][t] = sum_r dh_pre[r][j] x[r][t] (thread j's own column of s_hp: no barrier)
-  {
-    float dw1[MLP_IN];
+  // dH = dZ W2, then dL/dh_pre = dH * gelu'(h_pre) in place of h_pre: 3 x 8 tiles, k over 32 outputs
+  for (uint32_t tile = w; tile < 24; tile += 4) {
+    const uint32_t rt = tile / 8, jt = tile % 8, hj = 16 * jt + c;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int t = 0; t < MLP_IN; ++t) dw1[t] = 0.0f;
-#pragma unroll 1
-    for (int rr = 0; rr < STEP_R; ++rr) {
-      const float dp = s_hp[rr][j];
+    for (int s = 0; s < MLP_OUT / 4; ++s)
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(s_z[(16 * rt + c) * SZ + 4 * s + g],
+                                                 W2[(4 * s + g) * MLP_H + hj], acc, 0, 0, 0);
 #pragma unroll
-      for (int t = 0; t < MLP_IN; ++t) dw1[t] = fmaf(dp, s_x[rr][t], dw1[t]);
+    for (int i = 0; i < 4; ++i) {
+      float* hp = s_hp + (16 * rt + 4 * g + i) * SH + hj;
+      *hp = acc[i] * gelu_grad(*hp);
     }
-#pragma unroll
-    for (int t = 0; t < MLP_IN; ++t) out[OFF_W1 + j * MLP_IN + t] = dw1[t];
   }
-  if (j < (uint32_t)MLP_OUT) {
+  __syncthreads();
+  // biases: column sums over the 48 rows, in row order
+  if (tid < (uint32_t)MLP_H) {
     float acc = 0.0f;
-    for (int rr = 0; rr < STEP_R; ++rr) acc += s_z[rr][j];
-    out[OFF_B2 + j] = acc;
+    for (int r = 0; r < STEP_R; ++r) acc += s_hp[r * SH + tid];
+    out[OFF_B1 + tid] = acc;
+  } else if (tid < (uint32_t)(MLP_H + MLP_OUT)) {
+    const uint32_t o = tid - MLP_H;
+    float acc = 0.0f;
+    for (int r = 0; r < STEP_R; ++r) acc += s_z[r * SZ + o];
+    out[OFF_B2 + o] = acc;
+  }
+  // dW1 = dH_pre^T X: 8 x 4 tiles of (16 hidden, 16 inputs), k over the 48 rows
+  for (uint32_t tile = w; tile < 32; tile += 4) {
+    const uint32_t jt = tile / 4, tt = tile % 4;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int s = 0; s < STEP_R / 4; ++s) {
+      const uint32_t r = 4 * s + g;
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(s_hp[r * SH + 16 * jt + c], s_x[r * SX + 16 * tt + c], acc,
+                                                 0, 0, 0);
+    }
+    const uint32_t t = 16 * tt + c;
+    if (t < (uint32_t)MLP_IN) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) out[OFF_W1 + (16 * jt + 4 * g + i) * MLP_IN + t] = acc[i];
+    }
   }
 }
+
+size_t mlp_step_smem() { return STEP_SMEM; }
 
 // Block of 256 threads = 32 parameters x 8 groups: group g sums the partials
 // of blocks g, g + 8, ... in order, then the 8 group sums are added in a fixed
