@@ -215,7 +215,7 @@ int psq_encoder_grad(const psq_encoder* e, const float* d_data, uint64_t n, uint
   EHIPC(hipMemcpy(d_rows, rows, 3 * B * 4, hipMemcpyHostToDevice));
   EHIPC(hipFuncSetAttribute((const void*)k_mlp_step, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)mlp_step_smem()));
-  hipLaunchKernelGGL(k_mlp_step, dim3(nb), dim3(256), mlp_step_smem(), 0, d_data, d_rows, (uint32_t)B, nullptr, 0u,
+  hipLaunchKernelGGL(k_mlp_step, dim3(nb), dim3(64 * MLP_STEP_WAVES), mlp_step_smem(), 0, d_data, d_rows, (uint32_t)B, nullptr, 0u,
                      nullptr, 0u, 0u, 0ull, 0ull, e->wr, e->mu, e->inv_scale, margin, partial, lt);
   EHIPC(hipGetLastError());
   hipLaunchKernelGGL(k_mlp_reduce_adam, dim3(blocks_for(NP, 32)), dim3(256), 0, 0, partial, nb,
@@ -304,7 +304,7 @@ int psq_encoder_train(psq_encoder* e, const float* d_data, uint64_t n, uint64_t 
   EHIPC(hipFuncSetAttribute((const void*)k_mlp_step, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)mlp_step_smem()));
   for (uint64_t t = 0; t < steps; ++t) {
-    hipLaunchKernelGGL(k_mlp_step, dim3(nb), dim3(256), mlp_step_smem(), s, d_x, nullptr, B, d_pool, (uint32_t)A,
+    hipLaunchKernelGGL(k_mlp_step, dim3(nb), dim3(64 * MLP_STEP_WAVES), mlp_step_smem(), s, d_x, nullptr, B, d_pool, (uint32_t)A,
                        d_knn, kp, (uint32_t)ns, o->seed, e->steps_done + t, e->wr, e->mu,
                        e->inv_scale, margin, partial, lt);
     c1 *= b1;

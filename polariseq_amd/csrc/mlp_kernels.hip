@@ -148,12 +148,13 @@ __global__ __launch_bounds__(ENC_WAVES * 64) void k_encode_mfma(
 //   dW2 = dZ^T gelu(H_pre)   (32 x 128, k = 48)      dH = dZ W2              (48 x 128, k = 32)
 //   dW1 = dH_pre^T X         (128 x 52, k = 48),     dH_pre = dH * gelu'(H_pre)
 // Activations live in LDS at row strides of 20 mod 64 words, so the 16 rows of
-// a tile read by c fall on distinct banks. Four waves take the tiles of each
-// product in turn.
+// a tile read by c fall on distinct banks. The block's STEP_WAVES waves take
+// the tiles of each product in turn.
 // k_mlp_reduce_adam: sums the blocks' partials in a fixed order and takes the
 // Adam step (deterministic for a seed).
 constexpr int STEP_R = 3 * MLP_STEP_T;  // rows per block
 constexpr int SX = 84, SH = 148, SZ = 52;  // LDS row strides (floats)
+constexpr int STEP_WAVES = MLP_STEP_WAVES;
 constexpr size_t STEP_SMEM = (size_t)STEP_R * (SX + 2 * SH + SZ) * sizeof(float) + STEP_R * 4;
 
 __device__ inline uint32_t step_row(const uint32_t* __restrict__ rows, uint32_t B, uint32_t b,
@@ -168,7 +169,7 @@ __device__ inline uint32_t step_row(const uint32_t* __restrict__ rows, uint32_t 
   return (uint32_t)hash_index(seed, key, 3, ns);
 }
 
-__global__ __launch_bounds__(256) void k_mlp_step(
+__global__ __launch_bounds__(64 * STEP_WAVES) void k_mlp_step(
     const float* __restrict__ X, const uint32_t* __restrict__ rows, uint32_t B,
     const uint32_t* __restrict__ pool, uint32_t A, const uint32_t* __restrict__ knn, uint32_t kk,
     uint32_t ns, uint64_t seed, uint64_t step, const float* __restrict__ pr, float mu,
@@ -196,7 +197,7 @@ __global__ __launch_bounds__(256) void k_mlp_step(
   const float* W1 = pr + OFF_W1;
   const float* W2 = pr + OFF_W2;
   // layer 1: 3 x 8 tiles of (16 rows, 16 hidden units)
-  for (uint32_t tile = w; tile < 24; tile += 4) {
+  for (uint32_t tile = w; tile < 24; tile += STEP_WAVES) {
     const uint32_t rt = tile / 8, ht = tile % 8, hj = 16 * ht + c;
     f32x4 acc;
 #pragma unroll
@@ -216,7 +217,7 @@ __global__ __launch_bounds__(256) void k_mlp_step(
   }
   __syncthreads();
   // layer 2: 3 x 2 tiles of (16 rows, 16 outputs)
-  for (uint32_t tile = w; tile < 6; tile += 4) {
+  for (uint32_t tile = w; tile < 6; tile += STEP_WAVES) {
     const uint32_t rt = tile / 2, ot = tile % 2, oc = 16 * ot + c;
     f32x4 acc;
 #pragma unroll
@@ -267,7 +268,7 @@ __global__ __launch_bounds__(256) void k_mlp_step(
   __syncthreads();
   float* out = partial + (size_t)blockIdx.x * N_PARAMS;
   // dW2 = dZ^T gelu(H_pre): 2 x 8 tiles of (16 outputs, 16 hidden), k over the 48 rows
-  for (uint32_t tile = w; tile < 16; tile += 4) {
+  for (uint32_t tile = w; tile < 16; tile += STEP_WAVES) {
     const uint32_t ot = tile / 8, jt = tile % 8;
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -280,7 +281,7 @@ __global__ __launch_bounds__(256) void k_mlp_step(
     for (int i = 0; i < 4; ++i) out[OFF_W2 + (16 * ot + 4 * g + i) * MLP_H + 16 * jt + c] = acc[i];
   }
   // dH = dZ W2, then dL/dh_pre = dH * gelu'(h_pre) in place of h_pre: 3 x 8 tiles, k over 32 outputs
-  for (uint32_t tile = w; tile < 24; tile += 4) {
+  for (uint32_t tile = w; tile < 24; tile += STEP_WAVES) {
     const uint32_t rt = tile / 8, jt = tile % 8, hj = 16 * jt + c;
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -306,7 +307,7 @@ __global__ __launch_bounds__(256) void k_mlp_step(
     out[OFF_B2 + o] = acc;
   }
   // dW1 = dH_pre^T X: 8 x 4 tiles of (16 hidden, 16 inputs), k over the 48 rows
-  for (uint32_t tile = w; tile < 32; tile += 4) {
+  for (uint32_t tile = w; tile < 32; tile += STEP_WAVES) {
     const uint32_t jt = tile / 4, tt = tile % 4;
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
