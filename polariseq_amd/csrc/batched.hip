@@ -332,7 +332,7 @@ extern "C" int psq_knn_exact_batched_device(const float* d_data, uint64_t n_tota
     BTRY(dalloc(bsi, (size_t)h_novf * k, &si));
     hipLaunchKernelGGL(k_exact_fallback, dim3(gsize(h_novf, 64)), dim3(64), 0, s, Xp, (uint32_t)n_total,
                        (uint32_t)d, dp, nullptr, (uint32_t)k, T, ovf, h_novf, sd, si, d_idx, d_dist, d_off,
-                       (uint32_t)n_samples);
+                       (uint32_t)n_samples, 1.0);
     BHIPC(hipGetLastError());
   }
   BHIPC(hipEventRecord(e3, s));

@@ -1199,6 +1199,11 @@ int set_device(int32_t device) {
 // segment and at least k/16 segments, a radix-select tau per query, pass 2 into
 // a candidate buffer of 2k + 256, an LDS sort per query; queries in chunks that
 // bound the pass-1 and candidate buffers.
+// The fallback's FP32 VALU distance is within (d + 4) 2^-24 of the exact one,
+// relative: the matrix-core threshold scaled by this bounds it for every true
+// neighbour.
+double exact_fallback_scale(uint64_t d) { return 1.0 + 4.0 * ((double)d + 8.0) * 5.9604644775390625e-08; }
+
 int knn_exact_big(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d_queries,
                   uint64_t nq, uint64_t k, uint32_t* d_idx, float* d_dist, hipStream_t s) {
   Workspace* ws;
@@ -1211,7 +1216,8 @@ int knn_exact_big(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d
   // points keep 64 per segment (and at most one segment per 64 points) so that
   // the union of the lists still holds k values.
   const uint32_t K = n >= 64 * k ? 16u : (n >= 32 * k ? 32u : 64u);
-  uint32_t nseg = (uint32_t)((k + 15) / 16 * 128 / K);
+  // every segment gives each query four lists (one per lane group of pass 1)
+  uint32_t nseg = (uint32_t)((k + 15) / 16 * 128 / K / 4);
   if (K == 64) nseg = (uint32_t)std::min<uint64_t>((n + 63) / 64, nseg);
   nseg = std::max<uint32_t>(nseg, 1);
   const uint32_t seg_len = round_up((uint32_t)((n + nseg - 1) / nseg), 64);
@@ -1220,13 +1226,16 @@ int knn_exact_big(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d
   uint32_t pmax = 1;
   while (pmax < cap) pmax <<= 1;
   // chunk of queries: pass-1 lists and candidates within about 4 GiB
-  const uint64_t per_q = (uint64_t)nseg * K * 4 + (uint64_t)cap * 4 + 16;
+  const uint32_t L = nseg * 4 * K;  // pass-1 values per query
+  const uint64_t per_q = (uint64_t)L * 4 + (uint64_t)cap * 4 + 16;
   const uint32_t qc = (uint32_t)std::max<uint64_t>(
       EX_BIG_QMIN, std::min<uint64_t>(nq, (4ull << 30) / per_q / 256 * 256));
-  float *Xp, *partial, *T;
-  uint32_t *cnt, *cand, *ovf, *novf, *qidx = nullptr;
+  float *Xp, *partial, *T, *xn;
+  uint32_t *cnt, *cand, *ovf, *novf, *pmx, *qidx = nullptr;
   TRY(wbuf(ws, "Xp", (size_t)n * dp, &Xp));
-  TRY(wbuf(ws, "exb_partial", (size_t)qc * nseg * K, &partial));
+  TRY(wbuf(ws, "ex_xn", n, &xn));
+  TRY(wbuf(ws, "ex_pmax", 1, &pmx));
+  TRY(wbuf(ws, "exb_partial", (size_t)qc * L, &partial));
   TRY(wbuf(ws, "ex_T", qc, &T));
   TRY(wbuf(ws, "ex_cnt", qc, &cnt));
   TRY(wbuf(ws, "exb_cand", (size_t)qc * cap, &cand));
@@ -1236,20 +1245,25 @@ int knn_exact_big(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d
   hipLaunchKernelGGL(k_pad_rows, dim3(std::min<unsigned>(grid_for((size_t)n * dp, 256), 65536)),
                      dim3(256), 0, s, d_data, (uint32_t)n, (uint32_t)d, dp, Xp);
   HIPC(hipGetLastError());
-  const size_t smem = (size_t)64 * dp * sizeof(float);
+  TRY(fill_u32(pmx, 1, 0, s));
+  hipLaunchKernelGGL(k_exact_norms, dim3(grid_for(n, 256)), dim3(256), 0, s, Xp, (uint32_t)n, (uint32_t)d,
+                     dp, xn, pmx);
+  HIPC(hipGetLastError());
+  const size_t smem = exm_smem((uint32_t)d);
   const uint32_t dfix = d == 50 ? 50u : (d == 30 ? 30u : 0u);
-  void (*p1)(const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t, uint32_t,
-             uint32_t, float*) =
+  void (*p1)(const float*, const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t,
+             uint32_t, uint32_t, float*) =
       K == 16   ? (dfix == 50 ? k_exact_pass1<50, 16> : (dfix == 30 ? k_exact_pass1<30, 16> : k_exact_pass1<0, 16>))
       : K == 32 ? (dfix == 50 ? k_exact_pass1<50, 32> : (dfix == 30 ? k_exact_pass1<30, 32> : k_exact_pass1<0, 32>))
                 : (dfix == 50 ? k_exact_pass1<50, 64> : (dfix == 30 ? k_exact_pass1<30, 64> : k_exact_pass1<0, 64>));
-  void (*p2)(const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t, uint32_t,
-             const float*, uint32_t*, uint32_t*, uint32_t) = dfix == 50 ? k_exact_pass2<50> : (dfix == 30 ? k_exact_pass2<30> : k_exact_pass2<0>);
+  void (*p2)(const float*, const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t,
+             uint32_t, const float*, uint32_t*, uint32_t*, uint32_t) =
+      dfix == 50 ? k_exact_pass2<50> : (dfix == 30 ? k_exact_pass2<30> : k_exact_pass2<0>);
   if (smem > 65536) {
     HIPC(hipFuncSetAttribute((const void*)p1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     HIPC(hipFuncSetAttribute((const void*)p2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
   }
-  const size_t tau_smem = (size_t)nseg * K * 4;
+  const size_t tau_smem = (size_t)L * 4;
   if (tau_smem > 65536)
     HIPC(hipFuncSetAttribute((const void*)k_exact_tau_big, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)tau_smem));
@@ -1264,16 +1278,16 @@ int knn_exact_big(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d
       hipLaunchKernelGGL(k_iota_u32, dim3(grid_for(qn, 256)), dim3(256), 0, s, qidx, (uint32_t)c0, qn);
       HIPC(hipGetLastError());
     }
-    const uint32_t qblocks = (qn + 255) / 256;
-    hipLaunchKernelGGL(p1, dim3(qblocks, nseg), dim3(256), smem, s, Xp, (uint32_t)n, (uint32_t)d, dp,
+    const uint32_t qblocks = (qn + EXM_QUERIES - 1) / EXM_QUERIES;
+    hipLaunchKernelGGL(p1, dim3(qblocks, nseg), dim3(256), smem, s, Xp, xn, (uint32_t)n, (uint32_t)d, dp,
                        qp, qn, seg_len, nseg, partial);
     HIPC(hipGetLastError());
-    hipLaunchKernelGGL(k_exact_tau_big, dim3(qn), dim3(256), tau_smem, s, partial, qn, nseg, K,
-                       (uint32_t)k, (uint32_t)d, T);
+    hipLaunchKernelGGL(k_exact_tau_big, dim3(qn), dim3(256), tau_smem, s, partial, qn, L, (uint32_t)k,
+                       (uint32_t)d, qp, xn, pmx, T);
     HIPC(hipGetLastError());
     TRY(fill_u32(cnt, qn, 0, s));
     TRY(fill_u32(novf, 1, 0, s));
-    hipLaunchKernelGGL(p2, dim3(qblocks, nseg), dim3(256), smem, s, Xp, (uint32_t)n, (uint32_t)d, dp,
+    hipLaunchKernelGGL(p2, dim3(qblocks, nseg), dim3(256), smem, s, Xp, xn, (uint32_t)n, (uint32_t)d, dp,
                        qp, qn, seg_len, T, cnt, cand, cap);
     HIPC(hipGetLastError());
     uint32_t* io = d_idx + c0 * k;
@@ -1291,7 +1305,7 @@ int knn_exact_big(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d
       TRY(wbuf(ws, "ex_fb_i", (size_t)h_novf * k, &si));
       hipLaunchKernelGGL(k_exact_fallback, dim3(grid_for(h_novf, 64)), dim3(64), 0, s, Xp,
                          (uint32_t)n, (uint32_t)d, dp, qp, (uint32_t)k, T, ovf, h_novf, sd, si, io, dq,
-                         nullptr, 0u);
+                         nullptr, 0u, exact_fallback_scale(d));
       HIPC(hipGetLastError());
       HIPC(hipStreamSynchronize(s));
     }
@@ -1327,7 +1341,7 @@ constexpr uint32_t EX_CAND_CAP = 128;  // pass-2 candidates per query (k <= 64 p
 // 64 points per segment. Any count gives the same result (each segment keeps
 // K >= k values, so the k-th of the union bounds the true k-th from above).
 uint32_t exact_small_nseg(uint64_t n, uint64_t nq) {
-  const uint32_t qblocks = (uint32_t)((nq + 255) / 256);
+  const uint32_t qblocks = (uint32_t)((nq + EXM_QUERIES - 1) / EXM_QUERIES);
   return std::max<uint32_t>(1, (uint32_t)std::min<uint64_t>((n + 63) / 64, (2048 + qblocks - 1) / qblocks));
 }
 uint32_t exact_small_K(uint64_t k) { return k <= 16 ? 16 : (k <= 32 ? 32 : 64); }
@@ -1336,7 +1350,8 @@ uint32_t exact_small_K(uint64_t k) { return k <= 16 ? 16 : (k <= 32 ? 32 : 64); 
 // fallback's lists reserved for every query.
 size_t exact_small_bytes(uint64_t n, uint64_t d, uint64_t nq, uint64_t k, uint32_t nseg) {
   const size_t dp = round_up((uint32_t)d, 32);
-  return BufSrc::span(n * dp, 4) + BufSrc::span(nq * nseg * exact_small_K(k), 4) +
+  return BufSrc::span(n * dp, 4) + BufSrc::span(n, 4) + BufSrc::span(1, 4) +
+         BufSrc::span(nq * nseg * 4 * exact_small_K(k), 4) +
          3 * BufSrc::span(nq, 4) + BufSrc::span(nq * EX_CAND_CAP, 4) + BufSrc::span(1, 4) +
          BufSrc::span(nq * k, 8) + BufSrc::span(nq * k, 4);
 }
@@ -1350,14 +1365,16 @@ int knn_exact_small(BufSrc& B, const float* d_data, uint64_t n, uint64_t d, cons
   const uint32_t dp = round_up((uint32_t)d, 32);
   const uint32_t K = exact_small_K(k);
   const uint32_t cap = EX_CAND_CAP;
-  const uint32_t qblocks = (uint32_t)((nq + 255) / 256);
+  const uint32_t qblocks = (uint32_t)((nq + EXM_QUERIES - 1) / EXM_QUERIES);
   uint32_t nseg = std::max<uint32_t>(1, nseg_in);
   uint32_t seg_len = round_up((uint32_t)((n + nseg - 1) / nseg), 64);
   nseg = (uint32_t)((n + seg_len - 1) / seg_len);
-  float *Xp, *partial, *T;
-  uint32_t *cnt, *cand, *ovf, *novf;
+  float *Xp, *partial, *T, *xn;
+  uint32_t *cnt, *cand, *ovf, *novf, *pmx;
   TRY(B.take("Xp", (size_t)n * dp, &Xp));
-  TRY(B.take("ex_partial", (size_t)nq * nseg * K, &partial));
+  TRY(B.take("ex_xn", n, &xn));
+  TRY(B.take("ex_pmax", 1, &pmx));
+  TRY(B.take("ex_partial", (size_t)nq * nseg * 4 * K, &partial));
   TRY(B.take("ex_T", nq, &T));
   TRY(B.take("ex_cnt", nq, &cnt));
   TRY(B.take("ex_cand", (size_t)nq * cap, &cand));
@@ -1368,25 +1385,30 @@ int knn_exact_small(BufSrc& B, const float* d_data, uint64_t n, uint64_t d, cons
   HIPC(hipGetLastError());
   TRY(fill_u32(cnt, nq, 0, s));
   TRY(fill_u32(novf, 1, 0, s));
-  const size_t smem = (size_t)64 * dp * sizeof(float);
+  TRY(fill_u32(pmx, 1, 0, s));
+  hipLaunchKernelGGL(k_exact_norms, dim3(grid_for(n, 256)), dim3(256), 0, s, Xp, (uint32_t)n, (uint32_t)d,
+                     dp, xn, pmx);
+  HIPC(hipGetLastError());
+  const size_t smem = exm_smem((uint32_t)d);
   const uint32_t dfix = d == 50 ? 50u : (d == 30 ? 30u : 0u);
   dim3 g1(qblocks, nseg);
-  void (*p1)(const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t, uint32_t,
-             uint32_t, float*);
+  void (*p1)(const float*, const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t,
+             uint32_t, uint32_t, float*);
   if (K == 16) p1 = dfix == 50 ? k_exact_pass1<50, 16> : (dfix == 30 ? k_exact_pass1<30, 16> : k_exact_pass1<0, 16>);
   else if (K == 32) p1 = dfix == 50 ? k_exact_pass1<50, 32> : (dfix == 30 ? k_exact_pass1<30, 32> : k_exact_pass1<0, 32>);
   else p1 = dfix == 50 ? k_exact_pass1<50, 64> : (dfix == 30 ? k_exact_pass1<30, 64> : k_exact_pass1<0, 64>);
   if (smem > 65536) HIPC(hipFuncSetAttribute((const void*)p1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-  hipLaunchKernelGGL(p1, g1, dim3(256), smem, s, Xp, (uint32_t)n, (uint32_t)d, dp, d_queries,
+  hipLaunchKernelGGL(p1, g1, dim3(256), smem, s, Xp, xn, (uint32_t)n, (uint32_t)d, dp, d_queries,
                      (uint32_t)nq, seg_len, nseg, partial);
   HIPC(hipGetLastError());
   hipLaunchKernelGGL(k_exact_tau, dim3(grid_for(nq, 128)), dim3(128), 0, s, partial, (uint32_t)nq,
-                     nseg, K, (uint32_t)k, (uint32_t)d, T);
+                     nseg * 4 * K, (uint32_t)k, (uint32_t)d, d_queries, xn, pmx, T);
   HIPC(hipGetLastError());
-  void (*p2)(const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t, uint32_t,
-             const float*, uint32_t*, uint32_t*, uint32_t) = dfix == 50 ? k_exact_pass2<50> : (dfix == 30 ? k_exact_pass2<30> : k_exact_pass2<0>);
+  void (*p2)(const float*, const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t,
+             uint32_t, const float*, uint32_t*, uint32_t*, uint32_t) =
+      dfix == 50 ? k_exact_pass2<50> : (dfix == 30 ? k_exact_pass2<30> : k_exact_pass2<0>);
   if (smem > 65536) HIPC(hipFuncSetAttribute((const void*)p2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-  hipLaunchKernelGGL(p2, g1, dim3(256), smem, s, Xp, (uint32_t)n, (uint32_t)d, dp, d_queries,
+  hipLaunchKernelGGL(p2, g1, dim3(256), smem, s, Xp, xn, (uint32_t)n, (uint32_t)d, dp, d_queries,
                      (uint32_t)nq, seg_len, T, cnt, cand, cap);
   HIPC(hipGetLastError());
   hipLaunchKernelGGL(k_exact_rerank, dim3(grid_for(nq, 4)), dim3(256), 0, s, Xp, (uint32_t)n,
@@ -1403,7 +1425,7 @@ int knn_exact_small(BufSrc& B, const float* d_data, uint64_t n, uint64_t d, cons
     TRY(B.take("ex_fb_i", (size_t)h_novf * k, &si));
     hipLaunchKernelGGL(k_exact_fallback, dim3(grid_for(h_novf, 64)), dim3(64), 0, s, Xp,
                        (uint32_t)n, (uint32_t)d, dp, d_queries, (uint32_t)k, T, ovf, h_novf, sd, si,
-                       d_idx, d_dist, nullptr, 0u);
+                       d_idx, d_dist, nullptr, 0u, exact_fallback_scale(d));
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(s));
   }

@@ -1,32 +1,42 @@
-// Exact kNN for gfx950: FP32 filter with a proven error margin, FP64 re-rank.
+// Exact kNN for gfx950: FP32 matrix-core filter with a proven error margin,
+// FP64 re-rank.
 //
 // Order and tie rule follow by_distance / knn_exact of the reference
 // (crates/polariseq-core/src/doublets.rs:544-651): distance, then index, the
-// query itself excluded. The ranking distance is FP64, Σ (double(a)−double(b))²
+// query itself excluded. The ranking distance is FP64, sum (double(a)-double(b))^2
 // summed in dimension order with no fused multiply-add, so the result is
 // bit-exact against the CPU oracle that computes the same sequence.
 //
-// pass 1  one lane per query, points streamed through LDS and read by
-//         broadcast; each lane keeps its K smallest FP32 distances per point
-//         segment (registers, sorted insertion). For d = 50 (the kNN graph's
-//         PCA width) and d = 30 (Scrublet's) the query row is held in
-//         registers; other widths read it through L1 for every point.
-// tau    the k-th smallest over all segments -> threshold T = tau * F, where
-//         F = 1 + 4 (d + 8) 2^-24 bounds the FP32 error of both ranks (each
-//         FP32 distance is within (d + 4) u of the exact value, relative,
-//         because every term is non-negative).
-// pass 2  the same scan; every point with d32 <= T becomes a candidate.
-//         The true top-k is inside: for a true neighbour p,
-//         d32(p) <= (1 + e) d(p) <= (1 + e) tau* <= tau (1 + e) / (1 - e) <= T.
+// norms   xn[p] = |p|^2 in FP32 (dimension order), and P = max_p xn[p].
+// pass 1  a block of 64 queries (16 per wave) streams a segment of points
+//         through LDS, 64 at a time; per 16 points one v_mfma_f32_16x16x4f32
+//         chain over the dimensions gives the 16 x 16 dot products (the
+//         reference's tiled kNN also works on matrix products,
+//         doublets.rs:557-651), and e(p, q) = max(0, (xn[p] + xn[q]) - 2 p.q).
+//         Lane (c, g) of a wave sees query c against points 4g .. 4g + 3 of every
+//         16 and keeps its K smallest e: four lists per query and segment.
+// tau    the k-th smallest e over all lists -> threshold T = tau + 2 E(q), with
+//         E(q) = 2.1 (d + 8) 2^-24 (xn[q] + P) + 2^-110 (xn[q] + P + 1), a bound on
+//         |e(p, q) - |p - q|^2| for every p: the dot product, both norms and the
+//         two additions each err by at most (d + 2) 2^-24 times |p|^2 + |q|^2 (any
+//         summation order), clamping at 0 only moves e towards the exact value,
+//         and the last term covers denormal products flushed to zero.
+// pass 2  the same scan; every point with e <= T becomes a candidate.
+//         The true top-k is inside: k points have e <= tau, so the true k-th
+//         distance is at most tau + E, and a true neighbour p has
+//         e(p) <= |p - q|^2 + E <= tau + 2 E = T.
 // rerank  one wave per query ranks its candidates by (d64, index).
 // fallback  a query with more candidates than the buffer (many exact ties)
-//         is finished by one thread scanning every point.
+//         is finished by one thread scanning every point, with the VALU FP32
+//         distance (within (d + 4) 2^-24 of the exact one, relative) against T
+//         scaled by 1 + 4 (d + 8) 2^-24.
 #include "psq_kernels.hpp"
 
 namespace psq {
 
-constexpr int EX_BLOCK = 256;  // queries per block (one per lane)
-constexpr int EX_TILE = 64;    // points per LDS tile
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+constexpr int EX_BLOCK = 256;  // queries per block of the batched form (one per lane)
+constexpr int EX_TILE = 64;    // points per LDS tile of the batched form
 
 // FP32 distance of the filter: four FMA chains, a fixed order, so pass 1 and
 // pass 2 produce the same bits.
@@ -98,63 +108,145 @@ __device__ inline void load_tile(const float* __restrict__ Xp, uint32_t dp, uint
   for (uint32_t e = threadIdx.x; e < n4; e += blockDim.x) dst[e] = src[e];
 }
 
+// ── norms and the matrix-core filter ──
+// xn[p] = sum_t x_t^2 (FP32, dimension order); pmax = max over p, as the bits of
+// a non-negative float (unsigned order = value order).
+__global__ void k_exact_norms(const float* __restrict__ Xp, uint32_t n, uint32_t d, uint32_t dp,
+                              float* __restrict__ xn, uint32_t* __restrict__ pmax) {
+  __shared__ uint32_t s_max;
+  if (threadIdx.x == 0) s_max = 0;
+  __syncthreads();
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) {
+    const float* r = Xp + (size_t)p * dp;
+    float a = 0.0f;
+    for (uint32_t t = 0; t < d; ++t) a = fmaf(r[t], r[t], a);
+    xn[p] = a;
+    atomicMax(&s_max, __float_as_uint(a));
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) atomicMax(pmax, s_max);
+}
+
+// FP64 threshold T = tau + 2 E(q), rounded up to FP32 (see the file header).
+__device__ inline float exact_threshold(float tau, float xq, const uint32_t* pmax, uint32_t d) {
+  if (!(tau < __builtin_inff())) return __builtin_inff();
+  const double P = (double)__uint_as_float(*pmax);
+  const double nrm = (double)xq + P;
+  const double E = 2.1 * ((double)d + 8.0) * 5.9604644775390625e-08 * nrm +
+                   7.703719777548943e-34 * (nrm + 1.0);  // 2^-110
+  const double Td = (double)tau + 2.0 * E;
+  float Tf = (float)Td;
+  if ((double)Tf < Td) Tf = nextafterf(Tf, __builtin_inff());
+  return Tf;
+}
+
+
+constexpr int EXM_P = 64;  // points per LDS tile
+
+// The LDS tile holds 64 points, point r at r * TS floats with its dimensions
+// reordered so that the KS values lane group g reads (dims 4 s + g) are
+// contiguous: element (g, s) at g * KS + s. TS = 4 KS, or 4 KS + 4 when that
+// is 0 mod 8, keeps the 64 lanes of a read on distinct banks.
+__host__ __device__ constexpr uint32_t exm_ts(uint32_t ks) { return (4 * ks) % 8 == 0 ? 4 * ks + 4 : 4 * ks; }
+__host__ size_t exm_smem(uint32_t d) {
+  const uint32_t ks = (d + 3) / 4;
+  return ((size_t)EXM_P * exm_ts(ks) + EXM_P) * sizeof(float);
+}
+
+// Stage points [base, base + np) into the tile and their norms into tn.
+__device__ inline void exm_stage(const float* __restrict__ Xp, const float* __restrict__ xn, uint32_t dp,
+                                 uint32_t KS, uint32_t TS, uint32_t base, uint32_t np, float* tile,
+                                 float* tn) {
+  const uint32_t per = 4 * KS;
+  for (uint32_t e = threadIdx.x; e < (uint32_t)EXM_P * per; e += blockDim.x) {
+    const uint32_t r = e / per, j = e - r * per;  // j = dimension 4 s + g
+    const uint32_t sidx = j / 4, gg = j % 4;
+    tile[r * TS + gg * KS + sidx] = r < np ? Xp[(size_t)(base + r) * dp + j] : 0.0f;
+  }
+  for (uint32_t r = threadIdx.x; r < (uint32_t)EXM_P; r += blockDim.x) tn[r] = r < np ? xn[base + r] : 0.0f;
+}
+
+// The filter values of one 16-point sub-tile for this lane's query: e[i] for
+// point 16 t + 4 g + i of the tile.
+template <int KSM>
+__device__ inline f32x4_t exm_tile_dots(const float* tile, uint32_t TS, uint32_t KS, uint32_t t,
+                                        uint32_t c, uint32_t g, const float (&qf)[KSM]) {
+  f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  const float* a = tile + (16 * t + c) * TS + g * KS;
+#pragma unroll
+  for (int s = 0; s < KSM; ++s)
+    if ((uint32_t)s < KS) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], qf[s], acc, 0, 0, 0);
+  return acc;
+}
+
 template <int D, int K>
-__global__ __launch_bounds__(EX_BLOCK) void k_exact_pass1(const float* __restrict__ Xp, uint32_t n,
-                                                          uint32_t d, uint32_t dp,
-                                                          const uint32_t* __restrict__ queries,
-                                                          uint32_t nq, uint32_t seg_len,
-                                                          uint32_t nseg, float* __restrict__ partial) {
-  extern __shared__ float tile[];  // EX_TILE * dp
-  const uint32_t qslot = blockIdx.x * EX_BLOCK + threadIdx.x;
+__global__ __launch_bounds__(256) void k_exact_pass1(const float* __restrict__ Xp,
+                                                     const float* __restrict__ xn, uint32_t n,
+                                                     uint32_t d, uint32_t dp,
+                                                     const uint32_t* __restrict__ queries,
+                                                     uint32_t nq, uint32_t seg_len, uint32_t nseg,
+                                                     float* __restrict__ partial) {
+  constexpr int KSM = D > 0 ? (D + 3) / 4 : 64;
+  extern __shared__ float exm_lds[];
+  const uint32_t KS = D > 0 ? (uint32_t)KSM : (d + 3) / 4, TS = exm_ts(KS);
+  float* tile = exm_lds;
+  float* tn = tile + (size_t)EXM_P * TS;
+  const uint32_t l = threadIdx.x % 64, w = threadIdx.x / 64, c = l % 16, g = l / 16;
+  const uint32_t qslot = blockIdx.x * EXM_QUERIES + w * 16 + c;
   const bool qvalid = qslot < nq;
   const uint32_t qi = qvalid ? (queries ? queries[qslot] : qslot) : 0u;
-  constexpr int DR = D > 0 ? D : 1;
-  float qv[DR];
-  if (D > 0) {
+  float qf[KSM];
 #pragma unroll
-    for (int t = 0; t < DR; ++t) qv[t] = Xp[(size_t)qi * dp + t];
-  }
-  const float* qrow = Xp + (size_t)qi * dp;
+  for (int s = 0; s < KSM; ++s) qf[s] = (uint32_t)s < KS ? Xp[(size_t)qi * dp + 4 * s + g] : 0.0f;
+  const float xq = xn[qi];
   float top[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) top[j] = __builtin_inff();
   const uint32_t s0 = blockIdx.y * seg_len;
   const uint32_t s1 = min(n, s0 + seg_len);
-  for (uint32_t base = s0; base < s1; base += EX_TILE) {
-    const uint32_t np = min((uint32_t)EX_TILE, s1 - base);
+  for (uint32_t base = s0; base < s1; base += EXM_P) {
+    const uint32_t np = min((uint32_t)EXM_P, s1 - base);
     __syncthreads();
-    load_tile(Xp, dp, base, np, tile);
+    exm_stage(Xp, xn, dp, KS, TS, base, np, tile, tn);
     __syncthreads();
-    for (uint32_t j = 0; j < np; ++j) {
-      const float* pr = tile + (size_t)j * dp;
-      float dist = D > 0 ? filt_dist_fixed<DR>(pr, qv) : filt_dist_rt(pr, qrow, (int)d);
-      if (base + j == qi) dist = __builtin_inff();
-      if (dist < top[K - 1]) {
+    for (uint32_t t = 0; t < (np + 15) / 16; ++t) {
+      const f32x4_t acc = exm_tile_dots<KSM>(tile, TS, KS, t, c, g, qf);
 #pragma unroll
-        for (int t = K - 1; t > 0; --t) {
-          float a = top[t - 1];
-          top[t] = dist < a ? a : (dist < top[t] ? dist : top[t]);
+      for (int i = 0; i < 4; ++i) {
+        const uint32_t r = 16 * t + 4 * g + i;
+        float e = fmaxf((tn[r] + xq) - 2.0f * acc[i], 0.0f);
+        if (r >= np || base + r == qi) e = __builtin_inff();
+        if (e < top[K - 1]) {
+#pragma unroll
+          for (int u = K - 1; u > 0; --u) {
+            const float a = top[u - 1];
+            top[u] = e < a ? a : (e < top[u] ? e : top[u]);
+          }
+          top[0] = e < top[0] ? e : top[0];
         }
-        top[0] = dist < top[0] ? dist : top[0];
       }
     }
   }
   if (qvalid) {
-    float* out = partial + ((size_t)qslot * nseg + blockIdx.y) * K;
+    float* out = partial + (((size_t)qslot * nseg + blockIdx.y) * 4 + g) * K;
 #pragma unroll
     for (int j = 0; j < K; ++j) out[j] = top[j];
   }
 }
 
-__global__ void k_exact_tau(const float* __restrict__ partial, uint32_t nq, uint32_t nseg,
-                            uint32_t K, uint32_t k, uint32_t d, float* __restrict__ T) {
+// tau over a query's L = 4 nseg K pass-1 values (K <= 64, k <= 64), then T.
+__global__ void k_exact_tau(const float* __restrict__ partial, uint32_t nq, uint32_t L, uint32_t k,
+                            uint32_t d, const uint32_t* __restrict__ queries,
+                            const float* __restrict__ xn, const uint32_t* __restrict__ pmax,
+                            float* __restrict__ T) {
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nq) return;
   float top[64];
 #pragma unroll
   for (int j = 0; j < 64; ++j) top[j] = __builtin_inff();
-  const float* src = partial + (size_t)q * nseg * K;
-  for (uint32_t e = 0; e < nseg * K; ++e) {
+  const float* src = partial + (size_t)q * L;
+  for (uint32_t e = 0; e < L; ++e) {
     float v = src[e];
     if (v < top[63]) {
 #pragma unroll
@@ -169,52 +261,49 @@ __global__ void k_exact_tau(const float* __restrict__ partial, uint32_t nq, uint
 #pragma unroll
   for (int j = 0; j < 64; ++j)
     if ((uint32_t)j == k - 1) tau = top[j];
-  if (tau >= __builtin_inff()) {
-    T[q] = tau;
-    return;
-  }
-  // F = 1 + 4 (d + 8) 2^-24, applied in FP64 and rounded up to FP32; the small
-  // absolute term covers underflow of squared differences.
-  double F = 1.0 + 4.0 * ((double)d + 8.0) * 5.9604644775390625e-08;
-  double Td = (double)tau * F + 1e-30;
-  float Tf = (float)Td;
-  if ((double)Tf < Td) Tf = nextafterf(Tf, __builtin_inff());
-  T[q] = Tf;
+  T[q] = exact_threshold(tau, xn[queries ? queries[q] : q], pmax, d);
 }
 
 template <int D>
-__global__ __launch_bounds__(EX_BLOCK) void k_exact_pass2(const float* __restrict__ Xp, uint32_t n,
-                                                          uint32_t d, uint32_t dp,
-                                                          const uint32_t* __restrict__ queries,
-                                                          uint32_t nq, uint32_t seg_len,
-                                                          const float* __restrict__ T,
-                                                          uint32_t* __restrict__ cnt,
-                                                          uint32_t* __restrict__ cand, uint32_t cap) {
-  extern __shared__ float tile[];
-  const uint32_t qslot = blockIdx.x * EX_BLOCK + threadIdx.x;
+__global__ __launch_bounds__(256) void k_exact_pass2(const float* __restrict__ Xp,
+                                                     const float* __restrict__ xn, uint32_t n,
+                                                     uint32_t d, uint32_t dp,
+                                                     const uint32_t* __restrict__ queries,
+                                                     uint32_t nq, uint32_t seg_len,
+                                                     const float* __restrict__ T,
+                                                     uint32_t* __restrict__ cnt,
+                                                     uint32_t* __restrict__ cand, uint32_t cap) {
+  constexpr int KSM = D > 0 ? (D + 3) / 4 : 64;
+  extern __shared__ float exm_lds[];
+  const uint32_t KS = D > 0 ? (uint32_t)KSM : (d + 3) / 4, TS = exm_ts(KS);
+  float* tile = exm_lds;
+  float* tn = tile + (size_t)EXM_P * TS;
+  const uint32_t l = threadIdx.x % 64, w = threadIdx.x / 64, c = l % 16, g = l / 16;
+  const uint32_t qslot = blockIdx.x * EXM_QUERIES + w * 16 + c;
   const bool qvalid = qslot < nq;
   const uint32_t qi = qvalid ? (queries ? queries[qslot] : qslot) : 0u;
   const float thr = qvalid ? T[qslot] : -1.0f;
-  constexpr int DR = D > 0 ? D : 1;
-  float qv[DR];
-  if (D > 0) {
+  float qf[KSM];
 #pragma unroll
-    for (int t = 0; t < DR; ++t) qv[t] = Xp[(size_t)qi * dp + t];
-  }
-  const float* qrow = Xp + (size_t)qi * dp;
+  for (int s = 0; s < KSM; ++s) qf[s] = (uint32_t)s < KS ? Xp[(size_t)qi * dp + 4 * s + g] : 0.0f;
+  const float xq = xn[qi];
   const uint32_t s0 = blockIdx.y * seg_len;
   const uint32_t s1 = min(n, s0 + seg_len);
-  for (uint32_t base = s0; base < s1; base += EX_TILE) {
-    const uint32_t np = min((uint32_t)EX_TILE, s1 - base);
+  for (uint32_t base = s0; base < s1; base += EXM_P) {
+    const uint32_t np = min((uint32_t)EXM_P, s1 - base);
     __syncthreads();
-    load_tile(Xp, dp, base, np, tile);
+    exm_stage(Xp, xn, dp, KS, TS, base, np, tile, tn);
     __syncthreads();
-    for (uint32_t j = 0; j < np; ++j) {
-      const float* pr = tile + (size_t)j * dp;
-      float dist = D > 0 ? filt_dist_fixed<DR>(pr, qv) : filt_dist_rt(pr, qrow, (int)d);
-      if (qvalid && dist <= thr && base + j != qi) {
-        uint32_t pos = atomicAdd(&cnt[qslot], 1u);
-        if (pos < cap) cand[(size_t)qslot * cap + pos] = base + j;
+    for (uint32_t t = 0; t < (np + 15) / 16; ++t) {
+      const f32x4_t acc = exm_tile_dots<KSM>(tile, TS, KS, t, c, g, qf);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const uint32_t r = 16 * t + 4 * g + i;
+        const float e = fmaxf((tn[r] + xq) - 2.0f * acc[i], 0.0f);
+        if (r < np && e <= thr && base + r != qi) {
+          const uint32_t pos = atomicAdd(&cnt[qslot], 1u);
+          if (pos < cap) cand[(size_t)qslot * cap + pos] = base + r;
+        }
       }
     }
   }
@@ -292,13 +381,18 @@ __global__ void k_exact_fallback(const float* __restrict__ Xp, uint32_t n, uint3
                                  uint32_t n_overflow, double* __restrict__ sd,
                                  uint32_t* __restrict__ si, uint32_t* __restrict__ idx_out,
                                  float* __restrict__ dist_out, const uint64_t* __restrict__ off,
-                                 uint32_t S) {
+                                 uint32_t S, double thr_scale) {
   const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
   if (o >= n_overflow) return;
   const uint32_t qs = overflow[o];
   const uint32_t qi = queries ? queries[qs] : qs;
   const float* qrow = Xp + (size_t)qi * dp;
-  const float thr = T[qs];
+  // the matrix-core threshold bounds the filter value; the VALU distance here
+  // is within (d + 4) 2^-24 of the exact one, so T scaled by thr_scale >= 1 + that
+  // bound keeps every true neighbour (thr_scale = 1 for the batched form's own T)
+  double thd = (double)T[qs] * thr_scale;
+  float thr = (float)thd;
+  if ((double)thr < thd) thr = nextafterf(thr, __builtin_inff());
   double* ld = sd + (size_t)o * k;
   uint32_t* li = si + (size_t)o * k;
   uint32_t cntk = 0;
@@ -342,18 +436,19 @@ __global__ void k_exact_fallback(const float* __restrict__ Xp, uint32_t n, uint3
 // threshold built from it keeps the proof of pass 2. The selection and the
 // re-rank run per query in LDS.
 
-// tau: the k-th smallest of a query's nseg * K pass-1 values, by radix select
+// tau: the k-th smallest of a query's L pass-1 values, by radix select
 // on the FP32 bits (four 8-bit digits; the values are >= 0, so bit order is
-// value order). One block per query; dynamic LDS: nseg * K values.
+// value order). One block per query; dynamic LDS: L values.
 __global__ __launch_bounds__(256) void k_exact_tau_big(const float* __restrict__ partial,
-                                                       uint32_t nq, uint32_t nseg, uint32_t K,
-                                                       uint32_t k, uint32_t d,
+                                                       uint32_t nq, uint32_t L, uint32_t k, uint32_t d,
+                                                       const uint32_t* __restrict__ queries,
+                                                       const float* __restrict__ xn,
+                                                       const uint32_t* __restrict__ pmax,
                                                        float* __restrict__ T) {
   extern __shared__ uint32_t sv[];
   __shared__ uint32_t hist[256];
   __shared__ uint32_t s_prefix, s_mask, s_rank;
   const uint32_t q = blockIdx.x, tid = threadIdx.x;
-  const uint32_t L = nseg * K;
   const float* src = partial + (size_t)q * L;
   for (uint32_t e = tid; e < L; e += blockDim.x) sv[e] = __float_as_uint(src[e]);
   if (tid == 0) {
@@ -381,16 +476,7 @@ __global__ __launch_bounds__(256) void k_exact_tau_big(const float* __restrict__
     __syncthreads();
   }
   if (tid != 0) return;
-  const float tau = __uint_as_float(s_prefix);
-  if (!(tau < __builtin_inff())) {
-    T[q] = __builtin_inff();
-    return;
-  }
-  double F = 1.0 + 4.0 * ((double)d + 8.0) * 5.9604644775390625e-08;
-  double Td = (double)tau * F + 1e-30;
-  float Tf = (float)Td;
-  if ((double)Tf < Td) Tf = nextafterf(Tf, __builtin_inff());
-  T[q] = Tf;
+  T[q] = exact_threshold(__uint_as_float(s_prefix), xn[queries ? queries[q] : q], pmax, d);
 }
 
 // One block per query: the FP64 distance of every candidate, a bitonic sort of
@@ -456,33 +542,30 @@ __global__ void k_iota_u32(uint32_t* __restrict__ q, uint32_t q0, uint32_t count
   if (i < count) q[i] = q0 + i;
 }
 
-template __global__ void k_exact_pass1<0, 16>(const float*, uint32_t, uint32_t, uint32_t,
-                                              const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
-template __global__ void k_exact_pass1<0, 32>(const float*, uint32_t, uint32_t, uint32_t,
-                                              const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
-template __global__ void k_exact_pass1<0, 64>(const float*, uint32_t, uint32_t, uint32_t,
-                                              const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
-template __global__ void k_exact_pass1<30, 16>(const float*, uint32_t, uint32_t, uint32_t,
-                                               const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
-template __global__ void k_exact_pass1<30, 32>(const float*, uint32_t, uint32_t, uint32_t,
-                                               const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
-template __global__ void k_exact_pass1<30, 64>(const float*, uint32_t, uint32_t, uint32_t,
-                                               const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
-template __global__ void k_exact_pass1<50, 16>(const float*, uint32_t, uint32_t, uint32_t,
-                                               const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
-template __global__ void k_exact_pass1<50, 32>(const float*, uint32_t, uint32_t, uint32_t,
-                                               const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
-template __global__ void k_exact_pass1<50, 64>(const float*, uint32_t, uint32_t, uint32_t,
-                                               const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
-template __global__ void k_exact_pass2<0>(const float*, uint32_t, uint32_t, uint32_t,
-                                          const uint32_t*, uint32_t, uint32_t, const float*,
-                                          uint32_t*, uint32_t*, uint32_t);
-template __global__ void k_exact_pass2<30>(const float*, uint32_t, uint32_t, uint32_t,
-                                           const uint32_t*, uint32_t, uint32_t, const float*,
-                                           uint32_t*, uint32_t*, uint32_t);
-template __global__ void k_exact_pass2<50>(const float*, uint32_t, uint32_t, uint32_t,
-                                           const uint32_t*, uint32_t, uint32_t, const float*,
-                                           uint32_t*, uint32_t*, uint32_t);
+template __global__ void k_exact_pass1<0, 16>(const float*, const float*, uint32_t, uint32_t, uint32_t,
+    const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+template __global__ void k_exact_pass1<0, 32>(const float*, const float*, uint32_t, uint32_t, uint32_t,
+    const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+template __global__ void k_exact_pass1<0, 64>(const float*, const float*, uint32_t, uint32_t, uint32_t,
+    const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+template __global__ void k_exact_pass1<30, 16>(const float*, const float*, uint32_t, uint32_t, uint32_t,
+    const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+template __global__ void k_exact_pass1<30, 32>(const float*, const float*, uint32_t, uint32_t, uint32_t,
+    const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+template __global__ void k_exact_pass1<30, 64>(const float*, const float*, uint32_t, uint32_t, uint32_t,
+    const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+template __global__ void k_exact_pass1<50, 16>(const float*, const float*, uint32_t, uint32_t, uint32_t,
+    const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+template __global__ void k_exact_pass1<50, 32>(const float*, const float*, uint32_t, uint32_t, uint32_t,
+    const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+template __global__ void k_exact_pass1<50, 64>(const float*, const float*, uint32_t, uint32_t, uint32_t,
+    const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+template __global__ void k_exact_pass2<0>(const float*, const float*, uint32_t, uint32_t, uint32_t,
+    const uint32_t*, uint32_t, uint32_t, const float*, uint32_t*, uint32_t*, uint32_t);
+template __global__ void k_exact_pass2<30>(const float*, const float*, uint32_t, uint32_t, uint32_t,
+    const uint32_t*, uint32_t, uint32_t, const float*, uint32_t*, uint32_t*, uint32_t);
+template __global__ void k_exact_pass2<50>(const float*, const float*, uint32_t, uint32_t, uint32_t,
+    const uint32_t*, uint32_t, uint32_t, const float*, uint32_t*, uint32_t*, uint32_t);
 
 }  // namespace psq
 

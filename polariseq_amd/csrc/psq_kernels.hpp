@@ -137,14 +137,18 @@ __global__ void k_gather_rows(const float* src, uint32_t d, const uint32_t* idx,
 __global__ void k_moments(const float* X, uint32_t n, uint32_t d, uint32_t chunk, double* partial);
 
 // exact kNN (exact_kernels.hip)
+__global__ void k_exact_norms(const float* Xp, uint32_t n, uint32_t d, uint32_t dp, float* xn,
+                              uint32_t* pmax);
+size_t exm_smem(uint32_t d);  // dynamic LDS of k_exact_pass1 / k_exact_pass2
+constexpr uint32_t EXM_QUERIES = 64;  // queries per block of the two passes
 template <int D, int K>
-__global__ void k_exact_pass1(const float* Xp, uint32_t n, uint32_t d, uint32_t dp,
+__global__ void k_exact_pass1(const float* Xp, const float* xn, uint32_t n, uint32_t d, uint32_t dp,
                               const uint32_t* queries, uint32_t nq, uint32_t seg_len,
                               uint32_t nseg, float* partial);
-__global__ void k_exact_tau(const float* partial, uint32_t nq, uint32_t nseg, uint32_t K,
-                            uint32_t k, uint32_t d, float* T);
+__global__ void k_exact_tau(const float* partial, uint32_t nq, uint32_t L, uint32_t k, uint32_t d,
+                            const uint32_t* queries, const float* xn, const uint32_t* pmax, float* T);
 template <int D>
-__global__ void k_exact_pass2(const float* Xp, uint32_t n, uint32_t d, uint32_t dp,
+__global__ void k_exact_pass2(const float* Xp, const float* xn, uint32_t n, uint32_t d, uint32_t dp,
                               const uint32_t* queries, uint32_t nq, uint32_t seg_len,
                               const float* T, uint32_t* cnt, uint32_t* cand, uint32_t cap);
 __global__ void k_exact_rerank(const float* Xp, uint32_t n, uint32_t d, uint32_t dp,
@@ -153,8 +157,8 @@ __global__ void k_exact_rerank(const float* Xp, uint32_t n, uint32_t d, uint32_t
                                uint32_t* idx_out, float* dist_out, uint32_t* overflow,
                                uint32_t* n_overflow);
 constexpr uint32_t EX_KBIG_MAX = 2048;  // largest k of the exact path
-__global__ void k_exact_tau_big(const float* partial, uint32_t nq, uint32_t nseg, uint32_t K, uint32_t k,
-                                uint32_t d, float* T);
+__global__ void k_exact_tau_big(const float* partial, uint32_t nq, uint32_t L, uint32_t k, uint32_t d,
+                                const uint32_t* queries, const float* xn, const uint32_t* pmax, float* T);
 __global__ void k_exact_rerank_big(const float* Xp, uint32_t n, uint32_t d, uint32_t dp,
                                    const uint32_t* queries, uint32_t nq, uint32_t k,
                                    const uint32_t* cnt, const uint32_t* cand, uint32_t cap,
@@ -165,7 +169,7 @@ __global__ void k_exact_fallback(const float* Xp, uint32_t n, uint32_t d, uint32
                                  const uint32_t* queries, uint32_t k, const float* T,
                                  const uint32_t* overflow, uint32_t n_overflow, double* sd,
                                  uint32_t* si, uint32_t* idx_out, float* dist_out, const uint64_t* off,
-                                 uint32_t S);
+                                 uint32_t S, double thr_scale);
 template <int D, int K>
 __global__ void k_bexact_pass1(const float* Xp, uint32_t d, uint32_t dp, const uint64_t* off,
                                const uint32_t* units, uint32_t U, uint32_t seg_len, uint32_t tile,
