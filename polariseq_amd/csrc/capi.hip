@@ -850,6 +850,9 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
     TRY(build_forest(ws, R, codes, 32u, 32u, Xp, G, s, static_cast<uint32_t*>(hp) + 64,
                      reorder ? &rn : nullptr));
   }
+  hipEvent_t e_forest;
+  TRY(evp.make(&e_forest));
+  HIPC(hipEventRecord(e_forest, s));
   const uint32_t* perm = rn.perm;
   const uint32_t* inv = rn.inv;
   if (rn.Xq) Xp = rn.Xq;
@@ -862,9 +865,15 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   // ── iterations ──
   psq_nndescent_stats st;
   std::memset(&st, 0, sizeof(st));
-  std::vector<hipEvent_t> jb, je;
+  std::vector<hipEvent_t> jb, je, ib, ie;
   uint64_t iters = 0;
   for (uint64_t it = 0; it < R.n_iters; ++it) {
+    hipEvent_t i0, i1;
+    TRY(evp.make(&i0));
+    TRY(evp.make(&i1));
+    HIPC(hipEventRecord(i0, s));
+    ib.push_back(i0);
+    ie.push_back(i1);
     TRY(fill_u64(reinterpret_cast<uint64_t*>(ctr), (size_t)CTR_STRIPES * CTR_WORDS, 0, s));
     TRY(build_reverse(ws, RP, RB, G, n, k, R.seed, it, roff, rev, s));
     auto sk = (k <= 64 && S <= 256) ? k_sample_select<1>
@@ -888,6 +897,7 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
     unsigned long long* csum = ctr + (size_t)CTR_STRIPES * CTR_WORDS;
     hipLaunchKernelGGL(k_ctr_reduce, dim3(1), dim3(64), 0, s, ctr, csum);
     HIPC(hipGetLastError());
+    HIPC(hipEventRecord(i1, s));
     HIPC(hipMemcpyAsync(hctr, csum, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
     const uint64_t changed = hctr[2];
@@ -936,6 +946,22 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   for (size_t i = 0; i < jb.size() && i < 64; ++i) st.ms_join[i] = ms_between(jb[i], je[i]);
   st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   g_stats = st;
+  // POLARISEQ_TRACE (any value): the stage times of nndescent.rs:757-1001 on
+  // stderr, as HIP-event times of this call's stream
+  if (std::getenv("POLARISEQ_TRACE")) {
+    const double sec = 1e-3;
+    std::fprintf(stderr, "[nndescent] rp-forest build + leaf pairs (%llu trees, leaf %llu): %.4fs\n",
+                 (unsigned long long)st.n_trees, (unsigned long long)R.leaf_size,
+                 ms_between(e_start, e_forest) * sec);
+    std::fprintf(stderr, "[nndescent] random fill: %.4fs\n", ms_between(e_forest, e_init) * sec);
+    for (size_t i = 0; i < ib.size() && i < jb.size(); ++i) {
+      std::fprintf(stderr, "[nndescent] iter %zu: candidates: %.4fs\n", i, ms_between(ib[i], jb[i]) * sec);
+      std::fprintf(stderr, "[nndescent] iter %zu: join, %llu changed: %.4fs\n", i,
+                   (unsigned long long)(i < 64 ? st.changed[i] : 0), ms_between(jb[i], ie[i]) * sec);
+    }
+    std::fprintf(stderr, "[nndescent] extract: %.4fs\n", st.ms_extract * sec);
+    std::fprintf(stderr, "[nndescent] total: %.4fs\n", st.ms_total * sec);
+  }
   return PSQ_OK;
 }
 

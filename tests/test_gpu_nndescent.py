@@ -231,3 +231,21 @@ def test_device_budget_surfaces_as_an_error():
         N.lib().psq_set_device_budget(0)
     g = P.nndescent_dense(ds.mixture(500, 50, 5, seed=0), 15, init="random")
     assert len(g["indices"]) == 500 * 15
+
+
+def test_trace_prints_the_stage_times(capfd, monkeypatch):
+    # nndescent.rs:757-1001: POLARISEQ_TRACE (any value) prints per-stage times
+    # to stderr; here one line per stage and iteration, then the total
+    x = ds.mixture(3000, 50, 6, seed=2)
+    monkeypatch.setenv("POLARISEQ_TRACE", "1")
+    P.nndescent_dense(x, 10, seed=1, n_trees=3)
+    it = P.last_stats()["iterations"]
+    err = capfd.readouterr().err
+    lines = [ln for ln in err.splitlines() if ln.startswith("[nndescent]")]
+    assert lines[0].startswith("[nndescent] rp-forest build + leaf pairs (3 trees, leaf 10):")
+    assert sum(": candidates:" in ln for ln in lines) == it
+    assert sum("changed:" in ln for ln in lines) == it
+    assert lines[-2].startswith("[nndescent] extract:") and lines[-1].startswith("[nndescent] total:")
+    monkeypatch.delenv("POLARISEQ_TRACE")
+    P.nndescent_dense(x, 10, seed=1, n_trees=3)
+    assert "[nndescent]" not in capfd.readouterr().err
