@@ -193,10 +193,11 @@ int psq_cosine_distances_device(float* d_dist, uint64_t count, void* stream);
  * f32[n_components] = max(eigenvalue, 0) / trace of the centred Gram, and, if
  * components is not NULL, the principal axes f32[n_features][n_components], with the
  * embedding's column signs (the centred matrix times the axes is the embedding).
- * Up to 4096 genes this is the reference's own route; from 4097 to 16384 the
+ * Up to 4096 genes this is the reference's own route; from 4097 to 32768 the
  * reference switches to randomized SVD and this build stays exact (its result
- * meets the reference's acceptance bounds against the exact SVD, and more);
- * n_features > 16384 returns PSQ_E_UNSUPPORTED; n_obs or n_features 0, or n_components 0 or above
+ * meets the reference's acceptance bounds against the exact SVD, and more; the
+ * Gram product costs n_obs n_features^2 flops, so wide matrices of many cells are
+ * slow); n_features > 32768 returns PSQ_E_UNSUPPORTED; n_obs or n_features 0, or n_components 0 or above
  * min(n_obs, n_features), PSQ_E_INVALID with the reference's message. A run
  * repeats bit for bit. psq_pca_csr_device takes device pointers and runs on
  * the given stream of the current device. Unlike the neighbour-graph entries,

@@ -66,8 +66,10 @@ using psq_host::fail;
 namespace psq {
 
 // The reference takes the Gram route up to 4096 genes (pca.rs:201) and randomized
-// SVD above; this build stays exact up to 16384 (C: 2 GiB of f64 in HBM).
-constexpr uint32_t PCA_MAX_FEATURES = 16384;
+// SVD above; this build stays exact up to 32768 (C and the padded Gram: 8 GiB of
+// f64 each in HBM; m^2 stays below 2^32 work-items for the one-thread-per-entry
+// passes).
+constexpr uint32_t PCA_MAX_FEATURES = 32768;
 constexpr uint32_t PCA_MAX_BLOCK = 1024;          // subspace block b
 constexpr uint32_t PCA_LDS_BLOCK = 80;            // b x b f64 (50 KB) kept in LDS
 constexpr uint32_t PCA_SEGMENTS = 64;             // row segments of the column sums

@@ -19,8 +19,8 @@ Where the reference differs:
   |cos| > 0.9999 per column).
 - More than 4,096 genes: the reference switches to randomized block Krylov SVD
   (``oversample``, ``n_iter`` and ``seed`` configure it); this build keeps the
-  exact Gram route up to 16,384 genes (the Gram matrix is 2 GiB of float64 in
-  HBM), whose answer is the one the randomized route approximates (the
+  exact Gram route up to 32,768 genes (the Gram matrix is 8 GiB of float64 in
+  HBM; its product costs n_obs x genes^2 flops), whose answer is the one the randomized route approximates (the
   reference's own test bounds it against the exact SVD: ratios within 5e-3,
   principal angles below 1 degree), and raises ``RuntimeError`` above that. The
   three arguments are accepted for the signature's sake.

@@ -249,8 +249,8 @@ def test_argument_errors():
     mat = counts(100, 20, 0.3, 1)
     with pytest.raises(RuntimeError, match="exceeds min"):
         gp.pca_sparse(mat, 21, True, 10, 4, 0)
-    with pytest.raises(RuntimeError, match="16384"):
-        gp.pca_sparse(counts(10, 16385, 0.01, 2), 2, True, 10, 4, 0)
+    with pytest.raises(RuntimeError, match="32768"):
+        gp.pca_sparse(counts(10, 32769, 0.01, 2), 2, True, 10, 4, 0)
     ip, ix, dx = parts(mat)
     bad = ix.copy()
     bad[3] = 20

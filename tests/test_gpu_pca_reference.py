@@ -126,3 +126,22 @@ def test_dense_input():
     u, s, _ = np.linalg.svd(xc, full_matrices=False)
     np.testing.assert_allclose(r["variance_ratio"], s[:8] ** 2 / (s ** 2).sum(), rtol=1e-5)
     assert subspace_angle_deg(r["embedding"].astype(np.float64), u[:, :8] * s[:8]) < 1e-3
+
+
+def test_more_genes_than_the_reference_keeps_exact():
+    """20,000 genes, past the reference's switch to randomized SVD (pca.rs:201,
+    4,096): the exact route's embedding against the thin SVD of the centred
+    matrix (from the eigenvectors of X_c X_c^T, the cells being fewer), held to
+    the exact route's bounds."""
+    x = low_rank_counts(n=2500, m=20000, rank=6, seed=11, density=0.05)
+    r = gp.pca_sparse(x, 10, True, 10, 4, 0)
+    xc = x.toarray().astype(np.float64)
+    xc -= xc.mean(axis=0)
+    w, u = np.linalg.eigh(xc @ xc.T)
+    order = np.argsort(w)[::-1]
+    s = np.sqrt(np.maximum(w[order], 0.0))
+    emb_ref = u[:, order[:10]] * s[:10]
+    vr_ref = s[:10] ** 2 / (s ** 2).sum()
+    emb = r["embedding"].astype(np.float64)
+    np.testing.assert_allclose(r["variance_ratio"], vr_ref, rtol=1e-5)
+    assert subspace_angle_deg(emb[:, :6], emb_ref[:, :6]) < 1e-3

@@ -246,8 +246,8 @@ def test_abi_rejects_bad_shapes_before_device_work():
     assert rc == N.PSQ_E_INVALID and msg == "n_components (10) exceeds min(n_obs, n_features) (5)"
     rc, _ = _abi(50, 5, 0)
     assert rc == N.PSQ_E_INVALID
-    rc, msg = _abi(10, 16385, 5)
-    assert rc == N.PSQ_E_UNSUPPORTED and "16384" in msg
+    rc, msg = _abi(10, 32769, 5)
+    assert rc == N.PSQ_E_UNSUPPORTED and "32768" in msg
     rc, _ = _abi(10, 5, 2, scale=1, max_value=-1.0)
     assert rc == N.PSQ_E_INVALID
     rc, _ = _abi(10, 5, 2, scale=1, max_value=float("nan"))
