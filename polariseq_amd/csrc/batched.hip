@@ -42,9 +42,9 @@ using psq_host::fail;
 namespace {
 
 struct Tile {
-  uint32_t block;    // queries per block (one lane per query)
+  uint32_t block;    // threads per block (EXM_QUERIES queries, 16 per wave)
   uint32_t tile;     // points per LDS tile
-  uint32_t seg_len;  // points per work unit
+  uint32_t seg_len;  // points per work unit (the tuned choice)
 };
 
 // Cache of tuned tiles per shape class: (dims, K, log2 of the largest sample).
@@ -93,12 +93,12 @@ void make_plan(const std::vector<uint64_t>& off, const Tile& t, uint32_t K, Plan
     const uint64_t ns = off[s + 1] - off[s];
     P.nseg[s] = (uint32_t)std::max<uint64_t>(1, (ns + t.seg_len - 1) / t.seg_len);
     P.part_off[s] = acc;
-    acc += ns * P.nseg[s] * K;
+    acc += ns * P.nseg[s] * 4 * K;  // four lists per query and segment
   }
   P.partial_floats = acc;
   for (uint32_t s : order) {
     const uint64_t ns = off[s + 1] - off[s];
-    const uint32_t qb = (uint32_t)((ns + t.block - 1) / t.block);
+    const uint32_t qb = (uint32_t)((ns + EXM_QUERIES - 1) / EXM_QUERIES);
     for (uint32_t b = 0; b < qb; ++b)
       for (uint32_t g = 0; g < P.nseg[s]; ++g) {
         P.units.push_back(s);
@@ -120,29 +120,29 @@ int occupancy_grid(const void* kern, unsigned block, size_t smem, unsigned* grid
 }
 
 template <int D, int K>
-int run_pass1(const float* Xp, uint32_t d, uint32_t dp, const uint64_t* d_off, const uint32_t* d_units,
-              uint32_t U, const Tile& t, const uint32_t* d_nseg, const uint64_t* d_poff,
-              uint32_t* d_counter, float* d_partial, unsigned grid, hipStream_t s) {
-  const size_t smem = (size_t)t.tile * dp * sizeof(float);
+int run_pass1(const float* Xp, const float* xn, uint32_t d, uint32_t dp, const uint64_t* d_off,
+              const uint32_t* d_units, uint32_t U, const Tile& t, const uint32_t* d_nseg,
+              const uint64_t* d_poff, uint32_t* d_counter, float* d_partial, unsigned grid, hipStream_t s) {
+  const size_t smem = exm_smem(d);
   auto kern = k_bexact_pass1<D, K>;
   if (smem > 65536)
     BHIPC(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
   BTRY(occupancy_grid((const void*)kern, t.block, smem, &grid));
   BHIPC(hipMemsetAsync(d_counter, 0, 4, s));
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(t.block), smem, s, Xp, d, dp, d_off, d_units, U, t.seg_len,
-                     t.tile, d_nseg, d_poff, d_counter, d_partial);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(t.block), smem, s, Xp, xn, d, dp, d_off, d_units, U, t.seg_len,
+                     d_nseg, d_poff, d_counter, d_partial);
   BHIPC(hipGetLastError());
   return PSQ_OK;
 }
 
 template <int D>
-int dispatch_pass1(uint32_t K, const float* Xp, uint32_t d, uint32_t dp, const uint64_t* d_off,
-                   const uint32_t* d_units, uint32_t U, const Tile& t, const uint32_t* d_nseg,
-                   const uint64_t* d_poff, uint32_t* d_counter, float* d_partial, unsigned grid,
-                   hipStream_t s) {
-  if (K == 16) return run_pass1<D, 16>(Xp, d, dp, d_off, d_units, U, t, d_nseg, d_poff, d_counter, d_partial, grid, s);
-  if (K == 32) return run_pass1<D, 32>(Xp, d, dp, d_off, d_units, U, t, d_nseg, d_poff, d_counter, d_partial, grid, s);
-  return run_pass1<D, 64>(Xp, d, dp, d_off, d_units, U, t, d_nseg, d_poff, d_counter, d_partial, grid, s);
+int dispatch_pass1(uint32_t K, const float* Xp, const float* xn, uint32_t d, uint32_t dp,
+                   const uint64_t* d_off, const uint32_t* d_units, uint32_t U, const Tile& t,
+                   const uint32_t* d_nseg, const uint64_t* d_poff, uint32_t* d_counter, float* d_partial,
+                   unsigned grid, hipStream_t s) {
+  if (K == 16) return run_pass1<D, 16>(Xp, xn, d, dp, d_off, d_units, U, t, d_nseg, d_poff, d_counter, d_partial, grid, s);
+  if (K == 32) return run_pass1<D, 32>(Xp, xn, d, dp, d_off, d_units, U, t, d_nseg, d_poff, d_counter, d_partial, grid, s);
+  return run_pass1<D, 64>(Xp, xn, d, dp, d_off, d_units, U, t, d_nseg, d_poff, d_counter, d_partial, grid, s);
 }
 
 
@@ -171,11 +171,13 @@ extern "C" int psq_knn_exact_batched_device(const float* d_data, uint64_t n_tota
   std::vector<uint64_t> off(offsets, offsets + n_samples + 1);
   uint64_t maxs = 0;
   for (uint64_t i = 0; i < n_samples; ++i) maxs = std::max(maxs, off[i + 1] - off[i]);
-  DevBuf bXp, bOff, bCnt, bCand, bT, bOvf, bNovf, bCounter;
-  float *Xp, *T;
+  DevBuf bXp, bOff, bCnt, bCand, bT, bOvf, bNovf, bCounter, bXn, bPmax;
+  float *Xp, *T, *xn;
   uint64_t* d_off;
-  uint32_t *cnt, *cand, *ovf, *novf, *counter;
+  uint32_t *cnt, *cand, *ovf, *novf, *counter, *pmx;
   BTRY(dalloc(bXp, n_total * dp, &Xp));
+  BTRY(dalloc(bXn, n_total, &xn));
+  BTRY(dalloc(bPmax, 1, &pmx));
   BTRY(dalloc(bOff, n_samples + 1, &d_off));
   BTRY(dalloc(bCnt, n_total, &cnt));
   BTRY(dalloc(bCand, n_total * cap, &cand));
@@ -189,11 +191,15 @@ extern "C" int psq_knn_exact_batched_device(const float* d_data, uint64_t n_tota
   BHIPC(hipMemcpyAsync(d_off, off.data(), (n_samples + 1) * 8, hipMemcpyHostToDevice, s));
   BHIPC(hipMemsetAsync(cnt, 0, n_total * 4, s));
   BHIPC(hipMemsetAsync(novf, 0, 4, s));
+  BHIPC(hipMemsetAsync(pmx, 0, 4, s));
+  hipLaunchKernelGGL(k_exact_norms, dim3(gsize(n_total, 256)), dim3(256), 0, s, Xp, (uint32_t)n_total,
+                     (uint32_t)d, dp, xn, pmx);
+  BHIPC(hipGetLastError());
 
   // ── autotune (once per shape class) ──
   const uint32_t cls = (uint32_t)std::ceil(std::log2((double)std::max<uint64_t>(maxs, 2)));
   const auto key = std::make_tuple((uint32_t)d, K, cls);
-  Tile best{256, 64, 16384};
+  Tile best{256, EXM_P_TILE, 16384};
   bool have = false;
   double tune_ms = 0.0;
   uint32_t tried = 0;
@@ -215,8 +221,8 @@ extern "C" int psq_knn_exact_batched_device(const float* d_data, uint64_t n_tota
     // express the probe as one sample starting at off[big]
     std::vector<uint64_t> probe_off = {0, pn};
     float best_ms = 1e30f;
-    const Tile cands[] = {{256, 32, 8192},  {256, 64, 8192},  {256, 128, 8192}, {256, 64, 32768},
-                          {128, 64, 8192},  {128, 128, 16384}, {64, 64, 4096},  {256, 128, 32768}};
+    const Tile cands[] = {{256, EXM_P_TILE, 4096}, {256, EXM_P_TILE, 8192}, {256, EXM_P_TILE, 16384},
+                          {256, EXM_P_TILE, 32768}};
     DevBuf bpo, bpu, bpn, bpp, bpart;
     uint64_t* d_poff2;
     uint32_t *d_units2, *d_nseg2;
@@ -227,8 +233,9 @@ extern "C" int psq_knn_exact_batched_device(const float* d_data, uint64_t n_tota
     hipEvent_t a, b;
     BHIPC(hipEventCreate(&a));
     BHIPC(hipEventCreate(&b));
-    // the probe reads rows at Xp + off[big] * dp
+    // the probe reads rows at Xp + off[big] * dp (and their norms)
     const float* Xprobe = Xp + off[big] * dp;
+    const float* xnprobe = xn + off[big];
     size_t max_units = 0, max_part = 0;
     std::vector<Plan> plans(sizeof(cands) / sizeof(cands[0]));
     for (size_t c = 0; c < plans.size(); ++c) {
@@ -242,7 +249,6 @@ extern "C" int psq_knn_exact_batched_device(const float* d_data, uint64_t n_tota
     BTRY(dalloc(bpart, max_part, &d_part2));
     for (size_t c = 0; c < plans.size(); ++c) {
       const Tile& t = cands[c];
-      if ((size_t)t.tile * dp * 4 > 160 * 1024) continue;
       const Plan& P = plans[c];
       const uint32_t U = (uint32_t)(P.units.size() / 3);
       BHIPC(hipMemcpyAsync(d_units2, P.units.data(), P.units.size() * 4, hipMemcpyHostToDevice, s));
@@ -251,11 +257,11 @@ extern "C" int psq_knn_exact_batched_device(const float* d_data, uint64_t n_tota
       unsigned grid = 0;  // sized per kernel at its launch
       BHIPC(hipEventRecord(a, s));
       if (dfix == 50)
-        BTRY(dispatch_pass1<50>(K, Xprobe, (uint32_t)d, dp, d_poff2, d_units2, U, t, d_nseg2, d_pofs, counter, d_part2, grid, s));
+        BTRY(dispatch_pass1<50>(K, Xprobe, xnprobe, (uint32_t)d, dp, d_poff2, d_units2, U, t, d_nseg2, d_pofs, counter, d_part2, grid, s));
       else if (dfix == 30)
-        BTRY(dispatch_pass1<30>(K, Xprobe, (uint32_t)d, dp, d_poff2, d_units2, U, t, d_nseg2, d_pofs, counter, d_part2, grid, s));
+        BTRY(dispatch_pass1<30>(K, Xprobe, xnprobe, (uint32_t)d, dp, d_poff2, d_units2, U, t, d_nseg2, d_pofs, counter, d_part2, grid, s));
       else
-        BTRY(dispatch_pass1<0>(K, Xprobe, (uint32_t)d, dp, d_poff2, d_units2, U, t, d_nseg2, d_pofs, counter, d_part2, grid, s));
+        BTRY(dispatch_pass1<0>(K, Xprobe, xnprobe, (uint32_t)d, dp, d_poff2, d_units2, U, t, d_nseg2, d_pofs, counter, d_part2, grid, s));
       BHIPC(hipEventRecord(b, s));
       BHIPC(hipEventSynchronize(b));
       float ms = 0;
@@ -296,24 +302,24 @@ extern "C" int psq_knn_exact_batched_device(const float* d_data, uint64_t n_tota
   BHIPC(hipEventCreate(&e3));
   BHIPC(hipEventRecord(e0, s));
   if (dfix == 50)
-    BTRY(dispatch_pass1<50>(K, Xp, (uint32_t)d, dp, d_off, d_units, U, best, d_nseg, d_poff, counter, d_partial, grid, s));
+    BTRY(dispatch_pass1<50>(K, Xp, xn, (uint32_t)d, dp, d_off, d_units, U, best, d_nseg, d_poff, counter, d_partial, grid, s));
   else if (dfix == 30)
-    BTRY(dispatch_pass1<30>(K, Xp, (uint32_t)d, dp, d_off, d_units, U, best, d_nseg, d_poff, counter, d_partial, grid, s));
+    BTRY(dispatch_pass1<30>(K, Xp, xn, (uint32_t)d, dp, d_off, d_units, U, best, d_nseg, d_poff, counter, d_partial, grid, s));
   else
-    BTRY(dispatch_pass1<0>(K, Xp, (uint32_t)d, dp, d_off, d_units, U, best, d_nseg, d_poff, counter, d_partial, grid, s));
+    BTRY(dispatch_pass1<0>(K, Xp, xn, (uint32_t)d, dp, d_off, d_units, U, best, d_nseg, d_poff, counter, d_partial, grid, s));
   BHIPC(hipEventRecord(e1, s));
   hipLaunchKernelGGL(k_bexact_tau, dim3(gsize(n_total, 128)), dim3(128), 0, s, d_partial, d_off,
-                     (uint32_t)n_samples, d_nseg, d_poff, n_total, K, (uint32_t)k, (uint32_t)d, T);
+                     (uint32_t)n_samples, d_nseg, d_poff, n_total, K, (uint32_t)k, (uint32_t)d, xn, pmx, T);
   BHIPC(hipGetLastError());
   {
-    const size_t smem = (size_t)best.tile * dp * sizeof(float);
+    const size_t smem = exm_smem((uint32_t)d);
     auto kern = dfix == 50 ? k_bexact_pass2<50> : (dfix == 30 ? k_bexact_pass2<30> : k_bexact_pass2<0>);
     if (smem > 65536)
       BHIPC(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     BTRY(occupancy_grid((const void*)kern, best.block, smem, &grid));
     BHIPC(hipMemsetAsync(counter, 0, 4, s));
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(best.block), smem, s, Xp, (uint32_t)d, dp, d_off, d_units, U,
-                       best.seg_len, best.tile, T, counter, cnt, cand, cap);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(best.block), smem, s, Xp, xn, (uint32_t)d, dp, d_off, d_units, U,
+                       best.seg_len, T, counter, cnt, cand, cap);
     BHIPC(hipGetLastError());
   }
   BHIPC(hipEventRecord(e2, s));
@@ -332,7 +338,7 @@ extern "C" int psq_knn_exact_batched_device(const float* d_data, uint64_t n_tota
     BTRY(dalloc(bsi, (size_t)h_novf * k, &si));
     hipLaunchKernelGGL(k_exact_fallback, dim3(gsize(h_novf, 64)), dim3(64), 0, s, Xp, (uint32_t)n_total,
                        (uint32_t)d, dp, nullptr, (uint32_t)k, T, ovf, h_novf, sd, si, d_idx, d_dist, d_off,
-                       (uint32_t)n_samples, 1.0);
+                       (uint32_t)n_samples, exact_fallback_scale(d));
     BHIPC(hipGetLastError());
   }
   BHIPC(hipEventRecord(e3, s));

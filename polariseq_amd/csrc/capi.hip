@@ -1225,11 +1225,6 @@ int set_device(int32_t device) {
 // segment and at least k/16 segments, a radix-select tau per query, pass 2 into
 // a candidate buffer of 2k + 256, an LDS sort per query; queries in chunks that
 // bound the pass-1 and candidate buffers.
-// The fallback's FP32 VALU distance is within (d + 4) 2^-24 of the exact one,
-// relative: the matrix-core threshold scaled by this bounds it for every true
-// neighbour.
-double exact_fallback_scale(uint64_t d) { return 1.0 + 4.0 * ((double)d + 8.0) * 5.9604644775390625e-08; }
-
 int knn_exact_big(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d_queries,
                   uint64_t nq, uint64_t k, uint32_t* d_idx, float* d_dist, hipStream_t s) {
   Workspace* ws;

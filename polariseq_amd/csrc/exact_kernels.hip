@@ -35,33 +35,8 @@
 namespace psq {
 
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
-constexpr int EX_BLOCK = 256;  // queries per block of the batched form (one per lane)
-constexpr int EX_TILE = 64;    // points per LDS tile of the batched form
-
-// FP32 distance of the filter: four FMA chains, a fixed order, so pass 1 and
-// pass 2 produce the same bits.
-template <int D>
-__device__ inline float filt_dist_fixed(const float* __restrict__ p, const float (&q)[D]) {
-  float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
-#pragma unroll
-  for (int t = 0; t < D; t += 4) {
-    float d0 = p[t] - q[t];
-    a0 = fmaf(d0, d0, a0);
-    if (t + 1 < D) {
-      float d1 = p[t + 1] - q[t + 1];
-      a1 = fmaf(d1, d1, a1);
-    }
-    if (t + 2 < D) {
-      float d2 = p[t + 2] - q[t + 2];
-      a2 = fmaf(d2, d2, a2);
-    }
-    if (t + 3 < D) {
-      float d3 = p[t + 3] - q[t + 3];
-      a3 = fmaf(d3, d3, a3);
-    }
-  }
-  return (a0 + a1) + (a2 + a3);
-}
+// FP32 distance of the fallback: four FMA chains, dimensions t = 4j + r in
+// chain r, the chains summed as (a0 + a1) + (a2 + a3).
 __device__ inline float filt_dist_rt(const float* __restrict__ p, const float* __restrict__ q, int d) {
   float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
   int t = 0;
@@ -99,15 +74,6 @@ __device__ inline double dist64(const float* __restrict__ a, const float* __rest
   return s;
 }
 
-__device__ inline void load_tile(const float* __restrict__ Xp, uint32_t dp, uint32_t p0,
-                                 uint32_t np, float* tile) {
-  // np points of dp floats, float4 at a time
-  const uint32_t n4 = np * dp / 4;
-  const float4* src = reinterpret_cast<const float4*>(Xp + (size_t)p0 * dp);
-  float4* dst = reinterpret_cast<float4*>(tile);
-  for (uint32_t e = threadIdx.x; e < n4; e += blockDim.x) dst[e] = src[e];
-}
-
 // ── norms and the matrix-core filter ──
 // xn[p] = sum_t x_t^2 (FP32, dimension order); pmax = max over p, as the bits of
 // a non-negative float (unsigned order = value order).
@@ -142,7 +108,7 @@ __device__ inline float exact_threshold(float tau, float xq, const uint32_t* pma
 }
 
 
-constexpr int EXM_P = 64;  // points per LDS tile
+constexpr int EXM_P = EXM_P_TILE;  // points per LDS tile
 
 // The LDS tile holds 64 points, point r at r * TS floats with its dimensions
 // reordered so that the KS values lane group g reads (dims 4 s + g) are
@@ -576,20 +542,24 @@ template __global__ void k_exact_pass2<50>(const float*, const float*, uint32_t,
 // excluded), as Scrublet's per-sample kNN does (crates/polariseq-core/src/
 // doublets.rs:1085-1113, 1366-1374). Work units (sample, query block, point
 // segment) are pulled from an atomic counter by a grid sized to the device;
-// every block leaves the loop once the counter passes the unit count. The
-// block size (queries per block), the LDS tile and the segment length come
-// from the autotuner (capi.hip).
+// every block leaves the loop once the counter passes the unit count. A unit's
+// 64 queries run the matrix-core filter of the single-set form above; the
+// segment length comes from the autotuner (batched.hip).
 namespace psq {
 
 template <int D, int K>
 __global__ __launch_bounds__(256) void k_bexact_pass1(
-    const float* __restrict__ Xp, uint32_t d, uint32_t dp, const uint64_t* __restrict__ off,
-    const uint32_t* __restrict__ units, uint32_t U, uint32_t seg_len, uint32_t tile,
+    const float* __restrict__ Xp, const float* __restrict__ xn, uint32_t d, uint32_t dp,
+    const uint64_t* __restrict__ off, const uint32_t* __restrict__ units, uint32_t U, uint32_t seg_len,
     const uint32_t* __restrict__ nseg_of, const uint64_t* __restrict__ part_off,
     uint32_t* __restrict__ counter, float* __restrict__ partial) {
-  extern __shared__ float btile[];
+  constexpr int KSM = D > 0 ? (D + 3) / 4 : 64;
+  extern __shared__ float exm_lds[];
   __shared__ uint32_t s_u;
-  constexpr int DR = D > 0 ? D : 1;
+  const uint32_t KS = D > 0 ? (uint32_t)KSM : (d + 3) / 4, TS = exm_ts(KS);
+  float* tile = exm_lds;
+  float* tn = tile + (size_t)EXM_P * TS;
+  const uint32_t l = threadIdx.x % 64, w = threadIdx.x / 64, c = l % 16, g = l / 16;
   for (;;) {
     if (threadIdx.x == 0) s_u = atomicAdd(counter, 1u);
     __syncthreads();
@@ -599,56 +569,54 @@ __global__ __launch_bounds__(256) void k_bexact_pass1(
     const uint32_t s = units[3 * u], qb = units[3 * u + 1], seg = units[3 * u + 2];
     const uint64_t o = off[s];
     const uint32_t ns = (uint32_t)(off[s + 1] - o);
-    const uint32_t ql = qb * blockDim.x + threadIdx.x;
+    const uint32_t ql = qb * EXM_QUERIES + w * 16 + c;
     const bool qvalid = ql < ns;
     const uint64_t qi = o + (qvalid ? ql : 0u);
-    float qv[DR];
-    if (D > 0) {
+    float qf[KSM];
 #pragma unroll
-      for (int t = 0; t < DR; ++t) qv[t] = Xp[qi * dp + t];
-    }
-    const float* qrow = Xp + qi * dp;
+    for (int t = 0; t < KSM; ++t) qf[t] = (uint32_t)t < KS ? Xp[qi * dp + 4 * t + g] : 0.0f;
+    const float xq = xn[qi];
     float top[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) top[j] = __builtin_inff();
     const uint32_t s0 = seg * seg_len, s1 = min(ns, s0 + seg_len);
-    for (uint32_t base = s0; base < s1; base += tile) {
-      const uint32_t np = min(tile, s1 - base);
+    for (uint32_t base = s0; base < s1; base += EXM_P) {
+      const uint32_t np = min((uint32_t)EXM_P, s1 - base);
       __syncthreads();
-      {
-        const uint32_t n4 = np * dp / 4;
-        const float4* src = reinterpret_cast<const float4*>(Xp + (o + base) * dp);
-        float4* dst = reinterpret_cast<float4*>(btile);
-        for (uint32_t e = threadIdx.x; e < n4; e += blockDim.x) dst[e] = src[e];
-      }
+      exm_stage(Xp, xn, dp, KS, TS, (uint32_t)(o + base), np, tile, tn);
       __syncthreads();
-      for (uint32_t j = 0; j < np; ++j) {
-        const float* pr = btile + (size_t)j * dp;
-        float dist = D > 0 ? filt_dist_fixed<DR>(pr, qv) : filt_dist_rt(pr, qrow, (int)d);
-        if (base + j == ql) dist = __builtin_inff();
-        if (dist < top[K - 1]) {
+      for (uint32_t t = 0; t < (np + 15) / 16; ++t) {
+        const f32x4_t acc = exm_tile_dots<KSM>(tile, TS, KS, t, c, g, qf);
 #pragma unroll
-          for (int t = K - 1; t > 0; --t) {
-            float a = top[t - 1];
-            top[t] = dist < a ? a : (dist < top[t] ? dist : top[t]);
+        for (int i = 0; i < 4; ++i) {
+          const uint32_t r = 16 * t + 4 * g + i;
+          float e = fmaxf((tn[r] + xq) - 2.0f * acc[i], 0.0f);
+          if (r >= np || base + r == ql) e = __builtin_inff();
+          if (e < top[K - 1]) {
+#pragma unroll
+            for (int v = K - 1; v > 0; --v) {
+              const float a = top[v - 1];
+              top[v] = e < a ? a : (e < top[v] ? e : top[v]);
+            }
+            top[0] = e < top[0] ? e : top[0];
           }
-          top[0] = dist < top[0] ? dist : top[0];
         }
       }
     }
     if (qvalid) {
-      float* out = partial + part_off[s] + ((size_t)ql * nseg_of[s] + seg) * K;
+      float* out = partial + part_off[s] + (((size_t)ql * nseg_of[s] + seg) * 4 + g) * K;
 #pragma unroll
       for (int j = 0; j < K; ++j) out[j] = top[j];
     }
   }
 }
 
-// tau per global query: the k-th smallest over its sample's segments.
+// tau per global query: the k-th smallest over its sample's 4 nseg lists, then T.
 __global__ void k_bexact_tau(const float* __restrict__ partial, const uint64_t* __restrict__ off,
                              uint32_t S, const uint32_t* __restrict__ nseg_of,
                              const uint64_t* __restrict__ part_off, uint64_t nq, uint32_t K,
-                             uint32_t k, uint32_t d, float* __restrict__ T) {
+                             uint32_t k, uint32_t d, const float* __restrict__ xn,
+                             const uint32_t* __restrict__ pmax, float* __restrict__ T) {
   const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nq) return;
   uint32_t lo = 0, hi = S;  // sample of g: off[lo] <= g < off[lo + 1]
@@ -658,12 +626,12 @@ __global__ void k_bexact_tau(const float* __restrict__ partial, const uint64_t* 
   }
   const uint32_t s = lo;
   const uint64_t ql = g - off[s];
-  const uint32_t ns = nseg_of[s];
+  const uint32_t L = 4 * nseg_of[s] * K;
   float top[64];
 #pragma unroll
   for (int j = 0; j < 64; ++j) top[j] = __builtin_inff();
-  const float* src = partial + part_off[s] + ql * ns * K;
-  for (uint32_t e = 0; e < ns * K; ++e) {
+  const float* src = partial + part_off[s] + ql * L;
+  for (uint32_t e = 0; e < L; ++e) {
     const float v = src[e];
     if (v < top[63]) {
 #pragma unroll
@@ -678,26 +646,22 @@ __global__ void k_bexact_tau(const float* __restrict__ partial, const uint64_t* 
 #pragma unroll
   for (int j = 0; j < 64; ++j)
     if ((uint32_t)j == k - 1) tau = top[j];
-  if (tau >= __builtin_inff()) {
-    T[g] = tau;
-    return;
-  }
-  const double F = 1.0 + 4.0 * ((double)d + 8.0) * 5.9604644775390625e-08;
-  const double Td = (double)tau * F + 1e-30;
-  float Tf = (float)Td;
-  if ((double)Tf < Td) Tf = nextafterf(Tf, __builtin_inff());
-  T[g] = Tf;
+  T[g] = exact_threshold(tau, xn[g], pmax, d);
 }
 
 template <int D>
 __global__ __launch_bounds__(256) void k_bexact_pass2(
-    const float* __restrict__ Xp, uint32_t d, uint32_t dp, const uint64_t* __restrict__ off,
-    const uint32_t* __restrict__ units, uint32_t U, uint32_t seg_len, uint32_t tile,
+    const float* __restrict__ Xp, const float* __restrict__ xn, uint32_t d, uint32_t dp,
+    const uint64_t* __restrict__ off, const uint32_t* __restrict__ units, uint32_t U, uint32_t seg_len,
     const float* __restrict__ T, uint32_t* __restrict__ counter, uint32_t* __restrict__ cnt,
     uint32_t* __restrict__ cand, uint32_t cap) {
-  extern __shared__ float btile[];
+  constexpr int KSM = D > 0 ? (D + 3) / 4 : 64;
+  extern __shared__ float exm_lds[];
   __shared__ uint32_t s_u;
-  constexpr int DR = D > 0 ? D : 1;
+  const uint32_t KS = D > 0 ? (uint32_t)KSM : (d + 3) / 4, TS = exm_ts(KS);
+  float* tile = exm_lds;
+  float* tn = tile + (size_t)EXM_P * TS;
+  const uint32_t l = threadIdx.x % 64, w = threadIdx.x / 64, c = l % 16, g = l / 16;
   for (;;) {
     if (threadIdx.x == 0) s_u = atomicAdd(counter, 1u);
     __syncthreads();
@@ -707,74 +671,59 @@ __global__ __launch_bounds__(256) void k_bexact_pass2(
     const uint32_t s = units[3 * u], qb = units[3 * u + 1], seg = units[3 * u + 2];
     const uint64_t o = off[s];
     const uint32_t ns = (uint32_t)(off[s + 1] - o);
-    const uint32_t ql = qb * blockDim.x + threadIdx.x;
+    const uint32_t ql = qb * EXM_QUERIES + w * 16 + c;
     const bool qvalid = ql < ns;
     const uint64_t qi = o + (qvalid ? ql : 0u);
     const float thr = qvalid ? T[qi] : -1.0f;
-    float qv[DR];
-    if (D > 0) {
+    float qf[KSM];
 #pragma unroll
-      for (int t = 0; t < DR; ++t) qv[t] = Xp[qi * dp + t];
-    }
-    const float* qrow = Xp + qi * dp;
+    for (int t = 0; t < KSM; ++t) qf[t] = (uint32_t)t < KS ? Xp[qi * dp + 4 * t + g] : 0.0f;
+    const float xq = xn[qi];
     const uint32_t s0 = seg * seg_len, s1 = min(ns, s0 + seg_len);
-    for (uint32_t base = s0; base < s1; base += tile) {
-      const uint32_t np = min(tile, s1 - base);
+    for (uint32_t base = s0; base < s1; base += EXM_P) {
+      const uint32_t np = min((uint32_t)EXM_P, s1 - base);
       __syncthreads();
-      {
-        const uint32_t n4 = np * dp / 4;
-        const float4* src = reinterpret_cast<const float4*>(Xp + (o + base) * dp);
-        float4* dst = reinterpret_cast<float4*>(btile);
-        for (uint32_t e = threadIdx.x; e < n4; e += blockDim.x) dst[e] = src[e];
-      }
+      exm_stage(Xp, xn, dp, KS, TS, (uint32_t)(o + base), np, tile, tn);
       __syncthreads();
-      for (uint32_t j = 0; j < np; ++j) {
-        const float* pr = btile + (size_t)j * dp;
-        const float dist = D > 0 ? filt_dist_fixed<DR>(pr, qv) : filt_dist_rt(pr, qrow, (int)d);
-        if (qvalid && dist <= thr && base + j != ql) {
-          const uint32_t pos = atomicAdd(&cnt[qi], 1u);
-          if (pos < cap) cand[qi * cap + pos] = (uint32_t)(o + base + j);
+      for (uint32_t t = 0; t < (np + 15) / 16; ++t) {
+        const f32x4_t acc = exm_tile_dots<KSM>(tile, TS, KS, t, c, g, qf);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const uint32_t r = 16 * t + 4 * g + i;
+          const float e = fmaxf((tn[r] + xq) - 2.0f * acc[i], 0.0f);
+          if (r < np && e <= thr && base + r != ql) {
+            const uint32_t pos = atomicAdd(&cnt[qi], 1u);
+            if (pos < cap) cand[qi * cap + pos] = (uint32_t)(o + base + r);
+          }
         }
       }
     }
   }
 }
 
-template __global__ void k_bexact_pass1<0, 16>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
-                                               uint32_t, uint32_t, uint32_t, const uint32_t*, const uint64_t*,
-                                               uint32_t*, float*);
-template __global__ void k_bexact_pass1<0, 32>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
-                                               uint32_t, uint32_t, uint32_t, const uint32_t*, const uint64_t*,
-                                               uint32_t*, float*);
-template __global__ void k_bexact_pass1<0, 64>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
-                                               uint32_t, uint32_t, uint32_t, const uint32_t*, const uint64_t*,
-                                               uint32_t*, float*);
-template __global__ void k_bexact_pass1<30, 16>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
-                                                uint32_t, uint32_t, uint32_t, const uint32_t*, const uint64_t*,
-                                                uint32_t*, float*);
-template __global__ void k_bexact_pass1<30, 32>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
-                                                uint32_t, uint32_t, uint32_t, const uint32_t*, const uint64_t*,
-                                                uint32_t*, float*);
-template __global__ void k_bexact_pass1<30, 64>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
-                                                uint32_t, uint32_t, uint32_t, const uint32_t*, const uint64_t*,
-                                                uint32_t*, float*);
-template __global__ void k_bexact_pass1<50, 16>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
-                                                uint32_t, uint32_t, uint32_t, const uint32_t*, const uint64_t*,
-                                                uint32_t*, float*);
-template __global__ void k_bexact_pass1<50, 32>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
-                                                uint32_t, uint32_t, uint32_t, const uint32_t*, const uint64_t*,
-                                                uint32_t*, float*);
-template __global__ void k_bexact_pass1<50, 64>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
-                                                uint32_t, uint32_t, uint32_t, const uint32_t*, const uint64_t*,
-                                                uint32_t*, float*);
-template __global__ void k_bexact_pass2<0>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
-                                           uint32_t, uint32_t, uint32_t, const float*, uint32_t*, uint32_t*,
-                                           uint32_t*, uint32_t);
-template __global__ void k_bexact_pass2<30>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
-                                            uint32_t, uint32_t, uint32_t, const float*, uint32_t*, uint32_t*,
-                                            uint32_t*, uint32_t);
-template __global__ void k_bexact_pass2<50>(const float*, uint32_t, uint32_t, const uint64_t*, const uint32_t*,
-                                            uint32_t, uint32_t, uint32_t, const float*, uint32_t*, uint32_t*,
-                                            uint32_t*, uint32_t);
+template __global__ void k_bexact_pass1<0, 16>(const float*, const float*, uint32_t, uint32_t,
+    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t*, float*);
+template __global__ void k_bexact_pass1<0, 32>(const float*, const float*, uint32_t, uint32_t,
+    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t*, float*);
+template __global__ void k_bexact_pass1<0, 64>(const float*, const float*, uint32_t, uint32_t,
+    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t*, float*);
+template __global__ void k_bexact_pass1<30, 16>(const float*, const float*, uint32_t, uint32_t,
+    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t*, float*);
+template __global__ void k_bexact_pass1<30, 32>(const float*, const float*, uint32_t, uint32_t,
+    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t*, float*);
+template __global__ void k_bexact_pass1<30, 64>(const float*, const float*, uint32_t, uint32_t,
+    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t*, float*);
+template __global__ void k_bexact_pass1<50, 16>(const float*, const float*, uint32_t, uint32_t,
+    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t*, float*);
+template __global__ void k_bexact_pass1<50, 32>(const float*, const float*, uint32_t, uint32_t,
+    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t*, float*);
+template __global__ void k_bexact_pass1<50, 64>(const float*, const float*, uint32_t, uint32_t,
+    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t*, float*);
+template __global__ void k_bexact_pass2<0>(const float*, const float*, uint32_t, uint32_t,
+    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const float*, uint32_t*, uint32_t*, uint32_t*, uint32_t);
+template __global__ void k_bexact_pass2<30>(const float*, const float*, uint32_t, uint32_t,
+    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const float*, uint32_t*, uint32_t*, uint32_t*, uint32_t);
+template __global__ void k_bexact_pass2<50>(const float*, const float*, uint32_t, uint32_t,
+    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const float*, uint32_t*, uint32_t*, uint32_t*, uint32_t);
 
 }  // namespace psq

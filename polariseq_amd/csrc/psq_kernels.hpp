@@ -141,6 +141,7 @@ __global__ void k_exact_norms(const float* Xp, uint32_t n, uint32_t d, uint32_t 
                               uint32_t* pmax);
 size_t exm_smem(uint32_t d);  // dynamic LDS of k_exact_pass1 / k_exact_pass2
 constexpr uint32_t EXM_QUERIES = 64;  // queries per block of the two passes
+constexpr uint32_t EXM_P_TILE = 64;   // points per LDS tile of the two passes
 template <int D, int K>
 __global__ void k_exact_pass1(const float* Xp, const float* xn, uint32_t n, uint32_t d, uint32_t dp,
                               const uint32_t* queries, uint32_t nq, uint32_t seg_len,
@@ -171,18 +172,20 @@ __global__ void k_exact_fallback(const float* Xp, uint32_t n, uint32_t d, uint32
                                  uint32_t* si, uint32_t* idx_out, float* dist_out, const uint64_t* off,
                                  uint32_t S, double thr_scale);
 template <int D, int K>
-__global__ void k_bexact_pass1(const float* Xp, uint32_t d, uint32_t dp, const uint64_t* off,
-                               const uint32_t* units, uint32_t U, uint32_t seg_len, uint32_t tile,
-                               const uint32_t* nseg_of, const uint64_t* part_off, uint32_t* counter,
-                               float* partial);
+__global__ void k_bexact_pass1(const float* Xp, const float* xn, uint32_t d, uint32_t dp, const uint64_t* off,
+                               const uint32_t* units, uint32_t U, uint32_t seg_len, const uint32_t* nseg_of,
+                               const uint64_t* part_off, uint32_t* counter, float* partial);
 __global__ void k_bexact_tau(const float* partial, const uint64_t* off, uint32_t S,
                              const uint32_t* nseg_of, const uint64_t* part_off, uint64_t nq, uint32_t K,
-                             uint32_t k, uint32_t d, float* T);
+                             uint32_t k, uint32_t d, const float* xn, const uint32_t* pmax, float* T);
 template <int D>
-__global__ void k_bexact_pass2(const float* Xp, uint32_t d, uint32_t dp, const uint64_t* off,
-                               const uint32_t* units, uint32_t U, uint32_t seg_len, uint32_t tile,
-                               const float* T, uint32_t* counter, uint32_t* cnt, uint32_t* cand,
-                               uint32_t cap);
+__global__ void k_bexact_pass2(const float* Xp, const float* xn, uint32_t d, uint32_t dp, const uint64_t* off,
+                               const uint32_t* units, uint32_t U, uint32_t seg_len, const float* T,
+                               uint32_t* counter, uint32_t* cnt, uint32_t* cand, uint32_t cap);
+// The fallback's FP32 VALU distance is within (d + 4) 2^-24 of the exact one,
+// relative: the matrix-core threshold scaled by this bounds it for every true
+// neighbour.
+inline double exact_fallback_scale(uint64_t d) { return 1.0 + 4.0 * ((double)d + 8.0) * 5.9604644775390625e-08; }
 
 // Level-2 partition and reverse lists (partition.hip; layout in psq_partition.hpp)
 __global__ void k_part_l2_count(const uint32_t* sstart, const uint32_t* pj1, uint32_t sshift,
