@@ -96,11 +96,13 @@ void make_plan(const std::vector<uint64_t>& off, const Tile& t, uint32_t K, Plan
     acc += ns * P.nseg[s] * 4 * K;  // four lists per query and segment
   }
   P.partial_floats = acc;
+  // within a sample, query blocks vary fastest: the blocks running at once
+  // stream the same segment, which the L2 then serves
   for (uint32_t s : order) {
     const uint64_t ns = off[s + 1] - off[s];
     const uint32_t qb = (uint32_t)((ns + EXM_QUERIES - 1) / EXM_QUERIES);
-    for (uint32_t b = 0; b < qb; ++b)
-      for (uint32_t g = 0; g < P.nseg[s]; ++g) {
+    for (uint32_t g = 0; g < P.nseg[s]; ++g)
+      for (uint32_t b = 0; b < qb; ++b) {
         P.units.push_back(s);
         P.units.push_back(b);
         P.units.push_back(g);
@@ -122,7 +124,7 @@ int occupancy_grid(const void* kern, unsigned block, size_t smem, unsigned* grid
 template <int D, int K>
 int run_pass1(const float* Xp, const float* xn, uint32_t d, uint32_t dp, const uint64_t* d_off,
               const uint32_t* d_units, uint32_t U, const Tile& t, const uint32_t* d_nseg,
-              const uint64_t* d_poff, uint32_t* d_counter, float* d_partial, unsigned grid, hipStream_t s) {
+              const uint64_t* d_poff, uint32_t k, uint32_t* d_counter, float* d_partial, unsigned grid, hipStream_t s) {
   const size_t smem = exm_smem(d);
   auto kern = k_bexact_pass1<D, K>;
   if (smem > 65536)
@@ -130,7 +132,7 @@ int run_pass1(const float* Xp, const float* xn, uint32_t d, uint32_t dp, const u
   BTRY(occupancy_grid((const void*)kern, t.block, smem, &grid));
   BHIPC(hipMemsetAsync(d_counter, 0, 4, s));
   hipLaunchKernelGGL(kern, dim3(grid), dim3(t.block), smem, s, Xp, xn, d, dp, d_off, d_units, U, t.seg_len,
-                     d_nseg, d_poff, d_counter, d_partial);
+                     d_nseg, d_poff, k, d_counter, d_partial);
   BHIPC(hipGetLastError());
   return PSQ_OK;
 }
@@ -138,11 +140,11 @@ int run_pass1(const float* Xp, const float* xn, uint32_t d, uint32_t dp, const u
 template <int D>
 int dispatch_pass1(uint32_t K, const float* Xp, const float* xn, uint32_t d, uint32_t dp,
                    const uint64_t* d_off, const uint32_t* d_units, uint32_t U, const Tile& t,
-                   const uint32_t* d_nseg, const uint64_t* d_poff, uint32_t* d_counter, float* d_partial,
+                   const uint32_t* d_nseg, const uint64_t* d_poff, uint32_t k, uint32_t* d_counter, float* d_partial,
                    unsigned grid, hipStream_t s) {
-  if (K == 16) return run_pass1<D, 16>(Xp, xn, d, dp, d_off, d_units, U, t, d_nseg, d_poff, d_counter, d_partial, grid, s);
-  if (K == 32) return run_pass1<D, 32>(Xp, xn, d, dp, d_off, d_units, U, t, d_nseg, d_poff, d_counter, d_partial, grid, s);
-  return run_pass1<D, 64>(Xp, xn, d, dp, d_off, d_units, U, t, d_nseg, d_poff, d_counter, d_partial, grid, s);
+  if (K == 16) return run_pass1<D, 16>(Xp, xn, d, dp, d_off, d_units, U, t, d_nseg, d_poff, k, d_counter, d_partial, grid, s);
+  if (K == 32) return run_pass1<D, 32>(Xp, xn, d, dp, d_off, d_units, U, t, d_nseg, d_poff, k, d_counter, d_partial, grid, s);
+  return run_pass1<D, 64>(Xp, xn, d, dp, d_off, d_units, U, t, d_nseg, d_poff, k, d_counter, d_partial, grid, s);
 }
 
 
@@ -257,11 +259,11 @@ extern "C" int psq_knn_exact_batched_device(const float* d_data, uint64_t n_tota
       unsigned grid = 0;  // sized per kernel at its launch
       BHIPC(hipEventRecord(a, s));
       if (dfix == 50)
-        BTRY(dispatch_pass1<50>(K, Xprobe, xnprobe, (uint32_t)d, dp, d_poff2, d_units2, U, t, d_nseg2, d_pofs, counter, d_part2, grid, s));
+        BTRY(dispatch_pass1<50>(K, Xprobe, xnprobe, (uint32_t)d, dp, d_poff2, d_units2, U, t, d_nseg2, d_pofs, (uint32_t)k, counter, d_part2, grid, s));
       else if (dfix == 30)
-        BTRY(dispatch_pass1<30>(K, Xprobe, xnprobe, (uint32_t)d, dp, d_poff2, d_units2, U, t, d_nseg2, d_pofs, counter, d_part2, grid, s));
+        BTRY(dispatch_pass1<30>(K, Xprobe, xnprobe, (uint32_t)d, dp, d_poff2, d_units2, U, t, d_nseg2, d_pofs, (uint32_t)k, counter, d_part2, grid, s));
       else
-        BTRY(dispatch_pass1<0>(K, Xprobe, xnprobe, (uint32_t)d, dp, d_poff2, d_units2, U, t, d_nseg2, d_pofs, counter, d_part2, grid, s));
+        BTRY(dispatch_pass1<0>(K, Xprobe, xnprobe, (uint32_t)d, dp, d_poff2, d_units2, U, t, d_nseg2, d_pofs, (uint32_t)k, counter, d_part2, grid, s));
       BHIPC(hipEventRecord(b, s));
       BHIPC(hipEventSynchronize(b));
       float ms = 0;
@@ -302,11 +304,11 @@ extern "C" int psq_knn_exact_batched_device(const float* d_data, uint64_t n_tota
   BHIPC(hipEventCreate(&e3));
   BHIPC(hipEventRecord(e0, s));
   if (dfix == 50)
-    BTRY(dispatch_pass1<50>(K, Xp, xn, (uint32_t)d, dp, d_off, d_units, U, best, d_nseg, d_poff, counter, d_partial, grid, s));
+    BTRY(dispatch_pass1<50>(K, Xp, xn, (uint32_t)d, dp, d_off, d_units, U, best, d_nseg, d_poff, (uint32_t)k, counter, d_partial, grid, s));
   else if (dfix == 30)
-    BTRY(dispatch_pass1<30>(K, Xp, xn, (uint32_t)d, dp, d_off, d_units, U, best, d_nseg, d_poff, counter, d_partial, grid, s));
+    BTRY(dispatch_pass1<30>(K, Xp, xn, (uint32_t)d, dp, d_off, d_units, U, best, d_nseg, d_poff, (uint32_t)k, counter, d_partial, grid, s));
   else
-    BTRY(dispatch_pass1<0>(K, Xp, xn, (uint32_t)d, dp, d_off, d_units, U, best, d_nseg, d_poff, counter, d_partial, grid, s));
+    BTRY(dispatch_pass1<0>(K, Xp, xn, (uint32_t)d, dp, d_off, d_units, U, best, d_nseg, d_poff, (uint32_t)k, counter, d_partial, grid, s));
   BHIPC(hipEventRecord(e1, s));
   hipLaunchKernelGGL(k_bexact_tau, dim3(gsize(n_total, 128)), dim3(128), 0, s, d_partial, d_off,
                      (uint32_t)n_samples, d_nseg, d_poff, n_total, K, (uint32_t)k, (uint32_t)d, xn, pmx, T);

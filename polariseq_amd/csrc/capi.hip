@@ -1273,7 +1273,7 @@ int knn_exact_big(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d
   const size_t smem = exm_smem((uint32_t)d);
   const uint32_t dfix = d == 50 ? 50u : (d == 30 ? 30u : 0u);
   void (*p1)(const float*, const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t,
-             uint32_t, uint32_t, float*) =
+             uint32_t, uint32_t, uint32_t, float*) =
       K == 16   ? (dfix == 50 ? k_exact_pass1<50, 16> : (dfix == 30 ? k_exact_pass1<30, 16> : k_exact_pass1<0, 16>))
       : K == 32 ? (dfix == 50 ? k_exact_pass1<50, 32> : (dfix == 30 ? k_exact_pass1<30, 32> : k_exact_pass1<0, 32>))
                 : (dfix == 50 ? k_exact_pass1<50, 64> : (dfix == 30 ? k_exact_pass1<30, 64> : k_exact_pass1<0, 64>));
@@ -1301,7 +1301,7 @@ int knn_exact_big(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d
     }
     const uint32_t qblocks = (qn + EXM_QUERIES - 1) / EXM_QUERIES;
     hipLaunchKernelGGL(p1, dim3(qblocks, nseg), dim3(256), smem, s, Xp, xn, (uint32_t)n, (uint32_t)d, dp,
-                       qp, qn, seg_len, nseg, partial);
+                       qp, qn, seg_len, nseg, (uint32_t)k, partial);
     HIPC(hipGetLastError());
     hipLaunchKernelGGL(k_exact_tau_big, dim3(qn), dim3(256), tau_smem, s, partial, qn, L, (uint32_t)k,
                        (uint32_t)d, qp, xn, pmx, T);
@@ -1414,13 +1414,13 @@ int knn_exact_small(BufSrc& B, const float* d_data, uint64_t n, uint64_t d, cons
   const uint32_t dfix = d == 50 ? 50u : (d == 30 ? 30u : 0u);
   dim3 g1(qblocks, nseg);
   void (*p1)(const float*, const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t,
-             uint32_t, uint32_t, float*);
+             uint32_t, uint32_t, uint32_t, float*);
   if (K == 16) p1 = dfix == 50 ? k_exact_pass1<50, 16> : (dfix == 30 ? k_exact_pass1<30, 16> : k_exact_pass1<0, 16>);
   else if (K == 32) p1 = dfix == 50 ? k_exact_pass1<50, 32> : (dfix == 30 ? k_exact_pass1<30, 32> : k_exact_pass1<0, 32>);
   else p1 = dfix == 50 ? k_exact_pass1<50, 64> : (dfix == 30 ? k_exact_pass1<30, 64> : k_exact_pass1<0, 64>);
   if (smem > 65536) HIPC(hipFuncSetAttribute((const void*)p1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
   hipLaunchKernelGGL(p1, g1, dim3(256), smem, s, Xp, xn, (uint32_t)n, (uint32_t)d, dp, d_queries,
-                     (uint32_t)nq, seg_len, nseg, partial);
+                     (uint32_t)nq, seg_len, nseg, (uint32_t)k, partial);
   HIPC(hipGetLastError());
   hipLaunchKernelGGL(k_exact_tau, dim3(grid_for(nq, 128)), dim3(128), 0, s, partial, (uint32_t)nq,
                      nseg * 4 * K, (uint32_t)k, (uint32_t)d, d_queries, xn, pmx, T);

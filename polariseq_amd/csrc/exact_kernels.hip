@@ -110,40 +110,145 @@ __device__ inline float exact_threshold(float tau, float xq, const uint32_t* pma
 
 constexpr int EXM_P = EXM_P_TILE;  // points per LDS tile
 
-// The LDS tile holds 64 points, point r at r * TS floats with its dimensions
-// reordered so that the KS values lane group g reads (dims 4 s + g) are
-// contiguous: element (g, s) at g * KS + s. TS = 4 KS, or 4 KS + 4 when that
-// is 0 mod 8, keeps the 64 lanes of a read on distinct banks.
+// The LDS tile holds 64 points, point r at r * TS floats, its dimensions in
+// order (dimensions d .. 4 KS - 1 are the zero padding of Xp). Lane group g
+// takes dimensions g KS .. g KS + KS - 1 of point and query, one per matrix
+// step s: word (16 t + c) TS + g KS + s of a read lands on 64 distinct banks
+// when TS = 4 KS is 4 mod 8, and TS = 4 KS + 4 restores that when 4 KS is
+// 0 mod 8.
 __host__ __device__ constexpr uint32_t exm_ts(uint32_t ks) { return (4 * ks) % 8 == 0 ? 4 * ks + 4 : 4 * ks; }
 __host__ size_t exm_smem(uint32_t d) {
   const uint32_t ks = (d + 3) / 4;
   return ((size_t)EXM_P * exm_ts(ks) + EXM_P) * sizeof(float);
 }
+constexpr uint32_t EXM_THREADS = 4 * WAVE;  // 16 queries per wave
 
-// Stage points [base, base + np) into the tile and their norms into tn.
+// Stage rows [base, base + np) of Xp into the tile, four floats at a time, and
+// their norms into tn. Tile rows past np repeat row base + np - 1 (finite) with
+// norm +inf, so their filter values are +inf. Each thread issues its loads before its first store: the
+// addresses are clamped instead of guarded, and the empty asm keeps the
+// compiler from moving a load under the store's condition.
+template <int KSM>
 __device__ inline void exm_stage(const float* __restrict__ Xp, const float* __restrict__ xn, uint32_t dp,
                                  uint32_t KS, uint32_t TS, uint32_t base, uint32_t np, float* tile,
                                  float* tn) {
-  const uint32_t per = 4 * KS;
-  for (uint32_t e = threadIdx.x; e < (uint32_t)EXM_P * per; e += blockDim.x) {
-    const uint32_t r = e / per, j = e - r * per;  // j = dimension 4 s + g
-    const uint32_t sidx = j / 4, gg = j % 4;
-    tile[r * TS + gg * KS + sidx] = r < np ? Xp[(size_t)(base + r) * dp + j] : 0.0f;
+  constexpr int NV = (EXM_P * KSM + EXM_THREADS - 1) / EXM_THREADS;  // float4s per thread
+  constexpr int CH = NV < 4 ? NV : 4;
+  const uint32_t tot = EXM_P * KS, last = base + np - 1;
+  float nv = xn[min(base + threadIdx.x % EXM_P, last)];
+  asm volatile("" : "+v"(nv));
+  for (uint32_t e0 = threadIdx.x; e0 < tot; e0 += CH * EXM_THREADS) {
+    float4 v[CH];
+#pragma unroll
+    for (int h = 0; h < CH; ++h) {
+      const uint32_t e = min(e0 + h * EXM_THREADS, tot - 1);
+      const uint32_t r = e / KS, j = e - r * KS;
+      v[h] = *reinterpret_cast<const float4*>(Xp + (size_t)min(base + r, last) * dp + 4 * j);
+    }
+#pragma unroll
+    for (int h = 0; h < CH; ++h) {
+      asm volatile("" : "+v"(v[h].x), "+v"(v[h].y), "+v"(v[h].z), "+v"(v[h].w));
+      const uint32_t e = e0 + h * EXM_THREADS;
+      if (e < tot) {
+        const uint32_t r = e / KS, j = e - r * KS;
+        *reinterpret_cast<float4*>(tile + r * TS + 4 * j) = v[h];
+      }
+    }
   }
-  for (uint32_t r = threadIdx.x; r < (uint32_t)EXM_P; r += blockDim.x) tn[r] = r < np ? xn[base + r] : 0.0f;
+  if (threadIdx.x < (uint32_t)EXM_P) tn[threadIdx.x] = threadIdx.x < np ? nv : __builtin_inff();
 }
 
-// The filter values of one 16-point sub-tile for this lane's query: e[i] for
-// point 16 t + 4 g + i of the tile.
+// The lane's query registers: qf[s] = dimension g KS + s of row qi.
 template <int KSM>
-__device__ inline f32x4_t exm_tile_dots(const float* tile, uint32_t TS, uint32_t KS, uint32_t t,
-                                        uint32_t c, uint32_t g, const float (&qf)[KSM]) {
-  f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
-  const float* a = tile + (16 * t + c) * TS + g * KS;
+__device__ inline void exm_load_query(const float* __restrict__ Xp, uint32_t dp, uint32_t KS, uint64_t qi,
+                                      uint32_t g, float (&qf)[KSM]) {
 #pragma unroll
-  for (int s = 0; s < KSM; ++s)
-    if ((uint32_t)s < KS) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], qf[s], acc, 0, 0, 0);
-  return acc;
+  for (int s = 0; s < KSM; ++s) qf[s] = (uint32_t)s < KS ? Xp[qi * dp + g * KS + s] : 0.0f;
+}
+
+// Scan rows [p0, p1) for the lane's query (16 queries per wave, lane (c, g)
+// holding query c): per 64-row tile, four 16-row v_mfma_f32_16x16x4f32 chains
+// over the dimensions, interleaved, give the dot products, and f(base, np, e)
+// runs once per tile with e[4 t + i] = (xn[p] + xn[q]) - 2 p.q for row
+// p = base + 16 t + 4 g + i (+inf for rows past np). The clamp at 0 is left
+// out: it only matters for bounding e from below, and the error bound holds
+// without it.
+template <int KSM, typename F>
+__device__ inline void exm_scan(const float* __restrict__ Xp, const float* __restrict__ xn, uint32_t dp,
+                                uint32_t KS, uint32_t TS, uint32_t p0, uint32_t p1, float* tile, float* tn,
+                                uint32_t c, uint32_t g, const float (&qf)[KSM], float xq, F&& f) {
+  for (uint32_t base = p0; base < p1; base += EXM_P) {
+    const uint32_t np = min((uint32_t)EXM_P, p1 - base);
+    __syncthreads();
+    exm_stage<KSM>(Xp, xn, dp, KS, TS, base, np, tile, tn);
+    __syncthreads();
+    f32x4_t acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    const float* a = tile + c * TS + g * KS;
+#pragma unroll
+    for (int s = 0; s < KSM; ++s)
+      if ((uint32_t)s < KS) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[16 * t * TS + s], qf[s], acc[t], 0, 0, 0);
+      }
+    float e[16];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const float4 nr = *reinterpret_cast<const float4*>(tn + 16 * t + 4 * g);
+      e[4 * t + 0] = fmaf(-2.0f, acc[t][0], nr.x + xq);
+      e[4 * t + 1] = fmaf(-2.0f, acc[t][1], nr.y + xq);
+      e[4 * t + 2] = fmaf(-2.0f, acc[t][2], nr.z + xq);
+      e[4 * t + 3] = fmaf(-2.0f, acc[t][3], nr.w + xq);
+    }
+    f(base, np, e);
+  }
+}
+
+// Smallest of the lane's 16 values of a tile.
+__device__ inline float exm_min16(const float (&e)[16]) {
+  float m0 = fminf(fminf(e[0], e[1]), e[2]), m1 = fminf(fminf(e[3], e[4]), e[5]);
+  float m2 = fminf(fminf(e[6], e[7]), e[8]), m3 = fminf(fminf(e[9], e[10]), e[11]);
+  float m4 = fminf(fminf(e[12], e[13]), e[14]);
+  return fminf(fminf(fminf(m0, m1), fminf(m2, m3)), fminf(m4, e[15]));
+}
+
+// Insert v into the ascending list top[0..K): entry u becomes the median of
+// (top[u-1], v, top[u]).
+template <int K>
+__device__ inline void exm_insert(float (&top)[K], float v) {
+#pragma unroll
+  for (int u = K - 1; u > 0; --u) top[u] = __builtin_amdgcn_fmed3f(top[u - 1], v, top[u]);
+  top[0] = fminf(top[0], v);
+}
+
+// Pass 1 of one tile for the lane's list top (K smallest values of rows
+// 4 g .. 4 g + 3 of every 16, the query row qi excluded). A value at or above
+// bnd cannot change the query's k smallest over its four lists: the four lanes
+// of the query together hold at least k values at or below the shared bound
+// (jb = ceil(k / 4) - 1 < K / 4: every list's entry jb, maximised over the four
+// lanes), and the lane's own K >= k values bound its list. Only lanes with a
+// value below the bound enter the insertion.
+template <int K>
+__device__ inline void exm_pass1_tile(float (&top)[K], float& shared_bnd, uint32_t jb, uint32_t base,
+                                      uint32_t g, uint32_t qi, const float (&e)[16]) {
+  const float bnd = fminf(top[K - 1], shared_bnd);
+  if (exm_min16(e) < bnd) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const uint32_t p = base + 16 * (j / 4) + 4 * g + (j % 4);
+      if (e[j] < fminf(top[K - 1], shared_bnd) && p != qi) exm_insert<K>(top, e[j]);
+    }
+  }
+  if (jb < (uint32_t)K / 4) {
+    float b = top[0];
+#pragma unroll
+    for (int j = 1; j < K / 4; ++j) b = (uint32_t)j == jb ? top[j] : b;
+    b = fmaxf(b, __shfl_xor(b, 16, WAVE));
+    b = fmaxf(b, __shfl_xor(b, 32, WAVE));
+    shared_bnd = b;
+  }
 }
 
 template <int D, int K>
@@ -152,7 +257,7 @@ __global__ __launch_bounds__(256) void k_exact_pass1(const float* __restrict__ X
                                                      uint32_t d, uint32_t dp,
                                                      const uint32_t* __restrict__ queries,
                                                      uint32_t nq, uint32_t seg_len, uint32_t nseg,
-                                                     float* __restrict__ partial) {
+                                                     uint32_t k, float* __restrict__ partial) {
   constexpr int KSM = D > 0 ? (D + 3) / 4 : 64;
   extern __shared__ float exm_lds[];
   const uint32_t KS = D > 0 ? (uint32_t)KSM : (d + 3) / 4, TS = exm_ts(KS);
@@ -163,37 +268,18 @@ __global__ __launch_bounds__(256) void k_exact_pass1(const float* __restrict__ X
   const bool qvalid = qslot < nq;
   const uint32_t qi = qvalid ? (queries ? queries[qslot] : qslot) : 0u;
   float qf[KSM];
-#pragma unroll
-  for (int s = 0; s < KSM; ++s) qf[s] = (uint32_t)s < KS ? Xp[(size_t)qi * dp + 4 * s + g] : 0.0f;
+  exm_load_query<KSM>(Xp, dp, KS, qi, g, qf);
   const float xq = xn[qi];
   float top[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) top[j] = __builtin_inff();
+  float sb = __builtin_inff();
+  const uint32_t jb = (k + 3) / 4 - 1;
   const uint32_t s0 = blockIdx.y * seg_len;
-  const uint32_t s1 = min(n, s0 + seg_len);
-  for (uint32_t base = s0; base < s1; base += EXM_P) {
-    const uint32_t np = min((uint32_t)EXM_P, s1 - base);
-    __syncthreads();
-    exm_stage(Xp, xn, dp, KS, TS, base, np, tile, tn);
-    __syncthreads();
-    for (uint32_t t = 0; t < (np + 15) / 16; ++t) {
-      const f32x4_t acc = exm_tile_dots<KSM>(tile, TS, KS, t, c, g, qf);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const uint32_t r = 16 * t + 4 * g + i;
-        float e = fmaxf((tn[r] + xq) - 2.0f * acc[i], 0.0f);
-        if (r >= np || base + r == qi) e = __builtin_inff();
-        if (e < top[K - 1]) {
-#pragma unroll
-          for (int u = K - 1; u > 0; --u) {
-            const float a = top[u - 1];
-            top[u] = e < a ? a : (e < top[u] ? e : top[u]);
-          }
-          top[0] = e < top[0] ? e : top[0];
-        }
-      }
-    }
-  }
+  exm_scan<KSM>(Xp, xn, dp, KS, TS, s0, min(n, s0 + seg_len), tile, tn, c, g, qf, xq,
+                [&](uint32_t base, uint32_t, const float (&e)[16]) {
+                  exm_pass1_tile<K>(top, sb, jb, base, g, qi, e);
+                });
   if (qvalid) {
     float* out = partial + (((size_t)qslot * nseg + blockIdx.y) * 4 + g) * K;
 #pragma unroll
@@ -250,29 +336,22 @@ __global__ __launch_bounds__(256) void k_exact_pass2(const float* __restrict__ X
   const uint32_t qi = qvalid ? (queries ? queries[qslot] : qslot) : 0u;
   const float thr = qvalid ? T[qslot] : -1.0f;
   float qf[KSM];
-#pragma unroll
-  for (int s = 0; s < KSM; ++s) qf[s] = (uint32_t)s < KS ? Xp[(size_t)qi * dp + 4 * s + g] : 0.0f;
+  exm_load_query<KSM>(Xp, dp, KS, qi, g, qf);
   const float xq = xn[qi];
   const uint32_t s0 = blockIdx.y * seg_len;
-  const uint32_t s1 = min(n, s0 + seg_len);
-  for (uint32_t base = s0; base < s1; base += EXM_P) {
-    const uint32_t np = min((uint32_t)EXM_P, s1 - base);
-    __syncthreads();
-    exm_stage(Xp, xn, dp, KS, TS, base, np, tile, tn);
-    __syncthreads();
-    for (uint32_t t = 0; t < (np + 15) / 16; ++t) {
-      const f32x4_t acc = exm_tile_dots<KSM>(tile, TS, KS, t, c, g, qf);
+  exm_scan<KSM>(Xp, xn, dp, KS, TS, s0, min(n, s0 + seg_len), tile, tn, c, g, qf, xq,
+                [&](uint32_t base, uint32_t np, const float (&e)[16]) {
+                  if (exm_min16(e) <= thr) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const uint32_t r = 16 * t + 4 * g + i;
-        const float e = fmaxf((tn[r] + xq) - 2.0f * acc[i], 0.0f);
-        if (r < np && e <= thr && base + r != qi) {
-          const uint32_t pos = atomicAdd(&cnt[qslot], 1u);
-          if (pos < cap) cand[(size_t)qslot * cap + pos] = base + r;
-        }
-      }
-    }
-  }
+                    for (int j = 0; j < 16; ++j) {
+                      const uint32_t r = 16 * (j / 4) + 4 * g + (j % 4);
+                      if (e[j] <= thr && r < np && base + r != qi) {
+                        const uint32_t pos = atomicAdd(&cnt[qslot], 1u);
+                        if (pos < cap) cand[(size_t)qslot * cap + pos] = base + r;
+                      }
+                    }
+                  }
+                });
 }
 
 // One wave per query: rank each candidate by (d64, index) among all of them.
@@ -509,23 +588,23 @@ __global__ void k_iota_u32(uint32_t* __restrict__ q, uint32_t q0, uint32_t count
 }
 
 template __global__ void k_exact_pass1<0, 16>(const float*, const float*, uint32_t, uint32_t, uint32_t,
-    const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+    const uint32_t*, uint32_t, uint32_t, uint32_t, uint32_t, float*);
 template __global__ void k_exact_pass1<0, 32>(const float*, const float*, uint32_t, uint32_t, uint32_t,
-    const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+    const uint32_t*, uint32_t, uint32_t, uint32_t, uint32_t, float*);
 template __global__ void k_exact_pass1<0, 64>(const float*, const float*, uint32_t, uint32_t, uint32_t,
-    const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+    const uint32_t*, uint32_t, uint32_t, uint32_t, uint32_t, float*);
 template __global__ void k_exact_pass1<30, 16>(const float*, const float*, uint32_t, uint32_t, uint32_t,
-    const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+    const uint32_t*, uint32_t, uint32_t, uint32_t, uint32_t, float*);
 template __global__ void k_exact_pass1<30, 32>(const float*, const float*, uint32_t, uint32_t, uint32_t,
-    const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+    const uint32_t*, uint32_t, uint32_t, uint32_t, uint32_t, float*);
 template __global__ void k_exact_pass1<30, 64>(const float*, const float*, uint32_t, uint32_t, uint32_t,
-    const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+    const uint32_t*, uint32_t, uint32_t, uint32_t, uint32_t, float*);
 template __global__ void k_exact_pass1<50, 16>(const float*, const float*, uint32_t, uint32_t, uint32_t,
-    const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+    const uint32_t*, uint32_t, uint32_t, uint32_t, uint32_t, float*);
 template __global__ void k_exact_pass1<50, 32>(const float*, const float*, uint32_t, uint32_t, uint32_t,
-    const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+    const uint32_t*, uint32_t, uint32_t, uint32_t, uint32_t, float*);
 template __global__ void k_exact_pass1<50, 64>(const float*, const float*, uint32_t, uint32_t, uint32_t,
-    const uint32_t*, uint32_t, uint32_t, uint32_t, float*);
+    const uint32_t*, uint32_t, uint32_t, uint32_t, uint32_t, float*);
 template __global__ void k_exact_pass2<0>(const float*, const float*, uint32_t, uint32_t, uint32_t,
     const uint32_t*, uint32_t, uint32_t, const float*, uint32_t*, uint32_t*, uint32_t);
 template __global__ void k_exact_pass2<30>(const float*, const float*, uint32_t, uint32_t, uint32_t,
@@ -551,7 +630,7 @@ template <int D, int K>
 __global__ __launch_bounds__(256) void k_bexact_pass1(
     const float* __restrict__ Xp, const float* __restrict__ xn, uint32_t d, uint32_t dp,
     const uint64_t* __restrict__ off, const uint32_t* __restrict__ units, uint32_t U, uint32_t seg_len,
-    const uint32_t* __restrict__ nseg_of, const uint64_t* __restrict__ part_off,
+    const uint32_t* __restrict__ nseg_of, const uint64_t* __restrict__ part_off, uint32_t k,
     uint32_t* __restrict__ counter, float* __restrict__ partial) {
   constexpr int KSM = D > 0 ? (D + 3) / 4 : 64;
   extern __shared__ float exm_lds[];
@@ -567,42 +646,24 @@ __global__ __launch_bounds__(256) void k_bexact_pass1(
     __syncthreads();
     if (u >= U) break;
     const uint32_t s = units[3 * u], qb = units[3 * u + 1], seg = units[3 * u + 2];
-    const uint64_t o = off[s];
-    const uint32_t ns = (uint32_t)(off[s + 1] - o);
+    const uint32_t o = (uint32_t)off[s];
+    const uint32_t ns = (uint32_t)off[s + 1] - o;
     const uint32_t ql = qb * EXM_QUERIES + w * 16 + c;
     const bool qvalid = ql < ns;
-    const uint64_t qi = o + (qvalid ? ql : 0u);
+    const uint32_t qi = o + (qvalid ? ql : 0u);
     float qf[KSM];
-#pragma unroll
-    for (int t = 0; t < KSM; ++t) qf[t] = (uint32_t)t < KS ? Xp[qi * dp + 4 * t + g] : 0.0f;
+    exm_load_query<KSM>(Xp, dp, KS, qi, g, qf);
     const float xq = xn[qi];
     float top[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) top[j] = __builtin_inff();
-    const uint32_t s0 = seg * seg_len, s1 = min(ns, s0 + seg_len);
-    for (uint32_t base = s0; base < s1; base += EXM_P) {
-      const uint32_t np = min((uint32_t)EXM_P, s1 - base);
-      __syncthreads();
-      exm_stage(Xp, xn, dp, KS, TS, (uint32_t)(o + base), np, tile, tn);
-      __syncthreads();
-      for (uint32_t t = 0; t < (np + 15) / 16; ++t) {
-        const f32x4_t acc = exm_tile_dots<KSM>(tile, TS, KS, t, c, g, qf);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const uint32_t r = 16 * t + 4 * g + i;
-          float e = fmaxf((tn[r] + xq) - 2.0f * acc[i], 0.0f);
-          if (r >= np || base + r == ql) e = __builtin_inff();
-          if (e < top[K - 1]) {
-#pragma unroll
-            for (int v = K - 1; v > 0; --v) {
-              const float a = top[v - 1];
-              top[v] = e < a ? a : (e < top[v] ? e : top[v]);
-            }
-            top[0] = e < top[0] ? e : top[0];
-          }
-        }
-      }
-    }
+    float sb = __builtin_inff();
+    const uint32_t jb = (k + 3) / 4 - 1;
+    const uint32_t s0 = o + seg * seg_len;
+    exm_scan<KSM>(Xp, xn, dp, KS, TS, s0, o + min(ns, seg * seg_len + seg_len), tile, tn, c, g, qf, xq,
+                  [&](uint32_t base, uint32_t, const float (&e)[16]) {
+                    exm_pass1_tile<K>(top, sb, jb, base, g, qi, e);
+                  });
     if (qvalid) {
       float* out = partial + part_off[s] + (((size_t)ql * nseg_of[s] + seg) * 4 + g) * K;
 #pragma unroll
@@ -669,56 +730,50 @@ __global__ __launch_bounds__(256) void k_bexact_pass2(
     __syncthreads();
     if (u >= U) break;
     const uint32_t s = units[3 * u], qb = units[3 * u + 1], seg = units[3 * u + 2];
-    const uint64_t o = off[s];
-    const uint32_t ns = (uint32_t)(off[s + 1] - o);
+    const uint32_t o = (uint32_t)off[s];
+    const uint32_t ns = (uint32_t)off[s + 1] - o;
     const uint32_t ql = qb * EXM_QUERIES + w * 16 + c;
     const bool qvalid = ql < ns;
-    const uint64_t qi = o + (qvalid ? ql : 0u);
+    const uint32_t qi = o + (qvalid ? ql : 0u);
     const float thr = qvalid ? T[qi] : -1.0f;
     float qf[KSM];
-#pragma unroll
-    for (int t = 0; t < KSM; ++t) qf[t] = (uint32_t)t < KS ? Xp[qi * dp + 4 * t + g] : 0.0f;
+    exm_load_query<KSM>(Xp, dp, KS, qi, g, qf);
     const float xq = xn[qi];
-    const uint32_t s0 = seg * seg_len, s1 = min(ns, s0 + seg_len);
-    for (uint32_t base = s0; base < s1; base += EXM_P) {
-      const uint32_t np = min((uint32_t)EXM_P, s1 - base);
-      __syncthreads();
-      exm_stage(Xp, xn, dp, KS, TS, (uint32_t)(o + base), np, tile, tn);
-      __syncthreads();
-      for (uint32_t t = 0; t < (np + 15) / 16; ++t) {
-        const f32x4_t acc = exm_tile_dots<KSM>(tile, TS, KS, t, c, g, qf);
+    const uint32_t s0 = o + seg * seg_len;
+    exm_scan<KSM>(Xp, xn, dp, KS, TS, s0, o + min(ns, seg * seg_len + seg_len), tile, tn, c, g, qf, xq,
+                  [&](uint32_t base, uint32_t np, const float (&e)[16]) {
+                    if (exm_min16(e) <= thr) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const uint32_t r = 16 * t + 4 * g + i;
-          const float e = fmaxf((tn[r] + xq) - 2.0f * acc[i], 0.0f);
-          if (r < np && e <= thr && base + r != ql) {
-            const uint32_t pos = atomicAdd(&cnt[qi], 1u);
-            if (pos < cap) cand[qi * cap + pos] = (uint32_t)(o + base + r);
-          }
-        }
-      }
-    }
+                      for (int j = 0; j < 16; ++j) {
+                        const uint32_t r = 16 * (j / 4) + 4 * g + (j % 4);
+                        if (e[j] <= thr && r < np && base + r != qi) {
+                          const uint32_t pos = atomicAdd(&cnt[qi], 1u);
+                          if (pos < cap) cand[(size_t)qi * cap + pos] = base + r;
+                        }
+                      }
+                    }
+                  });
   }
 }
 
 template __global__ void k_bexact_pass1<0, 16>(const float*, const float*, uint32_t, uint32_t,
-    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t*, float*);
+    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t, uint32_t*, float*);
 template __global__ void k_bexact_pass1<0, 32>(const float*, const float*, uint32_t, uint32_t,
-    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t*, float*);
+    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t, uint32_t*, float*);
 template __global__ void k_bexact_pass1<0, 64>(const float*, const float*, uint32_t, uint32_t,
-    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t*, float*);
+    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t, uint32_t*, float*);
 template __global__ void k_bexact_pass1<30, 16>(const float*, const float*, uint32_t, uint32_t,
-    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t*, float*);
+    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t, uint32_t*, float*);
 template __global__ void k_bexact_pass1<30, 32>(const float*, const float*, uint32_t, uint32_t,
-    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t*, float*);
+    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t, uint32_t*, float*);
 template __global__ void k_bexact_pass1<30, 64>(const float*, const float*, uint32_t, uint32_t,
-    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t*, float*);
+    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t, uint32_t*, float*);
 template __global__ void k_bexact_pass1<50, 16>(const float*, const float*, uint32_t, uint32_t,
-    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t*, float*);
+    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t, uint32_t*, float*);
 template __global__ void k_bexact_pass1<50, 32>(const float*, const float*, uint32_t, uint32_t,
-    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t*, float*);
+    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t, uint32_t*, float*);
 template __global__ void k_bexact_pass1<50, 64>(const float*, const float*, uint32_t, uint32_t,
-    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t*, float*);
+    const uint64_t*, const uint32_t*, uint32_t, uint32_t, const uint32_t*, const uint64_t*, uint32_t, uint32_t*, float*);
 template __global__ void k_bexact_pass2<0>(const float*, const float*, uint32_t, uint32_t,
     const uint64_t*, const uint32_t*, uint32_t, uint32_t, const float*, uint32_t*, uint32_t*, uint32_t*, uint32_t);
 template __global__ void k_bexact_pass2<30>(const float*, const float*, uint32_t, uint32_t,

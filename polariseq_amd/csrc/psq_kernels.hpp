@@ -145,7 +145,7 @@ constexpr uint32_t EXM_P_TILE = 64;   // points per LDS tile of the two passes
 template <int D, int K>
 __global__ void k_exact_pass1(const float* Xp, const float* xn, uint32_t n, uint32_t d, uint32_t dp,
                               const uint32_t* queries, uint32_t nq, uint32_t seg_len,
-                              uint32_t nseg, float* partial);
+                              uint32_t nseg, uint32_t k, float* partial);
 __global__ void k_exact_tau(const float* partial, uint32_t nq, uint32_t L, uint32_t k, uint32_t d,
                             const uint32_t* queries, const float* xn, const uint32_t* pmax, float* T);
 template <int D>
@@ -174,7 +174,7 @@ __global__ void k_exact_fallback(const float* Xp, uint32_t n, uint32_t d, uint32
 template <int D, int K>
 __global__ void k_bexact_pass1(const float* Xp, const float* xn, uint32_t d, uint32_t dp, const uint64_t* off,
                                const uint32_t* units, uint32_t U, uint32_t seg_len, const uint32_t* nseg_of,
-                               const uint64_t* part_off, uint32_t* counter, float* partial);
+                               const uint64_t* part_off, uint32_t k, uint32_t* counter, float* partial);
 __global__ void k_bexact_tau(const float* partial, const uint64_t* off, uint32_t S,
                              const uint32_t* nseg_of, const uint64_t* part_off, uint64_t nq, uint32_t K,
                              uint32_t k, uint32_t d, const float* xn, const uint32_t* pmax, float* T);
