@@ -42,7 +42,7 @@ using psq_host::fail;
 namespace {
 
 struct Tile {
-  uint32_t block;    // threads per block (EXM_QUERIES queries, 16 per wave)
+  uint32_t block;    // threads per block (four waves of 16 or 32 queries)
   uint32_t tile;     // points per LDS tile
   uint32_t seg_len;  // points per work unit (the tuned choice)
 };
@@ -72,13 +72,15 @@ int dalloc(DevBuf& b, size_t count, T** out) {
 unsigned gsize(size_t c, unsigned b) { return (unsigned)std::max<size_t>(1, (c + b - 1) / b); }
 
 struct Plan {
-  std::vector<uint32_t> units, nseg;
+  std::vector<uint32_t> units, units2, nseg;  // units of pass 1 and of pass 2
   std::vector<uint64_t> part_off;
   uint64_t partial_floats = 0;
 };
 
-// Units ordered largest sample first (as the reference schedules batches).
-void make_plan(const std::vector<uint64_t>& off, const Tile& t, uint32_t K, Plan& P) {
+// Units ordered largest sample first (as the reference schedules batches);
+// qpb1 and qpb2 are the queries per block of the two passes.
+void make_plan(const std::vector<uint64_t>& off, const Tile& t, uint32_t K, uint32_t qpb1, uint32_t qpb2,
+               Plan& P) {
   const size_t S = off.size() - 1;
   std::vector<uint32_t> order(S);
   for (size_t s = 0; s < S; ++s) order[s] = (uint32_t)s;
@@ -86,6 +88,7 @@ void make_plan(const std::vector<uint64_t>& off, const Tile& t, uint32_t K, Plan
     return off[a + 1] - off[a] > off[b + 1] - off[b];
   });
   P.units.clear();
+  P.units2.clear();
   P.nseg.assign(S, 0);
   P.part_off.assign(S, 0);
   uint64_t acc = 0;
@@ -98,15 +101,19 @@ void make_plan(const std::vector<uint64_t>& off, const Tile& t, uint32_t K, Plan
   P.partial_floats = acc;
   // within a sample, query blocks vary fastest: the blocks running at once
   // stream the same segment, which the L2 then serves
-  for (uint32_t s : order) {
-    const uint64_t ns = off[s + 1] - off[s];
-    const uint32_t qb = (uint32_t)((ns + EXM_QUERIES - 1) / EXM_QUERIES);
-    for (uint32_t g = 0; g < P.nseg[s]; ++g)
-      for (uint32_t b = 0; b < qb; ++b) {
-        P.units.push_back(s);
-        P.units.push_back(b);
-        P.units.push_back(g);
-      }
+  for (int pass = 0; pass < 2; ++pass) {
+    std::vector<uint32_t>& u = pass == 0 ? P.units : P.units2;
+    const uint32_t qpb = pass == 0 ? qpb1 : qpb2;
+    for (uint32_t s : order) {
+      const uint64_t ns = off[s + 1] - off[s];
+      const uint32_t qb = (uint32_t)((ns + qpb - 1) / qpb);
+      for (uint32_t g = 0; g < P.nseg[s]; ++g)
+        for (uint32_t b = 0; b < qb; ++b) {
+          u.push_back(s);
+          u.push_back(b);
+          u.push_back(g);
+        }
+    }
   }
 }
 
@@ -170,6 +177,7 @@ extern "C" int psq_knn_exact_batched_device(const float* d_data, uint64_t n_tota
   const uint32_t K = k <= 16 ? 16 : (k <= 32 ? 32 : 64);
   const uint32_t cap = 128;
   const uint32_t dfix = d == 50 ? 50u : (d == 30 ? 30u : 0u);
+  const uint32_t qpb1 = exm_queries((int)dfix, (int)K), qpb2 = exm_queries((int)dfix, 0);
   std::vector<uint64_t> off(offsets, offsets + n_samples + 1);
   uint64_t maxs = 0;
   for (uint64_t i = 0; i < n_samples; ++i) maxs = std::max(maxs, off[i + 1] - off[i]);
@@ -241,7 +249,7 @@ extern "C" int psq_knn_exact_batched_device(const float* d_data, uint64_t n_tota
     size_t max_units = 0, max_part = 0;
     std::vector<Plan> plans(sizeof(cands) / sizeof(cands[0]));
     for (size_t c = 0; c < plans.size(); ++c) {
-      make_plan(probe_off, cands[c], K, plans[c]);
+      make_plan(probe_off, cands[c], K, qpb1, qpb2, plans[c]);
       max_units = std::max(max_units, plans[c].units.size());
       max_part = std::max<size_t>(max_part, plans[c].partial_floats);
     }
@@ -283,17 +291,19 @@ extern "C" int psq_knn_exact_batched_device(const float* d_data, uint64_t n_tota
 
   // ── the three persistent passes over every sample ──
   Plan P;
-  make_plan(off, best, K, P);
-  const uint32_t U = (uint32_t)(P.units.size() / 3);
-  DevBuf bUnits, bNseg, bPoff, bPart;
-  uint32_t *d_units, *d_nseg;
+  make_plan(off, best, K, qpb1, qpb2, P);
+  const uint32_t U = (uint32_t)(P.units.size() / 3), U2 = (uint32_t)(P.units2.size() / 3);
+  DevBuf bUnits, bUnits2, bNseg, bPoff, bPart;
+  uint32_t *d_units, *d_units2, *d_nseg;
   uint64_t* d_poff;
   float* d_partial;
   BTRY(dalloc(bUnits, P.units.size(), &d_units));
+  BTRY(dalloc(bUnits2, P.units2.size(), &d_units2));
   BTRY(dalloc(bNseg, n_samples, &d_nseg));
   BTRY(dalloc(bPoff, n_samples, &d_poff));
   BTRY(dalloc(bPart, P.partial_floats, &d_partial));
   BHIPC(hipMemcpyAsync(d_units, P.units.data(), P.units.size() * 4, hipMemcpyHostToDevice, s));
+  BHIPC(hipMemcpyAsync(d_units2, P.units2.data(), P.units2.size() * 4, hipMemcpyHostToDevice, s));
   BHIPC(hipMemcpyAsync(d_nseg, P.nseg.data(), n_samples * 4, hipMemcpyHostToDevice, s));
   BHIPC(hipMemcpyAsync(d_poff, P.part_off.data(), n_samples * 8, hipMemcpyHostToDevice, s));
   unsigned grid = 0;  // sized per kernel at its launch
@@ -320,7 +330,7 @@ extern "C" int psq_knn_exact_batched_device(const float* d_data, uint64_t n_tota
       BHIPC(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     BTRY(occupancy_grid((const void*)kern, best.block, smem, &grid));
     BHIPC(hipMemsetAsync(counter, 0, 4, s));
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(best.block), smem, s, Xp, xn, (uint32_t)d, dp, d_off, d_units, U,
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(best.block), smem, s, Xp, xn, (uint32_t)d, dp, d_off, d_units2, U2,
                        best.seg_len, T, counter, cnt, cand, cap);
     BHIPC(hipGetLastError());
   }

@@ -1299,8 +1299,8 @@ int knn_exact_big(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d
       hipLaunchKernelGGL(k_iota_u32, dim3(grid_for(qn, 256)), dim3(256), 0, s, qidx, (uint32_t)c0, qn);
       HIPC(hipGetLastError());
     }
-    const uint32_t qblocks = (qn + EXM_QUERIES - 1) / EXM_QUERIES;
-    hipLaunchKernelGGL(p1, dim3(qblocks, nseg), dim3(256), smem, s, Xp, xn, (uint32_t)n, (uint32_t)d, dp,
+    const uint32_t qpb1 = exm_queries((int)dfix, (int)K), qpb2 = exm_queries((int)dfix, 0);
+    hipLaunchKernelGGL(p1, dim3((qn + qpb1 - 1) / qpb1, nseg), dim3(256), smem, s, Xp, xn, (uint32_t)n, (uint32_t)d, dp,
                        qp, qn, seg_len, nseg, (uint32_t)k, partial);
     HIPC(hipGetLastError());
     hipLaunchKernelGGL(k_exact_tau_big, dim3(qn), dim3(256), tau_smem, s, partial, qn, L, (uint32_t)k,
@@ -1308,7 +1308,7 @@ int knn_exact_big(const float* d_data, uint64_t n, uint64_t d, const uint32_t* d
     HIPC(hipGetLastError());
     TRY(fill_u32(cnt, qn, 0, s));
     TRY(fill_u32(novf, 1, 0, s));
-    hipLaunchKernelGGL(p2, dim3(qblocks, nseg), dim3(256), smem, s, Xp, xn, (uint32_t)n, (uint32_t)d, dp,
+    hipLaunchKernelGGL(p2, dim3((qn + qpb2 - 1) / qpb2, nseg), dim3(256), smem, s, Xp, xn, (uint32_t)n, (uint32_t)d, dp,
                        qp, qn, seg_len, T, cnt, cand, cap);
     HIPC(hipGetLastError());
     uint32_t* io = d_idx + c0 * k;
@@ -1362,7 +1362,7 @@ constexpr uint32_t EX_CAND_CAP = 128;  // pass-2 candidates per query (k <= 64 p
 // 64 points per segment. Any count gives the same result (each segment keeps
 // K >= k values, so the k-th of the union bounds the true k-th from above).
 uint32_t exact_small_nseg(uint64_t n, uint64_t nq) {
-  const uint32_t qblocks = (uint32_t)((nq + EXM_QUERIES - 1) / EXM_QUERIES);
+  const uint32_t qblocks = (uint32_t)((nq + 63) / 64);  // 64-query groups
   return std::max<uint32_t>(1, (uint32_t)std::min<uint64_t>((n + 63) / 64, (2048 + qblocks - 1) / qblocks));
 }
 uint32_t exact_small_K(uint64_t k) { return k <= 16 ? 16 : (k <= 32 ? 32 : 64); }
@@ -1386,7 +1386,6 @@ int knn_exact_small(BufSrc& B, const float* d_data, uint64_t n, uint64_t d, cons
   const uint32_t dp = round_up((uint32_t)d, 32);
   const uint32_t K = exact_small_K(k);
   const uint32_t cap = EX_CAND_CAP;
-  const uint32_t qblocks = (uint32_t)((nq + EXM_QUERIES - 1) / EXM_QUERIES);
   uint32_t nseg = std::max<uint32_t>(1, nseg_in);
   uint32_t seg_len = round_up((uint32_t)((n + nseg - 1) / nseg), 64);
   nseg = (uint32_t)((n + seg_len - 1) / seg_len);
@@ -1412,7 +1411,8 @@ int knn_exact_small(BufSrc& B, const float* d_data, uint64_t n, uint64_t d, cons
   HIPC(hipGetLastError());
   const size_t smem = exm_smem((uint32_t)d);
   const uint32_t dfix = d == 50 ? 50u : (d == 30 ? 30u : 0u);
-  dim3 g1(qblocks, nseg);
+  const uint32_t qpb1 = exm_queries((int)dfix, (int)K), qpb2 = exm_queries((int)dfix, 0);
+  const dim3 g1((uint32_t)((nq + qpb1 - 1) / qpb1), nseg), g2((uint32_t)((nq + qpb2 - 1) / qpb2), nseg);
   void (*p1)(const float*, const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t,
              uint32_t, uint32_t, uint32_t, float*);
   if (K == 16) p1 = dfix == 50 ? k_exact_pass1<50, 16> : (dfix == 30 ? k_exact_pass1<30, 16> : k_exact_pass1<0, 16>);
@@ -1429,7 +1429,7 @@ int knn_exact_small(BufSrc& B, const float* d_data, uint64_t n, uint64_t d, cons
              uint32_t, const float*, uint32_t*, uint32_t*, uint32_t) =
       dfix == 50 ? k_exact_pass2<50> : (dfix == 30 ? k_exact_pass2<30> : k_exact_pass2<0>);
   if (smem > 65536) HIPC(hipFuncSetAttribute((const void*)p2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-  hipLaunchKernelGGL(p2, g1, dim3(256), smem, s, Xp, xn, (uint32_t)n, (uint32_t)d, dp, d_queries,
+  hipLaunchKernelGGL(p2, g2, dim3(256), smem, s, Xp, xn, (uint32_t)n, (uint32_t)d, dp, d_queries,
                      (uint32_t)nq, seg_len, T, cnt, cand, cap);
   HIPC(hipGetLastError());
   hipLaunchKernelGGL(k_exact_rerank, dim3(grid_for(nq, 4)), dim3(256), 0, s, Xp, (uint32_t)n,

@@ -8,19 +8,24 @@
 // bit-exact against the CPU oracle that computes the same sequence.
 //
 // norms   xn[p] = |p|^2 in FP32 (dimension order), and P = max_p xn[p].
-// pass 1  a block of 64 queries (16 per wave) streams a segment of points
-//         through LDS, 64 at a time; per 16 points one v_mfma_f32_16x16x4f32
-//         chain over the dimensions gives the 16 x 16 dot products (the
-//         reference's tiled kNN also works on matrix products,
-//         doublets.rs:557-651), and e(p, q) = max(0, (xn[p] + xn[q]) - 2 p.q).
+// pass 1  a block of four waves, each holding one or two sets of 16 queries
+//         (two at the compiled widths d = 30, 50), streams a segment of points
+//         through LDS, 64 at a time; per 16 points and query set one
+//         v_mfma_f32_16x16x4f32 chain over the dimensions gives the 16 x 16 dot
+//         products (the reference's tiled kNN also works on matrix products,
+//         doublets.rs:557-651), and e(p, q) = (xn[p] + xn[q]) - 2 p.q.
 //         Lane (c, g) of a wave sees query c against points 4g .. 4g + 3 of every
-//         16 and keeps its K smallest e: four lists per query and segment.
+//         16 and keeps its K smallest e: four lists per query and segment. A
+//         tile whose 16 values for the lane are all at or above the lane's bound
+//         skips the insertion; the bound is the list's K-th value or, tighter,
+//         the largest of the query's four lists' ceil(k/4)-th values (together
+//         at least k values), which no later value at or above it can change.
 // tau    the k-th smallest e over all lists -> threshold T = tau + 2 E(q), with
 //         E(q) = 2.1 (d + 8) 2^-24 (xn[q] + P) + 2^-110 (xn[q] + P + 1), a bound on
 //         |e(p, q) - |p - q|^2| for every p: the dot product, both norms and the
 //         two additions each err by at most (d + 2) 2^-24 times |p|^2 + |q|^2 (any
-//         summation order), clamping at 0 only moves e towards the exact value,
-//         and the last term covers denormal products flushed to zero.
+//         summation order), and the last term covers denormal products flushed
+//         to zero.
 // pass 2  the same scan; every point with e <= T becomes a candidate.
 //         The true top-k is inside: k points have e <= tau, so the true k-th
 //         distance is at most tau + E, and a true neighbour p has
@@ -115,47 +120,76 @@ constexpr int EXM_P = EXM_P_TILE;  // points per LDS tile
 // takes dimensions g KS .. g KS + KS - 1 of point and query, one per matrix
 // step s: word (16 t + c) TS + g KS + s of a read lands on 64 distinct banks
 // when TS = 4 KS is 4 mod 8, and TS = 4 KS + 4 restores that when 4 KS is
-// 0 mod 8.
+// 0 mod 8. The 64 norms follow the tile.
 __host__ __device__ constexpr uint32_t exm_ts(uint32_t ks) { return (4 * ks) % 8 == 0 ? 4 * ks + 4 : 4 * ks; }
 __host__ size_t exm_smem(uint32_t d) {
   const uint32_t ks = (d + 3) / 4;
   return ((size_t)EXM_P * exm_ts(ks) + EXM_P) * sizeof(float);
 }
-constexpr uint32_t EXM_THREADS = 4 * WAVE;  // 16 queries per wave
+constexpr uint32_t EXM_THREADS = 4 * WAVE;
+// Waves per SIMD the fixed-width passes are compiled for (a register budget
+// of 512 / w per lane); the generic width takes what it needs.
+__host__ __device__ constexpr int exm_min_waves(int D, int K) { return D > 0 && K <= 32 ? 4 : 1; }
+#define EXM_WAVES(D, K) __attribute__((amdgpu_waves_per_eu(exm_min_waves(D, K))))
 
-// Stage rows [base, base + np) of Xp into the tile, four floats at a time, and
-// their norms into tn. Tile rows past np repeat row base + np - 1 (finite) with
-// norm +inf, so their filter values are +inf. Each thread issues its loads before its first store: the
-// addresses are clamped instead of guarded, and the empty asm keeps the
-// compiler from moving a load under the store's condition.
-template <int KSM>
-__device__ inline void exm_stage(const float* __restrict__ Xp, const float* __restrict__ xn, uint32_t dp,
-                                 uint32_t KS, uint32_t TS, uint32_t base, uint32_t np, float* tile,
-                                 float* tn) {
-  constexpr int NV = (EXM_P * KSM + EXM_THREADS - 1) / EXM_THREADS;  // float4s per thread
-  constexpr int CH = NV < 4 ? NV : 4;
+// Rows of a tile in registers: rows [base, base + np) of Xp, four floats at a
+// time, and their norms. Rows past np repeat row base + np - 1 (finite). Every
+// load is issued before the first store: the addresses are clamped instead of
+// guarded, and the empty asm keeps the compiler from moving a load under the
+// store's condition.
+template <int NV>
+struct ExmRows {
+  float4 v[NV];
+  float nv;
+};
+
+template <int NV>
+__device__ inline void exm_fetch(ExmRows<NV>& R, const float* __restrict__ Xp, const float* __restrict__ xn,
+                                 uint32_t dp, uint32_t KS, uint32_t base, uint32_t np, uint32_t e0) {
   const uint32_t tot = EXM_P * KS, last = base + np - 1;
-  float nv = xn[min(base + threadIdx.x % EXM_P, last)];
-  asm volatile("" : "+v"(nv));
-  for (uint32_t e0 = threadIdx.x; e0 < tot; e0 += CH * EXM_THREADS) {
-    float4 v[CH];
+  R.nv = xn[min(base + threadIdx.x % EXM_P, last)];
 #pragma unroll
-    for (int h = 0; h < CH; ++h) {
-      const uint32_t e = min(e0 + h * EXM_THREADS, tot - 1);
+  for (int h = 0; h < NV; ++h) {
+    const uint32_t e = min(e0 + h * EXM_THREADS, tot - 1);
+    const uint32_t r = e / KS, j = e - r * KS;
+    R.v[h] = *reinterpret_cast<const float4*>(Xp + (size_t)min(base + r, last) * dp + 4 * j);
+  }
+}
+
+// Store the rows into the tile; tile norms past np are +inf, so the filter
+// values of the repeated rows are +inf.
+template <int NV>
+__device__ inline void exm_put(ExmRows<NV>& R, uint32_t KS, uint32_t TS, uint32_t np, uint32_t e0,
+                               float* tile, bool norms) {
+  const uint32_t tot = EXM_P * KS;
+#pragma unroll
+  for (int h = 0; h < NV; ++h) {
+    asm volatile("" : "+v"(R.v[h].x), "+v"(R.v[h].y), "+v"(R.v[h].z), "+v"(R.v[h].w));
+    const uint32_t e = e0 + h * EXM_THREADS;
+    if (e < tot) {
       const uint32_t r = e / KS, j = e - r * KS;
-      v[h] = *reinterpret_cast<const float4*>(Xp + (size_t)min(base + r, last) * dp + 4 * j);
-    }
-#pragma unroll
-    for (int h = 0; h < CH; ++h) {
-      asm volatile("" : "+v"(v[h].x), "+v"(v[h].y), "+v"(v[h].z), "+v"(v[h].w));
-      const uint32_t e = e0 + h * EXM_THREADS;
-      if (e < tot) {
-        const uint32_t r = e / KS, j = e - r * KS;
-        *reinterpret_cast<float4*>(tile + r * TS + 4 * j) = v[h];
-      }
+      *reinterpret_cast<float4*>(tile + r * TS + 4 * j) = R.v[h];
     }
   }
-  if (threadIdx.x < (uint32_t)EXM_P) tn[threadIdx.x] = threadIdx.x < np ? nv : __builtin_inff();
+  if (norms && threadIdx.x < (uint32_t)EXM_P) {
+    asm volatile("" : "+v"(R.nv));
+    tile[EXM_P * TS + threadIdx.x] = threadIdx.x < np ? R.nv : __builtin_inff();
+  }
+}
+
+// Stage a whole tile, in chunks of up to four float4 per thread (one chunk at
+// the fixed widths).
+template <int KSM>
+__device__ inline void exm_stage(const float* __restrict__ Xp, const float* __restrict__ xn, uint32_t dp,
+                                 uint32_t KS, uint32_t TS, uint32_t base, uint32_t np, float* tile) {
+  constexpr int NV = (EXM_P * KSM + EXM_THREADS - 1) / EXM_THREADS;
+  constexpr int CH = NV < 4 ? NV : 4;
+  const uint32_t tot = EXM_P * KS;
+  for (uint32_t e0 = threadIdx.x; e0 < tot; e0 += CH * EXM_THREADS) {
+    ExmRows<CH> R;
+    exm_fetch<CH>(R, Xp, xn, dp, KS, base, np, e0);
+    exm_put<CH>(R, KS, TS, np, e0, tile, e0 == threadIdx.x);
+  }
 }
 
 // The lane's query registers: qf[s] = dimension g KS + s of row qi.
@@ -166,43 +200,67 @@ __device__ inline void exm_load_query(const float* __restrict__ Xp, uint32_t dp,
   for (int s = 0; s < KSM; ++s) qf[s] = (uint32_t)s < KS ? Xp[qi * dp + g * KS + s] : 0.0f;
 }
 
-// Scan rows [p0, p1) for the lane's query (16 queries per wave, lane (c, g)
-// holding query c): per 64-row tile, four 16-row v_mfma_f32_16x16x4f32 chains
-// over the dimensions, interleaved, give the dot products, and f(base, np, e)
-// runs once per tile with e[4 t + i] = (xn[p] + xn[q]) - 2 p.q for row
-// p = base + 16 t + 4 g + i (+inf for rows past np). The clamp at 0 is left
-// out: it only matters for bounding e from below, and the error bound holds
+// The dot products of one tile against the wave's QS query sets (lane (c, g)
+// holds query c of each set): four 16-row v_mfma_f32_16x16x4f32 chains per set
+// over the dimensions, interleaved; then f(q, base, np, e) per set with
+// e[4 t + i] = (xn[p] + xn[q]) - 2 p.q for row p = base + 16 t + 4 g + i
+// (+inf for rows past np). The clamp at 0 is left out: the error bound holds
 // without it.
-template <int KSM, typename F>
-__device__ inline void exm_scan(const float* __restrict__ Xp, const float* __restrict__ xn, uint32_t dp,
-                                uint32_t KS, uint32_t TS, uint32_t p0, uint32_t p1, float* tile, float* tn,
-                                uint32_t c, uint32_t g, const float (&qf)[KSM], float xq, F&& f) {
-  for (uint32_t base = p0; base < p1; base += EXM_P) {
-    const uint32_t np = min((uint32_t)EXM_P, p1 - base);
-    __syncthreads();
-    exm_stage<KSM>(Xp, xn, dp, KS, TS, base, np, tile, tn);
-    __syncthreads();
-    f32x4_t acc[4];
+template <int KSM, int QS, typename F>
+__device__ inline void exm_tile(const float* tile, uint32_t KS, uint32_t TS, uint32_t base, uint32_t np,
+                                uint32_t c, uint32_t g, const float (&qf)[QS][KSM], const float (&xq)[QS],
+                                F&& f) {
+  f32x4_t acc[QS][4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
-    const float* a = tile + c * TS + g * KS;
+  for (int q = 0; q < QS; ++q)
 #pragma unroll
-    for (int s = 0; s < KSM; ++s)
-      if ((uint32_t)s < KS) {
+    for (int t = 0; t < 4; ++t) acc[q][t] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+  const float* a = tile + c * TS + g * KS;
+#pragma unroll
+  for (int s = 0; s < KSM; ++s)
+    if ((uint32_t)s < KS) {
+      float av[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) av[t] = a[16 * t * TS + s];
+#pragma unroll
+      for (int q = 0; q < QS; ++q)
 #pragma unroll
         for (int t = 0; t < 4; ++t)
-          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[16 * t * TS + s], qf[s], acc[t], 0, 0, 0);
-      }
+          acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t], qf[q][s], acc[q][t], 0, 0, 0);
+    }
+  const float* tn = tile + EXM_P * TS;
+#pragma unroll
+  for (int q = 0; q < QS; ++q) {
     float e[16];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const float4 nr = *reinterpret_cast<const float4*>(tn + 16 * t + 4 * g);
-      e[4 * t + 0] = fmaf(-2.0f, acc[t][0], nr.x + xq);
-      e[4 * t + 1] = fmaf(-2.0f, acc[t][1], nr.y + xq);
-      e[4 * t + 2] = fmaf(-2.0f, acc[t][2], nr.z + xq);
-      e[4 * t + 3] = fmaf(-2.0f, acc[t][3], nr.w + xq);
+      e[4 * t + 0] = fmaf(-2.0f, acc[q][t][0], nr.x + xq[q]);
+      e[4 * t + 1] = fmaf(-2.0f, acc[q][t][1], nr.y + xq[q]);
+      e[4 * t + 2] = fmaf(-2.0f, acc[q][t][2], nr.z + xq[q]);
+      e[4 * t + 3] = fmaf(-2.0f, acc[q][t][3], nr.w + xq[q]);
     }
-    f(base, np, e);
+    f(q, base, np, e);
+  }
+}
+
+// Scan rows [p0, p1) in 64-row tiles: stage a tile, then its dot products and
+// f. (Double-buffered staging, the next tile's loads in flight during the
+// products, measured no faster: the extra registers cost the occupancy the
+// two query sets need.) Every thread of the block calls this with the same
+// range.
+template <int D, int QS, typename F>
+__device__ inline void exm_scan(const float* __restrict__ Xp, const float* __restrict__ xn, uint32_t dp,
+                                uint32_t KS, uint32_t TS, uint32_t p0, uint32_t p1, float* lds, uint32_t c,
+                                uint32_t g, const float (&qf)[QS][D > 0 ? (D + 3) / 4 : 64],
+                                const float (&xq)[QS], F&& f) {
+  constexpr int KSM = D > 0 ? (D + 3) / 4 : 64;
+  for (uint32_t base = p0; base < p1; base += EXM_P) {
+    const uint32_t np = min((uint32_t)EXM_P, p1 - base);
+    __syncthreads();
+    exm_stage<KSM>(Xp, xn, dp, KS, TS, base, np, lds);
+    __syncthreads();
+    exm_tile<KSM, QS>(lds, KS, TS, base, np, c, g, qf, xq, f);
   }
 }
 
@@ -251,40 +309,59 @@ __device__ inline void exm_pass1_tile(float (&top)[K], float& shared_bnd, uint32
   }
 }
 
+// Pass 2 of one tile: every row with e <= thr (the query row excluded) is
+// appended to the query's candidates.
+__device__ inline void exm_pass2_tile(uint32_t base, uint32_t np, uint32_t g, uint32_t qi, float thr,
+                                      uint32_t* cnt_q, uint32_t* cand_q, uint32_t cap, const float (&e)[16]) {
+  if (exm_min16(e) <= thr) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const uint32_t r = 16 * (j / 4) + 4 * g + (j % 4);
+      if (e[j] <= thr && r < np && base + r != qi) {
+        const uint32_t pos = atomicAdd(cnt_q, 1u);
+        if (pos < cap) cand_q[pos] = base + r;
+      }
+    }
+  }
+}
+
 template <int D, int K>
-__global__ __launch_bounds__(256) void k_exact_pass1(const float* __restrict__ Xp,
+__global__ __launch_bounds__(256) EXM_WAVES(D, K) void k_exact_pass1(const float* __restrict__ Xp,
                                                      const float* __restrict__ xn, uint32_t n,
                                                      uint32_t d, uint32_t dp,
                                                      const uint32_t* __restrict__ queries,
                                                      uint32_t nq, uint32_t seg_len, uint32_t nseg,
                                                      uint32_t k, float* __restrict__ partial) {
   constexpr int KSM = D > 0 ? (D + 3) / 4 : 64;
+  constexpr int QS = (int)exm_qsets(D, K);
   extern __shared__ float exm_lds[];
   const uint32_t KS = D > 0 ? (uint32_t)KSM : (d + 3) / 4, TS = exm_ts(KS);
-  float* tile = exm_lds;
-  float* tn = tile + (size_t)EXM_P * TS;
   const uint32_t l = threadIdx.x % 64, w = threadIdx.x / 64, c = l % 16, g = l / 16;
-  const uint32_t qslot = blockIdx.x * EXM_QUERIES + w * 16 + c;
-  const bool qvalid = qslot < nq;
-  const uint32_t qi = qvalid ? (queries ? queries[qslot] : qslot) : 0u;
-  float qf[KSM];
-  exm_load_query<KSM>(Xp, dp, KS, qi, g, qf);
-  const float xq = xn[qi];
-  float top[K];
+  uint32_t qslot[QS], qi[QS];
+  float qf[QS][KSM], xq[QS], top[QS][K], sb[QS];
 #pragma unroll
-  for (int j = 0; j < K; ++j) top[j] = __builtin_inff();
-  float sb = __builtin_inff();
+  for (int q = 0; q < QS; ++q) {
+    qslot[q] = (blockIdx.x * 4 + w) * 16 * QS + q * 16 + c;
+    qi[q] = qslot[q] < nq ? (queries ? queries[qslot[q]] : qslot[q]) : 0u;
+    exm_load_query<KSM>(Xp, dp, KS, qi[q], g, qf[q]);
+    xq[q] = xn[qi[q]];
+    sb[q] = __builtin_inff();
+#pragma unroll
+    for (int j = 0; j < K; ++j) top[q][j] = __builtin_inff();
+  }
   const uint32_t jb = (k + 3) / 4 - 1;
   const uint32_t s0 = blockIdx.y * seg_len;
-  exm_scan<KSM>(Xp, xn, dp, KS, TS, s0, min(n, s0 + seg_len), tile, tn, c, g, qf, xq,
-                [&](uint32_t base, uint32_t, const float (&e)[16]) {
-                  exm_pass1_tile<K>(top, sb, jb, base, g, qi, e);
-                });
-  if (qvalid) {
-    float* out = partial + (((size_t)qslot * nseg + blockIdx.y) * 4 + g) * K;
+  exm_scan<D, QS>(Xp, xn, dp, KS, TS, s0, min(n, s0 + seg_len), exm_lds, c, g, qf, xq,
+                  [&](int q, uint32_t base, uint32_t, const float (&e)[16]) {
+                    exm_pass1_tile<K>(top[q], sb[q], jb, base, g, qi[q], e);
+                  });
 #pragma unroll
-    for (int j = 0; j < K; ++j) out[j] = top[j];
-  }
+  for (int q = 0; q < QS; ++q)
+    if (qslot[q] < nq) {
+      float* out = partial + (((size_t)qslot[q] * nseg + blockIdx.y) * 4 + g) * K;
+#pragma unroll
+      for (int j = 0; j < K; ++j) out[j] = top[q][j];
+    }
 }
 
 // tau over a query's L = 4 nseg K pass-1 values (K <= 64, k <= 64), then T.
@@ -317,7 +394,7 @@ __global__ void k_exact_tau(const float* __restrict__ partial, uint32_t nq, uint
 }
 
 template <int D>
-__global__ __launch_bounds__(256) void k_exact_pass2(const float* __restrict__ Xp,
+__global__ __launch_bounds__(256) EXM_WAVES(D, 0) void k_exact_pass2(const float* __restrict__ Xp,
                                                      const float* __restrict__ xn, uint32_t n,
                                                      uint32_t d, uint32_t dp,
                                                      const uint32_t* __restrict__ queries,
@@ -326,32 +403,27 @@ __global__ __launch_bounds__(256) void k_exact_pass2(const float* __restrict__ X
                                                      uint32_t* __restrict__ cnt,
                                                      uint32_t* __restrict__ cand, uint32_t cap) {
   constexpr int KSM = D > 0 ? (D + 3) / 4 : 64;
+  constexpr int QS = (int)exm_qsets(D, 0);
   extern __shared__ float exm_lds[];
   const uint32_t KS = D > 0 ? (uint32_t)KSM : (d + 3) / 4, TS = exm_ts(KS);
-  float* tile = exm_lds;
-  float* tn = tile + (size_t)EXM_P * TS;
   const uint32_t l = threadIdx.x % 64, w = threadIdx.x / 64, c = l % 16, g = l / 16;
-  const uint32_t qslot = blockIdx.x * EXM_QUERIES + w * 16 + c;
-  const bool qvalid = qslot < nq;
-  const uint32_t qi = qvalid ? (queries ? queries[qslot] : qslot) : 0u;
-  const float thr = qvalid ? T[qslot] : -1.0f;
-  float qf[KSM];
-  exm_load_query<KSM>(Xp, dp, KS, qi, g, qf);
-  const float xq = xn[qi];
-  const uint32_t s0 = blockIdx.y * seg_len;
-  exm_scan<KSM>(Xp, xn, dp, KS, TS, s0, min(n, s0 + seg_len), tile, tn, c, g, qf, xq,
-                [&](uint32_t base, uint32_t np, const float (&e)[16]) {
-                  if (exm_min16(e) <= thr) {
+  uint32_t qslot[QS], qi[QS];
+  float qf[QS][KSM], xq[QS], thr[QS];
 #pragma unroll
-                    for (int j = 0; j < 16; ++j) {
-                      const uint32_t r = 16 * (j / 4) + 4 * g + (j % 4);
-                      if (e[j] <= thr && r < np && base + r != qi) {
-                        const uint32_t pos = atomicAdd(&cnt[qslot], 1u);
-                        if (pos < cap) cand[(size_t)qslot * cap + pos] = base + r;
-                      }
-                    }
-                  }
-                });
+  for (int q = 0; q < QS; ++q) {
+    qslot[q] = (blockIdx.x * 4 + w) * 16 * QS + q * 16 + c;
+    const bool ok = qslot[q] < nq;
+    qi[q] = ok ? (queries ? queries[qslot[q]] : qslot[q]) : 0u;
+    thr[q] = ok ? T[qslot[q]] : -1.0f;
+    exm_load_query<KSM>(Xp, dp, KS, qi[q], g, qf[q]);
+    xq[q] = xn[qi[q]];
+  }
+  const uint32_t s0 = blockIdx.y * seg_len;
+  exm_scan<D, QS>(Xp, xn, dp, KS, TS, s0, min(n, s0 + seg_len), exm_lds, c, g, qf, xq,
+                  [&](int q, uint32_t base, uint32_t np, const float (&e)[16]) {
+                    exm_pass2_tile(base, np, g, qi[q], thr[q], &cnt[qslot[q]],
+                                   cand + (size_t)qslot[q] * cap, cap, e);
+                  });
 }
 
 // One wave per query: rank each candidate by (d64, index) among all of them.
@@ -627,18 +699,18 @@ template __global__ void k_exact_pass2<50>(const float*, const float*, uint32_t,
 namespace psq {
 
 template <int D, int K>
-__global__ __launch_bounds__(256) void k_bexact_pass1(
+__global__ __launch_bounds__(256) EXM_WAVES(D, K) void k_bexact_pass1(
     const float* __restrict__ Xp, const float* __restrict__ xn, uint32_t d, uint32_t dp,
     const uint64_t* __restrict__ off, const uint32_t* __restrict__ units, uint32_t U, uint32_t seg_len,
     const uint32_t* __restrict__ nseg_of, const uint64_t* __restrict__ part_off, uint32_t k,
     uint32_t* __restrict__ counter, float* __restrict__ partial) {
   constexpr int KSM = D > 0 ? (D + 3) / 4 : 64;
+  constexpr int QS = (int)exm_qsets(D, K);
   extern __shared__ float exm_lds[];
   __shared__ uint32_t s_u;
   const uint32_t KS = D > 0 ? (uint32_t)KSM : (d + 3) / 4, TS = exm_ts(KS);
-  float* tile = exm_lds;
-  float* tn = tile + (size_t)EXM_P * TS;
   const uint32_t l = threadIdx.x % 64, w = threadIdx.x / 64, c = l % 16, g = l / 16;
+  const uint32_t jb = (k + 3) / 4 - 1;
   for (;;) {
     if (threadIdx.x == 0) s_u = atomicAdd(counter, 1u);
     __syncthreads();
@@ -648,27 +720,30 @@ __global__ __launch_bounds__(256) void k_bexact_pass1(
     const uint32_t s = units[3 * u], qb = units[3 * u + 1], seg = units[3 * u + 2];
     const uint32_t o = (uint32_t)off[s];
     const uint32_t ns = (uint32_t)off[s + 1] - o;
-    const uint32_t ql = qb * EXM_QUERIES + w * 16 + c;
-    const bool qvalid = ql < ns;
-    const uint32_t qi = o + (qvalid ? ql : 0u);
-    float qf[KSM];
-    exm_load_query<KSM>(Xp, dp, KS, qi, g, qf);
-    const float xq = xn[qi];
-    float top[K];
+    uint32_t ql[QS], qi[QS];
+    float qf[QS][KSM], xq[QS], top[QS][K], sb[QS];
 #pragma unroll
-    for (int j = 0; j < K; ++j) top[j] = __builtin_inff();
-    float sb = __builtin_inff();
-    const uint32_t jb = (k + 3) / 4 - 1;
-    const uint32_t s0 = o + seg * seg_len;
-    exm_scan<KSM>(Xp, xn, dp, KS, TS, s0, o + min(ns, seg * seg_len + seg_len), tile, tn, c, g, qf, xq,
-                  [&](uint32_t base, uint32_t, const float (&e)[16]) {
-                    exm_pass1_tile<K>(top, sb, jb, base, g, qi, e);
-                  });
-    if (qvalid) {
-      float* out = partial + part_off[s] + (((size_t)ql * nseg_of[s] + seg) * 4 + g) * K;
+    for (int q = 0; q < QS; ++q) {
+      ql[q] = (qb * 4 + w) * 16 * QS + q * 16 + c;
+      qi[q] = o + (ql[q] < ns ? ql[q] : 0u);
+      exm_load_query<KSM>(Xp, dp, KS, qi[q], g, qf[q]);
+      xq[q] = xn[qi[q]];
+      sb[q] = __builtin_inff();
 #pragma unroll
-      for (int j = 0; j < K; ++j) out[j] = top[j];
+      for (int j = 0; j < K; ++j) top[q][j] = __builtin_inff();
     }
+    const uint32_t s0 = o + seg * seg_len;
+    exm_scan<D, QS>(Xp, xn, dp, KS, TS, s0, o + min(ns, seg * seg_len + seg_len), exm_lds, c, g, qf, xq,
+                    [&](int q, uint32_t base, uint32_t, const float (&e)[16]) {
+                      exm_pass1_tile<K>(top[q], sb[q], jb, base, g, qi[q], e);
+                    });
+#pragma unroll
+    for (int q = 0; q < QS; ++q)
+      if (ql[q] < ns) {
+        float* out = partial + part_off[s] + (((size_t)ql[q] * nseg_of[s] + seg) * 4 + g) * K;
+#pragma unroll
+        for (int j = 0; j < K; ++j) out[j] = top[q][j];
+      }
   }
 }
 
@@ -711,17 +786,16 @@ __global__ void k_bexact_tau(const float* __restrict__ partial, const uint64_t* 
 }
 
 template <int D>
-__global__ __launch_bounds__(256) void k_bexact_pass2(
+__global__ __launch_bounds__(256) EXM_WAVES(D, 0) void k_bexact_pass2(
     const float* __restrict__ Xp, const float* __restrict__ xn, uint32_t d, uint32_t dp,
     const uint64_t* __restrict__ off, const uint32_t* __restrict__ units, uint32_t U, uint32_t seg_len,
     const float* __restrict__ T, uint32_t* __restrict__ counter, uint32_t* __restrict__ cnt,
     uint32_t* __restrict__ cand, uint32_t cap) {
   constexpr int KSM = D > 0 ? (D + 3) / 4 : 64;
+  constexpr int QS = (int)exm_qsets(D, 0);
   extern __shared__ float exm_lds[];
   __shared__ uint32_t s_u;
   const uint32_t KS = D > 0 ? (uint32_t)KSM : (d + 3) / 4, TS = exm_ts(KS);
-  float* tile = exm_lds;
-  float* tn = tile + (size_t)EXM_P * TS;
   const uint32_t l = threadIdx.x % 64, w = threadIdx.x / 64, c = l % 16, g = l / 16;
   for (;;) {
     if (threadIdx.x == 0) s_u = atomicAdd(counter, 1u);
@@ -732,27 +806,22 @@ __global__ __launch_bounds__(256) void k_bexact_pass2(
     const uint32_t s = units[3 * u], qb = units[3 * u + 1], seg = units[3 * u + 2];
     const uint32_t o = (uint32_t)off[s];
     const uint32_t ns = (uint32_t)off[s + 1] - o;
-    const uint32_t ql = qb * EXM_QUERIES + w * 16 + c;
-    const bool qvalid = ql < ns;
-    const uint32_t qi = o + (qvalid ? ql : 0u);
-    const float thr = qvalid ? T[qi] : -1.0f;
-    float qf[KSM];
-    exm_load_query<KSM>(Xp, dp, KS, qi, g, qf);
-    const float xq = xn[qi];
-    const uint32_t s0 = o + seg * seg_len;
-    exm_scan<KSM>(Xp, xn, dp, KS, TS, s0, o + min(ns, seg * seg_len + seg_len), tile, tn, c, g, qf, xq,
-                  [&](uint32_t base, uint32_t np, const float (&e)[16]) {
-                    if (exm_min16(e) <= thr) {
+    uint32_t qi[QS];
+    float qf[QS][KSM], xq[QS], thr[QS];
 #pragma unroll
-                      for (int j = 0; j < 16; ++j) {
-                        const uint32_t r = 16 * (j / 4) + 4 * g + (j % 4);
-                        if (e[j] <= thr && r < np && base + r != qi) {
-                          const uint32_t pos = atomicAdd(&cnt[qi], 1u);
-                          if (pos < cap) cand[(size_t)qi * cap + pos] = base + r;
-                        }
-                      }
-                    }
-                  });
+    for (int q = 0; q < QS; ++q) {
+      const uint32_t ql = (qb * 4 + w) * 16 * QS + q * 16 + c;
+      qi[q] = o + (ql < ns ? ql : 0u);
+      thr[q] = ql < ns ? T[qi[q]] : -1.0f;
+      exm_load_query<KSM>(Xp, dp, KS, qi[q], g, qf[q]);
+      xq[q] = xn[qi[q]];
+    }
+    const uint32_t s0 = o + seg * seg_len;
+    exm_scan<D, QS>(Xp, xn, dp, KS, TS, s0, o + min(ns, seg * seg_len + seg_len), exm_lds, c, g, qf, xq,
+                    [&](int q, uint32_t base, uint32_t np, const float (&e)[16]) {
+                      exm_pass2_tile(base, np, g, qi[q], thr[q], &cnt[qi[q]], cand + (size_t)qi[q] * cap,
+                                     cap, e);
+                    });
   }
 }
 

@@ -140,8 +140,12 @@ __global__ void k_moments(const float* X, uint32_t n, uint32_t d, uint32_t chunk
 __global__ void k_exact_norms(const float* Xp, uint32_t n, uint32_t d, uint32_t dp, float* xn,
                               uint32_t* pmax);
 size_t exm_smem(uint32_t d);  // dynamic LDS of k_exact_pass1 / k_exact_pass2
-constexpr uint32_t EXM_QUERIES = 64;  // queries per block of the two passes
-constexpr uint32_t EXM_P_TILE = 64;   // points per LDS tile of the two passes
+constexpr uint32_t EXM_P_TILE = 64;  // points per LDS tile of the two passes
+// Query sets of 16 per wave in a filter pass: two at the fixed widths with
+// lists of at most 32 (K = 0 names pass 2), else one.
+__host__ __device__ constexpr uint32_t exm_qsets(int D, int K) { return D > 0 && K <= 32 ? 2u : 1u; }
+// Queries per block of a filter pass (four waves).
+__host__ __device__ constexpr uint32_t exm_queries(int D, int K) { return 64u * exm_qsets(D, K); }
 template <int D, int K>
 __global__ void k_exact_pass1(const float* Xp, const float* xn, uint32_t n, uint32_t d, uint32_t dp,
                               const uint32_t* queries, uint32_t nq, uint32_t seg_len,
