@@ -26,13 +26,15 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 
-// 8 centred dimensions [t0, t0 + 8) of an LDS row, rounded to BF16 (0 past d),
-// with x_i's 8 dimensions in registers (xr, from load8).
+// 8 centred dimensions [t0, t0 + 8) of an LDS row, rounded to BF16, with x_i's
+// 8 dimensions in registers (xr, from load8).
 // t0 is a multiple of 8 and rows are 16-byte aligned: two 16-byte reads per
 // operand, so the 16 rows of a tile, 52 floats apart, fill all 64 banks.
-// Reads stop at the row's padded stride rs (>= the last dimension read).
+// Reads stop at rs, the floats staged per row (d rounded up to 4); the staged
+// floats past d are the rows' zero padding, in the row and in x_i alike, so
+// every dimension past d centres to 0 without a test.
 __device__ inline bf16x8 centred_bf16x8_r(const float* row, const float (&xr)[8], uint32_t t0,
-                                          uint32_t d, uint32_t rs) {
+                                          uint32_t rs) {
   float r[8];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
@@ -43,7 +45,7 @@ __device__ inline bf16x8 centred_bf16x8_r(const float* row, const float (&xr)[8]
   }
   bf16x8 v;
 #pragma unroll
-  for (int u = 0; u < 8; ++u) v[u] = (__bf16)(t0 + u < d ? r[u] - xr[u] : 0.0f);
+  for (int u = 0; u < 8; ++u) v[u] = (__bf16)(r[u] - xr[u]);
   return v;
 }
 __device__ inline void load8(const float* p, uint32_t t0, uint32_t rs, float (&out)[8]) {
@@ -72,7 +74,7 @@ __global__ __launch_bounds__(64) void k_join_bf16(
   extern __shared__ __attribute__((aligned(16))) uint64_t lds64[];
   uint64_t* pb_key = lds64;
   float* rows = reinterpret_cast<float*>(lds64 + PB_CAP);
-  float* xi = rows + (size_t)R * RS;
+  float* xi = rows + R * RS;
   uint32_t* s_id = reinterpret_cast<uint32_t*>(xi + RS);
   float* s_thr = reinterpret_cast<float*>(s_id + R);
   float* s_nrm = s_thr + R;
@@ -90,20 +92,22 @@ __global__ __launch_bounds__(64) void k_join_bf16(
       return;
     const uint32_t nc = nn + no;
     rows_in += nc;
+    // floats staged per row: d rounded up to 4 (the padding is zero in every row)
+    const uint32_t rsv = D > 0 ? (uint32_t)(D + 3) / 4 * 4 : (d + 3) / 4 * 4;
     __syncthreads();
     for (uint32_t s = l; s < nc; s += WAVE) {
-      const float* row = rows + (size_t)s * RS;
+      const float* row = rows + __umul24(s, RS);
       float nrm = 0.0f;
-      for (uint32_t t = 0; t < d; t += 4) {
+      for (uint32_t t = 0; t < rsv; t += 4) {
         const float4 a = *reinterpret_cast<const float4*>(row + t);
         const float4 x = *reinterpret_cast<const float4*>(xi + t);
         const float av[4] = {a.x, a.y, a.z, a.w}, xv[4] = {x.x, x.y, x.z, x.w};
+        // a padding dimension adds fmaf(0, 0, nrm) = nrm
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-          if (t + u < d) {
-            const float b = (float)(__bf16)(av[u] - xv[u]);
-            nrm = fmaf(b, b, nrm);
-          }
+        for (int u = 0; u < 4; ++u) {
+          const float b = (float)(__bf16)(av[u] - xv[u]);
+          nrm = fmaf(b, b, nrm);
+        }
       }
       s_nrm[s] = nrm;
     }
@@ -124,7 +128,7 @@ __global__ __launch_bounds__(64) void k_join_bf16(
           const uint32_t pr = sv[e], a = pr & 0xFFFF, b = pr >> 16;
           ia = s_id[a];
           ib = s_id[b];
-          dist = lds_sq_dist<D>(rows + (size_t)a * RS, rows + (size_t)b * RS, d);
+          dist = lds_sq_dist<D>(rows + __umul24(a, RS), rows + __umul24(b, RS), d);
           pa = dist < s_thr[a];
           pb = dist < s_thr[b];
           ++exact;
@@ -151,12 +155,12 @@ __global__ __launch_bounds__(64) void k_join_bf16(
     const uint32_t lr = l % 16, lk = (l / 16) * 8;
     // this lane's dimensions of x_i for both MFMA steps, read once
     float xr0[8], xr1[8];
-    load8(xi, lk, RS, xr0);
-    load8(xi, 32 + lk, RS, xr1);
+    load8(xi, lk, rsv, xr0);
+    load8(xi, 32 + lk, rsv, xr1);
     for (uint32_t rt = 0; rt < rtiles; ++rt) {
-      const float* arow = rows + (size_t)(rt * 16 + lr) * RS;
-      const bf16x8 a0 = centred_bf16x8_r(arow, xr0, lk, d, RS);
-      const bf16x8 a1 = centred_bf16x8_r(arow, xr1, 32 + lk, d, RS);
+      const float* arow = rows + __umul24(rt * 16 + lr, RS);
+      const bf16x8 a0 = centred_bf16x8_r(arow, xr0, lk, rsv);
+      const bf16x8 a1 = centred_bf16x8_r(arow, xr1, 32 + lk, rsv);
       // the 4 rows of the tile this lane's outputs belong to: id, norm, slack * threshold
       uint32_t rid[4];
       float rnrm[4], rthr[4];
@@ -170,11 +174,11 @@ __global__ __launch_bounds__(64) void k_join_bf16(
       for (uint32_t ct = 0; ct < ctiles; ++ct) {
         // skip tiles entirely below the new x new diagonal
         if (ct * 16 + 15 < rt * 16 && ct * 16 + 15 < nn) continue;
-        const float* brow = rows + (size_t)(ct * 16 + lr) * RS;
+        const float* brow = rows + __umul24(ct * 16 + lr, RS);
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, centred_bf16x8_r(brow, xr0, lk, d, RS), acc, 0,
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, centred_bf16x8_r(brow, xr0, lk, rsv), acc, 0,
                                                       0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, centred_bf16x8_r(brow, xr1, 32 + lk, d, RS),
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, centred_bf16x8_r(brow, xr1, 32 + lk, rsv),
                                                       acc, 0, 0, 0);
         const uint32_t c = ct * 16 + lr;
         const uint32_t cid = s_id[c];

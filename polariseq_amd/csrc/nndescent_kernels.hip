@@ -359,7 +359,7 @@ __global__ __launch_bounds__(64) void k_join(
   extern __shared__ __attribute__((aligned(16))) float lds[];
   uint64_t* pb_key = reinterpret_cast<uint64_t*>(lds);
   float* rows = lds + 2 * PB_CAP;
-  uint32_t* s_id = reinterpret_cast<uint32_t*>(rows + (size_t)R * RS);
+  uint32_t* s_id = reinterpret_cast<uint32_t*>(rows + R * RS);
   float* s_thr = reinterpret_cast<float*>(s_id + R);
   uint32_t* pb_row = reinterpret_cast<uint32_t*>(s_thr + R);
   const uint32_t l = (uint32_t)lane_id();
@@ -388,7 +388,7 @@ __global__ __launch_bounds__(64) void k_join(
         ib = s_id[b];
         if (ia != ib) {
           ++pairs;
-          dist = lds_sq_dist<D>(rows + (size_t)a * RS, rows + (size_t)b * RS, d);
+          dist = lds_sq_dist<D>(rows + __umul24(a, RS), rows + __umul24(b, RS), d);
           pa = dist < s_thr[a];
           pb = dist < s_thr[b];
         }

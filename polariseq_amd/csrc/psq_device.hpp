@@ -438,28 +438,47 @@ __device__ inline bool stage_cell(const float* __restrict__ Xp, uint32_t d, uint
   const uint32_t v1 = (uint32_t)__shfl((int)ov, (int)((l + WAVE - nn) & (WAVE - 1)), WAVE);
   const float t0 = thr[l < nc ? v0 : i];
   const float t1 = thr[l + WAVE < nc ? v1 : i];
+  // Rows are copied in passes of WAVE / lpr rows, lpr lanes per row: lane l
+  // moves 16-byte chunk c = l % lpr of row s = pass * (WAVE / lpr) + l / lpr
+  // (chunks past c4 idle), so no lane divides by the row length. A pass never
+  // straddles row 64, so its rows' ids come from one of v0 / v1 (uniform).
   const uint32_t c4 = D > 0 ? (uint32_t)(D + 3) / 4 : (d + 3) / 4;
-  const uint32_t tot = (nc + (xi != nullptr ? 1u : 0u)) * c4;
-  for (uint32_t b0 = 0; b0 < tot; b0 += STAGE_BATCH * WAVE) {
+  const uint32_t lpr = c4 <= 16 ? 16u : (uint32_t)WAVE;
+  const uint32_t rpp = (uint32_t)WAVE / lpr;
+  const uint32_t lr = l / lpr, c = l % lpr;
+  const bool cact = c < c4;
+  const uint32_t cc = cact ? c : 0u;
+  const uint32_t nrows = nc + (xi != nullptr ? 1u : 0u);
+  const uint32_t npass = (nrows + rpp - 1) / rpp;
+  // LDS destinations as 32-bit float offsets from `rows` (x_i's row at xoff)
+  const uint32_t xoff = xi != nullptr ? (uint32_t)(xi - rows) : 0u;
+  for (uint32_t p0 = 0; p0 < npass; p0 += STAGE_BATCH) {
+    // passes past npass (uniform) are skipped
     float4 v[STAGE_BATCH];
 #pragma unroll
     for (int u = 0; u < STAGE_BATCH; ++u) {
-      const uint32_t e = b0 + (uint32_t)u * WAVE + l;
-      const uint32_t s = e / c4, c = e < tot ? e - s * c4 : 0u;
-      const uint32_t a = (uint32_t)__shfl((int)v0, (int)(s & (WAVE - 1)), WAVE);
-      const uint32_t b = (uint32_t)__shfl((int)v1, (int)(s & (WAVE - 1)), WAVE);
-      const uint32_t src = s >= nc ? i : (s < WAVE ? a : b);
-      v[u] = *reinterpret_cast<const float4*>(Xp + (size_t)src * dp + 4 * c);
+      if (p0 + (uint32_t)u >= npass) break;
+      const uint32_t base = (p0 + (uint32_t)u) * rpp;  // uniform
+      const uint32_t s = base + lr;
+      // every lane takes part in the permute (byte address of lane s mod 64); the
+      // select comes after
+      const uint32_t a = (uint32_t)__builtin_amdgcn_ds_bpermute(
+          (int)(((base & (WAVE - 1)) + lr) << 2), (int)(base < (uint32_t)WAVE ? v0 : v1));
+      const uint32_t src = s < nc ? a : i;
+      v[u] = *reinterpret_cast<const float4*>(Xp + (size_t)src * dp + 4 * cc);
     }
     // the values are pinned here, so no load is sunk into its (conditional) store
 #pragma unroll
-    for (int u = 0; u < STAGE_BATCH; ++u)
+    for (int u = 0; u < STAGE_BATCH; ++u) {
+      if (p0 + (uint32_t)u >= npass) break;
       asm volatile("" : "+v"(v[u].x), "+v"(v[u].y), "+v"(v[u].z), "+v"(v[u].w));
+    }
 #pragma unroll
     for (int u = 0; u < STAGE_BATCH; ++u) {
-      const uint32_t e = b0 + (uint32_t)u * WAVE + l;
-      const uint32_t s = e / c4, c = e - s * c4;
-      if (e < tot) *reinterpret_cast<float4*>((s < nc ? rows + (size_t)s * RS : xi) + 4 * c) = v[u];
+      if (p0 + (uint32_t)u >= npass) break;
+      const uint32_t s = (p0 + (uint32_t)u) * rpp + lr;
+      if (cact && s < nrows)
+        *reinterpret_cast<float4*>(rows + ((s < nc ? s * RS : xoff) + 4 * c)) = v[u];
     }
   }
   if (l < nc) {
