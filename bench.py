@@ -16,7 +16,8 @@ the same input:
     Adam), encodes every cell, and its codes seed the BF16 form at the operating
     point --learned-trees / --learned-heap-k / --learned-delta / --learned-slots
     (5 trees on the codes, rows of 26 while iterating, early stop at 0.07 n h
-    changes, 64 proposal slots per row).
+    changes, 64 proposal slots per row); the encoder trains for --train-steps Adam
+    steps of --train-batch triplets, hinge margin --train-margin.
 After the timed steps:
   * recall@15 of every form against the exact top-15 over all cells, for 100k
     random query cells (exact kNN on the GPU, FP64-ranked);
@@ -107,9 +108,12 @@ def main() -> int:
                     help="proposal slots per row, learned form (0 = the API default, 128 here)")
     ap.add_argument("--plain-heap-k", type=int, default=20,
                     help="rows while iterating of the plain_r95 form (output k)")
-    ap.add_argument("--train-steps", type=int, default=600)
+    ap.add_argument("--train-steps", type=int, default=100)
     ap.add_argument("--train-subset", type=int, default=200_000)
-    ap.add_argument("--train-anchors", type=int, default=16384)
+    ap.add_argument("--train-anchors", type=int, default=4096)
+    ap.add_argument("--train-batch", type=int, default=8192)
+    ap.add_argument("--train-lr", type=float, default=3e-3)
+    ap.add_argument("--train-margin", type=float, default=0.1)
     ap.add_argument("--recall-queries", type=int, default=100_000)
     ap.add_argument("--cpu-cells", type=int, default=4_060_000,
                     help="cells of the headline workload the CPU baseline runs on (all by default)")
@@ -176,7 +180,8 @@ def main() -> int:
         # weights on this embedding (triplet mining + Adam), then seeds the search
         enc.reset(args.seed)
         train_log.append(enc.train(xt, subset=args.train_subset, steps=args.train_steps,
-                                   anchors=args.train_anchors, seed=args.seed))
+                                   anchors=args.train_anchors, batch=args.train_batch,
+                                   lr=args.train_lr, margin=args.train_margin, seed=args.seed))
         return P.nndescent_device(xt, k, seed=args.seed, init="learned", model=enc,
                                   precision="bf16", n_trees=args.learned_trees,
                                   heap_k=args.learned_heap_k, delta=args.learned_delta,
@@ -358,7 +363,8 @@ def main() -> int:
                                                   forms_out["plain"]["cells_per_s"], 3)
     if head == "learned_bf16":
         line["encoder"] = {"steps": args.train_steps, "subset": args.train_subset,
-                           "anchors": args.train_anchors, "batch": 2048, "margin": 1.0,
+                           "anchors": args.train_anchors, "batch": args.train_batch,
+                           "lr": args.train_lr, "margin": args.train_margin,
                            "note": "trained from its initial weights inside every timed step"}
     print(json.dumps(line), flush=True)
     return 0

@@ -232,13 +232,13 @@ typedef struct psq_encoder psq_encoder;
 typedef struct psq_encoder_train_opts {
   uint64_t subset; /* cells in the training subset (0 -> 200000), evenly spaced */
   uint64_t k_pos;  /* positives drawn from the exact top-k_pos within the subset (0 -> 15) */
-  uint64_t batch;  /* triplets per step (0 -> 2048) */
-  uint64_t steps;  /* Adam steps (0 -> 300) */
-  float lr;        /* 0 -> 1e-3 */
-  float margin;    /* triplet margin on squared code distances (0 -> 1.0) */
+  uint64_t batch;  /* triplets per step (0 -> 8192) */
+  uint64_t steps;  /* Adam steps (0 -> 100) */
+  float lr;        /* 0 -> 3e-3 */
+  float margin;    /* triplet margin on squared code distances (0 -> 0.1) */
   uint64_t seed;
   uint64_t anchors; /* anchor pool: evenly spaced subset cells whose exact top-k_pos is
-                       mined (0 -> 16384); anchors are drawn from it, negatives from the subset */
+                       mined (0 -> 4096); anchors are drawn from it, negatives from the subset */
 } psq_encoder_train_opts;
 int psq_encoder_create(psq_encoder** out, uint64_t seed, int32_t device);
 void psq_encoder_destroy(psq_encoder* e);

@@ -10,7 +10,8 @@
 //    the centred vectors are short) rounded to BF16; one
 //    v_mfma_f32_16x16x32_bf16 pair per 16 x 16 tile gives the Gram block of new
 //    rows x all candidates, and d~(p,q) = |p~|^2 + |q~|^2 - 2 p~.q~ with the
-//    norms summed in FP32 from the same BF16 values;
+//    norms taken from the diagonal of each tile's own Gram block (the same
+//    BF16 values, the same matrix-core arithmetic);
 //  * a pair passes the screen when d~ < slack * thr of either endpoint; passing
 //    pairs are listed in LDS and, 64 at a time, get their exact f32 sq_dist
 //    (simd.rs:45-62 order) from the staged rows, which is compared with the
@@ -95,23 +96,6 @@ __global__ __launch_bounds__(64) void k_join_bf16(
     // floats staged per row: d rounded up to 4 (the padding is zero in every row)
     const uint32_t rsv = D > 0 ? (uint32_t)(D + 3) / 4 * 4 : (d + 3) / 4 * 4;
     __syncthreads();
-    for (uint32_t s = l; s < nc; s += WAVE) {
-      const float* row = rows + __umul24(s, RS);
-      float nrm = 0.0f;
-      for (uint32_t t = 0; t < rsv; t += 4) {
-        const float4 a = *reinterpret_cast<const float4*>(row + t);
-        const float4 x = *reinterpret_cast<const float4*>(xi + t);
-        const float av[4] = {a.x, a.y, a.z, a.w}, xv[4] = {x.x, x.y, x.z, x.w};
-        // a padding dimension adds fmaf(0, 0, nrm) = nrm
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const float b = (float)(__bf16)(av[u] - xv[u]);
-          nrm = fmaf(b, b, nrm);
-        }
-      }
-      s_nrm[s] = nrm;
-    }
-    __syncthreads();
 
     uint32_t pcnt = 0, scnt = 0;
     // Exact distances of the listed pairs, 64 at a time (all of them when `all`).
@@ -124,6 +108,7 @@ __global__ __launch_bounds__(64) void k_join_bf16(
         bool pa = false, pb = false;
         uint32_t ia = 0, ib = 0;
         float dist = 0.0f;
+        exact += min(scnt - done, (uint32_t)WAVE);
         if (e < scnt) {
           const uint32_t pr = sv[e], a = pr & 0xFFFF, b = pr >> 16;
           ia = s_id[a];
@@ -131,7 +116,6 @@ __global__ __launch_bounds__(64) void k_join_bf16(
           dist = lds_sq_dist<D>(rows + __umul24(a, RS), rows + __umul24(b, RS), d);
           pa = dist < s_thr[a];
           pb = dist < s_thr[b];
-          ++exact;
         }
         pb_append(pb_row, pb_key, pcnt, pa, ia, make_key(dist, ib, 1), sent);
         pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, bmask, seed, iter);
@@ -157,42 +141,64 @@ __global__ __launch_bounds__(64) void k_join_bf16(
     float xr0[8], xr1[8];
     load8(xi, lk, rsv, xr0);
     load8(xi, 32 + lk, rsv, xr1);
-    for (uint32_t rt = 0; rt < rtiles; ++rt) {
-      const float* arow = rows + __umul24(rt * 16 + lr, RS);
-      const bf16x8 a0 = centred_bf16x8_r(arow, xr0, lk, rsv);
-      const bf16x8 a1 = centred_bf16x8_r(arow, xr1, 32 + lk, rsv);
-      // the 4 rows of the tile this lane's outputs belong to: id, norm, slack * threshold
-      uint32_t rid[4];
-      float rnrm[4], rthr[4];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const uint32_t r = min(rt * 16 + (l / 16) * 4 + v, R - 1);
-        rid[v] = s_id[r];
-        rnrm[v] = s_nrm[r];
-        rthr[v] = slack * s_thr[r];
+    // Column tiles outermost: a tile's operands are built once per cell. In the
+    // MFMA layout lane l holds row l % 16, dimensions (l / 16) * 8 + [0, 8) of
+    // both operands, so a column tile's operands are also the A operands of the
+    // row tile of the same rows: kept for row tile 0, built again for later row
+    // tiles. Row tile 0 is never below the diagonal, and a
+    // row tile rt > ct is (nn > 16 rt > ct * 16 + 15), so every tile processed
+    // has rt <= ct and row tile rt's operands and norms exist when it is used.
+    // A row's norm |p~|^2 is the diagonal of its column tile's Gram block: lane l
+    // holds outputs (4 (l / 16) + v, l % 16), the diagonal at v = dv.
+    const int dv = (int)lr - 4 * (int)(l / 16);
+    bf16x8 t0a = {}, t0b = {};  // operands of row tile 0
+    for (uint32_t ct = 0; ct < ctiles; ++ct) {
+      const float* brow = rows + __umul24(ct * 16 + lr, RS);
+      const bf16x8 b0 = centred_bf16x8_r(brow, xr0, lk, rsv);
+      const bf16x8 b1 = centred_bf16x8_r(brow, xr1, 32 + lk, rsv);
+      {
+        f32x4 g = {0.0f, 0.0f, 0.0f, 0.0f};
+        g = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b0, b0, g, 0, 0, 0);
+        g = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b1, b1, g, 0, 0, 0);
+        const float gd = dv == 0 ? g[0] : (dv == 1 ? g[1] : (dv == 2 ? g[2] : g[3]));
+        if (dv >= 0 && dv < 4) s_nrm[ct * 16 + lr] = gd;
       }
-      for (uint32_t ct = 0; ct < ctiles; ++ct) {
+      if (ct == 0) {
+        t0a = b0;
+        t0b = b1;
+      }
+      __syncthreads();  // this tile's norms, written by other lanes
+      const uint32_t c = ct * 16 + lr;
+      const uint32_t cid = s_id[c];
+      const float cnrm = s_nrm[c], cthr = slack * s_thr[c];
+      const bool cval = c < nc, cnew = c < nn;
+      for (uint32_t rt = 0; rt < rtiles && rt <= ct; ++rt) {
         // skip tiles entirely below the new x new diagonal
         if (ct * 16 + 15 < rt * 16 && ct * 16 + 15 < nn) continue;
-        const float* brow = rows + __umul24(ct * 16 + lr, RS);
+        bf16x8 a0 = t0a, a1 = t0b;
+        if (rt > 0) {
+          const float* arow = rows + __umul24(rt * 16 + lr, RS);
+          a0 = centred_bf16x8_r(arow, xr0, lk, rsv);
+          a1 = centred_bf16x8_r(arow, xr1, 32 + lk, rsv);
+        }
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, centred_bf16x8_r(brow, xr0, lk, rsv), acc, 0,
-                                                      0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, centred_bf16x8_r(brow, xr1, 32 + lk, rsv),
-                                                      acc, 0, 0, 0);
-        const uint32_t c = ct * 16 + lr;
-        const uint32_t cid = s_id[c];
-        const float cnrm = s_nrm[c], cthr = slack * s_thr[c];
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc, 0, 0, 0);
+        // the 4 rows of this lane's outputs (16-byte aligned, all below R)
+        const uint32_t rb = rt * 16 + (l / 16) * 4;
+        const uint4 rid = *reinterpret_cast<const uint4*>(s_id + rb);
+        const float4 rn = *reinterpret_cast<const float4*>(s_nrm + rb);
+        const float4 rth = *reinterpret_cast<const float4*>(s_thr + rb);
+        const uint32_t ridv[4] = {rid.x, rid.y, rid.z, rid.w};
+        const float rnv[4] = {rn.x, rn.y, rn.z, rn.w}, rthv[4] = {rth.x, rth.y, rth.z, rth.w};
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const uint32_t r = rt * 16 + (l / 16) * 4 + v;
-          bool keep = false;
-          if (r < nn && c < nc && ((c < nn) ? (c > r) : (cid != rid[v]))) {
-            ++pairs;
-            float dt = rnrm[v] + cnrm - 2.0f * acc[v];
-            dt = dt > 0.0f ? dt : 0.0f;
-            keep = dt < rthr[v] || dt < cthr;
-          }
+          const uint32_t r = rb + v;
+          const bool valid = cval && r < nn && (cnew ? c > r : cid != ridv[v]);
+          pairs += (uint64_t)__popcll(__ballot(valid));
+          // d~ = |p~|^2 + |q~|^2 - 2 p~.q~ against slack x the larger threshold
+          const float dt = fmaf(-2.0f, acc[v], rnv[v] + cnrm);
+          const bool keep = valid && dt < fmaxf(slack * rthv[v], cthr);
           const uint64_t mask = __ballot(keep);
           if (keep) sv[scnt + (uint32_t)__popcll(mask & ((1ull << l) - 1ull))] = r | (c << 16);
           scnt += (uint32_t)__popcll(mask);
@@ -203,16 +209,11 @@ __global__ __launch_bounds__(64) void k_join_bf16(
     drain(true);
     pb_flush(pb_row, pb_key, pcnt, true, slots, touched, S, bmask, seed, iter);
   }();
-  uint64_t tp = pairs, te = exact;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    tp += shfl_xor64(tp, o);
-    te += shfl_xor64(te, o);
-  }
+  // pairs, exact, sent and rows_in are wave-uniform (counted from ballots)
   if (l == 0 && rows_in > 0) {
-    ctr_add(counters, 0, (unsigned long long)tp);
+    ctr_add(counters, 0, (unsigned long long)pairs);
     ctr_add(counters, 1, (unsigned long long)rows_in);
-    ctr_add(counters, 3, (unsigned long long)te);
+    ctr_add(counters, 3, (unsigned long long)exact);
     ctr_add(counters, 4, (unsigned long long)sent);
   }
 }
