@@ -404,8 +404,11 @@ uint32_t join_row_stride(uint64_t d) { return (uint32_t)(round_up((uint32_t)d, 4
 // Rows a join block holds in the launch over every cell. Fewer rows than 2m
 // mean less LDS per block and more resident blocks per CU: the join waits on
 // memory, and its time per cell falls with the blocks in flight. The cells with
-// more candidates are deferred to a second launch with room for 2m rows.
+// more candidates are deferred to launches over lists of cells: those with at
+// most JOIN_MID_ROWS candidates to one with room for that many rows, the rest
+// to one with room for 2m.
 constexpr uint32_t JOIN_SMALL_ROWS = 32;
+constexpr uint32_t JOIN_MID_ROWS = 48;
 
 struct JoinArgs {
   const float* Xp;
@@ -413,10 +416,11 @@ struct JoinArgs {
   const float* thr;
   uint64_t* slots;
   uint8_t* touched;
-  uint32_t* flag;   // n: cells the first launch leaves out
+  uint32_t* flag;   // n: cells of one deferred launch
   uint32_t* pos;    // n + 1: their exclusive scan
-  uint32_t* list;   // n: those cells in order
-  uint32_t* hcount; // pinned host word: their number
+  uint32_t* list;   // n: cells of the launch with JOIN_MID_ROWS rows, in order
+  uint32_t* list2;  // n: cells of the launch with room for 2m rows, in order
+  uint32_t* hcount; // pinned host words: the two lists' lengths
   hipEvent_t counted;
   unsigned long long* ctr;
   bool bf16;
@@ -450,22 +454,29 @@ int launch_join(Workspace* ws, const Resolved& R, const JoinArgs& a, uint64_t it
     return PSQ_OK;
   };
   if (full <= JOIN_SMALL_ROWS) return launch(full, JoinCells{nullptr, 0}, xcd_grid(n));
-  // the cells with more than JOIN_SMALL_ROWS candidates, listed in cell order
-  hipLaunchKernelGGL(k_join_defer_flags, dim3(grid_for(n, 256)), dim3(256), 0, s, a.ncnt, n,
-                     JOIN_SMALL_ROWS, a.flag);
-  HIPC(hipGetLastError());
-  TRY(scan_u32(ws, a.flag, a.pos, n, s));
-  hipLaunchKernelGGL(k_join_defer_list, dim3(grid_for(n, 256)), dim3(256), 0, s, a.flag, a.pos, n,
-                     a.list);
-  HIPC(hipGetLastError());
-  HIPC(hipMemcpyAsync(a.hcount, a.pos + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  // the cells with lo < candidates <= hi, listed in cell order, and their number
+  auto defer = [&](uint32_t lo, uint32_t hi, uint32_t* list, uint32_t* hcount) -> int {
+    hipLaunchKernelGGL(k_join_defer_flags, dim3(grid_for(n, 256)), dim3(256), 0, s, a.ncnt, n, lo,
+                       hi, a.flag);
+    HIPC(hipGetLastError());
+    TRY(scan_u32(ws, a.flag, a.pos, n, s));
+    hipLaunchKernelGGL(k_join_defer_list, dim3(grid_for(n, 256)), dim3(256), 0, s, a.flag, a.pos, n,
+                       list);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(hcount, a.pos + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    return PSQ_OK;
+  };
+  const uint32_t mid = std::min(full, JOIN_MID_ROWS);
+  TRY(defer(JOIN_SMALL_ROWS, mid, a.list, a.hcount));
+  if (mid < full) TRY(defer(mid, full, a.list2, a.hcount + 1));
   HIPC(hipEventRecord(a.counted, s));
   TRY(launch(JOIN_SMALL_ROWS, JoinCells{nullptr, 0}, xcd_grid(n)));
-  // the count is read while the first launch runs
+  // the counts are read while the first launch runs
   HIPC(hipEventSynchronize(a.counted));
-  const uint32_t cnt = *a.hcount;
-  if (cnt == 0) return PSQ_OK;
-  return launch(full, JoinCells{a.list, cnt}, xcd_grid(cnt));
+  const uint32_t c1 = *a.hcount, c2 = mid < full ? a.hcount[1] : 0u;
+  if (c1) TRY(launch(mid, JoinCells{a.list, c1}, xcd_grid(c1)));
+  if (c2) TRY(launch(full, JoinCells{a.list2, c2}, xcd_grid(c2)));
+  return PSQ_OK;
 }
 
 double ms_between(hipEvent_t a, hipEvent_t b) {
@@ -789,7 +800,7 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   uint64_t *G, *slots, *rev;
   float* thr;
   uint8_t* touched;
-  uint32_t *roff, *newl, *oldl, *ncnt, *rowcnt, *jflag, *jpos, *jlist;
+  uint32_t *roff, *newl, *oldl, *ncnt, *rowcnt, *jflag, *jpos, *jlist, *jlist2;
   unsigned long long* ctr;
   TRY(wbuf(ws, "Xp", (size_t)n * R.dp, &Xp));
   TRY(wbuf(ws, "G", (size_t)n * k, &G));
@@ -804,6 +815,7 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   TRY(wbuf(ws, "jflag", n, &jflag));
   TRY(wbuf(ws, "jpos", (size_t)n + 1, &jpos));
   TRY(wbuf(ws, "jlist", n, &jlist));
+  TRY(wbuf(ws, "jlist2", n, &jlist2));
   TRY(wbuf(ws, "rowcnt", n, &rowcnt));
   TRY(wbuf(ws, "counters", (size_t)(CTR_STRIPES + 1) * CTR_WORDS, &ctr));
   const PartShape RP = part_shape(n, 8);
@@ -887,8 +899,8 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
     HIPC(hipEventRecord(a, s));
     hipEvent_t counted;
     TRY(evp.make(&counted));
-    const JoinArgs ja{Xp,    newl,  oldl,  ncnt,  thr,     slots, touched, jflag, jpos,
-                      jlist, static_cast<uint32_t*>(hp) + 960, counted,   ctr,   bf16};
+    const JoinArgs ja{Xp,     newl,  oldl,  ncnt,  thr,     slots, touched, jflag, jpos, jlist,
+                      jlist2, static_cast<uint32_t*>(hp) + 960, counted,   ctr,   bf16};
     TRY(launch_join(ws, R, ja, it, s));
     HIPC(hipEventRecord(b, s));
     jb.push_back(a);

@@ -411,14 +411,14 @@ __global__ __launch_bounds__(64) void k_join(
   }
 }
 
-// Cells a join launch of R rows per block leaves out: flag[i] = 1 when cell i
-// has a new candidate and more than R candidates.
-__global__ void k_join_defer_flags(const uint32_t* __restrict__ ncnt, uint32_t n, uint32_t R,
-                                   uint32_t* __restrict__ flag) {
+// Cells of one deferred join launch: flag[i] = 1 when cell i has a new
+// candidate and lo < candidates <= hi.
+__global__ void k_join_defer_flags(const uint32_t* __restrict__ ncnt, uint32_t n, uint32_t lo,
+                                   uint32_t hi, uint32_t* __restrict__ flag) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t c = ncnt[i], nn = c & 0xFFFF, nc = nn + (c >> 16);
-  flag[i] = (nn > 0 && nc > R) ? 1u : 0u;
+  flag[i] = (nn > 0 && nc > lo && nc <= hi) ? 1u : 0u;
 }
 // The flagged cells in cell order: list[pos[i]] = i (pos = exclusive scan of flag).
 __global__ void k_join_defer_list(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,

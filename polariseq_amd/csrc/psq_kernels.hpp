@@ -27,7 +27,8 @@ __global__ void k_join(const float* Xp, uint32_t n, uint32_t d, uint32_t dp, uin
                        const uint32_t* newl, const uint32_t* oldl, const uint32_t* ncnt, uint32_t m,
                        uint32_t R, JoinCells jc, const float* thr, uint64_t* slots, uint8_t* touched,
                        uint32_t S, uint64_t seed, uint64_t iter, unsigned long long* counters);
-__global__ void k_join_defer_flags(const uint32_t* ncnt, uint32_t n, uint32_t R, uint32_t* flag);
+__global__ void k_join_defer_flags(const uint32_t* ncnt, uint32_t n, uint32_t lo, uint32_t hi,
+                                   uint32_t* flag);
 __global__ void k_join_defer_list(const uint32_t* flag, const uint32_t* pos, uint32_t n, uint32_t* list);
 template <int P>
 __global__ void k_merge_wide(uint64_t* G, uint32_t n, uint32_t k, uint64_t* slots, uint32_t S,
