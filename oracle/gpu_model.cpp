@@ -70,9 +70,14 @@ static inline float key_dist(uint64_t key) { return bitsf((uint32_t)(key >> 32))
 static inline uint64_t tree_hash(uint64_t tseed, uint64_t level, uint64_t a, uint64_t salt) {
   return splitmix(tseed ^ splitmix(level * GOLD + salt) ^ splitmix(a * GOLD + 0x51ED27ull));
 }
-static inline uint32_t slot_hash(uint64_t seed, uint64_t iter, uint32_t q) {
-  uint64_t h = splitmix((seed ^ 0xA5A5A5A5DEADBEEFull) + (iter << 32) + q);
-  return (uint32_t)(h >> 32);
+// Bucket of neighbour q among a row's n_buckets (a power of two, >= 8) in an
+// iteration: the top bits of (q xor salt) * 0x9E3779B1, the salt a splitmix of
+// (seed, iter).
+static inline uint32_t slot_hash(uint64_t seed, uint64_t iter, uint32_t q, size_t n_buckets) {
+  const uint32_t salt = (uint32_t)(splitmix((seed ^ 0xA5A5A5A5DEADBEEFull) + (iter << 32)) >> 32);
+  uint32_t bits = 0;
+  while (((size_t)1 << bits) < n_buckets) ++bits;
+  return bits ? ((q ^ salt) * 0x9E3779B1u) >> (32 - bits) : 0u;
 }
 
 static inline uint32_t leaf_group(size_t leaf_size) {
@@ -464,11 +469,11 @@ int64_t psq_o_gpu_model_nndescent(const float* data, uint64_t n_obs, uint64_t n_
         return false;
       };
       for (size_t s = 0; s < k; ++s)
-        if (rowp[s] != EMPTY_KEY) pinned[slot_hash(seed, iter, key_idx(rowp[s])) & (n_buckets - 1)]++;
+        if (rowp[s] != EMPTY_KEY) pinned[slot_hash(seed, iter, key_idx(rowp[s]), n_buckets)]++;
       for (uint32_t r = poff[p]; r < poff[p + 1]; ++r) {
         uint64_t key = pk[r];
         if (in_row(key_idx(key))) continue;
-        bk[slot_hash(seed, iter, key_idx(key)) & (n_buckets - 1)].push_back(key);
+        bk[slot_hash(seed, iter, key_idx(key), n_buckets)].push_back(key);
       }
       std::vector<uint64_t> pr;
       for (size_t bi = 0; bi < n_buckets; ++bi) {

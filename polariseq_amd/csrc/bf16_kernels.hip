@@ -82,7 +82,7 @@ __global__ __launch_bounds__(64) void k_join_bf16(
   uint32_t* pb_row = reinterpret_cast<uint32_t*>(s_nrm + R);
   uint32_t* sv = pb_row + PB_CAP;
   const uint32_t l = (uint32_t)lane_id();
-  const uint32_t bmask = S / BUCKET - 1;
+  const SlotHash sh = slot_hash_of(seed, iter, S);
   uint64_t pairs = 0, exact = 0, sent = 0, rows_in = 0;
   uint32_t i;
   if (!join_cell_of_block(jc, n, i)) return;
@@ -118,9 +118,9 @@ __global__ __launch_bounds__(64) void k_join_bf16(
           pb = dist < s_thr[b];
         }
         pb_append(pb_row, pb_key, pcnt, pa, ia, make_key(dist, ib, 1), sent);
-        pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, bmask, seed, iter);
+        pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, sh);
         pb_append(pb_row, pb_key, pcnt, pb, ib, make_key(dist, ia, 1), sent);
-        pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, bmask, seed, iter);
+        pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, sh);
         done += WAVE;
       }
       if (all) {
@@ -207,7 +207,7 @@ __global__ __launch_bounds__(64) void k_join_bf16(
       }
     }
     drain(true);
-    pb_flush(pb_row, pb_key, pcnt, true, slots, touched, S, bmask, seed, iter);
+    pb_flush(pb_row, pb_key, pcnt, true, slots, touched, S, sh);
   }();
   // pairs, exact, sent and rows_in are wave-uniform (counted from ballots)
   if (l == 0 && rows_in > 0) {

@@ -314,13 +314,13 @@ __global__ __launch_bounds__(64) void k_sample_select(
     }
   }
   // bucket layout
-  const uint32_t bmask = S / BUCKET - 1;
+  const SlotHash sh = slot_hash_of(seed, iter, S);
   for (uint32_t e = l; e < S; e += WAVE) s_pin[e] = EMPTY_KEY;
   __syncthreads();
 #pragma unroll
   for (int v = 0; v < KV; ++v)
     if (fvalid[v]) {
-      const uint32_t b = slot_hash(seed, iter, me[v]) & bmask;
+      const uint32_t b = slot_bucket(sh, me[v]);
       const uint32_t r = atomicAdd(&s_bcnt[b], 1u);
       if (r < BUCKET) s_pin[b * BUCKET + r] = pin_key(me[v]);
     }
@@ -363,7 +363,7 @@ __global__ __launch_bounds__(64) void k_join(
   float* s_thr = reinterpret_cast<float*>(s_id + R);
   uint32_t* pb_row = reinterpret_cast<uint32_t*>(s_thr + R);
   const uint32_t l = (uint32_t)lane_id();
-  const uint32_t bmask = S / BUCKET - 1;
+  const SlotHash sh = slot_hash_of(seed, iter, S);
   uint64_t pairs = 0, sent = 0, rows_in = 0;
   uint32_t i;
   if (!join_cell_of_block(jc, n, i)) return;
@@ -394,11 +394,11 @@ __global__ __launch_bounds__(64) void k_join(
         }
       }
       pb_append(pb_row, pb_key, pcnt, pa, ia, make_key(dist, ib, 1), sent);
-      pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, bmask, seed, iter);
+      pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, sh);
       pb_append(pb_row, pb_key, pcnt, pb, ib, make_key(dist, ia, 1), sent);
-      pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, bmask, seed, iter);
+      pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, sh);
     }
-    pb_flush(pb_row, pb_key, pcnt, true, slots, touched, S, bmask, seed, iter);
+    pb_flush(pb_row, pb_key, pcnt, true, slots, touched, S, sh);
   }();
   // counters[0] += pairs, [1] += candidate rows gathered, [4] += proposals sent
   uint64_t tot = pairs;

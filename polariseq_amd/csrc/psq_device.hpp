@@ -35,8 +35,19 @@ __host__ __device__ inline uint64_t tree_hash(uint64_t tseed, uint64_t level, ui
                                               uint64_t salt) {
   return splitmix(tseed ^ splitmix(level * GOLD + salt) ^ splitmix(a * GOLD + 0x51ED27ull));
 }
-__host__ __device__ inline uint32_t slot_hash(uint64_t seed, uint64_t iter, uint32_t q) {
-  return (uint32_t)(splitmix((seed ^ 0xA5A5A5A5DEADBEEFull) + (iter << 32) + q) >> 32);
+// Proposal buckets: neighbour q goes to bucket slot_bucket(salt, q, bits) of a
+// row's 2^bits buckets in an iteration, the top bits of a multiplicative hash
+// of q xor a per-iteration salt (one multiply per key; the salt is uniform).
+struct SlotHash {
+  uint32_t salt, bits;
+};
+__host__ __device__ inline SlotHash slot_hash_of(uint64_t seed, uint64_t iter, uint32_t S) {
+  uint32_t bits = 0;
+  while ((8u << bits) < S) ++bits;  // S / 8 buckets
+  return SlotHash{(uint32_t)(splitmix((seed ^ 0xA5A5A5A5DEADBEEFull) + (iter << 32)) >> 32), bits};
+}
+__host__ __device__ inline uint32_t slot_bucket(const SlotHash& h, uint32_t q) {
+  return h.bits ? ((q ^ h.salt) * 0x9E3779B1u) >> (32 - h.bits) : 0u;
 }
 
 // Row key: (dist_bits << 32) | (idx << 1) | is_new. dist >= 0, so the unsigned
@@ -280,7 +291,7 @@ __device__ inline void ctr_add(unsigned long long* ctr, uint32_t word, unsigned 
 
 // ── proposals: bucket cascade ──
 // Each row owns S/8 buckets of 8 slots; a neighbour q goes to bucket
-// slot_hash(seed, iter, q) & (S/8 - 1). Before the join, a row's buckets are
+// slot_bucket(slot_hash_of(seed, iter, S), q). Before the join, a row's buckets are
 // laid out with its current neighbours "pinned": pinned key = idx << 1 (dist
 // field 0), ascending, at most 8 per bucket. A proposal is stored as its row
 // key plus 1 << 32 (dist field + 1), so it sorts after every pinned key and
@@ -299,32 +310,38 @@ __device__ inline uint64_t bucket_to_row(uint64_t b) {
 // replaces that slot with one compare-and-swap (the first such slot when
 // several are equal, e.g. empty). A failed swap returns the slot's current
 // value and the choice is made again. Slots only ever decrease, so every value
-// read is the slot's value at some earlier time; two inserts of one key then
-// cannot both succeed in different slots (each would have to see the other's
-// slot as the larger, older one), and the content after all proposals is the
-// set of the 8 smallest distinct keys, whatever the order of arrival.
+// read is the slot's value at some earlier time; two inserts of one key cannot
+// both succeed in different slots (each would have to see the other's slot as
+// the larger, older one), and the content after all proposals is the set of
+// the 8 smallest distinct keys, whatever the order of arrival.
+// "Absent" is decided on the low word: every key of neighbour q in a row's
+// buckets, pinned (q << 1) or proposed (row key + 2^32), has low word
+// (q << 1) | flag, and a proposal of q always carries the same distance (the
+// distance of one pair is computed one way), so a low-word match is a match of
+// the whole key. An empty slot's low word (2^32 - 1) matches no index < 2^31 - 1.
 __device__ inline void propose(uint64_t* __restrict__ slots, uint8_t* __restrict__ touched,
-                               uint32_t row, uint64_t key, uint32_t S, uint32_t bmask,
-                               uint64_t seed, uint64_t iter) {
+                               uint32_t row, uint64_t key, uint32_t S, const SlotHash& sh) {
   const uint32_t q = key_idx(key);
-  uint32_t b = slot_hash(seed, iter, q) & bmask;
+  const uint32_t b = slot_bucket(sh, q);
   uint64_t* bk = slots + (size_t)row * S + (size_t)b * BUCKET;
   const ulonglong2* b2 = reinterpret_cast<const ulonglong2*>(bk);
-  const uint64_t bkey = bucket_key(key), pin = pin_key(q);
+  const uint64_t bkey = bucket_key(key);
+  const uint32_t qq = q << 1;
   const ulonglong2 a = b2[0], bb = b2[1], c = b2[2], e = b2[3];
   uint64_t v[8] = {a.x, a.y, bb.x, bb.y, c.x, c.y, e.x, e.y};
   for (;;) {
+    bool dup = false;
     uint64_t mx = v[0];
     int jm = 0;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      if (v[j] == bkey || v[j] == pin) return;
-      if (v[j] > mx) {
+      dup |= ((uint32_t)v[j] & ~1u) == qq;
+      if (j > 0 && v[j] > mx) {
         mx = v[j];
         jm = j;
       }
     }
-    if (bkey >= mx) return;  // 8 smaller keys present
+    if (dup || bkey >= mx) return;  // present, or 8 smaller keys present
     const uint64_t old = atomicCAS((unsigned long long*)&bk[jm], (unsigned long long)mx,
                                    (unsigned long long)bkey);
     if (old == mx) break;
@@ -533,16 +550,16 @@ __device__ inline void pb_append(uint32_t* pb_row, uint64_t* pb_key, uint32_t& c
 // Sends every full batch of 64 (all entries when `all`), keeping the rest.
 __device__ inline void pb_flush(uint32_t* pb_row, uint64_t* pb_key, uint32_t& cnt, bool all,
                                 uint64_t* __restrict__ slots, uint8_t* __restrict__ touched,
-                                uint32_t S, uint32_t bmask, uint64_t seed, uint64_t iter) {
+                                uint32_t S, const SlotHash& sh) {
   if (cnt < (all ? 1u : (uint32_t)WAVE)) return;
   const uint32_t l = (uint32_t)lane_id();
   __syncthreads();
   uint32_t done = 0;
   for (; done + WAVE <= cnt; done += WAVE)
-    propose(slots, touched, pb_row[done + l], pb_key[done + l], S, bmask, seed, iter);
+    propose(slots, touched, pb_row[done + l], pb_key[done + l], S, sh);
   const uint32_t rest = cnt - done;
   if (all) {
-    if (l < rest) propose(slots, touched, pb_row[done + l], pb_key[done + l], S, bmask, seed, iter);
+    if (l < rest) propose(slots, touched, pb_row[done + l], pb_key[done + l], S, sh);
     cnt = 0;
     return;
   }
