@@ -527,12 +527,17 @@ int launch_leaf_merge(uint32_t KM, uint32_t LG, const float* Xp, uint32_t d, uin
   const unsigned grid = grid_for(nl, bt / LG);
   void (*kern)(const float*, uint32_t, uint32_t, uint32_t, uint32_t, const uint32_t*,
                const uint32_t*, const uint32_t*, uint32_t, uint64_t*, uint32_t) = nullptr;
-  if (KM == 16 && LG == 16) kern = k_leaf_merge<D, 16, 16>;
-  else if (KM == 32 && LG == 16) kern = k_leaf_merge<D, 32, 16>;
-  else if (KM == 32 && LG == 32) kern = k_leaf_merge<D, 32, 32>;
-  else if (KM == 64 && LG == 16) kern = k_leaf_merge<D, 64, 16>;
-  else if (KM == 64 && LG == 32) kern = k_leaf_merge<D, 64, 32>;
-  else kern = k_leaf_merge<D, 64, 64>;
+  // rows of the common lengths get kernels with the length fixed at compile time
+  if (LG == 16 && k == 15) kern = k_leaf_merge<D, 15, 16, 15>;
+  else if (LG == 16 && k == 20) kern = k_leaf_merge<D, 20, 16, 20>;
+  else if (LG == 16 && k == 26) kern = k_leaf_merge<D, 26, 16, 26>;
+  else if (LG == 16 && k == 30) kern = k_leaf_merge<D, 30, 16, 30>;
+  else if (KM == 16 && LG == 16) kern = k_leaf_merge<D, 16, 16, 0>;
+  else if (KM == 32 && LG == 16) kern = k_leaf_merge<D, 32, 16, 0>;
+  else if (KM == 32 && LG == 32) kern = k_leaf_merge<D, 32, 32, 0>;
+  else if (KM == 64 && LG == 16) kern = k_leaf_merge<D, 64, 16, 0>;
+  else if (KM == 64 && LG == 32) kern = k_leaf_merge<D, 64, 32, 0>;
+  else kern = k_leaf_merge<D, 64, 64, 0>;
   if (smem > 65536) HIPC(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
   hipLaunchKernelGGL(kern, block_grid(grid), dim3(bt), smem, s, Xp, d, dp, RS, n, ids_t, gs, gc, nl, G, k);
   HIPC(hipGetLastError());

@@ -643,7 +643,12 @@ __device__ inline float reg_lds_sq_dist(const float (&me)[D > 0 ? D : 1], const 
   return (((acc[0] + acc[4]) + (acc[1] + acc[5])) + ((acc[2] + acc[6]) + (acc[3] + acc[7]))) + tail;
 }
 
-template <int D, int KMAX, int LG>
+// KE > 0: rows of exactly KE = KMAX entries (k == KE), so the row's worst is a
+// fixed register and no test on k remains; KE = 0: rows of k <= KMAX entries.
+// With KE, "already present" compares indices only: every key of neighbour q
+// in the row carries the one distance computed for the pair, so an index match
+// is a (dist, idx) match (an empty slot's index, 2^31 - 1, matches no point).
+template <int D, int KMAX, int LG, int KE>
 __global__ __launch_bounds__(256) void k_leaf_merge(
     const float* __restrict__ Xp, uint32_t d, uint32_t dp, uint32_t RS, uint32_t n,
     const uint32_t* __restrict__ ids_t, const uint32_t* __restrict__ g_start,
@@ -679,10 +684,12 @@ __global__ __launch_bounds__(256) void k_leaf_merge(
     }
     __syncthreads();
     if (valid) {
+      static_assert(KE == 0 || KE == KMAX, "KE is the exact row length");
+      const uint32_t kk = KE > 0 ? (uint32_t)KE : k;
       uint64_t row[KMAX];
-      uint64_t* gr = G + (size_t)p * k;
+      uint64_t* gr = G + (size_t)p * kk;
 #pragma unroll
-      for (int s = 0; s < KMAX; ++s) row[s] = (uint32_t)s < k ? gr[s] : EMPTY_KEY;
+      for (int s = 0; s < KMAX; ++s) row[s] = (uint32_t)s < kk ? gr[s] : EMPTY_KEY;
       const float* mine = grows + (size_t)j * RS;
       float me[DR];
       if (D > 0) {
@@ -704,17 +711,27 @@ __global__ __launch_bounds__(256) void k_leaf_merge(
         const uint32_t qi = gids[q];
         const uint64_t key = make_key(dist, qi, 1);
         uint64_t worst = EMPTY_KEY;
+        if (KE > 0) {
+          worst = row[KMAX - 1];
+        } else {
 #pragma unroll
-        for (int s = 0; s < KMAX; ++s)
-          if ((uint32_t)s == k - 1) worst = row[s];
+          for (int s = 0; s < KMAX; ++s)
+            if ((uint32_t)s == k - 1) worst = row[s];
+        }
         if (key >= worst) continue;
         bool dup = false;
+        if (KE > 0) {
+          const uint32_t qk = qi << 1;
 #pragma unroll
-        for (int s = 0; s < KMAX; ++s) dup |= ((uint32_t)s < k) && ((row[s] >> 1) == (key >> 1));
+          for (int s = 0; s < KMAX; ++s) dup |= ((uint32_t)row[s] & ~1u) == qk;
+        } else {
+#pragma unroll
+          for (int s = 0; s < KMAX; ++s) dup |= ((uint32_t)s < k) && ((row[s] >> 1) == (key >> 1));
+        }
         if (dup) continue;
 #pragma unroll
         for (int s = KMAX - 1; s > 0; --s) {
-          if ((uint32_t)s < k) {
+          if ((uint32_t)s < kk) {
             const uint64_t a = row[s - 1];
             row[s] = key < a ? a : (key < row[s] ? key : row[s]);
           }
@@ -723,7 +740,7 @@ __global__ __launch_bounds__(256) void k_leaf_merge(
       }
 #pragma unroll
       for (int s = 0; s < KMAX; ++s)
-        if ((uint32_t)s < k) gr[s] = row[s];
+        if ((uint32_t)s < kk) gr[s] = row[s];
     }
     __syncthreads();
   }
@@ -805,21 +822,22 @@ template __global__ void k_leaf_merge_wide<50, 4>(const float*, uint32_t, uint32
                                                   const uint32_t*, const uint32_t*, const uint32_t*,
                                                   uint32_t, uint64_t*, uint32_t, uint32_t);
 
-#define PSQ_LEAF_INST(D, KM, LG)                                                              \
-  template __global__ void k_leaf_merge<D, KM, LG>(const float*, uint32_t, uint32_t, uint32_t, \
-                                                   uint32_t, const uint32_t*, const uint32_t*, \
-                                                   const uint32_t*, uint32_t, uint64_t*, uint32_t);
-PSQ_LEAF_INST(0, 16, 16)
-PSQ_LEAF_INST(0, 32, 16)
-PSQ_LEAF_INST(0, 32, 32)
-PSQ_LEAF_INST(0, 64, 16)
-PSQ_LEAF_INST(0, 64, 32)
-PSQ_LEAF_INST(0, 64, 64)
-PSQ_LEAF_INST(50, 16, 16)
-PSQ_LEAF_INST(50, 32, 16)
-PSQ_LEAF_INST(50, 32, 32)
-PSQ_LEAF_INST(50, 64, 16)
-PSQ_LEAF_INST(50, 64, 32)
-PSQ_LEAF_INST(50, 64, 64)
+#define PSQ_LEAF_INST(D, KM, LG, KE)                                                              \
+  template __global__ void k_leaf_merge<D, KM, LG, KE>(const float*, uint32_t, uint32_t, uint32_t, \
+                                                       uint32_t, const uint32_t*, const uint32_t*, \
+                                                       const uint32_t*, uint32_t, uint64_t*, uint32_t);
+#define PSQ_LEAF_INST_D(D)     \
+  PSQ_LEAF_INST(D, 16, 16, 0)  \
+  PSQ_LEAF_INST(D, 32, 16, 0)  \
+  PSQ_LEAF_INST(D, 32, 32, 0)  \
+  PSQ_LEAF_INST(D, 64, 16, 0)  \
+  PSQ_LEAF_INST(D, 64, 32, 0)  \
+  PSQ_LEAF_INST(D, 64, 64, 0)  \
+  PSQ_LEAF_INST(D, 15, 16, 15) \
+  PSQ_LEAF_INST(D, 20, 16, 20) \
+  PSQ_LEAF_INST(D, 26, 16, 26) \
+  PSQ_LEAF_INST(D, 30, 16, 30)
+PSQ_LEAF_INST_D(0)
+PSQ_LEAF_INST_D(50)
 
 }  // namespace psq

@@ -95,7 +95,7 @@ template <int D, int KV>
 __global__ void k_leaf_merge_wide(const float* Xp, uint32_t d, uint32_t dp, uint32_t RS, uint32_t n,
                                   const uint32_t* ids_t, const uint32_t* g_start, const uint32_t* g_count,
                                   uint32_t nleaves, uint64_t* G, uint32_t k, uint32_t LG);
-template <int D, int KMAX, int LG>
+template <int D, int KMAX, int LG, int KE>
 __global__ void k_leaf_merge(const float* Xp, uint32_t d, uint32_t dp, uint32_t RS, uint32_t n,
                              const uint32_t* ids_t, const uint32_t* g_start, const uint32_t* g_count,
                              uint32_t nleaves, uint64_t* G, uint32_t k);
