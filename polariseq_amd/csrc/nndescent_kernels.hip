@@ -260,12 +260,12 @@ __global__ __launch_bounds__(64) void k_sample_select(
   // pass 1: every replacement is known before a forward key is used
   for (uint32_t j = 0; j < nvec; ++j) (void)rev_key(j * WAVE + l, true);
   __syncthreads();
-  // pass 2: selection
-  uint64_t bestN = EMPTY_KEY, bestO = EMPTY_KEY;
-  uint32_t nn = 0, no = 0;
-  for (uint32_t j = 0; j < nvec; ++j) {
+  // pass 2: selection. A row's lists are sets: the join pairs their members in
+  // any order (every proposal goes through the order-free bucket insert), so a
+  // class with at most m entries needs no sort; otherwise its m smallest keys
+  // (class, priority, neighbour) are kept.
+  auto key_of = [&](uint32_t j) -> uint64_t {
     const uint32_t t = j * WAVE + l;
-    uint64_t key;
     if (t < k) {
       uint64_t f = EMPTY_KEY;
       bool fv = false;
@@ -275,36 +275,61 @@ __global__ __launch_bounds__(64) void k_sample_select(
           f = fk[v];
           fv = fvalid[v];
         }
-      key = (fv && s_kkey[t] == EMPTY_KEY) ? f : EMPTY_KEY;
-    } else {
-      key = rev_key(t, false);
+      return (fv && s_kkey[t] == EMPTY_KEY) ? f : EMPTY_KEY;
     }
-    const uint64_t sk = wave_sort64(key);
-    const uint32_t cN = (uint32_t)__popcll(__ballot(sk != EMPTY_KEY && (sk >> 63) == 0));
-    const uint32_t cO = (uint32_t)__popcll(__ballot(sk != EMPTY_KEY && (sk >> 63) == 1));
-    const uint64_t vN = l < cN ? sk : EMPTY_KEY;
-    const uint64_t sh = shfl64(sk, (int)((cN + l) & (WAVE - 1)));
-    const uint64_t vO = l < cO ? sh : EMPTY_KEY;
-    if (j == 0) {
-      bestN = vN;
-      bestO = vO;
-    } else {
-      bestN = wave_merge_keep64(bestN, vN);
-      bestO = wave_merge_keep64(bestO, vO);
+    return rev_key(t, false);
+  };
+  uint32_t nn = 0, no = 0;
+  uint64_t lastN = 0;  // the largest new-class key kept
+  bool sorted = true;
+  if (nvec == 1) {
+    const uint64_t key = key_of(0);
+    const bool isN = key != EMPTY_KEY && (key >> 63) == 0;
+    const bool isO = key != EMPTY_KEY && (key >> 63) != 0;
+    const uint64_t mN = __ballot(isN), mO = __ballot(isO);
+    const uint32_t cN = (uint32_t)__popcll(mN), cO = (uint32_t)__popcll(mO);
+    if (cN <= m && cO <= m) {  // uniform: every entry is kept
+      const uint64_t below = (1ull << l) - 1ull;
+      if (isN) newl[(size_t)i * m + (uint32_t)__popcll(mN & below)] = (uint32_t)key;
+      if (isO) oldl[(size_t)i * m + (uint32_t)__popcll(mO & below)] = (uint32_t)key;
+      if (l >= cN && l < m) newl[(size_t)i * m + l] = EMPTY;
+      if (l >= cO && l < m) oldl[(size_t)i * m + l] = EMPTY;
+      nn = cN;
+      no = cO;
+      lastN = ~0ull >> 1;  // above every new-class key
+      sorted = false;
     }
-    nn += cN;
-    no += cO;
   }
-  nn = nn < m ? nn : m;
-  no = no < m ? no : m;
-  if (l < m) {
-    newl[(size_t)i * m + l] = l < nn ? (uint32_t)bestN : EMPTY;
-    oldl[(size_t)i * m + l] = l < no ? (uint32_t)bestO : EMPTY;
+  if (sorted) {
+    uint64_t bestN = EMPTY_KEY, bestO = EMPTY_KEY;
+    for (uint32_t j = 0; j < nvec; ++j) {
+      const uint64_t sk = wave_sort64(key_of(j));
+      const uint32_t cN = (uint32_t)__popcll(__ballot(sk != EMPTY_KEY && (sk >> 63) == 0));
+      const uint32_t cO = (uint32_t)__popcll(__ballot(sk != EMPTY_KEY && (sk >> 63) == 1));
+      const uint64_t vN = l < cN ? sk : EMPTY_KEY;
+      const uint64_t sh = shfl64(sk, (int)((cN + l) & (WAVE - 1)));
+      const uint64_t vO = l < cO ? sh : EMPTY_KEY;
+      if (j == 0) {
+        bestN = vN;
+        bestO = vO;
+      } else {
+        bestN = wave_merge_keep64(bestN, vN);
+        bestO = wave_merge_keep64(bestO, vO);
+      }
+      nn += cN;
+      no += cO;
+    }
+    nn = nn < m ? nn : m;
+    no = no < m ? no : m;
+    if (l < m) {
+      newl[(size_t)i * m + l] = l < nn ? (uint32_t)bestN : EMPTY;
+      oldl[(size_t)i * m + l] = l < no ? (uint32_t)bestO : EMPTY;
+    }
+    lastN = shfl64(bestN, (int)(nn > 0 ? nn - 1 : 0));
   }
   if (l == 0) ncnt[i] = nn | (no << 16);
   // flag update: the surviving new-class key of this neighbour (the entry's own
   // or the reverse entry that replaced it) is among the nn selected
-  const uint64_t lastN = shfl64(bestN, (int)(nn > 0 ? nn - 1 : 0));
 #pragma unroll
   for (int v = 0; v < KV; ++v) {
     const uint32_t f = v * WAVE + l;
@@ -475,7 +500,7 @@ __global__ __launch_bounds__(64) void k_merge(uint64_t* __restrict__ G, uint32_t
   if (total <= (uint32_t)WAVE) {
     const uint64_t pv = l < total ? s_prop[l] : EMPTY_KEY;
     const uint64_t sr = force ? wave_sort64(orow) : orow;
-    const uint64_t mg = wave_merge_keep64(sr, wave_sort64(pv));
+    const uint64_t mg = wave_merge_keep64(sr, wave_sort_prefix(pv, total));
     const uint32_t nall = (uint32_t)__popcll(__ballot(mg != EMPTY_KEY));
     const uint32_t nk = nall < k ? nall : k;
     const uint64_t kth = shfl64(mg, (int)(nk > 0 ? nk - 1 : 0));

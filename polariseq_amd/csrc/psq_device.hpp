@@ -107,11 +107,13 @@ __device__ inline uint32_t xcd_row() {
 }
 __host__ inline uint32_t xcd_grid(uint64_t n) { return (uint32_t)((n + XCDS - 1) / XCDS * XCDS); }
 
-// Bitonic sort of one u64 per lane, ascending by lane.
-__device__ inline uint64_t wave_sort64(uint64_t v) {
-  const int l = lane_id();
+// Bitonic sort of one u64 per lane within each segment of W lanes (W = 16,
+// 32, 64), each segment ascending by lane.
+template <int W>
+__device__ inline uint64_t wave_sort_seg(uint64_t v) {
+  const int l = lane_id() & (W - 1);
 #pragma unroll
-  for (int size = 2; size <= WAVE; size <<= 1) {
+  for (int size = 2; size <= W; size <<= 1) {
 #pragma unroll
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
       uint64_t o = shfl_xor64(v, stride);
@@ -123,22 +125,18 @@ __device__ inline uint64_t wave_sort64(uint64_t v) {
   }
   return v;
 }
+// Bitonic sort of one u64 per lane, ascending by lane.
+__device__ inline uint64_t wave_sort64(uint64_t v) { return wave_sort_seg<WAVE>(v); }
 // Two independent bitonic sorts of 32: lanes 0-31 and lanes 32-63, each
 // ascending by lane.
-__device__ inline uint64_t wave_sort32x2(uint64_t v) {
-  const int l = lane_id() & (WAVE / 2 - 1);
-#pragma unroll
-  for (int size = 2; size <= WAVE / 2; size <<= 1) {
-#pragma unroll
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      uint64_t o = shfl_xor64(v, stride);
-      bool up = ((l & size) == 0);
-      bool lower = ((l & stride) == 0);
-      uint64_t mn = v < o ? v : o, mx = v < o ? o : v;
-      v = (up == lower) ? mn : mx;
-    }
-  }
-  return v;
+__device__ inline uint64_t wave_sort32x2(uint64_t v) { return wave_sort_seg<WAVE / 2>(v); }
+// Sort of a vector whose first `cnt` lanes hold keys and whose other lanes hold
+// EMPTY_KEY (the largest key): the lanes past the first 16 or 32 are already in
+// order, so a segment sort of 16 or 32 sorts the whole vector when cnt fits.
+__device__ inline uint64_t wave_sort_prefix(uint64_t v, uint32_t cnt) {
+  if (cnt <= 16) return wave_sort_seg<16>(v);
+  if (cnt <= 32) return wave_sort_seg<32>(v);
+  return wave_sort64(v);
 }
 // v holds an ascending-by-lane bitonic sequence's first half property: sort a
 // bitonic sequence of 64 ascending.
