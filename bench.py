@@ -9,6 +9,9 @@ the same input:
   * plain_r95: the same algorithm with rows of --plain-heap-k (20) while
     iterating, the smallest row size at which it clears recall@15 0.95 here;
     the speed-up of the headline is quoted against it;
+  * plain_op: the plain FP32 join at the learned form's operating point (5-tree
+    forest on the raw rows, rows of 26, stop at 0.07 n h, 64 slots): what the
+    operating point alone gives, without BF16 or the encoder;
   * bf16_raw: the BF16 join on an RP forest of the raw rows, at the learned
     form's operating point (what the learned seeding replaces);
   * learned_bf16, the headline: in every step the encoder is trained from its
@@ -42,12 +45,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "cells/s to kNN graph at recall@15>=0.95, 4M x 50 PCs; learned+BF16 vs plain"
-FORMS = ["plain", "plain_r95", "bf16_raw", "learned_bf16"]
+FORMS = ["plain", "plain_r95", "plain_op", "bf16_raw", "learned_bf16"]
 FORM_DESC = {
     "plain": "plain GPU NN-Descent, the drop-in (FP32 join, 48-tree RP forest, the reference's "
              "defaults)",
     "plain_r95": "plain GPU NN-Descent with wider rows while iterating (FP32 join, 48-tree RP "
                  "forest, defaults otherwise)",
+    "plain_op": "plain GPU NN-Descent (FP32 join) with a 5-tree RP forest on the raw rows, at the "
+                "learned form's operating point",
     "bf16_raw": "BF16 MFMA join + FP32 re-rank, RP forest on the raw rows, at the learned "
                 "form's operating point",
     "learned_bf16": "learned seeding (encoder trained inside the step) + BF16 MFMA join with "
@@ -193,6 +198,10 @@ def main() -> int:
         if form == "plain_r95":
             return lambda: P.nndescent_device(xt, k, seed=args.seed, init=args.init,
                                               heap_k=args.plain_heap_k)
+        if form == "plain_op":
+            return lambda: P.nndescent_device(xt, k, seed=args.seed, init="rp_forest",
+                                              n_trees=args.learned_trees, heap_k=args.learned_heap_k,
+                                              delta=args.learned_delta, slots=args.learned_slots)
         if form == "bf16_raw":
             return lambda: P.nndescent_device(xt, k, seed=args.seed, init="rp_forest",
                                               precision="bf16", n_trees=args.learned_trees,
@@ -361,6 +370,9 @@ def main() -> int:
     if "plain" in results and head != "plain":
         line["speedup_vs_plain_defaults"] = round(forms_out[head]["cells_per_s"] /
                                                   forms_out["plain"]["cells_per_s"], 3)
+    if "plain_op" in results and head != "plain_op":
+        line["speedup_vs_plain_same_operating_point"] = round(
+            forms_out[head]["cells_per_s"] / forms_out["plain_op"]["cells_per_s"], 3)
     if head == "learned_bf16":
         line["encoder"] = {"steps": args.train_steps, "subset": args.train_subset,
                            "anchors": args.train_anchors, "batch": args.train_batch,
