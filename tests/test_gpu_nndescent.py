@@ -78,6 +78,18 @@ def test_wider_rows_bit_exact_against_the_oracle(orc, init, model_init):
     _same(g, m)
 
 
+# Rows of 15, 20, 26 and 30 entries with leaves of at most 16 points take the
+# leaf-pair kernels whose row length is fixed at compile time (d = 50 and any d).
+@pytest.mark.parametrize("d,heap_k", [(50, 20), (50, 26), (50, 30), (20, 15), (20, 26), (33, 30)])
+def test_leaf_pairs_of_fixed_row_lengths_bit_exact_against_the_oracle(orc, d, heap_k):
+    x = ds.mixture(7000, d, 12, seed=heap_k)
+    print(f"fixed-length leaf pairs d={d} heap_k={heap_k}", flush=True)
+    g = P.nndescent_dense(x, 15, seed=5, n_trees=4, heap_k=heap_k, max_iterations=2)
+    assert P.last_stats()["heap_k"] == heap_k
+    m, _ = orc.gpu_model_nndescent(x, 15, seed=5, init=0, n_trees=4, heap_k=heap_k, max_iterations=2)
+    _same(g, m)
+
+
 @pytest.mark.parametrize("n,d,k,seed,init,heap_k", [(4000, 50, 65, 1, 0, 0), (5000, 20, 99, 2, 0, 0),
                                                      (3000, 50, 99, 3, 1, 0), (3000, 16, 150, 4, 0, 0),
                                                      (2500, 50, 200, 5, 1, 0), (4000, 50, 40, 6, 0, 130),
