@@ -317,6 +317,14 @@ __device__ inline uint64_t bucket_to_row(uint64_t b) {
 // (q << 1) | flag, and a proposal of q always carries the same distance (the
 // distance of one pair is computed one way), so a low-word match is a match of
 // the whole key. An empty slot's low word (2^32 - 1) matches no index < 2^31 - 1.
+// Among several slots holding the largest value (empty slots: keys are
+// distinct), a key takes the first in its own rotation of the 8 slots (by a
+// hash of q), so concurrent inserts of different keys spread over the empty
+// slots instead of all swapping on the first one (27% of the swaps of the first
+// BF16 join failed that way). Two inserts of one key still cannot both succeed:
+// they scan in the same order, the empty slots only become fewer, so the later
+// reader either picks the same slot or finds the earlier pick already filled
+// (and its own swap on an older view fails).
 __device__ inline void propose(uint64_t* __restrict__ slots, uint8_t* __restrict__ touched,
                                uint32_t row, uint64_t key, uint32_t S, const SlotHash& sh) {
   const uint32_t q = key_idx(key);
@@ -325,21 +333,23 @@ __device__ inline void propose(uint64_t* __restrict__ slots, uint8_t* __restrict
   const ulonglong2* b2 = reinterpret_cast<const ulonglong2*>(bk);
   const uint64_t bkey = bucket_key(key);
   const uint32_t qq = q << 1;
+  const uint32_t rot = (q * 0x85EBCA6Bu) >> 29;  // 0..7
   const ulonglong2 a = b2[0], bb = b2[1], c = b2[2], e = b2[3];
   uint64_t v[8] = {a.x, a.y, bb.x, bb.y, c.x, c.y, e.x, e.y};
   for (;;) {
     bool dup = false;
     uint64_t mx = v[0];
-    int jm = 0;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       dup |= ((uint32_t)v[j] & ~1u) == qq;
-      if (j > 0 && v[j] > mx) {
-        mx = v[j];
-        jm = j;
-      }
+      mx = v[j] > mx ? v[j] : mx;
     }
     if (dup || bkey >= mx) return;  // present, or 8 smaller keys present
+    uint32_t eq = 0;  // slots holding the largest value
+#pragma unroll
+    for (int j = 0; j < 8; ++j) eq |= (v[j] == mx ? 1u : 0u) << j;
+    const uint32_t rq = ((eq >> rot) | (eq << (8 - rot))) & 0xFFu;  // not 0: mx is a slot's value
+    const int jm = (int)((__builtin_ctz(rq) + rot) & 7u);
     const uint64_t old = atomicCAS((unsigned long long*)&bk[jm], (unsigned long long)mx,
                                    (unsigned long long)bkey);
     if (old == mx) break;
