@@ -182,28 +182,43 @@ int psq_unit_rows_device(const float* d_x, uint64_t n, uint64_t d, float* d_out,
 int psq_cosine_distances_device(float* d_dist, uint64_t count, void* stream);
 
 /* PCA of a CSR matrix (n_obs x n_features; indptr u64[n_obs + 1], indices u32,
- * data f32) by the exact Gram route. Replaces pca_csr_into / pca_scaled_blocks_into
+ * data f32). Replaces pca_csr_into / pca_scaled_blocks_into
  * (crates/polariseq-core/src/pca.rs:255-290, 396-431) as bound by pca_sparse and
  * pca_sparse_scaled (crates/polariseq-python/src/lib.rs:395-467).
  * scale = 0: centre when center != 0 (implicitly; the matrix is never changed).
  * scale = 1: the pp.scale view (ddof-1 sd, 0 -> 1, values clipped to
  *   +-max_value; max_value = +inf for none, else >= 0), always centred.
+ * method: PSQ_PCA_GRAM, the exact Gram route (pca.rs:487-535), n_features <=
+ *   32768; PSQ_PCA_RANDOMIZED, randomized block Krylov SVD with implicit
+ *   centring (rsvd.rs:476-566, pca.rs:273-286), configured by oversample,
+ *   n_iter (the Krylov depth) and seed (the reference's ChaCha8 Gaussian
+ *   sketch, rsvd.rs:414-435), its basis (n_components + oversample)(n_iter + 1)
+ *   columns (capped at min(n_obs, n_features)) at most 1024, n_obs < 2^31;
+ *   PSQ_PCA_AUTO, the Gram route up to 32768 genes and the randomized route
+ *   above. The reference takes the randomized route above 4096 genes
+ *   (pca.rs:201); this build stays exact up to 32768 (the result meets the
+ *   reference's acceptance bounds against the exact SVD, and more; the Gram
+ *   product costs n_obs n_features^2 flops, so wide matrices of many cells are
+ *   slow there: PSQ_PCA_RANDOMIZED asks for the reference's route instead).
  * Outputs: embedding f32[n_obs][n_components] (columns signed so that each
  * column's first largest-magnitude entry is positive), variance_ratio
- * f32[n_components] = max(eigenvalue, 0) / trace of the centred Gram, and, if
- * components is not NULL, the principal axes f32[n_features][n_components], with the
- * embedding's column signs (the centred matrix times the axes is the embedding).
- * Up to 4096 genes this is the reference's own route; from 4097 to 32768 the
- * reference switches to randomized SVD and this build stays exact (its result
- * meets the reference's acceptance bounds against the exact SVD, and more; the
- * Gram product costs n_obs n_features^2 flops, so wide matrices of many cells are
- * slow); n_features > 32768 returns PSQ_E_UNSUPPORTED; n_obs or n_features 0, or n_components 0 or above
- * min(n_obs, n_features), PSQ_E_INVALID with the reference's message. A run
- * repeats bit for bit. psq_pca_csr_device takes device pointers and runs on
- * the given stream of the current device. Unlike the neighbour-graph entries,
- * PCA allocates its device buffers per call (freed before it returns, outside
- * the psq_set_device_budget cap); the rocBLAS handle it creates on a device's
+ * f32[n_components] (Gram: max(eigenvalue, 0) / trace of the centred Gram;
+ * randomized: s_j^2 / ||A - 1 mu^T||_F^2), and, if components is not NULL,
+ * the principal axes f32[n_features][n_components], with the embedding's
+ * column signs (Gram: the centred matrix times the axes is the embedding;
+ * randomized: the sketch's right singular vectors B^T W / s).
+ * n_obs or n_features 0, or n_components 0 or above min(n_obs, n_features),
+ * PSQ_E_INVALID with the reference's message. A run repeats bit for bit.
+ * The _device forms take device pointers and run on the given stream of the
+ * current device. psq_pca_csr and psq_pca_csr_device are the _ex forms with
+ * PSQ_PCA_AUTO and the reference's default options (pca.rs:73-83: oversample
+ * 10, n_iter 1, seed 0). Unlike the neighbour-graph entries, PCA allocates its
+ * device buffers per call (freed before it returns, outside the
+ * psq_set_device_budget cap); the rocBLAS handle it creates on a device's
  * first call lives until the process ends. */
+#define PSQ_PCA_AUTO 0
+#define PSQ_PCA_GRAM 1
+#define PSQ_PCA_RANDOMIZED 2
 int psq_pca_csr(const uint64_t* indptr, const uint32_t* indices, const float* data, uint64_t n_obs,
                 uint64_t n_features, uint64_t n_components, int32_t center, int32_t scale,
                 double max_value, float* embedding, float* variance_ratio, float* components,
@@ -212,16 +227,31 @@ int psq_pca_csr_device(const uint64_t* d_indptr, const uint32_t* d_indices, cons
                        uint64_t n_obs, uint64_t n_features, uint64_t n_components, int32_t center,
                        int32_t scale, double max_value, float* d_embedding, float* d_variance_ratio,
                        float* d_components, void* stream);
+int psq_pca_csr_ex(const uint64_t* indptr, const uint32_t* indices, const float* data, uint64_t n_obs,
+                   uint64_t n_features, uint64_t n_components, int32_t center, int32_t scale,
+                   double max_value, int32_t method, uint64_t oversample, uint64_t n_iter,
+                   uint64_t seed, float* embedding, float* variance_ratio, float* components,
+                   int32_t device);
+int psq_pca_csr_device_ex(const uint64_t* d_indptr, const uint32_t* d_indices, const float* d_data,
+                          uint64_t n_obs, uint64_t n_features, uint64_t n_components, int32_t center,
+                          int32_t scale, double max_value, int32_t method, uint64_t oversample,
+                          uint64_t n_iter, uint64_t seed, float* d_embedding, float* d_variance_ratio,
+                          float* d_components, void* stream);
 /* Phases of this thread's last successful PCA call, from HIP events on its
- * stream: the scaled view's statistics and values, the Gram pass (densified row
- * chunks, column sums and the DGEMM products; ms_gemm is those launches alone),
- * the eigensolver, and projection + signs + ratios. */
+ * stream. Gram route: the scaled view, the Gram (densified chunks, column sums
+ * and the DGEMM products; ms_gemm is those launches alone), the eigensolver,
+ * and projection + signs + ratios. Randomized route: ms_gram holds the column
+ * sums and every pass building the Krylov basis and B^T (ms_gemm their DGEMM
+ * launches), ms_eigen the small eigenproblem, ms_project U, the outputs and
+ * signs; iterations counts the passes over the matrix and block the basis
+ * width. */
 typedef struct psq_pca_report {
-  uint64_t iterations; /* block products with C in the eigensolver */
-  uint64_t block;      /* subspace block b */
-  uint64_t chunks;     /* dense row chunks of the Gram pass */
+  uint64_t iterations; /* Gram: block products with C in the eigensolver; randomized: passes */
+  uint64_t block;      /* Gram: subspace block b; randomized: Krylov basis width */
+  uint64_t chunks;     /* dense row chunks of a pass */
   uint64_t rows_per_chunk;
   double ms_scale, ms_gram, ms_gemm, ms_eigen, ms_project, ms_total;
+  uint64_t method; /* PSQ_PCA_GRAM or PSQ_PCA_RANDOMIZED */
 } psq_pca_report;
 int psq_pca_last_report(psq_pca_report* out);
 

@@ -17,6 +17,9 @@ PSQ_E_INVALID = -1
 PSQ_E_UNSUPPORTED = -2
 PSQ_E_DEVICE = -3
 PSQ_E_NOMEM = -4
+PSQ_PCA_AUTO = 0
+PSQ_PCA_GRAM = 1
+PSQ_PCA_RANDOMIZED = 2
 
 PRECISION = {"fp32": 0, "bf16": 1, "bf16_rerank": 1}
 INIT = {"rp_forest": 0, "random": 1, "learned": 2}
@@ -57,6 +60,8 @@ EXPORTS = (
     "psq_cosine_distances_device",
     "psq_pca_csr",
     "psq_pca_csr_device",
+    "psq_pca_csr_ex",
+    "psq_pca_csr_device_ex",
     "psq_pca_last_report",
 )
 
@@ -79,11 +84,13 @@ class PcaReport(ctypes.Structure):
         ("iterations", ctypes.c_uint64), ("block", ctypes.c_uint64), ("chunks", ctypes.c_uint64),
         ("rows_per_chunk", ctypes.c_uint64), ("ms_scale", ctypes.c_double),
         ("ms_gram", ctypes.c_double), ("ms_gemm", ctypes.c_double), ("ms_eigen", ctypes.c_double),
-        ("ms_project", ctypes.c_double), ("ms_total", ctypes.c_double),
+        ("ms_project", ctypes.c_double), ("ms_total", ctypes.c_double), ("method", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
-        return {f: getattr(self, f) for f, _ in self._fields_}
+        d = {f: getattr(self, f) for f, _ in self._fields_}
+        d["method"] = {PSQ_PCA_GRAM: "gram", PSQ_PCA_RANDOMIZED: "randomized"}.get(d["method"], d["method"])
+        return d
 
 
 class EncoderTrainOpts(ctypes.Structure):
@@ -254,6 +261,10 @@ def _declare(L: ctypes.CDLL) -> None:
     L.psq_pca_csr.argtypes = [P, P, P, U64, U64, U64, I32, I32, D, P, P, P, I32]
     L.psq_pca_csr_device.restype = ctypes.c_int
     L.psq_pca_csr_device.argtypes = [P, P, P, U64, U64, U64, I32, I32, D, P, P, P, P]
+    L.psq_pca_csr_ex.restype = ctypes.c_int
+    L.psq_pca_csr_ex.argtypes = [P, P, P, U64, U64, U64, I32, I32, D, I32, U64, U64, U64, P, P, P, I32]
+    L.psq_pca_csr_device_ex.restype = ctypes.c_int
+    L.psq_pca_csr_device_ex.argtypes = [P, P, P, U64, U64, U64, I32, I32, D, I32, U64, U64, U64, P, P, P, P]
     L.psq_pca_last_report.restype = ctypes.c_int
     L.psq_pca_last_report.argtypes = [ctypes.POINTER(PcaReport)]
     L.psq_encoder_mine_ms.restype = ctypes.c_int

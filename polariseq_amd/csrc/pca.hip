@@ -232,6 +232,14 @@ __global__ void k_pca_scale_rows(double* __restrict__ H, const double* __restric
   if (t < count) H[t] *= lam[t % nl];
 }
 
+// Columns c of Q (m x b, column-major) with dead[c] set to zero. Grid (x, b).
+__global__ void k_pca_zero_dead(double* __restrict__ Q, uint32_t m, const uint32_t* __restrict__ dead) {
+  const uint32_t c = blockIdx.y;
+  if (!dead[c]) return;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
+    Q[(size_t)c * m + i] = 0.0;
+}
+
 // Z += sigma Q (the shifted operator), elementwise.
 __global__ void k_pca_shift(double* __restrict__ Z, const double* __restrict__ Q, uint64_t count,
                             const double* __restrict__ tv) {
@@ -243,12 +251,17 @@ __global__ void k_pca_shift(double* __restrict__ Z, const double* __restrict__ Q
 // upper triangle read) plus shift I, in place; the strict lower triangle is
 // zeroed. shifted: shift = 11 (m b + b (b + 1)) u trace(G) (the shifted
 // CholeskyQR bound, u = 2^-53). Pivots are floored at 1e-28 of the largest
-// diagonal (never reached for a full-rank block). With b <= PCA_LDS_BLOCK the
+// diagonal (never reached for a full-rank block). Unshifted, a pivot at or
+// below 16 b u of the largest diagonal marks its column dependent on the
+// earlier ones (below what the Gram resolves): dead[k] = 1, row k of R
+// becomes the unit row, so nothing is divided by it, and the caller zeroes
+// the column; the others get dead[k] = 0. With b <= PCA_LDS_BLOCK the
 // factorisation runs in LDS.
 __global__ __launch_bounds__(256) void k_pca_chol(double* __restrict__ G, uint32_t b, uint32_t m,
-                                                  int shifted) {
+                                                  int shifted, uint32_t* __restrict__ dead) {
   extern __shared__ double lds[];
-  __shared__ double s_piv, s_shift, s_floor;
+  __shared__ double s_piv, s_shift, s_floor, s_tol;
+  __shared__ int s_dead;
   const uint32_t tid = threadIdx.x;
   const bool in_lds = b <= PCA_LDS_BLOCK;
   double* A = in_lds ? lds : G;
@@ -263,18 +276,27 @@ __global__ __launch_bounds__(256) void k_pca_chol(double* __restrict__ G, uint32
     }
     s_shift = shifted ? 11.0 * ((double)m * b + (double)b * (b + 1)) * 0x1.0p-53 * tr : 0.0;
     s_floor = fmax(mx * 1e-28, 1e-290);
+    s_tol = shifted ? -1.0 : 16.0 * (double)b * 0x1.0p-53 * mx;
   }
   __syncthreads();
-  const double shift = s_shift, floor_ = s_floor;
+  const double shift = s_shift, floor_ = s_floor, tol = s_tol;
   for (uint32_t k = 0; k < b; ++k) {
     if (tid == 0) {
       double d = A[(size_t)k * b + k] + shift;
+      const int dk = d <= tol;
       d = d > floor_ ? d : floor_;
-      const double r = sqrt(d);
+      const double r = dk ? 1.0 : sqrt(d);
       A[(size_t)k * b + k] = r;
       s_piv = r;
+      s_dead = dk;
+      dead[k] = (uint32_t)dk;
     }
     __syncthreads();
+    if (s_dead) {  // uniform: the unit row, no update of the rest
+      for (uint32_t j = k + 1 + tid; j < b; j += blockDim.x) A[(size_t)j * b + k] = 0.0;
+      __syncthreads();
+      continue;
+    }
     const double r = s_piv;
     for (uint32_t j = k + 1 + tid; j < b; j += blockDim.x) A[(size_t)j * b + k] /= r;  // R[k][j]
     __syncthreads();
@@ -559,6 +581,92 @@ __global__ void k_pca_ratio(const double* __restrict__ lam, const double* __rest
   if (j < k) ratio[j] = (float)(fmax(lam[j] - tv[1], 0.0) / tv[0]);
 }
 
+// ── randomized block Krylov route ──
+
+// Row m of each column c of X (column-major, leading dimension ld) becomes
+// -(mu^T X[0, m)[c]): a dense chunk's ones column then subtracts 1 mu^T X from
+// its product, the implicit centring (rsvd.rs:252-262). One block per column,
+// fixed-order tree sum.
+__global__ __launch_bounds__(256) void k_rsvd_mu_row(double* __restrict__ X, uint32_t ld, uint32_t m,
+                                                     const float* __restrict__ mu) {
+  __shared__ double part[256];
+  double* col = X + (size_t)blockIdx.x * ld;
+  double a = 0.0;
+  for (uint32_t j = threadIdx.x; j < m; j += blockDim.x) a += (double)mu[j] * col[j];
+  part[threadIdx.x] = a;
+  __syncthreads();
+  for (uint32_t o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) col[m] = -part[0];
+}
+
+// Row m of Z (the chunks' ones column times Y: the column sums 1^T Y) moved to
+// zs and cleared, for each of the cols columns.
+__global__ void k_rsvd_take_row(double* __restrict__ Z, uint32_t ld, uint32_t m, uint32_t cols,
+                                double* __restrict__ zs) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cols) return;
+  zs[c] = Z[(size_t)c * ld + m];
+  Z[(size_t)c * ld + m] = 0.0;
+}
+
+// Z[j][c] -= mu_j zs[c] for j < m: S^T Y - mu (1^T Y) (rsvd.rs:325-335).
+__global__ void k_rsvd_center_t(double* __restrict__ Z, uint32_t ld, uint32_t m, uint32_t cols,
+                                const float* __restrict__ mu, const double* __restrict__ zs) {
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x, count = (uint64_t)m * cols;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += step) {
+    const uint32_t c = (uint32_t)(t / m), j = (uint32_t)(t - (uint64_t)c * m);
+    Z[(size_t)c * ld + j] -= (double)mu[j] * zs[c];
+  }
+}
+
+// Sums of v^2 (f64) over the stored values, per block: block g takes the
+// elements g 256 + t + i G 256 in order, then a fixed-order tree sum.
+__global__ __launch_bounds__(256) void k_rsvd_sumsq(const float* __restrict__ v, uint64_t nnz,
+                                                    double* __restrict__ part) {
+  __shared__ double sp[256];
+  double a = 0.0;
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nnz; t += step) {
+    const double x = (double)v[t];
+    a += x * x;
+  }
+  sp[threadIdx.x] = a;
+  __syncthreads();
+  for (uint32_t o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sp[threadIdx.x] += sp[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sp[0];
+}
+
+// Singular values s_j = sqrt(max(lam_j, 0)) of the sketch (lam: eigenvalues of
+// B B^T, descending), their reciprocals (0 for s_j = 0) and the variance ratio
+// s_j^2 / total (pca.rs:149-166).
+__global__ void k_rsvd_values(const double* __restrict__ lam, uint32_t k, double total, double* __restrict__ sv,
+                              double* __restrict__ inv, float* __restrict__ ratio) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= k) return;
+  const double s = sqrt(fmax(lam[j], 0.0));
+  sv[j] = s;
+  inv[j] = s > 0.0 ? 1.0 / s : 0.0;
+  ratio[j] = (float)(s * s / total);
+}
+
+// out[i][j] = f32(U[i][j] sc[j]): U column-major (leading dimension ldu), out
+// row-major rows x k.
+__global__ void k_rsvd_out(const double* __restrict__ U, uint64_t ldu, uint64_t rows, uint32_t k,
+                           const double* __restrict__ sc, float* __restrict__ out) {
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x, count = rows * k;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += step) {
+    const uint64_t i = t / k;
+    const uint32_t j = (uint32_t)(t - i * k);
+    out[t] = (float)(U[(size_t)j * ldu + i] * sc[j]);
+  }
+}
+
 }  // namespace psq
 
 using namespace psq;
@@ -633,22 +741,339 @@ int blas_handle(hipStream_t s, rocblas_handle* out) {
 size_t small_lds(uint32_t b) { return b <= PCA_LDS_BLOCK ? (size_t)b * b * sizeof(double) : 0; }
 
 // Q (m x b, column-major) <- an orthonormal basis of its columns: shifted
-// CholeskyQR, then CholeskyQR twice (G = Q^T Q, R = chol(G), Q <- Q R^-1).
-int orthonormalise(rocblas_handle h, double* Q, uint32_t m, uint32_t b, double* G, hipStream_t s) {
+// CholeskyQR, then CholeskyQR twice (G = Q^T Q, R = chol(G), Q <- Q R^-1). The
+// first j columns keep spanning what the first j input columns span; a column
+// dependent on the earlier ones comes out zero (dead: b entries of scratch).
+int orthonormalise(rocblas_handle h, double* Q, uint32_t m, uint32_t b, double* G, uint32_t* dead,
+                   hipStream_t s) {
   const double one = 1.0, zero = 0.0;
   for (int pass = 0; pass < 3; ++pass) {
     PBLAS(rocblas_dsyrk(h, rocblas_fill_upper, rocblas_operation_transpose, b, m, &one, Q, m, &zero, G, b));
-    hipLaunchKernelGGL(k_pca_chol, dim3(1), dim3(256), small_lds(b), s, G, b, m, pass == 0 ? 1 : 0);
+    hipLaunchKernelGGL(k_pca_chol, dim3(1), dim3(256), small_lds(b), s, G, b, m, pass == 0 ? 1 : 0, dead);
     PHIPC(hipGetLastError());
     PBLAS(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
                         rocblas_diagonal_non_unit, m, b, &one, G, b, Q, m));
+    if (pass > 0) {
+      hipLaunchKernelGGL(k_pca_zero_dead, dim3(std::min(blocks(m, 256), 64u), b), dim3(256), 0, s, Q, m, dead);
+      PHIPC(hipGetLastError());
+    }
   }
   return PSQ_OK;
 }
 
+// ── the Gaussian sketch (rsvd.rs:414-435) ──
+// ChaCha with 8 rounds (Bernstein 2008; rand_chacha's ChaCha8Rng: 64-bit block
+// counter in words 12-13 from 0, stream 0 in words 14-15, blocks consumed in
+// counter order, a u64 as two consecutive words, low first), keyed by
+// rand_core's seed_from_u64 (eight PCG32 outputs, little-endian: state <-
+// state 6364136223846793005 + 11634580027462260723, then the XSH-RR output);
+// a uniform is the u64's top 53 bits times 2^-53, and each Box-Muller pair
+// (u1 floored at 1e-300) gives r cos(2 pi u2), r sin(2 pi u2) in f64, rounded
+// to f32, filling the rows x cols matrix row by row.
+struct ChaCha {
+  uint32_t key[8];
+  int rounds;
+  uint64_t counter = 0;
+  uint32_t buf[16];
+  int pos = 16;
+  static uint32_t rotl(uint32_t v, int r) { return (v << r) | (v >> (32 - r)); }
+  static void quarter(uint32_t* x, int a, int b, int c, int d) {
+    x[a] += x[b]; x[d] = rotl(x[d] ^ x[a], 16);
+    x[c] += x[d]; x[b] = rotl(x[b] ^ x[c], 12);
+    x[a] += x[b]; x[d] = rotl(x[d] ^ x[a], 8);
+    x[c] += x[d]; x[b] = rotl(x[b] ^ x[c], 7);
+  }
+  void block() {
+    const uint32_t in[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u, key[0], key[1], key[2],
+                             key[3], key[4], key[5], key[6], key[7], (uint32_t)counter,
+                             (uint32_t)(counter >> 32), 0u, 0u};
+    uint32_t x[16];
+    for (int i = 0; i < 16; ++i) x[i] = in[i];
+    for (int r = 0; r < rounds; r += 2) {
+      quarter(x, 0, 4, 8, 12);
+      quarter(x, 1, 5, 9, 13);
+      quarter(x, 2, 6, 10, 14);
+      quarter(x, 3, 7, 11, 15);
+      quarter(x, 0, 5, 10, 15);
+      quarter(x, 1, 6, 11, 12);
+      quarter(x, 2, 7, 8, 13);
+      quarter(x, 3, 4, 9, 14);
+    }
+    for (int i = 0; i < 16; ++i) buf[i] = x[i] + in[i];
+    ++counter;
+    pos = 0;
+  }
+  uint64_t next_u64() {
+    if (pos >= 16) block();
+    const uint64_t lo = buf[pos], hi = buf[pos + 1];
+    pos += 2;
+    return lo | (hi << 32);
+  }
+};
+
+std::vector<float> gaussian_rowmajor(uint64_t rows, uint64_t cols, uint64_t seed) {
+  ChaCha g;
+  g.rounds = 8;
+  uint64_t st = seed;
+  for (int i = 0; i < 8; ++i) {
+    st = st * 6364136223846793005ull + 11634580027462260723ull;
+    const uint32_t xs = (uint32_t)(((st >> 18) ^ st) >> 27), rot = (uint32_t)(st >> 59);
+    g.key[i] = (xs >> rot) | (xs << ((32 - rot) & 31));
+  }
+  const uint64_t count = rows * cols;
+  std::vector<float> out(count);
+  for (uint64_t i = 0; i < count; i += 2) {
+    const double u1 = std::max((double)(g.next_u64() >> 11) * 0x1.0p-53, 1e-300);
+    const double u2 = (double)(g.next_u64() >> 11) * 0x1.0p-53;
+    const double r = std::sqrt(-2.0 * std::log(u1)), th = 2.0 * M_PI * u2;
+    out[i] = (float)(r * std::cos(th));
+    if (i + 1 < count) out[i + 1] = (float)(r * std::sin(th));
+  }
+  return out;
+}
+
+// Dense f64 row chunks of the CSR matrix (rows of ld >= m + 1 elements, the
+// ones column at m, the rest of the padding zero), one chunk buffer reused.
+struct Chunks {
+  const uint64_t* ip;
+  const uint32_t* ix;
+  uint64_t n;
+  uint32_t m, ld, R;
+  double* D;
+  hipStream_t s;
+  int densify(uint64_t r0, uint32_t rc, const float* vals) const {
+    PHIPC(hipMemsetAsync(D, 0, (size_t)rc * ld * sizeof(double), s));
+    hipLaunchKernelGGL(k_pca_densify, dim3(blocks(rc, 4)), dim3(256), 0, s, ip, ix, vals, r0, rc, m, ld, D);
+    PHIPC(hipGetLastError());
+    return PSQ_OK;
+  }
+};
+
+// Column sums and sums of squares (f64) of the stored values, over the dense
+// chunks, per-segment partials folded in order.
+int column_sums(const Chunks& ch, const float* vals, Dev& dv, double* acc_s, double* acc_q) {
+  const hipStream_t s = ch.s;
+  const uint32_t P = std::min<uint32_t>(PCA_SEGMENTS, ch.R), m = ch.m;
+  double *ps_s, *ps_q;
+  PTRY(dv.alloc((size_t)P * m, &ps_s));
+  PTRY(dv.alloc((size_t)P * m, &ps_q));
+  PHIPC(hipMemsetAsync(acc_s, 0, m * sizeof(double), s));
+  PHIPC(hipMemsetAsync(acc_q, 0, m * sizeof(double), s));
+  for (uint64_t r0 = 0; r0 < ch.n; r0 += ch.R) {
+    const uint32_t rc = (uint32_t)std::min<uint64_t>(ch.R, ch.n - r0);
+    PTRY(ch.densify(r0, rc, vals));
+    const uint32_t Pc = std::min<uint32_t>(P, rc), seg = (rc + Pc - 1) / Pc;
+    hipLaunchKernelGGL(k_pca_colsums_part, dim3(blocks(m, 256), Pc), dim3(256), 0, s, ch.D, rc, m, ch.ld, seg,
+                       ps_s, ps_q);
+    hipLaunchKernelGGL(k_pca_colsums_fold, dim3(blocks(m, 256)), dim3(256), 0, s, ps_s, ps_q, Pc, m, acc_s, acc_q);
+    PHIPC(hipGetLastError());
+  }
+  return PSQ_OK;
+}
+
+// Basis width of the randomized route: l = min(k + oversample, min(n, m)) and
+// l (n_iter + 1) columns, at most min(n, m) (rsvd.rs:496-499), saturating.
+void rsvd_widths(uint64_t n, uint64_t m, uint64_t k, uint64_t oversample, uint64_t n_iter, uint64_t* l_out,
+                 uint64_t* width_out) {
+  const uint64_t most = std::min(n, m);
+  const uint64_t l = std::max<uint64_t>(1, oversample >= most ? most : std::min(k + oversample, most));
+  const uint64_t blocks_ = n_iter >= most ? most : n_iter + 1;
+  *l_out = l;
+  *width_out = blocks_ > most / l ? most : std::min(l * blocks_, most);
+}
+
+// Randomized block Krylov SVD of S - 1 mu^T (rsvd.rs:476-566): Q_0 = qr(A' Omega),
+// Q_d = qr(A' qr(A'^T Q_{d-1})) for d = 1 .. n_iter while d l < width (a last
+// block that does not fit is cut to the width), Q = qr([Q_0 .. Q_q]),
+// B^T = A'^T Q; with B B^T = W diag(s^2) W^T, U = Q W[:, :k], embedding = U s,
+// columns signed as in the Gram route, ratio = s^2 / ||A'||_F^2, components =
+// B^T W[:, :k] / s. Every product with A' is a pass over the dense f64 row
+// chunks (DGEMM with the chunk and its ones column, whose row carries the
+// centring); the QR factorisations are shifted CholeskyQR3 and the small
+// eigenproblem the cyclic Jacobi of the Gram route, so a run repeats bit for bit.
+// A rank-deficient block keeps its dependent columns at zero where Householder
+// QR would complete it with unit vectors orthogonal to the matrix's range:
+// either way they add nothing to B, and the components of s > 0 are the same.
+int rsvd_route(rocblas_handle h, Dev& dv, Events& evs, psq_pca_report& rep, const Chunks& ch, const float* S,
+               uint64_t nnz, uint32_t k, int32_t center, uint64_t oversample, uint64_t n_iter, uint64_t seed, float* d_emb,
+               float* d_ratio, float* d_comp, size_t ev_start, size_t ev_scaled) {
+  const hipStream_t s = ch.s;
+  const uint64_t n = ch.n;
+  const uint32_t m = ch.m, ld = ch.ld;
+  const double one = 1.0, zero = 0.0;
+  uint64_t l64, w64;
+  rsvd_widths(n, m, k, oversample, n_iter, &l64, &w64);
+  const uint32_t l = (uint32_t)l64, width = (uint32_t)w64;
+
+  // 1. column means (f32, rsvd.rs:155-177) and ||A'||_F^2 = sum over stored
+  //    values of v^2 - n sum mu^2 (rsvd.rs:186-208), the last sums on the host
+  //    in order
+  double *acc_s, *acc_q, *part;
+  PTRY(dv.alloc(m, &acc_s));
+  PTRY(dv.alloc(m, &acc_q));
+  PTRY(column_sums(ch, S, dv, acc_s, acc_q));
+  ++rep.iterations;
+  const uint32_t nnz_blocks = std::min<uint32_t>(1024, blocks(nnz, 256));
+  PTRY(dv.alloc(nnz_blocks, &part));
+  hipLaunchKernelGGL(k_rsvd_sumsq, dim3(nnz_blocks), dim3(256), 0, s, S, nnz, part);
+  PHIPC(hipGetLastError());
+  float* mu = nullptr;
+  std::vector<float> hmu;
+  if (center) {
+    PTRY(dv.alloc(m, &mu));
+    hipLaunchKernelGGL(k_pca_means, dim3(blocks(m, 256)), dim3(256), 0, s, acc_s, m, n, mu);
+    PHIPC(hipGetLastError());
+    hmu.resize(m);
+    PHIPC(hipMemcpyAsync(hmu.data(), mu, m * sizeof(float), hipMemcpyDeviceToHost, s));
+  }
+  std::vector<double> hq(nnz_blocks);
+  PHIPC(hipMemcpyAsync(hq.data(), part, nnz_blocks * sizeof(double), hipMemcpyDeviceToHost, s));
+  PHIPC(hipStreamSynchronize(s));
+  double total = 0.0, mu2 = 0.0;
+  for (double v : hq) total += v;
+  for (float v : hmu) mu2 += (double)v * (double)v;
+  total -= (double)n * mu2;
+  if (!std::isfinite(total)) return fail(PSQ_E_INVALID, "the matrix holds non-finite values");
+
+  // 2. the Krylov basis
+  double *X, *Y, *K, *Z, *G, *zs;
+  PTRY(dv.alloc((size_t)ld * l, &X));
+  PTRY(dv.alloc((size_t)n * l, &Y));
+  PTRY(dv.alloc((size_t)n * width, &K));
+  PTRY(dv.alloc((size_t)ld * width, &Z));
+  uint32_t* dead;
+  PTRY(dv.alloc((size_t)width * width, &G));
+  PTRY(dv.alloc(width, &zs));
+  PTRY(dv.alloc(width, &dead));
+  {
+    const std::vector<float> om = gaussian_rowmajor(m, l, seed);
+    std::vector<double> hx((size_t)ld * l, 0.0);
+    for (uint32_t j = 0; j < m; ++j)
+      for (uint32_t c = 0; c < l; ++c) hx[(size_t)c * ld + j] = (double)om[(size_t)j * l + c];
+    PHIPC(hipMemcpyAsync(X, hx.data(), hx.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    PHIPC(hipStreamSynchronize(s));
+  }
+  std::vector<std::pair<size_t, size_t>> ev_gemm;
+  // Yo (n x cols, leading dimension n) = A' Xi (Xi: ld x cols, row m = -mu^T Xi)
+  auto forward = [&](double* Xi, uint32_t cols, double* Yo) -> int {
+    if (mu) {
+      hipLaunchKernelGGL(k_rsvd_mu_row, dim3(cols), dim3(256), 0, s, Xi, ld, m, mu);
+      PHIPC(hipGetLastError());
+    }
+    for (uint64_t r0 = 0; r0 < n; r0 += ch.R) {
+      const uint32_t rc = (uint32_t)std::min<uint64_t>(ch.R, n - r0);
+      PTRY(ch.densify(r0, rc, S));
+      size_t a, b;
+      PTRY(evs.record(s, &a));
+      PBLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, rc, cols, ld, &one, ch.D, ld, Xi,
+                          ld, &zero, Yo + r0, n));
+      PTRY(evs.record(s, &b));
+      ev_gemm.emplace_back(a, b);
+    }
+    ++rep.iterations;
+    return PSQ_OK;
+  };
+  // Zo (ld x cols; rows m .. ld - 1 zero) = A'^T Yi (Yi: n x cols)
+  auto backward = [&](const double* Yi, uint32_t cols, double* Zo) -> int {
+    for (uint64_t r0 = 0; r0 < n; r0 += ch.R) {
+      const uint32_t rc = (uint32_t)std::min<uint64_t>(ch.R, n - r0);
+      PTRY(ch.densify(r0, rc, S));
+      size_t a, b;
+      PTRY(evs.record(s, &a));
+      PBLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, ld, cols, rc, &one, ch.D, ld, Yi + r0,
+                          n, r0 == 0 ? &zero : &one, Zo, ld));
+      PTRY(evs.record(s, &b));
+      ev_gemm.emplace_back(a, b);
+    }
+    hipLaunchKernelGGL(k_rsvd_take_row, dim3(blocks(cols, 64)), dim3(64), 0, s, Zo, ld, m, cols, zs);
+    if (mu)
+      hipLaunchKernelGGL(k_rsvd_center_t, dim3(stride_blocks((uint64_t)m * cols, 256)), dim3(256), 0, s, Zo, ld, m,
+                         cols, mu, zs);
+    PHIPC(hipGetLastError());
+    ++rep.iterations;
+    return PSQ_OK;
+  };
+  PTRY(forward(X, l, Y));
+  PTRY(orthonormalise(h, Y, (uint32_t)n, l, G, dead, s));
+  PHIPC(hipMemcpyAsync(K, Y, (size_t)n * l * sizeof(double), hipMemcpyDeviceToDevice, s));
+  for (uint64_t d = 1; d <= n_iter && d * l < width; ++d) {
+    PTRY(backward(Y, l, Z));
+    PTRY(orthonormalise(h, Z, ld, l, G, dead, s));
+    PTRY(forward(Z, l, Y));
+    PTRY(orthonormalise(h, Y, (uint32_t)n, l, G, dead, s));
+    const uint64_t cnt = std::min<uint64_t>(l, width - d * l);
+    PHIPC(hipMemcpyAsync(K + d * l * n, Y, (size_t)n * cnt * sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
+  PTRY(orthonormalise(h, K, (uint32_t)n, width, G, dead, s));
+  PTRY(backward(K, width, Z));  // B^T
+  size_t ev_basis;
+  PTRY(evs.record(s, &ev_basis));
+
+  // 3. B B^T = W diag(s^2) W^T (width x width), descending
+  double *Tb, *Wb, *Ws, *lam, *cs, *sv, *inv;
+  PTRY(dv.alloc((size_t)width * width, &Tb));
+  PTRY(dv.alloc((size_t)width * width, &Wb));
+  PTRY(dv.alloc((size_t)width * width, &Ws));
+  PTRY(dv.alloc(width, &lam));
+  PTRY(dv.alloc((size_t)4 * (width / 2 + 1), &cs));
+  PTRY(dv.alloc(k, &sv));
+  PTRY(dv.alloc(k, &inv));
+  PBLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, width, width, ld, &one, Z, ld, Z, ld,
+                      &zero, Tb, width));
+  hipLaunchKernelGGL(k_pca_jacobi, dim3(1), dim3(256), small_lds(width), s, Tb, width, Wb, Ws, lam, cs);
+  PHIPC(hipGetLastError());
+  size_t ev_eigen;
+  PTRY(evs.record(s, &ev_eigen));
+
+  // 4. U = Q W[:, :k] (into Y: l >= k columns), the outputs, signs
+  PBLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, k, width, &one, K,
+                      (rocblas_int)n, Ws, width, &zero, Y, (rocblas_int)n));
+  hipLaunchKernelGGL(k_rsvd_values, dim3(blocks(k, 64)), dim3(64), 0, s, lam, k, total, sv, inv, d_ratio);
+  hipLaunchKernelGGL(k_rsvd_out, dim3(stride_blocks(n * k, 256)), dim3(256), 0, s, Y, n, n, k, sv, d_emb);
+  PHIPC(hipGetLastError());
+  float* ppv;
+  uint64_t* ppi;
+  uint32_t* flip;
+  const uint32_t rps = std::max<uint32_t>(1, 256 / k);
+  const uint32_t Gs = (uint32_t)std::min<uint64_t>(2048, (n + 255) / 256);
+  const uint64_t seg = (n + Gs - 1) / Gs;
+  PTRY(dv.alloc((size_t)Gs * k, &ppv));
+  PTRY(dv.alloc((size_t)Gs * k, &ppi));
+  PTRY(dv.alloc(k, &flip));
+  hipLaunchKernelGGL(k_pca_sign_part, dim3(Gs), dim3(k * rps), 0, s, d_emb, n, k, rps, seg, ppv, ppi);
+  hipLaunchKernelGGL(k_pca_sign_pick, dim3(blocks(k, 64)), dim3(64), 0, s, d_emb, k, Gs, ppv, ppi, flip);
+  hipLaunchKernelGGL(k_pca_sign_apply, dim3(stride_blocks(n * k, 256)), dim3(256), 0, s, d_emb, n * k, k, flip);
+  PHIPC(hipGetLastError());
+  if (d_comp) {
+    // V = B^T W[:, :k] / s (into X: ld x l >= ld x k), with the embedding's signs
+    PBLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, ld, k, width, &one, Z, ld, Ws, width,
+                        &zero, X, ld));
+    hipLaunchKernelGGL(k_rsvd_out, dim3(stride_blocks((uint64_t)m * k, 256)), dim3(256), 0, s, X, ld, m, k, inv,
+                       d_comp);
+    hipLaunchKernelGGL(k_pca_sign_apply, dim3(stride_blocks((uint64_t)m * k, 256)), dim3(256), 0, s, d_comp,
+                       (uint64_t)m * k, k, flip);
+    PHIPC(hipGetLastError());
+  }
+  size_t ev_end;
+  PTRY(evs.record(s, &ev_end));
+  PHIPC(hipStreamSynchronize(s));  // the buffers freed on return are no longer in use
+  rep.method = PSQ_PCA_RANDOMIZED;
+  rep.block = width;
+  rep.rows_per_chunk = ch.R;
+  rep.chunks = (n + ch.R - 1) / ch.R;
+  rep.ms_scale = evs.ms(ev_start, ev_scaled);
+  rep.ms_gram = evs.ms(ev_scaled, ev_basis);
+  for (const auto& p : ev_gemm) rep.ms_gemm += evs.ms(p.first, p.second);
+  rep.ms_eigen = evs.ms(ev_basis, ev_eigen);
+  rep.ms_project = evs.ms(ev_eigen, ev_end);
+  rep.ms_total = evs.ms(ev_start, ev_end);
+  g_pca_report = rep;
+  return PSQ_OK;
+}
+
 int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t n, uint64_t m64, uint64_t k64,
-               int32_t center, int32_t scale, double max_value, float* d_emb, float* d_ratio, float* d_comp,
-               hipStream_t s) {
+               int32_t center, int32_t scale, double max_value, int32_t method, uint64_t oversample,
+               uint64_t n_iter, uint64_t seed, float* d_emb, float* d_ratio, float* d_comp, hipStream_t s) {
   const uint32_t m = (uint32_t)m64, kk = (uint32_t)k64;
   const double clip = max_value;  // +inf: no clipping
   if (scale) center = 1;          // pp.scale always centres (pca.rs:416-419)
@@ -685,27 +1110,24 @@ int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t 
 
   // dense f64 row chunks of at least 2^25 elements (256 MB) and 4096 rows, rows
   // of ld = m + 1 (the ones column) rounded up to 8
+  // (2^27 elements, 1 GiB, on the randomized route: its passes multiply a chunk
+  // by a few columns, so fewer, taller chunks keep the DGEMMs busy)
   const uint32_t ld = (m + 1 + 7) & ~7u;
+  const uint64_t chunk_elems = method == PSQ_PCA_RANDOMIZED ? (1ull << 27) : (1ull << 25);
   const uint32_t R = (uint32_t)std::max<uint64_t>(
-      1, std::min<uint64_t>(n, std::max<uint64_t>(4096, (1ull << 25) / ld)));
+      1, std::min<uint64_t>(n, std::max<uint64_t>(4096, chunk_elems / ld)));
   const uint32_t P = std::min<uint32_t>(PCA_SEGMENTS, R);
-  double *D, *acc_s, *acc_q = nullptr, *ps_s = nullptr, *ps_q = nullptr, *C, *Gb;
+  double *D, *acc_s = nullptr, *acc_q = nullptr, *ps_s = nullptr, *ps_q = nullptr, *C, *Gb;
   PTRY(dv.alloc((size_t)R * ld, &D));
-  PTRY(dv.alloc(m, &acc_s));
-  PTRY(dv.alloc((size_t)m * m, &C));
-  PTRY(dv.alloc((size_t)ld * ld, &Gb));
+  const Chunks ch{ip, ix, n, m, ld, R, D, s};
   const double one = 1.0, zero = 0.0;
-  auto densify = [&](uint64_t r0, uint32_t rc, const float* vals) -> int {
-    PHIPC(hipMemsetAsync(D, 0, (size_t)rc * ld * sizeof(double), s));
-    hipLaunchKernelGGL(k_pca_densify, dim3(blocks(rc, 4)), dim3(256), 0, s, ip, ix, vals, r0, rc, m, ld, D);
-    PHIPC(hipGetLastError());
-    return PSQ_OK;
-  };
+  auto densify = [&](uint64_t r0, uint32_t rc, const float* vals) -> int { return ch.densify(r0, rc, vals); };
 
   const float* S = x;  // the stored values the Gram and projection read
   if (scale) {
     double *mean, *inv, *zr;
     float* sv;
+    PTRY(dv.alloc(m, &acc_s));
     PTRY(dv.alloc(m, &acc_q));
     PTRY(dv.alloc((size_t)P * m, &ps_s));
     PTRY(dv.alloc((size_t)P * m, &ps_q));
@@ -734,8 +1156,14 @@ int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t 
     S = sv;
   }
   PTRY(evs.record(s, &ev_scaled));
+  if (method == PSQ_PCA_RANDOMIZED)
+    return rsvd_route(h, dv, evs, rep, ch, S, nnz, kk, center, oversample, n_iter, seed, d_emb, d_ratio, d_comp,
+                      ev_start, ev_scaled);
 
   // Gram of the stored values; the ones column's entries hold the column sums
+  if (!acc_s) PTRY(dv.alloc(m, &acc_s));
+  PTRY(dv.alloc((size_t)m * m, &C));
+  PTRY(dv.alloc((size_t)ld * ld, &Gb));
   PHIPC(hipMemsetAsync(Gb, 0, (size_t)ld * ld * sizeof(double), s));
   for (uint64_t r0 = 0; r0 < n; r0 += R) {
     const uint32_t rc = (uint32_t)std::min<uint64_t>(R, n - r0);
@@ -784,6 +1212,8 @@ int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t 
   PTRY(dv.alloc(b, &lam));
   PTRY(dv.alloc((size_t)4 * (b / 2 + 1), &cs));
   PTRY(dv.alloc(b, &res));
+  uint32_t* dead;
+  PTRY(dv.alloc(b, &dead));
   if (htv[0] <= 0.0) {
     // C = 0 (a PSD matrix with zero trace): no direction is preferred
     hipLaunchKernelGGL(k_pca_unit_block, dim3(blocks((uint64_t)m * kk, 256)), dim3(256), 0, s, Q, lam, m, kk);
@@ -791,7 +1221,7 @@ int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t 
   } else {
     hipLaunchKernelGGL(k_pca_start, dim3(blocks((uint64_t)m * b, 256)), dim3(256), 0, s, Q, m, b);
     PHIPC(hipGetLastError());
-    PTRY(orthonormalise(h, Q, m, b, G, s));
+    PTRY(orthonormalise(h, Q, m, b, G, dead, s));
     double *Y = nullptr, *H = nullptr;  // the filter's fourth block; lead x b products
     PTRY(dv.alloc((size_t)m * b, &Y));
     PTRY(dv.alloc((size_t)b * b, &H));
@@ -890,7 +1320,7 @@ int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t 
       Z = rest[0];
       Tmp = rest[1];
       Y = rest[2];
-      PTRY(orthonormalise(h, Q, m, b, G, s));
+      PTRY(orthonormalise(h, Q, m, b, G, dead, s));
     }
     if (!converged)
       return fail(PSQ_E_DEVICE, "the eigensolver did not converge in %u products", PCA_MAX_PRODUCTS);
@@ -933,11 +1363,15 @@ int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t 
   rep.ms_eigen = evs.ms(ev_gram, ev_eigen);
   rep.ms_project = evs.ms(ev_eigen, ev_end);
   rep.ms_total = evs.ms(ev_start, ev_end);
+  rep.method = PSQ_PCA_GRAM;
   g_pca_report = rep;
   return PSQ_OK;
 }
 
-int pca_validate(uint64_t n, uint64_t m, uint64_t k, int32_t scale, double max_value) {
+// Resolves PSQ_PCA_AUTO in *method: the Gram route up to PCA_MAX_FEATURES genes,
+// the randomized route above.
+int pca_validate(uint64_t n, uint64_t m, uint64_t k, int32_t scale, double max_value, int32_t* method,
+                 uint64_t oversample, uint64_t n_iter) {
   if (n == 0 || m == 0)
     return fail(PSQ_E_INVALID, "input matrix has zero rows (%llu) or columns (%llu)", (unsigned long long)n,
                 (unsigned long long)m);
@@ -947,30 +1381,48 @@ int pca_validate(uint64_t n, uint64_t m, uint64_t k, int32_t scale, double max_v
                 (unsigned long long)k, (unsigned long long)most);
   if (k == 0) return fail(PSQ_E_INVALID, "n_components must be >= 1");
   if (scale && !(max_value >= 0.0)) return fail(PSQ_E_INVALID, "max_value must be >= 0 (or +inf for none)");
-  if (m > PCA_MAX_FEATURES)
-    return fail(PSQ_E_UNSUPPORTED, "n_features (%llu) > %u is not supported (exact Gram route only)",
-                (unsigned long long)m, PCA_MAX_FEATURES);
   if (n >= (1ull << 40) / k) return fail(PSQ_E_UNSUPPORTED, "n_obs * n_components must be < 2^40");
+  if (*method != PSQ_PCA_AUTO && *method != PSQ_PCA_GRAM && *method != PSQ_PCA_RANDOMIZED)
+    return fail(PSQ_E_INVALID, "method must be 0 (auto), 1 (Gram) or 2 (randomized), got %d", *method);
+  if (*method == PSQ_PCA_AUTO) *method = m > PCA_MAX_FEATURES ? PSQ_PCA_RANDOMIZED : PSQ_PCA_GRAM;
+  if (*method == PSQ_PCA_GRAM) {
+    if (m > PCA_MAX_FEATURES)
+      return fail(PSQ_E_UNSUPPORTED, "n_features (%llu) > %u is not supported by the Gram route",
+                  (unsigned long long)m, PCA_MAX_FEATURES);
+    return PSQ_OK;
+  }
+  // rocBLAS dimensions are 32-bit: n_obs rows and ld = n_features + 1 rounded to 8
+  if (n > 0x7FFFFFFFull || m > 0x7FFFFFF0ull)
+    return fail(PSQ_E_UNSUPPORTED, "the randomized route takes at most 2^31 - 1 cells and 2^31 - 16 genes");
+  uint64_t l, width;
+  rsvd_widths(n, m, k, oversample, n_iter, &l, &width);
+  if (width > PCA_MAX_BLOCK)
+    return fail(PSQ_E_UNSUPPORTED,
+                "the Krylov basis of %llu columns ((n_components + oversample) (n_iter + 1)) exceeds %u",
+                (unsigned long long)width, PCA_MAX_BLOCK);
   return PSQ_OK;
 }
 
 }  // namespace
 
-extern "C" int psq_pca_csr_device(const uint64_t* d_indptr, const uint32_t* d_indices, const float* d_data,
-                                  uint64_t n_obs, uint64_t n_features, uint64_t n_components, int32_t center,
-                                  int32_t scale, double max_value, float* d_embedding, float* d_variance_ratio,
-                                  float* d_components, void* stream) {
-  PTRY(pca_validate(n_obs, n_features, n_components, scale, max_value));
+extern "C" int psq_pca_csr_device_ex(const uint64_t* d_indptr, const uint32_t* d_indices, const float* d_data,
+                                     uint64_t n_obs, uint64_t n_features, uint64_t n_components, int32_t center,
+                                     int32_t scale, double max_value, int32_t method, uint64_t oversample,
+                                     uint64_t n_iter, uint64_t seed, float* d_embedding, float* d_variance_ratio,
+                                     float* d_components, void* stream) {
+  PTRY(pca_validate(n_obs, n_features, n_components, scale, max_value, &method, oversample, n_iter));
   if (!d_indptr || !d_embedding || !d_variance_ratio) return fail(PSQ_E_INVALID, "NULL pointer argument");
   return pca_device(d_indptr, d_indices, d_data, n_obs, n_features, n_components, center, scale, max_value,
-                    d_embedding, d_variance_ratio, d_components, (hipStream_t)stream);
+                    method, oversample, n_iter, seed, d_embedding, d_variance_ratio, d_components,
+                    (hipStream_t)stream);
 }
 
-extern "C" int psq_pca_csr(const uint64_t* indptr, const uint32_t* indices, const float* data, uint64_t n_obs,
-                           uint64_t n_features, uint64_t n_components, int32_t center, int32_t scale,
-                           double max_value, float* embedding, float* variance_ratio, float* components,
-                           int32_t device) {
-  PTRY(pca_validate(n_obs, n_features, n_components, scale, max_value));
+extern "C" int psq_pca_csr_ex(const uint64_t* indptr, const uint32_t* indices, const float* data, uint64_t n_obs,
+                              uint64_t n_features, uint64_t n_components, int32_t center, int32_t scale,
+                              double max_value, int32_t method, uint64_t oversample, uint64_t n_iter,
+                              uint64_t seed, float* embedding, float* variance_ratio, float* components,
+                              int32_t device) {
+  PTRY(pca_validate(n_obs, n_features, n_components, scale, max_value, &method, oversample, n_iter));
   if (!indptr || !embedding || !variance_ratio) return fail(PSQ_E_INVALID, "NULL pointer argument");
   const uint64_t n = n_obs, m = n_features, k = n_components, nnz = indptr[n];
   if (nnz > 0 && (!indices || !data)) return fail(PSQ_E_INVALID, "NULL pointer argument");
@@ -994,12 +1446,31 @@ extern "C" int psq_pca_csr(const uint64_t* indptr, const uint32_t* indices, cons
     PHIPC(hipMemcpyAsync(d_ix, indices, nnz * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     PHIPC(hipMemcpyAsync(d_x, data, nnz * sizeof(float), hipMemcpyHostToDevice, s));
   }
-  PTRY(pca_device(d_ip, d_ix, d_x, n, m, k, center, scale, max_value, d_emb, d_ratio, d_comp, s));
+  PTRY(pca_device(d_ip, d_ix, d_x, n, m, k, center, scale, max_value, method, oversample, n_iter, seed, d_emb,
+                  d_ratio, d_comp, s));
   PHIPC(hipMemcpyAsync(embedding, d_emb, n * k * sizeof(float), hipMemcpyDeviceToHost, s));
   PHIPC(hipMemcpyAsync(variance_ratio, d_ratio, k * sizeof(float), hipMemcpyDeviceToHost, s));
   if (components) PHIPC(hipMemcpyAsync(components, d_comp, m * k * sizeof(float), hipMemcpyDeviceToHost, s));
   PHIPC(hipStreamSynchronize(s));
   return PSQ_OK;
+}
+
+// The reference's default PcaOptions (pca.rs:73-83): oversample 10, n_iter 1, seed 0.
+extern "C" int psq_pca_csr_device(const uint64_t* d_indptr, const uint32_t* d_indices, const float* d_data,
+                                  uint64_t n_obs, uint64_t n_features, uint64_t n_components, int32_t center,
+                                  int32_t scale, double max_value, float* d_embedding, float* d_variance_ratio,
+                                  float* d_components, void* stream) {
+  return psq_pca_csr_device_ex(d_indptr, d_indices, d_data, n_obs, n_features, n_components, center, scale,
+                               max_value, PSQ_PCA_AUTO, 10, 1, 0, d_embedding, d_variance_ratio, d_components,
+                               stream);
+}
+
+extern "C" int psq_pca_csr(const uint64_t* indptr, const uint32_t* indices, const float* data, uint64_t n_obs,
+                           uint64_t n_features, uint64_t n_components, int32_t center, int32_t scale,
+                           double max_value, float* embedding, float* variance_ratio, float* components,
+                           int32_t device) {
+  return psq_pca_csr_ex(indptr, indices, data, n_obs, n_features, n_components, center, scale, max_value,
+                        PSQ_PCA_AUTO, 10, 1, 0, embedding, variance_ratio, components, device);
 }
 
 extern "C" int psq_pca_last_report(psq_pca_report* out) {

@@ -18,12 +18,14 @@ Where the reference differs:
   the reference's own test states (pca.rs:861-905: variance ratios within 1e-4,
   |cos| > 0.9999 per column).
 - More than 4,096 genes: the reference switches to randomized block Krylov SVD
-  (``oversample``, ``n_iter`` and ``seed`` configure it); this build keeps the
-  exact Gram route up to 32,768 genes (the Gram matrix is 8 GiB of float64 in
-  HBM; its product costs n_obs x genes^2 flops), whose answer is the one the randomized route approximates (the
-  reference's own test bounds it against the exact SVD: ratios within 5e-3,
-  principal angles below 1 degree), and raises ``RuntimeError`` above that. The
-  three arguments are accepted for the signature's sake.
+  (``oversample``, ``n_iter`` and ``seed`` configure it, rsvd.rs:476-566); this
+  build keeps the exact Gram route up to 32,768 genes (the Gram matrix is 8 GiB
+  of float64 in HBM; its product costs n_obs x genes^2 flops), whose answer is
+  the one the randomized route approximates (the reference's own test bounds it
+  against the exact SVD: ratios within 5e-3, principal angles below 1 degree),
+  and takes the reference's randomized route, with the reference's ChaCha8
+  Gaussian sketch, above that. ``method="randomized"`` asks for the randomized
+  route at any size, ``method="gram"`` for the exact one (at most 32,768 genes).
 
 Errors follow the binding: ``RuntimeError("PCA failed: ...")`` for an empty
 matrix or too many components (the ``PcaError`` cases), ``ValueError`` for an
@@ -61,7 +63,17 @@ def _csr_parts(matrix):
     return indptr, indices, data, n, m
 
 
-def _run(matrix, n_components: int, center: bool, scale: bool, max_value, out, components, device: int):
+_METHODS = {"auto": N.PSQ_PCA_AUTO, "gram": N.PSQ_PCA_GRAM, "randomized": N.PSQ_PCA_RANDOMIZED}
+
+
+def _method(method: str) -> int:
+    if method not in _METHODS:
+        raise ValueError(f"method must be one of {sorted(_METHODS)}, got {method!r}")
+    return _METHODS[method]
+
+
+def _run(matrix, n_components: int, center: bool, scale: bool, max_value, out, components, device: int,
+         method: str = "auto", oversample: int = 10, n_iter: int = 1, seed: int = 0):
     indptr, indices, data, n, m = _csr_parts(matrix)
     k = int(n_components)
     if out is not None:
@@ -77,9 +89,11 @@ def _run(matrix, n_components: int, center: bool, scale: bool, max_value, out, c
     ratio = np.empty(max(k, 0), dtype=np.float32)
     mv = math.inf if max_value is None else float(max_value)
     L = N.lib()
-    rc = L.psq_pca_csr(indptr.ctypes.data, indices.ctypes.data, data.ctypes.data, n, m, max(k, 0),
-                       int(bool(center)), int(bool(scale)), mv, emb.ctypes.data, ratio.ctypes.data,
-                       None if components is None else components.ctypes.data, int(device))
+    meth = _method(method)
+    rc = L.psq_pca_csr_ex(indptr.ctypes.data, indices.ctypes.data, data.ctypes.data, n, m, max(k, 0),
+                          int(bool(center)), int(bool(scale)), mv, meth, int(oversample), int(n_iter),
+                          int(seed) & (2**64 - 1), emb.ctypes.data, ratio.ctypes.data,
+                          None if components is None else components.ctypes.data, int(device))
     if rc != N.PSQ_OK:
         msg = L.psq_last_error().decode("utf-8", "replace")
         raise N.PsqError(rc, f"PCA failed: {msg}")
@@ -87,25 +101,26 @@ def _run(matrix, n_components: int, center: bool, scale: bool, max_value, out, c
 
 
 def pca_sparse(matrix, n_components: int, center: bool, oversample: int, n_iter: int, seed: int,
-               out=None, *, components=None, device: int = 0) -> dict:
+               out=None, *, components=None, device: int = 0, method: str = "auto") -> dict:
     """lib.rs:395-436: PCA of a CSR matrix with implicit centering (``center``).
     Returns ``{"embedding": (n_obs, n_components) float32 or None when written
     into ``out``, "variance_ratio": float32[n_components], "n_components": k}``.
     ``components`` (optional, float32 (n_features, n_components)) receives the
-    principal axes the embedding was projected on."""
-    del oversample, n_iter, seed  # parameters of the reference's randomized route (> 4,096 genes)
-    return _run(matrix, n_components, center, False, None, out, components, device)
+    principal axes. ``oversample``, ``n_iter`` and ``seed`` configure the
+    randomized route (more than 32,768 genes, or ``method="randomized"``)."""
+    return _run(matrix, n_components, center, False, None, out, components, device, method, oversample, n_iter,
+                seed)
 
 
 def pca_sparse_scaled(matrix, n_components: int, max_value=None, seed: int = 0, out=None, *,
-                      components=None, device: int = 0) -> dict:
+                      components=None, device: int = 0, method: str = "auto") -> dict:
     """lib.rs:438-467: PCA of the per-gene standardised matrix (``pp.scale``,
     ddof 1, a constant gene divided by 1, values clipped to +-max_value), centred,
-    without densifying."""
-    del seed
+    without densifying. The randomized route takes the reference's default
+    options here (oversample 10, n_iter 1, pca.rs:73-83) and ``seed``."""
     if max_value is not None and not float(max_value) >= 0.0:
         raise ValueError("max_value must be >= 0")
-    return _run(matrix, n_components, True, True, max_value, out, components, device)
+    return _run(matrix, n_components, True, True, max_value, out, components, device, method, 10, 1, seed)
 
 
 def pca_dense(data, n_components: int, center: bool, seed: int, *, components=None,
@@ -116,12 +131,12 @@ def pca_dense(data, n_components: int, center: bool, seed: int, *, components=No
     with arbitrary column signs, or randomized SVD when large); here the matrix
     goes through the same exact Gram route as the sparse calls, as the CSR of
     its nonzero entries, so the columns carry canonical signs."""
-    del seed
     import scipy.sparse as sp
     x = np.asarray(data, dtype=np.float32)
     if x.ndim != 2:
         raise ValueError("data must be a 2-D float32 array")
-    return _run(sp.csr_matrix(x), n_components, center, False, None, None, components, device)
+    return _run(sp.csr_matrix(x), n_components, center, False, None, None, components, device, "auto", 10, 1,
+                seed)
 
 
 def pca_last_report() -> dict:
@@ -134,7 +149,8 @@ def pca_last_report() -> dict:
 
 
 def pca_csr_device(indptr, indices, data, n_features: int, n_components: int, *, center: bool = True,
-                   scale: bool = False, max_value=None, out=None, components=None):
+                   scale: bool = False, max_value=None, out=None, components=None, method: str = "auto",
+                   oversample: int = 10, n_iter: int = 1, seed: int = 0):
     """The same computation on CUDA tensors already on the device (uint64
     indptr as int64, int32 indices, float32 data); returns (embedding,
     variance_ratio) tensors on the current stream."""
@@ -163,9 +179,10 @@ def pca_csr_device(indptr, indices, data, n_features: int, n_components: int, *,
     ratio = torch.empty(k, dtype=torch.float32, device=indptr.device)
     mv = math.inf if max_value is None else float(max_value)
     L = N.lib()
-    rc = L.psq_pca_csr_device(indptr.data_ptr(), indices.data_ptr(), data.data_ptr(), n, int(n_features), k,
-                              int(bool(center)), int(bool(scale)), mv, out.data_ptr(), ratio.data_ptr(),
-                              None if components is None else components.data_ptr(), _stream_ptr())
+    rc = L.psq_pca_csr_device_ex(indptr.data_ptr(), indices.data_ptr(), data.data_ptr(), n, int(n_features), k,
+                                 int(bool(center)), int(bool(scale)), mv, _method(method), int(oversample),
+                                 int(n_iter), int(seed) & (2**64 - 1), out.data_ptr(), ratio.data_ptr(),
+                                 None if components is None else components.data_ptr(), _stream_ptr())
     if rc != N.PSQ_OK:
         msg = L.psq_last_error().decode("utf-8", "replace")
         raise N.PsqError(rc, f"PCA failed: {msg}")

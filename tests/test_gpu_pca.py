@@ -250,7 +250,7 @@ def test_argument_errors():
     with pytest.raises(RuntimeError, match="exceeds min"):
         gp.pca_sparse(mat, 21, True, 10, 4, 0)
     with pytest.raises(RuntimeError, match="32768"):
-        gp.pca_sparse(counts(10, 32769, 0.01, 2), 2, True, 10, 4, 0)
+        gp.pca_sparse(counts(10, 32769, 0.01, 2), 2, True, 10, 4, 0, method="gram")
     ip, ix, dx = parts(mat)
     bad = ix.copy()
     bad[3] = 20

@@ -246,8 +246,9 @@ def test_abi_rejects_bad_shapes_before_device_work():
     assert rc == N.PSQ_E_INVALID and msg == "n_components (10) exceeds min(n_obs, n_features) (5)"
     rc, _ = _abi(50, 5, 0)
     assert rc == N.PSQ_E_INVALID
-    rc, msg = _abi(10, 32769, 5)
-    assert rc == N.PSQ_E_UNSUPPORTED and "32768" in msg
+    # above 32768 genes the randomized route: its basis of (k + 10) 2 columns is capped at 1024
+    rc, msg = _abi(5000, 40000, 600)
+    assert rc == N.PSQ_E_UNSUPPORTED and "Krylov" in msg
     rc, _ = _abi(10, 5, 2, scale=1, max_value=-1.0)
     assert rc == N.PSQ_E_INVALID
     rc, _ = _abi(10, 5, 2, scale=1, max_value=float("nan"))
