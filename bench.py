@@ -134,6 +134,13 @@ def main() -> int:
                          "components")
     ap.add_argument("--pca-genes", type=int, default=2000,
                     help="genes of the pca workload (2000 = the atlas after gene selection)")
+    ap.add_argument("--pca-density", type=float, default=0.08,
+                    help="share of stored entries of the pca workload's matrix")
+    ap.add_argument("--pca-method", default="auto", choices=["auto", "gram", "randomized"],
+                    help="pca route: auto = exact Gram up to 32768 genes, randomized block Krylov above")
+    ap.add_argument("--pca-n-iter", type=int, default=4,
+                    help="Krylov depth of the randomized route (pp.pca passes 4)")
+    ap.add_argument("--pca-oversample", type=int, default=10)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 elementwise copies of 1 GiB (y = x * 1, a kernel of known "
                          "traffic: 1 GiB read + 1 GiB written) to scale the HBM counters by")
@@ -636,13 +643,14 @@ def scrublet_workload(args, torch, P, ds) -> int:
     return 0
 
 
-def pca_counts(torch, n, m, density, seed, chunk=65536):
+def pca_counts(torch, n, m, density, seed, chunk=None):
     """A synthetic log-normalised sparse expression matrix on the device with the
     structure PCA is for: ten gene programs of decaying strength (cell scores
     N(0, s_p), s = 6 ... 0.8; gene loadings N(0, 1/10)) on top of gene-specific
     count noise; entries kept with probability `density`; values
     log1p(floor(u * (1 + 4 g_j)) + 1 + max(program signal, 0)). CSR with sorted
     column indices per row."""
+    chunk = chunk or max(1024, min(65536, (1 << 27) // m))  # rows of at most 2^27 draws
     gen = torch.Generator(device="cuda")
     gen.manual_seed(seed)
     gscale = 1.0 + 4.0 * torch.rand(m, generator=gen, device="cuda")
@@ -668,54 +676,80 @@ def pca_counts(torch, n, m, density, seed, chunk=65536):
 
 
 def pca_workload(args, torch, P, ds) -> int:
-    """pp.pca on the atlas after gene selection: 4.06M cells x 2000 genes, 8%
-    stored (the CSR already in HBM), 50 centred components by the exact Gram
-    route of pca_sparse (core/pca.rs:487-535). A step is one
-    psq_pca_csr_device call: Gram pass (dense FP64 row chunks, rocBLAS DGEMM),
-    Chebyshev-accelerated subspace eigensolver, projection, signs. The roofline
-    is the Gram product (the algorithmic n m (m + 1) flops of the symmetric
-    Gram over its DGEMM launches' HIP-event time) against the FP64 matrix
-    peak. CPU baseline: the oracle (numpy/LAPACK, this host's cores) on
-    the first 20,000 cells; the GPU's result on the same sample is its parity
-    check (variance ratios, and each axis an eigenvector of the oracle's Gram
-    matrix)."""
+    """pp.pca of a sparse matrix whose CSR is already in HBM, 50 centred
+    components. Gram route (default: the atlas after gene selection, 4.06M
+    cells x 2000 genes, 8% stored; pca.rs:487-535): a step is one
+    psq_pca_csr_device call, the Gram pass (dense FP64 row chunks, rocBLAS
+    DGEMM), the Chebyshev-accelerated subspace eigensolver, projection, signs;
+    the roofline is the Gram product (the algorithmic n m (m + 1) flops of the
+    symmetric Gram over its DGEMM launches' HIP-event time) against the FP64
+    matrix peak. Randomized route (more than 32,768 genes, or --pca-method
+    randomized; rsvd.rs:476-566): a step is one call, the column sums and the
+    passes of the block Krylov SVD (oversample 10, n_iter 4 as pp.pca passes);
+    the roofline is its transposed-product DGEMMs (2 n m flops per column). CPU
+    baseline: the oracle (numpy/LAPACK or scipy, this host's cores) on the
+    first 20,000 cells; the GPU's result on the same sample is its parity check."""
     from polariseq_amd import pca as gp
-    n, m, k, dens = args.n, args.pca_genes, 50, 0.08
+    n, m, k, dens = args.n, args.pca_genes, 50, args.pca_density
+    method = args.pca_method
+    if method == "auto":
+        method = "randomized" if m > 32768 else "gram"
+    over, it = args.pca_oversample, args.pca_n_iter
     log(f"generating {n} x {m} sparse matrix ({dens:.0%} stored, seed {args.seed})")
     ip, ix, dx = pca_counts(torch, n, m, dens, args.seed)
     nnz = int(ix.numel())
     out = torch.empty((n, k), dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
+
+    def call():
+        return gp.pca_csr_device(ip, ix, dx, m, k, out=out, method=method, oversample=over, n_iter=it,
+                                 seed=args.seed)
     for _ in range(args.warmup):
-        gp.pca_csr_device(ip, ix, dx, m, k, out=out)
+        call()
     torch.cuda.synchronize()
     reps = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        _, vr = gp.pca_csr_device(ip, ix, dx, m, k, out=out)
+        _, vr = call()
         reps.append(gp.pca_last_report())
+        log(f"step {len(reps)}: {reps[-1]['ms_total']:.1f} ms")
     torch.cuda.synchronize()
     el = (time.perf_counter() - t0) / args.steps
     gemm_s = sum(r["ms_gemm"] for r in reps) / len(reps) / 1000.0
-    flops = float(n) * m * (m + 1)
     rep = reps[-1]
+    if method == "gram":
+        title = f"exact Gram PCA (50 components) of {n / 1e6:.2f}M cells x {m} genes"
+        flops = float(n) * m * (m + 1)
+        kern = "Gram DGEMM (rocBLAS, FP64; algorithmic flops n m (m + 1), the product computes twice that)"
+        traffic = measured_traffic("gram_dgemm", "pca_gram_hbm_traffic.md") if m == 2000 and n == 4_060_000 else None
+        note = "HBM bytes per DGEMM launch (one 16710-row chunk)"
+    else:
+        title = (f"randomized block Krylov PCA (50 components, oversample {over}, n_iter {it}) of "
+                 f"{n / 1e6:.2f}M cells x {m} genes")
+        most = min(n, m)
+        l = max(1, min(k + over, most))
+        width = min(l * (it + 1), most)
+        blocks_ = -(-width // l)
+        cols = l * (blocks_ - 1) + width  # the transposed products' columns: A'^T Q_d, then B^T
+        flops = 2.0 * n * m * cols
+        kern = ("transposed-product DGEMMs (rocBLAS, FP64; algorithmic flops 2 n m per column; the "
+                "products with A' are the sparse kernel)")
+        traffic, note = None, None
     line = {
-        "metric": f"cells/s, exact Gram PCA (50 components) of 4.06M cells x {m} genes",
+        "metric": f"cells/s, {title}",
         "value": round(n / el, 1), "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(1000 * el, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic log-normalised sparse counts with 10 gene programs (torch, seeded)",
         "config": {"workload": f"{n} cells x {m} genes, {nnz} stored ({dens:.0%}), {k} components, "
-                               "centred, CSR resident in HBM", "nnz": nnz},
+                               f"centred, {method} route, CSR resident in HBM", "nnz": nnz},
         "phases_ms": {kk: round(v, 3) for kk, v in rep.items() if kk.startswith("ms_")},
-        "solver": {"iterations": rep["iterations"], "block": rep["block"], "chunks": rep["chunks"],
-                   "rows_per_chunk": rep["rows_per_chunk"]},
-        "roofline": {"bound": "mfma", "kernel": "Gram DGEMM (rocBLAS, FP64; algorithmic flops n m (m + 1), the product computes twice that)",
+        "solver": {"method": rep["method"], "iterations": rep["iterations"], "block": rep["block"],
+                   "chunks": rep["chunks"], "rows_per_chunk": rep["rows_per_chunk"]},
+        "roofline": {"bound": "mfma", "kernel": kern,
                      "achieved": round(flops / gemm_s / 1e12, 2), "peak": 78.6, "unit": "TFLOP/s",
-                     "frac": round(flops / gemm_s / 1e12 / 78.6, 4),
-                     "traffic": (measured_traffic("gram_dgemm", "pca_gram_hbm_traffic.md")
-                                 if m == 2000 else None),
-                     "traffic_note": "HBM bytes per DGEMM launch (one 16710-row chunk)"},
+                     "frac": round(flops / gemm_s / 1e12 / 78.6, 4), "traffic": traffic,
+                     "traffic_note": note},
         "variance_ratio_head": [round(float(v), 6) for v in vr[:5].cpu().numpy()],
     }
     if not args.no_cpu:
@@ -725,23 +759,39 @@ def pca_workload(args, torch, P, ds) -> int:
         e = int(h_ip[-1])
         h_ix = ix[:e].cpu().numpy().astype(np.uint32)
         h_dx = dx[:e].cpu().numpy()
-        log(f"oracle Gram PCA of the first {ns} cells ...")
-        c0 = time.perf_counter()
-        want = opca.pca_gram(h_ip, h_ix, h_dx, ns, m, k, True)
-        cs = time.perf_counter() - c0
-        line["cpu_baseline"] = {"value": round(ns / cs, 1), "unit": "cells/s", "cores": host_threads(),
-                                "kind": "port",
-                                "sample": f"first {ns} cells of the same matrix, numpy/LAPACK Gram PCA"}
         comp = np.zeros((m, k), np.float32)
-        r = gp.pca_sparse((h_ip, h_ix, h_dx, (ns, m)), k, True, 10, 4, 0, components=comp)
-        c = opca.centred_gram(h_ip, h_ix, h_dx, ns, m, True)[0]
-        top = float(want["eigenvalues"][0])
-        res = max(float(np.linalg.norm(c @ comp[:, j].astype(np.float64)
-                                       - (comp[:, j] @ c @ comp[:, j]) * comp[:, j].astype(np.float64)))
-                  for j in range(k)) / top
-        line["parity_vs_oracle_sample"] = {
-            "variance_ratio_max_abs_diff": float(np.max(np.abs(r["variance_ratio"] - want["variance_ratio"]))),
-            "axis_residual_max_rel": res}
+        if method == "gram":
+            log(f"oracle Gram PCA of the first {ns} cells ...")
+            c0 = time.perf_counter()
+            want = opca.pca_gram(h_ip, h_ix, h_dx, ns, m, k, True)
+            cs = time.perf_counter() - c0
+            sample = f"first {ns} cells of the same matrix, numpy/LAPACK Gram PCA"
+            r = gp.pca_sparse((h_ip, h_ix, h_dx, (ns, m)), k, True, 10, 4, 0, components=comp, method="gram")
+            c = opca.centred_gram(h_ip, h_ix, h_dx, ns, m, True)[0]
+            top = float(want["eigenvalues"][0])
+            res = max(float(np.linalg.norm(c @ comp[:, j].astype(np.float64)
+                                           - (comp[:, j] @ c @ comp[:, j]) * comp[:, j].astype(np.float64)))
+                      for j in range(k)) / top
+            parity = {"variance_ratio_max_abs_diff": float(np.max(np.abs(r["variance_ratio"]
+                                                                         - want["variance_ratio"]))),
+                      "axis_residual_max_rel": res}
+        else:
+            from oracle import rsvd as orsvd
+            log(f"oracle randomized PCA of the first {ns} cells ...")
+            c0 = time.perf_counter()
+            want = orsvd.rsvd(h_ip, h_ix, h_dx, ns, m, k, over, it, args.seed, True)
+            cs = time.perf_counter() - c0
+            sample = f"first {ns} cells of the same matrix, numpy/scipy block Krylov SVD"
+            r = gp.pca_sparse((h_ip, h_ix, h_dx, (ns, m)), k, True, over, it, args.seed, components=comp,
+                              method="randomized")
+            scale = np.abs(want["embedding"]).max(axis=0)
+            parity = {"variance_ratio_max_rel_diff": float(np.max(np.abs(r["variance_ratio"] / want["variance_ratio"]
+                                                                         - 1.0))),
+                      "embedding_max_rel_diff": float(np.max(np.abs(r["embedding"] - want["embedding"])
+                                                             / scale[None, :]))}
+        line["cpu_baseline"] = {"value": round(ns / cs, 1), "unit": "cells/s", "cores": host_threads(),
+                                "kind": "port", "sample": sample}
+        line["parity_vs_oracle_sample"] = parity
     print(json.dumps(line), flush=True)
     return 0
 

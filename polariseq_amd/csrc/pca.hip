@@ -232,6 +232,14 @@ __global__ void k_pca_scale_rows(double* __restrict__ H, const double* __restric
   if (t < count) H[t] *= lam[t % nl];
 }
 
+// The strict lower triangle of the b x b matrix T (column-major) from its upper one.
+__global__ void k_pca_mirror(double* __restrict__ T, uint32_t b) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (uint64_t)b * b) return;
+  const uint32_t col = (uint32_t)(t / b), row = (uint32_t)(t % b);
+  if (row > col) T[t] = T[(size_t)row * b + col];
+}
+
 // Columns c of Q (m x b, column-major) with dead[c] set to zero. Grid (x, b).
 __global__ void k_pca_zero_dead(double* __restrict__ Q, uint32_t m, const uint32_t* __restrict__ dead) {
   const uint32_t c = blockIdx.y;
@@ -622,6 +630,70 @@ __global__ void k_rsvd_center_t(double* __restrict__ Z, uint32_t ld, uint32_t m,
   }
 }
 
+// Xr (row-major (m + 1) x cols) = rows 0 .. m of X (column-major, leading
+// dimension ld): the layout the sparse product gathers rows from.
+__global__ void k_rsvd_rows(const double* __restrict__ X, uint32_t ld, uint32_t m, uint32_t cols,
+                            double* __restrict__ Xr) {
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x, count = (uint64_t)(m + 1) * cols;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += step) {
+    const uint64_t j = t / cols;
+    Xr[t] = X[(t - j * cols) * ld + j];
+  }
+}
+
+// Y (column-major n x cols, leading dimension n) = S X + 1 X[m]: the product
+// of the CSR matrix with X, whose row m (-mu^T X) centres it (rsvd.rs:241-283).
+// One workgroup per 64 rows: wave w takes rows w, w + 4, ... of the tile, lanes
+// the columns c0 + lane, each sum over the row's stored entries in order (f64
+// fma); the tile goes through LDS and out column by column, lanes over rows.
+__global__ __launch_bounds__(256) void k_rsvd_spmm(const uint64_t* __restrict__ indptr,
+                                                   const uint32_t* __restrict__ indices,
+                                                   const float* __restrict__ val, uint64_t n,
+                                                   const double* __restrict__ Xr, uint32_t cols, uint32_t m,
+                                                   double* __restrict__ Y) {
+  __shared__ double tile[64 * 65];
+  const uint64_t row0 = (uint64_t)blockIdx.x * 64;
+  const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
+  for (uint32_t c0 = 0; c0 < cols; c0 += 64) {
+    const uint32_t c = c0 + lane;
+    const bool cact = c < cols;
+    for (uint32_t r = w; r < 64; r += 4) {
+      const uint64_t i = row0 + r;
+      double acc = 0.0;
+      if (i < n && cact) {
+        uint64_t e = indptr[i];
+        const uint64_t e1 = indptr[i + 1];
+        for (; e + 4 <= e1; e += 4) {
+          const uint32_t j0 = indices[e], j1 = indices[e + 1], j2 = indices[e + 2], j3 = indices[e + 3];
+          const double x0 = Xr[(size_t)j0 * cols + c], x1 = Xr[(size_t)j1 * cols + c];
+          const double x2 = Xr[(size_t)j2 * cols + c], x3 = Xr[(size_t)j3 * cols + c];
+          acc = fma((double)val[e], x0, acc);
+          acc = fma((double)val[e + 1], x1, acc);
+          acc = fma((double)val[e + 2], x2, acc);
+          acc = fma((double)val[e + 3], x3, acc);
+        }
+        for (; e < e1; ++e) acc = fma((double)val[e], Xr[(size_t)indices[e] * cols + c], acc);
+        acc += Xr[(size_t)m * cols + c];
+      }
+      tile[r * 65 + lane] = acc;
+    }
+    __syncthreads();
+    const uint64_t i = row0 + lane;
+    for (uint32_t cc = w; cc < 64 && c0 + cc < cols; cc += 4)
+      if (i < n) Y[(size_t)(c0 + cc) * n + i] = tile[lane * 65 + cc];
+    __syncthreads();
+  }
+}
+
+// G[t] = the np partials' sum, in order.
+__global__ void k_pca_fold(const double* __restrict__ Gp, uint32_t np, uint64_t count, double* __restrict__ G) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  double a = 0.0;
+  for (uint32_t p = 0; p < np; ++p) a += Gp[(size_t)p * count + t];
+  G[t] = a;
+}
+
 // Sums of v^2 (f64) over the stored values, per block: block g takes the
 // elements g 256 + t + i G 256 in order, then a fixed-order tree sum.
 __global__ __launch_bounds__(256) void k_rsvd_sumsq(const float* __restrict__ v, uint64_t nnz,
@@ -642,14 +714,16 @@ __global__ __launch_bounds__(256) void k_rsvd_sumsq(const float* __restrict__ v,
   if (threadIdx.x == 0) part[blockIdx.x] = sp[0];
 }
 
-// Singular values s_j = sqrt(max(lam_j, 0)) of the sketch (lam: eigenvalues of
-// B B^T, descending), their reciprocals (0 for s_j = 0) and the variance ratio
+// Singular values s_j = sqrt(max(lam_j - sigma, 0)) of the sketch (lam:
+// eigenvalues of B B^T + sigma I, descending; sigma = tv[1], 0 without tv),
+// their reciprocals (0 for s_j = 0) and the variance ratio
 // s_j^2 / total (pca.rs:149-166).
-__global__ void k_rsvd_values(const double* __restrict__ lam, uint32_t k, double total, double* __restrict__ sv,
-                              double* __restrict__ inv, float* __restrict__ ratio) {
+__global__ void k_rsvd_values(const double* __restrict__ lam, const double* __restrict__ tv, uint32_t k,
+                              double total, double* __restrict__ sv, double* __restrict__ inv,
+                              float* __restrict__ ratio) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= k) return;
-  const double s = sqrt(fmax(lam[j], 0.0));
+  const double s = sqrt(fmax(lam[j] - (tv ? tv[1] : 0.0), 0.0));
   sv[j] = s;
   inv[j] = s > 0.0 ? 1.0 / s : 0.0;
   ratio[j] = (float)(s * s / total);
@@ -740,15 +814,52 @@ int blas_handle(hipStream_t s, rocblas_handle* out) {
 
 size_t small_lds(uint32_t b) { return b <= PCA_LDS_BLOCK ? (size_t)b * b * sizeof(double) : 0; }
 
+// Row segments of the split Gram of a tall m x b block: one (a single DSYRK)
+// below 65,536 rows, else one per 4,096 rows, at most 256 and at most 2^25 / b^2
+// (the partials' size).
+uint32_t gram_parts(uint64_t m, uint32_t b) {
+  if (m < 65536) return 1;
+  const uint64_t cap = std::max<uint64_t>(2, (1ull << 25) / ((uint64_t)b * b));
+  return (uint32_t)std::max<uint64_t>(2, std::min<uint64_t>({m / 4096, 256, cap}));
+}
+
+// G (b x b; with parts > 1 full, else its upper triangle) = A^T A for the
+// column-major m x b block A. parts > 1: a batched DGEMM per row segment into
+// Gp ((parts + 1) b^2 doubles), folded in segment order, so that the product
+// of a tall block keeps the GPU busy.
+int gram_tall(rocblas_handle h, const double* A, uint32_t m, uint32_t b, double* G, double* Gp, uint32_t parts,
+              hipStream_t s) {
+  const double one = 1.0, zero = 0.0;
+  if (parts <= 1) {
+    PBLAS(rocblas_dsyrk(h, rocblas_fill_upper, rocblas_operation_transpose, b, m, &one, A, m, &zero, G, b));
+    return PSQ_OK;
+  }
+  const uint32_t seg = m / parts, rem = m - seg * parts;
+  const rocblas_stride bb = (rocblas_stride)b * b;
+  PBLAS(rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, b, b, seg, &one, A, m,
+                                      seg, A, m, seg, &zero, Gp, b, bb, parts));
+  uint32_t np = parts;
+  if (rem) {
+    const double* Ar = A + (size_t)seg * parts;
+    PBLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, b, b, rem, &one, Ar, m, Ar, m, &zero,
+                        Gp + (size_t)parts * bb, b));
+    ++np;
+  }
+  hipLaunchKernelGGL(k_pca_fold, dim3(blocks((uint64_t)bb, 256)), dim3(256), 0, s, Gp, np, (uint64_t)bb, G);
+  PHIPC(hipGetLastError());
+  return PSQ_OK;
+}
+
 // Q (m x b, column-major) <- an orthonormal basis of its columns: shifted
 // CholeskyQR, then CholeskyQR twice (G = Q^T Q, R = chol(G), Q <- Q R^-1). The
 // first j columns keep spanning what the first j input columns span; a column
 // dependent on the earlier ones comes out zero (dead: b entries of scratch).
-int orthonormalise(rocblas_handle h, double* Q, uint32_t m, uint32_t b, double* G, uint32_t* dead,
-                   hipStream_t s) {
-  const double one = 1.0, zero = 0.0;
+// The Gram is split over `parts` row segments (gram_tall, Gp its partials).
+int orthonormalise(rocblas_handle h, double* Q, uint32_t m, uint32_t b, double* G, uint32_t* dead, hipStream_t s,
+                   double* Gp = nullptr, uint32_t parts = 1) {
+  const double one = 1.0;
   for (int pass = 0; pass < 3; ++pass) {
-    PBLAS(rocblas_dsyrk(h, rocblas_fill_upper, rocblas_operation_transpose, b, m, &one, Q, m, &zero, G, b));
+    PTRY(gram_tall(h, Q, m, b, G, Gp, parts, s));
     hipLaunchKernelGGL(k_pca_chol, dim3(1), dim3(256), small_lds(b), s, G, b, m, pass == 0 ? 1 : 0, dead);
     PHIPC(hipGetLastError());
     PBLAS(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
@@ -882,323 +993,15 @@ void rsvd_widths(uint64_t n, uint64_t m, uint64_t k, uint64_t oversample, uint64
   *width_out = blocks_ > most / l ? most : std::min(l * blocks_, most);
 }
 
-// Randomized block Krylov SVD of S - 1 mu^T (rsvd.rs:476-566): Q_0 = qr(A' Omega),
-// Q_d = qr(A' qr(A'^T Q_{d-1})) for d = 1 .. n_iter while d l < width (a last
-// block that does not fit is cut to the width), Q = qr([Q_0 .. Q_q]),
-// B^T = A'^T Q; with B B^T = W diag(s^2) W^T, U = Q W[:, :k], embedding = U s,
-// columns signed as in the Gram route, ratio = s^2 / ||A'||_F^2, components =
-// B^T W[:, :k] / s. Every product with A' is a pass over the dense f64 row
-// chunks (DGEMM with the chunk and its ones column, whose row carries the
-// centring); the QR factorisations are shifted CholeskyQR3 and the small
-// eigenproblem the cyclic Jacobi of the Gram route, so a run repeats bit for bit.
-// A rank-deficient block keeps its dependent columns at zero where Householder
-// QR would complete it with unit vectors orthogonal to the matrix's range:
-// either way they add nothing to B, and the components of s > 0 are the same.
-int rsvd_route(rocblas_handle h, Dev& dv, Events& evs, psq_pca_report& rep, const Chunks& ch, const float* S,
-               uint64_t nnz, uint32_t k, int32_t center, uint64_t oversample, uint64_t n_iter, uint64_t seed, float* d_emb,
-               float* d_ratio, float* d_comp, size_t ev_start, size_t ev_scaled) {
-  const hipStream_t s = ch.s;
-  const uint64_t n = ch.n;
-  const uint32_t m = ch.m, ld = ch.ld;
+// Top kk eigenpairs of the symmetric m x m matrix C (column-major, upper
+// triangle read; tv = {trace, sigma} on the device, htrace the trace on the
+// host): Chebyshev-accelerated subspace iteration on C + sigma I with block
+// b = min(m, kk + max(16, kk / 2)). Q (m x b) holds the eigenvectors in its
+// first kk columns, descending; lam the eigenvalues of C + sigma I.
+int top_eigenpairs(rocblas_handle h, Dev& dv, const double* C, uint32_t m, uint32_t kk, const double* tv,
+                   double htrace, psq_pca_report& rep, double** Q_out, double** lam_out, uint32_t* b_out,
+                   hipStream_t s) {
   const double one = 1.0, zero = 0.0;
-  uint64_t l64, w64;
-  rsvd_widths(n, m, k, oversample, n_iter, &l64, &w64);
-  const uint32_t l = (uint32_t)l64, width = (uint32_t)w64;
-
-  // 1. column means (f32, rsvd.rs:155-177) and ||A'||_F^2 = sum over stored
-  //    values of v^2 - n sum mu^2 (rsvd.rs:186-208), the last sums on the host
-  //    in order
-  double *acc_s, *acc_q, *part;
-  PTRY(dv.alloc(m, &acc_s));
-  PTRY(dv.alloc(m, &acc_q));
-  PTRY(column_sums(ch, S, dv, acc_s, acc_q));
-  ++rep.iterations;
-  const uint32_t nnz_blocks = std::min<uint32_t>(1024, blocks(nnz, 256));
-  PTRY(dv.alloc(nnz_blocks, &part));
-  hipLaunchKernelGGL(k_rsvd_sumsq, dim3(nnz_blocks), dim3(256), 0, s, S, nnz, part);
-  PHIPC(hipGetLastError());
-  float* mu = nullptr;
-  std::vector<float> hmu;
-  if (center) {
-    PTRY(dv.alloc(m, &mu));
-    hipLaunchKernelGGL(k_pca_means, dim3(blocks(m, 256)), dim3(256), 0, s, acc_s, m, n, mu);
-    PHIPC(hipGetLastError());
-    hmu.resize(m);
-    PHIPC(hipMemcpyAsync(hmu.data(), mu, m * sizeof(float), hipMemcpyDeviceToHost, s));
-  }
-  std::vector<double> hq(nnz_blocks);
-  PHIPC(hipMemcpyAsync(hq.data(), part, nnz_blocks * sizeof(double), hipMemcpyDeviceToHost, s));
-  PHIPC(hipStreamSynchronize(s));
-  double total = 0.0, mu2 = 0.0;
-  for (double v : hq) total += v;
-  for (float v : hmu) mu2 += (double)v * (double)v;
-  total -= (double)n * mu2;
-  if (!std::isfinite(total)) return fail(PSQ_E_INVALID, "the matrix holds non-finite values");
-
-  // 2. the Krylov basis
-  double *X, *Y, *K, *Z, *G, *zs;
-  PTRY(dv.alloc((size_t)ld * l, &X));
-  PTRY(dv.alloc((size_t)n * l, &Y));
-  PTRY(dv.alloc((size_t)n * width, &K));
-  PTRY(dv.alloc((size_t)ld * width, &Z));
-  uint32_t* dead;
-  PTRY(dv.alloc((size_t)width * width, &G));
-  PTRY(dv.alloc(width, &zs));
-  PTRY(dv.alloc(width, &dead));
-  {
-    const std::vector<float> om = gaussian_rowmajor(m, l, seed);
-    std::vector<double> hx((size_t)ld * l, 0.0);
-    for (uint32_t j = 0; j < m; ++j)
-      for (uint32_t c = 0; c < l; ++c) hx[(size_t)c * ld + j] = (double)om[(size_t)j * l + c];
-    PHIPC(hipMemcpyAsync(X, hx.data(), hx.size() * sizeof(double), hipMemcpyHostToDevice, s));
-    PHIPC(hipStreamSynchronize(s));
-  }
-  std::vector<std::pair<size_t, size_t>> ev_gemm;
-  // Yo (n x cols, leading dimension n) = A' Xi (Xi: ld x cols, row m = -mu^T Xi)
-  auto forward = [&](double* Xi, uint32_t cols, double* Yo) -> int {
-    if (mu) {
-      hipLaunchKernelGGL(k_rsvd_mu_row, dim3(cols), dim3(256), 0, s, Xi, ld, m, mu);
-      PHIPC(hipGetLastError());
-    }
-    for (uint64_t r0 = 0; r0 < n; r0 += ch.R) {
-      const uint32_t rc = (uint32_t)std::min<uint64_t>(ch.R, n - r0);
-      PTRY(ch.densify(r0, rc, S));
-      size_t a, b;
-      PTRY(evs.record(s, &a));
-      PBLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, rc, cols, ld, &one, ch.D, ld, Xi,
-                          ld, &zero, Yo + r0, n));
-      PTRY(evs.record(s, &b));
-      ev_gemm.emplace_back(a, b);
-    }
-    ++rep.iterations;
-    return PSQ_OK;
-  };
-  // Zo (ld x cols; rows m .. ld - 1 zero) = A'^T Yi (Yi: n x cols)
-  auto backward = [&](const double* Yi, uint32_t cols, double* Zo) -> int {
-    for (uint64_t r0 = 0; r0 < n; r0 += ch.R) {
-      const uint32_t rc = (uint32_t)std::min<uint64_t>(ch.R, n - r0);
-      PTRY(ch.densify(r0, rc, S));
-      size_t a, b;
-      PTRY(evs.record(s, &a));
-      PBLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, ld, cols, rc, &one, ch.D, ld, Yi + r0,
-                          n, r0 == 0 ? &zero : &one, Zo, ld));
-      PTRY(evs.record(s, &b));
-      ev_gemm.emplace_back(a, b);
-    }
-    hipLaunchKernelGGL(k_rsvd_take_row, dim3(blocks(cols, 64)), dim3(64), 0, s, Zo, ld, m, cols, zs);
-    if (mu)
-      hipLaunchKernelGGL(k_rsvd_center_t, dim3(stride_blocks((uint64_t)m * cols, 256)), dim3(256), 0, s, Zo, ld, m,
-                         cols, mu, zs);
-    PHIPC(hipGetLastError());
-    ++rep.iterations;
-    return PSQ_OK;
-  };
-  PTRY(forward(X, l, Y));
-  PTRY(orthonormalise(h, Y, (uint32_t)n, l, G, dead, s));
-  PHIPC(hipMemcpyAsync(K, Y, (size_t)n * l * sizeof(double), hipMemcpyDeviceToDevice, s));
-  for (uint64_t d = 1; d <= n_iter && d * l < width; ++d) {
-    PTRY(backward(Y, l, Z));
-    PTRY(orthonormalise(h, Z, ld, l, G, dead, s));
-    PTRY(forward(Z, l, Y));
-    PTRY(orthonormalise(h, Y, (uint32_t)n, l, G, dead, s));
-    const uint64_t cnt = std::min<uint64_t>(l, width - d * l);
-    PHIPC(hipMemcpyAsync(K + d * l * n, Y, (size_t)n * cnt * sizeof(double), hipMemcpyDeviceToDevice, s));
-  }
-  PTRY(orthonormalise(h, K, (uint32_t)n, width, G, dead, s));
-  PTRY(backward(K, width, Z));  // B^T
-  size_t ev_basis;
-  PTRY(evs.record(s, &ev_basis));
-
-  // 3. B B^T = W diag(s^2) W^T (width x width), descending
-  double *Tb, *Wb, *Ws, *lam, *cs, *sv, *inv;
-  PTRY(dv.alloc((size_t)width * width, &Tb));
-  PTRY(dv.alloc((size_t)width * width, &Wb));
-  PTRY(dv.alloc((size_t)width * width, &Ws));
-  PTRY(dv.alloc(width, &lam));
-  PTRY(dv.alloc((size_t)4 * (width / 2 + 1), &cs));
-  PTRY(dv.alloc(k, &sv));
-  PTRY(dv.alloc(k, &inv));
-  PBLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, width, width, ld, &one, Z, ld, Z, ld,
-                      &zero, Tb, width));
-  hipLaunchKernelGGL(k_pca_jacobi, dim3(1), dim3(256), small_lds(width), s, Tb, width, Wb, Ws, lam, cs);
-  PHIPC(hipGetLastError());
-  size_t ev_eigen;
-  PTRY(evs.record(s, &ev_eigen));
-
-  // 4. U = Q W[:, :k] (into Y: l >= k columns), the outputs, signs
-  PBLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, k, width, &one, K,
-                      (rocblas_int)n, Ws, width, &zero, Y, (rocblas_int)n));
-  hipLaunchKernelGGL(k_rsvd_values, dim3(blocks(k, 64)), dim3(64), 0, s, lam, k, total, sv, inv, d_ratio);
-  hipLaunchKernelGGL(k_rsvd_out, dim3(stride_blocks(n * k, 256)), dim3(256), 0, s, Y, n, n, k, sv, d_emb);
-  PHIPC(hipGetLastError());
-  float* ppv;
-  uint64_t* ppi;
-  uint32_t* flip;
-  const uint32_t rps = std::max<uint32_t>(1, 256 / k);
-  const uint32_t Gs = (uint32_t)std::min<uint64_t>(2048, (n + 255) / 256);
-  const uint64_t seg = (n + Gs - 1) / Gs;
-  PTRY(dv.alloc((size_t)Gs * k, &ppv));
-  PTRY(dv.alloc((size_t)Gs * k, &ppi));
-  PTRY(dv.alloc(k, &flip));
-  hipLaunchKernelGGL(k_pca_sign_part, dim3(Gs), dim3(k * rps), 0, s, d_emb, n, k, rps, seg, ppv, ppi);
-  hipLaunchKernelGGL(k_pca_sign_pick, dim3(blocks(k, 64)), dim3(64), 0, s, d_emb, k, Gs, ppv, ppi, flip);
-  hipLaunchKernelGGL(k_pca_sign_apply, dim3(stride_blocks(n * k, 256)), dim3(256), 0, s, d_emb, n * k, k, flip);
-  PHIPC(hipGetLastError());
-  if (d_comp) {
-    // V = B^T W[:, :k] / s (into X: ld x l >= ld x k), with the embedding's signs
-    PBLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, ld, k, width, &one, Z, ld, Ws, width,
-                        &zero, X, ld));
-    hipLaunchKernelGGL(k_rsvd_out, dim3(stride_blocks((uint64_t)m * k, 256)), dim3(256), 0, s, X, ld, m, k, inv,
-                       d_comp);
-    hipLaunchKernelGGL(k_pca_sign_apply, dim3(stride_blocks((uint64_t)m * k, 256)), dim3(256), 0, s, d_comp,
-                       (uint64_t)m * k, k, flip);
-    PHIPC(hipGetLastError());
-  }
-  size_t ev_end;
-  PTRY(evs.record(s, &ev_end));
-  PHIPC(hipStreamSynchronize(s));  // the buffers freed on return are no longer in use
-  rep.method = PSQ_PCA_RANDOMIZED;
-  rep.block = width;
-  rep.rows_per_chunk = ch.R;
-  rep.chunks = (n + ch.R - 1) / ch.R;
-  rep.ms_scale = evs.ms(ev_start, ev_scaled);
-  rep.ms_gram = evs.ms(ev_scaled, ev_basis);
-  for (const auto& p : ev_gemm) rep.ms_gemm += evs.ms(p.first, p.second);
-  rep.ms_eigen = evs.ms(ev_basis, ev_eigen);
-  rep.ms_project = evs.ms(ev_eigen, ev_end);
-  rep.ms_total = evs.ms(ev_start, ev_end);
-  g_pca_report = rep;
-  return PSQ_OK;
-}
-
-int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t n, uint64_t m64, uint64_t k64,
-               int32_t center, int32_t scale, double max_value, int32_t method, uint64_t oversample,
-               uint64_t n_iter, uint64_t seed, float* d_emb, float* d_ratio, float* d_comp, hipStream_t s) {
-  const uint32_t m = (uint32_t)m64, kk = (uint32_t)k64;
-  const double clip = max_value;  // +inf: no clipping
-  if (scale) center = 1;          // pp.scale always centres (pca.rs:416-419)
-
-  uint64_t ends[2] = {0, 0};
-  PHIPC(hipMemcpyAsync(&ends[0], ip, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-  PHIPC(hipMemcpyAsync(&ends[1], ip + n, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-  PHIPC(hipStreamSynchronize(s));
-  if (ends[0] != 0) return fail(PSQ_E_INVALID, "indptr[0] must be 0");
-  const uint64_t nnz = ends[1];
-  if (nnz > 0 && (!ix || !x)) return fail(PSQ_E_INVALID, "NULL pointer argument");
-
-  rocblas_handle h = nullptr;
-  PTRY(blas_handle(s, &h));
-  Dev dv;
-  Events evs;
-  psq_pca_report rep{};
-  size_t ev_start, ev_scaled, ev_gram, ev_eigen, ev_end;
-  std::vector<std::pair<size_t, size_t>> ev_gemm;
-  PTRY(evs.record(s, &ev_start));
-  {
-    uint32_t* d_bad;
-    PTRY(dv.alloc(1, &d_bad));
-    PHIPC(hipMemsetAsync(d_bad, 0, 4, s));
-    hipLaunchKernelGGL(k_pca_check, dim3(stride_blocks(std::max(nnz, n), 256)), dim3(256), 0, s, ip, n, ix, nnz, m,
-                       d_bad);
-    PHIPC(hipGetLastError());
-    uint32_t bad = 0;
-    PHIPC(hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, s));
-    PHIPC(hipStreamSynchronize(s));
-    if (bad & 2) return fail(PSQ_E_INVALID, "indptr must not decrease");
-    if (bad & 1) return fail(PSQ_E_INVALID, "a column index is >= n_features");
-  }
-
-  // dense f64 row chunks of at least 2^25 elements (256 MB) and 4096 rows, rows
-  // of ld = m + 1 (the ones column) rounded up to 8
-  // (2^27 elements, 1 GiB, on the randomized route: its passes multiply a chunk
-  // by a few columns, so fewer, taller chunks keep the DGEMMs busy)
-  const uint32_t ld = (m + 1 + 7) & ~7u;
-  const uint64_t chunk_elems = method == PSQ_PCA_RANDOMIZED ? (1ull << 27) : (1ull << 25);
-  const uint32_t R = (uint32_t)std::max<uint64_t>(
-      1, std::min<uint64_t>(n, std::max<uint64_t>(4096, chunk_elems / ld)));
-  const uint32_t P = std::min<uint32_t>(PCA_SEGMENTS, R);
-  double *D, *acc_s = nullptr, *acc_q = nullptr, *ps_s = nullptr, *ps_q = nullptr, *C, *Gb;
-  PTRY(dv.alloc((size_t)R * ld, &D));
-  const Chunks ch{ip, ix, n, m, ld, R, D, s};
-  const double one = 1.0, zero = 0.0;
-  auto densify = [&](uint64_t r0, uint32_t rc, const float* vals) -> int { return ch.densify(r0, rc, vals); };
-
-  const float* S = x;  // the stored values the Gram and projection read
-  if (scale) {
-    double *mean, *inv, *zr;
-    float* sv;
-    PTRY(dv.alloc(m, &acc_s));
-    PTRY(dv.alloc(m, &acc_q));
-    PTRY(dv.alloc((size_t)P * m, &ps_s));
-    PTRY(dv.alloc((size_t)P * m, &ps_q));
-    PTRY(dv.alloc(m, &mean));
-    PTRY(dv.alloc(m, &inv));
-    PTRY(dv.alloc(m, &zr));
-    PTRY(dv.alloc(nnz, &sv));
-    PHIPC(hipMemsetAsync(acc_s, 0, m * sizeof(double), s));
-    PHIPC(hipMemsetAsync(acc_q, 0, m * sizeof(double), s));
-    for (uint64_t r0 = 0; r0 < n; r0 += R) {
-      const uint32_t rc = (uint32_t)std::min<uint64_t>(R, n - r0);
-      PTRY(densify(r0, rc, x));
-      const uint32_t Pc = std::min<uint32_t>(P, rc), seg = (rc + Pc - 1) / Pc;
-      hipLaunchKernelGGL(k_pca_colsums_part, dim3(blocks(m, 256), Pc), dim3(256), 0, s, D, rc, m, ld, seg, ps_s,
-                         ps_q);
-      hipLaunchKernelGGL(k_pca_colsums_fold, dim3(blocks(m, 256)), dim3(256), 0, s, ps_s, ps_q, Pc, m, acc_s,
-                         acc_q);
-      PHIPC(hipGetLastError());
-    }
-    hipLaunchKernelGGL(k_pca_scale_stats, dim3(blocks(m, 256)), dim3(256), 0, s, acc_s, acc_q, m, n, clip, mean,
-                       inv, zr);
-    if (nnz)
-      hipLaunchKernelGGL(k_pca_scale_values, dim3(stride_blocks(nnz, 256)), dim3(256), 0, s, ix, x, nnz, mean, inv, zr,
-                         clip, sv);
-    PHIPC(hipGetLastError());
-    S = sv;
-  }
-  PTRY(evs.record(s, &ev_scaled));
-  if (method == PSQ_PCA_RANDOMIZED)
-    return rsvd_route(h, dv, evs, rep, ch, S, nnz, kk, center, oversample, n_iter, seed, d_emb, d_ratio, d_comp,
-                      ev_start, ev_scaled);
-
-  // Gram of the stored values; the ones column's entries hold the column sums
-  if (!acc_s) PTRY(dv.alloc(m, &acc_s));
-  PTRY(dv.alloc((size_t)m * m, &C));
-  PTRY(dv.alloc((size_t)ld * ld, &Gb));
-  PHIPC(hipMemsetAsync(Gb, 0, (size_t)ld * ld * sizeof(double), s));
-  for (uint64_t r0 = 0; r0 < n; r0 += R) {
-    const uint32_t rc = (uint32_t)std::min<uint64_t>(R, n - r0);
-    PTRY(densify(r0, rc, S));
-    // D is row-major rc x ld, i.e. column-major A = ld x rc: G += A A^T. The full
-    // product (DGEMM, N/T) runs near the FP64 matrix peak where DSYRK's half
-    // product does not.
-    size_t a, b2;
-    PTRY(evs.record(s, &a));
-    PBLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, ld, ld, rc, &one, D, ld, D, ld,
-                        &one, Gb, ld));
-    PTRY(evs.record(s, &b2));
-    ev_gemm.emplace_back(a, b2);
-    ++rep.chunks;
-  }
-  rep.rows_per_chunk = R;
-  hipLaunchKernelGGL(k_pca_gram_unpack, dim3(blocks((uint64_t)m * m, 256)), dim3(256), 0, s, Gb, m, ld, C, acc_s);
-  PHIPC(hipGetLastError());
-  float* mu = nullptr;
-  if (center) {
-    PTRY(dv.alloc(m, &mu));
-    hipLaunchKernelGGL(k_pca_means, dim3(blocks(m, 256)), dim3(256), 0, s, acc_s, m, n, mu);
-  }
-  hipLaunchKernelGGL(k_pca_center, dim3(blocks((uint64_t)m * m, 256)), dim3(256), 0, s, C, m, n, mu);
-  double* tv;
-  PTRY(dv.alloc(2, &tv));
-  hipLaunchKernelGGL(k_pca_trace, dim3(1), dim3(1), 0, s, C, m, tv);
-  PHIPC(hipGetLastError());
-  PTRY(evs.record(s, &ev_gram));
-  double htv[2];
-  PHIPC(hipMemcpyAsync(htv, tv, sizeof(htv), hipMemcpyDeviceToHost, s));
-  PHIPC(hipStreamSynchronize(s));
-  if (!std::isfinite(htv[0])) return fail(PSQ_E_INVALID, "the matrix holds non-finite values");
-
-  // top k eigenpairs
   const uint32_t b = (uint32_t)std::min<uint64_t>(m, kk + std::max<uint32_t>(16, kk / 2));
   if (b > PCA_MAX_BLOCK) return fail(PSQ_E_UNSUPPORTED, "n_components > 680 is not supported");
   double *Q, *Z, *Tmp, *G, *Tb, *Wb, *Ws, *lam, *cs, *res;
@@ -1214,7 +1017,7 @@ int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t 
   PTRY(dv.alloc(b, &res));
   uint32_t* dead;
   PTRY(dv.alloc(b, &dead));
-  if (htv[0] <= 0.0) {
+  if (htrace <= 0.0) {
     // C = 0 (a PSD matrix with zero trace): no direction is preferred
     hipLaunchKernelGGL(k_pca_unit_block, dim3(blocks((uint64_t)m * kk, 256)), dim3(256), 0, s, Q, lam, m, kk);
     PHIPC(hipGetLastError());
@@ -1325,6 +1128,345 @@ int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t 
     if (!converged)
       return fail(PSQ_E_DEVICE, "the eigensolver did not converge in %u products", PCA_MAX_PRODUCTS);
   }
+  *Q_out = Q;
+  *lam_out = lam;
+  *b_out = b;
+  return PSQ_OK;
+}
+
+// Randomized block Krylov SVD of S - 1 mu^T (rsvd.rs:476-566): Q_0 = qr(A' Omega),
+// Q_d = qr(A' qr(A'^T Q_{d-1})) for d = 1 .. n_iter while d l < width (a last
+// block that does not fit is cut to the width), Q = qr([Q_0 .. Q_q]),
+// B^T = A'^T Q; with B B^T = W diag(s^2) W^T, U = Q W[:, :k], embedding = U s,
+// columns signed as in the Gram route, ratio = s^2 / ||A'||_F^2, components =
+// B^T W[:, :k] / s. A' X is a sparse product over the CSR rows (X staged
+// row-major, its row m = -mu^T X the centring); A'^T Y a pass over the dense
+// f64 row chunks (DGEMM with the chunk and its ones column, whose row gives
+// 1^T Y for the centring). The QR factorisations are shifted CholeskyQR3
+// (Grams of tall blocks split over row segments and folded in order) and the
+// small eigenproblem the Gram route's solvers, so a run repeats bit for bit. A
+// rank-deficient block keeps its dependent columns at zero where Householder QR
+// would complete it with unit vectors orthogonal to the matrix's range: either
+// way they add nothing to B, and the components of s > 0 are the same.
+int rsvd_route(rocblas_handle h, Dev& dv, Events& evs, psq_pca_report& rep, const Chunks& ch, const float* S,
+               uint64_t nnz, uint32_t k, int32_t center, uint64_t oversample, uint64_t n_iter, uint64_t seed, float* d_emb,
+               float* d_ratio, float* d_comp, size_t ev_start, size_t ev_scaled) {
+  const hipStream_t s = ch.s;
+  const uint64_t n = ch.n;
+  const uint32_t m = ch.m, ld = ch.ld;
+  const double one = 1.0, zero = 0.0;
+  uint64_t l64, w64;
+  rsvd_widths(n, m, k, oversample, n_iter, &l64, &w64);
+  const uint32_t l = (uint32_t)l64, width = (uint32_t)w64;
+
+  // 1. column means (f32, rsvd.rs:155-177) and ||A'||_F^2 = sum over stored
+  //    values of v^2 - n sum mu^2 (rsvd.rs:186-208), the last sums on the host
+  //    in order
+  double *acc_s, *acc_q, *part;
+  PTRY(dv.alloc(m, &acc_s));
+  PTRY(dv.alloc(m, &acc_q));
+  PTRY(column_sums(ch, S, dv, acc_s, acc_q));
+  ++rep.iterations;
+  const uint32_t nnz_blocks = std::min<uint32_t>(1024, blocks(nnz, 256));
+  PTRY(dv.alloc(nnz_blocks, &part));
+  hipLaunchKernelGGL(k_rsvd_sumsq, dim3(nnz_blocks), dim3(256), 0, s, S, nnz, part);
+  PHIPC(hipGetLastError());
+  float* mu = nullptr;
+  std::vector<float> hmu;
+  if (center) {
+    PTRY(dv.alloc(m, &mu));
+    hipLaunchKernelGGL(k_pca_means, dim3(blocks(m, 256)), dim3(256), 0, s, acc_s, m, n, mu);
+    PHIPC(hipGetLastError());
+    hmu.resize(m);
+    PHIPC(hipMemcpyAsync(hmu.data(), mu, m * sizeof(float), hipMemcpyDeviceToHost, s));
+  }
+  std::vector<double> hq(nnz_blocks);
+  PHIPC(hipMemcpyAsync(hq.data(), part, nnz_blocks * sizeof(double), hipMemcpyDeviceToHost, s));
+  PHIPC(hipStreamSynchronize(s));
+  double total = 0.0, mu2 = 0.0;
+  for (double v : hq) total += v;
+  for (float v : hmu) mu2 += (double)v * (double)v;
+  total -= (double)n * mu2;
+  if (!std::isfinite(total)) return fail(PSQ_E_INVALID, "the matrix holds non-finite values");
+
+  // 2. the Krylov basis: Y = A' X by the sparse product (X_0 = Omega, then
+  //    X = qr(A'^T Q_{d-1})), Q_d = qr(Y); then Q = qr(K) and B^T = A'^T Q
+  const uint32_t gp = gram_parts(n, width), gz = gram_parts(ld, width), gx = gram_parts(ld, l);
+  double *X, *Xr, *Y, *K, *Bt, *G, *Gp, *zs;
+  uint32_t* dead;
+  PTRY(dv.alloc((size_t)ld * l, &X));
+  PTRY(dv.alloc((size_t)(m + 1) * l, &Xr));
+  PTRY(dv.alloc((size_t)n * l, &Y));
+  PTRY(dv.alloc((size_t)n * width, &K));
+  PTRY(dv.alloc((size_t)ld * width, &Bt));
+  PTRY(dv.alloc((size_t)width * width, &G));
+  PTRY(dv.alloc(std::max<size_t>((size_t)(std::max(gp, gz) + 1) * width * width, (size_t)(gx + 1) * l * l), &Gp));
+  PTRY(dv.alloc(width, &zs));
+  PTRY(dv.alloc(width, &dead));
+  {
+    const std::vector<float> om = gaussian_rowmajor(m, l, seed);
+    std::vector<double> hx((size_t)ld * l, 0.0);
+    for (uint32_t j = 0; j < m; ++j)
+      for (uint32_t c = 0; c < l; ++c) hx[(size_t)c * ld + j] = (double)om[(size_t)j * l + c];
+    PHIPC(hipMemcpyAsync(X, hx.data(), hx.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    PHIPC(hipStreamSynchronize(s));
+  }
+  std::vector<std::pair<size_t, size_t>> ev_gemm;
+  // Y = A' X: X's row m = -mu^T X, X staged row-major, then the sparse product
+  auto forward = [&]() -> int {
+    if (mu) hipLaunchKernelGGL(k_rsvd_mu_row, dim3(l), dim3(256), 0, s, X, ld, m, mu);
+    hipLaunchKernelGGL(k_rsvd_rows, dim3(stride_blocks((uint64_t)(m + 1) * l, 256)), dim3(256), 0, s, X, ld, m, l,
+                       Xr);
+    hipLaunchKernelGGL(k_rsvd_spmm, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, s, ch.ip, ch.ix, S, n, Xr, l, m,
+                       Y);
+    PHIPC(hipGetLastError());
+    ++rep.iterations;
+    return PSQ_OK;
+  };
+  // Zo (ld x cols; rows m .. ld - 1 zero) = A'^T Yi (Yi: n x cols, leading dimension n)
+  auto backward = [&](const double* Yi, uint32_t cols, double* Zo) -> int {
+    for (uint64_t r0 = 0; r0 < n; r0 += ch.R) {
+      const uint32_t rc = (uint32_t)std::min<uint64_t>(ch.R, n - r0);
+      PTRY(ch.densify(r0, rc, S));
+      size_t a, b;
+      PTRY(evs.record(s, &a));
+      PBLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, ld, cols, rc, &one, ch.D, ld, Yi + r0,
+                          n, r0 == 0 ? &zero : &one, Zo, ld));
+      PTRY(evs.record(s, &b));
+      ev_gemm.emplace_back(a, b);
+    }
+    hipLaunchKernelGGL(k_rsvd_take_row, dim3(blocks(cols, 64)), dim3(64), 0, s, Zo, ld, m, cols, zs);
+    if (mu)
+      hipLaunchKernelGGL(k_rsvd_center_t, dim3(stride_blocks((uint64_t)m * cols, 256)), dim3(256), 0, s, Zo, ld, m,
+                         cols, mu, zs);
+    PHIPC(hipGetLastError());
+    ++rep.iterations;
+    return PSQ_OK;
+  };
+  PTRY(forward());
+  PTRY(orthonormalise(h, Y, (uint32_t)n, l, G, dead, s, Gp, gp));
+  PHIPC(hipMemcpyAsync(K, Y, (size_t)n * l * sizeof(double), hipMemcpyDeviceToDevice, s));
+  for (uint64_t d = 1; d <= n_iter && d * l < width; ++d) {
+    PTRY(backward(Y, l, X));
+    PTRY(orthonormalise(h, X, ld, l, G, dead, s, Gp, gx));
+    PTRY(forward());
+    PTRY(orthonormalise(h, Y, (uint32_t)n, l, G, dead, s, Gp, gp));
+    const uint64_t cnt = std::min<uint64_t>(l, width - d * l);
+    PHIPC(hipMemcpyAsync(K + d * l * n, Y, (size_t)n * cnt * sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
+  PTRY(orthonormalise(h, K, (uint32_t)n, width, G, dead, s, Gp, gp));
+  PTRY(backward(K, width, Bt));
+  size_t ev_basis;
+  PTRY(evs.record(s, &ev_basis));
+
+  // 3. B B^T = W diag(s^2) W^T (width x width), the top k (descending): the
+  //    cyclic Jacobi directly on a small basis, else the Gram route's
+  //    eigensolver
+  double *Tb, *Wk, *lam, *sv, *inv, *tv = nullptr;
+  PTRY(dv.alloc((size_t)width * width, &Tb));
+  PTRY(dv.alloc(k, &sv));
+  PTRY(dv.alloc(k, &inv));
+  PTRY(gram_tall(h, Bt, ld, width, Tb, Gp, gz, s));
+  if (width <= PCA_LDS_BLOCK) {
+    double *Wb, *cs;
+    PTRY(dv.alloc((size_t)width * width, &Wb));
+    PTRY(dv.alloc((size_t)width * width, &Wk));
+    PTRY(dv.alloc(width, &lam));
+    PTRY(dv.alloc((size_t)4 * (width / 2 + 1), &cs));
+    if (gz <= 1)  // DSYRK wrote the upper triangle; the Jacobi reads both
+      hipLaunchKernelGGL(k_pca_mirror, dim3(blocks((uint64_t)width * width, 256)), dim3(256), 0, s, Tb, width);
+    hipLaunchKernelGGL(k_pca_jacobi, dim3(1), dim3(256), small_lds(width), s, Tb, width, Wb, Wk, lam, cs);
+    PHIPC(hipGetLastError());
+  } else {
+    PTRY(dv.alloc(2, &tv));
+    hipLaunchKernelGGL(k_pca_trace, dim3(1), dim3(1), 0, s, Tb, width, tv);
+    PHIPC(hipGetLastError());
+    double htv[2];
+    PHIPC(hipMemcpyAsync(htv, tv, sizeof(htv), hipMemcpyDeviceToHost, s));
+    PHIPC(hipStreamSynchronize(s));
+    psq_pca_report er{};
+    uint32_t be;
+    PTRY(top_eigenpairs(h, dv, Tb, width, k, tv, htv[0], er, &Wk, &lam, &be, s));
+  }
+  size_t ev_eigen;
+  PTRY(evs.record(s, &ev_eigen));
+
+  // 4. U = Q W[:, :k] (into Y: l >= k columns), the outputs, signs
+  PBLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, k, width, &one, K,
+                      (rocblas_int)n, Wk, width, &zero, Y, (rocblas_int)n));
+  hipLaunchKernelGGL(k_rsvd_values, dim3(blocks(k, 64)), dim3(64), 0, s, lam, tv, k, total, sv, inv, d_ratio);
+  hipLaunchKernelGGL(k_rsvd_out, dim3(stride_blocks(n * k, 256)), dim3(256), 0, s, Y, n, n, k, sv, d_emb);
+  PHIPC(hipGetLastError());
+  float* ppv;
+  uint64_t* ppi;
+  uint32_t* flip;
+  const uint32_t rps = std::max<uint32_t>(1, 256 / k);
+  const uint32_t Gs = (uint32_t)std::min<uint64_t>(2048, (n + 255) / 256);
+  const uint64_t seg = (n + Gs - 1) / Gs;
+  PTRY(dv.alloc((size_t)Gs * k, &ppv));
+  PTRY(dv.alloc((size_t)Gs * k, &ppi));
+  PTRY(dv.alloc(k, &flip));
+  hipLaunchKernelGGL(k_pca_sign_part, dim3(Gs), dim3(k * rps), 0, s, d_emb, n, k, rps, seg, ppv, ppi);
+  hipLaunchKernelGGL(k_pca_sign_pick, dim3(blocks(k, 64)), dim3(64), 0, s, d_emb, k, Gs, ppv, ppi, flip);
+  hipLaunchKernelGGL(k_pca_sign_apply, dim3(stride_blocks(n * k, 256)), dim3(256), 0, s, d_emb, n * k, k, flip);
+  PHIPC(hipGetLastError());
+  if (d_comp) {
+    // V = B^T W[:, :k] / s (into X: ld x l >= ld x k), with the embedding's signs
+    PBLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, ld, k, width, &one, Bt, ld, Wk, width,
+                        &zero, X, ld));
+    hipLaunchKernelGGL(k_rsvd_out, dim3(stride_blocks((uint64_t)m * k, 256)), dim3(256), 0, s, X, ld, m, k, inv,
+                       d_comp);
+    hipLaunchKernelGGL(k_pca_sign_apply, dim3(stride_blocks((uint64_t)m * k, 256)), dim3(256), 0, s, d_comp,
+                       (uint64_t)m * k, k, flip);
+    PHIPC(hipGetLastError());
+  }
+  size_t ev_end;
+  PTRY(evs.record(s, &ev_end));
+  PHIPC(hipStreamSynchronize(s));  // the buffers freed on return are no longer in use
+  rep.method = PSQ_PCA_RANDOMIZED;
+  rep.block = width;
+  rep.rows_per_chunk = ch.R;
+  rep.chunks = (n + ch.R - 1) / ch.R;
+  rep.ms_scale = evs.ms(ev_start, ev_scaled);
+  rep.ms_gram = evs.ms(ev_scaled, ev_basis);
+  for (const auto& p : ev_gemm) rep.ms_gemm += evs.ms(p.first, p.second);
+  rep.ms_eigen = evs.ms(ev_basis, ev_eigen);
+  rep.ms_project = evs.ms(ev_eigen, ev_end);
+  rep.ms_total = evs.ms(ev_start, ev_end);
+  g_pca_report = rep;
+  return PSQ_OK;
+}
+
+int pca_device(const uint64_t* ip, const uint32_t* ix, const float* x, uint64_t n, uint64_t m64, uint64_t k64,
+               int32_t center, int32_t scale, double max_value, int32_t method, uint64_t oversample,
+               uint64_t n_iter, uint64_t seed, float* d_emb, float* d_ratio, float* d_comp, hipStream_t s) {
+  const uint32_t m = (uint32_t)m64, kk = (uint32_t)k64;
+  const double clip = max_value;  // +inf: no clipping
+  if (scale) center = 1;          // pp.scale always centres (pca.rs:416-419)
+
+  uint64_t ends[2] = {0, 0};
+  PHIPC(hipMemcpyAsync(&ends[0], ip, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  PHIPC(hipMemcpyAsync(&ends[1], ip + n, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  PHIPC(hipStreamSynchronize(s));
+  if (ends[0] != 0) return fail(PSQ_E_INVALID, "indptr[0] must be 0");
+  const uint64_t nnz = ends[1];
+  if (nnz > 0 && (!ix || !x)) return fail(PSQ_E_INVALID, "NULL pointer argument");
+
+  rocblas_handle h = nullptr;
+  PTRY(blas_handle(s, &h));
+  Dev dv;
+  Events evs;
+  psq_pca_report rep{};
+  size_t ev_start, ev_scaled, ev_gram, ev_eigen, ev_end;
+  std::vector<std::pair<size_t, size_t>> ev_gemm;
+  PTRY(evs.record(s, &ev_start));
+  {
+    uint32_t* d_bad;
+    PTRY(dv.alloc(1, &d_bad));
+    PHIPC(hipMemsetAsync(d_bad, 0, 4, s));
+    hipLaunchKernelGGL(k_pca_check, dim3(stride_blocks(std::max(nnz, n), 256)), dim3(256), 0, s, ip, n, ix, nnz, m,
+                       d_bad);
+    PHIPC(hipGetLastError());
+    uint32_t bad = 0;
+    PHIPC(hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, s));
+    PHIPC(hipStreamSynchronize(s));
+    if (bad & 2) return fail(PSQ_E_INVALID, "indptr must not decrease");
+    if (bad & 1) return fail(PSQ_E_INVALID, "a column index is >= n_features");
+  }
+
+  // dense f64 row chunks of at least 2^25 elements (256 MB) and 4096 rows, rows
+  // of ld = m + 1 (the ones column) rounded up to 8
+  // (2^27 elements, 1 GiB, on the randomized route: its passes multiply a chunk
+  // by a few columns, so fewer, taller chunks keep the DGEMMs busy)
+  const uint32_t ld = (m + 1 + 7) & ~7u;
+  const uint64_t chunk_elems = method == PSQ_PCA_RANDOMIZED ? (1ull << 27) : (1ull << 25);
+  const uint32_t R = (uint32_t)std::max<uint64_t>(
+      1, std::min<uint64_t>(n, std::max<uint64_t>(4096, chunk_elems / ld)));
+  const uint32_t P = std::min<uint32_t>(PCA_SEGMENTS, R);
+  double *D, *acc_s = nullptr, *acc_q = nullptr, *ps_s = nullptr, *ps_q = nullptr, *C, *Gb;
+  PTRY(dv.alloc((size_t)R * ld, &D));
+  const Chunks ch{ip, ix, n, m, ld, R, D, s};
+  const double one = 1.0;
+  auto densify = [&](uint64_t r0, uint32_t rc, const float* vals) -> int { return ch.densify(r0, rc, vals); };
+
+  const float* S = x;  // the stored values the Gram and projection read
+  if (scale) {
+    double *mean, *inv, *zr;
+    float* sv;
+    PTRY(dv.alloc(m, &acc_s));
+    PTRY(dv.alloc(m, &acc_q));
+    PTRY(dv.alloc((size_t)P * m, &ps_s));
+    PTRY(dv.alloc((size_t)P * m, &ps_q));
+    PTRY(dv.alloc(m, &mean));
+    PTRY(dv.alloc(m, &inv));
+    PTRY(dv.alloc(m, &zr));
+    PTRY(dv.alloc(nnz, &sv));
+    PHIPC(hipMemsetAsync(acc_s, 0, m * sizeof(double), s));
+    PHIPC(hipMemsetAsync(acc_q, 0, m * sizeof(double), s));
+    for (uint64_t r0 = 0; r0 < n; r0 += R) {
+      const uint32_t rc = (uint32_t)std::min<uint64_t>(R, n - r0);
+      PTRY(densify(r0, rc, x));
+      const uint32_t Pc = std::min<uint32_t>(P, rc), seg = (rc + Pc - 1) / Pc;
+      hipLaunchKernelGGL(k_pca_colsums_part, dim3(blocks(m, 256), Pc), dim3(256), 0, s, D, rc, m, ld, seg, ps_s,
+                         ps_q);
+      hipLaunchKernelGGL(k_pca_colsums_fold, dim3(blocks(m, 256)), dim3(256), 0, s, ps_s, ps_q, Pc, m, acc_s,
+                         acc_q);
+      PHIPC(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_pca_scale_stats, dim3(blocks(m, 256)), dim3(256), 0, s, acc_s, acc_q, m, n, clip, mean,
+                       inv, zr);
+    if (nnz)
+      hipLaunchKernelGGL(k_pca_scale_values, dim3(stride_blocks(nnz, 256)), dim3(256), 0, s, ix, x, nnz, mean, inv, zr,
+                         clip, sv);
+    PHIPC(hipGetLastError());
+    S = sv;
+  }
+  PTRY(evs.record(s, &ev_scaled));
+  if (method == PSQ_PCA_RANDOMIZED)
+    return rsvd_route(h, dv, evs, rep, ch, S, nnz, kk, center, oversample, n_iter, seed, d_emb, d_ratio, d_comp,
+                      ev_start, ev_scaled);
+
+  // Gram of the stored values; the ones column's entries hold the column sums
+  if (!acc_s) PTRY(dv.alloc(m, &acc_s));
+  PTRY(dv.alloc((size_t)m * m, &C));
+  PTRY(dv.alloc((size_t)ld * ld, &Gb));
+  PHIPC(hipMemsetAsync(Gb, 0, (size_t)ld * ld * sizeof(double), s));
+  for (uint64_t r0 = 0; r0 < n; r0 += R) {
+    const uint32_t rc = (uint32_t)std::min<uint64_t>(R, n - r0);
+    PTRY(densify(r0, rc, S));
+    // D is row-major rc x ld, i.e. column-major A = ld x rc: G += A A^T. The full
+    // product (DGEMM, N/T) runs near the FP64 matrix peak where DSYRK's half
+    // product does not.
+    size_t a, b2;
+    PTRY(evs.record(s, &a));
+    PBLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, ld, ld, rc, &one, D, ld, D, ld,
+                        &one, Gb, ld));
+    PTRY(evs.record(s, &b2));
+    ev_gemm.emplace_back(a, b2);
+    ++rep.chunks;
+  }
+  rep.rows_per_chunk = R;
+  hipLaunchKernelGGL(k_pca_gram_unpack, dim3(blocks((uint64_t)m * m, 256)), dim3(256), 0, s, Gb, m, ld, C, acc_s);
+  PHIPC(hipGetLastError());
+  float* mu = nullptr;
+  if (center) {
+    PTRY(dv.alloc(m, &mu));
+    hipLaunchKernelGGL(k_pca_means, dim3(blocks(m, 256)), dim3(256), 0, s, acc_s, m, n, mu);
+  }
+  hipLaunchKernelGGL(k_pca_center, dim3(blocks((uint64_t)m * m, 256)), dim3(256), 0, s, C, m, n, mu);
+  double* tv;
+  PTRY(dv.alloc(2, &tv));
+  hipLaunchKernelGGL(k_pca_trace, dim3(1), dim3(1), 0, s, C, m, tv);
+  PHIPC(hipGetLastError());
+  PTRY(evs.record(s, &ev_gram));
+  double htv[2];
+  PHIPC(hipMemcpyAsync(htv, tv, sizeof(htv), hipMemcpyDeviceToHost, s));
+  PHIPC(hipStreamSynchronize(s));
+  if (!std::isfinite(htv[0])) return fail(PSQ_E_INVALID, "the matrix holds non-finite values");
+
+  uint32_t b;
+  double *Q, *lam;
+  PTRY(top_eigenpairs(h, dv, C, m, kk, tv, htv[0], rep, &Q, &lam, &b, s));
 
   PTRY(evs.record(s, &ev_eigen));
   rep.block = b;
