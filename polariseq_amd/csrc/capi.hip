@@ -199,9 +199,15 @@ int fill_u8(uint8_t* p, size_t count, uint8_t v, hipStream_t s) {
   return PSQ_OK;
 }
 
-// Exclusive scan of n u32 values into out[0..n], out[n] = total.
+// Exclusive scan of n u32 values into out[0..n], out[n] = total. Up to one
+// block of values, the block's own total is out[n]: one launch.
 int scan_u32(Workspace* ws, const uint32_t* in, uint32_t* out, uint32_t n, hipStream_t s) {
   const uint32_t nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
+  if (nb == 1) {
+    hipLaunchKernelGGL(k_scan_local, dim3(1), dim3(SCAN_BLOCK), 0, s, in, out, n, out + n);
+    HIPC(hipGetLastError());
+    return PSQ_OK;
+  }
   uint32_t* sums = nullptr;
   TRY(wbuf(ws, "scan_sums", std::max<uint32_t>(nb, 1), &sums));
   if (n > 0) {
@@ -1439,8 +1445,15 @@ int knn_exact_small(BufSrc& B, const float* d_data, uint64_t n, uint64_t d, cons
   hipLaunchKernelGGL(p1, g1, dim3(256), smem, s, Xp, xn, (uint32_t)n, (uint32_t)d, dp, d_queries,
                      (uint32_t)nq, seg_len, nseg, (uint32_t)k, partial);
   HIPC(hipGetLastError());
-  hipLaunchKernelGGL(k_exact_tau, dim3(grid_for(nq, 128)), dim3(128), 0, s, partial, (uint32_t)nq,
-                     nseg * 4 * K, (uint32_t)k, (uint32_t)d, d_queries, xn, pmx, T);
+  // tau: a thread per query over many queries; a block per query (radix select)
+  // when the queries are too few to fill the GPU and their lists fit LDS
+  const uint32_t L = nseg * 4 * K;
+  if (nq <= 65536 && L <= 12288)
+    hipLaunchKernelGGL(k_exact_tau_big, dim3((uint32_t)nq), dim3(256), (size_t)L * 4, s, partial, (uint32_t)nq,
+                       L, (uint32_t)k, (uint32_t)d, d_queries, xn, pmx, T);
+  else
+    hipLaunchKernelGGL(k_exact_tau, dim3(grid_for(nq, 128)), dim3(128), 0, s, partial, (uint32_t)nq, L,
+                       (uint32_t)k, (uint32_t)d, d_queries, xn, pmx, T);
   HIPC(hipGetLastError());
   void (*p2)(const float*, const float*, uint32_t, uint32_t, uint32_t, const uint32_t*, uint32_t,
              uint32_t, const float*, uint32_t*, uint32_t*, uint32_t) =

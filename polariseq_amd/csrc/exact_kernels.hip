@@ -553,9 +553,10 @@ __global__ void k_exact_fallback(const float* __restrict__ Xp, uint32_t n, uint3
 // threshold built from it keeps the proof of pass 2. The selection and the
 // re-rank run per query in LDS.
 
-// tau: the k-th smallest of a query's L pass-1 values, by radix select
-// on the FP32 bits (four 8-bit digits; the values are >= 0, so bit order is
-// value order). One block per query; dynamic LDS: L values.
+// tau: the k-th smallest of a query's L pass-1 values, by radix select (four
+// 8-bit digits) on an order-preserving image of the FP32 bits (a value can
+// round below zero: negative values have their bits inverted, the others their
+// sign bit set). One block per query; dynamic LDS: L values.
 __global__ __launch_bounds__(256) void k_exact_tau_big(const float* __restrict__ partial,
                                                        uint32_t nq, uint32_t L, uint32_t k, uint32_t d,
                                                        const uint32_t* __restrict__ queries,
@@ -567,7 +568,10 @@ __global__ __launch_bounds__(256) void k_exact_tau_big(const float* __restrict__
   __shared__ uint32_t s_prefix, s_mask, s_rank;
   const uint32_t q = blockIdx.x, tid = threadIdx.x;
   const float* src = partial + (size_t)q * L;
-  for (uint32_t e = tid; e < L; e += blockDim.x) sv[e] = __float_as_uint(src[e]);
+  for (uint32_t e = tid; e < L; e += blockDim.x) {
+    const uint32_t b = __float_as_uint(src[e]);
+    sv[e] = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+  }
   if (tid == 0) {
     s_prefix = 0;
     s_mask = 0;
@@ -593,7 +597,9 @@ __global__ __launch_bounds__(256) void k_exact_tau_big(const float* __restrict__
     __syncthreads();
   }
   if (tid != 0) return;
-  T[q] = exact_threshold(__uint_as_float(s_prefix), xn[queries ? queries[q] : q], pmax, d);
+  const uint32_t u = s_prefix;
+  const float tau = __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+  T[q] = exact_threshold(tau, xn[queries ? queries[q] : q], pmax, d);
 }
 
 // One block per query: the FP64 distance of every candidate, a bitonic sort of

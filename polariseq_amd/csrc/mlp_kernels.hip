@@ -147,13 +147,21 @@ __global__ __launch_bounds__(ENC_WAVES * 64) void k_encode_mfma(
 //   H_pre = X W1^T + b1      (48 x 128, k = 52)      Z = gelu(H_pre) W2^T + b2 (48 x 32, k = 128)
 //   dW2 = dZ^T gelu(H_pre)   (32 x 128, k = 48)      dH = dZ W2              (48 x 128, k = 32)
 //   dW1 = dH_pre^T X         (128 x 52, k = 48),     dH_pre = dH * gelu'(H_pre)
-// Activations live in LDS at row strides of 20 mod 64 words, so the 16 rows of
-// a tile read by c fall on distinct banks. The block's STEP_WAVES waves take
-// the tiles of each product in turn.
+// Activations live in LDS at row strides of 4 (x, h) and 52 (z) mod 64 words,
+// so the 16 rows of a tile read by c fall on distinct banks, in 72 KB: two
+// blocks per CU. The block's STEP_WAVES waves take the tiles of each product in
+// turn. The partial gradients are stored by tiles of 32 parameters, the blocks'
+// slices of a tile side by side (pslot), so that the reduction streams them.
 // k_mlp_reduce_adam: sums the blocks' partials in a fixed order and takes the
 // Adam step (deterministic for a seed).
 constexpr int STEP_R = 3 * MLP_STEP_T;  // rows per block
-constexpr int SX = 84, SH = 148, SZ = 52;  // LDS row strides (floats)
+constexpr int SX = 68, SH = 132, SZ = 52;  // LDS row strides (floats)
+static_assert(N_PARAMS % 32 == 0, "the partials are stored by tiles of 32 parameters");
+
+// Where block blk of nb puts its partial of parameter p.
+__device__ inline size_t pslot(uint32_t p, uint32_t blk, uint32_t nb) {
+  return ((size_t)(p / 32) * nb + blk) * 32 + p % 32;
+}
 constexpr int STEP_WAVES = MLP_STEP_WAVES;
 constexpr size_t STEP_SMEM = (size_t)STEP_R * (SX + 2 * SH + SZ) * sizeof(float) + STEP_R * 4;
 
@@ -266,7 +274,7 @@ __global__ __launch_bounds__(64 * STEP_WAVES) void k_mlp_step(
     }
   }
   __syncthreads();
-  float* out = partial + (size_t)blockIdx.x * N_PARAMS;
+  auto put = [&](uint32_t p, float v) { partial[pslot(p, blockIdx.x, gridDim.x)] = v; };
   // dW2 = dZ^T gelu(H_pre): 2 x 8 tiles of (16 outputs, 16 hidden), k over the 48 rows
   for (uint32_t tile = w; tile < 16; tile += STEP_WAVES) {
     const uint32_t ot = tile / 8, jt = tile % 8;
@@ -278,7 +286,7 @@ __global__ __launch_bounds__(64 * STEP_WAVES) void k_mlp_step(
                                                  0, 0, 0);
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) out[OFF_W2 + (16 * ot + 4 * g + i) * MLP_H + 16 * jt + c] = acc[i];
+    for (int i = 0; i < 4; ++i) put(OFF_W2 + (16 * ot + 4 * g + i) * MLP_H + 16 * jt + c, acc[i]);
   }
   // dH = dZ W2, then dL/dh_pre = dH * gelu'(h_pre) in place of h_pre: 3 x 8 tiles, k over 32 outputs
   for (uint32_t tile = w; tile < 24; tile += STEP_WAVES) {
@@ -299,12 +307,12 @@ __global__ __launch_bounds__(64 * STEP_WAVES) void k_mlp_step(
   if (tid < (uint32_t)MLP_H) {
     float acc = 0.0f;
     for (int r = 0; r < STEP_R; ++r) acc += s_hp[r * SH + tid];
-    out[OFF_B1 + tid] = acc;
+    put(OFF_B1 + tid, acc);
   } else if (tid < (uint32_t)(MLP_H + MLP_OUT)) {
     const uint32_t o = tid - MLP_H;
     float acc = 0.0f;
     for (int r = 0; r < STEP_R; ++r) acc += s_z[r * SZ + o];
-    out[OFF_B2 + o] = acc;
+    put(OFF_B2 + o, acc);
   }
   // dW1 = dH_pre^T X: 8 x 4 tiles of (16 hidden, 16 inputs), k over the 48 rows
   for (uint32_t tile = w; tile < 32; tile += STEP_WAVES) {
@@ -319,7 +327,7 @@ __global__ __launch_bounds__(64 * STEP_WAVES) void k_mlp_step(
     const uint32_t t = 16 * tt + c;
     if (t < (uint32_t)MLP_IN) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) out[OFF_W1 + (16 * jt + 4 * g + i) * MLP_IN + t] = acc[i];
+      for (int i = 0; i < 4; ++i) put(OFF_W1 + (16 * jt + 4 * g + i) * MLP_IN + t, acc[i]);
     }
   }
 }
@@ -327,8 +335,9 @@ __global__ __launch_bounds__(64 * STEP_WAVES) void k_mlp_step(
 size_t mlp_step_smem() { return STEP_SMEM; }
 
 // Block of 256 threads = 32 parameters x 8 groups: group g sums the partials
-// of blocks g, g + 8, ... in order, then the 8 group sums are added in a fixed
-// tree.
+// of blocks g, g + 8, ... into 8 accumulators by turn (block g + 8 i into
+// accumulator i % 8, eight loads in flight), added in a fixed tree, then the 8
+// group sums are added in a fixed tree.
 __global__ __launch_bounds__(256) void k_mlp_reduce_adam(
     const float* __restrict__ partial, uint32_t nblocks, float* __restrict__ w,
     float* __restrict__ wr, float* __restrict__ m1, float* __restrict__ m2, float lr, float b1,
@@ -336,9 +345,15 @@ __global__ __launch_bounds__(256) void k_mlp_reduce_adam(
   __shared__ float s_g[8][33];
   const uint32_t pl = threadIdx.x % 32, grp = threadIdx.x / 32;
   const uint32_t p = blockIdx.x * 32 + pl;
-  float part = 0.0f;
-  if (p < (uint32_t)N_PARAMS)
-    for (uint32_t b = grp; b < nblocks; b += 8) part += partial[(size_t)b * N_PARAMS + p];
+  float q[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  if (p < (uint32_t)N_PARAMS) {
+    for (uint32_t b = grp; b < nblocks; b += 64) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        if (b + 8 * i < nblocks) q[i] += partial[pslot(p, b + 8 * i, nblocks)];
+    }
+  }
+  const float part = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7]));
   s_g[grp][pl] = part;
   __syncthreads();
   if (grp != 0 || p >= (uint32_t)N_PARAMS) return;
