@@ -586,6 +586,15 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
   if (cap > 0 && fin_smem > 65536)
     HIPC(hipFuncSetAttribute((const void*)k_forest_finish,
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)fin_smem));
+  // the event the level loop waits on for its counts (destroyed on return)
+  struct EventGuard {
+    hipEvent_t e = nullptr;
+    ~EventGuard() {
+      if (e) (void)hipEventDestroy(e);
+    }
+  } evg;
+  HIPC(hipEventCreateWithFlags(&evg.e, hipEventDisableTiming));
+  const hipEvent_t ev_count = evg.e;
   std::vector<uint64_t> hseeds(T);
   for (uint64_t t = 0; t < T; ++t) hseeds[t] = splitmix(R.seed ^ (t + 1));
   uint64_t* tseeds;
@@ -683,15 +692,17 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
       HIPC(hipGetLastError());
       TRY(scan_u32(ws, ntile, tileoff, A, s));
       HIPC(hipMemcpyAsync(hbuf, tileoff + A, 4, hipMemcpyDeviceToHost, s));
-      HIPC(hipStreamSynchronize(s));
+      HIPC(hipEventRecord(ev_count, s));
+      // the planes need no tile count: they run while the host reads it
+      hipLaunchKernelGGL(k_forest_planes, dim3(grid_for(A, 4)), dim3(256), 0, s, Xs, ds, dps, n,
+                         ids, ns[cur], nc[cur], nt[cur], A, tseeds + t0, L, normals, offsets,
+                         nabs);
+      HIPC(hipGetLastError());
+      HIPC(hipEventSynchronize(ev_count));
       const uint32_t NT = hbuf[0];
       if ((size_t)NT > tile_cap)
         return fail(PSQ_E_DEVICE, "forest bookkeeping overflow (tiles %u)", NT);
       hipLaunchKernelGGL(k_ftile_map, dim3(grid_for(NT, 256)), dim3(256), 0, s, tileoff, A, NT, tnode);
-      HIPC(hipGetLastError());
-      hipLaunchKernelGGL(k_forest_planes, dim3(grid_for(A, 4)), dim3(256), 0, s, Xs, ds, dps, n,
-                         ids, ns[cur], nc[cur], nt[cur], A, tseeds + t0, L, normals, offsets,
-                         nabs);
       HIPC(hipGetLastError());
       hipLaunchKernelGGL(k_forest_sides_tile, dim3(NT), dim3(256), 0, s, Xs, ds, dps, n, Xh, dh, ids,
                          ns[cur], nc[cur], nt[cur], tileoff, tnode, tseeds + t0, L, normals,
@@ -706,7 +717,16 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
       TRY(scan_u32(ws, nsm, soff, A, s));
       HIPC(hipMemcpyAsync(hbuf, aoff + A, 4, hipMemcpyDeviceToHost, s));
       HIPC(hipMemcpyAsync(hbuf + 1, soff + A, 4, hipMemcpyDeviceToHost, s));
-      HIPC(hipStreamSynchronize(s));
+      HIPC(hipEventRecord(ev_count, s));
+      // the points' moves need neither count (nor anything the emit writes): they
+      // run while the host reads the counts
+      hipLaunchKernelGGL(k_forest_scatter_tile, dim3(NT), dim3(256), 0, s, n, ids, ns[cur], nc[cur],
+                         nt[cur], tileoff, tnode, nlv, mode, flmask, frmask, scl, scr, tmp);
+      HIPC(hipGetLastError());
+      hipLaunchKernelGGL(k_forest_copyback_tile, dim3(NT), dim3(256), 0, s, n, ns[cur], nc[cur],
+                         nt[cur], tileoff, tnode, tmp, ids);
+      HIPC(hipGetLastError());
+      HIPC(hipEventSynchronize(ev_count));
       const uint32_t nextA = hbuf[0], newsmall = hbuf[1];
       if ((size_t)nextA > act_cap || (size_t)small_base + newsmall > node_cap)
         return fail(PSQ_E_DEVICE, "forest bookkeeping overflow (nodes %u, subtrees %u)", nextA,
@@ -714,12 +734,6 @@ int build_forest(Workspace* ws, const Resolved& R, const float* Xs, uint32_t ds,
       hipLaunchKernelGGL(k_forest_emit, dim3(grid_for(A, 256)), dim3(256), 0, s, ns[cur], nc[cur],
                          nt[cur], A, n, nlv, mode, aoff, soff, ls, cap, L, ns[nx], nc[nx], nt[nx],
                          child, lcnt_pos, small_base, sm_start, sm_count, sm_tree, sm_level);
-      HIPC(hipGetLastError());
-      hipLaunchKernelGGL(k_forest_scatter_tile, dim3(NT), dim3(256), 0, s, n, ids, ns[cur], nc[cur],
-                         nt[cur], tileoff, tnode, nlv, mode, flmask, frmask, scl, scr, tmp);
-      HIPC(hipGetLastError());
-      hipLaunchKernelGGL(k_forest_copyback_tile, dim3(NT), dim3(256), 0, s, n, ns[cur], nc[cur],
-                         nt[cur], tileoff, tnode, tmp, ids);
       HIPC(hipGetLastError());
       cur = nx;
       A = nextA;
