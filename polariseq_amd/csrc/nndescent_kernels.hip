@@ -200,6 +200,8 @@ __global__ __launch_bounds__(64) void k_sample_select(
   const uint32_t l = (uint32_t)lane_id();
   const uint32_t i = block_id();
   if (i >= n) return;  // block-uniform
+  // the reverse list's bounds, loaded together with the row
+  const uint32_t r0 = roff[i], r1 = roff[i + 1];
   uint64_t fkey[KV], fk[KV];
   bool fvalid[KV];
   uint32_t me[KV];
@@ -231,15 +233,24 @@ __global__ __launch_bounds__(64) void k_sample_select(
         if (atomicCAS(&s_ht[h], 0u, v * WAVE + l + 1) == 0u) break;
     }
   __syncthreads();
-  const uint32_t r0 = roff[i], nrev = roff[i + 1] - r0, T = k + nrev;
+  const uint32_t nrev = r1 - r0, T = k + nrev;
   const uint32_t nvec = (T + WAVE - 1) / WAVE;
-  // Reverse candidate t of vector j (t >= k): its key, or EMPTY when a forward
+  // Reverse entry of candidate t (t >= k; vectors 0 and 1 are kept in registers
+  // after their first read).
+  uint64_t rvc0 = 0, rvc1 = 0;
+  auto rev_entry = [&](uint32_t j, uint32_t t) -> uint64_t {
+    if (j == 0) return rvc0;
+    if (j == 1) return rvc1;
+    return rev[r0 + (t - k)];
+  };
+  // Reverse candidate t = j WAVE + l (t >= k): its key, or EMPTY when a forward
   // entry of the same class names the same neighbour and comes from the earlier
   // source row (i before the neighbour). When the reverse copy's source (the
   // neighbour) comes first, it replaces that forward entry (record = true).
-  auto rev_key = [&](uint32_t t, bool record) -> uint64_t {
+  auto rev_key = [&](uint32_t j, bool record) -> uint64_t {
+    const uint32_t t = j * WAVE + l;
     if (t < k || t >= T) return EMPTY_KEY;
-    const uint64_t rv = rev[r0 + (t - k)];
+    const uint64_t rv = rev_entry(j, t);
     const uint32_t src = (uint32_t)rv;
     const uint64_t cp = rv >> 32;  // class and priority
     // the forward entry with this neighbour, if any (forward neighbours are distinct)
@@ -258,7 +269,9 @@ __global__ __launch_bounds__(64) void k_sample_select(
     }
   };
   // pass 1: every replacement is known before a forward key is used
-  for (uint32_t j = 0; j < nvec; ++j) (void)rev_key(j * WAVE + l, true);
+  if (nvec > 0 && l >= k && l < T) rvc0 = rev[r0 + (l - k)];
+  if (nvec > 1 && WAVE + l >= k && WAVE + l < T) rvc1 = rev[r0 + (WAVE + l - k)];
+  for (uint32_t j = 0; j < nvec; ++j) (void)rev_key(j, true);
   __syncthreads();
   // pass 2: selection. A row's lists are sets: the join pairs their members in
   // any order (every proposal goes through the order-free bucket insert), so a
@@ -277,7 +290,7 @@ __global__ __launch_bounds__(64) void k_sample_select(
         }
       return (fv && s_kkey[t] == EMPTY_KEY) ? f : EMPTY_KEY;
     }
-    return rev_key(t, false);
+    return rev_key(j, false);
   };
   uint32_t nn = 0, no = 0;
   uint64_t lastN = 0;  // the largest new-class key kept
