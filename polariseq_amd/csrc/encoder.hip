@@ -218,7 +218,7 @@ int psq_encoder_grad(const psq_encoder* e, const float* d_data, uint64_t n, uint
   hipLaunchKernelGGL(k_mlp_step, dim3(nb), dim3(64 * MLP_STEP_WAVES), mlp_step_smem(), 0, d_data, d_rows, (uint32_t)B, nullptr, 0u,
                      nullptr, 0u, 0u, 0ull, 0ull, e->wr, e->mu, e->inv_scale, margin, partial, lt);
   EHIPC(hipGetLastError());
-  hipLaunchKernelGGL(k_mlp_reduce_adam, dim3(blocks_for(NP, 32)), dim3(256), 0, 0, partial, nb,
+  hipLaunchKernelGGL(k_mlp_reduce_adam, dim3(blocks_for(NP, 32)), dim3(1024), 0, 0, partial, nb,
                      nullptr, nullptr, nullptr, nullptr, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, grad);
   EHIPC(hipGetLastError());
   std::vector<float> hl(B);
@@ -309,7 +309,7 @@ int psq_encoder_train(psq_encoder* e, const float* d_data, uint64_t n, uint64_t 
                        e->inv_scale, margin, partial, lt);
     c1 *= b1;
     c2 *= b2;
-    hipLaunchKernelGGL(k_mlp_reduce_adam, dim3(blocks_for(NP, 32)), dim3(256), 0, s, partial, nb,
+    hipLaunchKernelGGL(k_mlp_reduce_adam, dim3(blocks_for(NP, 32)), dim3(1024), 0, s, partial, nb,
                        e->w, e->wr, e->m1, e->m2, lr, b1, b2, eps, (float)(1.0 - c1),
                        (float)(1.0 - c2), nullptr);
     EHIPC(hipGetLastError());

@@ -334,31 +334,36 @@ __global__ __launch_bounds__(64 * STEP_WAVES) void k_mlp_step(
 
 size_t mlp_step_smem() { return STEP_SMEM; }
 
-// Block of 256 threads = 32 parameters x 8 groups: group g sums the partials
-// of blocks g, g + 8, ... into 8 accumulators by turn (block g + 8 i into
-// accumulator i % 8, eight loads in flight), added in a fixed tree, then the 8
+// Block of 1024 threads = 32 parameters x 32 groups: group g sums the partials
+// of blocks g, g + 32, ... into 4 accumulators by turn (block g + 32 i into
+// accumulator i % 4, four loads in flight), added in a fixed tree, then the 32
 // group sums are added in a fixed tree.
-__global__ __launch_bounds__(256) void k_mlp_reduce_adam(
+__global__ __launch_bounds__(1024) void k_mlp_reduce_adam(
     const float* __restrict__ partial, uint32_t nblocks, float* __restrict__ w,
     float* __restrict__ wr, float* __restrict__ m1, float* __restrict__ m2, float lr, float b1,
     float b2, float eps, float c1, float c2, float* __restrict__ grad_out) {
-  __shared__ float s_g[8][33];
+  __shared__ float s_g[32][33];
   const uint32_t pl = threadIdx.x % 32, grp = threadIdx.x / 32;
   const uint32_t p = blockIdx.x * 32 + pl;
-  float q[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  float q[4] = {0.0f, 0.0f, 0.0f, 0.0f};
   if (p < (uint32_t)N_PARAMS) {
-    for (uint32_t b = grp; b < nblocks; b += 64) {
+    for (uint32_t b = grp; b < nblocks; b += 128) {
 #pragma unroll
-      for (int i = 0; i < 8; ++i)
-        if (b + 8 * i < nblocks) q[i] += partial[pslot(p, b + 8 * i, nblocks)];
+      for (int i = 0; i < 4; ++i)
+        if (b + 32 * i < nblocks) q[i] += partial[pslot(p, b + 32 * i, nblocks)];
     }
   }
-  const float part = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7]));
-  s_g[grp][pl] = part;
+  s_g[grp][pl] = (q[0] + q[1]) + (q[2] + q[3]);
   __syncthreads();
   if (grp != 0 || p >= (uint32_t)N_PARAMS) return;
-  const float gg = ((s_g[0][pl] + s_g[1][pl]) + (s_g[2][pl] + s_g[3][pl])) +
-                   ((s_g[4][pl] + s_g[5][pl]) + (s_g[6][pl] + s_g[7][pl]));
+  float t[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) t[i] = s_g[2 * i][pl] + s_g[2 * i + 1][pl];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) t[i] = t[2 * i] + t[2 * i + 1];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) t[i] = t[2 * i] + t[2 * i + 1];
+  const float gg = (t[0] + t[1]) + (t[2] + t[3]);
   if (grad_out) {  // gradient only (no step)
     grad_out[p] = gg;
     return;
