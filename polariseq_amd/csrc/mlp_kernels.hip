@@ -134,7 +134,7 @@ __global__ __launch_bounds__(ENC_WAVES * 64) void k_encode_mfma(
 }
 
 // ── one training step in two launches ──
-// k_mlp_step: a block of MLP_STEP_T triplets (3 T = 48 rows: three 16-row
+// k_mlp_step: a block of MLP_STEP_T triplets (3 T rows: RT = 3 T / 16 16-row
 // tiles), forward, triplet loss, backward, its partial gradient (N_PARAMS
 // floats per block). Triplets come from `rows` ([anchors | positives |
 // negatives] of the whole batch) or, when rows is NULL, are sampled here:
@@ -144,17 +144,19 @@ __global__ __launch_bounds__(ENC_WAVES * 64) void k_encode_mfma(
 // operand layout of k_encode_mfma: lane l, c = l % 16, g = l / 16, holds A[c][g]
 // and B[g][c], and D[4g + i][c] is its i-th output), with FP32 activations and
 // the BF16-valued weights:
-//   H_pre = X W1^T + b1      (48 x 128, k = 52)      Z = gelu(H_pre) W2^T + b2 (48 x 32, k = 128)
-//   dW2 = dZ^T gelu(H_pre)   (32 x 128, k = 48)      dH = dZ W2              (48 x 128, k = 32)
-//   dW1 = dH_pre^T X         (128 x 52, k = 48),     dH_pre = dH * gelu'(H_pre)
+//   H_pre = X W1^T + b1      (3T x 128, k = 52)      Z = gelu(H_pre) W2^T + b2 (3T x 32, k = 128)
+//   dW2 = dZ^T gelu(H_pre)   (32 x 128, k = 3T)      dH = dZ W2              (3T x 128, k = 32)
+//   dW1 = dH_pre^T X         (128 x 52, k = 3T),     dH_pre = dH * gelu'(H_pre)
 // Activations live in LDS at row strides of 4 (x, h) and 52 (z) mod 64 words,
-// so the 16 rows of a tile read by c fall on distinct banks, in 72 KB: two
-// blocks per CU. The block's STEP_WAVES waves take the tiles of each product in
-// turn. The partial gradients are stored by tiles of 32 parameters, the blocks'
+// so the 16 rows of a tile read by c fall on distinct banks: 144 KB for 96 rows,
+// one block of 16 waves per CU. The block's STEP_WAVES waves take the tiles of
+// each product in turn. The partial gradients are stored by tiles of 32 parameters, the blocks'
 // slices of a tile side by side (pslot), so that the reduction streams them.
 // k_mlp_reduce_adam: sums the blocks' partials in a fixed order and takes the
 // Adam step (deterministic for a seed).
 constexpr int STEP_R = 3 * MLP_STEP_T;  // rows per block
+constexpr int RT = STEP_R / 16;         // 16-row tiles per block
+static_assert(STEP_R % 16 == 0, "the block's rows are whole 16-row tiles");
 constexpr int SX = 68, SH = 132, SZ = 52;  // LDS row strides (floats)
 static_assert(N_PARAMS % 32 == 0, "the partials are stored by tiles of 32 parameters");
 
@@ -183,10 +185,10 @@ __global__ __launch_bounds__(64 * STEP_WAVES) void k_mlp_step(
     uint32_t ns, uint64_t seed, uint64_t step, const float* __restrict__ pr, float mu,
     float inv_scale, float margin, float* __restrict__ partial, float* __restrict__ loss_terms) {
   extern __shared__ float step_lds[];
-  float* s_x = step_lds;                  // [48][SX]: x standardised, columns 50..63 zero
-  float* s_hp = s_x + STEP_R * SX;        // [48][SH]: h_pre, then dL/dh_pre
-  float* s_h = s_hp + STEP_R * SH;        // [48][SH]: gelu(h_pre)
-  float* s_z = s_h + STEP_R * SH;         // [48][SZ]: z, then dL/dz
+  float* s_x = step_lds;                  // [3T][SX]: x standardised, columns 50..63 zero
+  float* s_hp = s_x + STEP_R * SX;        // [3T][SH]: h_pre, then dL/dh_pre
+  float* s_h = s_hp + STEP_R * SH;        // [3T][SH]: gelu(h_pre)
+  float* s_z = s_h + STEP_R * SH;         // [3T][SZ]: z, then dL/dz
   uint32_t* s_row = reinterpret_cast<uint32_t*>(s_z + STEP_R * SZ);
   const uint32_t tid = threadIdx.x, l = tid % 64, w = tid / 64, c = l % 16, g = l / 16;
   const uint32_t b0 = blockIdx.x * MLP_STEP_T;
@@ -205,7 +207,7 @@ __global__ __launch_bounds__(64 * STEP_WAVES) void k_mlp_step(
   const float* W1 = pr + OFF_W1;
   const float* W2 = pr + OFF_W2;
   // layer 1: 3 x 8 tiles of (16 rows, 16 hidden units)
-  for (uint32_t tile = w; tile < 24; tile += STEP_WAVES) {
+  for (uint32_t tile = w; tile < RT * 8; tile += STEP_WAVES) {
     const uint32_t rt = tile / 8, ht = tile % 8, hj = 16 * ht + c;
     f32x4 acc;
 #pragma unroll
@@ -225,7 +227,7 @@ __global__ __launch_bounds__(64 * STEP_WAVES) void k_mlp_step(
   }
   __syncthreads();
   // layer 2: 3 x 2 tiles of (16 rows, 16 outputs)
-  for (uint32_t tile = w; tile < 6; tile += STEP_WAVES) {
+  for (uint32_t tile = w; tile < RT * 2; tile += STEP_WAVES) {
     const uint32_t rt = tile / 2, ot = tile % 2, oc = 16 * ot + c;
     f32x4 acc;
 #pragma unroll
@@ -275,7 +277,7 @@ __global__ __launch_bounds__(64 * STEP_WAVES) void k_mlp_step(
   }
   __syncthreads();
   auto put = [&](uint32_t p, float v) { partial[pslot(p, blockIdx.x, gridDim.x)] = v; };
-  // dW2 = dZ^T gelu(H_pre): 2 x 8 tiles of (16 outputs, 16 hidden), k over the 48 rows
+  // dW2 = dZ^T gelu(H_pre): 2 x 8 tiles of (16 outputs, 16 hidden), k over the block's rows
   for (uint32_t tile = w; tile < 16; tile += STEP_WAVES) {
     const uint32_t ot = tile / 8, jt = tile % 8;
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -289,7 +291,7 @@ __global__ __launch_bounds__(64 * STEP_WAVES) void k_mlp_step(
     for (int i = 0; i < 4; ++i) put(OFF_W2 + (16 * ot + 4 * g + i) * MLP_H + 16 * jt + c, acc[i]);
   }
   // dH = dZ W2, then dL/dh_pre = dH * gelu'(h_pre) in place of h_pre: 3 x 8 tiles, k over 32 outputs
-  for (uint32_t tile = w; tile < 24; tile += STEP_WAVES) {
+  for (uint32_t tile = w; tile < RT * 8; tile += STEP_WAVES) {
     const uint32_t rt = tile / 8, jt = tile % 8, hj = 16 * jt + c;
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -303,7 +305,7 @@ __global__ __launch_bounds__(64 * STEP_WAVES) void k_mlp_step(
     }
   }
   __syncthreads();
-  // biases: column sums over the 48 rows, in row order
+  // biases: column sums over the block's rows, in row order
   if (tid < (uint32_t)MLP_H) {
     float acc = 0.0f;
     for (int r = 0; r < STEP_R; ++r) acc += s_hp[r * SH + tid];
@@ -314,7 +316,7 @@ __global__ __launch_bounds__(64 * STEP_WAVES) void k_mlp_step(
     for (int r = 0; r < STEP_R; ++r) acc += s_z[r * SZ + o];
     put(OFF_B2 + o, acc);
   }
-  // dW1 = dH_pre^T X: 8 x 4 tiles of (16 hidden, 16 inputs), k over the 48 rows
+  // dW1 = dH_pre^T X: 8 x 4 tiles of (16 hidden, 16 inputs), k over the block's rows
   for (uint32_t tile = w; tile < 32; tile += STEP_WAVES) {
     const uint32_t jt = tile / 4, tt = tile % 4;
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};

@@ -124,8 +124,8 @@ constexpr int ENC_WAVES = 4;
 __global__ void k_encode_pack(const float* pr, float* packed);
 __global__ void k_encode_mfma(const float* X, uint32_t rows, const float* packed, const float* pr,
                               float mu, float inv_scale, float* z_out);
-constexpr int MLP_STEP_T = 16;     // triplets per block of k_mlp_step
-constexpr int MLP_STEP_WAVES = 8;  // its waves
+constexpr int MLP_STEP_T = 32;     // triplets per block of k_mlp_step
+constexpr int MLP_STEP_WAVES = 16;  // its waves
 size_t mlp_step_smem();          // its dynamic LDS
 __global__ void k_mlp_step(const float* X, const uint32_t* rows, uint32_t B, const uint32_t* pool,
                            uint32_t A, const uint32_t* knn, uint32_t kk, uint32_t ns, uint64_t seed,
