@@ -118,7 +118,7 @@ def main() -> int:
     ap.add_argument("--train-anchors", type=int, default=4096)
     ap.add_argument("--train-batch", type=int, default=8192)
     ap.add_argument("--train-lr", type=float, default=3e-3)
-    ap.add_argument("--train-margin", type=float, default=0.1)
+    ap.add_argument("--train-margin", type=float, default=0.01)
     ap.add_argument("--recall-queries", type=int, default=100_000)
     ap.add_argument("--cpu-cells", type=int, default=4_060_000,
                     help="cells of the headline workload the CPU baseline runs on (all by default)")

@@ -265,7 +265,7 @@ typedef struct psq_encoder_train_opts {
   uint64_t batch;  /* triplets per step (0 -> 8192) */
   uint64_t steps;  /* Adam steps (0 -> 100) */
   float lr;        /* 0 -> 3e-3 */
-  float margin;    /* triplet margin on squared code distances (0 -> 0.1) */
+  float margin;    /* triplet margin on squared code distances (0 -> 0.01) */
   uint64_t seed;
   uint64_t anchors; /* anchor pool: evenly spaced subset cells whose exact top-k_pos is
                        mined (0 -> 4096); anchors are drawn from it, negatives from the subset */

@@ -255,7 +255,7 @@ int psq_encoder_train(psq_encoder* e, const float* d_data, uint64_t n, uint64_t 
   const uint32_t kp = (uint32_t)std::min<uint64_t>(o->k_pos ? o->k_pos : 15, ns - 1);
   const uint32_t B = (uint32_t)(o->batch ? o->batch : 8192);
   const uint64_t steps = o->steps ? o->steps : 100;
-  const float margin = o->margin > 0 ? o->margin : 0.1f;
+  const float margin = o->margin > 0 ? o->margin : 0.01f;
   const float lr = o->lr > 0 ? o->lr : 3e-3f;
   if (B > (1u << 24)) return fail(PSQ_E_INVALID, "batch must be <= 2^24");
   // subset: evenly spaced cells; pool: evenly spaced subset rows

@@ -46,7 +46,7 @@ class Encoder:
         N.check(N.lib().psq_encoder_reset(self._h, int(seed)))
 
     def train(self, x, *, subset: int = 200_000, k_pos: int = 15, batch: int = 8192,
-              steps: int = 100, lr: float = 3e-3, margin: float = 0.1, seed: int = 0,
+              steps: int = 100, lr: float = 3e-3, margin: float = 0.01, seed: int = 0,
               anchors: int = 4096) -> dict:
         """Train on a CUDA float32 tensor (n, 50) resident on the device: triplets
         from the exact top-k_pos (within an evenly spaced subset) of an anchor pool."""
