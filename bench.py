@@ -301,6 +301,7 @@ def main() -> int:
 
     cpu = None
     cpu_c1 = None
+    speedup_rows = []
     if not args.no_cpu:
         import oracle
         threads = host_threads()
@@ -332,6 +333,40 @@ def main() -> int:
             cpu["recall_at_15"] = round(float(pr.mean()), 5)
             cpu["recall_p05"] = round(float(np.percentile(pr, 5)), 4)
         log(f"CPU baseline: {nc / ct:,.0f} cells/s ({ct:.2f} s), recall {cpu.get('recall_at_15')}")
+        # the speed-up table: the GPU drop-in (the reference's defaults) against the CPU
+        # restatement on C1, C2 and this workload, both timed from the embedding resident
+        # (device / host memory) to the CSR graph resident
+        speedup_rows = []
+        for name, nn_, comps in (("C1", 20_000, 20), ("C2", 1_300_000, 60)):
+            xc = x1 if name == "C1" else ds.mixture(nn_, 50, comps, seed=0)
+            xct = torch.from_numpy(xc).cuda()
+            P.nndescent_device(xct, 15, seed=0)
+            torch.cuda.synchronize()
+            reps = 5 if name == "C1" else 2
+            t = time.perf_counter()
+            for _ in range(reps):
+                og = P.nndescent_device(xct, 15, seed=0)
+            torch.cuda.synchronize()
+            gt = (time.perf_counter() - t) / reps
+            if name == "C1":
+                qc, ec, cg, cs_ = np.arange(nn_, dtype=np.int32), e1, g1, c1t
+            else:
+                log(f"CPU restatement on C2 ({nn_} cells, {threads} threads) ...")
+                t = time.perf_counter()
+                cg, _ = oracle.nndescent_port(xc, 15, seed=0, threads=threads)
+                cs_ = time.perf_counter() - t
+                qc = np.sort(np.random.default_rng(12345).choice(nn_, 10_000, replace=False)).astype(np.int32)
+                ec = P.knn_exact_device(xct, 15, torch.from_numpy(qc).cuda())[0].cpu().numpy()
+            gq = og[1][: nn_ * 15].view(nn_, 15)[torch.from_numpy(qc).cuda().long()].cpu().numpy()
+            speedup_rows.append({
+                "workload": f"{name}: {nn_} x 50, {comps}-component mixture, k=15, seed 0",
+                "gpu_ms": round(1000 * gt, 2), "cpu_ms": round(1000 * cs_, 1),
+                "speedup": round(cs_ / gt, 1),
+                "gpu_recall_at_15": round(float(ds.recall(gq, ec).mean()), 5),
+                "cpu_recall_at_15": round(float(ds.recall(cg["indices"].reshape(nn_, 15)[qc], ec).mean()), 5),
+                "queries": int(len(qc))})
+            log(f"speed-up {name}: GPU {1000 * gt:.1f} ms, CPU {1000 * cs_:.0f} ms")
+            del xct
 
     forms_out = {}
     for form, r in results.items():
@@ -367,6 +402,19 @@ def main() -> int:
     }
     if cpu_c1 is not None:
         line["cpu_baseline_c1"] = cpu_c1
+    if cpu is not None and "plain" in results and cpu.get("recall_at_15") is not None:
+        pf = forms_out["plain"]
+        speedup_rows.append({
+            "workload": f"C5: {n} x 50, {args.components}-component mixture, k={k}, seed {args.seed}",
+            "gpu_ms": pf["ms_per_step"], "cpu_ms": round(1000 * ct, 1),
+            "speedup": round(1000 * ct / pf["ms_per_step"], 1),
+            "gpu_recall_at_15": pf["recall_at_15"], "cpu_recall_at_15": cpu["recall_at_15"],
+            "queries": int(nq)})
+    if cpu is not None:
+        line["speedup_table"] = {
+            "what": ("the GPU drop-in (plain form, the reference's defaults) against the reference's "
+                     f"CPU path restated (C++, {threads} host threads), same input and seed"),
+            "rows": speedup_rows}
     if "plain_r95" in results and head != "plain_r95":
         line["speedup_vs_plain"] = round(forms_out[head]["cells_per_s"] /
                                          forms_out["plain_r95"]["cells_per_s"], 3)
