@@ -72,6 +72,14 @@ def _method(method: str) -> int:
     return _METHODS[method]
 
 
+def _unsigned(**values) -> None:
+    """The binding's usize/u64 arguments: a negative value raises OverflowError, as
+    pyo3's conversion does."""
+    for name, v in values.items():
+        if int(v) < 0 or int(v) >= 1 << 64:
+            raise OverflowError(f"{name}: can't convert {int(v)} to an unsigned 64-bit integer")
+
+
 def _run(matrix, n_components: int, center: bool, scale: bool, max_value, out, components, device: int,
          method: str = "auto", oversample: int = 10, n_iter: int = 1, seed: int = 0):
     indptr, indices, data, n, m = _csr_parts(matrix)
@@ -90,9 +98,10 @@ def _run(matrix, n_components: int, center: bool, scale: bool, max_value, out, c
     mv = math.inf if max_value is None else float(max_value)
     L = N.lib()
     meth = _method(method)
+    _unsigned(oversample=oversample, n_iter=n_iter, seed=seed)
     rc = L.psq_pca_csr_ex(indptr.ctypes.data, indices.ctypes.data, data.ctypes.data, n, m, max(k, 0),
                           int(bool(center)), int(bool(scale)), mv, meth, int(oversample), int(n_iter),
-                          int(seed) & (2**64 - 1), emb.ctypes.data, ratio.ctypes.data,
+                          int(seed), emb.ctypes.data, ratio.ctypes.data,
                           None if components is None else components.ctypes.data, int(device))
     if rc != N.PSQ_OK:
         msg = L.psq_last_error().decode("utf-8", "replace")
@@ -178,10 +187,11 @@ def pca_csr_device(indptr, indices, data, n_features: int, n_components: int, *,
         _check_out(components, (int(n_features), k), "components")
     ratio = torch.empty(k, dtype=torch.float32, device=indptr.device)
     mv = math.inf if max_value is None else float(max_value)
+    _unsigned(oversample=oversample, n_iter=n_iter, seed=seed)
     L = N.lib()
     rc = L.psq_pca_csr_device_ex(indptr.data_ptr(), indices.data_ptr(), data.data_ptr(), n, int(n_features), k,
                                  int(bool(center)), int(bool(scale)), mv, _method(method), int(oversample),
-                                 int(n_iter), int(seed) & (2**64 - 1), out.data_ptr(), ratio.data_ptr(),
+                                 int(n_iter), int(seed), out.data_ptr(), ratio.data_ptr(),
                                  None if components is None else components.data_ptr(), _stream_ptr())
     if rc != N.PSQ_OK:
         msg = L.psq_last_error().decode("utf-8", "replace")

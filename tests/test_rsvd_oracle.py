@@ -234,3 +234,13 @@ def test_python_rejects_an_unknown_method():
     x = sp.csr_matrix(np.eye(5, dtype=np.float32))
     with pytest.raises(ValueError, match="method"):
         gp.pca_sparse(x, 2, True, 10, 1, 0, method="arpack")
+
+
+def test_python_rejects_negative_sketch_options():
+    # pyo3 converts usize / u64 and raises OverflowError on a negative value
+    x = sp.csr_matrix(np.eye(5, dtype=np.float32))
+    for kw in ({"oversample": -1}, {"n_iter": -2}, {"seed": -3}):
+        args = {"oversample": 10, "n_iter": 1, "seed": 0}
+        args.update(kw)
+        with pytest.raises(OverflowError):
+            gp.pca_sparse(x, 2, True, args["oversample"], args["n_iter"], args["seed"])
