@@ -8,7 +8,8 @@ singular values and variance ratios to 1e-6 relative, every embedding and
 component column to 1e-5 of its largest entry (float32 output), on the
 reference's default options and deeper Krylov bases, centred and not, the
 scaled view, more than 32,768 genes (where the automatic choice takes this
-route), a basis cut to the width, and a rank-deficient matrix. The reference's
+route), a basis cut to the width, the bare sketch (n_iter 0), one component
+without oversampling, and a rank-deficient matrix. The reference's
 acceptance bound against the exact SVD (python/polariseq/tests/test_pca.py:38-62)
 is checked on its own structured fixture, and a run repeats bit for bit.
 """
@@ -90,6 +91,25 @@ def test_randomized_matches_oracle(n, m, k, over, it, seed, center):
     check(emb, vr, comp, R.rsvd(ip, ix, dx, n, m, k, over, it, seed, center))
     rep = gp.pca_last_report()
     assert rep["method"] == "randomized" and rep["block"] == min((k + over) * (it + 1), n, m)
+
+
+@pytest.mark.parametrize("k,over,it", [(1, 0, 0), (8, 5, 0)])
+def test_bare_sketch_and_one_component(k, over, it):
+    # n_iter 0: the basis is the sketch's block alone; k 1, oversample 0: one column
+    print(f"rsvd k={k} over={over} it={it}", flush=True)
+    mat = structured(1500, 700, 6, 0.2, 31)
+    ip, ix, dx = parts(mat)
+    emb, vr, comp = run(mat, k, over, it, 9)
+    check(emb, vr, comp, R.rsvd(ip, ix, dx, 1500, 700, k, over, it, 9, True))
+
+
+def test_scaled_view_above_32768_genes():
+    print("rsvd scaled m=33000 auto", flush=True)
+    mat = sparse_structured(1200, 33000, 5, 150, 4)
+    ip, ix, dx = parts(mat)
+    emb, vr, comp = run(mat, 6, 10, 1, 2, scale=True, max_value=10.0, method="auto")
+    assert gp.pca_last_report()["method"] == "randomized"
+    check(emb, vr, comp, R.rsvd_scaled(ip, ix, dx, 1200, 33000, 6, 10.0, 2))
 
 
 def test_scaled_view():
