@@ -84,6 +84,7 @@ __global__ __launch_bounds__(64) void k_join_bf16(
   const uint32_t l = (uint32_t)lane_id();
   const SlotHash sh = slot_hash_of(seed, iter, S);
   uint64_t pairs = 0, exact = 0, sent = 0, rows_in = 0;
+  uint32_t ncas = 0;  // this lane's compare-and-swaps
   uint32_t i;
   if (!join_cell_of_block(jc, n, i)) return;
   [&] {
@@ -118,9 +119,9 @@ __global__ __launch_bounds__(64) void k_join_bf16(
           pb = dist < s_thr[b];
         }
         pb_append(pb_row, pb_key, pcnt, pa, ia, make_key(dist, ib, 1), sent);
-        pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, sh);
+        pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, sh, ncas);
         pb_append(pb_row, pb_key, pcnt, pb, ib, make_key(dist, ia, 1), sent);
-        pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, sh);
+        pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, sh, ncas);
         done += WAVE;
       }
       if (all) {
@@ -207,14 +208,18 @@ __global__ __launch_bounds__(64) void k_join_bf16(
       }
     }
     drain(true);
-    pb_flush(pb_row, pb_key, pcnt, true, slots, touched, S, sh);
+    pb_flush(pb_row, pb_key, pcnt, true, slots, touched, S, sh, ncas);
   }();
   // pairs, exact, sent and rows_in are wave-uniform (counted from ballots)
+  uint32_t cas = ncas;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) cas += __shfl_xor(cas, o, WAVE);
   if (l == 0 && rows_in > 0) {
     ctr_add(counters, 0, (unsigned long long)pairs);
     ctr_add(counters, 1, (unsigned long long)rows_in);
     ctr_add(counters, 3, (unsigned long long)exact);
     ctr_add(counters, 4, (unsigned long long)sent);
+    ctr_add(counters, 5, (unsigned long long)cas);
   }
 }
 

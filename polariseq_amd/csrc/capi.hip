@@ -935,7 +935,7 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
     hipLaunchKernelGGL(k_ctr_reduce, dim3(1), dim3(64), 0, s, ctr, csum);
     HIPC(hipGetLastError());
     HIPC(hipEventRecord(i1, s));
-    HIPC(hipMemcpyAsync(hctr, csum, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(hctr, csum, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
     const uint64_t changed = hctr[2];
     if (it < 64) {
@@ -944,6 +944,7 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
       st.cand_rows[it] = hctr[1];
       st.exact_pairs[it] = bf16 ? hctr[3] : hctr[0];
       st.proposals[it] = hctr[4];
+      st.cas[it] = hctr[5];
     }
     ++iters;
     if (changed <= R.stop_at) break;
@@ -1577,6 +1578,21 @@ int psq_last_stats(psq_nndescent_stats* out) {
 }
 
 void psq_set_device_budget(uint64_t bytes) { g_budget = bytes; }
+
+int psq_calibrate_cas(uint64_t* d_table, uint64_t n_slots, uint64_t n_ops, uint64_t seed, void* stream) {
+  g_err.clear();
+  if (!d_table || n_slots < 2) return fail(PSQ_E_INVALID, "the table needs at least 2 slots");
+  constexpr uint64_t THREADS = 1ull << 20;  // 4096 blocks of 256
+  if (n_ops < THREADS || n_ops / THREADS > (1ull << 20))
+    return fail(PSQ_E_INVALID, "n_ops must be between 2^20 and 2^40");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // the sink is the table's slot 0: a value landing there is harmless
+  hipLaunchKernelGGL(k_calibrate_cas, dim3((unsigned)(THREADS / 256)), dim3(256), 0, s,
+                     reinterpret_cast<unsigned long long*>(d_table), n_slots, (uint32_t)(n_ops / THREADS), seed,
+                     reinterpret_cast<unsigned long long*>(d_table));
+  HIPC(hipGetLastError());
+  return PSQ_OK;
+}
 
 int psq_release_workspace(void) {
   LOCK_DEVICE();

@@ -37,6 +37,22 @@ __global__ void k_ctr_reduce(const unsigned long long* __restrict__ ctr,
   for (uint32_t st = 0; st < CTR_STRIPES; ++st) t += ctr[(size_t)st * CTR_WORDS + w];
   out[w] = t;
 }
+// Random 64-bit compare-and-swaps for the calibration entry: thread t issues
+// `per` of them at slots splitmix(seed ^ (t per + j)) mod n_slots, each expecting 0;
+// the addresses do not depend on the results, so many are in flight, as in the
+// join's bucket inserts. The returned values are folded into *sink, which keeps
+// the operations live.
+__global__ void k_calibrate_cas(unsigned long long* __restrict__ table, uint64_t n_slots, uint32_t per,
+                                uint64_t seed, unsigned long long* __restrict__ sink) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long acc = 0;
+  for (uint32_t j = 0; j < per; ++j) {
+    const uint64_t h = splitmix(seed ^ (t * per + j));
+    acc ^= atomicCAS(&table[h % n_slots], 0ull, (unsigned long long)(h | 1));
+  }
+  if (acc == 1ull) sink[0] = acc;
+}
+
 __global__ void k_fill_u8(uint8_t* p, size_t count, uint8_t v) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -403,6 +419,7 @@ __global__ __launch_bounds__(64) void k_join(
   const uint32_t l = (uint32_t)lane_id();
   const SlotHash sh = slot_hash_of(seed, iter, S);
   uint64_t pairs = 0, sent = 0, rows_in = 0;
+  uint32_t ncas = 0;  // this lane's compare-and-swaps
   uint32_t i;
   if (!join_cell_of_block(jc, n, i)) return;
   [&] {
@@ -432,20 +449,26 @@ __global__ __launch_bounds__(64) void k_join(
         }
       }
       pb_append(pb_row, pb_key, pcnt, pa, ia, make_key(dist, ib, 1), sent);
-      pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, sh);
+      pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, sh, ncas);
       pb_append(pb_row, pb_key, pcnt, pb, ib, make_key(dist, ia, 1), sent);
-      pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, sh);
+      pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, sh, ncas);
     }
-    pb_flush(pb_row, pb_key, pcnt, true, slots, touched, S, sh);
+    pb_flush(pb_row, pb_key, pcnt, true, slots, touched, S, sh, ncas);
   }();
-  // counters[0] += pairs, [1] += candidate rows gathered, [4] += proposals sent
+  // counters[0] += pairs, [1] += candidate rows gathered, [4] += proposals sent,
+  // [5] += compare-and-swaps
   uint64_t tot = pairs;
+  uint32_t cas = ncas;
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) tot += shfl_xor64(tot, o);
+  for (int o = 32; o > 0; o >>= 1) {
+    tot += shfl_xor64(tot, o);
+    cas += __shfl_xor(cas, o, WAVE);
+  }
   if (l == 0 && rows_in > 0) {
     ctr_add(counters, 0, (unsigned long long)tot);
     ctr_add(counters, 1, (unsigned long long)rows_in);
     ctr_add(counters, 4, (unsigned long long)sent);
+    ctr_add(counters, 5, (unsigned long long)cas);
   }
 }
 

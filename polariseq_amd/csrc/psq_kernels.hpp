@@ -11,6 +11,8 @@ __global__ void k_fill_u32(uint32_t* p, size_t count, uint32_t v);
 __global__ void k_fill_u8(uint8_t* p, size_t count, uint8_t v);
 __global__ void k_pad_rows(const float* X, uint32_t n, uint32_t d, uint32_t dp, float* Xp);
 __global__ void k_scan_local(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* block_sums);
+__global__ void k_calibrate_cas(unsigned long long* table, uint64_t n_slots, uint32_t per, uint64_t seed,
+                                unsigned long long* sink);
 __global__ void k_scan_sums(uint32_t* sums, uint32_t nb, uint32_t* total_out);
 __global__ void k_scan_add(uint32_t* out, uint32_t n, const uint32_t* sums);
 
