@@ -80,29 +80,6 @@ def measured_traffic(kernel: str, summary: str = "join_4m_counters.md"):
     return None
 
 
-def calibrate_cas(torch, P) -> float:
-    """Rate (G/s) of random 64-bit compare-and-swaps in HBM: 2^30 of them on a
-    2 GiB table, the size of the join's bucket slots at 4.06M cells; best of 3
-    launches timed with events on the current stream."""
-    from polariseq_amd import _native as N
-    table = torch.zeros(1 << 28, dtype=torch.int64, device="cuda")
-    stream = torch.cuda.current_stream().cuda_stream
-    ops = 1 << 30
-    best = None
-    for rep in range(4):
-        table.zero_()
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record()
-        N.check(N.lib().psq_calibrate_cas(table.data_ptr(), table.numel(), ops, rep, stream))
-        b.record()
-        b.synchronize()
-        if rep > 0:
-            t = a.elapsed_time(b) / 1000.0
-            best = t if best is None else min(best, t)
-    del table
-    return ops / best / 1e9
-
-
 def host_threads() -> int:
     env = os.environ.get("OMP_NUM_THREADS")
     if env and env.isdigit() and int(env) > 0:
@@ -310,8 +287,6 @@ def main() -> int:
             join_ms += s_["ms_join"][it]
             launches += 1
     achieved = join_bytes / (join_ms / 1000.0) / 1e9 if join_ms > 0 else 0.0
-    join_cas = sum(s_["cas"][it] for s_ in stats for it in range(s_["iterations"]))
-    cas_peak = calibrate_cas(torch, P)
     kname = "k_join_bf16<50>" if "bf16" in head else "k_join<50>"
     default_workload = (n, k, args.components, args.seed, args.init) == (4_060_000, 15, 150, 0,
                                                                           "rp_forest")
@@ -323,15 +298,6 @@ def main() -> int:
                        "the cells with more than 32 candidates)",
                 "passes": launches, "avg_pass_ms": round(join_ms / max(launches, 1), 3),
                 "algorithmic_bytes_per_pass": round(join_bytes / max(launches, 1))}
-    if join_ms > 0 and cas_peak > 0:
-        # the bucket inserts' own bound: every device-scope compare-and-swap is a
-        # memory round trip; the peak is measured in this run (calibrate_cas)
-        cas_rate = join_cas / (join_ms / 1000.0) / 1e9
-        roofline["atomics"] = {"achieved": round(cas_rate, 2), "peak": round(cas_peak, 2),
-                               "unit": "G compare-and-swaps/s", "frac": round(cas_rate / cas_peak, 4),
-                               "per_pass": round(join_cas / max(launches, 1)),
-                               "peak_from": "random 64-bit compare-and-swaps on a 2 GiB table "
-                                            "(psq_calibrate_cas, 2^30 per launch, best of 3)"}
 
     cpu = None
     cpu_c1 = None

@@ -338,7 +338,6 @@ typedef struct psq_nndescent_stats {
   uint64_t proposals[64]; /* (row, neighbour) proposals the join sent to the buckets */
   double ms_init, ms_iterations, ms_extract, ms_total;
   double ms_join[64]; /* join kernel time per iteration (HIP events) */
-  uint64_t cas[64];   /* compare-and-swaps of the join's bucket inserts, per iteration */
 } psq_nndescent_stats;
 int psq_last_stats(psq_nndescent_stats* out);
 
@@ -352,14 +351,6 @@ int psq_release_workspace(void);
  * allocation past it fails with PSQ_E_NOMEM, so out-of-memory handling can be
  * tested from the host. */
 void psq_set_device_budget(uint64_t bytes);
-
-/* Measurement helper with no counterpart in the reference: issues about n_ops
- * random 64-bit compare-and-swaps (expecting 0) on the device table of n_slots
- * words (the caller zeroes it; it is left modified) on the given stream, and
- * returns without waiting. Timed by the caller, it gives the device's rate of
- * such operations, the bound of the join's bucket inserts. n_ops is rounded down
- * to a multiple of 2^20 and must lie in [2^20, 2^40]. */
-int psq_calibrate_cas(uint64_t* d_table, uint64_t n_slots, uint64_t n_ops, uint64_t seed, void* stream);
 
 #ifdef __cplusplus
 }

@@ -63,7 +63,6 @@ EXPORTS = (
     "psq_pca_csr_ex",
     "psq_pca_csr_device_ex",
     "psq_pca_last_report",
-    "psq_calibrate_cas",
 )
 
 
@@ -153,7 +152,6 @@ class NndescentStats(ctypes.Structure):
         ("ms_extract", ctypes.c_double),
         ("ms_total", ctypes.c_double),
         ("ms_join", ctypes.c_double * 64),
-        ("cas", ctypes.c_uint64 * 64),
     ]
 
     def as_dict(self) -> dict:
@@ -172,7 +170,6 @@ class NndescentStats(ctypes.Structure):
             "ms_init": float(self.ms_init), "ms_iterations": float(self.ms_iterations),
             "ms_extract": float(self.ms_extract), "ms_total": float(self.ms_total),
             "ms_join": [float(v) for v in self.ms_join[:it]],
-            "cas": [int(v) for v in self.cas[:it]],
         }
 
 
@@ -268,8 +265,6 @@ def _declare(L: ctypes.CDLL) -> None:
     L.psq_pca_csr_ex.argtypes = [P, P, P, U64, U64, U64, I32, I32, D, I32, U64, U64, U64, P, P, P, I32]
     L.psq_pca_csr_device_ex.restype = ctypes.c_int
     L.psq_pca_csr_device_ex.argtypes = [P, P, P, U64, U64, U64, I32, I32, D, I32, U64, U64, U64, P, P, P, P]
-    L.psq_calibrate_cas.restype = ctypes.c_int
-    L.psq_calibrate_cas.argtypes = [P, U64, U64, U64, P]
     L.psq_pca_last_report.restype = ctypes.c_int
     L.psq_pca_last_report.argtypes = [ctypes.POINTER(PcaReport)]
     L.psq_encoder_mine_ms.restype = ctypes.c_int

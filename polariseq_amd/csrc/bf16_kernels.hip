@@ -71,7 +71,7 @@ __global__ __launch_bounds__(64) void k_join_bf16(
     const uint32_t* __restrict__ ncnt, uint32_t m, uint32_t R, JoinCells jc,
     const float* __restrict__ thr, float slack, uint64_t* __restrict__ slots,
     uint8_t* __restrict__ touched, uint32_t S, uint64_t seed, uint64_t iter,
-    unsigned long long* __restrict__ counters) {
+    uint32_t* __restrict__ rec) {
   extern __shared__ __attribute__((aligned(16))) uint64_t lds64[];
   uint64_t* pb_key = lds64;
   float* rows = reinterpret_cast<float*>(lds64 + PB_CAP);
@@ -84,7 +84,6 @@ __global__ __launch_bounds__(64) void k_join_bf16(
   const uint32_t l = (uint32_t)lane_id();
   const SlotHash sh = slot_hash_of(seed, iter, S);
   uint64_t pairs = 0, exact = 0, sent = 0, rows_in = 0;
-  uint32_t ncas = 0;  // this lane's compare-and-swaps
   uint32_t i;
   if (!join_cell_of_block(jc, n, i)) return;
   [&] {
@@ -119,9 +118,9 @@ __global__ __launch_bounds__(64) void k_join_bf16(
           pb = dist < s_thr[b];
         }
         pb_append(pb_row, pb_key, pcnt, pa, ia, make_key(dist, ib, 1), sent);
-        pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, sh, ncas);
+        pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, sh);
         pb_append(pb_row, pb_key, pcnt, pb, ib, make_key(dist, ia, 1), sent);
-        pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, sh, ncas);
+        pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, sh);
         done += WAVE;
       }
       if (all) {
@@ -208,28 +207,19 @@ __global__ __launch_bounds__(64) void k_join_bf16(
       }
     }
     drain(true);
-    pb_flush(pb_row, pb_key, pcnt, true, slots, touched, S, sh, ncas);
+    pb_flush(pb_row, pb_key, pcnt, true, slots, touched, S, sh);
   }();
   // pairs, exact, sent and rows_in are wave-uniform (counted from ballots)
-  uint32_t cas = ncas;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) cas += __shfl_xor(cas, o, WAVE);
-  if (l == 0 && rows_in > 0) {
-    ctr_add(counters, 0, (unsigned long long)pairs);
-    ctr_add(counters, 1, (unsigned long long)rows_in);
-    ctr_add(counters, 3, (unsigned long long)exact);
-    ctr_add(counters, 4, (unsigned long long)sent);
-    ctr_add(counters, 5, (unsigned long long)cas);
-  }
+  join_record(rec, i, l, (uint32_t)pairs, (uint32_t)rows_in, (uint32_t)exact, (uint32_t)sent);
 }
 
 template __global__ void k_join_bf16<0>(const float*, uint32_t, uint32_t, uint32_t, uint32_t,
                                         const uint32_t*, const uint32_t*, const uint32_t*, uint32_t,
                                         uint32_t, JoinCells, const float*, float, uint64_t*, uint8_t*,
-                                        uint32_t, uint64_t, uint64_t, unsigned long long*);
+                                        uint32_t, uint64_t, uint64_t, uint32_t*);
 template __global__ void k_join_bf16<50>(const float*, uint32_t, uint32_t, uint32_t, uint32_t,
                                          const uint32_t*, const uint32_t*, const uint32_t*, uint32_t,
                                          uint32_t, JoinCells, const float*, float, uint64_t*, uint8_t*,
-                                         uint32_t, uint64_t, uint64_t, unsigned long long*);
+                                         uint32_t, uint64_t, uint64_t, uint32_t*);
 
 }  // namespace psq

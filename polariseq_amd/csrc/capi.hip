@@ -380,25 +380,31 @@ int resolve(uint64_t n, uint64_t d, const psq_nndescent_opts* o, Resolved* r) {
   return PSQ_OK;
 }
 
+// Merges the slots into the rows; counter word 2 += entries changed. Each row
+// writes its count to chg[row] (zero when it is left alone) and one pass sums
+// them, instead of an atomic per changed row on the shared counter lines.
 int launch_merge(const Resolved& R, uint64_t* G, uint64_t* slots, uint8_t* touched, int force,
-                 unsigned long long* changed, hipStream_t s) {
+                 uint32_t* chg, unsigned long long* changed, hipStream_t s) {
   const dim3 grid = block_grid(R.n), block(64);
   const uint32_t n = (uint32_t)R.n, k = (uint32_t)R.k, S = (uint32_t)R.S;
   if (k > 64 || S > 256) {  // rows or slots past one wave's registers: sort in LDS
     const uint32_t need = k + S;
     auto wk = need <= 512 ? k_merge_wide<512> : (need <= 1024 ? k_merge_wide<1024> : k_merge_wide<2048>);
-    hipLaunchKernelGGL(wk, grid, dim3(256), 0, s, G, n, k, slots, S, touched, force, changed);
+    hipLaunchKernelGGL(wk, grid, dim3(256), 0, s, G, n, k, slots, S, touched, force, chg);
   } else if (S == 64)
     hipLaunchKernelGGL((k_merge<1, 2>), grid, block, 0, s, G, n, k, slots, S, touched, force,
-                       changed);
+                       chg);
   else if (S == 128)
     hipLaunchKernelGGL((k_merge<2, 4>), grid, block, 0, s, G, n, k, slots, S, touched, force,
-                       changed);
+                       chg);
   else if (S == 256)
     hipLaunchKernelGGL((k_merge<4, 8>), grid, block, 0, s, G, n, k, slots, S, touched, force,
-                       changed);
+                       chg);
   else
     return fail(PSQ_E_UNSUPPORTED, "slots must be 64, 128 or 256");
+  HIPC(hipGetLastError());
+  hipLaunchKernelGGL(k_sum_u32_to, dim3(std::min<uint32_t>(grid_for(n, 256), 1024)), dim3(256), 0, s, chg,
+                     n, changed, 2u);
   HIPC(hipGetLastError());
   return PSQ_OK;
 }
@@ -429,6 +435,7 @@ struct JoinArgs {
   uint32_t* hcount; // pinned host words: the two lists' lengths
   hipEvent_t counted;
   unsigned long long* ctr;
+  uint32_t* rec;    // n * JOIN_REC_WORDS: the cells' join records
   bool bf16;
 };
 
@@ -451,15 +458,25 @@ int launch_join(Workspace* ws, const Resolved& R, const JoinArgs& a, uint64_t it
     if (a.bf16)
       hipLaunchKernelGGL(bk, block_grid(grid), dim3(64), smem, s, a.Xp, n, (uint32_t)R.d, R.dp, RS, a.newl,
                          a.oldl, a.ncnt, m, rows, jc, a.thr, BF16_SLACK, a.slots, a.touched,
-                         (uint32_t)R.S, R.seed, iter, a.ctr);
+                         (uint32_t)R.S, R.seed, iter, a.rec);
     else
       hipLaunchKernelGGL(pk, block_grid(grid), dim3(64), smem, s, a.Xp, n, (uint32_t)R.d, R.dp, RS, a.newl,
                          a.oldl, a.ncnt, m, rows, jc, a.thr, a.slots, a.touched, (uint32_t)R.S,
-                         R.seed, iter, a.ctr);
+                         R.seed, iter, a.rec);
     HIPC(hipGetLastError());
     return PSQ_OK;
   };
-  if (full <= JOIN_SMALL_ROWS) return launch(full, JoinCells{nullptr, 0}, xcd_grid(n));
+  // the records into the counters, after every launch of the pass
+  auto reduce = [&]() -> int {
+    hipLaunchKernelGGL(k_join_rec_reduce, dim3(std::min<uint32_t>(grid_for(n, 256), 1024)), dim3(256), 0,
+                       s, a.rec, n, a.ctr);
+    HIPC(hipGetLastError());
+    return PSQ_OK;
+  };
+  if (full <= JOIN_SMALL_ROWS) {
+    TRY(launch(full, JoinCells{nullptr, 0}, xcd_grid(n)));
+    return reduce();
+  }
   // the cells with lo < candidates <= hi, listed in cell order, and their number
   auto defer = [&](uint32_t lo, uint32_t hi, uint32_t* list, uint32_t* hcount) -> int {
     hipLaunchKernelGGL(k_join_defer_flags, dim3(grid_for(n, 256)), dim3(256), 0, s, a.ncnt, n, lo,
@@ -482,7 +499,7 @@ int launch_join(Workspace* ws, const Resolved& R, const JoinArgs& a, uint64_t it
   const uint32_t c1 = *a.hcount, c2 = mid < full ? a.hcount[1] : 0u;
   if (c1) TRY(launch(mid, JoinCells{a.list, c1}, xcd_grid(c1)));
   if (c2) TRY(launch(full, JoinCells{a.list2, c2}, xcd_grid(c2)));
-  return PSQ_OK;
+  return reduce();
 }
 
 double ms_between(hipEvent_t a, hipEvent_t b) {
@@ -843,6 +860,10 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   TRY(wbuf(ws, "jlist2", n, &jlist2));
   TRY(wbuf(ws, "rowcnt", n, &rowcnt));
   TRY(wbuf(ws, "counters", (size_t)(CTR_STRIPES + 1) * CTR_WORDS, &ctr));
+  uint32_t* jrec;
+  TRY(wbuf(ws, "jrec", (size_t)n * JOIN_REC_WORDS, &jrec));
+  uint32_t* mchg;
+  TRY(wbuf(ws, "mchg", n, &mchg));
   const PartShape RP = part_shape(n, 8);
   PartBufs RB;
   TRY(part_bufs(ws, "rv", RP, (uint64_t)n * k, &RB));
@@ -896,7 +917,7 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
   hipLaunchKernelGGL(k_random_fill, dim3(grid_for(n, 256)), dim3(256), 0, s, Xp, n,
                      (uint32_t)R.d, R.dp, G, k, R.seed, touched);
   HIPC(hipGetLastError());
-  TRY(launch_merge(R, G, slots, touched, 1, ctr, s));
+  TRY(launch_merge(R, G, slots, touched, 1, mchg, ctr, s));
   HIPC(hipEventRecord(e_init, s));
 
   // ── iterations ──
@@ -925,17 +946,17 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
     hipEvent_t counted;
     TRY(evp.make(&counted));
     const JoinArgs ja{Xp,     newl,  oldl,  ncnt,  thr,     slots, touched, jflag, jpos, jlist,
-                      jlist2, static_cast<uint32_t*>(hp) + 960, counted,   ctr,   bf16};
+                      jlist2, static_cast<uint32_t*>(hp) + 960, counted,   ctr,   jrec, bf16};
     TRY(launch_join(ws, R, ja, it, s));
     HIPC(hipEventRecord(b, s));
     jb.push_back(a);
     je.push_back(b);
-    TRY(launch_merge(R, G, slots, touched, 0, ctr, s));
+    TRY(launch_merge(R, G, slots, touched, 0, mchg, ctr, s));
     unsigned long long* csum = ctr + (size_t)CTR_STRIPES * CTR_WORDS;
     hipLaunchKernelGGL(k_ctr_reduce, dim3(1), dim3(64), 0, s, ctr, csum);
     HIPC(hipGetLastError());
     HIPC(hipEventRecord(i1, s));
-    HIPC(hipMemcpyAsync(hctr, csum, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(hctr, csum, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
     const uint64_t changed = hctr[2];
     if (it < 64) {
@@ -944,7 +965,6 @@ int nndescent_device_impl(const float* d_data, uint64_t n_obs, uint64_t n_dim,
       st.cand_rows[it] = hctr[1];
       st.exact_pairs[it] = bf16 ? hctr[3] : hctr[0];
       st.proposals[it] = hctr[4];
-      st.cas[it] = hctr[5];
     }
     ++iters;
     if (changed <= R.stop_at) break;
@@ -1578,21 +1598,6 @@ int psq_last_stats(psq_nndescent_stats* out) {
 }
 
 void psq_set_device_budget(uint64_t bytes) { g_budget = bytes; }
-
-int psq_calibrate_cas(uint64_t* d_table, uint64_t n_slots, uint64_t n_ops, uint64_t seed, void* stream) {
-  g_err.clear();
-  if (!d_table || n_slots < 2) return fail(PSQ_E_INVALID, "the table needs at least 2 slots");
-  constexpr uint64_t THREADS = 1ull << 20;  // 4096 blocks of 256
-  if (n_ops < THREADS || n_ops / THREADS > (1ull << 20))
-    return fail(PSQ_E_INVALID, "n_ops must be between 2^20 and 2^40");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  // the sink is the table's slot 0: a value landing there is harmless
-  hipLaunchKernelGGL(k_calibrate_cas, dim3((unsigned)(THREADS / 256)), dim3(256), 0, s,
-                     reinterpret_cast<unsigned long long*>(d_table), n_slots, (uint32_t)(n_ops / THREADS), seed,
-                     reinterpret_cast<unsigned long long*>(d_table));
-  HIPC(hipGetLastError());
-  return PSQ_OK;
-}
 
 int psq_release_workspace(void) {
   LOCK_DEVICE();

@@ -37,20 +37,46 @@ __global__ void k_ctr_reduce(const unsigned long long* __restrict__ ctr,
   for (uint32_t st = 0; st < CTR_STRIPES; ++st) t += ctr[(size_t)st * CTR_WORDS + w];
   out[w] = t;
 }
-// Random 64-bit compare-and-swaps for the calibration entry: thread t issues
-// `per` of them at slots splitmix(seed ^ (t per + j)) mod n_slots, each expecting 0;
-// the addresses do not depend on the results, so many are in flight, as in the
-// join's bucket inserts. The returned values are folded into *sink, which keeps
-// the operations live.
-__global__ void k_calibrate_cas(unsigned long long* __restrict__ table, uint64_t n_slots, uint32_t per,
-                                uint64_t seed, unsigned long long* __restrict__ sink) {
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long acc = 0;
-  for (uint32_t j = 0; j < per; ++j) {
-    const uint64_t h = splitmix(seed ^ (t * per + j));
-    acc ^= atomicCAS(&table[h % n_slots], 0ull, (unsigned long long)(h | 1));
+// Adds the join's per-cell records (words 0, 1, 3, 4) to the counter stripes: a
+// grid-stride pass of coalesced 16-byte reads, then one atomic per wave and word.
+__global__ __launch_bounds__(256) void k_join_rec_reduce(const uint32_t* __restrict__ rec, uint32_t n,
+                                                         unsigned long long* __restrict__ counters) {
+  uint64_t a0 = 0, a1 = 0, a3 = 0, a4 = 0;
+  const size_t step = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
+    const uint4 lo = *reinterpret_cast<const uint4*>(rec + i * JOIN_REC_WORDS);
+    const uint32_t hi = rec[i * JOIN_REC_WORDS + 4];
+    a0 += lo.x;
+    a1 += lo.y;
+    a3 += lo.w;
+    a4 += hi;
   }
-  if (acc == 1ull) sink[0] = acc;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    a0 += shfl_xor64(a0, o);
+    a1 += shfl_xor64(a1, o);
+    a3 += shfl_xor64(a3, o);
+    a4 += shfl_xor64(a4, o);
+  }
+  if (lane_id() == 0) {
+    ctr_add(counters, 0, a0);
+    ctr_add(counters, 1, a1);
+    ctr_add(counters, 3, a3);
+    ctr_add(counters, 4, a4);
+  }
+}
+
+// counters[stripe][word] += the sum of v[0, n): a grid-stride pass, then one
+// atomic per wave.
+__global__ __launch_bounds__(256) void k_sum_u32_to(const uint32_t* __restrict__ v, uint32_t n,
+                                                    unsigned long long* __restrict__ counters,
+                                                    uint32_t word) {
+  uint64_t a = 0;
+  const size_t step = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) a += v[i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) a += shfl_xor64(a, o);
+  if (lane_id() == 0 && a) ctr_add(counters, word, a);
 }
 
 __global__ void k_fill_u8(uint8_t* p, size_t count, uint8_t v) {
@@ -408,7 +434,7 @@ __global__ __launch_bounds__(64) void k_join(
     const uint32_t* __restrict__ newl, const uint32_t* __restrict__ oldl,
     const uint32_t* __restrict__ ncnt, uint32_t m, uint32_t R, JoinCells jc,
     const float* __restrict__ thr, uint64_t* __restrict__ slots, uint8_t* __restrict__ touched,
-    uint32_t S, uint64_t seed, uint64_t iter, unsigned long long* __restrict__ counters) {
+    uint32_t S, uint64_t seed, uint64_t iter, uint32_t* __restrict__ rec) {
   // LDS: pb_key[PB_CAP] u64 | rows[R][RS] f32 | ids[R] | thr[R] | pb_row[PB_CAP]
   extern __shared__ __attribute__((aligned(16))) float lds[];
   uint64_t* pb_key = reinterpret_cast<uint64_t*>(lds);
@@ -419,7 +445,6 @@ __global__ __launch_bounds__(64) void k_join(
   const uint32_t l = (uint32_t)lane_id();
   const SlotHash sh = slot_hash_of(seed, iter, S);
   uint64_t pairs = 0, sent = 0, rows_in = 0;
-  uint32_t ncas = 0;  // this lane's compare-and-swaps
   uint32_t i;
   if (!join_cell_of_block(jc, n, i)) return;
   [&] {
@@ -449,27 +474,18 @@ __global__ __launch_bounds__(64) void k_join(
         }
       }
       pb_append(pb_row, pb_key, pcnt, pa, ia, make_key(dist, ib, 1), sent);
-      pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, sh, ncas);
+      pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, sh);
       pb_append(pb_row, pb_key, pcnt, pb, ib, make_key(dist, ia, 1), sent);
-      pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, sh, ncas);
+      pb_flush(pb_row, pb_key, pcnt, false, slots, touched, S, sh);
     }
-    pb_flush(pb_row, pb_key, pcnt, true, slots, touched, S, sh, ncas);
+    pb_flush(pb_row, pb_key, pcnt, true, slots, touched, S, sh);
   }();
-  // counters[0] += pairs, [1] += candidate rows gathered, [4] += proposals sent,
-  // [5] += compare-and-swaps
-  uint64_t tot = pairs;
-  uint32_t cas = ncas;
+  // the cell's record: pairs, candidate rows gathered, proposals sent (every
+  // pair is exact here; word 3 stays zero)
+  uint32_t tot = (uint32_t)pairs;
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    tot += shfl_xor64(tot, o);
-    cas += __shfl_xor(cas, o, WAVE);
-  }
-  if (l == 0 && rows_in > 0) {
-    ctr_add(counters, 0, (unsigned long long)tot);
-    ctr_add(counters, 1, (unsigned long long)rows_in);
-    ctr_add(counters, 4, (unsigned long long)sent);
-    ctr_add(counters, 5, (unsigned long long)cas);
-  }
+  for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, WAVE);
+  join_record(rec, i, l, tot, (uint32_t)rows_in, 0u, (uint32_t)sent);
 }
 
 // Cells of one deferred join launch: flag[i] = 1 when cell i has a new
@@ -503,12 +519,15 @@ template <int NR, int NT>
 __global__ __launch_bounds__(64) void k_merge(uint64_t* __restrict__ G, uint32_t n, uint32_t k,
                                               uint64_t* __restrict__ slots, uint32_t S,
                                               uint8_t* __restrict__ touched, int force,
-                                              unsigned long long* __restrict__ changed) {
+                                              uint32_t* __restrict__ chg) {
   __shared__ uint64_t s_prop[WAVE];
   const uint32_t l = (uint32_t)lane_id();
   const uint32_t p = block_id();
   if (p >= n) return;
-  if (!force && !touched[p]) return;
+  if (!force && !touched[p]) {
+    if (l == 0) chg[p] = 0u;
+    return;
+  }
   uint64_t* row = G + (size_t)p * k;
   const uint64_t* sl = slots + (size_t)p * S;
   const uint64_t orow = l < k ? row[l] : EMPTY_KEY;
@@ -517,7 +536,10 @@ __global__ __launch_bounds__(64) void k_merge(uint64_t* __restrict__ G, uint32_t
     // leave rows sorted and the random fill marks the rows it extends, so a
     // sorted row is left as it is
     const uint64_t next = shfl64(orow, (int)((l + 1) & (WAVE - 1)));
-    if (!__ballot(l + 1 < k && orow > next)) return;
+    if (!__ballot(l + 1 < k && orow > next)) {
+      if (l == 0) chg[p] = 0u;
+      return;
+    }
   }
   if (l == 0) touched[p] = 0;
   uint64_t v[NR];
@@ -577,7 +599,7 @@ __global__ __launch_bounds__(64) void k_merge(uint64_t* __restrict__ G, uint32_t
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) nchanged += __shfl_xor(nchanged, o, WAVE);
   }
-  if (l == 0 && nchanged) ctr_add(changed, 2, (unsigned long long)nchanged);
+  if (l == 0) chg[p] = nchanged;
 }
 
 // Merge for rows of more than 64 entries or more than 256 slots: one block of
@@ -589,7 +611,7 @@ template <int P>
 __global__ __launch_bounds__(256) void k_merge_wide(uint64_t* __restrict__ G, uint32_t n, uint32_t k,
                                                     uint64_t* __restrict__ slots, uint32_t S,
                                                     uint8_t* __restrict__ touched, int force,
-                                                    unsigned long long* __restrict__ changed) {
+                                                    uint32_t* __restrict__ chg) {
   constexpr int NTH = 256, EPT = P / NTH;
   __shared__ uint64_t s_key[P];
   __shared__ uint32_t s_wsum[NTH / WAVE];
@@ -598,14 +620,20 @@ __global__ __launch_bounds__(256) void k_merge_wide(uint64_t* __restrict__ G, ui
   const uint32_t p = block_id();
   if (p >= n) return;  // block-uniform
   const bool was = touched[p] != 0;
-  if (!force && !was) return;
+  if (!force && !was) {
+    if (tid == 0) chg[p] = 0u;
+    return;
+  }
   uint64_t* row = G + (size_t)p * k;
   const uint64_t* sl = slots + (size_t)p * S;
   const uint64_t orow = tid < k ? row[tid] : EMPTY_KEY;  // k <= 256
   if (force && !was) {
     // the forced pass only sorts: a sorted row is left as it is
     const bool desc = tid + 1 < k && orow > row[tid + 1];
-    if (!__syncthreads_or(desc)) return;
+    if (!__syncthreads_or(desc)) {
+      if (tid == 0) chg[p] = 0u;
+      return;
+    }
   }
   __syncthreads();
   if (tid == 0) touched[p] = 0;
@@ -665,14 +693,14 @@ __global__ __launch_bounds__(256) void k_merge_wide(uint64_t* __restrict__ G, ui
   const uint64_t kth = s_kth;
   const bool kept_old = nk > 0 && orow != EMPTY_KEY && (orow >> 1) <= (kth >> 1);
   const uint32_t nold = (uint32_t)__syncthreads_count(kept_old);
-  if (tid == 0 && nk > nold) ctr_add(changed, 2, (unsigned long long)(nk - nold));
+  if (tid == 0) chg[p] = nk > nold ? nk - nold : 0u;
 }
 template __global__ void k_merge_wide<512>(uint64_t*, uint32_t, uint32_t, uint64_t*, uint32_t, uint8_t*,
-                                           int, unsigned long long*);
+                                           int, uint32_t*);
 template __global__ void k_merge_wide<1024>(uint64_t*, uint32_t, uint32_t, uint64_t*, uint32_t, uint8_t*,
-                                            int, unsigned long long*);
+                                            int, uint32_t*);
 template __global__ void k_merge_wide<2048>(uint64_t*, uint32_t, uint32_t, uint64_t*, uint32_t, uint8_t*,
-                                            int, unsigned long long*);
+                                            int, uint32_t*);
 
 // ── extraction (nndescent.rs:977-1002) ──
 // Rows hold k entries; the graph takes the first ko (ko <= k) of each. With a
@@ -735,15 +763,15 @@ __global__ void k_renumber(uint32_t* __restrict__ ids, size_t total, const uint3
 template __global__ void k_join<0>(const float*, uint32_t, uint32_t, uint32_t, uint32_t,
                                    const uint32_t*, const uint32_t*, const uint32_t*, uint32_t,
                                    uint32_t, JoinCells, const float*, uint64_t*, uint8_t*, uint32_t,
-                                   uint64_t, uint64_t, unsigned long long*);
+                                   uint64_t, uint64_t, uint32_t*);
 template __global__ void k_join<50>(const float*, uint32_t, uint32_t, uint32_t, uint32_t,
                                     const uint32_t*, const uint32_t*, const uint32_t*, uint32_t,
                                     uint32_t, JoinCells, const float*, uint64_t*, uint8_t*, uint32_t,
-                                    uint64_t, uint64_t, unsigned long long*);
+                                    uint64_t, uint64_t, uint32_t*);
 
 #define PSQ_MERGE_INST(NR, NT)                                                                 \
   template __global__ void k_merge<NR, NT>(uint64_t*, uint32_t, uint32_t, uint64_t*, uint32_t,   \
-                                           uint8_t*, int, unsigned long long*);
+                                           uint8_t*, int, uint32_t*);
 PSQ_MERGE_INST(1, 2)
 PSQ_MERGE_INST(2, 4)
 PSQ_MERGE_INST(4, 8)

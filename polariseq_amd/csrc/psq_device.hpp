@@ -280,12 +280,27 @@ __device__ inline float sq_dist_fixed(PA p, PB q) {
 
 // ── statistics counters ──
 // Counters are striped over CTR_STRIPES lines of CTR_WORDS u64: a block adds to
-// its own stripe and k_ctr_reduce sums the stripes, so millions of per-wave
-// additions do not serialise on one address. Words: 0 pairs, 1 candidate rows,
-// 2 changed entries, 3 exact pairs, 4 proposals, 5 compare-and-swaps.
+// its own stripe and k_ctr_reduce sums the stripes. Words: 0 pairs, 1 candidate
+// rows, 2 changed entries, 3 exact pairs, 4 proposals.
 constexpr uint32_t CTR_STRIPES = 256, CTR_WORDS = 8;
 __device__ inline void ctr_add(unsigned long long* ctr, uint32_t word, unsigned long long v) {
   atomicAdd(&ctr[(size_t)(blockIdx.x % CTR_STRIPES) * CTR_WORDS + word], v);
+}
+
+// The join's per-cell record: JOIN_REC_WORDS u32 per cell, word w counting what
+// counter word w counts (0 pairs, 1 candidate rows, 3 exact pairs, 4 proposals;
+// 2, 5, 6 and 7 stay zero). The launch over every cell writes every cell's
+// record, zeros for the cells it leaves to a deferred launch, which runs later
+// on the same stream and overwrites them; k_join_rec_reduce then adds the
+// records to the counter stripes. One 32-byte store per cell replaces four
+// atomics per cell on CTR_STRIPES shared lines.
+constexpr uint32_t JOIN_REC_WORDS = 8;
+__device__ inline void join_record(uint32_t* rec, uint32_t i, uint32_t l, uint32_t pairs,
+                                   uint32_t rows, uint32_t exact, uint32_t sent) {
+  if (l < JOIN_REC_WORDS) {
+    const uint32_t v = l == 0 ? pairs : l == 1 ? rows : l == 3 ? exact : l == 4 ? sent : 0u;
+    rec[(size_t)i * JOIN_REC_WORDS + l] = v;
+  }
 }
 
 // ── proposals: bucket cascade ──
@@ -327,7 +342,7 @@ __device__ inline uint64_t bucket_to_row(uint64_t b) {
 // reader either picks the same slot or finds the earlier pick already filled
 // (and its own swap on an older view fails).
 __device__ inline void propose(uint64_t* __restrict__ slots, uint8_t* __restrict__ touched,
-                               uint32_t row, uint64_t key, uint32_t S, const SlotHash& sh, uint32_t& ncas) {
+                               uint32_t row, uint64_t key, uint32_t S, const SlotHash& sh) {
   const uint32_t q = key_idx(key);
   const uint32_t b = slot_bucket(sh, q);
   uint64_t* bk = slots + (size_t)row * S + (size_t)b * BUCKET;
@@ -353,7 +368,6 @@ __device__ inline void propose(uint64_t* __restrict__ slots, uint8_t* __restrict
     const int jm = (int)((__builtin_ctz(rq) + rot) & 7u);
     const uint64_t old = atomicCAS((unsigned long long*)&bk[jm], (unsigned long long)mx,
                                    (unsigned long long)bkey);
-    ++ncas;
     if (old == mx) break;
 #pragma unroll
     for (int j = 0; j < 8; ++j) v[j] = j == jm ? old : v[j];
@@ -560,16 +574,16 @@ __device__ inline void pb_append(uint32_t* pb_row, uint64_t* pb_key, uint32_t& c
 // Sends every full batch of 64 (all entries when `all`), keeping the rest.
 __device__ inline void pb_flush(uint32_t* pb_row, uint64_t* pb_key, uint32_t& cnt, bool all,
                                 uint64_t* __restrict__ slots, uint8_t* __restrict__ touched,
-                                uint32_t S, const SlotHash& sh, uint32_t& ncas) {
+                                uint32_t S, const SlotHash& sh) {
   if (cnt < (all ? 1u : (uint32_t)WAVE)) return;
   const uint32_t l = (uint32_t)lane_id();
   __syncthreads();
   uint32_t done = 0;
   for (; done + WAVE <= cnt; done += WAVE)
-    propose(slots, touched, pb_row[done + l], pb_key[done + l], S, sh, ncas);
+    propose(slots, touched, pb_row[done + l], pb_key[done + l], S, sh);
   const uint32_t rest = cnt - done;
   if (all) {
-    if (l < rest) propose(slots, touched, pb_row[done + l], pb_key[done + l], S, sh, ncas);
+    if (l < rest) propose(slots, touched, pb_row[done + l], pb_key[done + l], S, sh);
     cnt = 0;
     return;
   }

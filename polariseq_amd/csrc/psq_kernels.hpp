@@ -11,14 +11,14 @@ __global__ void k_fill_u32(uint32_t* p, size_t count, uint32_t v);
 __global__ void k_fill_u8(uint8_t* p, size_t count, uint8_t v);
 __global__ void k_pad_rows(const float* X, uint32_t n, uint32_t d, uint32_t dp, float* Xp);
 __global__ void k_scan_local(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* block_sums);
-__global__ void k_calibrate_cas(unsigned long long* table, uint64_t n_slots, uint32_t per, uint64_t seed,
-                                unsigned long long* sink);
 __global__ void k_scan_sums(uint32_t* sums, uint32_t nb, uint32_t* total_out);
 __global__ void k_scan_add(uint32_t* out, uint32_t n, const uint32_t* sums);
 
 __global__ void k_random_fill(const float* Xp, uint32_t n, uint32_t d, uint32_t dp, uint64_t* G,
                               uint32_t k, uint64_t seed, uint8_t* touched);
 __global__ void k_ctr_reduce(const unsigned long long* ctr, unsigned long long* out);
+__global__ void k_join_rec_reduce(const uint32_t* rec, uint32_t n, unsigned long long* counters);
+__global__ void k_sum_u32_to(const uint32_t* v, uint32_t n, unsigned long long* counters, uint32_t word);
 template <int KV>
 __global__ void k_sample_select(uint64_t* G, uint32_t n, uint32_t k, uint32_t m, uint64_t seed,
                                 uint64_t iter, const uint32_t* roff, const uint64_t* rev,
@@ -28,16 +28,16 @@ template <int D>
 __global__ void k_join(const float* Xp, uint32_t n, uint32_t d, uint32_t dp, uint32_t RS,
                        const uint32_t* newl, const uint32_t* oldl, const uint32_t* ncnt, uint32_t m,
                        uint32_t R, JoinCells jc, const float* thr, uint64_t* slots, uint8_t* touched,
-                       uint32_t S, uint64_t seed, uint64_t iter, unsigned long long* counters);
+                       uint32_t S, uint64_t seed, uint64_t iter, uint32_t* rec);
 __global__ void k_join_defer_flags(const uint32_t* ncnt, uint32_t n, uint32_t lo, uint32_t hi,
                                    uint32_t* flag);
 __global__ void k_join_defer_list(const uint32_t* flag, const uint32_t* pos, uint32_t n, uint32_t* list);
 template <int P>
 __global__ void k_merge_wide(uint64_t* G, uint32_t n, uint32_t k, uint64_t* slots, uint32_t S,
-                             uint8_t* touched, int force, unsigned long long* changed);
+                             uint8_t* touched, int force, uint32_t* chg);
 template <int NR, int NT>
 __global__ void k_merge(uint64_t* G, uint32_t n, uint32_t k, uint64_t* slots, uint32_t S,
-                        uint8_t* touched, int force, unsigned long long* changed);
+                        uint8_t* touched, int force, uint32_t* chg);
 __global__ void k_row_counts(const uint64_t* G, uint32_t n, uint32_t k, uint32_t ko, const uint32_t* inv,
                              uint32_t* cnt);
 __global__ void k_extract(const uint64_t* G, uint32_t n, uint32_t k, uint32_t ko, const uint32_t* perm,
@@ -108,7 +108,7 @@ __global__ void k_join_bf16(const float* Xp, uint32_t n, uint32_t d, uint32_t dp
                             const uint32_t* newl, const uint32_t* oldl, const uint32_t* ncnt,
                             uint32_t m, uint32_t R, JoinCells jc, const float* thr, float slack,
                             uint64_t* slots, uint8_t* touched, uint32_t S, uint64_t seed,
-                            uint64_t iter, unsigned long long* counters);
+                            uint64_t iter, uint32_t* rec);
 
 // learned encoder (mlp_kernels.hip)
 constexpr int MLP_N_PARAMS = 128 * 50 + 128 + 32 * 128 + 32;
